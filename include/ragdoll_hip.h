@@ -1,0 +1,182 @@
+/*
+ * ragdoll_hip — MI355X (gfx950) H.264 P-frame analysis behind a plain C ABI.
+ *
+ * The library replaces the per-macroblock P analysis of ragdoll's Balanced encoder: the seam
+ * `EncState::code_p_mb` inside the raster loop of `encode_frame_body`
+ * (crates/ragdoll-codec/src/h264/enc.rs:3011-3191, called at :2423-2429), together with the
+ * post-frame reference preparation of that loop (deblock :3776-3881, extend_borders :164-178,
+ * cur/refp swap and prev_mb_mv record :2470-2503). A whole P frame is analysed on the GPU in
+ * anti-diagonal macroblock wavefront order; decisions, quantized levels and the reconstruction
+ * come back bit-exact with the CPU encoder, for the entropy coder to serialise in raster order.
+ *
+ * Conventions (mirroring the reference):
+ *  - Errors: every entry point returns an rgh_status; the codes map one-to-one onto the
+ *    reference's RagdollError variants (crates/ragdoll-core/src/error.rs:4-31) used by the
+ *    encoder (Unsupported for non-I420 input, enc.rs:2235-2240; InvalidDimensions), plus the
+ *    device and allocation failures a GPU adds. Nothing aborts across the boundary.
+ *  - Ownership: the caller owns its frames (read-only here); a context owns every persistent
+ *    device buffer, allocated once at rgh_create ("kept across frames so steady-state encoding
+ *    allocates nothing per macroblock", enc.rs:215-217).
+ *  - Threading: a context is used by one thread at a time, like one H264Encoder per GOP worker
+ *    (crates/ragdoll-pipeline/src/lib.rs:110-155, 575-590). Concurrency comes from many
+ *    contexts; rgh_analyse_batch runs one P frame of each of several contexts in one set of
+ *    launches.
+ */
+#ifndef RAGDOLL_HIP_H
+#define RAGDOLL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rgh_status {
+    RGH_OK = 0,
+    RGH_ERR_INVALID_ARGUMENT = 1,   /* RagdollError::InvalidArgument */
+    RGH_ERR_UNSUPPORTED = 2,        /* RagdollError::Unsupported (e.g. non-I420, enc.rs:2235) */
+    RGH_ERR_INVALID_DIMENSIONS = 3, /* RagdollError::InvalidDimensions */
+    RGH_ERR_NO_REFERENCE = 4,       /* a P frame without a reference picture */
+    RGH_ERR_DEVICE = 5,             /* HIP runtime failure */
+    RGH_ERR_OOM = 6                 /* device allocation failure */
+} rgh_status;
+
+/* Encoder profile (crates/ragdoll-codec/src/h264.rs:30-38 H264Profile). */
+typedef enum rgh_profile {
+    RGH_PROFILE_BASELINE = 0, /* Constrained Baseline, CAVLC, 4x4 transform */
+    RGH_PROFILE_MAIN = 1,     /* CABAC, 4x4 transform */
+    RGH_PROFILE_HIGH = 2      /* CABAC, adaptive 8x8 transform (enc.rs:3170-3179) */
+} rgh_profile;
+
+/* Context configuration: the fields of H264Encoder::new(w, h, fps, qp).with_profile(p)
+ * (h264.rs:3032-3065) that the P analysis reads, plus the search parameters the reference
+ * hard-codes (enc.rs:1207 diamond iterations = 16, enc.rs:3106 qpel iterations = 2). */
+typedef struct rgh_config {
+    int32_t width;      /* display width, even, > 0 */
+    int32_t height;     /* display height, even, > 0 */
+    int32_t profile;    /* rgh_profile */
+    int32_t qp;         /* fixed QP 1..51 (RateControlMode::FixedQp) */
+    int32_t me_range;   /* diamond iterations; 0 selects the reference's 16 */
+    int32_t qpel_iters; /* quarter-pel iterations; 0 selects the reference's 2 */
+    int32_t device;     /* HIP device ordinal */
+    int32_t flags;      /* RGH_FLAG_* */
+} rgh_config;
+
+/* rgh_config.flags: skip the device copy of each frame's reconstruction before deblocking
+ * (rgh_get_outputs with which_recon = 0 then fails with RGH_ERR_INVALID_ARGUMENT). */
+#define RGH_FLAG_NO_UNDEBLOCKED_COPY 1
+
+/* An I420 picture: three planes with their strides; width/height are display sizes.
+ * Chroma planes are (width+1)/2 x (height+1)/2. */
+typedef struct rgh_i420 {
+    const uint8_t *y;
+    const uint8_t *u;
+    const uint8_t *v;
+    int32_t stride_y;
+    int32_t stride_u;
+    int32_t stride_v;
+    int32_t width;
+    int32_t height;
+} rgh_i420;
+
+/* A writable I420 picture at coded size (mb_w*16 x mb_h*16, chroma half). */
+typedef struct rgh_i420_out {
+    uint8_t *y;
+    uint8_t *u;
+    uint8_t *v;
+    int32_t stride_y;
+    int32_t stride_u;
+    int32_t stride_v;
+} rgh_i420_out;
+
+/* Macroblock types of rgh_mb_decision.mb_type. */
+enum {
+    RGH_MB_P_SKIP = 0,
+    RGH_MB_P_L0_16X16 = 1,
+    RGH_MB_I16X16 = 2,
+    RGH_MB_I4X4 = 3,
+    RGH_MB_I_PCM = 4
+};
+
+/* Fixed 32-byte decision record per macroblock, raster order. */
+typedef struct rgh_mb_decision {
+    uint8_t mb_type;       /* RGH_MB_* */
+    uint8_t i16_mode;      /* I_16x16 prediction mode (0 V, 1 H, 2 DC, 3 plane) */
+    uint8_t chroma_mode;   /* intra chroma prediction mode (0 DC, 1 H, 2 V, 3 plane) */
+    uint8_t cbp_luma;      /* coded_block_pattern luma bits (8x8 quadrants) */
+    uint8_t cbp_chroma;    /* 0, 1 (DC only) or 2 (DC + AC) */
+    uint8_t transform_8x8; /* transform_size_8x8_flag of a coded P_L0_16x16 */
+    uint8_t qp;            /* QP_Y of the macroblock */
+    uint8_t flags;         /* bit 0: P_Skip found by the probe before the search */
+    int16_t mv[2];         /* quarter-pel vector (P_Skip: the skip vector; intra: 0) */
+    int16_t mvp[2];        /* motion vector predictor; mvd = mv - mvp */
+    uint32_t cost;         /* motion-estimation cost (SAD + lambda * mv bits) of the result */
+    uint16_t n_sad;        /* full-pel 16x16 SAD evaluations */
+    uint16_t n_subpel;     /* sub-pel SAD evaluations */
+    uint16_t n_satd;       /* 16x16 SATD evaluations */
+    uint16_t n_hpel;       /* half-pel windows interpolated */
+    uint16_t nnz;          /* 4x4 luma blocks holding levels, bit 4*by+bx */
+    uint16_t reserved;
+} rgh_mb_decision;
+
+/* Quantized levels per macroblock (int16 elements), the reference's scan layout:
+ *   [0, 256)   luma: 16 blocks (decoding order) x 16 zig-zag levels; I_16x16 AC blocks keep
+ *              slot 0 empty; with the 8x8 transform, 4 blocks x 64 levels (8x8 zig-zag)
+ *   [256, 272) I_16x16 luma DC, zig-zag
+ *   [272, 280) chroma DC, U then V (2x2 raster)
+ *   [280, 408) chroma AC, U then V, 4 blocks x 16 zig-zag levels (slot 0 empty) */
+#define RGH_LEVELS_PER_MB 408
+
+/* Creates a context for one stream. Allocates every device buffer it will use.
+ * Replaces: H264Encoder::new(..).with_profile(..) (h264.rs:3032-3065) and
+ * EncState::prepare's allocation (enc.rs:223-300). */
+rgh_status rgh_create(const rgh_config *cfg, void **ctx_out);
+
+/* Frees the context. */
+rgh_status rgh_destroy(void *ctx);
+
+/* Installs the reconstruction of an IDR picture coded elsewhere as the reference: the picture
+ * (coded size, already deblocked) is padded by edge replication and every previous-frame
+ * motion vector is reset to zero (an IDR frame is all intra).
+ * Replaces: the post-IDR part of encode_frame_body (enc.rs:2470-2503). */
+rgh_status rgh_set_reference(void *ctx, const rgh_i420 *recon);
+
+/* Analyses one P frame whose I420 source is in host memory. Runs every macroblock's
+ * decision, residual quantization and reconstruction on the GPU in wavefront order, then
+ * deblocks the reconstruction, extends its borders and makes it the next reference.
+ * Any output pointer may be NULL. decisions: mb_w*mb_h records; levels: mb_w*mb_h *
+ * RGH_LEVELS_PER_MB int16; recon: the deblocked reconstruction at coded size.
+ * Replaces: the P-frame body of encode_frame_body (enc.rs:2382-2434 with code_p_mb
+ * :3011-3191) and its post-frame reference preparation (:2470-2503). */
+rgh_status rgh_analyse_p_frame(void *ctx, const rgh_i420 *src, rgh_mb_decision *decisions,
+                               int16_t *levels, const rgh_i420_out *recon);
+
+/* Batched form: one P frame of each of n contexts, whose sources are already in device
+ * memory, analysed by one set of kernel launches on one HIP stream (all contexts must have
+ * been created on the same device). Results stay on the device; read them with
+ * rgh_get_outputs. No counterpart in the reference, whose GOP workers run one encoder per
+ * thread (crates/ragdoll-pipeline/src/lib.rs:543-603); this is that wave of GOPs on one GPU. */
+rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src_device);
+
+/* Copies the last analysed frame's results to host memory (any pointer may be NULL).
+ * which_recon: 0 = before deblocking, 1 = deblocked reference. */
+rgh_status rgh_get_outputs(void *ctx, rgh_mb_decision *decisions, int16_t *levels,
+                           const rgh_i420_out *recon, int32_t which_recon);
+
+/* Macroblock grid of a context. */
+rgh_status rgh_get_mb_dims(void *ctx, int32_t *mb_w, int32_t *mb_h);
+
+/* Waits for the context's queued GPU work. */
+rgh_status rgh_synchronize(void *ctx);
+
+/* HIP stream (hipStream_t) the context's work runs on, for callers that time it. */
+void *rgh_get_stream(void *ctx);
+
+/* A human-readable description of a status code. */
+const char *rgh_status_str(int32_t status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

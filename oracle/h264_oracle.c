@@ -2721,6 +2721,11 @@ long ora_encode_frame(ora_enc *e, const uint8_t *y, const uint8_t *u, const uint
                 code_intra(e, wp, mb_x, mb_y, slice_qp, &s);
             else
                 code_p_mb(e, wp, mb_x, mb_y, slice_qp, &s);
+            uint16_t nnz = 0;
+            for (int by = 0; by < 4; by++)
+                for (int bx = 0; bx < 4; bx++)
+                    if (e->luma_nnz[(mb_y * 4 + by) * mb_w * 4 + mb_x * 4 + bx]) nnz |= (uint16_t)(1u << (by * 4 + bx));
+            e->dec[mb_y * mb_w + mb_x].nnz = nnz;
         }
     if (wp) {
         if (!is_idr && e->skip_run > 0) put_ue(wp, (uint64_t)e->skip_run);

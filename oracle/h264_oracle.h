@@ -49,7 +49,8 @@ typedef struct ora_mb_decision {
     uint16_t n_subpel;    /* sub-pel SAD (or SAD-avg) evaluations */
     uint16_t n_satd;      /* 16x16 SATD evaluations (inter + intra plane) */
     uint16_t n_hpel;      /* half-pel windows interpolated */
-    uint32_t reserved;
+    uint16_t nnz;         /* non-zero 4x4 luma blocks after coding, bit 4*by+bx (raster) */
+    uint16_t reserved;
 } ora_mb_decision;
 
 /* Levels per macroblock, int16 elements (layout shared with the product). */

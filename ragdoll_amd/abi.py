@@ -1,0 +1,175 @@
+"""ctypes binding of include/ragdoll_hip.h (the library is loaded on first use)."""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libragdoll_hip.so")
+LEVELS_PER_MB = 408
+PROFILES = {"baseline": 0, "main": 1, "high": 2}
+FLAG_NO_UNDEBLOCKED_COPY = 1
+
+DECISION_DTYPE = np.dtype([
+    ("mb_type", "u1"), ("i16_mode", "u1"), ("chroma_mode", "u1"), ("cbp_luma", "u1"),
+    ("cbp_chroma", "u1"), ("transform_8x8", "u1"), ("qp", "u1"), ("flags", "u1"),
+    ("mv", "<i2", (2,)), ("mvp", "<i2", (2,)), ("cost", "<u4"), ("n_sad", "<u2"),
+    ("n_subpel", "<u2"), ("n_satd", "<u2"), ("n_hpel", "<u2"), ("nnz", "<u2"), ("reserved", "<u2"),
+])
+assert DECISION_DTYPE.itemsize == 32
+
+EXPORTS = ["rgh_create", "rgh_destroy", "rgh_set_reference", "rgh_analyse_p_frame",
+           "rgh_analyse_batch", "rgh_get_outputs", "rgh_get_mb_dims", "rgh_synchronize",
+           "rgh_get_stream", "rgh_status_str"]
+
+
+class RgConfig(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("profile", ctypes.c_int32),
+                ("qp", ctypes.c_int32), ("me_range", ctypes.c_int32), ("qpel_iters", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+class RgI420(ctypes.Structure):
+    _fields_ = [("y", ctypes.c_void_p), ("u", ctypes.c_void_p), ("v", ctypes.c_void_p),
+                ("stride_y", ctypes.c_int32), ("stride_u", ctypes.c_int32), ("stride_v", ctypes.c_int32),
+                ("width", ctypes.c_int32), ("height", ctypes.c_int32)]
+
+
+class RgI420Out(ctypes.Structure):
+    _fields_ = [("y", ctypes.c_void_p), ("u", ctypes.c_void_p), ("v", ctypes.c_void_p),
+                ("stride_y", ctypes.c_int32), ("stride_u", ctypes.c_int32), ("stride_v", ctypes.c_int32)]
+
+
+class RghError(RuntimeError):
+    def __init__(self, status, what):
+        self.status = status
+        super().__init__("%s: %s (status %d)" % (what, _lib.rgh_status_str(status).decode(), status))
+
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libragdoll_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libragdoll_hip.so is missing: run __graft_entry__.build() first")
+    L = ctypes.CDLL(path)
+    P, I32 = ctypes.c_void_p, ctypes.c_int32
+    L.rgh_create.restype = I32
+    L.rgh_create.argtypes = [ctypes.POINTER(RgConfig), ctypes.POINTER(ctypes.c_void_p)]
+    L.rgh_destroy.restype = I32
+    L.rgh_destroy.argtypes = [P]
+    L.rgh_set_reference.restype = I32
+    L.rgh_set_reference.argtypes = [P, ctypes.POINTER(RgI420)]
+    L.rgh_analyse_p_frame.restype = I32
+    L.rgh_analyse_p_frame.argtypes = [P, ctypes.POINTER(RgI420), P, P, ctypes.POINTER(RgI420Out)]
+    L.rgh_analyse_batch.restype = I32
+    L.rgh_analyse_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), I32, ctypes.POINTER(RgI420)]
+    L.rgh_get_outputs.restype = I32
+    L.rgh_get_outputs.argtypes = [P, P, P, ctypes.POINTER(RgI420Out), I32]
+    L.rgh_get_mb_dims.restype = I32
+    L.rgh_get_mb_dims.argtypes = [P, ctypes.POINTER(I32), ctypes.POINTER(I32)]
+    L.rgh_synchronize.restype = I32
+    L.rgh_synchronize.argtypes = [P]
+    L.rgh_get_stream.restype = P
+    L.rgh_get_stream.argtypes = [P]
+    L.rgh_status_str.restype = ctypes.c_char_p
+    L.rgh_status_str.argtypes = [I32]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def _check(status, what):
+    if status != 0:
+        raise RghError(status, what)
+
+
+def host_i420(y, u, v, width, height):
+    return RgI420(_ptr(y), _ptr(u), _ptr(v), y.strides[0], u.strides[0], v.strides[0], width, height)
+
+
+class StreamAnalyzer:
+    """One stream's P-frame analysis context (mirror of an H264Encoder's P path)."""
+
+    def __init__(self, width, height, qp, profile="baseline", me_range=0, qpel_iters=0, device=0,
+                 keep_undeblocked=True):
+        self.L = load_library()
+        cfg = RgConfig(width, height, PROFILES[profile], qp, me_range, qpel_iters, device,
+                       0 if keep_undeblocked else FLAG_NO_UNDEBLOCKED_COPY)
+        h = ctypes.c_void_p()
+        _check(self.L.rgh_create(ctypes.byref(cfg), ctypes.byref(h)), "rgh_create")
+        self.h = h
+        self.width, self.height = width, height
+        mw, mh = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.L.rgh_get_mb_dims(self.h, ctypes.byref(mw), ctypes.byref(mh)), "rgh_get_mb_dims")
+        self.mb_w, self.mb_h = mw.value, mh.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.rgh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_reference(self, y, u, v):
+        y, u, v = (np.ascontiguousarray(p, np.uint8) for p in (y, u, v))
+        W, H = self.mb_w * 16, self.mb_h * 16
+        if y.shape != (H, W):
+            raise ValueError("reference must be at coded size %dx%d" % (W, H))
+        _check(self.L.rgh_set_reference(self.h, ctypes.byref(host_i420(y, u, v, W, H))), "rgh_set_reference")
+
+    def analyse(self, y, u, v, want_levels=True, want_recon=True):
+        """Analyses one P frame; returns (decisions, levels, (y, u, v) deblocked recon)."""
+        y, u, v = (np.ascontiguousarray(p, np.uint8) for p in (y, u, v))
+        n = self.mb_w * self.mb_h
+        dec = np.zeros(n, DECISION_DTYPE)
+        lv = np.zeros((n, LEVELS_PER_MB), np.int16) if want_levels else None
+        rec = None
+        out = None
+        if want_recon:
+            W, H = self.mb_w * 16, self.mb_h * 16
+            rec = (np.zeros((H, W), np.uint8), np.zeros((H // 2, W // 2), np.uint8),
+                   np.zeros((H // 2, W // 2), np.uint8))
+            out = RgI420Out(_ptr(rec[0]), _ptr(rec[1]), _ptr(rec[2]), W, W // 2, W // 2)
+        _check(self.L.rgh_analyse_p_frame(self.h, ctypes.byref(host_i420(y, u, v, self.width, self.height)),
+                                          _ptr(dec), _ptr(lv), ctypes.byref(out) if out else None),
+               "rgh_analyse_p_frame")
+        return dec, lv, rec
+
+    def outputs(self, which_recon=1, want_levels=True):
+        n = self.mb_w * self.mb_h
+        W, H = self.mb_w * 16, self.mb_h * 16
+        dec = np.zeros(n, DECISION_DTYPE)
+        lv = np.zeros((n, LEVELS_PER_MB), np.int16) if want_levels else None
+        rec = (np.zeros((H, W), np.uint8), np.zeros((H // 2, W // 2), np.uint8), np.zeros((H // 2, W // 2), np.uint8))
+        out = RgI420Out(_ptr(rec[0]), _ptr(rec[1]), _ptr(rec[2]), W, W // 2, W // 2)
+        _check(self.L.rgh_get_outputs(self.h, _ptr(dec), _ptr(lv), ctypes.byref(out), which_recon), "rgh_get_outputs")
+        return dec, lv, rec
+
+    def stream(self):
+        return self.L.rgh_get_stream(self.h)
+
+    def synchronize(self):
+        _check(self.L.rgh_synchronize(self.h), "rgh_synchronize")
+
+
+def analyse_batch(analyzers, device_frames):
+    """One P frame for each analyzer from device-resident I420 planes.
+
+    device_frames: sequence of (y_ptr, u_ptr, v_ptr, stride_y, stride_u, stride_v) of device
+    memory holding each stream's source frame."""
+    L = load_library()
+    n = len(analyzers)
+    arr = (ctypes.c_void_p * n)(*[a.h.value for a in analyzers])
+    srcs = (RgI420 * n)()
+    for i, (a, f) in enumerate(zip(analyzers, device_frames)):
+        srcs[i] = RgI420(f[0], f[1], f[2], f[3], f[4], f[5], a.width, a.height)
+    _check(L.rgh_analyse_batch(arr, n, srcs), "rgh_analyse_batch")

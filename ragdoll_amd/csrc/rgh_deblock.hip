@@ -1,0 +1,226 @@
+// In-loop deblocking (ITU-T H.264 8.7) of an analysed P frame, one wave64 per macroblock in
+// the same anti-diagonal wavefront order as the analysis, then border extension of the
+// deblocked picture, which becomes the next reference.
+//
+// Restates ragdoll's `EncState::deblock` (crates/ragdoll-codec/src/h264/enc.rs:3776-3881), its
+// `EdgeFilter` (:3902-3942), the edge kernels of crates/ragdoll-simd/src/h264_deblock.rs
+// (:69 luma, :206 chroma scalar references) and `Plane::extend_borders` (enc.rs:164-178).
+//
+// Wavefront safety: macroblock (x, y) filters its left and top edges, writing up to 3 samples
+// into its left and top neighbours and reading 4; every macroblock whose filtering it must see
+// ((x-1, y), (x, y-1), (x+1, y-1)) lies on an earlier step t = x + 2y, every macroblock that
+// must see its result lies on a later one, and macroblocks of one step are 32 or more samples
+// apart. Each macroblock works on a 20x20 luma (10x10 chroma) copy in LDS and writes it back.
+#include "rgh_device.h"
+#include "rgh_tables.h"
+
+namespace {
+
+struct DbLds {
+    uint8_t y[20 * 20];
+    uint8_t u[10 * 10];
+    uint8_t v[10 * 10];
+};
+
+struct Edge {
+    int on;       // filter this edge
+    int alpha, beta, strong;
+    int tc0[4];   // per 4-line segment; -1 = segment not filtered
+};
+
+// enc.rs:3911 EdgeFilter::new
+__device__ Edge make_edge(int qp, const int bs[4]) {
+    Edge E;
+    int ia = iclamp(qp, 0, 51);
+    E.alpha = c_alpha[ia];
+    E.beta = c_beta[ia];
+    E.on = E.alpha != 0 && E.beta != 0;
+    E.strong = bs[0] == 4;
+    for (int k = 0; k < 4; k++) E.tc0[k] = bs[k] == 0 ? -1 : (bs[k] >= 4 ? 0 : (int)c_tc0[ia][bs[k] - 1]);
+    return E;
+}
+
+// simd/h264_deblock.rs:69 h264_luma_deblock_scalar, one line (p3..q3 at q0 - 4s .. q0 + 3s).
+__device__ void luma_line(uint8_t *b, int q0, int s, const Edge &E, int k) {
+    int tc0 = E.tc0[k >> 2];
+    if (!E.strong && tc0 < 0) return;
+    int p0 = b[q0 - s], q0v = b[q0], p1 = b[q0 - 2 * s], q1 = b[q0 + s];
+    if (iabs(p0 - q0v) >= E.alpha || iabs(p1 - p0) >= E.beta || iabs(q1 - q0v) >= E.beta) return;
+    int p2 = b[q0 - 3 * s], q2 = b[q0 + 2 * s];
+    int ap = iabs(p2 - p0) < E.beta, aq = iabs(q2 - q0v) < E.beta;
+    if (!E.strong) {
+        int tc = tc0 + ap + aq;
+        int d = iclamp((((q0v - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
+        b[q0 - s] = (uint8_t)clip255(p0 + d);
+        b[q0] = (uint8_t)clip255(q0v - d);
+        if (ap) b[q0 - 2 * s] = (uint8_t)(p1 + iclamp((p2 + ((p0 + q0v + 1) >> 1) - (p1 << 1)) >> 1, -tc0, tc0));
+        if (aq) b[q0 + s] = (uint8_t)(q1 + iclamp((q2 + ((p0 + q0v + 1) >> 1) - (q1 << 1)) >> 1, -tc0, tc0));
+    } else {
+        int p3 = b[q0 - 4 * s], q3 = b[q0 + 3 * s];
+        int small = iabs(p0 - q0v) < ((E.alpha >> 2) + 2);
+        if (ap && small) {
+            b[q0 - s] = (uint8_t)((p2 + 2 * p1 + 2 * p0 + 2 * q0v + q1 + 4) >> 3);
+            b[q0 - 2 * s] = (uint8_t)((p2 + p1 + p0 + q0v + 2) >> 2);
+            b[q0 - 3 * s] = (uint8_t)((2 * p3 + 3 * p2 + p1 + p0 + q0v + 4) >> 3);
+        } else {
+            b[q0 - s] = (uint8_t)((2 * p1 + p0 + q1 + 2) >> 2);
+        }
+        if (aq && small) {
+            b[q0] = (uint8_t)((p1 + 2 * p0 + 2 * q0v + 2 * q1 + q2 + 4) >> 3);
+            b[q0 + s] = (uint8_t)((p0 + q0v + q1 + q2 + 2) >> 2);
+            b[q0 + 2 * s] = (uint8_t)((2 * q3 + 3 * q2 + q1 + q0v + p0 + 4) >> 3);
+        } else {
+            b[q0] = (uint8_t)((2 * q1 + q0v + p1 + 2) >> 2);
+        }
+    }
+}
+
+// simd/h264_deblock.rs:206 h264_chroma_deblock_scalar, one line (p1..q1).
+__device__ void chroma_line(uint8_t *b, int q0, int s, const Edge &E, int k) {
+    int tc0 = E.tc0[k >> 1];
+    if (!E.strong && tc0 < 0) return;
+    int p1 = b[q0 - 2 * s], p0 = b[q0 - s], q0v = b[q0], q1 = b[q0 + s];
+    if (iabs(p0 - q0v) >= E.alpha || iabs(p1 - p0) >= E.beta || iabs(q1 - q0v) >= E.beta) return;
+    if (E.strong) {
+        b[q0 - s] = (uint8_t)((2 * p1 + p0 + q1 + 2) >> 2);
+        b[q0] = (uint8_t)((2 * q1 + q0v + p1 + 2) >> 2);
+    } else {
+        int tc = tc0 + 1;
+        int d = iclamp((((q0v - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
+        b[q0 - s] = (uint8_t)clip255(p0 + d);
+        b[q0] = (uint8_t)clip255(q0v - d);
+    }
+}
+
+__device__ __forceinline__ int is_intra(const rgh_mb_decision &d) { return d.mb_type >= 2; }
+__device__ __forceinline__ int quiet(const rgh_mb_decision &d) { return !is_intra(d) && d.cbp_luma == 0; }
+__device__ __forceinline__ uint32_t nz_col(uint32_t nnz, int c) {
+    return ((nnz >> c) & 1) | (((nnz >> (4 + c)) & 1) << 1) | (((nnz >> (8 + c)) & 1) << 2) | (((nnz >> (12 + c)) & 1) << 3);
+}
+
+__device__ void deblock_mb(const RghJob &J, int mb_x, int mb_y, DbLds &S) {
+    const int lane = threadIdx.x, mb_w = J.mb_w, m = mb_y * mb_w + mb_x;
+    const rgh_mb_decision Q = J.dec[m];
+    rgh_mb_decision L, T;
+    if (mb_x > 0) L = J.dec[m - 1];
+    if (mb_y > 0) T = J.dec[m - mb_w];
+    auto mveq = [](const rgh_mb_decision &a, const rgh_mb_decision &b) { return a.mv[0] == b.mv[0] && a.mv[1] == b.mv[1]; };
+    if (quiet(Q) && (mb_x == 0 || (quiet(L) && mveq(L, Q))) && (mb_y == 0 || (quiet(T) && mveq(T, Q)))) return;
+
+    const int x0 = mb_x * 16, y0 = mb_y * 16, cx0 = mb_x * 8, cy0 = mb_y * 8;
+    for (int i = lane; i < 400; i += 64) S.y[i] = J.cur_y[(long)(y0 - 4 + i / 20) * J.sy + x0 - 4 + i % 20];
+    for (int i = lane; i < 100; i += 64) {
+        long o = (long)(cy0 - 2 + i / 10) * J.sc + cx0 - 2 + i % 10;
+        S.u[i] = J.cur_u[o];
+        S.v[i] = J.cur_v[o];
+    }
+    __syncthreads();
+
+    const int intra = is_intra(Q), internal = intra || Q.cbp_luma != 0, skip_odd = Q.transform_8x8;
+    const int qp_q = Q.qp, qpc_q = c_chroma_qp[iclamp(qp_q, 0, 51)];
+    for (int dir = 0; dir < 2; dir++) {
+        for (int e = 0; e < 4; e++) {
+            const rgh_mb_decision *P;
+            if (e == 0) {
+                if (dir == 0 && mb_x > 0)
+                    P = &L;
+                else if (dir == 1 && mb_y > 0)
+                    P = &T;
+                else
+                    continue;
+            } else {
+                if (!internal || ((e & 1) && skip_odd)) continue;
+                P = &Q;
+            }
+            int bs[4];
+            if (intra || is_intra(*P)) {
+                for (int k = 0; k < 4; k++) bs[k] = e == 0 ? 4 : 3;
+            } else {
+                uint32_t coded;
+                if (dir == 0 && e == 0)
+                    coded = nz_col(Q.nnz, 0) | nz_col(P->nnz, 3);
+                else if (dir == 0)
+                    coded = nz_col(Q.nnz, e) | nz_col(Q.nnz, e - 1);
+                else if (e == 0)
+                    coded = (Q.nnz & 0xF) | ((uint32_t)P->nnz >> 12);
+                else
+                    coded = ((Q.nnz >> (4 * e)) & 0xF) | ((Q.nnz >> (4 * (e - 1))) & 0xF);
+                int motion = e == 0 && (iabs(Q.mv[0] - P->mv[0]) >= 4 || iabs(Q.mv[1] - P->mv[1]) >= 4);
+                if (!coded && !motion) continue;
+                for (int k = 0; k < 4; k++) bs[k] = ((coded >> k) & 1) ? 2 : motion;
+            }
+            int qp = qp_q, qpc = qpc_q;
+            if (P != &Q) {
+                int qp_p = P->qp;
+                qp = (qp_q + qp_p + 1) >> 1;
+                qpc = (qpc_q + c_chroma_qp[iclamp(qp_p, 0, 51)] + 1) >> 1;
+            }
+            Edge EL = make_edge(qp, bs);
+            if (EL.on && lane < 16) {
+                if (dir == 0)
+                    luma_line(S.y, (4 + lane) * 20 + 4 + e * 4, 1, EL, lane);
+                else
+                    luma_line(S.y, (4 + e * 4) * 20 + 4 + lane, 20, EL, lane);
+            }
+            if (!(e & 1)) {
+                Edge EC = make_edge(qpc, bs);
+                if (EC.on && lane >= 16 && lane < 32) {
+                    int k = lane & 7;
+                    uint8_t *pl = lane < 24 ? S.u : S.v;
+                    if (dir == 0)
+                        chroma_line(pl, (2 + k) * 10 + 2 + e * 2, 1, EC, k);
+                    else
+                        chroma_line(pl, (2 + e * 2) * 10 + 2 + k, 10, EC, k);
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // Write back all but the untouched top-left corner.
+    for (int i = lane; i < 400; i += 64) {
+        int r = i / 20, c = i % 20;
+        if (r < 4 && c < 4) continue;
+        J.cur_y[(long)(y0 - 4 + r) * J.sy + x0 - 4 + c] = S.y[i];
+    }
+    for (int i = lane; i < 100; i += 64) {
+        int r = i / 10, c = i % 10;
+        if (r < 2 && c < 2) continue;
+        long o = (long)(cy0 - 2 + r) * J.sc + cx0 - 2 + c;
+        J.cur_u[o] = S.u[i];
+        J.cur_v[o] = S.v[i];
+    }
+}
+
+}  // namespace
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_deblock_step(const RghJob *__restrict__ jobs, int t) {
+    const RghJob J = jobs[blockIdx.y];
+    if (!J.deblock || t >= J.steps) return;
+    int a = t - J.mb_w + 1;
+    int ylo = a > 0 ? (a + 1) >> 1 : 0, yhi = min(J.mb_h - 1, t >> 1);
+    int y = ylo + (int)blockIdx.x;
+    if (y > yhi) return;
+    __shared__ DbLds S;
+    deblock_mb(J, t - 2 * y, y, S);
+}
+
+// Edge replication of the padding of job blockIdx.z's current picture (enc.rs:164
+// extend_borders): plane blockIdx.y (0 Y, 1 U, 2 V), padded row blockIdx.x.
+extern "C" __global__ __launch_bounds__(256) void rgh_k_extend(const RghJob *__restrict__ jobs) {
+    const RghJob J = jobs[blockIdx.z];
+    const int plane = blockIdx.y;
+    const int pad = plane ? RGH_PAD_C : RGH_PAD_Y;
+    const int w = plane ? J.mb_w * 8 : J.mb_w * 16, h = plane ? J.mb_h * 8 : J.mb_h * 16;
+    const int s = plane ? J.sc : J.sy;
+    const int prow = blockIdx.x;
+    if (prow >= h + 2 * pad) return;
+    uint8_t *base = plane == 0 ? J.cur_y : (plane == 1 ? J.cur_u : J.cur_v);  // sample (0, 0)
+    const int y = prow - pad;
+    const int sy = iclamp(y, 0, h - 1);
+    const bool interior_row = y >= 0 && y < h;
+    for (int c = threadIdx.x; c < s; c += blockDim.x) {
+        int x = c - pad;
+        if (interior_row && x >= 0 && x < w) continue;
+        base[(long)y * s + x] = base[(long)sy * s + iclamp(x, 0, w - 1)];
+    }
+}

@@ -1,0 +1,56 @@
+// Device-side definitions shared by the analysis and deblocking kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ragdoll_hip.h"
+
+#define RGH_PAD_Y 48      // luma padding of reconstructed pictures (enc.rs:56)
+#define RGH_PAD_C 16      // chroma padding (enc.rs:58)
+#define RGH_MV_MARGIN 28  // how far a vector may move a block past the edge (enc.rs:61)
+
+// One stream's P frame in a batched launch. Plane pointers address sample (0, 0) inside the
+// padding, so negative offsets reach the padding.
+struct RghJob {
+    const uint8_t *src_y, *src_u, *src_v;
+    int32_t src_sy, src_su, src_sv, width, height;
+    const uint8_t *ref_y, *ref_u, *ref_v;
+    uint8_t *cur_y, *cur_u, *cur_v;
+    int32_t sy, sc;            // padded strides
+    int32_t mb_w, mb_h, qp, me_iters, qpel_iters, t8x8;
+    rgh_mb_decision *dec;
+    const rgh_mb_decision *prev;
+    int16_t *levels;
+    uint32_t sb_luma, sb_luma_ssd, sb_chroma, sb_chroma_ssd, sb_probe;
+    int32_t steps;             // wavefront steps of the frame: (mb_w - 1) + 2 (mb_h - 1) + 1
+    int32_t deblock;           // 0 when the slice disables the loop filter
+};
+
+// Sum of a 32-bit value over the 64 lanes of the wave, broadcast as a scalar. DPP quad and row
+// mirrors reduce each row of 16 lanes; the four row sums are read out with readlane. Every lane
+// of the wave must be active.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);  // row_mirror
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 0) + (uint32_t)__builtin_amdgcn_readlane((int)v, 16) +
+           (uint32_t)__builtin_amdgcn_readlane((int)v, 32) + (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+}
+
+// Four bytes at any address, from the two aligned dwords that hold them. The second dword must
+// lie inside the allocation (true for every padded-plane read of the kernels).
+__device__ __forceinline__ uint32_t ld4u(const uint8_t *p) {
+    uintptr_t a = (uintptr_t)p;
+    const uint32_t *q = (const uint32_t *)(a & ~(uintptr_t)3);
+    return __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(a & 3));
+}
+
+__device__ __forceinline__ int clip255(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
+__device__ __forceinline__ int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
+__device__ __forceinline__ uint32_t byte_of(uint32_t w, int i) { return (w >> (8 * i)) & 0xFF; }
+
+// Column and row of 4x4 block b in decoding order (enc.rs:432 blk_xy).
+__device__ __forceinline__ int blk_x(int b) { return (b & 1) + ((b >> 1) & 2); }
+__device__ __forceinline__ int blk_y(int b) { return ((b >> 1) & 1) + ((b >> 2) & 2); }

@@ -66,10 +66,14 @@ class _Rng:
         return (self.next() >> 11) % n
 
     def bytes(self, n):
-        out = np.empty(n, np.uint64)
-        for i in range(n):
-            out[i] = self.next()
-        return out
+        """n 64-bit words of a splitmix64 stream keyed by the next generator output
+        (vectorized; used for textures and noise fields)."""
+        key = np.uint64(self.next())
+        with np.errstate(over="ignore"):
+            z = key + (np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15))
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            return z ^ (z >> np.uint64(31))
 
 
 def _gauss_table():

@@ -13,7 +13,7 @@ DECISION_DTYPE = np.dtype([
     ("mb_type", "u1"), ("i16_mode", "u1"), ("chroma_mode", "u1"), ("cbp_luma", "u1"),
     ("cbp_chroma", "u1"), ("transform_8x8", "u1"), ("qp", "u1"), ("flags", "u1"),
     ("mv", "<i2", (2,)), ("mvp", "<i2", (2,)), ("cost", "<u4"), ("n_sad", "<u2"),
-    ("n_subpel", "<u2"), ("n_satd", "<u2"), ("n_hpel", "<u2"), ("reserved", "<u4"),
+    ("n_subpel", "<u2"), ("n_satd", "<u2"), ("n_hpel", "<u2"), ("nnz", "<u2"), ("reserved", "<u2"),
 ])
 assert DECISION_DTYPE.itemsize == 32
 
