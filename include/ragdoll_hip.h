@@ -173,6 +173,15 @@ rgh_status rgh_synchronize(void *ctx);
 /* HIP stream (hipStream_t) the context's work runs on, for callers that time it. */
 void *rgh_get_stream(void *ctx);
 
+/* Device-time measurement of the P-frame launches (HIP events on the context's stream).
+ * When enabled, each rgh_analyse_p_frame / rgh_analyse_batch records events around its
+ * analysis wavefront and its deblock + border-extension launches; rgh_get_timing waits for the
+ * last such frame and returns both spans in milliseconds and the number of analysis kernel
+ * launches they covered. No counterpart in the reference (its telemetry is wall-clock
+ * PipelineTelemetry, crates/ragdoll-pipeline/src/lib.rs:62-76). */
+rgh_status rgh_enable_timing(void *ctx, int32_t enable);
+rgh_status rgh_get_timing(void *ctx, float *analyse_ms, float *deblock_ms, int32_t *analyse_launches);
+
 /* A human-readable description of a status code. */
 const char *rgh_status_str(int32_t status);
 

@@ -20,7 +20,7 @@ assert DECISION_DTYPE.itemsize == 32
 
 EXPORTS = ["rgh_create", "rgh_destroy", "rgh_set_reference", "rgh_analyse_p_frame",
            "rgh_analyse_batch", "rgh_get_outputs", "rgh_get_mb_dims", "rgh_synchronize",
-           "rgh_get_stream", "rgh_status_str"]
+           "rgh_get_stream", "rgh_enable_timing", "rgh_get_timing", "rgh_status_str"]
 
 
 class RgConfig(ctypes.Structure):
@@ -76,6 +76,11 @@ def load_library(path=LIB_PATH):
     L.rgh_synchronize.argtypes = [P]
     L.rgh_get_stream.restype = P
     L.rgh_get_stream.argtypes = [P]
+    L.rgh_enable_timing.restype = I32
+    L.rgh_enable_timing.argtypes = [P, I32]
+    L.rgh_get_timing.restype = I32
+    L.rgh_get_timing.argtypes = [P, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                 ctypes.POINTER(I32)]
     L.rgh_status_str.restype = ctypes.c_char_p
     L.rgh_status_str.argtypes = [I32]
     _lib = L
@@ -153,6 +158,15 @@ class StreamAnalyzer:
         out = RgI420Out(_ptr(rec[0]), _ptr(rec[1]), _ptr(rec[2]), W, W // 2, W // 2)
         _check(self.L.rgh_get_outputs(self.h, _ptr(dec), _ptr(lv), ctypes.byref(out), which_recon), "rgh_get_outputs")
         return dec, lv, rec
+
+    def enable_timing(self, on=True):
+        _check(self.L.rgh_enable_timing(self.h, 1 if on else 0), "rgh_enable_timing")
+
+    def timing(self):
+        """(analysis ms, deblock+extend ms, analysis launches) of the last timed frame."""
+        a, d, n = ctypes.c_float(), ctypes.c_float(), ctypes.c_int32()
+        _check(self.L.rgh_get_timing(self.h, ctypes.byref(a), ctypes.byref(d), ctypes.byref(n)), "rgh_get_timing")
+        return a.value, d.value, n.value
 
     def stream(self):
         return self.L.rgh_get_stream(self.h)

@@ -67,6 +67,9 @@ struct Ctx {
     bool analysed = false;
     bool keep_pre = true;
     uint32_t sb[5] = {0, 0, 0, 0, 0};
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};  // analysis start, analysis end, frame end
+    bool timing = false, timed = false;
+    int timed_launches = 0;
 };
 
 rgh_status hip_status(hipError_t e) {
@@ -94,6 +97,8 @@ void free_ctx(Ctx *c) {
     for (auto &b : c->pre)
         if (b) (void)hipFree(b);
     if (c->jobs_dev) (void)hipFree(c->jobs_dev);
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -147,8 +152,11 @@ rgh_status run_jobs(Ctx *const *ctxs, int n, const std::vector<RghJob> &jobs, hi
         rows = std::max(rows, ctxs[i]->rows_y);
     }
     dim3 grid((unsigned)width, (unsigned)n);
+    Ctx *c0 = ctxs[0];
+    if (c0->timing) HIP_TRY(hipEventRecord(c0->ev[0], stream));
     for (int t = 0; t < steps; t++) hipLaunchKernelGGL(rgh_k_analyse_step, grid, dim3(64), 0, stream, jobs_dev, t);
     HIP_TRY(hipGetLastError());
+    if (c0->timing) HIP_TRY(hipEventRecord(c0->ev[1], stream));
     for (int i = 0; i < n; i++) {
         Ctx *c = ctxs[i];
         if (c->keep_pre) {
@@ -163,6 +171,11 @@ rgh_status run_jobs(Ctx *const *ctxs, int n, const std::vector<RghJob> &jobs, hi
     for (int t = 0; t < steps; t++) hipLaunchKernelGGL(rgh_k_deblock_step, grid, dim3(64), 0, stream, jobs_dev, t);
     hipLaunchKernelGGL(rgh_k_extend, dim3((unsigned)rows, 3, (unsigned)n), dim3(256), 0, stream, jobs_dev);
     HIP_TRY(hipGetLastError());
+    if (c0->timing) {
+        HIP_TRY(hipEventRecord(c0->ev[2], stream));
+        c0->timed = true;
+        c0->timed_launches = steps;
+    }
     for (int i = 0; i < n; i++) {
         Ctx *c = ctxs[i];
         c->ref = 1 - c->ref;
@@ -430,5 +443,28 @@ rgh_status rgh_synchronize(void *ctx) {
 }
 
 void *rgh_get_stream(void *ctx) { return ctx ? static_cast<Ctx *>(ctx)->stream : nullptr; }
+
+rgh_status rgh_enable_timing(void *ctx, int32_t enable) {
+    if (!ctx) return RGH_ERR_INVALID_ARGUMENT;
+    Ctx *c = static_cast<Ctx *>(ctx);
+    HIP_TRY(hipSetDevice(c->device));
+    if (enable && !c->ev[0])
+        for (auto &e : c->ev) HIP_TRY(hipEventCreate(&e));
+    c->timing = enable != 0;
+    c->timed = false;
+    return RGH_OK;
+}
+
+rgh_status rgh_get_timing(void *ctx, float *analyse_ms, float *deblock_ms, int32_t *launches) {
+    if (!ctx || !analyse_ms || !deblock_ms || !launches) return RGH_ERR_INVALID_ARGUMENT;
+    Ctx *c = static_cast<Ctx *>(ctx);
+    if (!c->timed) return RGH_ERR_INVALID_ARGUMENT;
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipEventSynchronize(c->ev[2]));
+    HIP_TRY(hipEventElapsedTime(analyse_ms, c->ev[0], c->ev[1]));
+    HIP_TRY(hipEventElapsedTime(deblock_ms, c->ev[1], c->ev[2]));
+    *launches = c->timed_launches;
+    return RGH_OK;
+}
 
 }  // extern "C"

@@ -1,0 +1,52 @@
+"""The C ABI library loads and exports every entry point include/ragdoll_hip.h declares."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import ragdoll_amd
+from ragdoll_amd import abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared():
+    text = open(os.path.join(ROOT, "include", "ragdoll_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rgh_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = abi.load_library()
+    names = declared()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(abi.EXPORTS)
+
+
+def test_status_strings():
+    L = abi.load_library()
+    assert L.rgh_status_str(0) == b"ok"
+    assert L.rgh_status_str(3) == b"invalid dimensions"
+    assert L.rgh_status_str(999) == b"unknown status"
+
+
+def test_argument_errors_return_codes_before_touching_a_device():
+    L = abi.load_library()
+    h = ctypes.c_void_p()
+    assert L.rgh_create(None, ctypes.byref(h)) == 1
+    for w, hh, prof, qp in [(0, 16, 0, 26), (17, 16, 0, 26), (16, 16, 7, 26), (16, 16, 0, 0), (16, 16, 0, 52)]:
+        cfg = abi.RgConfig(w, hh, prof, qp, 0, 0, 0, 0)
+        assert L.rgh_create(ctypes.byref(cfg), ctypes.byref(h)) in (2, 3)
+    assert L.rgh_destroy(None) == 1
+    assert L.rgh_analyse_p_frame(None, None, None, None, None) == 1
+    assert L.rgh_analyse_batch(None, 0, None) == 1
+
+
+def test_record_layouts_match_between_product_and_oracle():
+    from oracle_lib import DECISION_DTYPE, LEVELS_PER_MB
+    assert DECISION_DTYPE == abi.DECISION_DTYPE
+    assert LEVELS_PER_MB == abi.LEVELS_PER_MB
+    assert ctypes.sizeof(abi.RgConfig) == 32
