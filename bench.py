@@ -34,13 +34,40 @@ QPS = (24, 30, 36)
 W, H = 1920, 1080
 
 
-def hip_lib():
-    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
-        try:
-            return ctypes.CDLL(name)
-        except OSError:
-            continue
-    raise RuntimeError("libamdhip64.so not found")
+class Hip:
+    """The ROCm HIP runtime that libragdoll_hip.so itself links (one runtime per process: a
+    second copy, such as the one bundled with PyTorch, must not be initialised beside it)."""
+
+    def __init__(self):
+        self.L = None
+        for name in ("/opt/rocm/lib/libamdhip64.so.7", "libamdhip64.so.7"):
+            try:
+                self.L = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.L is None:
+            raise RuntimeError("libamdhip64.so.7 not found")
+        self.L.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.L.hipFree.argtypes = [ctypes.c_void_p]
+
+    def check(self, rc, what):
+        if rc != 0:
+            raise RuntimeError("%s failed with HIP error %d" % (what, rc))
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        p = ctypes.c_void_p()
+        self.check(self.L.hipMalloc(ctypes.byref(p), arr.nbytes), "hipMalloc")
+        self.check(self.L.hipMemcpy(p, arr.ctypes.data, arr.nbytes, 1), "hipMemcpy")  # host to device
+        return p.value
+
+    def free(self, p):
+        self.L.hipFree(ctypes.c_void_p(p))
+
+    def synchronize(self):
+        self.check(self.L.hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
 def render_frames(n_frames):
@@ -106,7 +133,6 @@ def main():
         print("steps + warmup must be <= %d (P frames per GOP)" % (GOP - 1), file=sys.stderr)
         sys.exit(2)
 
-    import torch
     from ragdoll_amd import StreamAnalyzer, analyse_batch
 
     n_frames = 120
@@ -114,12 +140,12 @@ def main():
     frames = render_frames(n_frames)
     print("rendered %d frames in %.1f s" % (n_frames, time.time() - t0), file=sys.stderr, flush=True)
 
-    dev = torch.device("cuda:0")
+    hip = Hip()
     cw, ch = W // 2, H // 2
-    dy = torch.from_numpy(np.stack([f[0] for f in frames])).to(dev)
-    du = torch.from_numpy(np.stack([f[1] for f in frames])).to(dev)
-    dv = torch.from_numpy(np.stack([f[2] for f in frames])).to(dev)
-    torch.cuda.synchronize()
+    dy = hip.upload(np.stack([f[0] for f in frames]))
+    du = hip.upload(np.stack([f[1] for f in frames]))
+    dv = hip.upload(np.stack([f[2] for f in frames]))
+    hip.synchronize()
 
     n_gops = n_frames // GOP
     streams = [(qp, g) for qp in QPS for g in range(n_gops)]
@@ -135,14 +161,14 @@ def main():
     mbs_per_frame = ans[0].mb_w * ans[0].mb_h
 
     def frame_ptrs(i):
-        return (dy[i].data_ptr(), du[i].data_ptr(), dv[i].data_ptr(), W, cw, cw)
+        return (dy + i * W * H, du + i * cw * ch, dv + i * cw * ch, W, cw, cw)
 
     def step(k):
         analyse_batch(ans, [frame_ptrs(g * GOP + 1 + k) for (_, g) in streams])
 
     for k in range(args.warmup):
         step(k)
-    torch.cuda.synchronize()
+    hip.synchronize()
     analyse_ms = deblock_ms = 0.0
     launches = 0
     t0 = time.perf_counter()
@@ -152,7 +178,7 @@ def main():
         analyse_ms += a_ms
         deblock_ms += d_ms
         launches += n_l
-    torch.cuda.synchronize()
+    hip.synchronize()
     el = time.perf_counter() - t0
 
     total_mbs = args.steps * len(streams) * mbs_per_frame
@@ -184,6 +210,8 @@ def main():
     }
     for a in ans:
         a.close()
+    for p in (dy, du, dv):
+        hip.free(p)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(frames, args.cpu_seconds)
     print(json.dumps(out), flush=True)

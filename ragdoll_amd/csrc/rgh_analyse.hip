@@ -31,23 +31,23 @@ constexpr int WP_COLS = 24;  // half-pel plane columns (HPEL_COLS)
 constexpr int HR_ROWS = 23;  // unrolled horizontal-tap rows
 
 struct MbLds {
-    uint8_t src[256], srcu[64], srcv[64];
-    uint8_t pred[256], predu[64], predv[64];
-    uint8_t tmp[256];
-    uint8_t sr[SR_H * SR_W];
-    int16_t hr[HR_ROWS * WP_COLS];
-    uint8_t ph[WP_ROWS * WP_COLS], pv[WP_ROWS * WP_COLS], pc[WP_ROWS * WP_COLS];
-    int16_t coef[16][16];  // luma levels, decoding order, raster positions
-    int16_t q8[4][64];     // 8x8 levels, raster positions
-    int32_t t8[4][64];     // 8x8 transform intermediate
-    int16_t craster[2][4][16];
-    int16_t cdc_raw[2][4];
-    int16_t cdc[2][4];
-    int16_t cdcr[2][4];
-    int16_t dc_raw[16], dcq[16], dcr[16];
-    int32_t bnz[16], bbig[16], bscore[16], cscore[8], cmask[8], b8score[4];
-    uint8_t etop[16], eleft[16], ctop[2][8], cleft[2][8];
-    int32_t etl, ctl[2];
+    alignas(16) uint8_t src[256], srcu[64], srcv[64];
+    alignas(16) uint8_t pred[256], predu[64], predv[64];
+    alignas(16) uint8_t tmp[256];
+    alignas(16) uint8_t sr[SR_H * SR_W];
+    alignas(16) int16_t hr[HR_ROWS * WP_COLS];
+    alignas(16) uint8_t ph[WP_ROWS * WP_COLS], pv[WP_ROWS * WP_COLS], pc[WP_ROWS * WP_COLS];
+    alignas(16) int16_t coef[16][16];  // luma levels, decoding order, raster positions
+    alignas(16) int16_t q8[4][64];     // 8x8 levels, raster positions
+    alignas(16) int32_t t8[4][64];     // 8x8 transform intermediate
+    alignas(16) int16_t craster[2][4][16];
+    alignas(16) int16_t cdc_raw[2][4];
+    alignas(16) int16_t cdc[2][4];
+    alignas(16) int16_t cdcr[2][4];
+    alignas(16) int16_t dc_raw[16], dcq[16], dcr[16];
+    alignas(16) int32_t bnz[16], bbig[16], bscore[16], cscore[8], cmask[8], b8score[4];
+    alignas(16) uint8_t etop[16], eleft[16], ctop[2][8], cleft[2][8];
+    alignas(16) int32_t etl, ctl[2];
 };
 
 __device__ __forceinline__ uint32_t se_bits(int d) {  // enc.rs:546
@@ -200,43 +200,111 @@ __device__ __forceinline__ void add_idct4(uint8_t *dst, int s, int x, int y, con
 // ---------------------------------------------------------------------------------------
 // Reference staging and the half-pel window (simd/h264_enc.rs:704-838).
 // ---------------------------------------------------------------------------------------
-// Copies reference rows [oy, oy + rows) x columns [ox, ox + SR_COLS) (ox a multiple of 4,
-// rows <= SR_H) into S.sr with aligned dword loads.
-__device__ void stage_region(const RghJob &J, int ox, int oy, int rows, MbLds &S) {
+// Copies reference rows [oy, oy + ROWS) x columns [ox, ox + SR_COLS) (ox a multiple of 4)
+// into S.sr: every lane issues all of its aligned dword loads before the first LDS store.
+template <int ROWS>
+__device__ void stage_region(const RghJob &J, int ox, int oy, MbLds &S) {
+    static_assert(ROWS <= SR_H, "region too tall");
+    constexpr int DW = SR_COLS / 4;  // 17 dwords a row
+    constexpr int N = ROWS * DW, PER = (N + 63) / 64;
     const int lane = threadIdx.x;
     const uint8_t *base = J.ref_y + (long)oy * J.sy + ox;
-    constexpr int DW = SR_COLS / 4;  // 17 dwords a row
-    for (int i = lane; i < rows * DW; i += 64) {
-        int k = i / DW, j = i - k * DW;
-        *(uint32_t *)&S.sr[k * SR_W + j * 4] = *(const uint32_t *)(base + (long)k * J.sy + j * 4);
+    uint32_t v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = lane + 64 * k, r = i / DW, j = i - r * DW;
+        v[k] = i < N ? *(const uint32_t *)(base + (long)r * J.sy + j * 4) : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+        const int i = lane + 64 * k, r = i / DW, j = i - r * DW;
+        if (i < N) *(uint32_t *)&S.sr[r * SR_W + j * 4] = v[k];
     }
     __syncthreads();
 }
 
+// Horizontal 6-tap sums of four consecutive outputs from the nine bytes starting at byte o of
+// the three dwords w[0..2].
+__device__ __forceinline__ void tap6x4(const uint32_t *w, int o, int *out) {
+    const uint64_t lo = ((uint64_t)w[1] << 32) | w[0];
+    const uint64_t hi = ((uint64_t)w[2] << 32) | w[1];
+    int b[9];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const int k = o + i;  // 0..11
+        b[i] = k < 8 ? (int)((lo >> (8 * k)) & 0xFF) : (int)((hi >> (8 * (k - 4))) & 0xFF);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) out[i] = tap6(b[i], b[i + 1], b[i + 2], b[i + 3], b[i + 4], b[i + 5]);
+}
+
 // The b (ph), h (pv) and j (pc) planes over 24 x 18 samples whose top-left integer sample
-// (relative -1, -1 of the full-pel anchor) is region sample (wx0, wy0); the region must hold
-// rows wy0 - 2 .. wy0 + 20 and columns wx0 - 2 .. wx0 + 28 (hpel_window_scalar :765 and
-// hpel_window_centre_scalar :820, the centre computed up front).
+// (-1, -1 from the full-pel anchor) is region sample (wx0, wy0); the region holds rows
+// wy0 - 2 .. wy0 + 20 and columns wx0 - 2 .. wx0 + 28 (simd/h264_enc.rs:765 hpel_window_scalar
+// and :820 hpel_window_centre_scalar, the centre computed up front). Work items produce four
+// adjacent outputs from dword LDS reads: pass 1 the unrounded horizontal taps of rows -2 .. 20
+// with the b plane and the h plane; pass 2 the j plane from the kept taps.
 __device__ void compute_window(MbLds &S, int wx0, int wy0) {
     const int lane = threadIdx.x;
-    for (int i = lane; i < HR_ROWS * WP_COLS; i += 64) {
-        int k = i / WP_COLS, c = i - k * WP_COLS;
-        const uint8_t *p = &S.sr[(wy0 - 2 + k) * SR_W + wx0 + c - 2];
-        S.hr[i] = (int16_t)tap6(p[0], p[1], p[2], p[3], p[4], p[5]);
-    }
-    for (int i = lane; i < WP_ROWS * WP_COLS; i += 64) {
-        int r = i / WP_COLS, c = i - r * WP_COLS;
-        const uint8_t *p = &S.sr[(wy0 - 2 + r) * SR_W + wx0 + c];
-        int v = tap6(p[0], p[SR_W], p[2 * SR_W], p[3 * SR_W], p[4 * SR_W], p[5 * SR_W]);
-        S.pv[i] = (uint8_t)clip255((v + 16) >> 5);
+    constexpr int NA = HR_ROWS * 6, NB = WP_ROWS * 6;
+#pragma unroll
+    for (int it = 0; it < (NA + NB + 63) / 64; it++) {
+        const int item = lane + 64 * it;
+        if (item < NA) {
+            const int k = item / 6, g = item - k * 6;
+            const int x = wx0 + 4 * g - 2, xa = x & ~3;
+            const uint8_t *row = &S.sr[(wy0 - 2 + k) * SR_W + xa];
+            const uint32_t w[3] = {*(const uint32_t *)row, *(const uint32_t *)(row + 4), *(const uint32_t *)(row + 8)};
+            int h[4];
+            tap6x4(w, x - xa, h);
+            uint32_t *hp = (uint32_t *)&S.hr[k * WP_COLS + 4 * g];
+            hp[0] = ((uint32_t)(uint16_t)h[0]) | ((uint32_t)(uint16_t)h[1] << 16);
+            hp[1] = ((uint32_t)(uint16_t)h[2]) | ((uint32_t)(uint16_t)h[3] << 16);
+            if (k >= 2 && k < 2 + WP_ROWS) {
+                uint32_t o = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++) o |= (uint32_t)clip255((h[i] + 16) >> 5) << (8 * i);
+                *(uint32_t *)&S.ph[(k - 2) * WP_COLS + 4 * g] = o;
+            }
+        } else if (item < NA + NB) {
+            const int i2 = item - NA, r = i2 / 6, g = i2 - r * 6;
+            uint32_t w[6];
+#pragma unroll
+            for (int t = 0; t < 6; t++) w[t] = lds4u(&S.sr[(wy0 - 2 + r + t) * SR_W], wx0 + 4 * g);
+            uint32_t o = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int v = tap6(byte_of(w[0], i), byte_of(w[1], i), byte_of(w[2], i), byte_of(w[3], i), byte_of(w[4], i),
+                                   byte_of(w[5], i));
+                o |= (uint32_t)clip255((v + 16) >> 5) << (8 * i);
+            }
+            *(uint32_t *)&S.pv[r * WP_COLS + 4 * g] = o;
+        }
     }
     __syncthreads();
-    for (int i = lane; i < WP_ROWS * WP_COLS; i += 64) {
-        int r = i / WP_COLS, c = i - r * WP_COLS;
-        const int16_t *t = &S.hr[r * WP_COLS + c];
-        S.ph[i] = (uint8_t)clip255((t[2 * WP_COLS] + 16) >> 5);
-        int j = tap6(t[0], t[WP_COLS], t[2 * WP_COLS], t[3 * WP_COLS], t[4 * WP_COLS], t[5 * WP_COLS]);
-        S.pc[i] = (uint8_t)clip255((j + 512) >> 10);
+#pragma unroll
+    for (int it = 0; it < (NB + 63) / 64; it++) {
+        const int item = lane + 64 * it;
+        if (item < NB) {
+            const int r = item / 6, g = item - r * 6;
+            int t[6][4];
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const uint32_t *hp = (const uint32_t *)&S.hr[(r + q) * WP_COLS + 4 * g];
+                const uint32_t a = hp[0], b = hp[1];
+                t[q][0] = (int16_t)(a & 0xFFFF);
+                t[q][1] = (int16_t)(a >> 16);
+                t[q][2] = (int16_t)(b & 0xFFFF);
+                t[q][3] = (int16_t)(b >> 16);
+            }
+            uint32_t o = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int j = tap6(t[0][i], t[1][i], t[2][i], t[3][i], t[4][i], t[5][i]);
+                o |= (uint32_t)clip255((j + 512) >> 10) << (8 * i);
+            }
+            *(uint32_t *)&S.pc[r * WP_COLS + 4 * g] = o;
+        }
     }
     __syncthreads();
 }
@@ -271,12 +339,11 @@ __device__ uint32_t window_pred4(int x0, int y0, int bx, int by, int ox, int oy,
     auto fetch4 = [&](int k, int dx, int dy) -> uint32_t {
         if (k == 0) return lds4u(&S.sr[(iy + dy + rr - oy) * SR_W], ix + dx + c0 - ox);
         const uint8_t *pl = k == 1 ? S.ph : (k == 2 ? S.pv : S.pc);
-        const uint8_t *p = &pl[(wy + dy + rr) * WP_COLS + wx + dx + c0];
-        return (uint32_t)p[0] | ((uint32_t)p[1] << 8) | ((uint32_t)p[2] << 16) | ((uint32_t)p[3] << 24);
+        return lds4u(&pl[(wy + dy + rr) * WP_COLS], wx + dx + c0);
     };
-    uint32_t a = fetch4(ka, dxa, 0);
+    const uint32_t a = fetch4(ka, dxa, 0);
     if (kb < 0) return a;
-    uint32_t b = fetch4(kb, dxb, dyb);
+    const uint32_t b = fetch4(kb, dxb, dyb);
     // Rounded byte-wise average (a + b + 1) >> 1 without carries between bytes.
     return (a | b) - (((a ^ b) >> 1) & 0x7F7F7F7Fu);
 }
@@ -662,7 +729,7 @@ __device__ void analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S) {
             // (the window table equals h264_luma_mc: enc.rs:4030 test).
             const int fbx = x0 + (pskx >> 2), fby = y0 + (psky >> 2);
             const int ox = (fbx - 3) & ~3, oy = fby - 3;
-            stage_region(J, ox, oy, 23, S);
+            stage_region<23>(J, ox, oy, S);
             compute_window(S, fbx - 1 - ox, fby - 1 - oy);
             predw = window_pred4(x0, y0, fbx, fby, ox, oy, pskx, psky, S);
         }
@@ -754,26 +821,35 @@ __device__ void analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S) {
         // them; positions beyond it (me_range > 16) read the reference directly.
         const int bxa = x0 + bestx, bya = y0 + besty;
         const int ox = (bxa - 19) & ~3, oy = bya - 19;
-        stage_region(J, ox, oy, SR_H, S);
-        auto sad_at = [&](int fx, int fy) {
+        stage_region<SR_H>(J, ox, oy, S);
+        auto sad_part = [&](int fx, int fy) {
             const int px = x0 + fx - ox, py = y0 + fy - oy;
             uint32_t w;
             if (px >= 0 && px + 16 <= SR_COLS && py >= 0 && py + 16 <= SR_H)
                 w = lds4u(&S.sr[(py + rr) * SR_W], px + c0);
             else
                 w = ld4u(rb + (long)fy * J.sy + fx);
-            return wave_sum(__builtin_amdgcn_sad_u8(srcw, w, 0));
+            return __builtin_amdgcn_sad_u8(srcw, w, 0);
         };
         // Small diamond (enc.rs:1207-1272): the first of the cheapest in-bounds points below the
         // best moves, until none does or the iterations run out.
         for (int it = 0; it < J.me_iters; it++) {
             const int px[4] = {bestx, bestx - 1, bestx + 1, bestx};
             const int py[4] = {besty - 1, besty, besty, besty + 1};
+            int ok[4];
+            uint32_t part[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                ok[k] = px[k] >= imn0 && px[k] <= imx0 && py[k] >= imn1 && py[k] <= imx1;
+                part[k] = ok[k] ? sad_part(px[k], py[k]) : 0u;
+            }
+            uint32_t sums[4];
+            wave_sum4(part, sums);
             int moved = -1;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                if (px[k] < imn0 || px[k] > imx0 || py[k] < imn1 || py[k] > imx1) continue;
-                const uint32_t cst = sad_at(px[k], py[k]) + mv_cost(px[k] * 4, py[k] * 4);
+                if (!ok[k]) continue;
+                const uint32_t cst = sums[k] + mv_cost(px[k] * 4, py[k] * 4);
                 D.n_sad++;
                 if (cst < best_cost) {
                     best_cost = cst;
@@ -793,7 +869,7 @@ __device__ void analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S) {
         if (fbx - 3 < rox || fbx + 29 > rox + SR_COLS || fby - 3 < roy || fby + 20 > roy + SR_H) {
             rox = (fbx - 3) & ~3;
             roy = fby - 3;
-            stage_region(J, rox, roy, 23, S);
+            stage_region<23>(J, rox, roy, S);
         }
         if (best_cost - mv_cost(bestx * 4, besty * 4) <= J.sb_luma) {
             predw = lds4u(&S.sr[(fby - roy + rr) * SR_W], fbx - rox + c0);
@@ -809,12 +885,21 @@ __device__ void analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S) {
                 for (int it = 0; it < iters; it++) {
                     const int fx = bmx, fy = bmy;
                     const int dx[4] = {0, -step, step, 0}, dy[4] = {-step, 0, 0, step};
+                    int ok[4];
+                    uint32_t part[4];
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
                         const int mx = fx + dx[k], my = fy + dy[k];
-                        if (iabs(mx - cx) > 3 || iabs(my - cy) > 3 || !in_bounds(mx, my)) continue;
-                        const uint32_t w = window_pred4(x0, y0, fbx, fby, rox, roy, mx, my, S);
-                        const uint32_t cst = wave_sum(__builtin_amdgcn_sad_u8(srcw, w, 0)) + mv_cost(mx, my);
+                        ok[k] = iabs(mx - cx) <= 3 && iabs(my - cy) <= 3 && in_bounds(mx, my);
+                        part[k] = ok[k] ? __builtin_amdgcn_sad_u8(srcw, window_pred4(x0, y0, fbx, fby, rox, roy, mx, my, S), 0) : 0u;
+                    }
+                    uint32_t sums[4];
+                    wave_sum4(part, sums);
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        if (!ok[k]) continue;
+                        const int mx = fx + dx[k], my = fy + dy[k];
+                        const uint32_t cst = sums[k] + mv_cost(mx, my);
                         D.n_subpel++;
                         if (cst < me_cost) {
                             me_cost = cst;

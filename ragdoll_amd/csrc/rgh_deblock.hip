@@ -10,16 +10,16 @@
 // into its left and top neighbours and reading 4; every macroblock whose filtering it must see
 // ((x-1, y), (x, y-1), (x+1, y-1)) lies on an earlier step t = x + 2y, every macroblock that
 // must see its result lies on a later one, and macroblocks of one step are 32 or more samples
-// apart. Each macroblock works on a 20x20 luma (10x10 chroma) copy in LDS and writes it back.
+// apart. Each macroblock works on a 20x20 luma (10x12 chroma) copy in LDS and writes it back.
 #include "rgh_device.h"
 #include "rgh_tables.h"
 
 namespace {
 
 struct DbLds {
-    uint8_t y[20 * 20];
-    uint8_t u[10 * 10];
-    uint8_t v[10 * 10];
+    uint8_t y[20 * 20];   // rows y0-4 .., columns x0-4 ..
+    uint8_t u[10 * 12];   // rows cy0-2 .., columns cx0-4 ..
+    uint8_t v[10 * 12];
 };
 
 struct Edge {
@@ -108,11 +108,28 @@ __device__ void deblock_mb(const RghJob &J, int mb_x, int mb_y, DbLds &S) {
     if (quiet(Q) && (mb_x == 0 || (quiet(L) && mveq(L, Q))) && (mb_y == 0 || (quiet(T) && mveq(T, Q)))) return;
 
     const int x0 = mb_x * 16, y0 = mb_y * 16, cx0 = mb_x * 8, cy0 = mb_y * 8;
-    for (int i = lane; i < 400; i += 64) S.y[i] = J.cur_y[(long)(y0 - 4 + i / 20) * J.sy + x0 - 4 + i % 20];
-    for (int i = lane; i < 100; i += 64) {
-        long o = (long)(cy0 - 2 + i / 10) * J.sc + cx0 - 2 + i % 10;
-        S.u[i] = J.cur_u[o];
-        S.v[i] = J.cur_v[o];
+    // Stage luma rows y0-4 .. y0+15, columns x0-4 .. x0+15 (5 aligned dwords a row) and chroma
+    // rows cy0-2 .. cy0+7, columns cx0-4 .. cx0+7 (3 aligned dwords a row) of both planes; all
+    // loads are issued before the LDS stores.
+    uint32_t ly[2], lc;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int i = lane + 64 * k;
+        ly[k] = i < 100 ? *(const uint32_t *)&J.cur_y[(long)(y0 - 4 + i / 5) * J.sy + x0 - 4 + (i % 5) * 4] : 0u;
+    }
+    {
+        const int p = lane / 30, i = lane % 30;
+        const uint8_t *pl = p ? J.cur_v : J.cur_u;
+        lc = lane < 60 ? *(const uint32_t *)&pl[(long)(cy0 - 2 + i / 3) * J.sc + cx0 - 4 + (i % 3) * 4] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int i = lane + 64 * k;
+        if (i < 100) *(uint32_t *)&S.y[(i / 5) * 20 + (i % 5) * 4] = ly[k];
+    }
+    if (lane < 60) {
+        const int p = lane / 30, i = lane % 30;
+        *(uint32_t *)&(p ? S.v : S.u)[(i / 3) * 12 + (i % 3) * 4] = lc;
     }
     __syncthreads();
 
@@ -168,26 +185,26 @@ __device__ void deblock_mb(const RghJob &J, int mb_x, int mb_y, DbLds &S) {
                     int k = lane & 7;
                     uint8_t *pl = lane < 24 ? S.u : S.v;
                     if (dir == 0)
-                        chroma_line(pl, (2 + k) * 10 + 2 + e * 2, 1, EC, k);
+                        chroma_line(pl, (2 + k) * 12 + 4 + e * 2, 1, EC, k);
                     else
-                        chroma_line(pl, (2 + e * 2) * 10 + 2 + k, 10, EC, k);
+                        chroma_line(pl, (2 + e * 2) * 12 + 4 + k, 12, EC, k);
                 }
             }
             __syncthreads();
         }
     }
-    // Write back all but the untouched top-left corner.
-    for (int i = lane; i < 400; i += 64) {
-        int r = i / 20, c = i % 20;
-        if (r < 4 && c < 4) continue;
-        J.cur_y[(long)(y0 - 4 + r) * J.sy + x0 - 4 + c] = S.y[i];
+    // Write back all but the untouched top-left corner, as aligned dwords.
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int i = lane + 64 * k, r = i / 5, c = i % 5;
+        if (i < 100 && !(r < 4 && c == 0))
+            *(uint32_t *)&J.cur_y[(long)(y0 - 4 + r) * J.sy + x0 - 4 + c * 4] = *(const uint32_t *)&S.y[r * 20 + c * 4];
     }
-    for (int i = lane; i < 100; i += 64) {
-        int r = i / 10, c = i % 10;
-        if (r < 2 && c < 2) continue;
-        long o = (long)(cy0 - 2 + r) * J.sc + cx0 - 2 + c;
-        J.cur_u[o] = S.u[i];
-        J.cur_v[o] = S.v[i];
+    if (lane < 60) {
+        const int p = lane / 30, i = lane % 30, r = i / 3, c = i % 3;
+        uint8_t *pl = p ? J.cur_v : J.cur_u;
+        if (!(r < 2 && c == 0))
+            *(uint32_t *)&pl[(long)(cy0 - 2 + r) * J.sc + cx0 - 4 + c * 4] = *(const uint32_t *)&(p ? S.v : S.u)[r * 12 + c * 4];
     }
 }
 
