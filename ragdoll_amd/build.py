@@ -6,9 +6,11 @@
   Makefile.
 """
 import os
+import re
 import shutil
 import subprocess
 import sys
+import tempfile
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -16,6 +18,14 @@ CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libragdoll_hip.so")
 SOURCES = ["rgh_analyse.hip", "rgh_deblock.hip", "rgh_api.hip"]
 ARCH = os.environ.get("RGH_OFFLOAD_ARCH", "gfx950")
+# Device instructions no kernel may contain. v_ashr_pk_{u8,i8}_i32: the backend assumes they
+# zero bits 31:16 of their destination, which MI355X leaves unchanged. Packed FP32 arithmetic:
+# kept out of all code (the compile flags below disable it; this catches a regression).
+FORBIDDEN = re.compile(r"\b(v_ashr_pk_[ui]8_i32|v_pk_add_f32|v_pk_mul_f32|v_pk_fma_f32)\b")
+FLAGS = ["--offload-arch=" + ARCH, "-O3", "-std=c++17",
+         # packed FP32 instructions stay off: no code here wants them.
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Wall", "-Wno-unused-function", "-Wno-unused-result"]
 
 
 def hipcc():
@@ -25,16 +35,29 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+def check_device_code(verbose=False):
+    """Compiles every source's device code to assembly and rejects forbidden instructions."""
+    with tempfile.TemporaryDirectory() as tmp:
+        for src in SOURCES:
+            out = os.path.join(tmp, src + ".s")
+            cmd = [hipcc()] + FLAGS + ["--cuda-device-only", "-S", "-I", os.path.join(ROOT, "include"),
+                                       "-o", out, os.path.join(CSRC, src)]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True, stderr=subprocess.DEVNULL)
+            with open(out) as f:
+                bad = sorted({m.group(1) for m in FORBIDDEN.finditer(f.read())})
+            if bad:
+                raise RuntimeError("%s: device code contains %s" % (src, ", ".join(bad)))
+
+
 def build_library(verbose=False):
-    cmd = [hipcc(), "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-shared",
-           # packed FP32 instructions stay off: no code here wants them.
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-Wall", "-Wno-unused-function", "-Wno-unused-result",
-           "-I", os.path.join(ROOT, "include"),
-           "-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES]
+    cmd = [hipcc()] + FLAGS + ["-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
+                               "-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
+    check_device_code(verbose)
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

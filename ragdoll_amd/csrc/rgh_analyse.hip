@@ -263,7 +263,7 @@ __device__ void compute_window(MbLds &S, int wx0, int wy0) {
             if (k >= 2 && k < 2 + WP_ROWS) {
                 uint32_t o = 0;
 #pragma unroll
-                for (int i = 0; i < 4; i++) o |= (uint32_t)clip255((h[i] + 16) >> 5) << (8 * i);
+                for (int i = 0; i < 4; i++) o |= sat_shr<5>(h[i] + 16) << (8 * i);
                 *(uint32_t *)&S.ph[(k - 2) * WP_COLS + 4 * g] = o;
             }
         } else if (item < NA + NB) {
@@ -276,7 +276,7 @@ __device__ void compute_window(MbLds &S, int wx0, int wy0) {
             for (int i = 0; i < 4; i++) {
                 const int v = tap6(byte_of(w[0], i), byte_of(w[1], i), byte_of(w[2], i), byte_of(w[3], i), byte_of(w[4], i),
                                    byte_of(w[5], i));
-                o |= (uint32_t)clip255((v + 16) >> 5) << (8 * i);
+                o |= sat_shr<5>(v + 16) << (8 * i);
             }
             *(uint32_t *)&S.pv[r * WP_COLS + 4 * g] = o;
         }
@@ -301,7 +301,7 @@ __device__ void compute_window(MbLds &S, int wx0, int wy0) {
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int j = tap6(t[0][i], t[1][i], t[2][i], t[3][i], t[4][i], t[5][i]);
-                o |= (uint32_t)clip255((j + 512) >> 10) << (8 * i);
+                o |= sat_shr<10>(j + 512) << (8 * i);
             }
             *(uint32_t *)&S.pc[r * WP_COLS + 4 * g] = o;
         }
@@ -999,7 +999,7 @@ __device__ void analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S) {
                 pcc = (5 * Vv + 32) >> 6;
                 uint32_t w = 0;
 #pragma unroll
-                for (int i = 0; i < 4; i++) w |= (uint32_t)clip255((pa + pb * (c0 + i - 7) + pcc * (rr - 7) + 16) >> 5) << (8 * i);
+                for (int i = 0; i < 4; i++) w |= sat_shr<5>(pa + pb * (c0 + i - 7) + pcc * (rr - 7) + 16) << (8 * i);
                 *(uint32_t *)&S.tmp[lane * 4] = w;
                 __syncthreads();
                 const uint32_t ps = satd16(S.src, S.tmp);
@@ -1017,7 +1017,7 @@ __device__ void analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S) {
                     for (int i = 0; i < 4; i++) {
                         int v = im == 0 ? (has_t ? S.etop[c0 + i] : 128)
                                         : (im == 1 ? (has_l ? S.eleft[rr] : 128)
-                                                   : (im == 2 ? dc : clip255((pa + pb * (c0 + i - 7) + pcc * (rr - 7) + 16) >> 5)));
+                                                   : (im == 2 ? dc : (int)sat_shr<5>(pa + pb * (c0 + i - 7) + pcc * (rr - 7) + 16)));
                         w |= (uint32_t)v << (8 * i);
                     }
                     *(uint32_t *)&S.pred[lane * 4] = w;
@@ -1072,7 +1072,7 @@ __device__ void analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S) {
                                 Vv += (i + 1) * (l[4 + i] - pl);
                             }
                             const int a = 16 * (t[7] + l[7]), b = (17 * Hh + 16) >> 5, c = (17 * Vv + 16) >> 5;
-                            v = clip255((a + b * (cc - 3) + c * (cr - 3) + 16) >> 5);
+                            v = (int)sat_shr<5>(a + b * (cc - 3) + c * (cr - 3) + 16);
                         }
                         smp[p] = (uint32_t)v;
                     }

@@ -64,6 +64,17 @@ __device__ __forceinline__ uint32_t ld4u(const uint8_t *p) {
 }
 
 __device__ __forceinline__ int clip255(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
+
+// clip255(v >> S) (arithmetic shift), computed as a clamp before a logical shift. Written as
+// shift-then-clip, two such results packed into one dword are selected by the gfx950 backend as
+// v_ashr_pk_u8_i32, after which it assumes bits 31:16 of that register are zero; on MI355X they
+// keep the register's previous contents and corrupt the bytes packed above. This form never
+// matches that instruction (the build rejects any code object that contains it).
+template <int S>
+__device__ __forceinline__ uint32_t sat_shr(int v) {
+    constexpr int HI = (256 << S) - 1;
+    return (uint32_t)(v < 0 ? 0 : (v > HI ? HI : v)) >> S;
+}
 __device__ __forceinline__ int iclamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 __device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
 __device__ __forceinline__ uint32_t byte_of(uint32_t w, int i) { return (w >> (8 * i)) & 0xFF; }
