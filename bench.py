@@ -70,6 +70,34 @@ class Hip:
         self.check(self.L.hipDeviceSynchronize(), "hipDeviceSynchronize")
 
 
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` on this workload, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (profiles/traffic.json, copy-calibrated), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            k = json.load(f)["kernels"][kernel]
+        return round(k["fetch_bytes"] + k["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def calibrate_copy(mib, repeats=4):
+    """A plain device-to-device copy of known size: reading FETCH_SIZE / WRITE_SIZE for it gives
+    the factor between those counters and the bytes a kernel moves."""
+    hip = Hip()
+    n = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    hip.check(hip.L.hipMalloc(ctypes.byref(a), n), "hipMalloc")
+    hip.check(hip.L.hipMalloc(ctypes.byref(b), n), "hipMalloc")
+    hip.check(hip.L.hipMemset(a, 1, ctypes.c_size_t(n)), "hipMemset")
+    for _ in range(repeats):
+        hip.check(hip.L.hipMemcpy(b, a, ctypes.c_size_t(n), 3), "hipMemcpy")  # device to device
+    hip.synchronize()
+    hip.free(a.value)
+    hip.free(b.value)
+    print(json.dumps({"copy_bytes": n, "repeats": repeats}), flush=True)
+
+
 def render_frames(n_frames):
     from ragdoll_amd.synth import CONFIGS, SceneGenerator, config_seed
     kw = dict(CONFIGS["1080p"])
@@ -124,7 +152,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run device-to-device copies of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (independent GOP streams would "
               "shard across GPUs with no exchange)", file=sys.stderr)
@@ -204,7 +237,7 @@ def main():
                    "form": "plain", "streams": len(streams), "mbs_per_frame": mbs_per_frame,
                    "me_range": 16, "analyse_ms": round(analyse_ms, 3), "deblock_ms": round(deblock_ms, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("rgh_k_analyse_step"),
                      "kernel": "rgh_k_analyse_step", "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "mbs_per_launch": round(mbs_per_launch, 2), "bytes_per_mb": MB_BYTES},
     }
