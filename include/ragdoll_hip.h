@@ -61,6 +61,8 @@ typedef struct rgh_config {
     int32_t qpel_iters; /* quarter-pel iterations; 0 selects the reference's 2 */
     int32_t device;     /* HIP device ordinal */
     int32_t flags;      /* RGH_FLAG_* */
+    int32_t ring;       /* reconstructed pictures kept (2..64; 0 selects 2): how many frames of
+                           one stream rgh_analyse_frames can have in flight, plus one */
 } rgh_config;
 
 /* rgh_config.flags: skip the device copy of each frame's reconstruction before deblocking
@@ -159,7 +161,23 @@ rgh_status rgh_analyse_p_frame(void *ctx, const rgh_i420 *src, rgh_mb_decision *
  * thread (crates/ragdoll-pipeline/src/lib.rs:543-603); this is that wave of GOPs on one GPU. */
 rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src_device);
 
-/* Copies the last analysed frame's results to host memory (any pointer may be NULL).
+/* Pipelined form: n_frames consecutive P frames of each of n contexts (same device), sources
+ * and outputs in device memory, analysed by one schedule of kernel launches. Frame k + 1 of a
+ * stream starts `lag` wavefront steps after frame k (0 selects 20) and each of its macroblocks
+ * proceeds as soon as the reference samples it can read are final, so up to ring - 1 frames
+ * of a stream are in flight at once; the results are identical to analysing the frames one by
+ * one. src_device[c * n_frames + k] is frame k of context c; decisions[c * n_frames + k]
+ * receives its mb_w*mb_h records and levels[c * n_frames + k] its mb_w*mb_h*RGH_LEVELS_PER_MB
+ * levels (device pointers, kept until the call returns). The last frame's reconstruction becomes
+ * each context's reference. Returns after the work completes. No counterpart in the reference,
+ * which encodes a GOP's frames one after another on one thread
+ * (crates/ragdoll-pipeline/src/lib.rs:575-590); x264's frame threads are the analogue. */
+rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src_device,
+                              rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag);
+
+/* Copies the last analysed frame's results to host memory (any pointer may be NULL). After
+ * rgh_analyse_frames the levels are read from that call's last levels buffer, which must still
+ * be allocated.
  * which_recon: 0 = before deblocking, 1 = deblocked reference. */
 rgh_status rgh_get_outputs(void *ctx, rgh_mb_decision *decisions, int16_t *levels,
                            const rgh_i420_out *recon, int32_t which_recon);
@@ -173,17 +191,21 @@ rgh_status rgh_synchronize(void *ctx);
 /* HIP stream (hipStream_t) the context's work runs on, for callers that time it. */
 void *rgh_get_stream(void *ctx);
 
-/* Device-time measurement of the P-frame launches (HIP events on the context's stream).
- * When enabled, each rgh_analyse_p_frame / rgh_analyse_batch records events around its
- * analysis wavefront and its deblock + border-extension launches; rgh_get_timing waits for the
- * last such frame and returns both spans in milliseconds and the number of analysis kernel
- * launches they covered. No counterpart in the reference (its telemetry is wall-clock
- * PipelineTelemetry, crates/ragdoll-pipeline/src/lib.rs:62-76). */
+/* Device-time measurement (HIP events on the context's stream). When enabled on the first
+ * context of a call, each analysis call records events around its schedule of frame-step
+ * launches (analysis, deblocking and border extension fused); rgh_get_timing waits for the last
+ * such call and returns its span in milliseconds, 0 for deblock_ms (no separate deblock pass),
+ * and the number of frame-step launches. No counterpart in the reference (its telemetry is
+ * wall-clock PipelineTelemetry, crates/ragdoll-pipeline/src/lib.rs:62-76). */
 rgh_status rgh_enable_timing(void *ctx, int32_t enable);
-rgh_status rgh_get_timing(void *ctx, float *analyse_ms, float *deblock_ms, int32_t *analyse_launches);
+rgh_status rgh_get_timing(void *ctx, float *analyse_ms, float *deblock_ms, int32_t *launches);
 
 /* A human-readable description of a status code. */
 const char *rgh_status_str(int32_t status);
+
+/* The calling thread's last failure in detail (the failing runtime call and its message), or
+ * "" — the text RagdollError::Internal(String) would carry (crates/ragdoll-core/src/error.rs). */
+const char *rgh_last_error(void);
 
 #ifdef __cplusplus
 }

@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 from .abi import (DECISION_DTYPE, LEVELS_PER_MB, PROFILES, RghError, StreamAnalyzer, load_library,
-                  analyse_batch)
+                  analyse_batch, analyse_frames)
 
 __all__ = ["DECISION_DTYPE", "LEVELS_PER_MB", "PROFILES", "RghError", "StreamAnalyzer",
-           "load_library", "analyse_batch"]
+           "load_library", "analyse_batch", "analyse_frames"]

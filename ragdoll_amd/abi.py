@@ -19,14 +19,14 @@ DECISION_DTYPE = np.dtype([
 assert DECISION_DTYPE.itemsize == 32
 
 EXPORTS = ["rgh_create", "rgh_destroy", "rgh_set_reference", "rgh_analyse_p_frame",
-           "rgh_analyse_batch", "rgh_get_outputs", "rgh_get_mb_dims", "rgh_synchronize",
-           "rgh_get_stream", "rgh_enable_timing", "rgh_get_timing", "rgh_status_str"]
+           "rgh_analyse_batch", "rgh_analyse_frames", "rgh_get_outputs", "rgh_get_mb_dims", "rgh_synchronize",
+           "rgh_get_stream", "rgh_enable_timing", "rgh_get_timing", "rgh_status_str", "rgh_last_error"]
 
 
 class RgConfig(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("profile", ctypes.c_int32),
                 ("qp", ctypes.c_int32), ("me_range", ctypes.c_int32), ("qpel_iters", ctypes.c_int32),
-                ("device", ctypes.c_int32), ("flags", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("flags", ctypes.c_int32), ("ring", ctypes.c_int32)]
 
 
 class RgI420(ctypes.Structure):
@@ -43,7 +43,9 @@ class RgI420Out(ctypes.Structure):
 class RghError(RuntimeError):
     def __init__(self, status, what):
         self.status = status
-        super().__init__("%s: %s (status %d)" % (what, _lib.rgh_status_str(status).decode(), status))
+        detail = _lib.rgh_last_error().decode()
+        super().__init__("%s: %s (status %d)%s" % (what, _lib.rgh_status_str(status).decode(), status,
+                                                    " — " + detail if detail else ""))
 
 
 _lib = None
@@ -68,6 +70,9 @@ def load_library(path=LIB_PATH):
     L.rgh_analyse_p_frame.argtypes = [P, ctypes.POINTER(RgI420), P, P, ctypes.POINTER(RgI420Out)]
     L.rgh_analyse_batch.restype = I32
     L.rgh_analyse_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), I32, ctypes.POINTER(RgI420)]
+    L.rgh_analyse_frames.restype = I32
+    L.rgh_analyse_frames.argtypes = [ctypes.POINTER(ctypes.c_void_p), I32, I32, ctypes.POINTER(RgI420),
+                                     ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), I32]
     L.rgh_get_outputs.restype = I32
     L.rgh_get_outputs.argtypes = [P, P, P, ctypes.POINTER(RgI420Out), I32]
     L.rgh_get_mb_dims.restype = I32
@@ -83,6 +88,8 @@ def load_library(path=LIB_PATH):
                                  ctypes.POINTER(I32)]
     L.rgh_status_str.restype = ctypes.c_char_p
     L.rgh_status_str.argtypes = [I32]
+    L.rgh_last_error.restype = ctypes.c_char_p
+    L.rgh_last_error.argtypes = []
     _lib = L
     return L
 
@@ -104,10 +111,10 @@ class StreamAnalyzer:
     """One stream's P-frame analysis context (mirror of an H264Encoder's P path)."""
 
     def __init__(self, width, height, qp, profile="baseline", me_range=0, qpel_iters=0, device=0,
-                 keep_undeblocked=True):
+                 keep_undeblocked=True, ring=0):
         self.L = load_library()
         cfg = RgConfig(width, height, PROFILES[profile], qp, me_range, qpel_iters, device,
-                       0 if keep_undeblocked else FLAG_NO_UNDEBLOCKED_COPY)
+                       0 if keep_undeblocked else FLAG_NO_UNDEBLOCKED_COPY, ring)
         h = ctypes.c_void_p()
         _check(self.L.rgh_create(ctypes.byref(cfg), ctypes.byref(h)), "rgh_create")
         self.h = h
@@ -163,7 +170,7 @@ class StreamAnalyzer:
         _check(self.L.rgh_enable_timing(self.h, 1 if on else 0), "rgh_enable_timing")
 
     def timing(self):
-        """(analysis ms, deblock+extend ms, analysis launches) of the last timed frame."""
+        """(schedule ms, 0.0, frame-step launches) of the last timed call."""
         a, d, n = ctypes.c_float(), ctypes.c_float(), ctypes.c_int32()
         _check(self.L.rgh_get_timing(self.h, ctypes.byref(a), ctypes.byref(d), ctypes.byref(n)), "rgh_get_timing")
         return a.value, d.value, n.value
@@ -187,3 +194,26 @@ def analyse_batch(analyzers, device_frames):
     for i, (a, f) in enumerate(zip(analyzers, device_frames)):
         srcs[i] = RgI420(f[0], f[1], f[2], f[3], f[4], f[5], a.width, a.height)
     _check(L.rgh_analyse_batch(arr, n, srcs), "rgh_analyse_batch")
+
+
+def analyse_frames(analyzers, device_frames, decisions, levels, lag=0):
+    """n_frames consecutive P frames of every analyzer in one pipelined schedule.
+
+    device_frames[i][k]: (y_ptr, u_ptr, v_ptr, stride_y, stride_u, stride_v) of frame k of
+    analyzer i in device memory; decisions[i][k] / levels[i][k]: device addresses receiving its
+    mb_w*mb_h decision records / levels."""
+    L = load_library()
+    n = len(analyzers)
+    nf = len(device_frames[0])
+    arr = (ctypes.c_void_p * n)(*[a.h.value for a in analyzers])
+    srcs = (RgI420 * (n * nf))()
+    dp = (ctypes.c_void_p * (n * nf))()
+    lp = (ctypes.c_void_p * (n * nf))()
+    for i, a in enumerate(analyzers):
+        if len(device_frames[i]) != nf or len(decisions[i]) != nf or len(levels[i]) != nf:
+            raise ValueError("every analyzer needs the same number of frames and outputs")
+        for k, f in enumerate(device_frames[i]):
+            srcs[i * nf + k] = RgI420(f[0], f[1], f[2], f[3], f[4], f[5], a.width, a.height)
+            dp[i * nf + k] = decisions[i][k]
+            lp[i * nf + k] = levels[i][k]
+    _check(L.rgh_analyse_frames(arr, n, nf, srcs, dp, lp, lag), "rgh_analyse_frames")

@@ -12,6 +12,7 @@
 // the quarter-pel candidates; the chroma prediction. Reductions are DPP + readlane; transform
 // and quantization run one 4x4 block per lane in registers, 8x8 blocks one row or column per
 // lane through LDS.
+#include "rgh_deblock.h"
 #include "rgh_device.h"
 #include "rgh_tables.h"
 
@@ -613,7 +614,10 @@ __device__ void recon8(MbLds &S, int cbpl, int qp) {
 // ---------------------------------------------------------------------------------------
 // One macroblock (enc.rs:3011 code_p_mb and everything it calls).
 // ---------------------------------------------------------------------------------------
-__device__ void analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S) {
+//
+// Returns false, having written nothing to global memory, when a reference sample the
+// macroblock may read is not final yet for a reader in launch `seq`.
+__device__ bool analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S, uint32_t seq) {
     const int lane = threadIdx.x;
     const int rr = lane >> 2, c0 = (lane & 3) * 4;
     const int mb_w = J.mb_w, idx = mb_y * mb_w + mb_x;
@@ -708,6 +712,37 @@ __device__ void analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S) {
     const int bmax0 = 4 * (W - 16 - x0 + RGH_MV_MARGIN), bmax1 = min(4 * (H - 16 - y0 + RGH_MV_MARGIN), 2044);
     auto in_bounds = [&](int mx, int my) { return mx >= bmin0 && mx <= bmax0 && my >= bmin1 && my <= bmax1; };
     auto mv_cost = [&](int mx, int my) { return lambda * (se_bits(mx - mvpx) + se_bits(my - mvpy)); };
+
+    // ---- Readiness of the reference (when it may still be in flight). ----
+    // Every reference read stays within [c - 22 - r, c + 68 + r] x [c - 19 - r, c + 36 + r] of
+    // some clamped integer candidate c (the staged 68x55 search region, the skip-probe window and
+    // the chroma taps around a vector at most 16 + r samples from c), r = 0 for at most 16
+    // diamond iterations, else the iteration count. The zero candidate keeps this macroblock's
+    // co-located decision inside. Macroblock (a, b) of the reference is final, padding
+    // included, once its item (a + 1, b + 1) is stamped.
+    if (J.ref_fin) {
+        const int imn0 = (bmin0 + 3) >> 2, imn1 = (bmin1 + 3) >> 2, imx0 = bmax0 >> 2, imx1 = bmax1 >> 2;
+        const int cvx[5] = {mvpx, 0, ax, bx_, colx}, cvy[5] = {mvpy, 0, ay, by_, coly};
+        int lx = 1 << 30, hx = -(1 << 30), ly = 1 << 30, hy = -(1 << 30);
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const int fx = iclamp((cvx[k] + 2) >> 2, imn0, imx0), fy = iclamp((cvy[k] + 2) >> 2, imn1, imx1);
+            lx = min(lx, fx);
+            hx = max(hx, fx);
+            ly = min(ly, fy);
+            hy = max(hy, fy);
+        }
+        const int r = J.me_iters > 16 ? J.me_iters : 0;
+        const int ax0 = iclamp((x0 + lx - 22 - r) >> 4, 0, mb_w - 1), ax1 = iclamp((x0 + hx + 68 + r) >> 4, 0, mb_w - 1);
+        const int ay0 = iclamp((y0 + ly - 19 - r) >> 4, 0, J.mb_h - 1), ay1 = iclamp((y0 + hy + 36 + r) >> 4, 0, J.mb_h - 1);
+        const int nx = ax1 - ax0 + 1, n = nx * (ay1 - ay0 + 1);
+        int wait = 0;
+        for (int i = lane; i < n; i += 64) {
+            const int a = ax0 + i % nx, b = ay0 + i / nx;
+            wait |= !stamp_final(J.ref_fin[(b + 1) * (mb_w + 1) + a + 1], J.ref_serial, seq);
+        }
+        if (__ballot(wait)) return false;
+    }
 
     rgh_mb_decision D;
     memset(&D, 0, sizeof(D));
@@ -1325,6 +1360,11 @@ __device__ void analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S) {
     *(uint32_t *)&J.cur_y[(long)(y0 + rr) * J.sy + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
     J.cur_u[(long)(cy0 + (lane >> 3)) * J.sc + cx0 + (lane & 7)] = S.predu[lane];
     J.cur_v[(long)(cy0 + (lane >> 3)) * J.sc + cx0 + (lane & 7)] = S.predv[lane];
+    if (J.pre_y) {
+        *(uint32_t *)&J.pre_y[(long)(y0 + rr) * W + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
+        J.pre_u[(long)(cy0 + (lane >> 3)) * (W / 2) + cx0 + (lane & 7)] = S.predu[lane];
+        J.pre_v[(long)(cy0 + (lane >> 3)) * (W / 2) + cx0 + (lane & 7)] = S.predv[lane];
+    }
 
     int16_t *lv = J.levels + (long)idx * RGH_LEVELS_PER_MB;
     uint32_t nnz = 0;
@@ -1365,15 +1405,72 @@ __device__ void analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S) {
         D.nnz = (uint16_t)nnz;
         J.dec[idx] = D;
     }
+    return true;
 }
 
-extern "C" __global__ __launch_bounds__(64) void rgh_k_analyse_step(const RghJob *__restrict__ jobs, int t) {
-    const RghJob J = jobs[blockIdx.y];
-    if (t >= J.steps) return;
-    const int a = t - J.mb_w + 1;
-    const int ylo = a > 0 ? (a + 1) >> 1 : 0, yhi = min(J.mb_h - 1, t >> 1);
-    const int y = ylo + (int)blockIdx.x;
-    if (y > yhi) return;
-    __shared__ MbLds S;
-    analyse_mb(J, t - 2 * y, y, S);
+union StepLds {
+    MbLds a;
+    DbLds d;
+};
+
+// One launch of a frame schedule. Block row blockIdx.y is frame job j0 + blockIdx.y; the frame's
+// current wavefront step is s = launch - base - delay. Blocks [0, na) analyse the macroblocks of
+// step s; blocks [na, na + nd) run the deblock/pad items (x, y), 0 <= x <= mb_w, 0 <= y <= mb_h,
+// of step s - 4 (item (x, y) deblocks macroblock (x, y) when it exists and pads macroblock
+// (x - 1, y - 1)). The lag of 4 steps lets every analysis that reads undeblocked samples of
+// (x, y) or of the samples its edges modify ((x + 1, y), (x - 1, y + 1), (x, y + 1),
+// (x + 1, y + 1)) finish first.
+//
+// A macroblock whose reference samples are not final, or the first step of a frame whose picture
+// slot is still read by a later frame, defers: it raises the frame's delay for the next launch
+// and the whole frame repeats the step then (items already done skip themselves by their
+// stamps). Nothing waits inside a launch.
+extern "C" __global__ __launch_bounds__(64) void rgh_k_frame_step(const RghJob *__restrict__ jobs, int j0, int launch,
+                                                                   uint32_t seq) {
+    const RghJob J = jobs[j0 + (int)blockIdx.y];
+    RghFrameCtl *ctl = J.ctl;
+    const int lane = threadIdx.x;
+    const int delay = ctl->delay[launch & 1];
+    const int s = launch - J.base - delay;
+    int *next_delay = &ctl->delay[(launch + 1) & 1];
+    if (blockIdx.x == 0 && lane == 0 && delay > 0) atomicMax(next_delay, delay);
+    const int na = min(J.mb_h, (J.mb_w + 1) >> 1);
+    const int b = (int)blockIdx.x;
+    __shared__ StepLds S;
+    if (b < na) {
+        if (s < 0 || s >= J.steps) return;
+        const int a = s - J.mb_w + 1;
+        const int y = (a > 0 ? (a + 1) >> 1 : 0) + b;
+        if (y > min(J.mb_h - 1, s >> 1)) return;
+        const int x = s - 2 * y, m = y * J.mb_w + x;
+        if ((uint32_t)(J.ana[m] >> 32) == J.serial) return;  // analysed in an earlier launch
+        bool ok = true;
+        if (s == 0 && J.wait_analysed) ok = __atomic_load_n(J.wait_analysed, __ATOMIC_RELAXED) >= J.wait_total;
+        if (ok) ok = analyse_mb(J, x, y, S.a, seq);
+        if (lane == 0) {
+            if (ok) {
+                J.ana[m] = make_stamp(J.serial, seq);
+                atomicAdd(&ctl->analysed, 1);
+            } else {
+                atomicMax(next_delay, delay + 1);
+            }
+        }
+    } else {
+        const int sd = s - 4;
+        if (sd < 0 || sd > J.steps + 2) return;
+        const int a = sd - J.mb_w;
+        const int y = (a > 0 ? (a + 1) >> 1 : 0) + (b - na);
+        if (y > min(J.mb_h, sd >> 1)) return;
+        const int x = sd - 2 * y, m = y * (J.mb_w + 1) + x;
+        if ((uint32_t)(J.fin[m] >> 32) == J.serial) return;  // done in an earlier launch
+        if (x < J.mb_w && y < J.mb_h && J.deblock) deblock_mb(J, x, y, S.d);
+        if (x > 0 && y > 0) pad_mb(J, x - 1, y - 1);
+        if (lane == 0) J.fin[m] = make_stamp(J.serial, seq);
+    }
+}
+
+// Fills n 64-bit words (stamps) with v.
+extern "C" __global__ __launch_bounds__(256) void rgh_k_fill_u64(uint64_t *p, int n, uint64_t v) {
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) p[i] = v;
 }

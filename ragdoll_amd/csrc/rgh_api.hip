@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -10,9 +12,9 @@
 
 #include "rgh_device.h"
 
-extern "C" __global__ void rgh_k_analyse_step(const RghJob *jobs, int t);
-extern "C" __global__ void rgh_k_deblock_step(const RghJob *jobs, int t);
+extern "C" __global__ void rgh_k_frame_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_extend(const RghJob *jobs);
+extern "C" __global__ void rgh_k_fill_u64(uint64_t *p, int n, uint64_t v);
 
 namespace {
 
@@ -47,48 +49,76 @@ void zero_bounds(int qp, uint32_t *sad, uint32_t *ssd, uint32_t *probe) {
 struct Picture {
     uint8_t *base[3] = {nullptr, nullptr, nullptr};  // allocation
     uint8_t *org[3] = {nullptr, nullptr, nullptr};   // sample (0, 0)
+    uint64_t *ana = nullptr;  // analysis stamps of the frame written here, mb_w x mb_h
+    uint64_t *fin = nullptr;  // deblock/pad item stamps, (mb_w + 1) x (mb_h + 1)
+    uint32_t serial = 0;      // frame held (0: none)
 };
+
+constexpr int kDefaultLag = 20;  // wavefront steps between consecutive frames of a stream
+constexpr int kDeblockLag = 4;   // deblock items run this many steps behind the analysis
+constexpr int kMaxDelay = 64;    // launches a frame stays in the planned range past its end
 
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int width = 0, height = 0, mb_w = 0, mb_h = 0, profile = 0, qp = 0, me_iters = 16, qpel_iters = 2;
     int sy = 0, sc = 0, rows_y = 0, rows_c = 0;
-    Picture pic[2];
-    int ref = 0;  // index of the reference picture; the other is written
-    rgh_mb_decision *dec[2] = {nullptr, nullptr};
-    int dcur = 0;  // decisions written by the next frame
+    std::vector<Picture> pic;  // ring of reconstructed pictures
+    int ref = 0;               // slot of the reference picture (the last frame analysed)
+    uint32_t serial = 0;       // serial number of the reference frame
+    rgh_mb_decision *dec[2] = {nullptr, nullptr};  // context-owned decision buffers
+    int dcur = 0;              // decisions the next single frame writes
+    const rgh_mb_decision *prev = nullptr;  // the reference frame's decisions
     int16_t *levels = nullptr;
+    const int16_t *last_levels = nullptr;  // levels of the last analysed frame
     uint8_t *src[3] = {nullptr, nullptr, nullptr};  // staging for host sources
     uint8_t *pre[3] = {nullptr, nullptr, nullptr};  // undeblocked copy of the last frame
     RghJob *jobs_dev = nullptr;
+    RghFrameCtl *ctl_dev = nullptr;
+    RghFrameCtl *ctl_host = nullptr;  // pinned readback
     int jobs_cap = 0;
     bool has_ref = false;
     bool analysed = false;
     bool keep_pre = true;
     uint32_t sb[5] = {0, 0, 0, 0, 0};
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};  // analysis start, analysis end, frame end
+    hipEvent_t ev[2] = {nullptr, nullptr};  // schedule start, schedule end
     bool timing = false, timed = false;
     int timed_launches = 0;
 };
+
+// Launch numbers stamped into completion stamps; strictly increasing over the process.
+std::atomic<uint32_t> g_seq{1};
+
+// Description of the last failure on this thread (rgh_last_error).
+thread_local char g_last_error[256] = "";
+
+void set_error(const char *what, const char *detail, int line) {
+    std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s (rgh_api.hip:%d)", what, detail, line);
+}
 
 rgh_status hip_status(hipError_t e) {
     if (e == hipSuccess) return RGH_OK;
     if (e == hipErrorOutOfMemory) return RGH_ERR_OOM;
     return RGH_ERR_DEVICE;
 }
-#define HIP_TRY(expr)                                     \
-    do {                                                  \
-        hipError_t e__ = (expr);                          \
-        if (e__ != hipSuccess) return hip_status(e__);    \
+#define HIP_TRY(expr)                                            \
+    do {                                                         \
+        hipError_t e__ = (expr);                                 \
+        if (e__ != hipSuccess) {                                 \
+            set_error(#expr, hipGetErrorString(e__), __LINE__);  \
+            return hip_status(e__);                              \
+        }                                                        \
     } while (0)
 
 void free_ctx(Ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    for (auto &p : c->pic)
+    for (auto &p : c->pic) {
         for (auto &b : p.base)
             if (b) (void)hipFree(b);
+        if (p.ana) (void)hipFree(p.ana);
+        if (p.fin) (void)hipFree(p.fin);
+    }
     for (auto &d : c->dec)
         if (d) (void)hipFree(d);
     if (c->levels) (void)hipFree(c->levels);
@@ -97,24 +127,35 @@ void free_ctx(Ctx *c) {
     for (auto &b : c->pre)
         if (b) (void)hipFree(b);
     if (c->jobs_dev) (void)hipFree(c->jobs_dev);
+    if (c->ctl_dev) (void)hipFree(c->ctl_dev);
+    if (c->ctl_host) (void)hipHostFree(c->ctl_host);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
-RghJob make_job(Ctx *c, const uint8_t *sy, const uint8_t *su, const uint8_t *sv, int ssy, int ssu, int ssv) {
+int frame_steps(const Ctx *c) { return (c->mb_w - 1) + 2 * (c->mb_h - 1) + 1; }
+// Launch offsets a frame needs: its analysis steps, then the deblock/pad items (their grid
+// reaches 3 steps further) kKDeblockLag behind.
+int frame_span(const Ctx *c) { return frame_steps(c) + 3 + kDeblockLag; }
+
+// Frame k of a call on context c: analysed into slot (ref + 1 + k) % ring, with the frame of
+// slot (ref + k) % ring as its reference.
+RghJob make_job(Ctx *c, int k, const rgh_i420 &src, rgh_mb_decision *dec, const rgh_mb_decision *prev,
+                int16_t *levels, int base, RghFrameCtl *ctl, const RghFrameCtl *wait) {
     RghJob j;
     std::memset(&j, 0, sizeof(j));
-    j.src_y = sy;
-    j.src_u = su;
-    j.src_v = sv;
-    j.src_sy = ssy;
-    j.src_su = ssu;
-    j.src_sv = ssv;
+    const int R = (int)c->pic.size();
+    const Picture &r = c->pic[(c->ref + k) % R], &w = c->pic[(c->ref + 1 + k) % R];
+    j.src_y = src.y;
+    j.src_u = src.u;
+    j.src_v = src.v;
+    j.src_sy = src.stride_y;
+    j.src_su = src.stride_u;
+    j.src_sv = src.stride_v;
     j.width = c->width;
     j.height = c->height;
-    const Picture &r = c->pic[c->ref], &w = c->pic[1 - c->ref];
     j.ref_y = r.org[0];
     j.ref_u = r.org[1];
     j.ref_v = r.org[2];
@@ -129,68 +170,132 @@ RghJob make_job(Ctx *c, const uint8_t *sy, const uint8_t *su, const uint8_t *sv,
     j.me_iters = c->me_iters;
     j.qpel_iters = c->qpel_iters;
     j.t8x8 = c->profile == RGH_PROFILE_HIGH;
-    j.dec = c->dec[c->dcur];
-    j.prev = c->dec[1 - c->dcur];
-    j.levels = c->levels;
+    j.dec = dec;
+    j.prev = prev;
+    j.levels = levels;
     j.sb_luma = c->sb[0];
     j.sb_luma_ssd = c->sb[1];
     j.sb_chroma = c->sb[2];
     j.sb_chroma_ssd = c->sb[3];
     j.sb_probe = c->sb[4];
-    j.steps = (c->mb_w - 1) + 2 * (c->mb_h - 1) + 1;
+    j.steps = frame_steps(c);
     j.deblock = c->qp != 0;
+    j.base = base;
+    j.serial = c->serial + 1 + (uint32_t)k;
+    j.ref_serial = c->serial + (uint32_t)k;
+    j.ana = w.ana;
+    j.fin = w.fin;
+    j.ref_fin = r.fin;
+    j.ctl = ctl;
+    if (wait) {
+        j.wait_analysed = &wait->analysed;
+        j.wait_total = c->mb_w * c->mb_h;
+    }
     return j;
 }
 
-// Launches the wavefront of one P frame for every job, on `stream`.
-rgh_status run_jobs(Ctx *const *ctxs, int n, const std::vector<RghJob> &jobs, hipStream_t stream, RghJob *jobs_dev) {
-    HIP_TRY(hipMemcpyAsync(jobs_dev, jobs.data(), sizeof(RghJob) * (size_t)n, hipMemcpyHostToDevice, stream));
-    int steps = 0, width = 0, rows = 0;
-    for (int i = 0; i < n; i++) {
-        steps = std::max(steps, jobs[i].steps);
-        width = std::max(width, std::min(jobs[i].mb_h, (jobs[i].mb_w + 1) / 2));
-        rows = std::max(rows, ctxs[i]->rows_y);
-    }
-    dim3 grid((unsigned)width, (unsigned)n);
+rgh_status ensure_jobs(Ctx *c, int n);
+
+// Runs the schedule of `jobs` (frame-major: job k * n + i is frame k of context i, planned to
+// start at launch k * lag) on context 0's stream and waits for it. Launch L runs every frame
+// planned to be active then; a frame that fell behind (deferred macroblocks) keeps running in
+// later launches, and after the planned launches the per-frame delays are read back and the
+// launches continue until every frame has run all of its steps.
+rgh_status run_schedule(Ctx *const *ctxs, int n, int n_frames, std::vector<RghJob> &jobs, int lag) {
     Ctx *c0 = ctxs[0];
+    const int nj = n * n_frames;
+    std::vector<int> span((size_t)nj);
+    int gx = 1, max_span = 0;
+    for (int j = 0; j < nj; j++) {
+        const Ctx *c = ctxs[j % n];
+        span[(size_t)j] = frame_span(c);
+        max_span = std::max(max_span, span[(size_t)j]);
+        gx = std::max(gx, std::min(c->mb_h, (c->mb_w + 1) / 2) + std::min(c->mb_h + 1, (c->mb_w + 2) / 2));
+    }
+    hipStream_t stream = c0->stream;
+    HIP_TRY(hipMemcpyAsync(c0->jobs_dev, jobs.data(), sizeof(RghJob) * (size_t)nj, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemsetAsync(c0->ctl_dev, 0, sizeof(RghFrameCtl) * (size_t)nj, stream));
     if (c0->timing) HIP_TRY(hipEventRecord(c0->ev[0], stream));
-    for (int t = 0; t < steps; t++) hipLaunchKernelGGL(rgh_k_analyse_step, grid, dim3(64), 0, stream, jobs_dev, t);
+    auto launch = [&](int L, int j_lo, int j_hi) -> rgh_status {
+        const uint32_t seq = g_seq.fetch_add(1);
+        hipLaunchKernelGGL(rgh_k_frame_step, dim3((unsigned)gx, (unsigned)(j_hi - j_lo)), dim3(64), 0, stream,
+                           c0->jobs_dev, j_lo, L, seq);
+        return RGH_OK;
+    };
+    // Planned launches: frame k is active over [k * lag, k * lag + span + kMaxDelay).
+    const int planned = (n_frames - 1) * lag + max_span;
+    std::vector<int> last((size_t)nj, -1);  // last launch each frame took part in
+    int L = 0;
+    for (; L < planned; L++) {
+        int k_hi = std::min(n_frames - 1, L / lag);
+        int k_lo = 0;
+        while (k_lo <= k_hi && k_lo * lag + max_span + kMaxDelay <= L) k_lo++;
+        if (k_lo > k_hi) continue;
+        launch(L, k_lo * n, (k_hi + 1) * n);
+        for (int j = k_lo * n; j < (k_hi + 1) * n; j++) last[(size_t)j] = L;
+    }
     HIP_TRY(hipGetLastError());
-    if (c0->timing) HIP_TRY(hipEventRecord(c0->ev[1], stream));
-    for (int i = 0; i < n; i++) {
-        Ctx *c = ctxs[i];
-        if (c->keep_pre) {
-            const Picture &w = c->pic[1 - c->ref];
-            HIP_TRY(hipMemcpy2DAsync(c->pre[0], (size_t)c->mb_w * 16, w.org[0], (size_t)c->sy, (size_t)c->mb_w * 16,
-                                     (size_t)c->mb_h * 16, hipMemcpyDeviceToDevice, stream));
-            for (int p = 1; p < 3; p++)
-                HIP_TRY(hipMemcpy2DAsync(c->pre[p], (size_t)c->mb_w * 8, w.org[p], (size_t)c->sc, (size_t)c->mb_w * 8,
-                                         (size_t)c->mb_h * 8, hipMemcpyDeviceToDevice, stream));
+    // Completion: frames that fell behind run on until done. A frame's next step after its last
+    // launch P is P + 1 - base - delay[(P + 1) & 1]; rejoining at launch L it needs the delay
+    // L - base - next.
+    int extra_total = 0;
+    for (;;) {
+        HIP_TRY(hipMemcpyAsync(c0->ctl_host, c0->ctl_dev, sizeof(RghFrameCtl) * (size_t)nj, hipMemcpyDeviceToHost,
+                               stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        int lo = nj, hi = -1, need = 0;
+        for (int j = 0; j < nj; j++) {
+            const int base = jobs[(size_t)j].base, P = last[(size_t)j];
+            const int next = P < 0 ? 0 : P + 1 - base - c0->ctl_host[j].delay[(P + 1) & 1];
+            if (next >= span[(size_t)j]) continue;
+            lo = std::min(lo, j);
+            hi = std::max(hi, j);
+            need = std::max(need, span[(size_t)j] - next);
+            c0->ctl_host[j].delay[0] = c0->ctl_host[j].delay[1] = L - base - next;
         }
+        if (hi < 0) break;
+        extra_total += need;
+        if (extra_total > 64 * max_span + 4096) {  // no progress: a defect
+            set_error("run_schedule", "frames stopped making progress", __LINE__);
+            return RGH_ERR_DEVICE;
+        }
+        HIP_TRY(hipMemcpyAsync(c0->ctl_dev + lo, c0->ctl_host + lo, sizeof(RghFrameCtl) * (size_t)(hi - lo + 1),
+                               hipMemcpyHostToDevice, stream));
+        for (int e = 0; e < need; e++, L++) launch(L, lo, hi + 1);
+        HIP_TRY(hipGetLastError());
+        for (int j = lo; j <= hi; j++) last[(size_t)j] = L - 1;
     }
-    for (int t = 0; t < steps; t++) hipLaunchKernelGGL(rgh_k_deblock_step, grid, dim3(64), 0, stream, jobs_dev, t);
-    hipLaunchKernelGGL(rgh_k_extend, dim3((unsigned)rows, 3, (unsigned)n), dim3(256), 0, stream, jobs_dev);
-    HIP_TRY(hipGetLastError());
     if (c0->timing) {
-        HIP_TRY(hipEventRecord(c0->ev[2], stream));
+        HIP_TRY(hipEventRecord(c0->ev[1], stream));
         c0->timed = true;
-        c0->timed_launches = steps;
-    }
-    for (int i = 0; i < n; i++) {
-        Ctx *c = ctxs[i];
-        c->ref = 1 - c->ref;
-        c->dcur = 1 - c->dcur;
-        c->analysed = true;
+        c0->timed_launches = L;
     }
     return RGH_OK;
+}
+
+// After a call analysed n_frames frames of c: the last one is the reference.
+void advance(Ctx *c, int n_frames, const rgh_mb_decision *last_dec) {
+    const int R = (int)c->pic.size();
+    for (int k = 0; k < n_frames; k++) c->pic[(c->ref + 1 + k) % R].serial = c->serial + 1 + (uint32_t)k;
+    c->ref = (c->ref + n_frames) % R;
+    c->serial += (uint32_t)n_frames;
+    c->prev = last_dec;
+    c->analysed = true;
 }
 
 rgh_status ensure_jobs(Ctx *c, int n) {
     if (c->jobs_cap >= n) return RGH_OK;
     if (c->jobs_dev) HIP_TRY(hipFree(c->jobs_dev));
+    if (c->ctl_dev) HIP_TRY(hipFree(c->ctl_dev));
+    if (c->ctl_host) HIP_TRY(hipHostFree(c->ctl_host));
     c->jobs_dev = nullptr;
-    int cap = std::max(n, 8);
+    c->ctl_dev = nullptr;
+    c->ctl_host = nullptr;
+    c->jobs_cap = 0;
+    const int cap = std::max(n, 8);
     HIP_TRY(hipMalloc(&c->jobs_dev, sizeof(RghJob) * (size_t)cap));
+    HIP_TRY(hipMalloc(&c->ctl_dev, sizeof(RghFrameCtl) * (size_t)cap));
+    HIP_TRY(hipHostMalloc(&c->ctl_host, sizeof(RghFrameCtl) * (size_t)cap, hipHostMallocDefault));
     c->jobs_cap = cap;
     return RGH_OK;
 }
@@ -205,6 +310,8 @@ rgh_status check_i420(const rgh_i420 *p, int w, int h) {
 }  // namespace
 
 extern "C" {
+
+const char *rgh_last_error(void) { return g_last_error; }
 
 const char *rgh_status_str(int32_t s) {
     switch (s) {
@@ -230,6 +337,7 @@ rgh_status rgh_create(const rgh_config *cfg, void **ctx_out) {
     if (cfg->qp < 1 || cfg->qp > 51) return RGH_ERR_UNSUPPORTED;
     if (cfg->me_range < 0 || cfg->me_range > 1024 || cfg->qpel_iters < 0 || cfg->qpel_iters > 8)
         return RGH_ERR_INVALID_ARGUMENT;
+    if (cfg->ring < 0 || cfg->ring == 1 || cfg->ring > 64) return RGH_ERR_INVALID_ARGUMENT;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RGH_ERR_DEVICE;
     if (cfg->device < 0 || cfg->device >= ndev) return RGH_ERR_INVALID_ARGUMENT;
@@ -269,6 +377,8 @@ rgh_status rgh_create(const rgh_config *cfg, void **ctx_out) {
     if (hipSetDevice(c->device) != hipSuccess) return fail(RGH_ERR_DEVICE);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(RGH_ERR_DEVICE);
     const size_t ny = (size_t)c->sy * c->rows_y + 64, nc = (size_t)c->sc * c->rows_c + 64;
+    const size_t nmb = (size_t)c->mb_w * c->mb_h, nit = (size_t)(c->mb_w + 1) * (c->mb_h + 1);
+    c->pic.resize((size_t)(cfg->ring ? cfg->ring : 2));
     for (auto &p : c->pic) {
         size_t sizes[3] = {ny, nc, nc};
         for (int k = 0; k < 3; k++) {
@@ -278,15 +388,21 @@ rgh_status rgh_create(const rgh_config *cfg, void **ctx_out) {
             int pad = k ? RGH_PAD_C : RGH_PAD_Y, s = k ? c->sc : c->sy;
             p.org[k] = p.base[k] + (size_t)pad * s + pad;
         }
-    }
-    const size_t n = (size_t)c->mb_w * c->mb_h;
-    for (auto &d : c->dec) {
-        hipError_t e = hipMalloc(&d, n * sizeof(rgh_mb_decision));
+        hipError_t e = hipMalloc(&p.ana, nmb * sizeof(uint64_t));
+        if (e == hipSuccess) e = hipMalloc(&p.fin, nit * sizeof(uint64_t));
         if (e != hipSuccess) return fail(hip_status(e));
-        if (hipMemset(d, 0, n * sizeof(rgh_mb_decision)) != hipSuccess) return fail(RGH_ERR_DEVICE);
+        // Serial 0 is never a frame's: zeroed stamps mean "not done".
+        if (hipMemset(p.ana, 0, nmb * sizeof(uint64_t)) != hipSuccess ||
+            hipMemset(p.fin, 0, nit * sizeof(uint64_t)) != hipSuccess)
+            return fail(RGH_ERR_DEVICE);
+    }
+    for (auto &d : c->dec) {
+        hipError_t e = hipMalloc(&d, nmb * sizeof(rgh_mb_decision));
+        if (e != hipSuccess) return fail(hip_status(e));
+        if (hipMemset(d, 0, nmb * sizeof(rgh_mb_decision)) != hipSuccess) return fail(RGH_ERR_DEVICE);
     }
     {
-        hipError_t e = hipMalloc(&c->levels, n * RGH_LEVELS_PER_MB * sizeof(int16_t));
+        hipError_t e = hipMalloc(&c->levels, nmb * RGH_LEVELS_PER_MB * sizeof(int16_t));
         if (e != hipSuccess) return fail(hip_status(e));
     }
     const size_t srcs[3] = {(size_t)c->width * c->height, (size_t)((c->width + 1) / 2) * ((c->height + 1) / 2),
@@ -294,8 +410,10 @@ rgh_status rgh_create(const rgh_config *cfg, void **ctx_out) {
     for (int k = 0; k < 3; k++) {
         hipError_t e = hipMalloc(&c->src[k], srcs[k]);
         if (e != hipSuccess) return fail(hip_status(e));
-        e = hipMalloc(&c->pre[k], k ? (size_t)W * H / 4 : (size_t)W * H);
-        if (e != hipSuccess) return fail(hip_status(e));
+        if (c->keep_pre) {
+            e = hipMalloc(&c->pre[k], k ? (size_t)W * H / 4 : (size_t)W * H);
+            if (e != hipSuccess) return fail(hip_status(e));
+        }
     }
     rgh_status s = ensure_jobs(c, 1);
     if (s != RGH_OK) return fail(s);
@@ -320,46 +438,138 @@ rgh_status rgh_set_reference(void *ctx, const rgh_i420 *recon) {
     if (s != RGH_OK) return s;
     HIP_TRY(hipSetDevice(c->device));
     // The picture goes into the reference slot and is padded there.
-    Picture &p = c->pic[c->ref];
+    Picture &p = c->pic[(size_t)c->ref];
     HIP_TRY(hipMemcpy2DAsync(p.org[0], (size_t)c->sy, recon->y, (size_t)recon->stride_y, (size_t)W, (size_t)H,
                              hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpy2DAsync(p.org[1], (size_t)c->sc, recon->u, (size_t)recon->stride_u, (size_t)W / 2, (size_t)H / 2,
                              hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpy2DAsync(p.org[2], (size_t)c->sc, recon->v, (size_t)recon->stride_v, (size_t)W / 2, (size_t)H / 2,
                              hipMemcpyHostToDevice, c->stream));
-    RghJob j = make_job(c, nullptr, nullptr, nullptr, 0, 0, 0);
+    RghJob j;
+    std::memset(&j, 0, sizeof(j));
     j.cur_y = p.org[0];
     j.cur_u = p.org[1];
     j.cur_v = p.org[2];
+    j.sy = c->sy;
+    j.sc = c->sc;
+    j.mb_w = c->mb_w;
+    j.mb_h = c->mb_h;
     HIP_TRY(hipMemcpyAsync(c->jobs_dev, &j, sizeof(j), hipMemcpyHostToDevice, c->stream));
     hipLaunchKernelGGL(rgh_k_extend, dim3((unsigned)c->rows_y, 3, 1), dim3(256), 0, c->stream, c->jobs_dev);
+    // A new serial; every item of the picture is final for any later launch.
+    c->serial += 1;
+    p.serial = c->serial;
+    const int nit = (c->mb_w + 1) * (c->mb_h + 1);
+    hipLaunchKernelGGL(rgh_k_fill_u64, dim3((unsigned)((nit + 255) / 256)), dim3(256), 0, c->stream, p.fin, nit,
+                       (uint64_t)c->serial << 32);
     HIP_TRY(hipGetLastError());
     // An IDR picture is all intra: every co-located vector of the next frame is zero.
-    HIP_TRY(hipMemsetAsync(c->dec[1 - c->dcur], 0, sizeof(rgh_mb_decision) * (size_t)c->mb_w * c->mb_h, c->stream));
+    rgh_mb_decision *z = c->dec[1 - c->dcur];
+    HIP_TRY(hipMemsetAsync(z, 0, sizeof(rgh_mb_decision) * (size_t)c->mb_w * c->mb_h, c->stream));
+    c->prev = z;
     HIP_TRY(hipStreamSynchronize(c->stream));
     c->has_ref = true;
     return RGH_OK;
 }
 
+// One frame of each context, outputs into the contexts' own buffers.
+rgh_status analyse_single(Ctx *const *cs, int n, const rgh_i420 *src) {
+    Ctx *c0 = cs[0];
+    HIP_TRY(hipSetDevice(c0->device));
+    rgh_status s = ensure_jobs(c0, n);
+    if (s != RGH_OK) return s;
+    std::vector<RghJob> jobs((size_t)n);
+    for (int i = 0; i < n; i++) {
+        Ctx *c = cs[i];
+        jobs[(size_t)i] = make_job(c, 0, src[i], c->dec[c->dcur], c->prev, c->levels, 0, c0->ctl_dev + i, nullptr);
+        if (c->keep_pre) {
+            jobs[(size_t)i].pre_y = c->pre[0];
+            jobs[(size_t)i].pre_u = c->pre[1];
+            jobs[(size_t)i].pre_v = c->pre[2];
+        }
+    }
+    s = run_schedule(cs, n, 1, jobs, kDefaultLag);
+    if (s != RGH_OK) return s;
+    for (int i = 0; i < n; i++) {
+        Ctx *c = cs[i];
+        advance(c, 1, c->dec[c->dcur]);
+        c->last_levels = c->levels;
+        c->dcur = 1 - c->dcur;
+    }
+    return RGH_OK;
+}
+
 rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src) {
     if (!ctxs || n <= 0 || !src) return RGH_ERR_INVALID_ARGUMENT;
-    std::vector<RghJob> jobs((size_t)n);
     std::vector<Ctx *> cs((size_t)n);
     for (int i = 0; i < n; i++) {
         Ctx *c = static_cast<Ctx *>(ctxs[i]);
         if (!c) return RGH_ERR_INVALID_ARGUMENT;
         if (c->device != static_cast<Ctx *>(ctxs[0])->device) return RGH_ERR_INVALID_ARGUMENT;
         if (!c->has_ref) return RGH_ERR_NO_REFERENCE;
+        for (int k = 0; k < i; k++)
+            if (cs[(size_t)k] == c) return RGH_ERR_INVALID_ARGUMENT;
         rgh_status s = check_i420(&src[i], c->width, c->height);
         if (s != RGH_OK) return s;
-        cs[i] = c;
-        jobs[i] = make_job(c, src[i].y, src[i].u, src[i].v, src[i].stride_y, src[i].stride_u, src[i].stride_v);
+        cs[(size_t)i] = c;
     }
+    return analyse_single(cs.data(), n, src);
+}
+
+rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src,
+                              rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag) {
+    if (!ctxs || n <= 0 || n_frames <= 0 || !src || !decisions || !levels || lag < 0) return RGH_ERR_INVALID_ARGUMENT;
+    if ((int64_t)n * n_frames > (1 << 20)) return RGH_ERR_INVALID_ARGUMENT;
+    std::vector<Ctx *> cs((size_t)n);
+    for (int i = 0; i < n; i++) {
+        Ctx *c = static_cast<Ctx *>(ctxs[i]);
+        if (!c) return RGH_ERR_INVALID_ARGUMENT;
+        if (c->device != static_cast<Ctx *>(ctxs[0])->device) return RGH_ERR_INVALID_ARGUMENT;
+        if (!c->has_ref) return RGH_ERR_NO_REFERENCE;
+        for (int k = 0; k < i; k++)
+            if (cs[(size_t)k] == c) return RGH_ERR_INVALID_ARGUMENT;
+        for (int k = 0; k < n_frames; k++) {
+            const size_t q = (size_t)i * n_frames + k;
+            rgh_status s = check_i420(&src[q], c->width, c->height);
+            if (s != RGH_OK) return s;
+            if (!decisions[q] || !levels[q]) return RGH_ERR_INVALID_ARGUMENT;
+        }
+        cs[(size_t)i] = c;
+    }
+    if (lag == 0) lag = kDefaultLag;
     Ctx *c0 = cs[0];
     HIP_TRY(hipSetDevice(c0->device));
-    rgh_status s = ensure_jobs(c0, n);
+    const int nj = n * n_frames;
+    rgh_status s = ensure_jobs(c0, nj);
     if (s != RGH_OK) return s;
-    return run_jobs(cs.data(), n, jobs, c0->stream, c0->jobs_dev);
+    std::vector<RghJob> jobs((size_t)nj);
+    for (int k = 0; k < n_frames; k++)
+        for (int i = 0; i < n; i++) {
+            Ctx *c = cs[(size_t)i];
+            const size_t q = (size_t)i * n_frames + k;
+            const int R = (int)c->pic.size();
+            // The picture slot of frame k last served as the reference of frame k + 1 - R.
+            const int wk = k + 1 - R;
+            const RghFrameCtl *wait = wk >= 0 ? c0->ctl_dev + (size_t)wk * n + i : nullptr;
+            const rgh_mb_decision *prev = k ? decisions[q - 1] : c->prev;
+            jobs[(size_t)k * n + i] =
+                make_job(c, k, src[q], decisions[q], prev, levels[q], k * lag, c0->ctl_dev + (size_t)k * n + i, wait);
+        }
+    s = run_schedule(cs.data(), n, n_frames, jobs, lag);
+    if (s != RGH_OK) return s;
+    // The last frame's decisions carry the co-located vectors into the next call.
+    for (int i = 0; i < n; i++) {
+        Ctx *c = cs[(size_t)i];
+        rgh_mb_decision *keep = c->dec[c->dcur];
+        HIP_TRY(hipMemcpyAsync(keep, decisions[(size_t)i * n_frames + n_frames - 1],
+                               sizeof(rgh_mb_decision) * (size_t)c->mb_w * c->mb_h, hipMemcpyDeviceToDevice,
+                               c0->stream));
+        advance(c, n_frames, keep);
+        c->last_levels = levels[(size_t)i * n_frames + n_frames - 1];
+        c->dcur = 1 - c->dcur;
+    }
+    HIP_TRY(hipStreamSynchronize(c0->stream));
+    return RGH_OK;
 }
 
 rgh_status rgh_analyse_p_frame(void *ctx, const rgh_i420 *src, rgh_mb_decision *decisions, int16_t *levels,
@@ -377,14 +587,11 @@ rgh_status rgh_analyse_p_frame(void *ctx, const rgh_i420 *src, rgh_mb_decision *
                              hipMemcpyHostToDevice, c->stream));
     HIP_TRY(hipMemcpy2DAsync(c->src[2], (size_t)cw, src->v, (size_t)src->stride_v, (size_t)cw, (size_t)ch,
                              hipMemcpyHostToDevice, c->stream));
-    std::vector<RghJob> jobs(1, make_job(c, c->src[0], c->src[1], c->src[2], c->width, cw, cw));
+    rgh_i420 dsrc = {c->src[0], c->src[1], c->src[2], c->width, cw, cw, c->width, c->height};
     Ctx *cs[1] = {c};
-    s = run_jobs(cs, 1, jobs, c->stream, c->jobs_dev);
+    s = analyse_single(cs, 1, &dsrc);
     if (s != RGH_OK) return s;
-    s = rgh_get_outputs(ctx, decisions, levels, recon, 1);
-    if (s != RGH_OK) return s;
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return RGH_OK;
+    return rgh_get_outputs(ctx, decisions, levels, recon, 1);
 }
 
 rgh_status rgh_get_outputs(void *ctx, rgh_mb_decision *decisions, int16_t *levels, const rgh_i420_out *recon,
@@ -394,12 +601,11 @@ rgh_status rgh_get_outputs(void *ctx, rgh_mb_decision *decisions, int16_t *level
     if (!c->analysed) return RGH_ERR_NO_REFERENCE;
     HIP_TRY(hipSetDevice(c->device));
     const size_t n = (size_t)c->mb_w * c->mb_h;
-    // After a frame the written decisions became "previous" and the written picture the reference.
+    // The last analysed frame's decisions are the reference's; its picture is the reference.
     if (decisions)
-        HIP_TRY(hipMemcpyAsync(decisions, c->dec[1 - c->dcur], n * sizeof(rgh_mb_decision), hipMemcpyDeviceToHost,
-                               c->stream));
+        HIP_TRY(hipMemcpyAsync(decisions, c->prev, n * sizeof(rgh_mb_decision), hipMemcpyDeviceToHost, c->stream));
     if (levels)
-        HIP_TRY(hipMemcpyAsync(levels, c->levels, n * RGH_LEVELS_PER_MB * sizeof(int16_t), hipMemcpyDeviceToHost,
+        HIP_TRY(hipMemcpyAsync(levels, c->last_levels, n * RGH_LEVELS_PER_MB * sizeof(int16_t), hipMemcpyDeviceToHost,
                                c->stream));
     if (recon) {
         const int W = c->mb_w * 16, H = c->mb_h * 16;
@@ -411,7 +617,7 @@ rgh_status rgh_get_outputs(void *ctx, rgh_mb_decision *decisions, int16_t *level
             ss[0] = (size_t)W;
             ss[1] = ss[2] = (size_t)W / 2;
         } else {
-            for (int k = 0; k < 3; k++) sp[k] = c->pic[c->ref].org[k];
+            for (int k = 0; k < 3; k++) sp[k] = c->pic[(size_t)c->ref].org[k];
             ss[0] = (size_t)c->sy;
             ss[1] = ss[2] = (size_t)c->sc;
         }
@@ -460,9 +666,9 @@ rgh_status rgh_get_timing(void *ctx, float *analyse_ms, float *deblock_ms, int32
     Ctx *c = static_cast<Ctx *>(ctx);
     if (!c->timed) return RGH_ERR_INVALID_ARGUMENT;
     HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipEventSynchronize(c->ev[2]));
+    HIP_TRY(hipEventSynchronize(c->ev[1]));
     HIP_TRY(hipEventElapsedTime(analyse_ms, c->ev[0], c->ev[1]));
-    HIP_TRY(hipEventElapsedTime(deblock_ms, c->ev[1], c->ev[2]));
+    *deblock_ms = 0.0f;
     *launches = c->timed_launches;
     return RGH_OK;
 }

@@ -1,6 +1,7 @@
 // In-loop deblocking (ITU-T H.264 8.7) of an analysed P frame, one wave64 per macroblock in
-// the same anti-diagonal wavefront order as the analysis, then border extension of the
-// deblocked picture, which becomes the next reference.
+// the same anti-diagonal wavefront order as the analysis, and border extension of the deblocked
+// picture, which becomes the next reference. Device code only: the frame-step kernel that runs
+// these items lives with the analysis (rgh_analyse.hip).
 //
 // Restates ragdoll's `EncState::deblock` (crates/ragdoll-codec/src/h264/enc.rs:3776-3881), its
 // `EdgeFilter` (:3902-3942), the edge kernels of crates/ragdoll-simd/src/h264_deblock.rs
@@ -11,6 +12,11 @@
 // ((x-1, y), (x, y-1), (x+1, y-1)) lies on an earlier step t = x + 2y, every macroblock that
 // must see its result lies on a later one, and macroblocks of one step are 32 or more samples
 // apart. Each macroblock works on a 20x20 luma (10x12 chroma) copy in LDS and writes it back.
+//
+// Border extension is split by macroblock: the padding samples whose nearest interior sample
+// lies in border macroblock (x, y) are written once that macroblock is final, i.e. after the
+// deblock items (x + 1, y) and (x, y + 1) that still modify it.
+#pragma once
 #include "rgh_device.h"
 #include "rgh_tables.h"
 
@@ -36,13 +42,20 @@ __device__ Edge make_edge(int qp, const int bs[4]) {
     E.beta = c_beta[ia];
     E.on = E.alpha != 0 && E.beta != 0;
     E.strong = bs[0] == 4;
+#pragma unroll
     for (int k = 0; k < 4; k++) E.tc0[k] = bs[k] == 0 ? -1 : (bs[k] >= 4 ? 0 : (int)c_tc0[ia][bs[k] - 1]);
     return E;
 }
 
+// tc0 of 4-line segment g, selected without indexing the array (a dynamic index would put
+// the edge in scratch memory).
+__device__ __forceinline__ int seg_tc0(const Edge &E, int g) {
+    return g == 0 ? E.tc0[0] : (g == 1 ? E.tc0[1] : (g == 2 ? E.tc0[2] : E.tc0[3]));
+}
+
 // simd/h264_deblock.rs:69 h264_luma_deblock_scalar, one line (p3..q3 at q0 - 4s .. q0 + 3s).
 __device__ void luma_line(uint8_t *b, int q0, int s, const Edge &E, int k) {
-    int tc0 = E.tc0[k >> 2];
+    int tc0 = seg_tc0(E, k >> 2);
     if (!E.strong && tc0 < 0) return;
     int p0 = b[q0 - s], q0v = b[q0], p1 = b[q0 - 2 * s], q1 = b[q0 + s];
     if (iabs(p0 - q0v) >= E.alpha || iabs(p1 - p0) >= E.beta || iabs(q1 - q0v) >= E.beta) return;
@@ -77,7 +90,7 @@ __device__ void luma_line(uint8_t *b, int q0, int s, const Edge &E, int k) {
 
 // simd/h264_deblock.rs:206 h264_chroma_deblock_scalar, one line (p1..q1).
 __device__ void chroma_line(uint8_t *b, int q0, int s, const Edge &E, int k) {
-    int tc0 = E.tc0[k >> 1];
+    int tc0 = seg_tc0(E, k >> 1);
     if (!E.strong && tc0 < 0) return;
     int p1 = b[q0 - 2 * s], p0 = b[q0 - s], q0v = b[q0], q1 = b[q0 + s];
     if (iabs(p0 - q0v) >= E.alpha || iabs(p1 - p0) >= E.beta || iabs(q1 - q0v) >= E.beta) return;
@@ -208,18 +221,33 @@ __device__ void deblock_mb(const RghJob &J, int mb_x, int mb_y, DbLds &S) {
     }
 }
 
-}  // namespace
-
-extern "C" __global__ __launch_bounds__(64) void rgh_k_deblock_step(const RghJob *__restrict__ jobs, int t) {
-    const RghJob J = jobs[blockIdx.y];
-    if (!J.deblock || t >= J.steps) return;
-    int a = t - J.mb_w + 1;
-    int ylo = a > 0 ? (a + 1) >> 1 : 0, yhi = min(J.mb_h - 1, t >> 1);
-    int y = ylo + (int)blockIdx.x;
-    if (y > yhi) return;
-    __shared__ DbLds S;
-    deblock_mb(J, t - 2 * y, y, S);
+// enc.rs:164 extend_borders, restricted to the padding samples whose nearest interior sample
+// lies in border macroblock (X, Y) of the current picture, all three planes. Rows and columns
+// are dword aligned (strides are multiples of 32, paddings of 16).
+__device__ void pad_mb(const RghJob &J, int X, int Y) {
+    const int mb_w = J.mb_w, mb_h = J.mb_h;
+    if (X != 0 && X != mb_w - 1 && Y != 0 && Y != mb_h - 1) return;
+    const int lane = threadIdx.x;
+#pragma unroll 1
+    for (int p = 0; p < 3; p++) {
+        const int B = p ? 8 : 16, pad = p ? RGH_PAD_C : RGH_PAD_Y, s = p ? J.sc : J.sy;
+        const int w = mb_w * B, h = mb_h * B;
+        uint8_t *o = p == 0 ? J.cur_y : (p == 1 ? J.cur_u : J.cur_v);
+        const int r0 = Y == 0 ? -pad : Y * B, r1 = Y == mb_h - 1 ? h + pad : (Y + 1) * B;
+        const int c0 = X == 0 ? -pad : X * B, c1 = X == mb_w - 1 ? s - pad : (X + 1) * B;
+        const int nd = (c1 - c0) >> 2, n = nd * (r1 - r0);
+        for (int i = lane; i < n; i += 64) {
+            const int r = r0 + i / nd, c = c0 + 4 * (i % nd);
+            const bool cin = c >= 0 && c < w;
+            if (cin && r >= 0 && r < h) continue;
+            const uint8_t *srow = o + (long)iclamp(r, 0, h - 1) * s;
+            const uint32_t v = cin ? *(const uint32_t *)(srow + c) : 0x01010101u * srow[c < 0 ? 0 : w - 1];
+            *(uint32_t *)(o + (long)r * s + c) = v;
+        }
+    }
 }
+
+}  // namespace
 
 // Edge replication of the padding of job blockIdx.z's current picture (enc.rs:164
 // extend_borders): plane blockIdx.y (0 Y, 1 U, 2 V), padded row blockIdx.x.

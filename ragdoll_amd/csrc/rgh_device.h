@@ -9,21 +9,50 @@
 #define RGH_PAD_C 16      // chroma padding (enc.rs:58)
 #define RGH_MV_MARGIN 28  // how far a vector may move a block past the edge (enc.rs:61)
 
-// One stream's P frame in a batched launch. Plane pointers address sample (0, 0) inside the
+// Progress of one frame of a schedule, in device memory. delay[launch & 1] is the number of
+// launches the frame has fallen behind its planned start (its macroblocks wait for reference
+// samples that are not final yet); blocks of launch L read delay[L & 1] and raise
+// delay[(L + 1) & 1]. analysed counts finished analysis items.
+struct RghFrameCtl {
+    int32_t delay[2];
+    int32_t analysed;
+    int32_t pad_;
+};
+
+// A completion stamp: frame serial number in the high half, schedule launch number in the low
+// half. A stamp counts as final for a reader in launch `seq` only when it was written by an
+// earlier launch (kernel boundaries make the data it covers visible).
+__device__ __forceinline__ bool stamp_final(uint64_t st, uint32_t serial, uint32_t seq) {
+    return (uint32_t)(st >> 32) == serial && (uint32_t)st < seq;
+}
+__device__ __forceinline__ uint64_t make_stamp(uint32_t serial, uint32_t seq) { return ((uint64_t)serial << 32) | seq; }
+
+// One P frame of one stream in a schedule. Plane pointers address sample (0, 0) inside the
 // padding, so negative offsets reach the padding.
 struct RghJob {
     const uint8_t *src_y, *src_u, *src_v;
     int32_t src_sy, src_su, src_sv, width, height;
     const uint8_t *ref_y, *ref_u, *ref_v;
     uint8_t *cur_y, *cur_u, *cur_v;
+    uint8_t *pre_y, *pre_u, *pre_v;  // optional copy of the reconstruction before deblocking (W x H)
     int32_t sy, sc;            // padded strides
     int32_t mb_w, mb_h, qp, me_iters, qpel_iters, t8x8;
-    rgh_mb_decision *dec;
-    const rgh_mb_decision *prev;
+    rgh_mb_decision *dec;      // this frame's decisions (written)
+    const rgh_mb_decision *prev;  // the reference frame's decisions (co-located vectors)
     int16_t *levels;
     uint32_t sb_luma, sb_luma_ssd, sb_chroma, sb_chroma_ssd, sb_probe;
     int32_t steps;             // wavefront steps of the frame: (mb_w - 1) + 2 (mb_h - 1) + 1
     int32_t deblock;           // 0 when the slice disables the loop filter
+    // Schedule.
+    int32_t base;              // launch of the frame's first wavefront step
+    uint32_t serial, ref_serial;  // this frame's and its reference's serial numbers
+    uint64_t *ana;             // analysis stamps, mb_w x mb_h
+    uint64_t *fin;             // deblock/pad item stamps of this picture, (mb_w + 1) x (mb_h + 1)
+    const uint64_t *ref_fin;   // the same for the reference picture
+    RghFrameCtl *ctl;
+    const int32_t *wait_analysed;  // analysis count of the frame that last reads the picture slot
+    int32_t wait_total;        // ... which must reach this value before the first write
+    int32_t pad_;
 };
 
 // Sum of a 32-bit value over the 64 lanes of the wave, broadcast as a scalar. DPP quad and row

@@ -49,4 +49,4 @@ def test_record_layouts_match_between_product_and_oracle():
     from oracle_lib import DECISION_DTYPE, LEVELS_PER_MB
     assert DECISION_DTYPE == abi.DECISION_DTYPE
     assert LEVELS_PER_MB == abi.LEVELS_PER_MB
-    assert ctypes.sizeof(abi.RgConfig) == 32
+    assert ctypes.sizeof(abi.RgConfig) == 36  # nine int32 fields of rgh_config

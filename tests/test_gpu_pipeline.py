@@ -1,0 +1,102 @@
+"""The pipelined schedule (rgh_analyse_frames) vs the CPU oracle, bit for bit.
+
+Several frames of a stream are in flight at once: frame k + 1 starts `lag` wavefront steps after
+frame k and its macroblocks wait (by deferral) for the reference samples they can read. A small
+lag makes most macroblocks defer; a ring of 2 pictures serialises the frames through the
+picture-reuse gate. Every frame's decisions and levels, and the last frame's deblocked
+reconstruction, must equal the oracle's.
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import OracleEncoder
+from ragdoll_amd import DECISION_DTYPE, LEVELS_PER_MB, StreamAnalyzer, analyse_frames
+from ragdoll_amd.hip import Hip
+from ragdoll_amd.synth import CONFIGS, SceneGenerator, config_seed, fixture_frame
+
+pytestmark = pytest.mark.gpu
+
+
+def scene(name, n, seed_extra=0):
+    kw = dict(CONFIGS[name])
+    w, h = kw.pop("width"), kw.pop("height")
+    g = SceneGenerator(w, h, seed=config_seed({"cif": 0, "720p": 1, "1080p": 2}[name], seed_extra), **kw)
+    return w, h, [g.next_frame() for _ in range(n)]
+
+
+def oracle_run(frames, w, h, qp, profile):
+    ora = OracleEncoder(w, h, qp, profile=profile, keyint=1 << 20)
+    out = []
+    for i, f in enumerate(frames):
+        ora.encode(*f)
+        if i == 0:
+            idr = ora.recon(deblocked=True)
+        else:
+            out.append((ora.decisions(), ora.levels()))
+    return idr, out, ora.recon(deblocked=True)
+
+
+def run_pipeline(streams, lag, ring, calls=1, label=""):
+    """streams: list of (frames, w, h, qp, profile). Runs frames 1.. of every stream through
+    rgh_analyse_frames (split into `calls` consecutive calls) and checks them against the oracle."""
+    hip = Hip()
+    ans, refs, bufs = [], [], []
+    try:
+        for frames, w, h, qp, profile in streams:
+            idr, want, final = oracle_run(frames, w, h, qp, profile)
+            a = StreamAnalyzer(w, h, qp, profile=profile, keep_undeblocked=False, ring=ring)
+            a.set_reference(*idr)
+            n = a.mb_w * a.mb_h
+            srcs, decs, lvs = [], [], []
+            for y, u, v in frames[1:]:
+                dy, du, dv = hip.upload(y), hip.upload(u), hip.upload(v)
+                bufs += [dy, du, dv]
+                srcs.append((dy, du, dv, y.shape[1], u.shape[1], v.shape[1]))
+                d, l = hip.alloc(n * 32), hip.alloc(n * LEVELS_PER_MB * 2)
+                hip.memset(d, 0xEE, n * 32)
+                bufs += [d, l]
+                decs.append(d)
+                lvs.append(l)
+            ans.append(a)
+            refs.append((want, final, srcs, decs, lvs, n))
+        nf = len(streams[0][0]) - 1
+        cuts = [round(i * nf / calls) for i in range(calls + 1)]
+        for c in range(calls):
+            sl = slice(cuts[c], cuts[c + 1])
+            analyse_frames(ans, [r[2][sl] for r in refs], [r[3][sl] for r in refs], [r[4][sl] for r in refs], lag=lag)
+        for s, (a, (want, final, srcs, decs, lvs, n)) in enumerate(zip(ans, refs)):
+            for k in range(nf):
+                dec = hip.download(decs[k], np.zeros(n, DECISION_DTYPE))
+                lv = hip.download(lvs[k], np.zeros((n, LEVELS_PER_MB), np.int16))
+                odec, olv = want[k]
+                bad = np.nonzero(dec.view(np.uint8).reshape(-1, 32) != odec.view(np.uint8).reshape(-1, 32))[0]
+                assert len(bad) == 0, "%s stream %d frame %d: decision of MB %d differs\n gpu %s\n ora %s" % (
+                    label, s, k + 1, bad[0], dec[bad[0]], odec[bad[0]])
+                badl = np.nonzero((lv != olv).any(axis=1))[0]
+                assert len(badl) == 0, "%s stream %d frame %d: levels of MB %d differ" % (label, s, k + 1, badl[0])
+            _, _, rec = a.outputs(which_recon=1, want_levels=False)
+            for p, (g, o) in enumerate(zip(rec, final)):
+                assert np.array_equal(g, o), "%s stream %d: final deblocked plane %d differs" % (label, s, p)
+    finally:
+        for a in ans:
+            a.close()
+        for b in bufs:
+            hip.free(b)
+
+
+# (12, 2): frames fall far enough behind to leave the planned launch range and rejoin later.
+@pytest.mark.parametrize("lag,ring", [(20, 8), (3, 8), (1, 4), (1, 2), (12, 2)])
+def test_cif_pipeline(lag, ring):
+    w, h, frames = scene("cif", 8)
+    print("pipeline cif lag %d ring %d" % (lag, ring), flush=True)
+    run_pipeline([(frames, w, h, 28, "baseline")], lag, ring, label="cif lag %d ring %d" % (lag, ring))
+
+
+def test_multi_stream_two_calls():
+    # Three streams of different sizes and profiles in one schedule, split over two calls.
+    print("pipeline multi-stream", flush=True)
+    w, h, cif = scene("cif", 6, 3)
+    fx = [fixture_frame(160, 90, i) for i in range(6)]
+    w7, h7, f720 = scene("720p", 6, 5)
+    run_pipeline([(cif, w, h, 26, "high"), (fx, 160, 90, 20, "baseline"), (f720, w7, h7, 30, "main")],
+                 lag=6, ring=6, calls=2, label="multi")
