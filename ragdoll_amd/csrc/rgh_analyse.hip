@@ -31,24 +31,33 @@ constexpr int WP_ROWS = 18;  // half-pel plane rows (simd/h264_enc.rs:710 HPEL_W
 constexpr int WP_COLS = 24;  // half-pel plane columns (HPEL_COLS)
 constexpr int HR_ROWS = 23;  // unrolled horizontal-tap rows
 
+// LDS of one macroblock's wave. The motion-search arrays (staged reference region, half-pel
+// window) and the intra/residual arrays are never live together (a barrier separates the last
+// prediction read from the region and the first residual write), so they share storage.
 struct MbLds {
     alignas(16) uint8_t src[256], srcu[64], srcv[64];
     alignas(16) uint8_t pred[256], predu[64], predv[64];
-    alignas(16) uint8_t tmp[256];
-    alignas(16) uint8_t sr[SR_H * SR_W];
-    alignas(16) int16_t hr[HR_ROWS * WP_COLS];
-    alignas(16) uint8_t ph[WP_ROWS * WP_COLS], pv[WP_ROWS * WP_COLS], pc[WP_ROWS * WP_COLS];
-    alignas(16) int16_t coef[16][16];  // luma levels, decoding order, raster positions
-    alignas(16) int16_t q8[4][64];     // 8x8 levels, raster positions
-    alignas(16) int32_t t8[4][64];     // 8x8 transform intermediate
-    alignas(16) int16_t craster[2][4][16];
-    alignas(16) int16_t cdc_raw[2][4];
-    alignas(16) int16_t cdc[2][4];
-    alignas(16) int16_t cdcr[2][4];
-    alignas(16) int16_t dc_raw[16], dcq[16], dcr[16];
-    alignas(16) int32_t bnz[16], bbig[16], bscore[16], cscore[8], cmask[8], b8score[4];
     alignas(16) uint8_t etop[16], eleft[16], ctop[2][8], cleft[2][8];
     alignas(16) int32_t etl, ctl[2];
+    union {
+        struct {
+            alignas(16) uint8_t sr[SR_H * SR_W];
+            alignas(16) int16_t hr[HR_ROWS * WP_COLS];
+            alignas(16) uint8_t ph[WP_ROWS * WP_COLS], pv[WP_ROWS * WP_COLS], pc[WP_ROWS * WP_COLS];
+        };
+        struct {
+            alignas(16) uint8_t tmp[256];
+            alignas(16) int16_t coef[16][16];  // luma levels, decoding order, raster positions
+            alignas(16) int16_t q8[4][64];     // 8x8 levels, raster positions
+            alignas(16) int32_t t8[4][64];     // 8x8 transform intermediate
+            alignas(16) int16_t craster[2][4][16];
+            alignas(16) int16_t cdc_raw[2][4];
+            alignas(16) int16_t cdc[2][4];
+            alignas(16) int16_t cdcr[2][4];
+            alignas(16) int16_t dc_raw[16], dcq[16], dcr[16];
+            alignas(16) int32_t bnz[16], bbig[16], bscore[16], cscore[8], cmask[8], b8score[4];
+        };
+    };
 };
 
 __device__ __forceinline__ uint32_t se_bits(int d) {  // enc.rs:546
@@ -204,18 +213,21 @@ __device__ __forceinline__ void add_idct4(uint8_t *dst, int s, int x, int y, con
 // Copies reference rows [oy, oy + ROWS) x columns [ox, ox + SR_COLS) (ox a multiple of 4)
 // into S.sr: every lane issues all of its aligned dword loads before the first LDS store.
 template <int ROWS>
-__device__ void stage_region(const RghJob &J, int ox, int oy, MbLds &S) {
+__device__ void stage_region(const RghJobD &J, int ox, int oy, MbLds &S) {
     static_assert(ROWS <= SR_H, "region too tall");
     constexpr int DW = SR_COLS / 4;  // 17 dwords a row
     constexpr int N = ROWS * DW, PER = (N + 63) / 64;
     const int lane = threadIdx.x;
-    const uint8_t *base = J.ref_y + (long)oy * J.sy + ox;
+    const RGH_G uint8_t *base = J.ref_y + (long)oy * J.sy + ox;
     uint32_t v[PER];
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const int i = lane + 64 * k, r = i / DW, j = i - r * DW;
-        v[k] = i < N ? *(const uint32_t *)(base + (long)r * J.sy + j * 4) : 0u;
+        v[k] = i < N ? *(const RGH_G uint32_t *)(base + (long)r * J.sy + j * 4) : 0u;
     }
+    // Earlier reads of this storage (the previous region, or residual arrays sharing it) are
+    // complete before it is overwritten.
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < PER; k++) {
         const int i = lane + 64 * k, r = i / DW, j = i - r * DW;
@@ -392,7 +404,7 @@ __device__ uint32_t satd16(const uint8_t *a, const uint8_t *b) {
 struct ChromaTaps {
     uint32_t u[4], v[4];
 };
-__device__ __forceinline__ ChromaTaps chroma_taps(const RghJob &J, int mb_x, int mb_y, int mx, int my) {
+__device__ __forceinline__ ChromaTaps chroma_taps(const RghJobD &J, int mb_x, int mb_y, int mx, int my) {
     const int lane = threadIdx.x, cr = lane >> 3, cc = lane & 7;
     const long off = (long)(mb_y * 8 + (my >> 3) + cr) * J.sc + mb_x * 8 + (mx >> 3) + cc;
     ChromaTaps t;
@@ -617,7 +629,7 @@ __device__ void recon8(MbLds &S, int cbpl, int qp) {
 //
 // Returns false, having written nothing to global memory, when a reference sample the
 // macroblock may read is not final yet for a reader in launch `seq`.
-__device__ bool analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S, uint32_t seq) {
+__device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint32_t seq) {
     const int lane = threadIdx.x;
     const int rr = lane >> 2, c0 = (lane & 3) * 4;
     const int mb_w = J.mb_w, idx = mb_y * mb_w + mb_x;
@@ -632,9 +644,9 @@ __device__ bool analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S, uint32
     uint32_t srcw = 0;
     {
         const int sy = min(y0 + rr, J.height - 1), sx = x0 + c0;
-        const uint8_t *row = J.src_y + (long)sy * J.src_sy;
+        const RGH_G uint8_t *row = J.src_y + (long)sy * J.src_sy;
         if (sx + 3 < J.width && (((uintptr_t)(row + sx)) & 3) == 0) {
-            srcw = *(const uint32_t *)(row + sx);
+            srcw = *(const RGH_G uint32_t *)(row + sx);
         } else {
 #pragma unroll
             for (int i = 0; i < 4; i++) srcw |= (uint32_t)row[min(sx + i, J.width - 1)] << (8 * i);
@@ -650,19 +662,19 @@ __device__ bool analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S, uint32
     // Neighbour motion: left, top, top-right (else top-left) decisions of this frame.
     int lref = -2, tref = -2, dref = -2, ax = 0, ay = 0, bx_ = 0, by_ = 0, cxv = 0, cyv = 0;
     if (mb_x > 0) {
-        const rgh_mb_decision &d = J.dec[idx - 1];
+        const rgh_mb_decision d = load_dec(&J.dec[idx - 1]);
         lref = d.mb_type >= 2 ? -1 : 0;
         ax = d.mv[0];
         ay = d.mv[1];
     }
     if (mb_y > 0) {
-        const rgh_mb_decision &d = J.dec[idx - mb_w];
+        const rgh_mb_decision d = load_dec(&J.dec[idx - mb_w]);
         tref = d.mb_type >= 2 ? -1 : 0;
         bx_ = d.mv[0];
         by_ = d.mv[1];
         const int ci = mb_x + 1 < mb_w ? idx - mb_w + 1 : (mb_x > 0 ? idx - mb_w - 1 : -1);
         if (ci >= 0) {
-            const rgh_mb_decision &c = J.dec[ci];
+            const rgh_mb_decision c = load_dec(&J.dec[ci]);
             dref = c.mb_type >= 2 ? -1 : 0;
             cxv = c.mv[0];
             cyv = c.mv[1];
@@ -675,7 +687,7 @@ __device__ bool analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S, uint32
         S.eleft[lane] = mb_x > 0 ? J.cur_y[(long)(y0 + lane) * J.sy + x0 - 1] : 0;
     } else if (lane < 32) {
         int p = (lane >> 3) & 1, k = lane & 7;
-        const uint8_t *cp = p ? J.cur_v : J.cur_u;
+        const RGH_G uint8_t *cp = p ? J.cur_v : J.cur_u;
         S.ctop[p][k] = mb_y > 0 ? cp[(long)(cy0 - 1) * J.sc + cx0 + k] : 0;
         S.cleft[p][k] = mb_x > 0 ? cp[(long)(cy0 + k) * J.sc + cx0 - 1] : 0;
     } else if (lane == 32) {
@@ -826,7 +838,7 @@ __device__ bool analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S, uint32
         const int present[5] = {1, 1, mb_x > 0, mb_y > 0, 1};
         int fxs[5], fys[5], live[5];
         uint32_t rw[5];
-        const uint8_t *rb = J.ref_y + (long)(y0 + rr) * J.sy + x0 + c0;
+        const RGH_G uint8_t *rb = J.ref_y + (long)(y0 + rr) * J.sy + x0 + c0;
 #pragma unroll
         for (int k = 0; k < 5; k++) {
             fxs[k] = iclamp((cvx[k] + 2) >> 2, imn0, imx0);
@@ -1366,7 +1378,7 @@ __device__ bool analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S, uint32
         J.pre_v[(long)(cy0 + (lane >> 3)) * (W / 2) + cx0 + (lane & 7)] = S.predv[lane];
     }
 
-    int16_t *lv = J.levels + (long)idx * RGH_LEVELS_PER_MB;
+    RGH_G int16_t *lv = J.levels + (long)idx * RGH_LEVELS_PER_MB;
     uint32_t nnz = 0;
     if (mb_type == 1 || mb_type == 2) {
         if (mb_type == 1 && use8) {
@@ -1403,7 +1415,7 @@ __device__ bool analyse_mb(const RghJob &J, int mb_x, int mb_y, MbLds &S, uint32
         D.mv[0] = (int16_t)mvx;
         D.mv[1] = (int16_t)mvy;
         D.nnz = (uint16_t)nnz;
-        J.dec[idx] = D;
+        store_dec(&J.dec[idx], D);
     }
     return true;
 }
@@ -1427,12 +1439,12 @@ union StepLds {
 // stamps). Nothing waits inside a launch.
 extern "C" __global__ __launch_bounds__(64) void rgh_k_frame_step(const RghJob *__restrict__ jobs, int j0, int launch,
                                                                    uint32_t seq) {
-    const RghJob J = jobs[j0 + (int)blockIdx.y];
-    RghFrameCtl *ctl = J.ctl;
+    const RghJobD J = device_view(jobs[j0 + (int)blockIdx.y]);
+    RGH_G RghFrameCtl *ctl = J.ctl;
     const int lane = threadIdx.x;
     const int delay = ctl->delay[launch & 1];
     const int s = launch - J.base - delay;
-    int *next_delay = &ctl->delay[(launch + 1) & 1];
+    int *next_delay = (int *)&ctl->delay[(launch + 1) & 1];
     if (blockIdx.x == 0 && lane == 0 && delay > 0) atomicMax(next_delay, delay);
     const int na = min(J.mb_h, (J.mb_w + 1) >> 1);
     const int b = (int)blockIdx.x;
@@ -1450,7 +1462,7 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_frame_step(const RghJob *
         if (lane == 0) {
             if (ok) {
                 J.ana[m] = make_stamp(J.serial, seq);
-                atomicAdd(&ctl->analysed, 1);
+                atomicAdd((int *)&ctl->analysed, 1);
             } else {
                 atomicMax(next_delay, delay + 1);
             }

@@ -111,12 +111,12 @@ __device__ __forceinline__ uint32_t nz_col(uint32_t nnz, int c) {
     return ((nnz >> c) & 1) | (((nnz >> (4 + c)) & 1) << 1) | (((nnz >> (8 + c)) & 1) << 2) | (((nnz >> (12 + c)) & 1) << 3);
 }
 
-__device__ void deblock_mb(const RghJob &J, int mb_x, int mb_y, DbLds &S) {
+__device__ void deblock_mb(const RghJobD &J, int mb_x, int mb_y, DbLds &S) {
     const int lane = threadIdx.x, mb_w = J.mb_w, m = mb_y * mb_w + mb_x;
-    const rgh_mb_decision Q = J.dec[m];
-    rgh_mb_decision L, T;
-    if (mb_x > 0) L = J.dec[m - 1];
-    if (mb_y > 0) T = J.dec[m - mb_w];
+    const rgh_mb_decision Q = load_dec(&J.dec[m]);
+    rgh_mb_decision L{}, T{};
+    if (mb_x > 0) L = load_dec(&J.dec[m - 1]);
+    if (mb_y > 0) T = load_dec(&J.dec[m - mb_w]);
     auto mveq = [](const rgh_mb_decision &a, const rgh_mb_decision &b) { return a.mv[0] == b.mv[0] && a.mv[1] == b.mv[1]; };
     if (quiet(Q) && (mb_x == 0 || (quiet(L) && mveq(L, Q))) && (mb_y == 0 || (quiet(T) && mveq(T, Q)))) return;
 
@@ -128,12 +128,12 @@ __device__ void deblock_mb(const RghJob &J, int mb_x, int mb_y, DbLds &S) {
 #pragma unroll
     for (int k = 0; k < 2; k++) {
         const int i = lane + 64 * k;
-        ly[k] = i < 100 ? *(const uint32_t *)&J.cur_y[(long)(y0 - 4 + i / 5) * J.sy + x0 - 4 + (i % 5) * 4] : 0u;
+        ly[k] = i < 100 ? *(const RGH_G uint32_t *)&J.cur_y[(long)(y0 - 4 + i / 5) * J.sy + x0 - 4 + (i % 5) * 4] : 0u;
     }
     {
         const int p = lane / 30, i = lane % 30;
-        const uint8_t *pl = p ? J.cur_v : J.cur_u;
-        lc = lane < 60 ? *(const uint32_t *)&pl[(long)(cy0 - 2 + i / 3) * J.sc + cx0 - 4 + (i % 3) * 4] : 0u;
+        const RGH_G uint8_t *pl = p ? J.cur_v : J.cur_u;
+        lc = lane < 60 ? *(const RGH_G uint32_t *)&pl[(long)(cy0 - 2 + i / 3) * J.sc + cx0 - 4 + (i % 3) * 4] : 0u;
     }
 #pragma unroll
     for (int k = 0; k < 2; k++) {
@@ -150,38 +150,41 @@ __device__ void deblock_mb(const RghJob &J, int mb_x, int mb_y, DbLds &S) {
     const int qp_q = Q.qp, qpc_q = c_chroma_qp[iclamp(qp_q, 0, 51)];
     for (int dir = 0; dir < 2; dir++) {
         for (int e = 0; e < 4; e++) {
-            const rgh_mb_decision *P;
-            if (e == 0) {
-                if (dir == 0 && mb_x > 0)
-                    P = &L;
-                else if (dir == 1 && mb_y > 0)
-                    P = &T;
-                else
-                    continue;
-            } else {
-                if (!internal || ((e & 1) && skip_odd)) continue;
-                P = &Q;
+            // The macroblock on the p side of the edge: the left or top neighbour for edge 0, else
+            // this one (selected field by field: a pointer to a local record would live in
+            // scratch memory).
+            const bool ext = e == 0;
+            if (ext) {
+                if (!(dir == 0 ? mb_x > 0 : mb_y > 0)) continue;
+            } else if (!internal || ((e & 1) && skip_odd)) {
+                continue;
             }
+            const rgh_mb_decision &N = dir == 0 ? L : T;
+            const int p_intra = ext ? is_intra(N) : intra;
+            const uint32_t p_nnz = ext ? N.nnz : Q.nnz;
+            const int p_mvx = ext ? N.mv[0] : Q.mv[0], p_mvy = ext ? N.mv[1] : Q.mv[1];
+            const int qp_p = ext ? N.qp : Q.qp;
             int bs[4];
-            if (intra || is_intra(*P)) {
-                for (int k = 0; k < 4; k++) bs[k] = e == 0 ? 4 : 3;
+            if (intra || p_intra) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) bs[k] = ext ? 4 : 3;
             } else {
                 uint32_t coded;
-                if (dir == 0 && e == 0)
-                    coded = nz_col(Q.nnz, 0) | nz_col(P->nnz, 3);
+                if (dir == 0 && ext)
+                    coded = nz_col(Q.nnz, 0) | nz_col(p_nnz, 3);
                 else if (dir == 0)
                     coded = nz_col(Q.nnz, e) | nz_col(Q.nnz, e - 1);
-                else if (e == 0)
-                    coded = (Q.nnz & 0xF) | ((uint32_t)P->nnz >> 12);
+                else if (ext)
+                    coded = (Q.nnz & 0xF) | (p_nnz >> 12);
                 else
                     coded = ((Q.nnz >> (4 * e)) & 0xF) | ((Q.nnz >> (4 * (e - 1))) & 0xF);
-                int motion = e == 0 && (iabs(Q.mv[0] - P->mv[0]) >= 4 || iabs(Q.mv[1] - P->mv[1]) >= 4);
+                int motion = ext && (iabs(Q.mv[0] - p_mvx) >= 4 || iabs(Q.mv[1] - p_mvy) >= 4);
                 if (!coded && !motion) continue;
+#pragma unroll
                 for (int k = 0; k < 4; k++) bs[k] = ((coded >> k) & 1) ? 2 : motion;
             }
             int qp = qp_q, qpc = qpc_q;
-            if (P != &Q) {
-                int qp_p = P->qp;
+            if (ext) {
                 qp = (qp_q + qp_p + 1) >> 1;
                 qpc = (qpc_q + c_chroma_qp[iclamp(qp_p, 0, 51)] + 1) >> 1;
             }
@@ -211,20 +214,20 @@ __device__ void deblock_mb(const RghJob &J, int mb_x, int mb_y, DbLds &S) {
     for (int k = 0; k < 2; k++) {
         const int i = lane + 64 * k, r = i / 5, c = i % 5;
         if (i < 100 && !(r < 4 && c == 0))
-            *(uint32_t *)&J.cur_y[(long)(y0 - 4 + r) * J.sy + x0 - 4 + c * 4] = *(const uint32_t *)&S.y[r * 20 + c * 4];
+            *(RGH_G uint32_t *)&J.cur_y[(long)(y0 - 4 + r) * J.sy + x0 - 4 + c * 4] = *(const uint32_t *)&S.y[r * 20 + c * 4];
     }
     if (lane < 60) {
         const int p = lane / 30, i = lane % 30, r = i / 3, c = i % 3;
-        uint8_t *pl = p ? J.cur_v : J.cur_u;
+        RGH_G uint8_t *pl = p ? J.cur_v : J.cur_u;
         if (!(r < 2 && c == 0))
-            *(uint32_t *)&pl[(long)(cy0 - 2 + r) * J.sc + cx0 - 4 + c * 4] = *(const uint32_t *)&(p ? S.v : S.u)[r * 12 + c * 4];
+            *(RGH_G uint32_t *)&pl[(long)(cy0 - 2 + r) * J.sc + cx0 - 4 + c * 4] = *(const uint32_t *)&(p ? S.v : S.u)[r * 12 + c * 4];
     }
 }
 
 // enc.rs:164 extend_borders, restricted to the padding samples whose nearest interior sample
 // lies in border macroblock (X, Y) of the current picture, all three planes. Rows and columns
 // are dword aligned (strides are multiples of 32, paddings of 16).
-__device__ void pad_mb(const RghJob &J, int X, int Y) {
+__device__ void pad_mb(const RghJobD &J, int X, int Y) {
     const int mb_w = J.mb_w, mb_h = J.mb_h;
     if (X != 0 && X != mb_w - 1 && Y != 0 && Y != mb_h - 1) return;
     const int lane = threadIdx.x;
@@ -232,7 +235,7 @@ __device__ void pad_mb(const RghJob &J, int X, int Y) {
     for (int p = 0; p < 3; p++) {
         const int B = p ? 8 : 16, pad = p ? RGH_PAD_C : RGH_PAD_Y, s = p ? J.sc : J.sy;
         const int w = mb_w * B, h = mb_h * B;
-        uint8_t *o = p == 0 ? J.cur_y : (p == 1 ? J.cur_u : J.cur_v);
+        RGH_G uint8_t *o = p == 0 ? J.cur_y : (p == 1 ? J.cur_u : J.cur_v);
         const int r0 = Y == 0 ? -pad : Y * B, r1 = Y == mb_h - 1 ? h + pad : (Y + 1) * B;
         const int c0 = X == 0 ? -pad : X * B, c1 = X == mb_w - 1 ? s - pad : (X + 1) * B;
         const int nd = (c1 - c0) >> 2, n = nd * (r1 - r0);
@@ -240,9 +243,9 @@ __device__ void pad_mb(const RghJob &J, int X, int Y) {
             const int r = r0 + i / nd, c = c0 + 4 * (i % nd);
             const bool cin = c >= 0 && c < w;
             if (cin && r >= 0 && r < h) continue;
-            const uint8_t *srow = o + (long)iclamp(r, 0, h - 1) * s;
-            const uint32_t v = cin ? *(const uint32_t *)(srow + c) : 0x01010101u * srow[c < 0 ? 0 : w - 1];
-            *(uint32_t *)(o + (long)r * s + c) = v;
+            const RGH_G uint8_t *srow = o + (long)iclamp(r, 0, h - 1) * s;
+            const uint32_t v = cin ? *(const RGH_G uint32_t *)(srow + c) : 0x01010101u * srow[c < 0 ? 0 : w - 1];
+            *(RGH_G uint32_t *)(o + (long)r * s + c) = v;
         }
     }
 }
@@ -252,14 +255,14 @@ __device__ void pad_mb(const RghJob &J, int X, int Y) {
 // Edge replication of the padding of job blockIdx.z's current picture (enc.rs:164
 // extend_borders): plane blockIdx.y (0 Y, 1 U, 2 V), padded row blockIdx.x.
 extern "C" __global__ __launch_bounds__(256) void rgh_k_extend(const RghJob *__restrict__ jobs) {
-    const RghJob J = jobs[blockIdx.z];
+    const RghJobD J = device_view(jobs[blockIdx.z]);
     const int plane = blockIdx.y;
     const int pad = plane ? RGH_PAD_C : RGH_PAD_Y;
     const int w = plane ? J.mb_w * 8 : J.mb_w * 16, h = plane ? J.mb_h * 8 : J.mb_h * 16;
     const int s = plane ? J.sc : J.sy;
     const int prow = blockIdx.x;
     if (prow >= h + 2 * pad) return;
-    uint8_t *base = plane == 0 ? J.cur_y : (plane == 1 ? J.cur_u : J.cur_v);  // sample (0, 0)
+    RGH_G uint8_t *base = plane == 0 ? J.cur_y : (plane == 1 ? J.cur_u : J.cur_v);  // sample (0, 0)
     const int y = prow - pad;
     const int sy = iclamp(y, 0, h - 1);
     const bool interior_row = y >= 0 && y < h;

@@ -55,6 +55,86 @@ struct RghJob {
     int32_t pad_;
 };
 
+// The device-side view of a job: the same fields with its buffers typed as global memory, so
+// that every access compiles to global (not flat) loads and stores — flat accesses also wait
+// on LDS and scalar traffic.
+#define RGH_G __attribute__((address_space(1)))
+struct RghJobD {
+    const RGH_G uint8_t *src_y, *src_u, *src_v;
+    int32_t src_sy, src_su, src_sv, width, height;
+    const RGH_G uint8_t *ref_y, *ref_u, *ref_v;
+    RGH_G uint8_t *cur_y, *cur_u, *cur_v;
+    RGH_G uint8_t *pre_y, *pre_u, *pre_v;
+    int32_t sy, sc;
+    int32_t mb_w, mb_h, qp, me_iters, qpel_iters, t8x8;
+    RGH_G rgh_mb_decision *dec;
+    const RGH_G rgh_mb_decision *prev;
+    RGH_G int16_t *levels;
+    uint32_t sb_luma, sb_luma_ssd, sb_chroma, sb_chroma_ssd, sb_probe;
+    int32_t steps, deblock, base;
+    uint32_t serial, ref_serial;
+    RGH_G uint64_t *ana;
+    RGH_G uint64_t *fin;
+    const RGH_G uint64_t *ref_fin;
+    RGH_G RghFrameCtl *ctl;
+    const RGH_G int32_t *wait_analysed;
+    int32_t wait_total;
+};
+
+template <typename T>
+__device__ __forceinline__ RGH_G T *gp(T *p) {
+    return (RGH_G T *)p;
+}
+
+__device__ __forceinline__ RghJobD device_view(const RghJob &j) {
+    RghJobD d;
+    d.src_y = gp(j.src_y);
+    d.src_u = gp(j.src_u);
+    d.src_v = gp(j.src_v);
+    d.src_sy = j.src_sy;
+    d.src_su = j.src_su;
+    d.src_sv = j.src_sv;
+    d.width = j.width;
+    d.height = j.height;
+    d.ref_y = gp(j.ref_y);
+    d.ref_u = gp(j.ref_u);
+    d.ref_v = gp(j.ref_v);
+    d.cur_y = gp(j.cur_y);
+    d.cur_u = gp(j.cur_u);
+    d.cur_v = gp(j.cur_v);
+    d.pre_y = gp(j.pre_y);
+    d.pre_u = gp(j.pre_u);
+    d.pre_v = gp(j.pre_v);
+    d.sy = j.sy;
+    d.sc = j.sc;
+    d.mb_w = j.mb_w;
+    d.mb_h = j.mb_h;
+    d.qp = j.qp;
+    d.me_iters = j.me_iters;
+    d.qpel_iters = j.qpel_iters;
+    d.t8x8 = j.t8x8;
+    d.dec = gp(j.dec);
+    d.prev = gp(j.prev);
+    d.levels = gp(j.levels);
+    d.sb_luma = j.sb_luma;
+    d.sb_luma_ssd = j.sb_luma_ssd;
+    d.sb_chroma = j.sb_chroma;
+    d.sb_chroma_ssd = j.sb_chroma_ssd;
+    d.sb_probe = j.sb_probe;
+    d.steps = j.steps;
+    d.deblock = j.deblock;
+    d.base = j.base;
+    d.serial = j.serial;
+    d.ref_serial = j.ref_serial;
+    d.ana = gp(j.ana);
+    d.fin = gp(j.fin);
+    d.ref_fin = gp(j.ref_fin);
+    d.ctl = gp(j.ctl);
+    d.wait_analysed = gp(j.wait_analysed);
+    d.wait_total = j.wait_total;
+    return d;
+}
+
 // Sum of a 32-bit value over the 64 lanes of the wave, broadcast as a scalar. DPP quad and row
 // mirrors reduce each row of 16 lanes; the four row sums are read out with readlane. Every lane
 // of the wave must be active.
@@ -84,12 +164,28 @@ __device__ __forceinline__ void wave_sum4(const uint32_t *in, uint32_t *out) {
                  (uint32_t)__builtin_amdgcn_readlane((int)v[k], 32) + (uint32_t)__builtin_amdgcn_readlane((int)v[k], 48);
 }
 
-// Four bytes at any address, from the two aligned dwords that hold them. The second dword must
-// lie inside the allocation (true for every padded-plane read of the kernels).
-__device__ __forceinline__ uint32_t ld4u(const uint8_t *p) {
-    uintptr_t a = (uintptr_t)p;
-    const uint32_t *q = (const uint32_t *)(a & ~(uintptr_t)3);
-    return __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(a & 3));
+// Four bytes at any address of global memory, from the two aligned dwords that hold them. The
+// second dword must lie inside the allocation (true for every padded-plane read of the kernels).
+__device__ __forceinline__ uint32_t ld4u(const RGH_G uint8_t *p) {
+    const uint32_t s = (uint32_t)(uintptr_t)p & 3;
+    const RGH_G uint32_t *q = (const RGH_G uint32_t *)(p - s);
+    return __builtin_amdgcn_alignbyte(q[1], q[0], s);
+}
+
+// A decision record from / to global memory, as four 64-bit words.
+__device__ __forceinline__ rgh_mb_decision load_dec(const RGH_G rgh_mb_decision *p) {
+    const RGH_G uint64_t *q = (const RGH_G uint64_t *)p;
+    const uint64_t w[4] = {q[0], q[1], q[2], q[3]};
+    rgh_mb_decision d;
+    __builtin_memcpy(&d, w, sizeof(d));
+    return d;
+}
+__device__ __forceinline__ void store_dec(RGH_G rgh_mb_decision *p, const rgh_mb_decision &d) {
+    uint64_t w[4];
+    __builtin_memcpy(w, &d, sizeof(d));
+    RGH_G uint64_t *q = (RGH_G uint64_t *)p;
+#pragma unroll
+    for (int i = 0; i < 4; i++) q[i] = w[i];
 }
 
 __device__ __forceinline__ int clip255(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
