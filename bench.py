@@ -12,8 +12,9 @@ are resident in HBM before the timed region; decisions and levels stay in HBM.
 The IDR frames are not coded by this path: each GOP's first reference is its IDR source frame,
 installed with rgh_set_reference before each step (outside the timed region).
 
-The frame lag of the schedule (wavefront steps between consecutive frames of a stream) is
-picked by a small autotuner during warm-up unless --lag is given.
+The schedule's frame lag (wavefront steps between consecutive frames of a stream) and its number
+of concurrent groups (schedules on separate HIP streams) are picked by a small autotuner during
+warm-up unless --lag / --groups are given.
 
 Prints one JSON line (see the keys below). --gpus above 1 is refused: this build measures one
 GPU.
@@ -127,6 +128,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--lag", type=int, default=0, help="frame lag in wavefront steps (0: autotune)")
+    ap.add_argument("--groups", type=int, default=0, help="concurrent schedules (0: autotune)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -166,28 +168,30 @@ def main():
     lvls = [[hip.alloc(mbs_per_frame * LEVELS_PER_MB * 2) for _ in range(GOP - 1)] for _ in streams]
     hip.synchronize()
 
-    def step(lag):
+    def step(sched):
+        lag, groups = sched
         for a, ref in zip(ans, idr):
             a.set_reference(*ref)
         t = time.perf_counter()
-        analyse_frames(ans, srcs, decs, lvls, lag=lag)
+        analyse_frames(ans, srcs, decs, lvls, lag=lag, groups=groups)
         el = time.perf_counter() - t
         ms, _, launches = ans[0].timing()
         return el, ms, launches
 
-    # Warm-up, with the frame lag autotuned unless given.
-    lags = [args.lag] if args.lag else [12, 16, 20, 24, 28, 36]
+    # Warm-up, with the schedule (frame lag, concurrent groups) autotuned unless given.
+    lags = [args.lag] if args.lag else [12, 16, 20, 24]
+    groups = [args.groups] if args.groups else [1, 2, 3, 4, 6]
     tune = {}
     for lag in lags:
-        tune[lag] = min(step(lag)[0] for _ in range(max(1, args.warmup if args.lag else 1)))
-    lag = min(tune, key=tune.get)
-    if not args.lag:
-        for _ in range(args.warmup):
-            step(lag)
+        for g in groups:
+            tune[(lag, g)] = step((lag, g))[0]
+    sched = min(tune, key=tune.get)
+    for _ in range(args.warmup):
+        step(sched)
     el = dev_ms = 0.0
     launches = 0
     for _ in range(args.steps):
-        e, ms, nl = step(lag)
+        e, ms, nl = step(sched)
         el += e
         dev_ms += ms
         launches += nl
@@ -214,8 +218,8 @@ def main():
                                "keyint 30, 12 GOP streams x 29 P frames per step in one pipelined schedule, "
                                "bit-exact with the CPU oracle",
                    "form": "plain", "streams": len(streams), "p_frames_per_stream": GOP - 1,
-                   "mbs_per_frame": mbs_per_frame, "me_range": 16, "lag": lag,
-                   "lag_tuning_s": {str(k): round(v, 4) for k, v in tune.items()},
+                   "mbs_per_frame": mbs_per_frame, "me_range": 16, "lag": sched[0], "groups": sched[1],
+                   "schedule_tuning_s": {"%d/%d" % k: round(v, 4) for k, v in tune.items()},
                    "device_ms_per_step": round(dev_ms / args.steps, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("rgh_k_frame_step"),

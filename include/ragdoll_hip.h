@@ -168,12 +168,16 @@ rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src_d
  * of a stream are in flight at once; the results are identical to analysing the frames one by
  * one. src_device[c * n_frames + k] is frame k of context c; decisions[c * n_frames + k]
  * receives its mb_w*mb_h records and levels[c * n_frames + k] its mb_w*mb_h*RGH_LEVELS_PER_MB
- * levels (device pointers, kept until the call returns). The last frame's reconstruction becomes
+ * levels (device pointers, kept until the call returns). groups > 1 splits the contexts into that
+ * many schedules (context i in group i % groups) whose launches run concurrently on their own
+ * streams, so one schedule's slowest macroblocks do not leave the device idle (0 selects 1).
+ * The last frame's reconstruction becomes
  * each context's reference. Returns after the work completes. No counterpart in the reference,
  * which encodes a GOP's frames one after another on one thread
  * (crates/ragdoll-pipeline/src/lib.rs:575-590); x264's frame threads are the analogue. */
 rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src_device,
-                              rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag);
+                              rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag,
+                              int32_t groups);
 
 /* Copies the last analysed frame's results to host memory (any pointer may be NULL). After
  * rgh_analyse_frames the levels are read from that call's last levels buffer, which must still

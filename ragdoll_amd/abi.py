@@ -72,7 +72,7 @@ def load_library(path=LIB_PATH):
     L.rgh_analyse_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), I32, ctypes.POINTER(RgI420)]
     L.rgh_analyse_frames.restype = I32
     L.rgh_analyse_frames.argtypes = [ctypes.POINTER(ctypes.c_void_p), I32, I32, ctypes.POINTER(RgI420),
-                                     ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), I32]
+                                     ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), I32, I32]
     L.rgh_get_outputs.restype = I32
     L.rgh_get_outputs.argtypes = [P, P, P, ctypes.POINTER(RgI420Out), I32]
     L.rgh_get_mb_dims.restype = I32
@@ -196,7 +196,7 @@ def analyse_batch(analyzers, device_frames):
     _check(L.rgh_analyse_batch(arr, n, srcs), "rgh_analyse_batch")
 
 
-def analyse_frames(analyzers, device_frames, decisions, levels, lag=0):
+def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0):
     """n_frames consecutive P frames of every analyzer in one pipelined schedule.
 
     device_frames[i][k]: (y_ptr, u_ptr, v_ptr, stride_y, stride_u, stride_v) of frame k of
@@ -216,4 +216,4 @@ def analyse_frames(analyzers, device_frames, decisions, levels, lag=0):
             srcs[i * nf + k] = RgI420(f[0], f[1], f[2], f[3], f[4], f[5], a.width, a.height)
             dp[i * nf + k] = decisions[i][k]
             lp[i * nf + k] = levels[i][k]
-    _check(L.rgh_analyse_frames(arr, n, nf, srcs, dp, lp, lag), "rgh_analyse_frames")
+    _check(L.rgh_analyse_frames(arr, n, nf, srcs, dp, lp, lag, groups), "rgh_analyse_frames")

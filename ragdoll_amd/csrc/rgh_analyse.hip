@@ -12,6 +12,8 @@
 // the quarter-pel candidates; the chroma prediction. Reductions are DPP + readlane; transform
 // and quantization run one 4x4 block per lane in registers, 8x8 blocks one row or column per
 // lane through LDS.
+#include <stddef.h>
+
 #include "rgh_deblock.h"
 #include "rgh_device.h"
 #include "rgh_tables.h"
@@ -361,6 +363,64 @@ __device__ uint32_t window_pred4(int x0, int y0, int bx, int by, int ox, int oy,
     return (a | b) - (((a ^ b) >> 1) & 0x7F7F7F7Fu);
 }
 
+// The quarter-pel phase table of window_pred4 packed one byte per phase (my & 3) * 4 + (mx & 3):
+// bits 0-1 plane a, 2-3 plane b (= a for a single plane), 4 column step of a, 5 column step
+// of b, 6 row step of b; planes 0 full, 1 b (ph), 2 h (pv), 3 j (pc).
+constexpr uint64_t kPhaseLo = 0x290d090821050400ull, kPhaseHi = 0x564746422b0f0e0aull;
+
+// Byte offset in MbLds of sample (column dx, row dy) of plane k for the block at quarter-pel
+// vector (mx, my), given the window anchor (bx, by) and the region origin (ox, oy), and that
+// plane's row stride.
+__device__ __forceinline__ int plane_offset(int k, int x0, int y0, int bx, int by, int ox, int oy, int mx, int my, int dx,
+                                            int dy, int &stride) {
+    const int ix = x0 + (mx >> 2) + dx, iy = y0 + (my >> 2) + dy;
+    if (k == 0) {
+        stride = SR_W;
+        return (int)offsetof(MbLds, sr) + (iy - oy) * SR_W + ix - ox;
+    }
+    stride = WP_COLS;
+    const int po = k == 1 ? (int)offsetof(MbLds, ph) : (k == 2 ? (int)offsetof(MbLds, pv) : (int)offsetof(MbLds, pc));
+    return po + (iy - by + 1) * WP_COLS + ix - bx + 1;
+}
+
+// SADs of the 49 quarter-pel positions within +-3 quarter samples of the full-pel centre
+// (cx, cy) through the window anchored at (bx, by), one position per lane: lane q holds the SAD
+// at (cx + q % 7 - 3, cy + q / 7 - 3) (lanes 49-63 repeat position 48). Each lane walks the
+// 16x16 block through its own plane pair (window_pred4's table), five aligned dwords a row per
+// plane; the source rows are read at one address by all lanes.
+__device__ uint32_t subpel_sads(int x0, int y0, int bx, int by, int ox, int oy, int cx, int cy, MbLds &S) {
+    const int q = min((int)threadIdx.x, 48);
+    const int mx = cx + q % 7 - 3, my = cy + q / 7 - 3;
+    const int ph = (my & 3) * 4 + (mx & 3);
+    const uint32_t e = (uint32_t)(((ph < 8 ? kPhaseLo : kPhaseHi) >> (8 * (ph & 7))) & 0xFF);
+    int sa, sb;
+    const int oa = plane_offset(e & 3, x0, y0, bx, by, ox, oy, mx, my, (e >> 4) & 1, 0, sa);
+    const int ob = plane_offset((e >> 2) & 3, x0, y0, bx, by, ox, oy, mx, my, (e >> 5) & 1, (e >> 6) & 1, sb);
+    const uint8_t *base = (const uint8_t *)&S;
+    const uint32_t *qa = (const uint32_t *)(base + (oa & ~3)), *qb = (const uint32_t *)(base + (ob & ~3));
+    const uint32_t fa = (uint32_t)(oa & 3), fb = (uint32_t)(ob & 3);
+    const int wa = sa >> 2, wb = sb >> 2;
+    const uint32_t *src = (const uint32_t *)S.src;
+    uint32_t sad = 0;
+#pragma unroll 4
+    for (int r = 0; r < 16; r++) {
+        uint32_t da[5], db[5];
+#pragma unroll
+        for (int c = 0; c < 5; c++) {
+            da[c] = qa[r * wa + c];
+            db[c] = qb[r * wb + c];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const uint32_t a = __builtin_amdgcn_alignbyte(da[c + 1], da[c], fa);
+            const uint32_t b = __builtin_amdgcn_alignbyte(db[c + 1], db[c], fb);
+            const uint32_t p = (a | b) - (((a ^ b) >> 1) & 0x7F7F7F7Fu);
+            sad = __builtin_amdgcn_sad_u8(src[r * 4 + c], p, sad);
+        }
+    }
+    return sad;
+}
+
 __device__ __forceinline__ uint32_t ssd4(uint32_t a, uint32_t b) {
     uint32_t s = 0;
 #pragma unroll
@@ -659,41 +719,80 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
         src_u = J.src_u[(long)csy * J.src_su + csx];
         src_v = J.src_v[(long)csy * J.src_sv + csx];
     }
-    // Neighbour motion: left, top, top-right (else top-left) decisions of this frame.
-    int lref = -2, tref = -2, dref = -2, ax = 0, ay = 0, bx_ = 0, by_ = 0, cxv = 0, cyv = 0;
-    if (mb_x > 0) {
-        const rgh_mb_decision d = load_dec(&J.dec[idx - 1]);
-        lref = d.mb_type >= 2 ? -1 : 0;
-        ax = d.mv[0];
-        ay = d.mv[1];
+    // Neighbour motion (left, top, top-right else top-left decisions of this frame, and the
+    // co-located decision of the reference frame) and intra edges (enc.rs:1457 i16_edges, :1476:
+    // the undeblocked current picture) in one round trip: every lane issues one aligned dword
+    // load before any value is used. Lanes 0-3 read decision words 0 (mb_type) and 2 (mv);
+    // lanes 0-15 the left luma column, 16-31 the left chroma columns, 32-35 the top luma row,
+    // 36-39 the top chroma rows, 40-42 the top-left corners. Unavailable neighbours read a valid
+    // stand-in address and are discarded.
+    const bool has_l = mb_x > 0, has_t = mb_y > 0, has_c = has_t && (mb_x + 1 < mb_w || has_l);
+    const int ci = mb_x + 1 < mb_w ? idx - mb_w + 1 : idx - mb_w - 1;
+    uint32_t dw0 = 0, dw2 = 0;
+    if (lane < 4) {
+        const int di = lane == 0 ? (has_l ? idx - 1 : idx) : lane == 1 ? (has_t ? idx - mb_w : idx) : (has_c ? ci : idx);
+        const RGH_G uint32_t *dp = (const RGH_G uint32_t *)(lane == 3 ? &J.prev[idx] : &J.dec[di]);
+        dw0 = dp[0];
+        dw2 = dp[2];
     }
-    if (mb_y > 0) {
-        const rgh_mb_decision d = load_dec(&J.dec[idx - mb_w]);
-        tref = d.mb_type >= 2 ? -1 : 0;
-        bx_ = d.mv[0];
-        by_ = d.mv[1];
-        const int ci = mb_x + 1 < mb_w ? idx - mb_w + 1 : (mb_x > 0 ? idx - mb_w - 1 : -1);
-        if (ci >= 0) {
-            const rgh_mb_decision c = load_dec(&J.dec[ci]);
-            dref = c.mb_type >= 2 ? -1 : 0;
-            cxv = c.mv[0];
-            cyv = c.mv[1];
+    uint32_t ew = 0;
+    int eshift = 0;
+    if (lane < 43) {
+        const RGH_G uint8_t *ea;
+        if (lane < 16) {
+            ea = J.cur_y + (long)(y0 + lane) * J.sy + x0 - 1;
+        } else if (lane < 32) {
+            const RGH_G uint8_t *cp = ((lane >> 3) & 1) ? J.cur_v : J.cur_u;
+            ea = cp + (long)(cy0 + (lane & 7)) * J.sc + cx0 - 1;
+        } else if (lane < 36) {
+            ea = J.cur_y + (long)(y0 - 1) * J.sy + x0 + 4 * (lane - 32);
+        } else if (lane < 40) {
+            const RGH_G uint8_t *cp = ((lane - 36) >> 1) ? J.cur_v : J.cur_u;
+            ea = cp + (long)(cy0 - 1) * J.sc + cx0 + 4 * ((lane - 36) & 1);
+        } else {
+            const RGH_G uint8_t *cp = lane == 40 ? J.cur_y : (lane == 41 ? J.cur_u : J.cur_v);
+            ea = cp + (long)((lane == 40 ? y0 : cy0) - 1) * (lane == 40 ? J.sy : J.sc) + (lane == 40 ? x0 : cx0) - 1;
+        }
+        eshift = 8 * (int)((uintptr_t)ea & 3);
+        ew = *(const RGH_G uint32_t *)(ea - ((uintptr_t)ea & 3));
+    }
+    int lref = -2, tref = -2, dref = -2, ax = 0, ay = 0, bx_ = 0, by_ = 0, cxv = 0, cyv = 0;
+    {
+        const uint32_t l0 = __builtin_amdgcn_readlane(dw0, 0), l2 = __builtin_amdgcn_readlane(dw2, 0);
+        const uint32_t t0 = __builtin_amdgcn_readlane(dw0, 1), t2 = __builtin_amdgcn_readlane(dw2, 1);
+        const uint32_t c0w = __builtin_amdgcn_readlane(dw0, 2), c2 = __builtin_amdgcn_readlane(dw2, 2);
+        if (has_l) {
+            lref = (l0 & 0xFF) >= 2 ? -1 : 0;
+            ax = (int16_t)(l2 & 0xFFFF);
+            ay = (int16_t)(l2 >> 16);
+        }
+        if (has_t) {
+            tref = (t0 & 0xFF) >= 2 ? -1 : 0;
+            bx_ = (int16_t)(t2 & 0xFFFF);
+            by_ = (int16_t)(t2 >> 16);
+        }
+        if (has_c) {
+            dref = (c0w & 0xFF) >= 2 ? -1 : 0;
+            cxv = (int16_t)(c2 & 0xFFFF);
+            cyv = (int16_t)(c2 >> 16);
         }
     }
-    const int colx = J.prev[idx].mv[0], coly = J.prev[idx].mv[1];
-    // Intra edges from the undeblocked current picture (enc.rs:1457 i16_edges, :1476).
-    if (lane < 16) {
-        S.etop[lane] = mb_y > 0 ? J.cur_y[(long)(y0 - 1) * J.sy + x0 + lane] : 0;
-        S.eleft[lane] = mb_x > 0 ? J.cur_y[(long)(y0 + lane) * J.sy + x0 - 1] : 0;
-    } else if (lane < 32) {
-        int p = (lane >> 3) & 1, k = lane & 7;
-        const RGH_G uint8_t *cp = p ? J.cur_v : J.cur_u;
-        S.ctop[p][k] = mb_y > 0 ? cp[(long)(cy0 - 1) * J.sc + cx0 + k] : 0;
-        S.cleft[p][k] = mb_x > 0 ? cp[(long)(cy0 + k) * J.sc + cx0 - 1] : 0;
-    } else if (lane == 32) {
-        S.etl = (mb_x > 0 && mb_y > 0) ? J.cur_y[(long)(y0 - 1) * J.sy + x0 - 1] : 128;
-        S.ctl[0] = (mb_x > 0 && mb_y > 0) ? J.cur_u[(long)(cy0 - 1) * J.sc + cx0 - 1] : 128;
-        S.ctl[1] = (mb_x > 0 && mb_y > 0) ? J.cur_v[(long)(cy0 - 1) * J.sc + cx0 - 1] : 128;
+    const uint32_t col2 = __builtin_amdgcn_readlane(dw2, 3);
+    const int colx = (int16_t)(col2 & 0xFFFF), coly = (int16_t)(col2 >> 16);
+    {
+        const uint32_t eb = (ew >> eshift) & 0xFF;
+        if (lane < 16)
+            S.eleft[lane] = has_l ? (uint8_t)eb : 0;
+        else if (lane < 32)
+            S.cleft[(lane >> 3) & 1][lane & 7] = has_l ? (uint8_t)eb : 0;
+        else if (lane < 36)
+            *(uint32_t *)&S.etop[4 * (lane - 32)] = has_t ? ew : 0u;
+        else if (lane < 40)
+            *(uint32_t *)&S.ctop[(lane - 36) >> 1][4 * ((lane - 36) & 1)] = has_t ? ew : 0u;
+        else if (lane == 40)
+            S.etl = has_l && has_t ? (int)eb : 128;
+        else if (lane < 43)
+            S.ctl[lane - 41] = has_l && has_t ? (int)eb : 128;
     }
     *(uint32_t *)&S.src[lane * 4] = srcw;
     S.srcu[lane] = (uint8_t)src_u;
@@ -725,36 +824,65 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
     auto in_bounds = [&](int mx, int my) { return mx >= bmin0 && mx <= bmax0 && my >= bmin1 && my <= bmax1; };
     auto mv_cost = [&](int mx, int my) { return lambda * (se_bits(mx - mvpx) + se_bits(my - mvpy)); };
 
-    // ---- Readiness of the reference (when it may still be in flight). ----
-    // Every reference read stays within [c - 22 - r, c + 68 + r] x [c - 19 - r, c + 36 + r] of
-    // some clamped integer candidate c (the staged 68x55 search region, the skip-probe window and
-    // the chroma taps around a vector at most 16 + r samples from c), r = 0 for at most 16
-    // diamond iterations, else the iteration count. The zero candidate keeps this macroblock's
-    // co-located decision inside. Macroblock (a, b) of the reference is final, padding
-    // included, once its item (a + 1, b + 1) is stamped.
-    if (J.ref_fin) {
-        const int imn0 = (bmin0 + 3) >> 2, imn1 = (bmin1 + 3) >> 2, imx0 = bmax0 >> 2, imx1 = bmax1 >> 2;
-        const int cvx[5] = {mvpx, 0, ax, bx_, colx}, cvy[5] = {mvpy, 0, ay, by_, coly};
-        int lx = 1 << 30, hx = -(1 << 30), ly = 1 << 30, hy = -(1 << 30);
+    // Integer-search candidates (enc.rs:1185 full_pel, :3081-3095): [mvp, 0, left, top,
+    // co-located], each rounded to full-pel, clamped and deduplicated. Fixed slots keep the
+    // reference's list order; absent neighbours are not live.
+    const int imn0 = (bmin0 + 3) >> 2, imn1 = (bmin1 + 3) >> 2, imx0 = bmax0 >> 2, imx1 = bmax1 >> 2;
+    const int cvx[5] = {mvpx, 0, ax, bx_, colx}, cvy[5] = {mvpy, 0, ay, by_, coly};
+    const int present[5] = {1, 1, mb_x > 0, mb_y > 0, 1};
+    int fxs[5], fys[5], live[5];
 #pragma unroll
-        for (int k = 0; k < 5; k++) {
-            const int fx = iclamp((cvx[k] + 2) >> 2, imn0, imx0), fy = iclamp((cvy[k] + 2) >> 2, imn1, imx1);
-            lx = min(lx, fx);
-            hx = max(hx, fx);
-            ly = min(ly, fy);
-            hy = max(hy, fy);
+    for (int k = 0; k < 5; k++) {
+        fxs[k] = iclamp((cvx[k] + 2) >> 2, imn0, imx0);
+        fys[k] = iclamp((cvy[k] + 2) >> 2, imn1, imx1);
+        live[k] = present[k];
+#pragma unroll
+        for (int j = 0; j < k; j++) live[k] &= !(live[j] && fxs[j] == fxs[k] && fys[j] == fys[k]);
+    }
+
+    // ---- Round trip 2: readiness stamps, candidate rows and the skip probe's samples. ----
+    // Readiness of the reference (when it may still be in flight): every reference read stays
+    // within [c - 22 - r, c + 68 + r] x [c - 19 - r, c + 36 + r] of some candidate c (the
+    // staged 68x55 search region, the skip-probe window and the chroma taps around a vector at
+    // most 16 + r samples from c), r = 0 for at most 16 diamond iterations, else the iteration
+    // count. The zero candidate keeps this macroblock's co-located decision inside. Macroblock
+    // (a, b) of the reference is final, padding included, once its item (a + 1, b + 1) is
+    // stamped. The candidate rows and probe samples are loaded before the verdict (loads only;
+    // nothing is written before it).
+    int wait = 0;
+    if (J.ref_fin) {
+        int lx = fxs[0], hx = fxs[0], ly = fys[0], hy = fys[0];
+#pragma unroll
+        for (int k = 1; k < 5; k++) {
+            lx = min(lx, fxs[k]);
+            hx = max(hx, fxs[k]);
+            ly = min(ly, fys[k]);
+            hy = max(hy, fys[k]);
         }
         const int r = J.me_iters > 16 ? J.me_iters : 0;
         const int ax0 = iclamp((x0 + lx - 22 - r) >> 4, 0, mb_w - 1), ax1 = iclamp((x0 + hx + 68 + r) >> 4, 0, mb_w - 1);
         const int ay0 = iclamp((y0 + ly - 19 - r) >> 4, 0, J.mb_h - 1), ay1 = iclamp((y0 + hy + 36 + r) >> 4, 0, J.mb_h - 1);
         const int nx = ax1 - ax0 + 1, n = nx * (ay1 - ay0 + 1);
-        int wait = 0;
         for (int i = lane; i < n; i += 64) {
             const int a = ax0 + i % nx, b = ay0 + i / nx;
             wait |= !stamp_final(J.ref_fin[(b + 1) * (mb_w + 1) + a + 1], J.ref_serial, seq);
         }
-        if (__ballot(wait)) return false;
     }
+    uint32_t rw[5];
+    {
+        const RGH_G uint8_t *rb = J.ref_y + (long)(y0 + rr) * J.sy + x0 + c0;
+#pragma unroll
+        for (int k = 0; k < 5; k++) rw[k] = live[k] ? ld4u(rb + (long)fys[k] * J.sy + fxs[k]) : 0;
+    }
+    const bool probe = in_bounds(pskx, psky);
+    const int int_y = ((pskx | psky) & 3) == 0;
+    uint32_t predw = 0, pu = 0, pv = 0;
+    ChromaTaps ct{};
+    if (probe) {
+        ct = chroma_taps(J, mb_x, mb_y, pskx, psky);
+        if (int_y) predw = ld4u(J.ref_y + (long)(y0 + (psky >> 2) + rr) * J.sy + x0 + (pskx >> 2) + c0);
+    }
+    if (__ballot(wait)) return false;
 
     rgh_mb_decision D;
     memset(&D, 0, sizeof(D));
@@ -762,16 +890,11 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
     D.mvp[1] = (int16_t)mvpy;
     D.qp = (uint8_t)qp;
     int mb_type = -1, mvx = 0, mvy = 0;
-    uint32_t predw = 0, pu = 0, pv = 0;
     int cbpl = 0, cbpc = 0, use8 = 0, i16_mode = 0, cmode = 0;
 
-    // ---- Round trip 2: skip probe (enc.rs:2906 try_p_skip; Balanced probes every MB). ----
-    if (in_bounds(pskx, psky)) {
-        const int int_y = ((pskx | psky) & 3) == 0;
-        ChromaTaps ct = chroma_taps(J, mb_x, mb_y, pskx, psky);
-        if (int_y) {
-            predw = ld4u(J.ref_y + (long)(y0 + (psky >> 2) + rr) * J.sy + x0 + (pskx >> 2) + c0);
-        } else {
+    // ---- Skip probe (enc.rs:2906 try_p_skip; Balanced probes every macroblock). ----
+    if (probe) {
+        if (!int_y) {
             // Spec luma MC at the skip vector through the window anchored at its full-pel part
             // (the window table equals h264_luma_mc: enc.rs:4030 test).
             const int fbx = x0 + (pskx >> 2), fby = y0 + (psky >> 2);
@@ -829,25 +952,8 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
     }
 
     if (mb_type < 0) {
-        // ---- Round trip 3: integer candidates (enc.rs:1185 full_pel). ----
-        // [mvp, 0, left, top, co-located], each rounded to full-pel, clamped and deduplicated;
-        // the first strictly cheapest wins.
-        const int imn0 = (bmin0 + 3) >> 2, imn1 = (bmin1 + 3) >> 2, imx0 = bmax0 >> 2, imx1 = bmax1 >> 2;
-        // Fixed slots keep the reference's list order; absent neighbours are not live.
-        const int cvx[5] = {mvpx, 0, ax, bx_, colx}, cvy[5] = {mvpy, 0, ay, by_, coly};
-        const int present[5] = {1, 1, mb_x > 0, mb_y > 0, 1};
-        int fxs[5], fys[5], live[5];
-        uint32_t rw[5];
+        // ---- Integer candidates: the first strictly cheapest wins. ----
         const RGH_G uint8_t *rb = J.ref_y + (long)(y0 + rr) * J.sy + x0 + c0;
-#pragma unroll
-        for (int k = 0; k < 5; k++) {
-            fxs[k] = iclamp((cvx[k] + 2) >> 2, imn0, imx0);
-            fys[k] = iclamp((cvy[k] + 2) >> 2, imn1, imx1);
-            live[k] = present[k];
-#pragma unroll
-            for (int j = 0; j < k; j++) live[k] &= !(live[j] && fxs[j] == fxs[k] && fys[j] == fys[k]);
-            rw[k] = live[k] ? ld4u(rb + (long)fys[k] * J.sy + fxs[k]) : 0;
-        }
         int bestx = 0, besty = 0;
         uint32_t best_cost = 0xFFFFFFFFu;
 #pragma unroll
@@ -926,27 +1032,22 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
             const int cx = bestx * 4, cy = besty * 4;
             compute_window(S, fbx - 1 - rox, fby - 1 - roy);
             D.n_hpel++;
+            // Every position the half-pel and quarter-pel diamonds can reach lies within +-3
+            // quarter samples of the centre: all 49 SADs at once, then the diamonds' sequential
+            // order over them (costs, bounds and strict-< ties as enc.rs:1283 sub_pel).
+            const uint32_t sads = subpel_sads(x0, y0, fbx, fby, rox, roy, cx, cy, S);
             int bmx = cx, bmy = cy;
             for (int pass = 0; pass < 2; pass++) {
                 const int step = pass == 0 ? 2 : 1, iters = pass == 0 ? 1 : J.qpel_iters;
                 for (int it = 0; it < iters; it++) {
                     const int fx = bmx, fy = bmy;
                     const int dx[4] = {0, -step, step, 0}, dy[4] = {-step, 0, 0, step};
-                    int ok[4];
-                    uint32_t part[4];
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
                         const int mx = fx + dx[k], my = fy + dy[k];
-                        ok[k] = iabs(mx - cx) <= 3 && iabs(my - cy) <= 3 && in_bounds(mx, my);
-                        part[k] = ok[k] ? __builtin_amdgcn_sad_u8(srcw, window_pred4(x0, y0, fbx, fby, rox, roy, mx, my, S), 0) : 0u;
-                    }
-                    uint32_t sums[4];
-                    wave_sum4(part, sums);
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        if (!ok[k]) continue;
-                        const int mx = fx + dx[k], my = fy + dy[k];
-                        const uint32_t cst = sums[k] + mv_cost(mx, my);
+                        if (iabs(mx - cx) > 3 || iabs(my - cy) > 3 || !in_bounds(mx, my)) continue;
+                        const int qq = (my - cy + 3) * 7 + (mx - cx + 3);
+                        const uint32_t cst = (uint32_t)__builtin_amdgcn_readlane((int)sads, qq) + mv_cost(mx, my);
                         D.n_subpel++;
                         if (cst < me_cost) {
                             me_cost = cst;

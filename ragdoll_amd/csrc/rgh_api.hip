@@ -136,9 +136,6 @@ void free_ctx(Ctx *c) {
 }
 
 int frame_steps(const Ctx *c) { return (c->mb_w - 1) + 2 * (c->mb_h - 1) + 1; }
-// Launch offsets a frame needs: its analysis steps, then the deblock/pad items (their grid
-// reaches 3 steps further) kKDeblockLag behind.
-int frame_span(const Ctx *c) { return frame_steps(c) + 3 + kDeblockLag; }
 
 // Frame k of a call on context c: analysed into slot (ref + 1 + k) % ring, with the frame of
 // slot (ref + k) % ring as its reference.
@@ -196,81 +193,123 @@ RghJob make_job(Ctx *c, int k, const rgh_i420 &src, rgh_mb_decision *dec, const 
 
 rgh_status ensure_jobs(Ctx *c, int n);
 
-// Runs the schedule of `jobs` (frame-major: job k * n + i is frame k of context i, planned to
-// start at launch k * lag) on context 0's stream and waits for it. Launch L runs every frame
-// planned to be active then; a frame that fell behind (deferred macroblocks) keeps running in
-// later launches, and after the planned launches the per-frame delays are read back and the
-// launches continue until every frame has run all of its steps.
-rgh_status run_schedule(Ctx *const *ctxs, int n, int n_frames, std::vector<RghJob> &jobs, int lag) {
-    Ctx *c0 = ctxs[0];
-    const int nj = n * n_frames;
-    std::vector<int> span((size_t)nj);
-    int gx = 1, max_span = 0;
-    for (int j = 0; j < nj; j++) {
-        const Ctx *c = ctxs[j % n];
-        span[(size_t)j] = frame_span(c);
-        max_span = std::max(max_span, span[(size_t)j]);
-        gx = std::max(gx, std::min(c->mb_h, (c->mb_w + 1) / 2) + std::min(c->mb_h + 1, (c->mb_w + 2) / 2));
-    }
-    hipStream_t stream = c0->stream;
-    HIP_TRY(hipMemcpyAsync(c0->jobs_dev, jobs.data(), sizeof(RghJob) * (size_t)nj, hipMemcpyHostToDevice, stream));
-    HIP_TRY(hipMemsetAsync(c0->ctl_dev, 0, sizeof(RghFrameCtl) * (size_t)nj, stream));
-    if (c0->timing) HIP_TRY(hipEventRecord(c0->ev[0], stream));
-    auto launch = [&](int L, int j_lo, int j_hi) -> rgh_status {
-        const uint32_t seq = g_seq.fetch_add(1);
-        hipLaunchKernelGGL(rgh_k_frame_step, dim3((unsigned)gx, (unsigned)(j_hi - j_lo)), dim3(64), 0, stream,
-                           c0->jobs_dev, j_lo, L, seq);
+// The launch schedule of `jobs` (frame-major: job k * n + i is frame k of context i, planned to
+// start at launch k * lag), on the owner context's stream with its job and progress buffers.
+// Launch L runs every frame planned to be active then; a frame that fell behind (deferred
+// macroblocks) keeps running in later launches, and after the planned launches the per-frame
+// delays are read back and the launches continue until every frame has run all of its steps.
+// Several schedules (disjoint contexts) can interleave their launches on their own streams.
+struct Schedule {
+    Ctx *owner = nullptr;
+    int n = 0, n_frames = 0, lag = 1, nj = 0;
+    std::vector<RghJob> jobs;
+    std::vector<int> span, last;
+    int gx = 1, max_span = 0, planned = 0, L = 0, extra_total = 0;
+
+    rgh_status begin() {
+        nj = n * n_frames;
+        span.assign((size_t)nj, 0);
+        last.assign((size_t)nj, -1);
+        for (int j = 0; j < nj; j++) {
+            const int mb_w = jobs[(size_t)j].mb_w, mb_h = jobs[(size_t)j].mb_h;
+            // The analysis steps, then the deblock/pad items (their grid reaches 3 steps
+            // further) kDeblockLag behind.
+            span[(size_t)j] = jobs[(size_t)j].steps + 3 + kDeblockLag;
+            max_span = std::max(max_span, span[(size_t)j]);
+            gx = std::max(gx, std::min(mb_h, (mb_w + 1) / 2) + std::min(mb_h + 1, (mb_w + 2) / 2));
+        }
+        planned = (n_frames - 1) * lag + max_span;
+        HIP_TRY(hipMemcpyAsync(owner->jobs_dev, jobs.data(), sizeof(RghJob) * (size_t)nj, hipMemcpyHostToDevice,
+                               owner->stream));
+        HIP_TRY(hipMemsetAsync(owner->ctl_dev, 0, sizeof(RghFrameCtl) * (size_t)nj, owner->stream));
         return RGH_OK;
-    };
-    // Planned launches: frame k is active over [k * lag, k * lag + span + kMaxDelay).
-    const int planned = (n_frames - 1) * lag + max_span;
-    std::vector<int> last((size_t)nj, -1);  // last launch each frame took part in
-    int L = 0;
-    for (; L < planned; L++) {
-        int k_hi = std::min(n_frames - 1, L / lag);
+    }
+    void launch(int j_lo, int j_hi) {
+        const uint32_t seq = g_seq.fetch_add(1);
+        hipLaunchKernelGGL(rgh_k_frame_step, dim3((unsigned)gx, (unsigned)(j_hi - j_lo)), dim3(64), 0, owner->stream,
+                           owner->jobs_dev, j_lo, L, seq);
+        for (int j = j_lo; j < j_hi; j++) last[(size_t)j] = L;
+        L++;
+    }
+    // The next planned launch (frame k active over [k * lag, k * lag + span + kMaxDelay)).
+    void planned_launch() {
+        const int k_hi = std::min(n_frames - 1, L / lag);
         int k_lo = 0;
         while (k_lo <= k_hi && k_lo * lag + max_span + kMaxDelay <= L) k_lo++;
-        if (k_lo > k_hi) continue;
-        launch(L, k_lo * n, (k_hi + 1) * n);
-        for (int j = k_lo * n; j < (k_hi + 1) * n; j++) last[(size_t)j] = L;
+        if (k_lo > k_hi) {
+            L++;
+            return;
+        }
+        launch(k_lo * n, (k_hi + 1) * n);
     }
-    HIP_TRY(hipGetLastError());
     // Completion: frames that fell behind run on until done. A frame's next step after its last
     // launch P is P + 1 - base - delay[(P + 1) & 1]; rejoining at launch L it needs the delay
     // L - base - next.
-    int extra_total = 0;
-    for (;;) {
-        HIP_TRY(hipMemcpyAsync(c0->ctl_host, c0->ctl_dev, sizeof(RghFrameCtl) * (size_t)nj, hipMemcpyDeviceToHost,
-                               stream));
-        HIP_TRY(hipStreamSynchronize(stream));
-        int lo = nj, hi = -1, need = 0;
-        for (int j = 0; j < nj; j++) {
-            const int base = jobs[(size_t)j].base, P = last[(size_t)j];
-            const int next = P < 0 ? 0 : P + 1 - base - c0->ctl_host[j].delay[(P + 1) & 1];
-            if (next >= span[(size_t)j]) continue;
-            lo = std::min(lo, j);
-            hi = std::max(hi, j);
-            need = std::max(need, span[(size_t)j] - next);
-            c0->ctl_host[j].delay[0] = c0->ctl_host[j].delay[1] = L - base - next;
+    rgh_status complete() {
+        for (;;) {
+            HIP_TRY(hipMemcpyAsync(owner->ctl_host, owner->ctl_dev, sizeof(RghFrameCtl) * (size_t)nj,
+                                   hipMemcpyDeviceToHost, owner->stream));
+            HIP_TRY(hipStreamSynchronize(owner->stream));
+            int lo = nj, hi = -1, need = 0;
+            for (int j = 0; j < nj; j++) {
+                const int base = jobs[(size_t)j].base, P = last[(size_t)j];
+                const int next = P < 0 ? 0 : P + 1 - base - owner->ctl_host[j].delay[(P + 1) & 1];
+                if (next >= span[(size_t)j]) continue;
+                lo = std::min(lo, j);
+                hi = std::max(hi, j);
+                need = std::max(need, span[(size_t)j] - next);
+                owner->ctl_host[j].delay[0] = owner->ctl_host[j].delay[1] = L - base - next;
+            }
+            if (hi < 0) return RGH_OK;
+            extra_total += need;
+            if (extra_total > 64 * max_span + 4096) {  // no progress: a defect
+                set_error("run_schedule", "frames stopped making progress", __LINE__);
+                return RGH_ERR_DEVICE;
+            }
+            HIP_TRY(hipMemcpyAsync(owner->ctl_dev + lo, owner->ctl_host + lo, sizeof(RghFrameCtl) * (size_t)(hi - lo + 1),
+                                   hipMemcpyHostToDevice, owner->stream));
+            for (int e = 0; e < need; e++) launch(lo, hi + 1);
+            HIP_TRY(hipGetLastError());
         }
-        if (hi < 0) break;
-        extra_total += need;
-        if (extra_total > 64 * max_span + 4096) {  // no progress: a defect
-            set_error("run_schedule", "frames stopped making progress", __LINE__);
-            return RGH_ERR_DEVICE;
-        }
-        HIP_TRY(hipMemcpyAsync(c0->ctl_dev + lo, c0->ctl_host + lo, sizeof(RghFrameCtl) * (size_t)(hi - lo + 1),
-                               hipMemcpyHostToDevice, stream));
-        for (int e = 0; e < need; e++, L++) launch(L, lo, hi + 1);
-        HIP_TRY(hipGetLastError());
-        for (int j = lo; j <= hi; j++) last[(size_t)j] = L - 1;
     }
-    if (c0->timing) {
-        HIP_TRY(hipEventRecord(c0->ev[1], stream));
-        c0->timed = true;
-        c0->timed_launches = L;
+};
+
+// Runs schedules concurrently (their launches interleaved across their streams) and waits for
+// all of them. Timing, when enabled on `timer`, spans the whole run.
+rgh_status run_schedules(std::vector<Schedule> &sch, Ctx *timer) {
+    if (timer->timing) HIP_TRY(hipEventRecord(timer->ev[0], timer->stream));
+    for (auto &s : sch) {
+        rgh_status st = s.begin();
+        if (st != RGH_OK) return st;
+    }
+    int most = 0;
+    for (auto &s : sch) most = std::max(most, s.planned);
+    for (int L = 0; L < most; L++)
+        for (auto &s : sch)
+            if (L < s.planned) s.planned_launch();
+    HIP_TRY(hipGetLastError());
+    int launches = 0;
+    for (auto &s : sch) {
+        rgh_status st = s.complete();
+        if (st != RGH_OK) return st;
+        launches = std::max(launches, s.L);
+    }
+    if (timer->timing) {
+        HIP_TRY(hipEventRecord(timer->ev[1], timer->stream));
+        timer->timed = true;
+        timer->timed_launches = launches;
     }
     return RGH_OK;
+}
+
+rgh_status run_schedule(Ctx *const *ctxs, int n, int n_frames, std::vector<RghJob> &jobs, int lag) {
+    std::vector<Schedule> sch(1);
+    sch[0].owner = ctxs[0];
+    sch[0].n = n;
+    sch[0].n_frames = n_frames;
+    sch[0].lag = lag;
+    sch[0].jobs = jobs;
+    return run_schedules(sch, ctxs[0]);
 }
 
 // After a call analysed n_frames frames of c: the last one is the reference.
@@ -517,8 +556,9 @@ rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src) 
 }
 
 rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src,
-                              rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag) {
-    if (!ctxs || n <= 0 || n_frames <= 0 || !src || !decisions || !levels || lag < 0) return RGH_ERR_INVALID_ARGUMENT;
+                              rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag, int32_t groups) {
+    if (!ctxs || n <= 0 || n_frames <= 0 || !src || !decisions || !levels || lag < 0 || groups < 0)
+        return RGH_ERR_INVALID_ARGUMENT;
     if ((int64_t)n * n_frames > (1 << 20)) return RGH_ERR_INVALID_ARGUMENT;
     std::vector<Ctx *> cs((size_t)n);
     for (int i = 0; i < n; i++) {
@@ -537,25 +577,46 @@ rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, co
         cs[(size_t)i] = c;
     }
     if (lag == 0) lag = kDefaultLag;
-    Ctx *c0 = cs[0];
-    HIP_TRY(hipSetDevice(c0->device));
-    const int nj = n * n_frames;
-    rgh_status s = ensure_jobs(c0, nj);
-    if (s != RGH_OK) return s;
-    std::vector<RghJob> jobs((size_t)nj);
-    for (int k = 0; k < n_frames; k++)
-        for (int i = 0; i < n; i++) {
-            Ctx *c = cs[(size_t)i];
-            const size_t q = (size_t)i * n_frames + k;
-            const int R = (int)c->pic.size();
-            // The picture slot of frame k last served as the reference of frame k + 1 - R.
-            const int wk = k + 1 - R;
-            const RghFrameCtl *wait = wk >= 0 ? c0->ctl_dev + (size_t)wk * n + i : nullptr;
-            const rgh_mb_decision *prev = k ? decisions[q - 1] : c->prev;
-            jobs[(size_t)k * n + i] =
-                make_job(c, k, src[q], decisions[q], prev, levels[q], k * lag, c0->ctl_dev + (size_t)k * n + i, wait);
-        }
-    s = run_schedule(cs.data(), n, n_frames, jobs, lag);
+    const int G = std::max(1, std::min(groups ? groups : 1, n));
+    HIP_TRY(hipSetDevice(cs[0]->device));
+    // Context i joins group i % G; each group runs on its first context's stream and buffers.
+    std::vector<Schedule> sch((size_t)G);
+    for (int g = 0; g < G; g++) {
+        Schedule &S = sch[(size_t)g];
+        std::vector<int> mine;
+        for (int i = g; i < n; i += G) mine.push_back(i);
+        const int m = (int)mine.size();
+        S.owner = cs[(size_t)mine[0]];
+        S.n = m;
+        S.n_frames = n_frames;
+        S.lag = lag;
+        rgh_status s = ensure_jobs(S.owner, m * n_frames);
+        if (s != RGH_OK) return s;
+        RghFrameCtl *ctl = S.owner->ctl_dev;
+        S.jobs.resize((size_t)m * n_frames);
+        for (int k = 0; k < n_frames; k++)
+            for (int t = 0; t < m; t++) {
+                const int i = mine[(size_t)t];
+                Ctx *c = cs[(size_t)i];
+                const size_t q = (size_t)i * n_frames + k;
+                const int R = (int)c->pic.size();
+                // The picture slot of frame k last served as the reference of frame k + 1 - R.
+                const int wk = k + 1 - R;
+                const RghFrameCtl *wait = wk >= 0 ? ctl + (size_t)wk * m + t : nullptr;
+                const rgh_mb_decision *prev = k ? decisions[q - 1] : c->prev;
+                S.jobs[(size_t)k * m + t] =
+                    make_job(c, k, src[q], decisions[q], prev, levels[q], k * lag, ctl + (size_t)k * m + t, wait);
+            }
+    }
+    // The other groups' streams start after everything queued on the first one.
+    hipEvent_t go = nullptr;
+    if (G > 1) {
+        HIP_TRY(hipEventCreateWithFlags(&go, hipEventDisableTiming));
+        HIP_TRY(hipEventRecord(go, cs[0]->stream));
+        for (int g = 1; g < G; g++) HIP_TRY(hipStreamWaitEvent(sch[(size_t)g].owner->stream, go, 0));
+    }
+    rgh_status s = run_schedules(sch, cs[0]);
+    if (go) (void)hipEventDestroy(go);
     if (s != RGH_OK) return s;
     // The last frame's decisions carry the co-located vectors into the next call.
     for (int i = 0; i < n; i++) {
@@ -563,12 +624,12 @@ rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, co
         rgh_mb_decision *keep = c->dec[c->dcur];
         HIP_TRY(hipMemcpyAsync(keep, decisions[(size_t)i * n_frames + n_frames - 1],
                                sizeof(rgh_mb_decision) * (size_t)c->mb_w * c->mb_h, hipMemcpyDeviceToDevice,
-                               c0->stream));
+                               cs[0]->stream));
         advance(c, n_frames, keep);
         c->last_levels = levels[(size_t)i * n_frames + n_frames - 1];
         c->dcur = 1 - c->dcur;
     }
-    HIP_TRY(hipStreamSynchronize(c0->stream));
+    HIP_TRY(hipStreamSynchronize(cs[0]->stream));
     return RGH_OK;
 }
 

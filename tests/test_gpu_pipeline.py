@@ -36,7 +36,7 @@ def oracle_run(frames, w, h, qp, profile):
     return idr, out, ora.recon(deblocked=True)
 
 
-def run_pipeline(streams, lag, ring, calls=1, label=""):
+def run_pipeline(streams, lag, ring, calls=1, groups=0, label=""):
     """streams: list of (frames, w, h, qp, profile). Runs frames 1.. of every stream through
     rgh_analyse_frames (split into `calls` consecutive calls) and checks them against the oracle."""
     hip = Hip()
@@ -63,7 +63,8 @@ def run_pipeline(streams, lag, ring, calls=1, label=""):
         cuts = [round(i * nf / calls) for i in range(calls + 1)]
         for c in range(calls):
             sl = slice(cuts[c], cuts[c + 1])
-            analyse_frames(ans, [r[2][sl] for r in refs], [r[3][sl] for r in refs], [r[4][sl] for r in refs], lag=lag)
+            analyse_frames(ans, [r[2][sl] for r in refs], [r[3][sl] for r in refs], [r[4][sl] for r in refs], lag=lag,
+                           groups=groups)
         for s, (a, (want, final, srcs, decs, lvs, n)) in enumerate(zip(ans, refs)):
             for k in range(nf):
                 dec = hip.download(decs[k], np.zeros(n, DECISION_DTYPE))
@@ -92,11 +93,13 @@ def test_cif_pipeline(lag, ring):
     run_pipeline([(frames, w, h, 28, "baseline")], lag, ring, label="cif lag %d ring %d" % (lag, ring))
 
 
-def test_multi_stream_two_calls():
-    # Three streams of different sizes and profiles in one schedule, split over two calls.
-    print("pipeline multi-stream", flush=True)
+@pytest.mark.parametrize("groups", [1, 2, 3])
+def test_multi_stream_two_calls(groups):
+    # Three streams of different sizes and profiles, split over two calls, in one schedule or
+    # in concurrent schedules on separate streams.
+    print("pipeline multi-stream groups %d" % groups, flush=True)
     w, h, cif = scene("cif", 6, 3)
     fx = [fixture_frame(160, 90, i) for i in range(6)]
     w7, h7, f720 = scene("720p", 6, 5)
     run_pipeline([(cif, w, h, 26, "high"), (fx, 160, 90, 20, "baseline"), (f720, w7, h7, 30, "main")],
-                 lag=6, ring=6, calls=2, label="multi")
+                 lag=6, ring=6, calls=2, groups=groups, label="multi groups %d" % groups)
