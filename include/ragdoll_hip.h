@@ -179,6 +179,29 @@ rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, co
                               rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag,
                               int32_t groups);
 
+/* Slice parameters of rgh_write_p_slice_cavlc: the picture's macroblock grid, the slice QP
+ * (equal to the PPS's initial QP and to every macroblock's), frame_num and pic_order_cnt_lsb as
+ * the encoder numbers them inside a GOP (k & 15 and 2k & 63 for the k-th picture after the
+ * IDR; log2_max_frame_num 4, log2_max_pic_order_cnt_lsb 6). */
+typedef struct rgh_slice_config {
+    int32_t mb_w;
+    int32_t mb_h;
+    int32_t qp;
+    int32_t frame_num;
+    int32_t poc_lsb;
+} rgh_slice_config;
+
+/* Entropy-codes one analysed P picture (decisions and levels in host memory, as
+ * rgh_get_outputs returns them) as the Constrained Baseline CAVLC slice the reference writes:
+ * one Annex B NAL unit (start code, nal_ref_idc 3, type 1, emulation prevention) into out.
+ * *out_len receives the size; RGH_ERR_INVALID_ARGUMENT if it exceeds cap, RGH_ERR_UNSUPPORTED
+ * for records CAVLC cannot carry here (8x8 transform, a QP other than the slice's).
+ * Replaces: the P slice of encode_frame_body (enc.rs:2234-2330 header, mb_skip_run and
+ * code_inter / code_i16 syntax :2779-3306, write_cavlc_residuals :976, write_cavlc_chroma
+ * :1009, h264.rs:2215 encode_cavlc_block, NAL escaping :2643-2672). */
+rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const rgh_mb_decision *decisions,
+                                   const int16_t *levels, uint8_t *out, int64_t cap, int64_t *out_len);
+
 /* Copies the last analysed frame's results to host memory (any pointer may be NULL). After
  * rgh_analyse_frames the levels are read from that call's last levels buffer, which must still
  * be allocated.

@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 from .abi import (DECISION_DTYPE, LEVELS_PER_MB, PROFILES, RghError, StreamAnalyzer, load_library,
-                  analyse_batch, analyse_frames)
+                  analyse_batch, analyse_frames, write_p_slice_cavlc)
 
 __all__ = ["DECISION_DTYPE", "LEVELS_PER_MB", "PROFILES", "RghError", "StreamAnalyzer",
-           "load_library", "analyse_batch", "analyse_frames"]
+           "load_library", "analyse_batch", "analyse_frames", "write_p_slice_cavlc"]

@@ -19,7 +19,7 @@ DECISION_DTYPE = np.dtype([
 assert DECISION_DTYPE.itemsize == 32
 
 EXPORTS = ["rgh_create", "rgh_destroy", "rgh_set_reference", "rgh_analyse_p_frame",
-           "rgh_analyse_batch", "rgh_analyse_frames", "rgh_get_outputs", "rgh_get_mb_dims", "rgh_synchronize",
+           "rgh_analyse_batch", "rgh_analyse_frames", "rgh_write_p_slice_cavlc", "rgh_get_outputs", "rgh_get_mb_dims", "rgh_synchronize",
            "rgh_get_stream", "rgh_enable_timing", "rgh_get_timing", "rgh_status_str", "rgh_last_error"]
 
 
@@ -27,6 +27,11 @@ class RgConfig(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("profile", ctypes.c_int32),
                 ("qp", ctypes.c_int32), ("me_range", ctypes.c_int32), ("qpel_iters", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("flags", ctypes.c_int32), ("ring", ctypes.c_int32)]
+
+
+class RgSliceConfig(ctypes.Structure):
+    _fields_ = [("mb_w", ctypes.c_int32), ("mb_h", ctypes.c_int32), ("qp", ctypes.c_int32),
+                ("frame_num", ctypes.c_int32), ("poc_lsb", ctypes.c_int32)]
 
 
 class RgI420(ctypes.Structure):
@@ -73,6 +78,9 @@ def load_library(path=LIB_PATH):
     L.rgh_analyse_frames.restype = I32
     L.rgh_analyse_frames.argtypes = [ctypes.POINTER(ctypes.c_void_p), I32, I32, ctypes.POINTER(RgI420),
                                      ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), I32, I32]
+    L.rgh_write_p_slice_cavlc.restype = I32
+    L.rgh_write_p_slice_cavlc.argtypes = [ctypes.POINTER(RgSliceConfig), P, P, P, ctypes.c_int64,
+                                          ctypes.POINTER(ctypes.c_int64)]
     L.rgh_get_outputs.restype = I32
     L.rgh_get_outputs.argtypes = [P, P, P, ctypes.POINTER(RgI420Out), I32]
     L.rgh_get_mb_dims.restype = I32
@@ -217,3 +225,19 @@ def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0)
             dp[i * nf + k] = decisions[i][k]
             lp[i * nf + k] = levels[i][k]
     _check(L.rgh_analyse_frames(arr, n, nf, srcs, dp, lp, lag, groups), "rgh_analyse_frames")
+
+
+def write_p_slice_cavlc(decisions, levels, mb_w, mb_h, qp, frame_num, poc_lsb):
+    """The Annex B NAL unit (bytes) of one analysed P picture, Constrained Baseline CAVLC."""
+    L = load_library()
+    dec = np.ascontiguousarray(decisions, DECISION_DTYPE)
+    lv = np.ascontiguousarray(levels, np.int16)
+    if dec.shape != (mb_w * mb_h,) or lv.shape != (mb_w * mb_h, LEVELS_PER_MB):
+        raise ValueError("decisions/levels do not match the macroblock grid")
+    cfg = RgSliceConfig(mb_w, mb_h, qp, frame_num, poc_lsb)
+    cap = mb_w * mb_h * 1024 + 64
+    out = np.zeros(cap, np.uint8)
+    n = ctypes.c_int64()
+    _check(L.rgh_write_p_slice_cavlc(ctypes.byref(cfg), _ptr(dec), _ptr(lv), _ptr(out), cap, ctypes.byref(n)),
+           "rgh_write_p_slice_cavlc")
+    return bytes(out[:n.value])

@@ -17,6 +17,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libragdoll_hip.so")
 SOURCES = ["rgh_analyse.hip", "rgh_api.hip"]
+HOST_SOURCES = ["rgh_cavlc.cpp"]
 ARCH = os.environ.get("RGH_OFFLOAD_ARCH", "gfx950")
 # Device instructions no kernel may contain. v_ashr_pk_{u8,i8}_i32: the backend assumes they
 # zero bits 31:16 of their destination, which MI355X leaves unchanged. Packed FP32 arithmetic:
@@ -53,7 +54,7 @@ def check_device_code(verbose=False):
 
 def build_library(verbose=False):
     cmd = [hipcc()] + FLAGS + ["-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
-                               "-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES]
+                               "-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES + HOST_SOURCES]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -1,0 +1,295 @@
+// Entropy coding of analysed P pictures: the CAVLC (Constrained Baseline) slice a ragdoll
+// encoder writes for the decisions and levels the GPU analysis returns. Host code.
+//
+// Restates the syntax writers of crates/ragdoll-codec/src/h264/enc.rs: the slice header of
+// SliceHeader::write (crates/ragdoll-codec/src/h264.rs:1071-1127) as encode_frame_body sets it
+// for a P picture (enc.rs:2234-2330), mb_skip_run and the per-macroblock syntax of
+// begin_coded_mb / code_inter / code_i16 (enc.rs:2779-3306), write_cavlc_residuals and
+// write_cavlc_chroma with their nC prediction (enc.rs:779-1054), the CAVLC block coder
+// (h264.rs:2215 encode_cavlc_block) and the NAL escaping of enc.rs:2643-2672.
+#include <stdint.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../include/ragdoll_hip.h"
+#include "rgh_cavlc_tables.h"
+
+namespace {
+
+using namespace rgh_cavlc;
+
+// MSB-first bit writer.
+class BitWriter {
+  public:
+    void put(uint32_t value, int len) {  // len <= 32
+        for (int i = len - 1; i >= 0; i--) bit((value >> i) & 1u);
+    }
+    void zeros(int n) {
+        for (int i = 0; i < n; i++) bit(0);
+    }
+    void ue(uint32_t v) {
+        const uint64_t x = (uint64_t)v + 1;
+        int len = 0;
+        while ((x >> len) > 1) len++;
+        zeros(len);
+        for (int i = len; i >= 0; i--) bit((uint32_t)(x >> i) & 1u);
+    }
+    void se(int v) { ue(v > 0 ? 2u * (uint32_t)v - 1u : 2u * (uint32_t)(-v)); }
+    void trailing() {  // rbsp_stop_one_bit and alignment
+        bit(1);
+        while (fill_ != 0) bit(0);
+    }
+    const std::vector<uint8_t> &bytes() const { return buf_; }
+
+  private:
+    void bit(uint32_t b) {
+        cur_ = (uint8_t)((cur_ << 1) | b);
+        if (++fill_ == 8) {
+            buf_.push_back(cur_);
+            cur_ = 0;
+            fill_ = 0;
+        }
+    }
+    std::vector<uint8_t> buf_;
+    uint8_t cur_ = 0;
+    int fill_ = 0;
+};
+
+int absi(int v) { return v < 0 ? -v : v; }
+
+// One residual block in CAVLC (9.2): coeff_token, trailing-one signs, levels with the adaptive
+// suffix length, total_zeros, run_before. sc holds maxc coefficients in scan order; nc < 0
+// selects the chroma DC tables.
+void cavlc_block(BitWriter &w, const int16_t *sc, int maxc, int nc) {
+    const bool cdc = nc < 0;
+    const int tbl = nc < 2 ? 0 : (nc < 4 ? 1 : (nc < 8 ? 2 : 3));
+    int levels[16], runs[16], n = 0, total_zeros = 0;
+    int last = maxc - 1;
+    while (last >= 0 && sc[last] == 0) last--;
+    if (last < 0) {
+        if (cdc)
+            w.put(kCdcTokenBits[0], kCdcTokenLen[0]);
+        else
+            w.put(kCoeffTokenBits[tbl][0], kCoeffTokenLen[tbl][0]);
+        return;
+    }
+    // Levels from the highest frequency down, each with the zeros below it up to the next one.
+    int run = 0;
+    for (int i = last; i >= 0; i--) {
+        if (sc[i] == 0) {
+            run++;
+            continue;
+        }
+        if (n > 0) {
+            runs[n - 1] = run;
+            total_zeros += run;
+        }
+        levels[n++] = sc[i];
+        run = 0;
+    }
+    runs[n - 1] = run;
+    total_zeros += run;
+    int t1 = 0;
+    while (t1 < n && t1 < 3 && absi(levels[t1]) == 1) t1++;
+    const int tok = 4 * n + t1;
+    if (cdc)
+        w.put(kCdcTokenBits[tok], kCdcTokenLen[tok]);
+    else
+        w.put(kCoeffTokenBits[tbl][tok], kCoeffTokenLen[tbl][tok]);
+    for (int i = 0; i < t1; i++) w.put(levels[i] < 0 ? 1u : 0u, 1);
+    uint32_t suffix_len = (n > 10 && t1 < 3) ? 1u : 0u;
+    for (int i = t1; i < n; i++) {
+        const int lv = levels[i];
+        int code = lv > 0 ? 2 * lv - 2 : -2 * lv - 1;
+        if (i == t1 && t1 < 3) code -= 2;  // the first level after fewer than 3 trailing ones
+        const uint32_t c = (uint32_t)code;
+        if (c < (14u << suffix_len)) {
+            w.zeros((int)(c >> suffix_len));
+            w.put(1, 1);
+            if (suffix_len) w.put(c & ((1u << suffix_len) - 1u), (int)suffix_len);
+        } else if (suffix_len == 0 && c < 30) {  // level_prefix 14, 4-bit suffix
+            w.zeros(14);
+            w.put(1, 1);
+            w.put(c - 14, 4);
+        } else if (suffix_len > 0 && c < (15u << suffix_len)) {
+            w.zeros(14);
+            w.put(1, 1);
+            w.put(c - (14u << suffix_len), (int)suffix_len);
+        } else {  // level_prefix 15, 12-bit suffix
+            uint32_t esc = c - (15u << suffix_len);
+            if (suffix_len == 0) esc -= 15;
+            w.zeros(15);
+            w.put(1, 1);
+            w.put(esc & 0xFFFu, 12);
+        }
+        if (suffix_len == 0) suffix_len = 1;
+        if ((uint32_t)absi(lv) > (3u << (suffix_len - 1)) && suffix_len < 6) suffix_len++;
+    }
+    if (n < maxc) {
+        if (cdc)
+            w.put(kCdcTzBits[n - 1][total_zeros], kCdcTzLen[n - 1][total_zeros]);
+        else
+            w.put(kTotalZerosBits[n - 1][total_zeros], kTotalZerosLen[n - 1][total_zeros]);
+    }
+    int zeros_left = total_zeros;
+    for (int i = 0; i < n - 1 && zeros_left > 0; i++) {
+        const int t = (zeros_left < 7 ? zeros_left : 7) - 1;
+        w.put(kRunBits[t][runs[i]], kRunLen[t][runs[i]]);
+        zeros_left -= runs[i];
+    }
+}
+
+int count_nz(const int16_t *v, int n) {
+    int c = 0;
+    for (int i = 0; i < n; i++) c += v[i] != 0;
+    return c;
+}
+
+// Non-zero coefficient counts of the picture's 4x4 blocks, for nC (9.2.1): luma on a
+// (4 mb_w) x (4 mb_h) grid, each chroma plane on (2 mb_w) x (2 mb_h); zero until coded.
+struct NnzGrid {
+    int gw, gh;
+    std::vector<uint8_t> v;
+    NnzGrid(int w, int h) : gw(w), gh(h), v((size_t)w * h, 0) {}
+    uint8_t &at(int x, int y) { return v[(size_t)y * gw + x]; }
+    int nc(int x, int y) {
+        const bool l = x > 0, t = y > 0;
+        if (l && t) return (at(x - 1, y) + at(x, y - 1) + 1) >> 1;
+        if (l) return at(x - 1, y);
+        if (t) return at(x, y - 1);
+        return 0;
+    }
+};
+
+int blk_x(int b) { return (b & 1) + ((b >> 1) & 2); }
+int blk_y(int b) { return ((b >> 1) & 1) + ((b >> 2) & 2); }
+
+// Level record offsets (RGH_LEVELS_PER_MB layout).
+constexpr int kLuma = 0, kLumaDc = 256, kChromaDc = 272, kChromaAc = 280;
+
+void write_chroma(BitWriter &w, NnzGrid *cg, int mb_x, int mb_y, int cbpc, const int16_t *lv) {
+    if (cbpc >= 1)
+        for (int p = 0; p < 2; p++) cavlc_block(w, lv + kChromaDc + 4 * p, 4, -1);
+    if (cbpc >= 2)
+        for (int p = 0; p < 2; p++)
+            for (int b = 0; b < 4; b++) {
+                const int x = 2 * mb_x + (b & 1), y = 2 * mb_y + (b >> 1);
+                const int16_t *ac = lv + kChromaAc + (p * 4 + b) * 16 + 1;
+                cavlc_block(w, ac, 15, cg[p].nc(x, y));
+                cg[p].at(x, y) = (uint8_t)count_nz(ac, 15);
+            }
+}
+
+uint32_t inter_cbp_code(int cbp) {
+    for (uint32_t i = 0; i < 48; i++)
+        if (kGolombToInterCbp[i] == cbp) return i;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const rgh_mb_decision *decisions,
+                                              const int16_t *levels, uint8_t *out, int64_t cap, int64_t *out_len) {
+    if (!cfg || !decisions || !levels || !out || !out_len || cap < 0) return RGH_ERR_INVALID_ARGUMENT;
+    const int mb_w = cfg->mb_w, mb_h = cfg->mb_h;
+    if (mb_w <= 0 || mb_h <= 0 || mb_w > 512 || mb_h > 512) return RGH_ERR_INVALID_DIMENSIONS;
+    if (cfg->qp < 0 || cfg->qp > 51 || cfg->frame_num < 0 || cfg->poc_lsb < 0) return RGH_ERR_INVALID_ARGUMENT;
+    BitWriter w;
+    // Slice header (h264.rs:1071 SliceHeader::write): first_mb_in_slice, slice_type P (5),
+    // pic_parameter_set_id, frame_num (4 bits), pic_order_cnt_lsb (6 bits),
+    // num_ref_idx_active_override_flag, ref_pic_list_modification_flag_l0,
+    // adaptive_ref_pic_marking_mode_flag, slice_qp_delta, disable_deblocking_filter_idc and,
+    // when the filter is on, its two offsets.
+    w.ue(0);
+    w.ue(5);
+    w.ue(0);
+    w.put((uint32_t)cfg->frame_num & 15u, 4);
+    w.put((uint32_t)cfg->poc_lsb & 63u, 6);
+    w.put(0, 1);
+    w.put(0, 1);
+    w.put(0, 1);
+    w.se(0);
+    const int disable = cfg->qp == 0 ? 1 : 0;
+    w.ue((uint32_t)disable);
+    if (disable != 1) {
+        w.se(0);
+        w.se(0);
+    }
+    NnzGrid lg(4 * mb_w, 4 * mb_h);
+    NnzGrid cg[2] = {NnzGrid(2 * mb_w, 2 * mb_h), NnzGrid(2 * mb_w, 2 * mb_h)};
+    uint32_t skip_run = 0;
+    for (int mb_y = 0; mb_y < mb_h; mb_y++)
+        for (int mb_x = 0; mb_x < mb_w; mb_x++) {
+            const int idx = mb_y * mb_w + mb_x;
+            const rgh_mb_decision &d = decisions[idx];
+            const int16_t *lv = levels + (size_t)idx * RGH_LEVELS_PER_MB;
+            if (d.mb_type == RGH_MB_P_SKIP) {
+                skip_run++;
+                continue;
+            }
+            if (d.qp != cfg->qp || d.transform_8x8) return RGH_ERR_UNSUPPORTED;
+            const int cbpl = d.cbp_luma, cbpc = d.cbp_chroma;
+            if (cbpc > 2 || (cbpl & ~15)) return RGH_ERR_INVALID_ARGUMENT;
+            w.ue(skip_run);
+            skip_run = 0;
+            if (d.mb_type == RGH_MB_P_L0_16X16) {
+                const int cbp = cbpl | (cbpc << 4);
+                w.ue(0);
+                w.se(d.mv[0] - d.mvp[0]);
+                w.se(d.mv[1] - d.mvp[1]);
+                w.ue(inter_cbp_code(cbp));
+                if (cbp) {
+                    w.se(0);  // mb_qp_delta: fixed QP
+                    for (int i8 = 0; i8 < 4; i8++)
+                        for (int k = 0; k < 4; k++) {
+                            const int b = i8 * 4 + k, x = 4 * mb_x + blk_x(b), y = 4 * mb_y + blk_y(b);
+                            if (!(cbpl & (1 << i8))) {
+                                lg.at(x, y) = 0;
+                                continue;
+                            }
+                            const int16_t *zz = lv + kLuma + 16 * b;
+                            cavlc_block(w, zz, 16, lg.nc(x, y));
+                            lg.at(x, y) = (uint8_t)count_nz(zz, 16);
+                        }
+                    write_chroma(w, cg, mb_x, mb_y, cbpc, lv);
+                }
+            } else if (d.mb_type == RGH_MB_I16X16) {
+                if (d.i16_mode > 3 || d.chroma_mode > 3 || (cbpl != 0 && cbpl != 15)) return RGH_ERR_INVALID_ARGUMENT;
+                w.ue(5u + 1u + d.i16_mode + 4u * (uint32_t)cbpc + (cbpl ? 12u : 0u));
+                w.ue(d.chroma_mode);
+                w.se(0);  // mb_qp_delta: fixed QP
+                cavlc_block(w, lv + kLumaDc, 16, lg.nc(4 * mb_x, 4 * mb_y));
+                if (cbpl)
+                    for (int b = 0; b < 16; b++) {
+                        const int x = 4 * mb_x + blk_x(b), y = 4 * mb_y + blk_y(b);
+                        const int16_t *ac = lv + kLuma + 16 * b + 1;
+                        cavlc_block(w, ac, 15, lg.nc(x, y));
+                        lg.at(x, y) = (uint8_t)count_nz(ac, 15);
+                    }
+                write_chroma(w, cg, mb_x, mb_y, cbpc, lv);
+            } else {
+                return RGH_ERR_UNSUPPORTED;  // I_4x4 / I_PCM are not produced by the P analysis
+            }
+        }
+    if (skip_run) w.ue(skip_run);
+    w.trailing();
+    // NAL unit (enc.rs:2643): start code, nal_ref_idc 3 / type 1, emulation prevention.
+    const std::vector<uint8_t> &rbsp = w.bytes();
+    std::vector<uint8_t> nal;
+    nal.reserve(rbsp.size() + rbsp.size() / 64 + 8);
+    nal.insert(nal.end(), {0, 0, 0, 1, (uint8_t)((3 << 5) | 1)});
+    int zeros = 0;
+    for (uint8_t b : rbsp) {
+        if (zeros == 2 && b <= 3) {
+            nal.push_back(3);
+            zeros = 0;
+        }
+        nal.push_back(b);
+        zeros = b == 0 ? zeros + 1 : 0;
+    }
+    *out_len = (int64_t)nal.size();
+    if ((int64_t)nal.size() > cap) return RGH_ERR_INVALID_ARGUMENT;
+    memcpy(out, nal.data(), nal.size());
+    return RGH_OK;
+}

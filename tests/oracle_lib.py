@@ -76,7 +76,7 @@ class OracleEncoder:
         if not self.h:
             raise ValueError("unsupported dimensions %dx%d" % (width, height))
         self.L.ora_set_me_range(self.h, me_range)
-        self.width, self.height = width, height
+        self.width, self.height, self.qp = width, height, qp
         self.mb_w = self.L.ora_mb_w(self.h)
         self.mb_h = self.L.ora_mb_h(self.h)
 

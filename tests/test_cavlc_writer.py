@@ -1,0 +1,75 @@
+"""The library's CAVLC slice writer (rgh_write_p_slice_cavlc) against the reference's bytes.
+
+Decisions and levels from the oracle are entropy-coded by the product writer; the slice NAL
+units must equal the reference's committed golden clips (qa/corpus/synth/*, extracted into
+tests/golden/*.nals) byte for byte, and the oracle's own slices on other content and QPs.
+Runs on the CPU: the writer is host code of libragdoll_hip.so.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle_lib import OracleEncoder, read_nals, split_annexb
+from ragdoll_amd import write_p_slice_cavlc
+from ragdoll_amd.synth import CONFIGS, SceneGenerator, config_seed, fixture_frame
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+META = json.load(open(os.path.join(HERE, "golden", "golden.json")))
+
+
+def p_slice(ora, k):
+    """The product writer's NAL for the oracle's last encoded picture, the k-th after its IDR."""
+    nal = write_p_slice_cavlc(ora.decisions(), ora.levels(), ora.mb_w, ora.mb_h, ora.qp, k & 15, (2 * k) & 63)
+    units = split_annexb(nal)
+    assert len(units) == 1
+    return units[0]
+
+
+@pytest.mark.parametrize("name", sorted(META))
+def test_writer_reproduces_golden_p_slices(name):
+    m = META[name]
+    gold = read_nals(os.path.join(HERE, "golden", name + ".nals"))
+    slices = gold[m["n_param_sets"]:]
+    checked = 0
+    for g0 in range(0, m["frames"], m["gop"]):
+        ora = OracleEncoder(m["width"], m["height"], m["qp"], fps=(m["fps"], 1), keyint=m["gop"] + 1)
+        for i in range(g0, min(g0 + m["gop"], m["frames"])):
+            ora.encode(*fixture_frame(m["width"], m["height"], i))
+            if i == g0:
+                continue
+            got = p_slice(ora, i - g0)
+            assert got == slices[i], "%s frame %d: slice differs (%d vs %d bytes)" % (name, i, len(got), len(slices[i]))
+            checked += 1
+    assert checked == m["frames"] - (m["frames"] + m["gop"] - 1) // m["gop"]
+
+
+@pytest.mark.parametrize("qp", [1, 10, 20, 28, 36, 51])
+def test_writer_matches_oracle_slices(qp):
+    # Synthetic CIF content with motion, skips, intra and large levels at low QP.
+    kw = dict(CONFIGS["cif"])
+    w, h = kw.pop("width"), kw.pop("height")
+    g = SceneGenerator(w, h, seed=config_seed(0, 100 + qp), **kw)
+    ora = OracleEncoder(w, h, qp, keyint=1 << 20)
+    types = set()
+    for k in range(5):
+        nals = split_annexb(ora.encode(*g.next_frame()))
+        if k == 0:
+            continue
+        assert p_slice(ora, k) == nals[-1], "qp %d frame %d" % (qp, k)
+        types |= set(np.unique(ora.decisions()["mb_type"]).tolist())
+    assert 2 in types or qp >= 36
+
+
+def test_writer_rejects_cabac_records():
+    ora = OracleEncoder(64, 48, 26, profile="high", keyint=1 << 20)
+    ora.encode(*fixture_frame(64, 48, 0))
+    ora.encode(*fixture_frame(64, 48, 1))
+    dec = ora.decisions().copy()
+    dec["transform_8x8"][:] = 0
+    dec["mb_type"][0] = 1
+    dec["transform_8x8"][0] = 1
+    from ragdoll_amd import RghError
+    with pytest.raises(RghError):
+        write_p_slice_cavlc(dec, ora.levels(), ora.mb_w, ora.mb_h, 26, 1, 2)
