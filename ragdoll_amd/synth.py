@@ -204,3 +204,22 @@ CONFIGS = {
 
 def config_seed(config_id, stream_id=0):
     return (0x52414744 ^ (config_id << 32) ^ stream_id) & M64
+
+
+# The batched configuration: 24 streams of uneven size (8 x 640x360, 8 x 1280x720,
+# 8 x 1920x1080), QP drawn uniformly from 22..38 by the stream's seed, search range (diamond
+# iterations) scaled by width; content of the size class's generator settings.
+BATCHED_CLASSES = [
+    dict(width=640, height=360, n_objects=6, max_speed=8, noise_sigma=2),
+    dict(width=1280, height=720, n_objects=12, max_speed=16, noise_sigma=2, pan=(0.75, 0)),
+    dict(width=1920, height=1080, n_objects=20, max_speed=32, noise_sigma=3, scene_cut_pct=10),
+]
+
+
+def batched_stream(stream_id):
+    """(generator kwargs incl. width/height, seed, qp, me_range) of batched stream 0..23."""
+    seed = config_seed(3, stream_id)
+    kw = dict(BATCHED_CLASSES[stream_id // 8])
+    qp = 22 + _Rng(seed).next() % 17
+    me_range = 16 * kw["width"] // 1280
+    return kw, seed, int(qp), me_range

@@ -24,8 +24,8 @@ def scene(name, n, seed_extra=0):
     return w, h, [g.next_frame() for _ in range(n)]
 
 
-def oracle_run(frames, w, h, qp, profile):
-    ora = OracleEncoder(w, h, qp, profile=profile, keyint=1 << 20)
+def oracle_run(frames, w, h, qp, profile, me_range=16):
+    ora = OracleEncoder(w, h, qp, profile=profile, keyint=1 << 20, me_range=me_range)
     out = []
     for i, f in enumerate(frames):
         ora.encode(*f)
@@ -37,14 +37,17 @@ def oracle_run(frames, w, h, qp, profile):
 
 
 def run_pipeline(streams, lag, ring, calls=1, groups=0, label=""):
-    """streams: list of (frames, w, h, qp, profile). Runs frames 1.. of every stream through
-    rgh_analyse_frames (split into `calls` consecutive calls) and checks them against the oracle."""
+    """streams: list of (frames, w, h, qp, profile[, me_range]). Runs frames 1.. of every stream
+    through rgh_analyse_frames (split into `calls` consecutive calls) and checks them against the
+    oracle."""
     hip = Hip()
     ans, refs, bufs = [], [], []
     try:
-        for frames, w, h, qp, profile in streams:
-            idr, want, final = oracle_run(frames, w, h, qp, profile)
-            a = StreamAnalyzer(w, h, qp, profile=profile, keep_undeblocked=False, ring=ring)
+        for st in streams:
+            frames, w, h, qp, profile = st[:5]
+            me_range = st[5] if len(st) > 5 else 16
+            idr, want, final = oracle_run(frames, w, h, qp, profile, me_range)
+            a = StreamAnalyzer(w, h, qp, profile=profile, me_range=me_range, keep_undeblocked=False, ring=ring)
             a.set_reference(*idr)
             n = a.mb_w * a.mb_h
             srcs, decs, lvs = [], [], []
@@ -103,3 +106,19 @@ def test_multi_stream_two_calls(groups):
     w7, h7, f720 = scene("720p", 6, 5)
     run_pipeline([(cif, w, h, 26, "high"), (fx, 160, 90, 20, "baseline"), (f720, w7, h7, 30, "main")],
                  lag=6, ring=6, calls=2, groups=groups, label="multi groups %d" % groups)
+
+
+def test_batched_uneven_streams():
+    # The batched configuration's shape at reduced length: one stream of each size class
+    # (640x360, 1280x720, 1920x1080) with its seeded QP and width-scaled search range, High
+    # decisions, all in one schedule of 2 concurrent groups.
+    from ragdoll_amd.synth import batched_stream
+    print("pipeline batched", flush=True)
+    streams = []
+    for sid in (3, 12, 17):
+        kw, seed, qp, me_range = batched_stream(sid)
+        kw = dict(kw)
+        w, h = kw.pop("width"), kw.pop("height")
+        g = SceneGenerator(w, h, seed=seed, **kw)
+        streams.append(([g.next_frame() for _ in range(4)], w, h, qp, "high", me_range))
+    run_pipeline(streams, lag=12, ring=4, groups=2, label="batched")
