@@ -111,8 +111,11 @@ typedef struct rgh_mb_decision {
     uint8_t transform_8x8; /* transform_size_8x8_flag of a coded P_L0_16x16 */
     uint8_t qp;            /* QP_Y of the macroblock */
     uint8_t flags;         /* bit 0: P_Skip found by the probe before the search */
-    int16_t mv[2];         /* quarter-pel vector (P_Skip: the skip vector; intra: 0) */
-    int16_t mvp[2];        /* motion vector predictor; mvd = mv - mvp */
+    int16_t mv[2];         /* quarter-pel vector (P_Skip: the skip vector; I_16x16: 0) */
+    int16_t mvp[2];        /* motion vector predictor; mvd = mv - mvp. For I_4x4 (IDR
+                              pictures) mv and mvp together hold the 16 intra 4x4 prediction
+                              modes, 4 bits each: block b (decoding order) in bits 4b..4b+3 of
+                              the little-endian 64-bit word at mv[0] */
     uint32_t cost;         /* motion-estimation cost (SAD + lambda * mv bits) of the result */
     uint16_t n_sad;        /* full-pel 16x16 SAD evaluations */
     uint16_t n_subpel;     /* sub-pel SAD evaluations */
@@ -154,6 +157,16 @@ rgh_status rgh_set_reference(void *ctx, const rgh_i420 *recon);
 rgh_status rgh_analyse_p_frame(void *ctx, const rgh_i420 *src, rgh_mb_decision *decisions,
                                int16_t *levels, const rgh_i420_out *recon);
 
+/* Analyses an IDR picture (host source): every macroblock's intra decision (I_16x16 analysis,
+ * then I_4x4 when the 16x16 SATD exceeds 40 lambda and it is cheaper), levels and
+ * reconstruction, then deblocking and border extension; the result becomes the reference and
+ * the co-located vectors of the next P frame are zero. Outputs as rgh_analyse_p_frame.
+ * Replaces: the IDR body of encode_frame_body (enc.rs:2382-2434 with code_intra :3311,
+ * analyse_i4 :3333, code_i4 :3441, code_i16 :3587) and its post-frame reference preparation
+ * (:2470-2503). */
+rgh_status rgh_analyse_idr(void *ctx, const rgh_i420 *src, rgh_mb_decision *decisions,
+                           int16_t *levels, const rgh_i420_out *recon);
+
 /* Batched form: one P frame of each of n contexts, whose sources are already in device
  * memory, analysed by one set of kernel launches on one HIP stream (all contexts must have
  * been created on the same device). Results stay on the device; read them with
@@ -171,13 +184,17 @@ rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src_d
  * levels (device pointers, kept until the call returns). groups > 1 splits the contexts into that
  * many schedules (context i in group i % groups) whose launches run concurrently on their own
  * streams, so one schedule's slowest macroblocks do not leave the device idle (0 selects 1).
+ * With RGH_FRAMES_FIRST_IDR in flags, frame 0 of every context is analysed as an IDR picture
+ * (as rgh_analyse_idr), so a whole GOP runs in one call; otherwise every context needs a
+ * reference.
  * The last frame's reconstruction becomes
  * each context's reference. Returns after the work completes. No counterpart in the reference,
  * which encodes a GOP's frames one after another on one thread
  * (crates/ragdoll-pipeline/src/lib.rs:575-590); x264's frame threads are the analogue. */
+#define RGH_FRAMES_FIRST_IDR 1 /* rgh_analyse_frames: frame 0 of every context is an IDR picture */
 rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src_device,
                               rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag,
-                              int32_t groups);
+                              int32_t groups, int32_t flags);
 
 /* Slice parameters of rgh_write_p_slice_cavlc: the picture's macroblock grid, the slice QP
  * (equal to the PPS's initial QP and to every macroblock's), frame_num and pic_order_cnt_lsb as
@@ -201,6 +218,28 @@ typedef struct rgh_slice_config {
  * :1009, h264.rs:2215 encode_cavlc_block, NAL escaping :2643-2672). */
 rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const rgh_mb_decision *decisions,
                                    const int16_t *levels, uint8_t *out, int64_t cap, int64_t *out_len);
+
+/* The IDR counterpart of rgh_write_p_slice_cavlc (frame_num / poc_lsb are 0 for an IDR and
+ * are ignored): one Annex B NAL unit of type 5 with idr_pic_id (the GOP's index, as
+ * encode_frame_body numbers IDRs: frame_count / keyint). Intra 4x4 modes come from the I_4x4
+ * records' mv words. Replaces: the IDR slice of encode_frame_body (enc.rs:2234-2330 header,
+ * code_i4 :3441 / code_i16 :3587 syntax, the predicted intra 4x4 mode of 8.3.1.1). */
+rgh_status rgh_write_idr_slice_cavlc(const rgh_slice_config *cfg, int32_t idr_pic_id,
+                                     const rgh_mb_decision *decisions, const int16_t *levels,
+                                     uint8_t *out, int64_t cap, int64_t *out_len);
+
+/* Stream parameters of rgh_write_headers. */
+typedef struct rgh_stream_config {
+    int32_t width, height; /* display size */
+    int32_t profile;       /* rgh_profile */
+    int32_t qp;            /* the PPS's initial QP (the fixed QP) */
+    int32_t fps_num, fps_den;
+} rgh_stream_config;
+
+/* The stream's SPS and PPS as two Annex B NAL units (types 7 and 8), as the encoder emits them
+ * before each IDR. Replaces: h264.rs:556 Sps::to_rbsp with the encoder's profile settings and
+ * h264.rs:806 Pps::to_rbsp. */
+rgh_status rgh_write_headers(const rgh_stream_config *cfg, uint8_t *out, int64_t cap, int64_t *out_len);
 
 /* Copies the last analysed frame's results to host memory (any pointer may be NULL). After
  * rgh_analyse_frames the levels are read from that call's last levels buffer, which must still

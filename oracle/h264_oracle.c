@@ -1971,6 +1971,13 @@ static void code_i4(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const mbs
     rec_luma4(lv, i4->q4);
     rec_chroma(lv, &c);
     d->mb_type = 3;
+    {
+        /* The record's mv/mvp words carry the 16 prediction modes, 4 bits each (block b in bits
+         * 4b..4b+3, decoding order); an intra macroblock has no motion. */
+        uint64_t packed = 0;
+        for (int b = 0; b < 16; b++) packed |= (uint64_t)i4->modes[b] << (4 * b);
+        memcpy(d->mv, &packed, sizeof(packed));
+    }
     d->chroma_mode = (uint8_t)cmode;
     d->cbp_luma = (uint8_t)i4->cbp_luma;
     d->cbp_chroma = (uint8_t)cbpc;

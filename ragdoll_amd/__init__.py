@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 from .abi import (DECISION_DTYPE, LEVELS_PER_MB, PROFILES, RghError, StreamAnalyzer, load_library,
-                  analyse_batch, analyse_frames, write_p_slice_cavlc)
+                  analyse_batch, analyse_frames, write_p_slice_cavlc, write_idr_slice_cavlc, write_headers)
 
 __all__ = ["DECISION_DTYPE", "LEVELS_PER_MB", "PROFILES", "RghError", "StreamAnalyzer",
-           "load_library", "analyse_batch", "analyse_frames", "write_p_slice_cavlc"]
+           "load_library", "analyse_batch", "analyse_frames", "write_p_slice_cavlc", "write_idr_slice_cavlc",
+           "write_headers"]

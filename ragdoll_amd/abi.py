@@ -18,8 +18,9 @@ DECISION_DTYPE = np.dtype([
 ])
 assert DECISION_DTYPE.itemsize == 32
 
-EXPORTS = ["rgh_create", "rgh_destroy", "rgh_set_reference", "rgh_analyse_p_frame",
-           "rgh_analyse_batch", "rgh_analyse_frames", "rgh_write_p_slice_cavlc", "rgh_get_outputs", "rgh_get_mb_dims", "rgh_synchronize",
+EXPORTS = ["rgh_create", "rgh_destroy", "rgh_set_reference", "rgh_analyse_p_frame", "rgh_analyse_idr",
+           "rgh_analyse_batch", "rgh_analyse_frames", "rgh_write_p_slice_cavlc",
+           "rgh_write_idr_slice_cavlc", "rgh_write_headers", "rgh_get_outputs", "rgh_get_mb_dims", "rgh_synchronize",
            "rgh_get_stream", "rgh_enable_timing", "rgh_get_timing", "rgh_status_str", "rgh_last_error"]
 
 
@@ -32,6 +33,11 @@ class RgConfig(ctypes.Structure):
 class RgSliceConfig(ctypes.Structure):
     _fields_ = [("mb_w", ctypes.c_int32), ("mb_h", ctypes.c_int32), ("qp", ctypes.c_int32),
                 ("frame_num", ctypes.c_int32), ("poc_lsb", ctypes.c_int32)]
+
+
+class RgStreamConfig(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("profile", ctypes.c_int32),
+                ("qp", ctypes.c_int32), ("fps_num", ctypes.c_int32), ("fps_den", ctypes.c_int32)]
 
 
 class RgI420(ctypes.Structure):
@@ -73,14 +79,21 @@ def load_library(path=LIB_PATH):
     L.rgh_set_reference.argtypes = [P, ctypes.POINTER(RgI420)]
     L.rgh_analyse_p_frame.restype = I32
     L.rgh_analyse_p_frame.argtypes = [P, ctypes.POINTER(RgI420), P, P, ctypes.POINTER(RgI420Out)]
+    L.rgh_analyse_idr.restype = I32
+    L.rgh_analyse_idr.argtypes = [P, ctypes.POINTER(RgI420), P, P, ctypes.POINTER(RgI420Out)]
     L.rgh_analyse_batch.restype = I32
     L.rgh_analyse_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), I32, ctypes.POINTER(RgI420)]
     L.rgh_analyse_frames.restype = I32
     L.rgh_analyse_frames.argtypes = [ctypes.POINTER(ctypes.c_void_p), I32, I32, ctypes.POINTER(RgI420),
-                                     ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), I32, I32]
+                                     ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), I32, I32, I32]
     L.rgh_write_p_slice_cavlc.restype = I32
     L.rgh_write_p_slice_cavlc.argtypes = [ctypes.POINTER(RgSliceConfig), P, P, P, ctypes.c_int64,
                                           ctypes.POINTER(ctypes.c_int64)]
+    L.rgh_write_idr_slice_cavlc.restype = I32
+    L.rgh_write_idr_slice_cavlc.argtypes = [ctypes.POINTER(RgSliceConfig), I32, P, P, P, ctypes.c_int64,
+                                            ctypes.POINTER(ctypes.c_int64)]
+    L.rgh_write_headers.restype = I32
+    L.rgh_write_headers.argtypes = [ctypes.POINTER(RgStreamConfig), P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
     L.rgh_get_outputs.restype = I32
     L.rgh_get_outputs.argtypes = [P, P, P, ctypes.POINTER(RgI420Out), I32]
     L.rgh_get_mb_dims.restype = I32
@@ -146,7 +159,12 @@ class StreamAnalyzer:
             raise ValueError("reference must be at coded size %dx%d" % (W, H))
         _check(self.L.rgh_set_reference(self.h, ctypes.byref(host_i420(y, u, v, W, H))), "rgh_set_reference")
 
-    def analyse(self, y, u, v, want_levels=True, want_recon=True):
+    def analyse_idr(self, y, u, v, want_levels=True, want_recon=True):
+        """Analyses an IDR picture (it becomes the reference); returns (decisions, levels,
+        (y, u, v) deblocked recon)."""
+        return self.analyse(y, u, v, want_levels, want_recon, _entry="rgh_analyse_idr")
+
+    def analyse(self, y, u, v, want_levels=True, want_recon=True, _entry="rgh_analyse_p_frame"):
         """Analyses one P frame; returns (decisions, levels, (y, u, v) deblocked recon)."""
         y, u, v = (np.ascontiguousarray(p, np.uint8) for p in (y, u, v))
         n = self.mb_w * self.mb_h
@@ -159,9 +177,8 @@ class StreamAnalyzer:
             rec = (np.zeros((H, W), np.uint8), np.zeros((H // 2, W // 2), np.uint8),
                    np.zeros((H // 2, W // 2), np.uint8))
             out = RgI420Out(_ptr(rec[0]), _ptr(rec[1]), _ptr(rec[2]), W, W // 2, W // 2)
-        _check(self.L.rgh_analyse_p_frame(self.h, ctypes.byref(host_i420(y, u, v, self.width, self.height)),
-                                          _ptr(dec), _ptr(lv), ctypes.byref(out) if out else None),
-               "rgh_analyse_p_frame")
+        _check(getattr(self.L, _entry)(self.h, ctypes.byref(host_i420(y, u, v, self.width, self.height)),
+                                       _ptr(dec), _ptr(lv), ctypes.byref(out) if out else None), _entry)
         return dec, lv, rec
 
     def outputs(self, which_recon=1, want_levels=True):
@@ -204,7 +221,10 @@ def analyse_batch(analyzers, device_frames):
     _check(L.rgh_analyse_batch(arr, n, srcs), "rgh_analyse_batch")
 
 
-def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0):
+FRAMES_FIRST_IDR = 1
+
+
+def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0, first_idr=False):
     """n_frames consecutive P frames of every analyzer in one pipelined schedule.
 
     device_frames[i][k]: (y_ptr, u_ptr, v_ptr, stride_y, stride_u, stride_v) of frame k of
@@ -224,7 +244,8 @@ def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0)
             srcs[i * nf + k] = RgI420(f[0], f[1], f[2], f[3], f[4], f[5], a.width, a.height)
             dp[i * nf + k] = decisions[i][k]
             lp[i * nf + k] = levels[i][k]
-    _check(L.rgh_analyse_frames(arr, n, nf, srcs, dp, lp, lag, groups), "rgh_analyse_frames")
+    _check(L.rgh_analyse_frames(arr, n, nf, srcs, dp, lp, lag, groups, FRAMES_FIRST_IDR if first_idr else 0),
+           "rgh_analyse_frames")
 
 
 def write_p_slice_cavlc(decisions, levels, mb_w, mb_h, qp, frame_num, poc_lsb):
@@ -240,4 +261,30 @@ def write_p_slice_cavlc(decisions, levels, mb_w, mb_h, qp, frame_num, poc_lsb):
     n = ctypes.c_int64()
     _check(L.rgh_write_p_slice_cavlc(ctypes.byref(cfg), _ptr(dec), _ptr(lv), _ptr(out), cap, ctypes.byref(n)),
            "rgh_write_p_slice_cavlc")
+    return bytes(out[:n.value])
+
+
+def write_idr_slice_cavlc(decisions, levels, mb_w, mb_h, qp, idr_pic_id):
+    """The Annex B NAL unit (bytes) of one analysed IDR picture, Constrained Baseline CAVLC."""
+    L = load_library()
+    dec = np.ascontiguousarray(decisions, DECISION_DTYPE)
+    lv = np.ascontiguousarray(levels, np.int16)
+    if dec.shape != (mb_w * mb_h,) or lv.shape != (mb_w * mb_h, LEVELS_PER_MB):
+        raise ValueError("decisions/levels do not match the macroblock grid")
+    cfg = RgSliceConfig(mb_w, mb_h, qp, 0, 0)
+    cap = mb_w * mb_h * 1024 + 64
+    out = np.zeros(cap, np.uint8)
+    n = ctypes.c_int64()
+    _check(L.rgh_write_idr_slice_cavlc(ctypes.byref(cfg), idr_pic_id, _ptr(dec), _ptr(lv), _ptr(out), cap,
+                                       ctypes.byref(n)), "rgh_write_idr_slice_cavlc")
+    return bytes(out[:n.value])
+
+
+def write_headers(width, height, qp, profile="baseline", fps=(30, 1)):
+    """SPS and PPS NAL units (bytes, Annex B) of a stream."""
+    L = load_library()
+    cfg = RgStreamConfig(width, height, PROFILES[profile], qp, fps[0], fps[1])
+    out = np.zeros(1024, np.uint8)
+    n = ctypes.c_int64()
+    _check(L.rgh_write_headers(ctypes.byref(cfg), _ptr(out), 1024, ctypes.byref(n)), "rgh_write_headers")
     return bytes(out[:n.value])

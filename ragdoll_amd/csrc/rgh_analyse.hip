@@ -48,6 +48,10 @@ struct MbLds {
             alignas(16) uint8_t ph[WP_ROWS * WP_COLS], pv[WP_ROWS * WP_COLS], pc[WP_ROWS * WP_COLS];
         };
         struct {
+            alignas(16) uint8_t cv[17 * 24];   // intra 4x4 reconstruction canvas (IDR), CV_W stride
+            alignas(16) int16_t q4i[16][16];   // intra 4x4 levels per block (decoding order), raster
+        };
+        struct {
             alignas(16) uint8_t tmp[256];
             alignas(16) int16_t coef[16][16];  // luma levels, decoding order, raster positions
             alignas(16) int16_t q8[4][64];     // 8x8 levels, raster positions
@@ -684,6 +688,294 @@ __device__ void recon8(MbLds &S, int cbpl, int qp) {
 }  // namespace
 
 // ---------------------------------------------------------------------------------------
+// Intra 16x16 (enc.rs:1635 analyse_i16_edges, :3587 code_i16), shared by P and IDR pictures.
+// ---------------------------------------------------------------------------------------
+struct I16Analysis {
+    uint32_t cost;
+    int mode, pa, pb, pc, dc;
+};
+
+// V, H and DC SATDs from one source Hadamard, plane SATD by prediction; the intra edges and
+// the source are in S.
+__device__ I16Analysis analyse_i16(MbLds &S, int mb_x, int mb_y, uint16_t &n_satd) {
+    const int lane = threadIdx.x, rr = lane >> 2, c0 = (lane & 3) * 4;
+    const int has_t = mb_y > 0, has_l = mb_x > 0, tl = S.etl;
+    uint32_t st = 0, sl = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        st += S.etop[i];
+        sl += S.eleft[i];
+    }
+    const int dc = (has_t && has_l) ? (int)((st + sl + 16) >> 5)
+                                    : (has_t ? (int)((st + 8) >> 4) : (has_l ? (int)((sl + 8) >> 4) : 128));
+    // simd/h264_enc.rs:628 intra_satd_x3_16x16: V, H, DC SATDs from one source Hadamard
+    // (unavailable edges read as zero, as the reference passes zero rows).
+    uint32_t sv = 0, sh = 0, sd = 0;
+    if (lane < 16) {
+        const int bx = lane & 3, by = lane >> 2;
+        int rows[4][4];
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            const uint32_t w = *(const uint32_t *)&S.src[(by * 4 + y) * 16 + bx * 4];
+            const int x0_ = byte_of(w, 0), x1 = byte_of(w, 1), x2 = byte_of(w, 2), x3 = byte_of(w, 3);
+            const int s01 = x0_ + x1, d01 = x0_ - x1, s23 = x2 + x3, d23 = x2 - x3;
+            rows[y][0] = s01 + s23;
+            rows[y][1] = s01 - s23;
+            rows[y][2] = d01 + d23;
+            rows[y][3] = d01 - d23;
+        }
+        int tv[4], th[4];
+        {
+            const int a0 = S.etop[bx * 4], a1 = S.etop[bx * 4 + 1], a2 = S.etop[bx * 4 + 2], a3 = S.etop[bx * 4 + 3];
+            tv[0] = 4 * (a0 + a1 + a2 + a3);
+            tv[1] = 4 * (a0 + a1 - a2 - a3);
+            tv[2] = 4 * (a0 - a1 + a2 - a3);
+            tv[3] = 4 * (a0 - a1 - a2 + a3);
+            const int l0 = S.eleft[by * 4], l1 = S.eleft[by * 4 + 1], l2 = S.eleft[by * 4 + 2], l3 = S.eleft[by * 4 + 3];
+            th[0] = 4 * (l0 + l1 + l2 + l3);
+            th[1] = 4 * (l0 + l1 - l2 - l3);
+            th[2] = 4 * (l0 - l1 + l2 - l3);
+            th[3] = 4 * (l0 - l1 - l2 + l3);
+        }
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            const int s01 = rows[0][h] + rows[1][h], d01 = rows[0][h] - rows[1][h];
+            const int s23 = rows[2][h] + rows[3][h], d23 = rows[2][h] - rows[3][h];
+            const int o[4] = {(int16_t)(s01 + s23), (int16_t)(s01 - s23), (int16_t)(d01 + d23), (int16_t)(d01 - d23)};
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int c = o[v];
+                sv += (uint32_t)iabs(c - (v == 0 ? tv[h] : 0));
+                sh += (uint32_t)iabs(c - (h == 0 ? th[v] : 0));
+                sd += (uint32_t)iabs(c - ((v == 0 && h == 0) ? 16 * dc : 0));
+            }
+        }
+    }
+    sv = wave_sum(sv) >> 1;
+    sh = wave_sum(sh) >> 1;
+    sd = wave_sum(sd) >> 1;
+    // enc.rs:1637 analyse_i16_edges: DC, then V, then H, then plane, strict improvements.
+    uint32_t ib = sd;
+    int im = 2;
+    if (has_t && sv < ib) { ib = sv; im = 0; }
+    if (has_l && sh < ib) { ib = sh; im = 1; }
+    int pa = 0, pb = 0, pcc = 0;
+    if (has_t && has_l) {
+        int Hh = 0, Vv = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int ptop = i == 7 ? tl : S.etop[6 - i], pl = i == 7 ? tl : S.eleft[6 - i];
+            Hh += (i + 1) * (S.etop[8 + i] - ptop);
+            Vv += (i + 1) * (S.eleft[8 + i] - pl);
+        }
+        pa = 16 * (S.etop[15] + S.eleft[15]);
+        pb = (5 * Hh + 32) >> 6;
+        pcc = (5 * Vv + 32) >> 6;
+        uint32_t w = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) w |= sat_shr<5>(pa + pb * (c0 + i - 7) + pcc * (rr - 7) + 16) << (8 * i);
+        *(uint32_t *)&S.tmp[lane * 4] = w;
+        __syncthreads();
+        const uint32_t ps = satd16(S.src, S.tmp);
+        n_satd++;
+        if (ps < ib) { ib = ps; im = 3; }
+    }
+    return {ib, im, pa, pb, pcc, dc};
+}
+
+struct ChromaIntra {
+    int mode;
+    uint32_t pu, pv;
+};
+
+// enc.rs:1523 analyse_chroma_intra: the chroma mode by SAD + 8 for non-DC modes (predictors
+// h264.rs:1967 and enc.rs:1588) from the chroma edges in S; the lane's predicted U and V samples.
+__device__ ChromaIntra chroma_intra(MbLds &S, int mb_x, int mb_y, uint32_t src_u, uint32_t src_v) {
+    const int lane = threadIdx.x;
+    const int has_t = mb_y > 0, has_l = mb_x > 0;
+    int cmode = 0;
+    const int cr = lane >> 3, cc = lane & 7;
+    uint32_t cbest = 0xFFFFFFFFu, bpu = 0, bpv = 0;
+    for (int mode = 0; mode < 4; mode++) {
+        const int ok = mode == 1 ? has_l : (mode == 2 ? has_t : (mode == 3 ? (has_t && has_l) : 1));
+        if (!ok) continue;
+        uint32_t smp[2];
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const uint8_t *t = S.ctop[p], *l = S.cleft[p];
+            int v;
+            if (mode == 0) {
+                uint32_t t0 = 0, t1 = 0, l0 = 0, l1 = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    t0 += t[i];
+                    t1 += t[4 + i];
+                    l0 += l[i];
+                    l1 += l[4 + i];
+                }
+                int d00, d01, d10, d11;
+                if (has_t && has_l) {
+                    d00 = (int)((t0 + l0 + 4) >> 3);
+                    d01 = (int)((t1 + 2) >> 2);
+                    d10 = (int)((l1 + 2) >> 2);
+                    d11 = (int)((t1 + l1 + 4) >> 3);
+                } else if (has_t) {
+                    d00 = d10 = (int)((t0 + 2) >> 2);
+                    d01 = d11 = (int)((t1 + 2) >> 2);
+                } else if (has_l) {
+                    d00 = d01 = (int)((l0 + 2) >> 2);
+                    d10 = d11 = (int)((l1 + 2) >> 2);
+                } else {
+                    d00 = d01 = d10 = d11 = 128;
+                }
+                v = cr < 4 ? (cc < 4 ? d00 : d01) : (cc < 4 ? d10 : d11);
+            } else if (mode == 1) {
+                v = l[cr];
+            } else if (mode == 2) {
+                v = t[cc];
+            } else {
+                int Hh = 0, Vv = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int ptop = i == 3 ? S.ctl[p] : t[2 - i], pl = i == 3 ? S.ctl[p] : l[2 - i];
+                    Hh += (i + 1) * (t[4 + i] - ptop);
+                    Vv += (i + 1) * (l[4 + i] - pl);
+                }
+                const int a = 16 * (t[7] + l[7]), b = (17 * Hh + 16) >> 5, c = (17 * Vv + 16) >> 5;
+                v = (int)sat_shr<5>(a + b * (cc - 3) + c * (cr - 3) + 16);
+            }
+            smp[p] = (uint32_t)v;
+        }
+        const uint32_t cost =
+            wave_sum((uint32_t)iabs((int)src_u - (int)smp[0]) + (uint32_t)iabs((int)src_v - (int)smp[1])) +
+            (mode ? 8u : 0u);
+        if (cost < cbest) {
+            cbest = cost;
+            cmode = mode;
+            bpu = smp[0];
+            bpv = smp[1];
+        }
+    }
+    return {cmode, bpu, bpv};
+}
+
+struct I16Code {
+    int cbpl, cbpc, cmode;
+    uint32_t pu, pv;
+};
+
+// Codes the macroblock as I_16x16 in mode A.mode: the prediction (into S.pred, then the
+// reconstruction), chroma mode and prediction, luma AC / DC levels (S.coef, S.dcq), chroma
+// levels and reconstruction.
+__device__ I16Code code_i16(MbLds &S, int mb_x, int mb_y, int qp, int qpc, const I16Analysis &A, uint32_t src_u,
+                            uint32_t src_v) {
+    const int lane = threadIdx.x, rr = lane >> 2, c0 = (lane & 3) * 4;
+    const int has_t = mb_y > 0, has_l = mb_x > 0;
+    const int im = A.mode, pa = A.pa, pb = A.pb, pcc = A.pc, dc = A.dc;
+    int cbpl = 0, cbpc = 0, cmode = 0;
+    uint32_t pu = 0, pv = 0;
+    {
+        uint32_t w = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int v = im == 0 ? (has_t ? S.etop[c0 + i] : 128)
+                            : (im == 1 ? (has_l ? S.eleft[rr] : 128)
+                                       : (im == 2 ? dc : (int)sat_shr<5>(pa + pb * (c0 + i - 7) + pcc * (rr - 7) + 16)));
+            w |= (uint32_t)v << (8 * i);
+        }
+        *(uint32_t *)&S.pred[lane * 4] = w;
+    }
+    const ChromaIntra ci = chroma_intra(S, mb_x, mb_y, src_u, src_v);
+    cmode = ci.mode;
+    pu = ci.pu;
+    pv = ci.pv;
+    S.predu[lane] = (uint8_t)pu;
+    S.predv[lane] = (uint8_t)pv;
+    __syncthreads();
+    // Luma: 4x4 transforms, AC with intra rounding, DC through the 4x4 Hadamard
+    // (enc.rs:3630-3660; quant_luma_dc :1959).
+    int acnz = 0;
+    if (lane < 16) {
+        int c[16];
+        sub_fdct4(S.src, S.pred, 16, blk_x(lane) * 4, blk_y(lane) * 4, c);
+        S.dc_raw[Z2R[lane]] = (int16_t)c[0];
+        quant4(c, qp, 1);
+        c[0] = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            S.coef[lane][i] = (int16_t)c[i];
+            acnz |= c[i] != 0;
+        }
+    }
+    cbpl = __ballot(acnz) ? 15 : 0;
+    __syncthreads();
+    if (lane == 0) {
+        // simd/fixed.rs:318 hadamard4x4_scalar (with >> 1), then the DC quantizer;
+        // h264.rs:1645 dequantize_idct_luma_dc4x4 for the reconstruction.
+        int t[16], had[16];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int i = r * 4;
+            const int s03 = S.dc_raw[i] + S.dc_raw[i + 3], s12 = S.dc_raw[i + 1] + S.dc_raw[i + 2];
+            const int d03 = S.dc_raw[i] - S.dc_raw[i + 3], d12 = S.dc_raw[i + 1] - S.dc_raw[i + 2];
+            t[i] = s03 + s12;
+            t[i + 1] = d03 + d12;
+            t[i + 2] = s03 - s12;
+            t[i + 3] = d03 - d12;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int s03 = t[c] + t[c + 12], s12 = t[c + 4] + t[c + 8], d03 = t[c] - t[c + 12], d12 = t[c + 4] - t[c + 8];
+            had[c] = (int16_t)((s03 + s12) >> 1);
+            had[c + 4] = (int16_t)((d03 + d12) >> 1);
+            had[c + 8] = (int16_t)((s03 - s12) >> 1);
+            had[c + 12] = (int16_t)((d03 - d12) >> 1);
+        }
+        const int qbits = 16 + qp / 6, mf = c_quant4_mf[qp % 6][0], f = (1 << qbits) / 3;
+        int dq[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int l = (iabs(had[i]) * mf + f) >> qbits;
+            dq[i] = had[i] < 0 ? -l : l;
+            S.dcq[i] = (int16_t)dq[i];
+        }
+        const int qmul = (c_dequant4_init[qp % 6][0] * 16) << (qp / 6 + 2);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int z0 = dq[4 * i] + dq[4 * i + 1], z1 = dq[4 * i] - dq[4 * i + 1];
+            const int z2 = dq[4 * i + 2] - dq[4 * i + 3], z3 = dq[4 * i + 2] + dq[4 * i + 3];
+            t[4 * i] = z0 + z3;
+            t[4 * i + 1] = z0 - z3;
+            t[4 * i + 2] = z1 - z2;
+            t[4 * i + 3] = z1 + z2;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int z0 = t[i] + t[8 + i], z1 = t[i] - t[8 + i], z2 = t[4 + i] - t[12 + i], z3 = t[4 + i] + t[12 + i];
+            S.dcr[i] = (int16_t)(((z0 + z3) * qmul + 128) >> 8);
+            S.dcr[4 + i] = (int16_t)(((z1 + z2) * qmul + 128) >> 8);
+            S.dcr[8 + i] = (int16_t)(((z1 - z2) * qmul + 128) >> 8);
+            S.dcr[12 + i] = (int16_t)(((z0 - z3) * qmul + 128) >> 8);
+        }
+    }
+    quant_chroma(S, qpc, 1);
+    cbpc = chroma_cbp(S);
+    if (lane < 16) {
+        int blk[16], any = 0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) blk[i] = S.coef[lane][i];
+        if (cbpl) dequant4(blk, qp);
+        blk[0] = S.dcr[Z2R[lane]];
+#pragma unroll
+        for (int i = 0; i < 16; i++) any |= blk[i];
+        if (any) add_idct4(S.pred, 16, blk_x(lane) * 4, blk_y(lane) * 4, blk);
+    }
+    __syncthreads();
+    recon_chroma(S, cbpc, qpc);
+    return {cbpl, cbpc, cmode, pu, pv};
+}
+
+// ---------------------------------------------------------------------------------------
 // One macroblock (enc.rs:3011 code_p_mb and everything it calls).
 // ---------------------------------------------------------------------------------------
 //
@@ -761,24 +1053,26 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
         const uint32_t l0 = __builtin_amdgcn_readlane(dw0, 0), l2 = __builtin_amdgcn_readlane(dw2, 0);
         const uint32_t t0 = __builtin_amdgcn_readlane(dw0, 1), t2 = __builtin_amdgcn_readlane(dw2, 1);
         const uint32_t c0w = __builtin_amdgcn_readlane(dw0, 2), c2 = __builtin_amdgcn_readlane(dw2, 2);
+        // Intra neighbours have no motion (an I_4x4 record's mv words hold its modes).
         if (has_l) {
             lref = (l0 & 0xFF) >= 2 ? -1 : 0;
-            ax = (int16_t)(l2 & 0xFFFF);
-            ay = (int16_t)(l2 >> 16);
+            ax = lref ? 0 : (int16_t)(l2 & 0xFFFF);
+            ay = lref ? 0 : (int16_t)(l2 >> 16);
         }
         if (has_t) {
             tref = (t0 & 0xFF) >= 2 ? -1 : 0;
-            bx_ = (int16_t)(t2 & 0xFFFF);
-            by_ = (int16_t)(t2 >> 16);
+            bx_ = tref ? 0 : (int16_t)(t2 & 0xFFFF);
+            by_ = tref ? 0 : (int16_t)(t2 >> 16);
         }
         if (has_c) {
             dref = (c0w & 0xFF) >= 2 ? -1 : 0;
-            cxv = (int16_t)(c2 & 0xFFFF);
-            cyv = (int16_t)(c2 >> 16);
+            cxv = dref ? 0 : (int16_t)(c2 & 0xFFFF);
+            cyv = dref ? 0 : (int16_t)(c2 >> 16);
         }
     }
-    const uint32_t col2 = __builtin_amdgcn_readlane(dw2, 3);
-    const int colx = (int16_t)(col2 & 0xFFFF), coly = (int16_t)(col2 >> 16);
+    const uint32_t col0 = __builtin_amdgcn_readlane(dw0, 3), col2 = __builtin_amdgcn_readlane(dw2, 3);
+    const bool col_intra = (col0 & 0xFF) >= 2;
+    const int colx = col_intra ? 0 : (int16_t)(col2 & 0xFFFF), coly = col_intra ? 0 : (int16_t)(col2 >> 16);
     {
         const uint32_t eb = (ew >> eshift) & 0xFF;
         if (lane < 16)
@@ -1073,252 +1367,18 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
         D.n_satd++;
         const uint32_t inter_cost = inter_satd + lambda * (se_bits(mvx - mvpx) + se_bits(mvy - mvpy) + 1);
         if (inter_satd > 24 * lambda) {
-            const int has_t = mb_y > 0, has_l = mb_x > 0, tl = S.etl;
-            uint32_t st = 0, sl = 0;
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                st += S.etop[i];
-                sl += S.eleft[i];
-            }
-            const int dc = (has_t && has_l) ? (int)((st + sl + 16) >> 5)
-                                            : (has_t ? (int)((st + 8) >> 4) : (has_l ? (int)((sl + 8) >> 4) : 128));
-            // simd/h264_enc.rs:628 intra_satd_x3_16x16: V, H, DC SATDs from one source Hadamard
-            // (unavailable edges read as zero, as the reference passes zero rows).
-            uint32_t sv = 0, sh = 0, sd = 0;
-            if (lane < 16) {
-                const int bx = lane & 3, by = lane >> 2;
-                int rows[4][4];
-#pragma unroll
-                for (int y = 0; y < 4; y++) {
-                    const uint32_t w = *(const uint32_t *)&S.src[(by * 4 + y) * 16 + bx * 4];
-                    const int x0_ = byte_of(w, 0), x1 = byte_of(w, 1), x2 = byte_of(w, 2), x3 = byte_of(w, 3);
-                    const int s01 = x0_ + x1, d01 = x0_ - x1, s23 = x2 + x3, d23 = x2 - x3;
-                    rows[y][0] = s01 + s23;
-                    rows[y][1] = s01 - s23;
-                    rows[y][2] = d01 + d23;
-                    rows[y][3] = d01 - d23;
-                }
-                int tv[4], th[4];
-                {
-                    const int a0 = S.etop[bx * 4], a1 = S.etop[bx * 4 + 1], a2 = S.etop[bx * 4 + 2], a3 = S.etop[bx * 4 + 3];
-                    tv[0] = 4 * (a0 + a1 + a2 + a3);
-                    tv[1] = 4 * (a0 + a1 - a2 - a3);
-                    tv[2] = 4 * (a0 - a1 + a2 - a3);
-                    tv[3] = 4 * (a0 - a1 - a2 + a3);
-                    const int l0 = S.eleft[by * 4], l1 = S.eleft[by * 4 + 1], l2 = S.eleft[by * 4 + 2], l3 = S.eleft[by * 4 + 3];
-                    th[0] = 4 * (l0 + l1 + l2 + l3);
-                    th[1] = 4 * (l0 + l1 - l2 - l3);
-                    th[2] = 4 * (l0 - l1 + l2 - l3);
-                    th[3] = 4 * (l0 - l1 - l2 + l3);
-                }
-#pragma unroll
-                for (int h = 0; h < 4; h++) {
-                    const int s01 = rows[0][h] + rows[1][h], d01 = rows[0][h] - rows[1][h];
-                    const int s23 = rows[2][h] + rows[3][h], d23 = rows[2][h] - rows[3][h];
-                    const int o[4] = {(int16_t)(s01 + s23), (int16_t)(s01 - s23), (int16_t)(d01 + d23), (int16_t)(d01 - d23)};
-#pragma unroll
-                    for (int v = 0; v < 4; v++) {
-                        const int c = o[v];
-                        sv += (uint32_t)iabs(c - (v == 0 ? tv[h] : 0));
-                        sh += (uint32_t)iabs(c - (h == 0 ? th[v] : 0));
-                        sd += (uint32_t)iabs(c - ((v == 0 && h == 0) ? 16 * dc : 0));
-                    }
-                }
-            }
-            sv = wave_sum(sv) >> 1;
-            sh = wave_sum(sh) >> 1;
-            sd = wave_sum(sd) >> 1;
-            // enc.rs:1637 analyse_i16_edges: DC, then V, then H, then plane, strict improvements.
-            uint32_t ib = sd;
-            int im = 2;
-            if (has_t && sv < ib) { ib = sv; im = 0; }
-            if (has_l && sh < ib) { ib = sh; im = 1; }
-            int pa = 0, pb = 0, pcc = 0;
-            if (has_t && has_l) {
-                int Hh = 0, Vv = 0;
-#pragma unroll
-                for (int i = 0; i < 8; i++) {
-                    const int ptop = i == 7 ? tl : S.etop[6 - i], pl = i == 7 ? tl : S.eleft[6 - i];
-                    Hh += (i + 1) * (S.etop[8 + i] - ptop);
-                    Vv += (i + 1) * (S.eleft[8 + i] - pl);
-                }
-                pa = 16 * (S.etop[15] + S.eleft[15]);
-                pb = (5 * Hh + 32) >> 6;
-                pcc = (5 * Vv + 32) >> 6;
-                uint32_t w = 0;
-#pragma unroll
-                for (int i = 0; i < 4; i++) w |= sat_shr<5>(pa + pb * (c0 + i - 7) + pcc * (rr - 7) + 16) << (8 * i);
-                *(uint32_t *)&S.tmp[lane * 4] = w;
-                __syncthreads();
-                const uint32_t ps = satd16(S.src, S.tmp);
-                D.n_satd++;
-                if (ps < ib) { ib = ps; im = 3; }
-            }
-            if (ib + lambda * 8 < inter_cost) {
+            const I16Analysis ia = analyse_i16(S, mb_x, mb_y, D.n_satd);
+            if (ia.cost + lambda * 8 < inter_cost) {
                 // enc.rs:3587 code_i16 in a P slice.
                 mb_type = 2;
-                i16_mode = im;
+                i16_mode = ia.mode;
                 mvx = mvy = 0;
-                {
-                    uint32_t w = 0;
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        int v = im == 0 ? (has_t ? S.etop[c0 + i] : 128)
-                                        : (im == 1 ? (has_l ? S.eleft[rr] : 128)
-                                                   : (im == 2 ? dc : (int)sat_shr<5>(pa + pb * (c0 + i - 7) + pcc * (rr - 7) + 16)));
-                        w |= (uint32_t)v << (8 * i);
-                    }
-                    *(uint32_t *)&S.pred[lane * 4] = w;
-                }
-                // Chroma mode by SAD + 8 for non-DC modes (enc.rs:1523 analyse_chroma_intra;
-                // predictors h264.rs:1967 and enc.rs:1588).
-                const int cr = lane >> 3, cc = lane & 7;
-                uint32_t cbest = 0xFFFFFFFFu, bpu = 0, bpv = 0;
-                for (int mode = 0; mode < 4; mode++) {
-                    const int ok = mode == 1 ? has_l : (mode == 2 ? has_t : (mode == 3 ? (has_t && has_l) : 1));
-                    if (!ok) continue;
-                    uint32_t smp[2];
-#pragma unroll
-                    for (int p = 0; p < 2; p++) {
-                        const uint8_t *t = S.ctop[p], *l = S.cleft[p];
-                        int v;
-                        if (mode == 0) {
-                            uint32_t t0 = 0, t1 = 0, l0 = 0, l1 = 0;
-#pragma unroll
-                            for (int i = 0; i < 4; i++) {
-                                t0 += t[i];
-                                t1 += t[4 + i];
-                                l0 += l[i];
-                                l1 += l[4 + i];
-                            }
-                            int d00, d01, d10, d11;
-                            if (has_t && has_l) {
-                                d00 = (int)((t0 + l0 + 4) >> 3);
-                                d01 = (int)((t1 + 2) >> 2);
-                                d10 = (int)((l1 + 2) >> 2);
-                                d11 = (int)((t1 + l1 + 4) >> 3);
-                            } else if (has_t) {
-                                d00 = d10 = (int)((t0 + 2) >> 2);
-                                d01 = d11 = (int)((t1 + 2) >> 2);
-                            } else if (has_l) {
-                                d00 = d01 = (int)((l0 + 2) >> 2);
-                                d10 = d11 = (int)((l1 + 2) >> 2);
-                            } else {
-                                d00 = d01 = d10 = d11 = 128;
-                            }
-                            v = cr < 4 ? (cc < 4 ? d00 : d01) : (cc < 4 ? d10 : d11);
-                        } else if (mode == 1) {
-                            v = l[cr];
-                        } else if (mode == 2) {
-                            v = t[cc];
-                        } else {
-                            int Hh = 0, Vv = 0;
-#pragma unroll
-                            for (int i = 0; i < 4; i++) {
-                                const int ptop = i == 3 ? S.ctl[p] : t[2 - i], pl = i == 3 ? S.ctl[p] : l[2 - i];
-                                Hh += (i + 1) * (t[4 + i] - ptop);
-                                Vv += (i + 1) * (l[4 + i] - pl);
-                            }
-                            const int a = 16 * (t[7] + l[7]), b = (17 * Hh + 16) >> 5, c = (17 * Vv + 16) >> 5;
-                            v = (int)sat_shr<5>(a + b * (cc - 3) + c * (cr - 3) + 16);
-                        }
-                        smp[p] = (uint32_t)v;
-                    }
-                    const uint32_t cost =
-                        wave_sum((uint32_t)iabs((int)src_u - (int)smp[0]) + (uint32_t)iabs((int)src_v - (int)smp[1])) +
-                        (mode ? 8u : 0u);
-                    if (cost < cbest) {
-                        cbest = cost;
-                        cmode = mode;
-                        bpu = smp[0];
-                        bpv = smp[1];
-                    }
-                }
-                pu = bpu;
-                pv = bpv;
-                S.predu[lane] = (uint8_t)pu;
-                S.predv[lane] = (uint8_t)pv;
-                __syncthreads();
-                // Luma: 4x4 transforms, AC with intra rounding, DC through the 4x4 Hadamard
-                // (enc.rs:3630-3660; quant_luma_dc :1959).
-                int acnz = 0;
-                if (lane < 16) {
-                    int c[16];
-                    sub_fdct4(S.src, S.pred, 16, blk_x(lane) * 4, blk_y(lane) * 4, c);
-                    S.dc_raw[Z2R[lane]] = (int16_t)c[0];
-                    quant4(c, qp, 1);
-                    c[0] = 0;
-#pragma unroll
-                    for (int i = 0; i < 16; i++) {
-                        S.coef[lane][i] = (int16_t)c[i];
-                        acnz |= c[i] != 0;
-                    }
-                }
-                cbpl = __ballot(acnz) ? 15 : 0;
-                __syncthreads();
-                if (lane == 0) {
-                    // simd/fixed.rs:318 hadamard4x4_scalar (with >> 1), then the DC quantizer;
-                    // h264.rs:1645 dequantize_idct_luma_dc4x4 for the reconstruction.
-                    int t[16], had[16];
-#pragma unroll
-                    for (int r = 0; r < 4; r++) {
-                        const int i = r * 4;
-                        const int s03 = S.dc_raw[i] + S.dc_raw[i + 3], s12 = S.dc_raw[i + 1] + S.dc_raw[i + 2];
-                        const int d03 = S.dc_raw[i] - S.dc_raw[i + 3], d12 = S.dc_raw[i + 1] - S.dc_raw[i + 2];
-                        t[i] = s03 + s12;
-                        t[i + 1] = d03 + d12;
-                        t[i + 2] = s03 - s12;
-                        t[i + 3] = d03 - d12;
-                    }
-#pragma unroll
-                    for (int c = 0; c < 4; c++) {
-                        const int s03 = t[c] + t[c + 12], s12 = t[c + 4] + t[c + 8], d03 = t[c] - t[c + 12], d12 = t[c + 4] - t[c + 8];
-                        had[c] = (int16_t)((s03 + s12) >> 1);
-                        had[c + 4] = (int16_t)((d03 + d12) >> 1);
-                        had[c + 8] = (int16_t)((s03 - s12) >> 1);
-                        had[c + 12] = (int16_t)((d03 - d12) >> 1);
-                    }
-                    const int qbits = 16 + qp / 6, mf = c_quant4_mf[qp % 6][0], f = (1 << qbits) / 3;
-                    int dq[16];
-#pragma unroll
-                    for (int i = 0; i < 16; i++) {
-                        const int l = (iabs(had[i]) * mf + f) >> qbits;
-                        dq[i] = had[i] < 0 ? -l : l;
-                        S.dcq[i] = (int16_t)dq[i];
-                    }
-                    const int qmul = (c_dequant4_init[qp % 6][0] * 16) << (qp / 6 + 2);
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int z0 = dq[4 * i] + dq[4 * i + 1], z1 = dq[4 * i] - dq[4 * i + 1];
-                        const int z2 = dq[4 * i + 2] - dq[4 * i + 3], z3 = dq[4 * i + 2] + dq[4 * i + 3];
-                        t[4 * i] = z0 + z3;
-                        t[4 * i + 1] = z0 - z3;
-                        t[4 * i + 2] = z1 - z2;
-                        t[4 * i + 3] = z1 + z2;
-                    }
-#pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        const int z0 = t[i] + t[8 + i], z1 = t[i] - t[8 + i], z2 = t[4 + i] - t[12 + i], z3 = t[4 + i] + t[12 + i];
-                        S.dcr[i] = (int16_t)(((z0 + z3) * qmul + 128) >> 8);
-                        S.dcr[4 + i] = (int16_t)(((z1 + z2) * qmul + 128) >> 8);
-                        S.dcr[8 + i] = (int16_t)(((z1 - z2) * qmul + 128) >> 8);
-                        S.dcr[12 + i] = (int16_t)(((z0 - z3) * qmul + 128) >> 8);
-                    }
-                }
-                quant_chroma(S, qpc, 1);
-                cbpc = chroma_cbp(S);
-                if (lane < 16) {
-                    int blk[16], any = 0;
-#pragma unroll
-                    for (int i = 0; i < 16; i++) blk[i] = S.coef[lane][i];
-                    if (cbpl) dequant4(blk, qp);
-                    blk[0] = S.dcr[Z2R[lane]];
-#pragma unroll
-                    for (int i = 0; i < 16; i++) any |= blk[i];
-                    if (any) add_idct4(S.pred, 16, blk_x(lane) * 4, blk_y(lane) * 4, blk);
-                }
-                __syncthreads();
-                recon_chroma(S, cbpc, qpc);
+                const I16Code ic = code_i16(S, mb_x, mb_y, qp, qpc, ia, src_u, src_v);
+                cbpl = ic.cbpl;
+                cbpc = ic.cbpc;
+                cmode = ic.cmode;
+                pu = ic.pu;
+                pv = ic.pv;
             }
         }
 
@@ -1521,6 +1581,370 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
     return true;
 }
 
+// ---------------------------------------------------------------------------------------
+// IDR pictures: enc.rs:3311 code_intra — I_16x16 analysis, then I_4x4 (enc.rs:3333 analyse_i4)
+// when the 16x16 SATD exceeds 40 lambda, coded by code_i4 (:3441) or code_i16 (:3587).
+// ---------------------------------------------------------------------------------------
+constexpr int CV_W = 24;  // canvas: row 0 = the top edge, rows 1-16 the macroblock; column 3 = the
+                          // left edge, 4-19 the macroblock, 20-23 the top-right edge
+
+// One 4x4 intra prediction (8.3.1.2, enc.rs:1489 pred4): t[0..7] the row above (with the
+// top-right samples), l[0..3] the column left, m the corner; unavailable edges are 128.
+__device__ __forceinline__ void pred4x4(int mode, const int *t, const int *l, int m, bool has_t, bool has_l, int *p) {
+    // Edge sample p[x, -1] (x = -1: the corner) and p[-1, y].
+    auto T = [&](int x) { return x < 0 ? m : t[x]; };
+    auto L = [&](int y) { return y < 0 ? m : l[y]; };
+#pragma unroll
+    for (int y = 0; y < 4; y++)
+#pragma unroll
+        for (int x = 0; x < 4; x++) {
+            int v;
+            switch (mode) {
+            case 0: v = t[x]; break;
+            case 1: v = l[y]; break;
+            case 2: {
+                const int st = t[0] + t[1] + t[2] + t[3], sl = l[0] + l[1] + l[2] + l[3];
+                v = has_t && has_l ? (st + sl + 4) >> 3 : (has_t ? (st + 2) >> 2 : (has_l ? (sl + 2) >> 2 : 128));
+            } break;
+            case 3:  // diagonal down left
+                v = (x == 3 && y == 3) ? (t[6] + 3 * t[7] + 2) >> 2 : (t[x + y] + 2 * t[x + y + 1] + t[x + y + 2] + 2) >> 2;
+                break;
+            case 4:  // diagonal down right
+                v = x > y ? (T(x - y - 2) + 2 * T(x - y - 1) + T(x - y) + 2) >> 2
+                          : (x < y ? (L(y - x - 2) + 2 * L(y - x - 1) + L(y - x) + 2) >> 2 : (t[0] + 2 * m + l[0] + 2) >> 2);
+                break;
+            case 5: {  // vertical right
+                const int z = 2 * x - y, b = x - (y >> 1);
+                v = z >= 0 ? ((z & 1) ? (T(b - 2) + 2 * T(b - 1) + T(b) + 2) >> 2 : (T(b - 1) + T(b) + 1) >> 1)
+                           : (z == -1 ? (l[0] + 2 * m + t[0] + 2) >> 2 : (L(y - 1) + 2 * L(y - 2) + L(y - 3) + 2) >> 2);
+            } break;
+            case 6: {  // horizontal down
+                const int z = 2 * y - x, b = y - (x >> 1);
+                v = z >= 0 ? ((z & 1) ? (L(b - 2) + 2 * L(b - 1) + L(b) + 2) >> 2 : (L(b - 1) + L(b) + 1) >> 1)
+                           : (z == -1 ? (l[0] + 2 * m + t[0] + 2) >> 2 : (T(x - 1) + 2 * T(x - 2) + T(x - 3) + 2) >> 2);
+            } break;
+            case 7: {  // vertical left
+                const int b = x + (y >> 1);
+                v = (y & 1) ? (t[b] + 2 * t[b + 1] + t[b + 2] + 2) >> 2 : (t[b] + t[b + 1] + 1) >> 1;
+            } break;
+            default: {  // horizontal up
+                const int z = x + 2 * y, b = y + (x >> 1);
+                v = z > 5 ? l[3]
+                          : (z == 5 ? (l[2] + 3 * l[3] + 2) >> 2
+                                    : ((z & 1) ? (l[b] + 2 * l[b + 1] + l[b + 2 < 4 ? b + 2 : 3] + 2) >> 2 : (l[b] + l[b + 1] + 1) >> 1));
+            } break;
+            }
+            p[y * 4 + x] = v;
+        }
+}
+
+// 4x4 SATD (sum of the Hadamard coefficients of the difference, halved; simd/h264_enc.rs:196).
+__device__ __forceinline__ uint32_t satd4x4(const int *d) {
+    int t[16];
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+        const int s03 = d[y * 4] + d[y * 4 + 3], s12 = d[y * 4 + 1] + d[y * 4 + 2];
+        const int d03 = d[y * 4] - d[y * 4 + 3], d12 = d[y * 4 + 1] - d[y * 4 + 2];
+        t[y * 4] = s03 + s12;
+        t[y * 4 + 1] = d03 + d12;
+        t[y * 4 + 2] = s03 - s12;
+        t[y * 4 + 3] = d03 - d12;
+    }
+    uint32_t sum = 0;
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+        const int s03 = t[x] + t[12 + x], s12 = t[4 + x] + t[8 + x], d03 = t[x] - t[12 + x], d12 = t[4 + x] - t[8 + x];
+        sum += (uint32_t)(iabs(s03 + s12) + iabs(d03 + d12) + iabs(s03 - s12) + iabs(d03 - d12));
+    }
+    return sum >> 1;
+}
+
+// Forward 4x4 integer transform of a residual held in registers (the transform of sub_fdct4).
+__device__ __forceinline__ void fdct4_regs(const int *d, int *out) {
+    int t[16];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const int s03 = d[r * 4] + d[r * 4 + 3], s12 = d[r * 4 + 1] + d[r * 4 + 2];
+        const int d03 = d[r * 4] - d[r * 4 + 3], d12 = d[r * 4 + 1] - d[r * 4 + 2];
+        t[r * 4] = s03 + s12;
+        t[r * 4 + 1] = (d03 << 1) + d12;
+        t[r * 4 + 2] = s03 - s12;
+        t[r * 4 + 3] = d03 - (d12 << 1);
+    }
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int s03 = t[c] + t[c + 12], s12 = t[c + 4] + t[c + 8], d03 = t[c] - t[c + 12], d12 = t[c + 4] - t[c + 8];
+        out[c] = (int16_t)(s03 + s12);
+        out[c + 4] = (int16_t)((d03 << 1) + d12);
+        out[c + 8] = (int16_t)(s03 - s12);
+        out[c + 12] = (int16_t)(d03 - (d12 << 1));
+    }
+}
+
+__device__ __forceinline__ int intra4_available(int mode, bool has_t, bool has_l) {  // h264.rs:3273
+    return (mode == 0 || mode == 3 || mode == 7) ? has_t : ((mode == 1 || mode == 8) ? has_l : (mode == 2 ? 1 : has_t && has_l));
+}
+
+// Mode of block (bx, by) (0..3 each) of a packed decoding-order mode word.
+__device__ __forceinline__ int mode_at(uint64_t packed, int bx, int by) {
+    const int b = (bx & 1) | ((by & 1) << 1) | ((bx >> 1) << 2) | ((by >> 1) << 3);
+    return (int)((packed >> (4 * b)) & 15);
+}
+
+// enc.rs:3333 analyse_i4 on the canvas (edges in row 0 / column 3): per block, every available
+// mode's SATD + lambda * (1 for the predicted mode, else 4) on its own lane, the first cheapest
+// kept, the block quantized (intra rounding) and reconstructed into the canvas. Returns false as
+// soon as the running cost (from 24 lambda) reaches budget. lmodes / tmodes: the left and top
+// macroblocks' packed modes (all DC when they are not I_4x4).
+__device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint32_t lambda, uint32_t budget,
+                           uint64_t lmodes, uint64_t tmodes, uint64_t &modes, int &cbpl) {
+    const int lane = threadIdx.x;
+    uint32_t total = 24 * lambda;
+    modes = 0;
+    cbpl = 0;
+    for (int b = 0; b < 16; b++) {
+        const int bx = blk_x(b), by = blk_y(b), px = 4 * bx, py = 4 * by;
+        const bool has_t = mb_y * 16 + py > 0, has_l = mb_x * 16 + px > 0;
+        const bool tr = !(b == 3 || b == 7 || b == 11 || b == 13 || b == 15) && (b != 5 || mb_x + 1 < mb_w);
+        const int lm = bx ? mode_at(modes, bx - 1, by) : mode_at(lmodes, 3, by);
+        const int tm = by ? mode_at(modes, bx, by - 1) : mode_at(tmodes, bx, 3);
+        const int pm = (mb_x * 4 + bx == 0 || mb_y * 4 + by == 0) ? 2 : min(lm, tm);
+        int p[16], d[16];
+        uint32_t cost = 0xFFFFFFFFu;
+        if (lane < 9 && intra4_available(lane, has_t, has_l)) {
+            int t[8], l[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                t[i] = has_t ? S.cv[py * CV_W + px + 4 + i] : 128;
+                l[i] = has_l ? S.cv[(py + 1 + i) * CV_W + px + 3] : 128;
+            }
+#pragma unroll
+            for (int i = 4; i < 8; i++) t[i] = has_t ? (tr ? S.cv[py * CV_W + px + 4 + i] : t[3]) : 128;
+            const int m = (has_t && has_l) ? S.cv[py * CV_W + px + 3] : 128;
+            pred4x4(lane, t, l, m, has_t, has_l, p);
+#pragma unroll
+            for (int i = 0; i < 16; i++) d[i] = (int)S.src[(py + (i >> 2)) * 16 + px + (i & 3)] - p[i];
+            cost = satd4x4(d) + lambda * (lane == pm ? 1u : 4u);
+        }
+        uint32_t best = 0xFFFFFFFFu;
+        int bm = 2;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+            const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)cost, k);
+            if (c < best) {
+                best = c;
+                bm = k;
+            }
+        }
+        total += best;
+        if (total >= budget) return false;
+        modes |= (uint64_t)bm << (4 * b);
+        int nz = 0;
+        if (lane == bm) {
+            int c[16];
+            fdct4_regs(d, c);
+            quant4(c, qp, 1);
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                S.q4i[b][i] = (int16_t)c[i];
+                nz |= c[i];
+            }
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                *(uint32_t *)&S.cv[(py + 1 + r) * CV_W + px + 4] = (uint32_t)p[r * 4] | ((uint32_t)p[r * 4 + 1] << 8) |
+                                                                 ((uint32_t)p[r * 4 + 2] << 16) | ((uint32_t)p[r * 4 + 3] << 24);
+            if (nz) {
+                dequant4(c, qp);
+                add_idct4(S.cv, CV_W, px + 4, py + 1, c);
+            }
+        }
+        if (__ballot(nz != 0)) cbpl |= 1 << (b >> 2);
+        __syncthreads();
+    }
+    return true;
+}
+
+// One IDR macroblock: decision record, levels and reconstruction.
+__device__ void analyse_intra_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S) {
+    const int lane = threadIdx.x, rr = lane >> 2, c0 = (lane & 3) * 4;
+    const int mb_w = J.mb_w, idx = mb_y * mb_w + mb_x;
+    const int x0 = mb_x * 16, y0 = mb_y * 16, cx0 = mb_x * 8, cy0 = mb_y * 8;
+    const int qp = J.qp, qpc = c_chroma_qp[qp];
+    const uint32_t lambda = c_lambda[qp];
+    const int W = mb_w * 16;
+    const bool has_l = mb_x > 0, has_t = mb_y > 0;
+
+    // Round trip: source, the left / top decisions (their intra 4x4 modes), the edges
+    // (luma top row with the top-right four samples, left column, corners; chroma edges).
+    uint32_t srcw = 0;
+    {
+        const int sy = min(y0 + rr, J.height - 1), sx = x0 + c0;
+        const RGH_G uint8_t *row = J.src_y + (long)sy * J.src_sy;
+        if (sx + 3 < J.width && (((uintptr_t)(row + sx)) & 3) == 0) {
+            srcw = *(const RGH_G uint32_t *)(row + sx);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) srcw |= (uint32_t)row[min(sx + i, J.width - 1)] << (8 * i);
+        }
+    }
+    uint32_t src_u, src_v;
+    {
+        const int cw = (J.width + 1) / 2, ch = (J.height + 1) / 2;
+        const int csy = min(cy0 + (lane >> 3), ch - 1), csx = min(cx0 + (lane & 7), cw - 1);
+        src_u = J.src_u[(long)csy * J.src_su + csx];
+        src_v = J.src_v[(long)csy * J.src_sv + csx];
+    }
+    uint32_t dw0 = 0, dw2 = 0, dw3 = 0;
+    if (lane < 2) {
+        const int di = lane == 0 ? (has_l ? idx - 1 : idx) : (has_t ? idx - mb_w : idx);
+        const RGH_G uint32_t *dp = (const RGH_G uint32_t *)&J.dec[di];
+        dw0 = dp[0];
+        dw2 = dp[2];
+        dw3 = dp[3];
+    }
+    uint32_t ew = 0;
+    int eshift = 0;
+    if (lane < 44) {
+        const RGH_G uint8_t *ea;
+        if (lane < 16) {
+            ea = J.cur_y + (long)(y0 + lane) * J.sy + x0 - 1;
+        } else if (lane < 32) {
+            const RGH_G uint8_t *cp = ((lane >> 3) & 1) ? J.cur_v : J.cur_u;
+            ea = cp + (long)(cy0 + (lane & 7)) * J.sc + cx0 - 1;
+        } else if (lane < 37) {  // top luma row and the top-right dword
+            ea = J.cur_y + (long)(y0 - 1) * J.sy + x0 + 4 * (lane - 32);
+        } else if (lane < 41) {
+            const RGH_G uint8_t *cp = ((lane - 37) >> 1) ? J.cur_v : J.cur_u;
+            ea = cp + (long)(cy0 - 1) * J.sc + cx0 + 4 * ((lane - 37) & 1);
+        } else {
+            const RGH_G uint8_t *cp = lane == 41 ? J.cur_y : (lane == 42 ? J.cur_u : J.cur_v);
+            ea = cp + (long)((lane == 41 ? y0 : cy0) - 1) * (lane == 41 ? J.sy : J.sc) + (lane == 41 ? x0 : cx0) - 1;
+        }
+        eshift = 8 * (int)((uintptr_t)ea & 3);
+        ew = *(const RGH_G uint32_t *)(ea - ((uintptr_t)ea & 3));
+    }
+    const bool has_tr = has_t && mb_x + 1 < mb_w;
+    {
+        const uint32_t eb = (ew >> eshift) & 0xFF;
+        if (lane < 16) {
+            S.eleft[lane] = has_l ? (uint8_t)eb : 0;
+        } else if (lane < 32) {
+            S.cleft[(lane >> 3) & 1][lane & 7] = has_l ? (uint8_t)eb : 0;
+        } else if (lane < 36) {
+            *(uint32_t *)&S.etop[4 * (lane - 32)] = has_t ? ew : 0u;
+        } else if (lane < 37) {
+        } else if (lane < 41) {
+            *(uint32_t *)&S.ctop[(lane - 37) >> 1][4 * ((lane - 37) & 1)] = has_t ? ew : 0u;
+        } else if (lane == 41) {
+            S.etl = has_l && has_t ? (int)eb : 128;
+        } else if (lane < 44) {
+            S.ctl[lane - 42] = has_l && has_t ? (int)eb : 128;
+        }
+    }
+    *(uint32_t *)&S.src[lane * 4] = srcw;
+    S.srcu[lane] = (uint8_t)src_u;
+    S.srcv[lane] = (uint8_t)src_v;
+    // Intra 4x4 modes of the neighbours (DC for anything but I_4x4).
+    const uint32_t l0 = __builtin_amdgcn_readlane(dw0, 0), t0 = __builtin_amdgcn_readlane(dw0, 1);
+    const uint64_t lpack = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(dw3, 0) << 32) | (uint32_t)__builtin_amdgcn_readlane(dw2, 0);
+    const uint64_t tpack = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane(dw3, 1) << 32) | (uint32_t)__builtin_amdgcn_readlane(dw2, 1);
+    const uint64_t all_dc = 0x2222222222222222ull;
+    const uint64_t lmodes = has_l && (l0 & 0xFF) == 3 ? lpack : all_dc, tmodes = has_t && (t0 & 0xFF) == 3 ? tpack : all_dc;
+    __syncthreads();
+
+    const uint32_t tr_word = has_tr ? (uint32_t)__builtin_amdgcn_readlane((int)ew, 36) : 0u;
+
+    uint16_t n_satd = 0;
+    const I16Analysis ia = analyse_i16(S, mb_x, mb_y, n_satd);
+    int mb_type = 2, cbpl = 0, cbpc = 0, cmode = 0;
+    uint32_t pu = 0, pv = 0;
+    uint64_t modes = 0;
+    bool i4 = false;
+    if (ia.cost > 40 * lambda) {
+        // The canvas shares storage with the 16x16 analysis' plane prediction: filled now.
+        __syncthreads();
+        if (lane < 16)
+            S.cv[(1 + lane) * CV_W + 3] = S.eleft[lane];
+        else if (lane < 20)
+            *(uint32_t *)&S.cv[4 + 4 * (lane - 16)] = *(const uint32_t *)&S.etop[4 * (lane - 16)];
+        else if (lane == 20)
+            *(uint32_t *)&S.cv[20] = tr_word;
+        else if (lane == 21)
+            S.cv[3] = (uint8_t)S.etl;
+        __syncthreads();
+        i4 = analyse_i4(S, mb_x, mb_y, mb_w, qp, lambda, ia.cost + 4 * lambda, lmodes, tmodes, modes, cbpl);
+    }
+    if (i4) {
+        // enc.rs:3441 code_i4: the canvas holds the luma reconstruction.
+        mb_type = 3;
+        *(uint32_t *)&S.pred[lane * 4] = *(const uint32_t *)&S.cv[(1 + rr) * CV_W + 4 + c0];
+        const ChromaIntra ci = chroma_intra(S, mb_x, mb_y, src_u, src_v);
+        cmode = ci.mode;
+        pu = ci.pu;
+        pv = ci.pv;
+        S.predu[lane] = (uint8_t)pu;
+        S.predv[lane] = (uint8_t)pv;
+        __syncthreads();
+        quant_chroma(S, qpc, 1);
+        cbpc = chroma_cbp(S);
+        recon_chroma(S, cbpc, qpc);
+    } else {
+        cbpl = 0;
+        const I16Code ic = code_i16(S, mb_x, mb_y, qp, qpc, ia, src_u, src_v);
+        cbpl = ic.cbpl;
+        cbpc = ic.cbpc;
+        cmode = ic.cmode;
+    }
+    __syncthreads();
+    *(RGH_G uint32_t *)&J.cur_y[(long)(y0 + rr) * J.sy + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
+    J.cur_u[(long)(cy0 + (lane >> 3)) * J.sc + cx0 + (lane & 7)] = S.predu[lane];
+    J.cur_v[(long)(cy0 + (lane >> 3)) * J.sc + cx0 + (lane & 7)] = S.predv[lane];
+    if (J.pre_y) {
+        *(RGH_G uint32_t *)&J.pre_y[(long)(y0 + rr) * W + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
+        J.pre_u[(long)(cy0 + (lane >> 3)) * (W / 2) + cx0 + (lane & 7)] = S.predu[lane];
+        J.pre_v[(long)(cy0 + (lane >> 3)) * (W / 2) + cx0 + (lane & 7)] = S.predv[lane];
+    }
+    RGH_G int16_t *lv = J.levels + (long)idx * RGH_LEVELS_PER_MB;
+    uint32_t nnz = 0;
+    {
+        int nzb = 0;
+        if (mb_type == 3) {
+            for (int i = lane; i < 256; i += 64) lv[i] = S.q4i[i >> 4][c_zigzag4[i & 15]];
+            if (lane < 16)
+#pragma unroll
+                for (int i = 0; i < 16; i++) nzb |= S.q4i[lane][i] != 0;
+        } else {
+            for (int i = lane; i < 256; i += 64) lv[i] = S.coef[i >> 4][c_zigzag4[i & 15]];
+            if (lane < 16 && cbpl)
+#pragma unroll
+                for (int i = 0; i < 16; i++) nzb |= S.coef[lane][i] != 0;
+        }
+        const uint64_t bl = __ballot(nzb);
+#pragma unroll
+        for (int b = 0; b < 16; b++)
+            if ((bl >> b) & 1) nnz |= 1u << (blk_y(b) * 4 + blk_x(b));
+    }
+    if (lane < 16) lv[256 + lane] = mb_type == 2 ? S.dcq[c_zigzag4[lane]] : 0;
+    if (lane < 8) lv[272 + lane] = S.cdc[lane >> 2][lane & 3];
+    for (int i = lane; i < 128; i += 64) {
+        const int p = i >> 6, b = (i >> 4) & 3, k = i & 15;
+        lv[280 + i] = (k == 0 || cbpc < 2) ? 0 : S.craster[p][b][c_zigzag4[k]];
+    }
+    if (lane == 0) {
+        rgh_mb_decision D;
+        memset(&D, 0, sizeof(D));
+        D.mb_type = (uint8_t)mb_type;
+        D.i16_mode = (uint8_t)(mb_type == 2 ? ia.mode : 0);
+        D.chroma_mode = (uint8_t)cmode;
+        D.cbp_luma = (uint8_t)cbpl;
+        D.cbp_chroma = (uint8_t)cbpc;
+        D.qp = (uint8_t)qp;
+        if (mb_type == 3) __builtin_memcpy(D.mv, &modes, sizeof(modes));
+        D.nnz = (uint16_t)nnz;
+        store_dec(&J.dec[idx], D);
+    }
+}
+
 union StepLds {
     MbLds a;
     DbLds d;
@@ -1559,7 +1983,12 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_frame_step(const RghJob *
         if ((uint32_t)(J.ana[m] >> 32) == J.serial) return;  // analysed in an earlier launch
         bool ok = true;
         if (s == 0 && J.wait_analysed) ok = __atomic_load_n(J.wait_analysed, __ATOMIC_RELAXED) >= J.wait_total;
-        if (ok) ok = analyse_mb(J, x, y, S.a, seq);
+        if (ok) {
+            if (J.intra)
+                analyse_intra_mb(J, x, y, S.a);
+            else
+                ok = analyse_mb(J, x, y, S.a, seq);
+        }
         if (lane == 0) {
             if (ok) {
                 J.ana[m] = make_stamp(J.serial, seq);

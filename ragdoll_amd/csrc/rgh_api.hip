@@ -140,7 +140,7 @@ int frame_steps(const Ctx *c) { return (c->mb_w - 1) + 2 * (c->mb_h - 1) + 1; }
 // Frame k of a call on context c: analysed into slot (ref + 1 + k) % ring, with the frame of
 // slot (ref + k) % ring as its reference.
 RghJob make_job(Ctx *c, int k, const rgh_i420 &src, rgh_mb_decision *dec, const rgh_mb_decision *prev,
-                int16_t *levels, int base, RghFrameCtl *ctl, const RghFrameCtl *wait) {
+                int16_t *levels, int base, RghFrameCtl *ctl, const RghFrameCtl *wait, bool intra = false) {
     RghJob j;
     std::memset(&j, 0, sizeof(j));
     const int R = (int)c->pic.size();
@@ -183,6 +183,11 @@ RghJob make_job(Ctx *c, int k, const rgh_i420 &src, rgh_mb_decision *dec, const 
     j.ana = w.ana;
     j.fin = w.fin;
     j.ref_fin = r.fin;
+    if (intra) {  // an IDR picture reads no reference
+        j.intra = 1;
+        j.ref_fin = nullptr;
+        j.prev = dec;
+    }
     j.ctl = ctl;
     if (wait) {
         j.wait_analysed = &wait->analysed;
@@ -512,7 +517,7 @@ rgh_status rgh_set_reference(void *ctx, const rgh_i420 *recon) {
 }
 
 // One frame of each context, outputs into the contexts' own buffers.
-rgh_status analyse_single(Ctx *const *cs, int n, const rgh_i420 *src) {
+rgh_status analyse_single(Ctx *const *cs, int n, const rgh_i420 *src, bool intra = false) {
     Ctx *c0 = cs[0];
     HIP_TRY(hipSetDevice(c0->device));
     rgh_status s = ensure_jobs(c0, n);
@@ -520,7 +525,7 @@ rgh_status analyse_single(Ctx *const *cs, int n, const rgh_i420 *src) {
     std::vector<RghJob> jobs((size_t)n);
     for (int i = 0; i < n; i++) {
         Ctx *c = cs[i];
-        jobs[(size_t)i] = make_job(c, 0, src[i], c->dec[c->dcur], c->prev, c->levels, 0, c0->ctl_dev + i, nullptr);
+        jobs[(size_t)i] = make_job(c, 0, src[i], c->dec[c->dcur], c->prev, c->levels, 0, c0->ctl_dev + i, nullptr, intra);
         if (c->keep_pre) {
             jobs[(size_t)i].pre_y = c->pre[0];
             jobs[(size_t)i].pre_u = c->pre[1];
@@ -534,6 +539,7 @@ rgh_status analyse_single(Ctx *const *cs, int n, const rgh_i420 *src) {
         advance(c, 1, c->dec[c->dcur]);
         c->last_levels = c->levels;
         c->dcur = 1 - c->dcur;
+        c->has_ref = true;
     }
     return RGH_OK;
 }
@@ -556,16 +562,19 @@ rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src) 
 }
 
 rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src,
-                              rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag, int32_t groups) {
-    if (!ctxs || n <= 0 || n_frames <= 0 || !src || !decisions || !levels || lag < 0 || groups < 0)
+                              rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag, int32_t groups,
+                              int32_t flags) {
+    if (!ctxs || n <= 0 || n_frames <= 0 || !src || !decisions || !levels || lag < 0 || groups < 0 ||
+        (flags & ~RGH_FRAMES_FIRST_IDR))
         return RGH_ERR_INVALID_ARGUMENT;
+    const bool first_idr = (flags & RGH_FRAMES_FIRST_IDR) != 0;
     if ((int64_t)n * n_frames > (1 << 20)) return RGH_ERR_INVALID_ARGUMENT;
     std::vector<Ctx *> cs((size_t)n);
     for (int i = 0; i < n; i++) {
         Ctx *c = static_cast<Ctx *>(ctxs[i]);
         if (!c) return RGH_ERR_INVALID_ARGUMENT;
         if (c->device != static_cast<Ctx *>(ctxs[0])->device) return RGH_ERR_INVALID_ARGUMENT;
-        if (!c->has_ref) return RGH_ERR_NO_REFERENCE;
+        if (!first_idr && !c->has_ref) return RGH_ERR_NO_REFERENCE;
         for (int k = 0; k < i; k++)
             if (cs[(size_t)k] == c) return RGH_ERR_INVALID_ARGUMENT;
         for (int k = 0; k < n_frames; k++) {
@@ -604,8 +613,8 @@ rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, co
                 const int wk = k + 1 - R;
                 const RghFrameCtl *wait = wk >= 0 ? ctl + (size_t)wk * m + t : nullptr;
                 const rgh_mb_decision *prev = k ? decisions[q - 1] : c->prev;
-                S.jobs[(size_t)k * m + t] =
-                    make_job(c, k, src[q], decisions[q], prev, levels[q], k * lag, ctl + (size_t)k * m + t, wait);
+                S.jobs[(size_t)k * m + t] = make_job(c, k, src[q], decisions[q], prev, levels[q], k * lag,
+                                                     ctl + (size_t)k * m + t, wait, first_idr && k == 0);
             }
     }
     // The other groups' streams start after everything queued on the first one.
@@ -628,18 +637,19 @@ rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, co
         advance(c, n_frames, keep);
         c->last_levels = levels[(size_t)i * n_frames + n_frames - 1];
         c->dcur = 1 - c->dcur;
+        c->has_ref = true;
     }
     HIP_TRY(hipStreamSynchronize(cs[0]->stream));
     return RGH_OK;
 }
 
-rgh_status rgh_analyse_p_frame(void *ctx, const rgh_i420 *src, rgh_mb_decision *decisions, int16_t *levels,
-                               const rgh_i420_out *recon) {
+rgh_status analyse_host_frame(void *ctx, const rgh_i420 *src, rgh_mb_decision *decisions, int16_t *levels,
+                              const rgh_i420_out *recon, bool intra) {
     if (!ctx) return RGH_ERR_INVALID_ARGUMENT;
     Ctx *c = static_cast<Ctx *>(ctx);
     rgh_status s = check_i420(src, c->width, c->height);
     if (s != RGH_OK) return s;
-    if (!c->has_ref) return RGH_ERR_NO_REFERENCE;
+    if (!intra && !c->has_ref) return RGH_ERR_NO_REFERENCE;
     HIP_TRY(hipSetDevice(c->device));
     const int cw = (c->width + 1) / 2, ch = (c->height + 1) / 2;
     HIP_TRY(hipMemcpy2DAsync(c->src[0], (size_t)c->width, src->y, (size_t)src->stride_y, (size_t)c->width,
@@ -650,9 +660,19 @@ rgh_status rgh_analyse_p_frame(void *ctx, const rgh_i420 *src, rgh_mb_decision *
                              hipMemcpyHostToDevice, c->stream));
     rgh_i420 dsrc = {c->src[0], c->src[1], c->src[2], c->width, cw, cw, c->width, c->height};
     Ctx *cs[1] = {c};
-    s = analyse_single(cs, 1, &dsrc);
+    s = analyse_single(cs, 1, &dsrc, intra);
     if (s != RGH_OK) return s;
     return rgh_get_outputs(ctx, decisions, levels, recon, 1);
+}
+
+rgh_status rgh_analyse_p_frame(void *ctx, const rgh_i420 *src, rgh_mb_decision *decisions, int16_t *levels,
+                               const rgh_i420_out *recon) {
+    return analyse_host_frame(ctx, src, decisions, levels, recon, false);
+}
+
+rgh_status rgh_analyse_idr(void *ctx, const rgh_i420 *src, rgh_mb_decision *decisions, int16_t *levels,
+                           const rgh_i420_out *recon) {
+    return analyse_host_frame(ctx, src, decisions, levels, recon, true);
 }
 
 rgh_status rgh_get_outputs(void *ctx, rgh_mb_decision *decisions, int16_t *levels, const rgh_i420_out *recon,

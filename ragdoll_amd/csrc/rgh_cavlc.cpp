@@ -1,5 +1,6 @@
-// Entropy coding of analysed P pictures: the CAVLC (Constrained Baseline) slice a ragdoll
-// encoder writes for the decisions and levels the GPU analysis returns. Host code.
+// Entropy coding of analysed pictures: the CAVLC (Constrained Baseline) slices a ragdoll encoder
+// writes for the decisions and levels the GPU analysis returns, and the stream's SPS / PPS.
+// Host code.
 //
 // Restates the syntax writers of crates/ragdoll-codec/src/h264/enc.rs: the slice header of
 // SliceHeader::write (crates/ragdoll-codec/src/h264.rs:1071-1127) as encode_frame_body sets it
@@ -187,6 +188,65 @@ uint32_t inter_cbp_code(int cbp) {
     return 0;
 }
 
+uint32_t intra4_cbp_code(int cbp) {
+    for (uint32_t i = 0; i < 48; i++)
+        if (kGolombToIntra4x4Cbp[i] == cbp) return i;
+    return 3;
+}
+
+// NAL unit (enc.rs:2643): start code, header byte, emulation prevention; appended to nal.
+void emit_nal(std::vector<uint8_t> &nal, int ref_idc, int type, const std::vector<uint8_t> &rbsp) {
+    nal.insert(nal.end(), {0, 0, 0, 1, (uint8_t)((ref_idc << 5) | type)});
+    int zeros = 0;
+    for (uint8_t b : rbsp) {
+        if (zeros == 2 && b <= 3) {
+            nal.push_back(3);
+            zeros = 0;
+        }
+        nal.push_back(b);
+        zeros = b == 0 ? zeros + 1 : 0;
+    }
+}
+
+rgh_status deliver(const std::vector<uint8_t> &nal, uint8_t *out, int64_t cap, int64_t *out_len) {
+    *out_len = (int64_t)nal.size();
+    if ((int64_t)nal.size() > cap) return RGH_ERR_INVALID_ARGUMENT;
+    memcpy(out, nal.data(), nal.size());
+    return RGH_OK;
+}
+
+// Luma 4x4 residual blocks of a coded 8x8 pattern (enc.rs:976 write_cavlc_residuals).
+void write_luma4x4(BitWriter &w, NnzGrid &lg, int mb_x, int mb_y, int cbpl, const int16_t *lv) {
+    for (int i8 = 0; i8 < 4; i8++)
+        for (int k = 0; k < 4; k++) {
+            const int b = i8 * 4 + k, x = 4 * mb_x + blk_x(b), y = 4 * mb_y + blk_y(b);
+            if (!(cbpl & (1 << i8))) {
+                lg.at(x, y) = 0;
+                continue;
+            }
+            const int16_t *zz = lv + kLuma + 16 * b;
+            cavlc_block(w, zz, 16, lg.nc(x, y));
+            lg.at(x, y) = (uint8_t)count_nz(zz, 16);
+        }
+}
+
+// I_16x16 macroblock body after mb_type: chroma mode, mb_qp_delta, DC, AC, chroma
+// (enc.rs:3587 code_i16).
+void write_i16_body(BitWriter &w, NnzGrid &lg, NnzGrid *cg, int mb_x, int mb_y, const rgh_mb_decision &d,
+                    const int16_t *lv) {
+    w.ue(d.chroma_mode);
+    w.se(0);  // mb_qp_delta: fixed QP
+    cavlc_block(w, lv + kLumaDc, 16, lg.nc(4 * mb_x, 4 * mb_y));
+    if (d.cbp_luma)
+        for (int b = 0; b < 16; b++) {
+            const int x = 4 * mb_x + blk_x(b), y = 4 * mb_y + blk_y(b);
+            const int16_t *ac = lv + kLuma + 16 * b + 1;
+            cavlc_block(w, ac, 15, lg.nc(x, y));
+            lg.at(x, y) = (uint8_t)count_nz(ac, 15);
+        }
+    write_chroma(w, cg, mb_x, mb_y, d.cbp_chroma, lv);
+}
+
 }  // namespace
 
 extern "C" rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const rgh_mb_decision *decisions,
@@ -241,55 +301,199 @@ extern "C" rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const
                 w.ue(inter_cbp_code(cbp));
                 if (cbp) {
                     w.se(0);  // mb_qp_delta: fixed QP
-                    for (int i8 = 0; i8 < 4; i8++)
-                        for (int k = 0; k < 4; k++) {
-                            const int b = i8 * 4 + k, x = 4 * mb_x + blk_x(b), y = 4 * mb_y + blk_y(b);
-                            if (!(cbpl & (1 << i8))) {
-                                lg.at(x, y) = 0;
-                                continue;
-                            }
-                            const int16_t *zz = lv + kLuma + 16 * b;
-                            cavlc_block(w, zz, 16, lg.nc(x, y));
-                            lg.at(x, y) = (uint8_t)count_nz(zz, 16);
-                        }
+                    write_luma4x4(w, lg, mb_x, mb_y, cbpl, lv);
                     write_chroma(w, cg, mb_x, mb_y, cbpc, lv);
                 }
             } else if (d.mb_type == RGH_MB_I16X16) {
                 if (d.i16_mode > 3 || d.chroma_mode > 3 || (cbpl != 0 && cbpl != 15)) return RGH_ERR_INVALID_ARGUMENT;
                 w.ue(5u + 1u + d.i16_mode + 4u * (uint32_t)cbpc + (cbpl ? 12u : 0u));
-                w.ue(d.chroma_mode);
-                w.se(0);  // mb_qp_delta: fixed QP
-                cavlc_block(w, lv + kLumaDc, 16, lg.nc(4 * mb_x, 4 * mb_y));
-                if (cbpl)
-                    for (int b = 0; b < 16; b++) {
-                        const int x = 4 * mb_x + blk_x(b), y = 4 * mb_y + blk_y(b);
-                        const int16_t *ac = lv + kLuma + 16 * b + 1;
-                        cavlc_block(w, ac, 15, lg.nc(x, y));
-                        lg.at(x, y) = (uint8_t)count_nz(ac, 15);
-                    }
-                write_chroma(w, cg, mb_x, mb_y, cbpc, lv);
+                write_i16_body(w, lg, cg, mb_x, mb_y, d, lv);
             } else {
                 return RGH_ERR_UNSUPPORTED;  // I_4x4 / I_PCM are not produced by the P analysis
             }
         }
     if (skip_run) w.ue(skip_run);
     w.trailing();
-    // NAL unit (enc.rs:2643): start code, nal_ref_idc 3 / type 1, emulation prevention.
-    const std::vector<uint8_t> &rbsp = w.bytes();
     std::vector<uint8_t> nal;
-    nal.reserve(rbsp.size() + rbsp.size() / 64 + 8);
-    nal.insert(nal.end(), {0, 0, 0, 1, (uint8_t)((3 << 5) | 1)});
-    int zeros = 0;
-    for (uint8_t b : rbsp) {
-        if (zeros == 2 && b <= 3) {
-            nal.push_back(3);
-            zeros = 0;
-        }
-        nal.push_back(b);
-        zeros = b == 0 ? zeros + 1 : 0;
+    emit_nal(nal, 3, 1, w.bytes());
+    return deliver(nal, out, cap, out_len);
+}
+
+extern "C" rgh_status rgh_write_idr_slice_cavlc(const rgh_slice_config *cfg, int32_t idr_pic_id,
+                                                const rgh_mb_decision *decisions, const int16_t *levels, uint8_t *out,
+                                                int64_t cap, int64_t *out_len) {
+    if (!cfg || !decisions || !levels || !out || !out_len || cap < 0 || idr_pic_id < 0) return RGH_ERR_INVALID_ARGUMENT;
+    const int mb_w = cfg->mb_w, mb_h = cfg->mb_h;
+    if (mb_w <= 0 || mb_h <= 0 || mb_w > 512 || mb_h > 512) return RGH_ERR_INVALID_DIMENSIONS;
+    if (cfg->qp < 0 || cfg->qp > 51) return RGH_ERR_INVALID_ARGUMENT;
+    BitWriter w;
+    // Slice header of an IDR picture: first_mb_in_slice, slice_type I (7), pps id, frame_num 0,
+    // idr_pic_id, pic_order_cnt_lsb 0, no_output_of_prior_pics / long_term_reference flags,
+    // slice_qp_delta, disable_deblocking_filter_idc (+ offsets).
+    w.ue(0);
+    w.ue(7);
+    w.ue(0);
+    w.put(0, 4);
+    w.ue((uint32_t)idr_pic_id);
+    w.put(0, 6);
+    w.put(0, 1);
+    w.put(0, 1);
+    w.se(0);
+    const int disable = cfg->qp == 0 ? 1 : 0;
+    w.ue((uint32_t)disable);
+    if (disable != 1) {
+        w.se(0);
+        w.se(0);
     }
-    *out_len = (int64_t)nal.size();
-    if ((int64_t)nal.size() > cap) return RGH_ERR_INVALID_ARGUMENT;
-    memcpy(out, nal.data(), nal.size());
-    return RGH_OK;
+    NnzGrid lg(4 * mb_w, 4 * mb_h);
+    NnzGrid cg[2] = {NnzGrid(2 * mb_w, 2 * mb_h), NnzGrid(2 * mb_w, 2 * mb_h)};
+    // Intra 4x4 prediction modes per 4x4 block (DC for I_16x16) for the predicted mode.
+    std::vector<uint8_t> modes((size_t)16 * mb_w * mb_h, 2);
+    const int gs = 4 * mb_w;
+    for (int mb_y = 0; mb_y < mb_h; mb_y++)
+        for (int mb_x = 0; mb_x < mb_w; mb_x++) {
+            const int idx = mb_y * mb_w + mb_x;
+            const rgh_mb_decision &d = decisions[idx];
+            const int16_t *lv = levels + (size_t)idx * RGH_LEVELS_PER_MB;
+            if (d.qp != cfg->qp || d.transform_8x8 || d.cbp_chroma > 2 || d.chroma_mode > 3) return RGH_ERR_UNSUPPORTED;
+            if (d.mb_type == RGH_MB_I4X4) {
+                const int cbpl = d.cbp_luma, cbp = cbpl | (d.cbp_chroma << 4);
+                uint64_t packed;
+                memcpy(&packed, d.mv, sizeof(packed));
+                w.ue(0);
+                for (int b = 0; b < 16; b++) {
+                    const int gx = 4 * mb_x + blk_x(b), gy = 4 * mb_y + blk_y(b);
+                    const int m = (int)((packed >> (4 * b)) & 15);
+                    if (m > 8) return RGH_ERR_INVALID_ARGUMENT;
+                    // 8.3.1.1: the predicted mode is the smaller of the left and top modes, DC at
+                    // the picture's edges.
+                    int pm = 2;
+                    if (gx > 0 && gy > 0) {
+                        const int a = modes[(size_t)gy * gs + gx - 1], t = modes[(size_t)(gy - 1) * gs + gx];
+                        pm = a < t ? a : t;
+                    }
+                    if (m == pm) {
+                        w.put(1, 1);
+                    } else {
+                        w.put(0, 1);
+                        w.put((uint32_t)(m < pm ? m : m - 1), 3);
+                    }
+                    modes[(size_t)gy * gs + gx] = (uint8_t)m;
+                }
+                w.ue(d.chroma_mode);
+                w.ue(intra4_cbp_code(cbp));
+                if (cbp) {
+                    w.se(0);  // mb_qp_delta: fixed QP
+                    write_luma4x4(w, lg, mb_x, mb_y, cbpl, lv);
+                    write_chroma(w, cg, mb_x, mb_y, d.cbp_chroma, lv);
+                }
+            } else if (d.mb_type == RGH_MB_I16X16) {
+                const int cbpl = d.cbp_luma;
+                if (d.i16_mode > 3 || (cbpl != 0 && cbpl != 15)) return RGH_ERR_INVALID_ARGUMENT;
+                w.ue(1u + d.i16_mode + 4u * (uint32_t)d.cbp_chroma + (cbpl ? 12u : 0u));
+                write_i16_body(w, lg, cg, mb_x, mb_y, d, lv);
+            } else {
+                return RGH_ERR_UNSUPPORTED;
+            }
+        }
+    w.trailing();
+    std::vector<uint8_t> nal;
+    emit_nal(nal, 3, 5, w.bytes());
+    return deliver(nal, out, cap, out_len);
+}
+
+extern "C" rgh_status rgh_write_headers(const rgh_stream_config *cfg, uint8_t *out, int64_t cap, int64_t *out_len) {
+    if (!cfg || !out || !out_len || cap < 0) return RGH_ERR_INVALID_ARGUMENT;
+    if (cfg->width <= 0 || cfg->height <= 0 || (cfg->width & 1) || (cfg->height & 1) || cfg->width > 8192 ||
+        cfg->height > 8192)
+        return RGH_ERR_INVALID_DIMENSIONS;
+    if (cfg->profile < RGH_PROFILE_BASELINE || cfg->profile > RGH_PROFILE_HIGH || cfg->qp < 0 || cfg->qp > 51 ||
+        cfg->fps_num <= 0 || cfg->fps_den <= 0)
+        return RGH_ERR_INVALID_ARGUMENT;
+    const int mb_w = (cfg->width + 15) / 16, mb_h = (cfg->height + 15) / 16;
+    const int profile_idc = cfg->profile == RGH_PROFILE_BASELINE ? 66 : (cfg->profile == RGH_PROFILE_MAIN ? 77 : 100);
+    std::vector<uint8_t> nal;
+    {
+        // h264.rs:556 Sps::to_rbsp: level 4.0, poc type 0, 2 reference frames, cropping to the
+        // display size, VUI with square pixels, limited-range BT.709 and the frame rate.
+        BitWriter w;
+        w.put((uint32_t)profile_idc, 8);
+        w.put(cfg->profile == RGH_PROFILE_BASELINE ? 0xC0u : (cfg->profile == RGH_PROFILE_MAIN ? 0x40u : 0x00u), 8);
+        w.put(40, 8);
+        w.ue(0);
+        if (profile_idc == 100) {
+            w.ue(1);  // chroma_format_idc 4:2:0
+            w.ue(0);
+            w.ue(0);
+            w.put(0, 1);
+            w.put(0, 1);
+        }
+        w.ue(0);  // log2_max_frame_num - 4
+        w.ue(0);  // pic_order_cnt_type
+        w.ue(2);  // log2_max_pic_order_cnt_lsb - 4
+        w.ue(2);  // max_num_ref_frames
+        w.put(0, 1);
+        w.ue((uint32_t)(mb_w - 1));
+        w.ue((uint32_t)(mb_h - 1));
+        w.put(1, 1);  // frame_mbs_only
+        w.put(1, 1);  // direct_8x8_inference
+        const int crop_r = (mb_w * 16 - cfg->width) / 2, crop_b = (mb_h * 16 - cfg->height) / 2;
+        w.put(crop_r || crop_b ? 1u : 0u, 1);
+        if (crop_r || crop_b) {
+            w.ue(0);
+            w.ue((uint32_t)crop_r);
+            w.ue(0);
+            w.ue((uint32_t)crop_b);
+        }
+        w.put(1, 1);  // vui_parameters_present
+        w.put(1, 1);  // aspect_ratio_info_present: 1:1
+        w.put(1, 8);
+        w.put(0, 1);  // overscan_info_present
+        w.put(1, 1);  // video_signal_type_present: unspecified format, limited range, BT.709
+        w.put(5, 3);
+        w.put(0, 1);
+        w.put(1, 1);
+        w.put(1, 8);
+        w.put(1, 8);
+        w.put(1, 8);
+        w.put(0, 1);  // chroma_loc_info_present
+        w.put(1, 1);  // timing_info_present
+        w.put((uint32_t)cfg->fps_den, 32);
+        w.put((uint32_t)cfg->fps_num * 2u, 32);
+        w.put(1, 1);  // fixed_frame_rate
+        w.put(0, 1);  // nal_hrd
+        w.put(0, 1);  // vcl_hrd
+        w.put(0, 1);  // pic_struct_present
+        w.put(0, 1);  // bitstream_restriction
+        w.trailing();
+        emit_nal(nal, 3, 7, w.bytes());
+    }
+    {
+        // h264.rs:806 Pps::to_rbsp: CABAC for Main/High, initial QP the stream's, deblocking
+        // control present, 8x8 transform for High.
+        BitWriter w;
+        w.ue(0);
+        w.ue(0);
+        w.put(cfg->profile != RGH_PROFILE_BASELINE ? 1u : 0u, 1);
+        w.put(0, 1);
+        w.ue(0);
+        w.ue(0);
+        w.ue(0);
+        w.put(0, 1);
+        w.put(0, 2);
+        w.se(cfg->qp - 26);
+        w.se(0);
+        w.se(0);
+        w.put(1, 1);
+        w.put(0, 1);
+        w.put(0, 1);
+        if (cfg->profile == RGH_PROFILE_HIGH) {
+            w.put(1, 1);
+            w.put(0, 1);
+            w.se(0);
+        }
+        w.trailing();
+        emit_nal(nal, 3, 8, w.bytes());
+    }
+    return deliver(nal, out, cap, out_len);
 }

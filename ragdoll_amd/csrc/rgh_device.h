@@ -43,6 +43,7 @@ struct RghJob {
     uint32_t sb_luma, sb_luma_ssd, sb_chroma, sb_chroma_ssd, sb_probe;
     int32_t steps;             // wavefront steps of the frame: (mb_w - 1) + 2 (mb_h - 1) + 1
     int32_t deblock;           // 0 when the slice disables the loop filter
+    int32_t intra;             // an IDR picture (no reference)
     // Schedule.
     int32_t base;              // launch of the frame's first wavefront step
     uint32_t serial, ref_serial;  // this frame's and its reference's serial numbers
@@ -71,7 +72,7 @@ struct RghJobD {
     const RGH_G rgh_mb_decision *prev;
     RGH_G int16_t *levels;
     uint32_t sb_luma, sb_luma_ssd, sb_chroma, sb_chroma_ssd, sb_probe;
-    int32_t steps, deblock, base;
+    int32_t steps, deblock, intra, base;
     uint32_t serial, ref_serial;
     RGH_G uint64_t *ana;
     RGH_G uint64_t *fin;
@@ -123,6 +124,7 @@ __device__ __forceinline__ RghJobD device_view(const RghJob &j) {
     d.sb_probe = j.sb_probe;
     d.steps = j.steps;
     d.deblock = j.deblock;
+    d.intra = j.intra;
     d.base = j.base;
     d.serial = j.serial;
     d.ref_serial = j.ref_serial;

@@ -94,6 +94,13 @@ class OracleEncoder:
         assert n >= 0
         return bytes(out[:n])
 
+    def headers(self):
+        """SPS and PPS NAL units (Annex B) as the oracle writes them before an IDR."""
+        out = np.empty(1024, np.uint8)
+        n = self.L.ora_headers(self.h, ptr(out), 1024)
+        assert n > 0
+        return bytes(out[:n])
+
     @property
     def last_was_idr(self):
         return bool(self.L.ora_last_was_idr(self.h))

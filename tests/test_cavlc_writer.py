@@ -1,4 +1,5 @@
-"""The library's CAVLC slice writer (rgh_write_p_slice_cavlc) against the reference's bytes.
+"""The library's CAVLC writers (rgh_write_p_slice_cavlc, rgh_write_idr_slice_cavlc,
+rgh_write_headers) against the reference's bytes.
 
 Decisions and levels from the oracle are entropy-coded by the product writer; the slice NAL
 units must equal the reference's committed golden clips (qa/corpus/synth/*, extracted into
@@ -12,7 +13,7 @@ import numpy as np
 import pytest
 
 from oracle_lib import OracleEncoder, read_nals, split_annexb
-from ragdoll_amd import write_p_slice_cavlc
+from ragdoll_amd import write_headers, write_idr_slice_cavlc, write_p_slice_cavlc
 from ragdoll_amd.synth import CONFIGS, SceneGenerator, config_seed, fixture_frame
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -73,3 +74,46 @@ def test_writer_rejects_cabac_records():
     from ragdoll_amd import RghError
     with pytest.raises(RghError):
         write_p_slice_cavlc(dec, ora.levels(), ora.mb_w, ora.mb_h, 26, 1, 2)
+
+
+@pytest.mark.parametrize("name", sorted(META))
+def test_writer_reproduces_golden_parameter_sets(name):
+    m = META[name]
+    gold = read_nals(os.path.join(HERE, "golden", name + ".nals"))
+    units = split_annexb(write_headers(m["width"], m["height"], m["qp"], m["profile"], fps=(m["fps"], 1)))
+    assert units == gold[:m["n_param_sets"]]
+
+
+@pytest.mark.parametrize("name", sorted(META))
+def test_writer_reproduces_golden_idr_slices(name):
+    m = META[name]
+    gold = read_nals(os.path.join(HERE, "golden", name + ".nals"))
+    slices = gold[m["n_param_sets"]:]
+    for g0 in range(0, m["frames"], m["gop"]):
+        ora = OracleEncoder(m["width"], m["height"], m["qp"], fps=(m["fps"], 1), keyint=m["gop"] + 1)
+        ora.encode(*fixture_frame(m["width"], m["height"], g0))
+        assert ora.last_was_idr
+        dec = ora.decisions()
+        assert (dec["mb_type"] == 3).any(), "the clip's IDR codes intra 4x4 macroblocks"
+        units = split_annexb(write_idr_slice_cavlc(dec, ora.levels(), ora.mb_w, ora.mb_h, ora.qp, 0))
+        assert units == [slices[g0]], "%s frame %d" % (name, g0)
+
+
+@pytest.mark.parametrize("qp", [1, 2, 12, 26, 40, 51])
+@pytest.mark.parametrize("profile", ["baseline", "main", "high"])
+def test_writer_matches_oracle_idr_slices_and_headers(qp, profile):
+    kw = dict(CONFIGS["cif"])
+    w, h = kw.pop("width"), kw.pop("height")
+    w, h = w - 6, h - 10  # cropping in the SPS
+    g = SceneGenerator(w, h, seed=config_seed(1, 200 + qp), **kw)
+    ora = OracleEncoder(w, h, qp, profile=profile, keyint=2, fps=(24000, 1001))
+    for k in range(3):
+        nals = split_annexb(ora.encode(*g.next_frame()))
+        if k == 1:
+            continue
+        assert write_headers(w, h, qp, profile, fps=(24000, 1001)) == ora.headers()
+        if profile != "baseline":
+            continue  # CABAC slices: not written by this library
+        assert split_annexb(ora.headers()) == nals[:2]
+        got = split_annexb(write_idr_slice_cavlc(ora.decisions(), ora.levels(), ora.mb_w, ora.mb_h, qp, k // 2))
+        assert got == nals[2:], "qp %d frame %d" % (qp, k)
