@@ -3,14 +3,13 @@
 Workload (the plain GPU form at scale): 1920x1080 I420, 120 frames of seeded synthetic content
 (20 objects, |v| <= 32 px/frame, 10% scene cuts, noise sigma 3), High profile decisions
 (adaptive 8x8 transform), QP 24/30/36, keyint 30: 3 QPs x 4 GOPs = 12 independent GOP streams,
-each an IDR then 29 P frames. A step is the whole job: the 29 P frames of all 12 streams (348
-frames, 2,839,680 macroblocks) analysed by one pipelined schedule (rgh_analyse_frames): every
-macroblock's decision, quantized levels and reconstruction, and the deblocking and border
-extension that make each next reference, with up to ~13 frames of a stream in flight. Sources
-are resident in HBM before the timed region; decisions and levels stay in HBM.
-
-The IDR frames are not coded by this path: each GOP's first reference is its IDR source frame,
-installed with rgh_set_reference before each step (outside the timed region).
+each an IDR then 29 P frames. A step is the whole job: all 30 frames of the 12 streams (12 IDR
+and 348 P frames) analysed by one pipelined schedule (rgh_analyse_frames with each stream's first
+frame its IDR): every macroblock's decision, quantized levels and reconstruction, and the
+deblocking and border extension that make each next reference, with up to ~13 frames of a
+stream in flight. Sources are resident in HBM before the timed region; decisions and levels
+stay in HBM. The value counts the P ("ME") macroblocks, 2,839,680 per step, over the time of
+the whole job, IDR pictures included.
 
 The schedule's frame lag (wavefront steps between consecutive frames of a stream) and its number
 of concurrent groups (schedules on separate HIP streams) are picked by a small autotuner during
@@ -36,21 +35,46 @@ sys.path.insert(0, ROOT)
 from ragdoll_amd.hip import Hip  # noqa: E402
 
 MB_BYTES = 1184  # algorithmic HBM bytes per P macroblock: 384 source + 384 reference + 384 recon + 32 decision
+IDR_MB_BYTES = 800  # per IDR macroblock: no reference read
 HBM_PEAK_GBS = 8000.0
+# Integer VALU peak of MI355X: 256 CUs x 4 SIMDs x 16 lanes per clock at 2.4 GHz = 39.3 T lane
+# operations per second (one wave64 VALU instruction per CU per clock).
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 GOP = 30
 QPS = (24, 30, 36)
 W, H = 1920, 1080
 
 
-def measured_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` on this workload, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE runs (profiles/traffic.json, copy-calibrated), or None."""
+def profile_record(kernel):
+    """The committed rocprofv3 counter figures of `kernel` on this workload
+    (profiles/traffic.json), or None."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            k = json.load(f)["kernels"][kernel]
-        return round(k["fetch_bytes"] + k["write_bytes"])
+            return json.load(f)["kernels"][kernel]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` (FETCH_SIZE / WRITE_SIZE, copy-calibrated), or None."""
+    k = profile_record(kernel)
+    if not k or "fetch_bytes" not in k:
+        return None
+    return round(k["fetch_bytes"] + k["write_bytes"])
+
+
+def valu_figure(kernel, mbs_per_launch, avg_launch_s):
+    """Integer VALU throughput of `kernel`: the counted INT32 + INT64 VALU instructions per
+    macroblock (SQ_INSTS_VALU_INT32 / _INT64 over the macroblocks of the profiled run, every
+    deferred attempt included) x 64 lanes x macroblocks per launch / launch time, against the
+    integer VALU peak; None without the committed counters."""
+    k = profile_record(kernel)
+    if not k or "valu_int_per_mb" not in k:
+        return None
+    tops = k["valu_int_per_mb"] * 64 * mbs_per_launch / avg_launch_s / 1e12
+    return {"achieved": round(tops, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s (int32 lane ops)",
+            "frac": tops / VALU_PEAK_TOPS, "valu_insts_per_mb": k.get("valu_per_mb"),
+            "valu_int_insts_per_mb": k["valu_int_per_mb"], "valu_issue_util": k.get("valu_issue_util")}
 
 
 def calibrate_copy(mib, repeats=4):
@@ -82,10 +106,10 @@ def render_frames(n_frames):
 
 def cpu_baseline(frames, seconds=12.0):
     """The CPU oracle (the reference's Balanced encoder restated in C, built -O3) on a bounded
-    sample of the same workload: the P frames of the 12 GOP streams on the host cores, one
-    thread per stream (like the reference's parallel_gop_map), the streams repeated until
-    `seconds` pass. Each thread times only its P frames (its IDR encodes are setup); the value
-    is the sum of the threads' P-frame rates."""
+    sample of the same workload: the 12 GOP streams on the host cores, one thread per stream
+    (like the reference's parallel_gop_map), the streams repeated until `seconds` pass. Each
+    thread times its IDR and P frames; the value is the sum of the threads' P-macroblock rates
+    over that time (the GPU figure's definition)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import OracleEncoder
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
@@ -100,14 +124,13 @@ def cpu_baseline(frames, seconds=12.0):
             for s in range(k, len(streams), threads):
                 qp, g = streams[s]
                 enc = OracleEncoder(W, H, qp, profile="high", keyint=1 << 20)
-                enc.encode(*frames[g * GOP])
-                for i in range(1, GOP):
+                for i in range(GOP):
                     t = time.perf_counter()
                     if t > deadline:
                         return
                     enc.encode(*frames[g * GOP + i])
                     busy[k] += time.perf_counter() - t
-                    done[k] += 1
+                    done[k] += i > 0
 
     t0 = time.perf_counter()
     ts = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
@@ -119,9 +142,10 @@ def cpu_baseline(frames, seconds=12.0):
     mbf = (W // 16) * ((H + 15) // 16)
     value = sum(done[k] * mbf / busy[k] for k in range(threads) if busy[k] > 0)
     return {"value": value, "unit": "macroblocks/s", "cores": threads, "kind": "port",
-            "sample": "%d P frames of the 12 streams (1080p, High decisions, QP 24/30/36), one thread per stream, "
-                      "%.1f s wall, by the C oracle (Balanced encoder restated: P analysis, reconstruction, "
-                      "deblocking; no entropy coding for High)" % (sum(done), el)}
+            "sample": "%d P frames (and their GOPs' IDR frames) of the 12 streams (1080p, High decisions, "
+                      "QP 24/30/36), one thread per stream, %.1f s wall, by the C oracle (Balanced encoder "
+                      "restated: IDR and P analysis, reconstruction, deblocking; no entropy coding for High)"
+                      % (sum(done), el)}
 
 
 def main():
@@ -180,8 +204,8 @@ def batched_gops():
 
 
 def run_workload(gops, args):
-    """Times the P frames of every GOP (an IDR installed as reference, then GOP - 1 P frames)
-    analysed by one rgh_analyse_frames call per step; returns the JSON record."""
+    """Times every GOP (its IDR, then GOP - 1 P frames) analysed by one rgh_analyse_frames call
+    per step; returns the JSON record."""
     from ragdoll_amd import LEVELS_PER_MB, StreamAnalyzer, analyse_frames
 
     hip = Hip()
@@ -192,28 +216,25 @@ def run_workload(gops, args):
             uploaded[id(p)] = (hip.upload(p), p)
         return uploaded[id(p)][0]
 
-    ans, idr, srcs, decs, lvls = [], [], [], [], []
+    ans, srcs, decs, lvls = [], [], [], []
     for gp in gops:
         a = StreamAnalyzer(gp["w"], gp["h"], gp["qp"], profile="high", me_range=gp["me"], keep_undeblocked=False,
-                           ring=len(gp["frames"]))
+                           ring=len(gp["frames"]) + 1)
         ans.append(a)
-        W16, H16 = a.mb_w * 16, a.mb_h * 16
-        pad = lambda p, ww, hh: np.pad(p, ((0, hh - p.shape[0]), (0, ww - p.shape[1])), mode="edge")
-        idr.append(tuple(pad(p, W16 >> (k > 0), H16 >> (k > 0)) for k, p in enumerate(gp["frames"][0])))
-        srcs.append([(dev(y), dev(u), dev(v), y.shape[1], u.shape[1], v.shape[1]) for y, u, v in gp["frames"][1:]])
+        srcs.append([(dev(y), dev(u), dev(v), y.shape[1], u.shape[1], v.shape[1]) for y, u, v in gp["frames"]])
         n = a.mb_w * a.mb_h
-        decs.append([hip.alloc(n * 32) for _ in gp["frames"][1:]])
-        lvls.append([hip.alloc(n * LEVELS_PER_MB * 2) for _ in gp["frames"][1:]])
+        decs.append([hip.alloc(n * 32) for _ in gp["frames"]])
+        lvls.append([hip.alloc(n * LEVELS_PER_MB * 2) for _ in gp["frames"]])
     ans[0].enable_timing(True)
     hip.synchronize()
     total_mbs = sum(a.mb_w * a.mb_h * (len(gp["frames"]) - 1) for a, gp in zip(ans, gops))
+    idr_mbs = sum(a.mb_w * a.mb_h for a in ans)
+    step_bytes = total_mbs * MB_BYTES + idr_mbs * IDR_MB_BYTES
 
     def step(sched):
         lag, groups = sched
-        for a, ref in zip(ans, idr):
-            a.set_reference(*ref)
         t = time.perf_counter()
-        analyse_frames(ans, srcs, decs, lvls, lag=lag, groups=groups)
+        analyse_frames(ans, srcs, decs, lvls, lag=lag, groups=groups, first_idr=True)
         el = time.perf_counter() - t
         ms, _, launches = ans[0].timing()
         return el, ms, launches
@@ -238,8 +259,8 @@ def run_workload(gops, args):
 
     value = total_mbs * args.steps / el
     avg_launch_s = dev_ms / 1e3 / launches
-    mbs_per_launch = total_mbs * args.steps / launches
-    achieved = MB_BYTES * mbs_per_launch / avg_launch_s / 1e9
+    mbs_per_launch = (total_mbs + idr_mbs) * args.steps / launches
+    achieved = step_bytes * args.steps / launches / avg_launch_s / 1e9
     headline = args.workload != "batched"
     out = {
         "metric": "1080p ME macroblocks/sec, 1 GPU: plain vs learned-proposal at <=1% BD-rate" if headline else
@@ -255,13 +276,16 @@ def run_workload(gops, args):
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": "plain GPU form: 1920x1080 P-frame analysis, High decisions, QP 24/30/36, "
-                               "keyint 30, 12 GOP streams x 29 P frames per step in one pipelined schedule, "
-                               "bit-exact with the CPU oracle" if headline else
+        "config": {"workload": "plain GPU form: 1920x1080 analysis, High decisions, QP 24/30/36, keyint 30, "
+                               "12 GOP streams x (1 IDR + 29 P frames) per step in one pipelined schedule, "
+                               "bit-exact with the CPU oracle; value counts P macroblocks" if headline else
                                "plain GPU form: 24 streams (8 x 640x360, 8 x 1280x720, 8 x 1920x1080), 60 frames "
                                "each (2 GOPs, keyint 30), seeded QP 22-38, search range scaled by width, High "
-                               "decisions, all 48 GOPs' P frames in one pipelined schedule",
+                               "decisions, all 48 GOPs (IDR + P frames) in one pipelined schedule; value counts "
+                               "P macroblocks",
                    "form": "plain", "gops": len(gops), "p_macroblocks_per_step": total_mbs,
+                   "idr_macroblocks_per_step": idr_mbs,
+                   "all_macroblocks_per_s": round((total_mbs + idr_mbs) * args.steps / el, 1),
                    "lag": sched[0], "groups": sched[1],
                    "schedule_tuning_s": {"%d/%d" % k: round(v, 4) for k, v in tune.items()},
                    "device_ms_per_step": round(dev_ms / args.steps, 3)},
@@ -269,8 +293,11 @@ def run_workload(gops, args):
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic("rgh_k_frame_step") if headline else None,
                      "kernel": "rgh_k_frame_step", "avg_launch_us": round(avg_launch_s * 1e6, 3),
-                     "mbs_per_launch": round(mbs_per_launch, 2), "bytes_per_mb": MB_BYTES},
+                     "mbs_per_launch": round(mbs_per_launch, 2), "bytes_per_p_mb": MB_BYTES,
+                     "bytes_per_idr_mb": IDR_MB_BYTES},
     }
+    if headline:
+        out["valu"] = valu_figure("rgh_k_frame_step", mbs_per_launch, avg_launch_s)
     for a in ans:
         a.close()
     for p in [q for q, _ in uploaded.values()] + [q for d in decs for q in d] + [q for l in lvls for q in l]:

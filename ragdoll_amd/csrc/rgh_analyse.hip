@@ -1588,98 +1588,18 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
 constexpr int CV_W = 24;  // canvas: row 0 = the top edge, rows 1-16 the macroblock; column 3 = the
                           // left edge, 4-19 the macroblock, 20-23 the top-right edge
 
-// One 4x4 intra prediction (8.3.1.2, enc.rs:1489 pred4): t[0..7] the row above (with the
-// top-right samples), l[0..3] the column left, m the corner; unavailable edges are 128.
-__device__ __forceinline__ void pred4x4(int mode, const int *t, const int *l, int m, bool has_t, bool has_l, int *p) {
-    // Edge sample p[x, -1] (x = -1: the corner) and p[-1, y].
-    auto T = [&](int x) { return x < 0 ? m : t[x]; };
-    auto L = [&](int y) { return y < 0 ? m : l[y]; };
-#pragma unroll
-    for (int y = 0; y < 4; y++)
-#pragma unroll
-        for (int x = 0; x < 4; x++) {
-            int v;
-            switch (mode) {
-            case 0: v = t[x]; break;
-            case 1: v = l[y]; break;
-            case 2: {
-                const int st = t[0] + t[1] + t[2] + t[3], sl = l[0] + l[1] + l[2] + l[3];
-                v = has_t && has_l ? (st + sl + 4) >> 3 : (has_t ? (st + 2) >> 2 : (has_l ? (sl + 2) >> 2 : 128));
-            } break;
-            case 3:  // diagonal down left
-                v = (x == 3 && y == 3) ? (t[6] + 3 * t[7] + 2) >> 2 : (t[x + y] + 2 * t[x + y + 1] + t[x + y + 2] + 2) >> 2;
-                break;
-            case 4:  // diagonal down right
-                v = x > y ? (T(x - y - 2) + 2 * T(x - y - 1) + T(x - y) + 2) >> 2
-                          : (x < y ? (L(y - x - 2) + 2 * L(y - x - 1) + L(y - x) + 2) >> 2 : (t[0] + 2 * m + l[0] + 2) >> 2);
-                break;
-            case 5: {  // vertical right
-                const int z = 2 * x - y, b = x - (y >> 1);
-                v = z >= 0 ? ((z & 1) ? (T(b - 2) + 2 * T(b - 1) + T(b) + 2) >> 2 : (T(b - 1) + T(b) + 1) >> 1)
-                           : (z == -1 ? (l[0] + 2 * m + t[0] + 2) >> 2 : (L(y - 1) + 2 * L(y - 2) + L(y - 3) + 2) >> 2);
-            } break;
-            case 6: {  // horizontal down
-                const int z = 2 * y - x, b = y - (x >> 1);
-                v = z >= 0 ? ((z & 1) ? (L(b - 2) + 2 * L(b - 1) + L(b) + 2) >> 2 : (L(b - 1) + L(b) + 1) >> 1)
-                           : (z == -1 ? (l[0] + 2 * m + t[0] + 2) >> 2 : (T(x - 1) + 2 * T(x - 2) + T(x - 3) + 2) >> 2);
-            } break;
-            case 7: {  // vertical left
-                const int b = x + (y >> 1);
-                v = (y & 1) ? (t[b] + 2 * t[b + 1] + t[b + 2] + 2) >> 2 : (t[b] + t[b + 1] + 1) >> 1;
-            } break;
-            default: {  // horizontal up
-                const int z = x + 2 * y, b = y + (x >> 1);
-                v = z > 5 ? l[3]
-                          : (z == 5 ? (l[2] + 3 * l[3] + 2) >> 2
-                                    : ((z & 1) ? (l[b] + 2 * l[b + 1] + l[b + 2 < 4 ? b + 2 : 3] + 2) >> 2 : (l[b] + l[b + 1] + 1) >> 1));
-            } break;
-            }
-            p[y * 4 + x] = v;
-        }
-}
-
-// 4x4 SATD (sum of the Hadamard coefficients of the difference, halved; simd/h264_enc.rs:196).
-__device__ __forceinline__ uint32_t satd4x4(const int *d) {
-    int t[16];
-#pragma unroll
-    for (int y = 0; y < 4; y++) {
-        const int s03 = d[y * 4] + d[y * 4 + 3], s12 = d[y * 4 + 1] + d[y * 4 + 2];
-        const int d03 = d[y * 4] - d[y * 4 + 3], d12 = d[y * 4 + 1] - d[y * 4 + 2];
-        t[y * 4] = s03 + s12;
-        t[y * 4 + 1] = d03 + d12;
-        t[y * 4 + 2] = s03 - s12;
-        t[y * 4 + 3] = d03 - d12;
-    }
-    uint32_t sum = 0;
-#pragma unroll
-    for (int x = 0; x < 4; x++) {
-        const int s03 = t[x] + t[12 + x], s12 = t[4 + x] + t[8 + x], d03 = t[x] - t[12 + x], d12 = t[4 + x] - t[8 + x];
-        sum += (uint32_t)(iabs(s03 + s12) + iabs(d03 + d12) + iabs(s03 - s12) + iabs(d03 - d12));
-    }
-    return sum >> 1;
-}
-
-// Forward 4x4 integer transform of a residual held in registers (the transform of sub_fdct4).
-__device__ __forceinline__ void fdct4_regs(const int *d, int *out) {
-    int t[16];
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int s03 = d[r * 4] + d[r * 4 + 3], s12 = d[r * 4 + 1] + d[r * 4 + 2];
-        const int d03 = d[r * 4] - d[r * 4 + 3], d12 = d[r * 4 + 1] - d[r * 4 + 2];
-        t[r * 4] = s03 + s12;
-        t[r * 4 + 1] = (d03 << 1) + d12;
-        t[r * 4 + 2] = s03 - s12;
-        t[r * 4 + 3] = d03 - (d12 << 1);
-    }
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int s03 = t[c] + t[c + 12], s12 = t[c + 4] + t[c + 8], d03 = t[c] - t[c + 12], d12 = t[c + 4] - t[c + 8];
-        out[c] = (int16_t)(s03 + s12);
-        out[c + 4] = (int16_t)((d03 << 1) + d12);
-        out[c + 8] = (int16_t)(s03 - s12);
-        out[c + 12] = (int16_t)(d03 - (d12 << 1));
-    }
-}
+// Intra 4x4 prediction taps (8.3.1.2, enc.rs:1489 pred4), for the lane of mode m = lane / 4 and
+// row y = lane % 4: one byte per pixel x, (kind << 5) | i over the edge line G, where G[0..3] =
+// p[-1, 3..0], G[4] = p[-1, -1], G[5..12] = p[0..7, -1] (the top-right substituted), G[13] =
+// p[7, -1], G[14..21] = p[-1, 0..3] then p[-1, 3] repeated. Kinds: 0 G[i]; 1 (G[i] + G[i + 1] + 1)
+// >> 1; 2 (G[i - 1] + 2 G[i] + G[i + 1] + 2) >> 2; 3 the DC value. Every prediction of the
+// standard reduces to one of these per pixel.
+__constant__ uint32_t c_i4_taps[36] = {
+    0x08070605, 0x08070605, 0x08070605, 0x08070605, 0x03030303, 0x02020202, 0x01010101, 0x00000000,
+    0x60606060, 0x60606060, 0x60606060, 0x60606060, 0x49484746, 0x4a494847, 0x4b4a4948, 0x4c4b4a49,
+    0x47464544, 0x46454443, 0x45444342, 0x44434241, 0x27262524, 0x47464544, 0x26252443, 0x46454442,
+    0x46454423, 0x44234322, 0x43224221, 0x42214120, 0x28272625, 0x49484746, 0x29282726, 0x4a494847,
+    0x502f4f2e, 0x5130502f, 0x52315130, 0x53325231};
 
 __device__ __forceinline__ int intra4_available(int mode, bool has_t, bool has_l) {  // h264.rs:3273
     return (mode == 0 || mode == 3 || mode == 7) ? has_t : ((mode == 1 || mode == 8) ? has_l : (mode == 2 ? 1 : has_t && has_l));
@@ -1691,14 +1611,57 @@ __device__ __forceinline__ int mode_at(uint64_t packed, int bx, int by) {
     return (int)((packed >> (4 * b)) & 15);
 }
 
-// enc.rs:3333 analyse_i4 on the canvas (edges in row 0 / column 3): per block, every available
-// mode's SATD + lambda * (1 for the predicted mode, else 4) on its own lane, the first cheapest
-// kept, the block quantized (intra rounding) and reconstructed into the canvas. Returns false as
-// soon as the running cost (from 24 lambda) reaches budget. lmodes / tmodes: the left and top
-// macroblocks' packed modes (all DC when they are not I_4x4).
+// Value of row r (0..3) of the lane's quad (DPP quad broadcast).
+template <int R>
+__device__ __forceinline__ int quad_row(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, R * 0x55, 0xF, 0xF, false);
+}
+
+// Canvas offset, relative to the block's top-left edge sample (row py, column px), of edge-line
+// entry i (see c_i4_taps), with (tr) or without the top-right samples.
+__device__ __forceinline__ int g4_offset(int i, bool tr) {
+    if (i < 4) return (4 - i) * CV_W + 3;
+    if (i == 4) return 3;
+    if (i < 14) {
+        const int k = min(i - 5, 7);
+        return 4 + ((k < 4 || tr) ? k : 3);
+    }
+    return (1 + min(i - 14, 3)) * CV_W + 3;
+}
+
+// enc.rs:3333 analyse_i4 on the canvas (edges in row 0 / column 3), lane-parallel: per block (in
+// decoding order) lanes 4m..4m+3 predict rows 0-3 of mode m; the SATD (simd/h264_enc.rs:196:
+// sum of the Hadamard coefficients, halved) runs its column pass across the quad; cost = SATD +
+// lambda * (1 for the predicted mode — min(left, top), DC at the picture's edges — else 4); the
+// first cheapest available mode wins. Every quad also runs the forward transform (rows in the
+// lane, columns across the quad), the intra quantizer, the dequantizer and the inverse
+// transform (rows, then columns, then (v + 32) >> 6) before the winner is known; the winner's
+// levels are kept and its reconstruction written to the canvas before the next block. Returns
+// false as soon as the running cost (from 24 lambda) reaches budget. lmodes / tmodes: the left
+// and top macroblocks' packed modes (all DC when they are not I_4x4).
 __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint32_t lambda, uint32_t budget,
                            uint64_t lmodes, uint64_t tmodes, uint64_t &modes, int &cbpl) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x, m = lane >> 2, y = lane & 3;
+    const uint32_t taps = c_i4_taps[lane < 36 ? lane : 0];
+    // Canvas offsets of the three edge entries around each pixel's tap (bytes 3x .. 3x + 2),
+    // with and without the top-right samples.
+    uint32_t off_tr[3] = {0, 0, 0}, off_nt[3] = {0, 0, 0};
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+        const int i = (int)((taps >> (8 * x)) & 31);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int e = max(i - 1 + k, 0), slot = 3 * x + k;
+            off_tr[slot >> 2] |= (uint32_t)g4_offset(e, true) << (8 * (slot & 3));
+            off_nt[slot >> 2] |= (uint32_t)g4_offset(e, false) << (8 * (slot & 3));
+        }
+    }
+    // Quantizer and dequantizer of the lane's row: position classes 0 / 2 / 0 / 2 (even rows) or
+    // 2 / 1 / 2 / 1 (odd rows) across the columns.
+    const int qshift = 15 + qp / 6, qrem = qp % 6, dsh = qp / 6;
+    const uint32_t qbias = (1u << qshift) / 3u;
+    const uint32_t mfe = (uint32_t)c_quant4_mf[qrem][(y & 1) ? 2 : 0], mfo = (uint32_t)c_quant4_mf[qrem][(y & 1) ? 1 : 2];
+    const int dqe = c_dequant4_init[qrem][(y & 1) ? 2 : 0] << dsh, dqo = c_dequant4_init[qrem][(y & 1) ? 1 : 2] << dsh;
     uint32_t total = 24 * lambda;
     modes = 0;
     cbpl = 0;
@@ -1709,28 +1672,88 @@ __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint3
         const int lm = bx ? mode_at(modes, bx - 1, by) : mode_at(lmodes, 3, by);
         const int tm = by ? mode_at(modes, bx, by - 1) : mode_at(tmodes, bx, 3);
         const int pm = (mb_x * 4 + bx == 0 || mb_y * 4 + by == 0) ? 2 : min(lm, tm);
-        int p[16], d[16];
-        uint32_t cost = 0xFFFFFFFFu;
-        if (lane < 9 && intra4_available(lane, has_t, has_l)) {
-            int t[8], l[4];
+        const uint8_t *cb = &S.cv[py * CV_W + px];
+        int dc;
+        {
+            const uint32_t tw = *(const uint32_t *)(cb + 4);
+            const int st = (int)__builtin_amdgcn_sad_u8(tw, 0u, 0u);
+            const int sl = cb[CV_W + 3] + cb[2 * CV_W + 3] + cb[3 * CV_W + 3] + cb[4 * CV_W + 3];
+            dc = has_t && has_l ? (st + sl + 4) >> 3 : (has_t ? (st + 2) >> 2 : (has_l ? (sl + 2) >> 2 : 128));
+        }
+        // Prediction and residual of the lane's row.
+        const uint32_t srow = *(const uint32_t *)&S.src[(py + y) * 16 + px];
+        int p[4], d[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                t[i] = has_t ? S.cv[py * CV_W + px + 4 + i] : 128;
-                l[i] = has_l ? S.cv[(py + 1 + i) * CV_W + px + 3] : 128;
+        for (int x = 0; x < 4; x++) {
+            const int k = (int)((taps >> (8 * x + 5)) & 7);
+            int g[3];
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const int slot = 3 * x + j;
+                const uint32_t w = tr ? off_tr[slot >> 2] : off_nt[slot >> 2];
+                g[j] = cb[(w >> (8 * (slot & 3))) & 0xFF];
             }
+            const int v = k == 0 ? g[1] : (k == 1 ? (g[1] + g[2] + 1) >> 1 : (k == 2 ? (g[0] + 2 * g[1] + g[2] + 2) >> 2 : dc));
+            p[x] = v;
+            d[x] = (int)byte_of(srow, x) - v;
+        }
+        // SATD: row Hadamard in the lane, column butterflies across the quad.
+        uint32_t sa;
+        {
+            const int s03 = d[0] + d[3], s12 = d[1] + d[2], d03 = d[0] - d[3], d12 = d[1] - d[2];
+            int h[4] = {s03 + s12, d03 + d12, s03 - s12, d03 - d12};
+            int acc = 0;
 #pragma unroll
-            for (int i = 4; i < 8; i++) t[i] = has_t ? (tr ? S.cv[py * CV_W + px + 4 + i] : t[3]) : 128;
-            const int m = (has_t && has_l) ? S.cv[py * CV_W + px + 3] : 128;
-            pred4x4(lane, t, l, m, has_t, has_l, p);
+            for (int c = 0; c < 4; c++) {
+                const int q1 = __builtin_amdgcn_update_dpp(0, h[c], 0xB1, 0xF, 0xF, false);  // row y ^ 1
+                const int a = (y & 1) ? q1 - h[c] : h[c] + q1;
+                const int q2 = __builtin_amdgcn_update_dpp(0, a, 0x4E, 0xF, 0xF, false);     // row y ^ 2
+                acc += iabs((y & 2) ? q2 - a : a + q2);
+            }
+            acc += __builtin_amdgcn_update_dpp(0, acc, 0xB1, 0xF, 0xF, false);
+            acc += __builtin_amdgcn_update_dpp(0, acc, 0x4E, 0xF, 0xF, false);
+            sa = (uint32_t)acc >> 1;
+        }
+        const uint32_t cost = (lane < 36 && intra4_available(m, has_t, has_l)) ? sa + lambda * (m == pm ? 1u : 4u)
+                                                                                 : 0xFFFFFFFFu;
+        // Forward transform (rows, then columns across the quad) and intra quantization.
+        int lv[4];
+        {
+            const int s03 = d[0] + d[3], s12 = d[1] + d[2], d03 = d[0] - d[3], d12 = d[1] - d[2];
+            const int t[4] = {s03 + s12, (d03 << 1) + d12, s03 - s12, d03 - (d12 << 1)};
 #pragma unroll
-            for (int i = 0; i < 16; i++) d[i] = (int)S.src[(py + (i >> 2)) * 16 + px + (i & 3)] - p[i];
-            cost = satd4x4(d) + lambda * (lane == pm ? 1u : 4u);
+            for (int c = 0; c < 4; c++) {
+                const int v0 = quad_row<0>(t[c]), v1 = quad_row<1>(t[c]), v2 = quad_row<2>(t[c]), v3 = quad_row<3>(t[c]);
+                const int cs03 = v0 + v3, cs12 = v1 + v2, cd03 = v0 - v3, cd12 = v1 - v2;
+                const int o = y == 0 ? cs03 + cs12 : (y == 1 ? (cd03 << 1) + cd12 : (y == 2 ? cs03 - cs12 : cd03 - (cd12 << 1)));
+                const int cv = (int16_t)o;
+                uint32_t q = ((uint32_t)iabs(cv) * ((c & 1) ? mfo : mfe) + qbias) >> qshift;
+                q = q > 32767u ? 32767u : q;
+                lv[c] = cv < 0 ? -(int)q : (int)q;
+            }
+        }
+        // Dequantization (simd/h264_enc.rs:461) and inverse transform (simd/fixed.rs:185) added to
+        // the prediction; all-zero levels leave the prediction.
+        uint32_t rec = 0;
+        {
+            int c[4];
+#pragma unroll
+            for (int x = 0; x < 4; x++) c[x] = iclamp(lv[x] * ((x & 1) ? dqo : dqe), -32768, 32767);
+            const int e0 = c[0] + c[2], e1 = c[0] - c[2], e2 = (c[1] >> 1) - c[3], e3 = c[1] + (c[3] >> 1);
+            const int t[4] = {e0 + e3, e1 + e2, e1 - e2, e0 - e3};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int f0 = quad_row<0>(t[k]), f1 = quad_row<1>(t[k]), f2 = quad_row<2>(t[k]), f3 = quad_row<3>(t[k]);
+                const int g0 = f0 + f2, g1 = f0 - f2, g2 = (f1 >> 1) - f3, g3 = f1 + (f3 >> 1);
+                const int o = y == 0 ? g0 + g3 : (y == 1 ? g1 + g2 : (y == 2 ? g1 - g2 : g0 - g3));
+                rec |= (uint32_t)clip255(p[k] + (int16_t)((o + 32) >> 6)) << (8 * k);
+            }
         }
         uint32_t best = 0xFFFFFFFFu;
         int bm = 2;
 #pragma unroll
         for (int k = 0; k < 9; k++) {
-            const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)cost, k);
+            const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)cost, 4 * k);
             if (c < best) {
                 best = c;
                 bm = k;
@@ -1739,26 +1762,13 @@ __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint3
         total += best;
         if (total >= budget) return false;
         modes |= (uint64_t)bm << (4 * b);
-        int nz = 0;
-        if (lane == bm) {
-            int c[16];
-            fdct4_regs(d, c);
-            quant4(c, qp, 1);
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                S.q4i[b][i] = (int16_t)c[i];
-                nz |= c[i];
-            }
-#pragma unroll
-            for (int r = 0; r < 4; r++)
-                *(uint32_t *)&S.cv[(py + 1 + r) * CV_W + px + 4] = (uint32_t)p[r * 4] | ((uint32_t)p[r * 4 + 1] << 8) |
-                                                                 ((uint32_t)p[r * 4 + 2] << 16) | ((uint32_t)p[r * 4 + 3] << 24);
-            if (nz) {
-                dequant4(c, qp);
-                add_idct4(S.cv, CV_W, px + 4, py + 1, c);
-            }
+        const bool mine = m == bm;
+        if (__ballot(mine && (lv[0] | lv[1] | lv[2] | lv[3]) != 0)) cbpl |= 1 << (b >> 2);
+        if (mine) {
+            *(uint2 *)&S.q4i[b][y * 4] = make_uint2((uint32_t)(uint16_t)lv[0] | ((uint32_t)(uint16_t)lv[1] << 16),
+                                                    (uint32_t)(uint16_t)lv[2] | ((uint32_t)(uint16_t)lv[3] << 16));
+            *(uint32_t *)&S.cv[(py + 1 + y) * CV_W + px + 4] = rec;
         }
-        if (__ballot(nz != 0)) cbpl |= 1 << (b >> 2);
         __syncthreads();
     }
     return true;
@@ -1962,8 +1972,11 @@ union StepLds {
 // slot is still read by a later frame, defers: it raises the frame's delay for the next launch
 // and the whole frame repeats the step then (items already done skip themselves by their
 // stamps). Nothing waits inside a launch.
-extern "C" __global__ __launch_bounds__(64) void rgh_k_frame_step(const RghJob *__restrict__ jobs, int j0, int launch,
-                                                                   uint32_t seq) {
+//
+// IDR jobs run the same schedule in their own kernel (rgh_k_intra_step), so that the intra 4x4
+// search's registers do not lower the occupancy of the P analysis.
+template <bool INTRA>
+__device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int j0, int launch, uint32_t seq) {
     const RghJobD J = device_view(jobs[j0 + (int)blockIdx.y]);
     RGH_G RghFrameCtl *ctl = J.ctl;
     const int lane = threadIdx.x;
@@ -1984,7 +1997,7 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_frame_step(const RghJob *
         bool ok = true;
         if (s == 0 && J.wait_analysed) ok = __atomic_load_n(J.wait_analysed, __ATOMIC_RELAXED) >= J.wait_total;
         if (ok) {
-            if (J.intra)
+            if constexpr (INTRA)
                 analyse_intra_mb(J, x, y, S.a);
             else
                 ok = analyse_mb(J, x, y, S.a, seq);
@@ -2009,6 +2022,16 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_frame_step(const RghJob *
         if (x > 0 && y > 0) pad_mb(J, x - 1, y - 1);
         if (lane == 0) J.fin[m] = make_stamp(J.serial, seq);
     }
+}
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_frame_step(const RghJob *__restrict__ jobs, int j0, int launch,
+                                                                   uint32_t seq) {
+    frame_step<false>(jobs, j0, launch, seq);
+}
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_intra_step(const RghJob *__restrict__ jobs, int j0, int launch,
+                                                                   uint32_t seq) {
+    frame_step<true>(jobs, j0, launch, seq);
 }
 
 // Fills n 64-bit words (stamps) with v.

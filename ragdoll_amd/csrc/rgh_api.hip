@@ -13,6 +13,7 @@
 #include "rgh_device.h"
 
 extern "C" __global__ void rgh_k_frame_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
+extern "C" __global__ void rgh_k_intra_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_extend(const RghJob *jobs);
 extern "C" __global__ void rgh_k_fill_u64(uint64_t *p, int n, uint64_t v);
 
@@ -206,13 +207,21 @@ rgh_status ensure_jobs(Ctx *c, int n);
 // Several schedules (disjoint contexts) can interleave their launches on their own streams.
 struct Schedule {
     Ctx *owner = nullptr;
-    int n = 0, n_frames = 0, lag = 1, nj = 0;
+    int n = 0, n_frames = 0, lag = 1, nj = 0, n_intra = 0;
     std::vector<RghJob> jobs;
     std::vector<int> span, last;
     int gx = 1, max_span = 0, planned = 0, L = 0, extra_total = 0;
 
     rgh_status begin() {
         nj = n * n_frames;
+        // IDR jobs lead (frame-major order, IDR only as a stream's first frame).
+        n_intra = 0;
+        while (n_intra < nj && jobs[(size_t)n_intra].intra) n_intra++;
+        for (int j = n_intra; j < nj; j++)
+            if (jobs[(size_t)j].intra) {
+                set_error("run_schedule", "IDR jobs must lead the schedule", __LINE__);
+                return RGH_ERR_INVALID_ARGUMENT;
+            }
         span.assign((size_t)nj, 0);
         last.assign((size_t)nj, -1);
         for (int j = 0; j < nj; j++) {
@@ -231,8 +240,15 @@ struct Schedule {
     }
     void launch(int j_lo, int j_hi) {
         const uint32_t seq = g_seq.fetch_add(1);
-        hipLaunchKernelGGL(rgh_k_frame_step, dim3((unsigned)gx, (unsigned)(j_hi - j_lo)), dim3(64), 0, owner->stream,
-                           owner->jobs_dev, j_lo, L, seq);
+        // IDR jobs (the leading n_intra) and P jobs of the launch, in one sequence number: a
+        // stamp of one is not final for the other within the launch.
+        const int split = std::min(std::max(n_intra, j_lo), j_hi);
+        if (split > j_lo)
+            hipLaunchKernelGGL(rgh_k_intra_step, dim3((unsigned)gx, (unsigned)(split - j_lo)), dim3(64), 0, owner->stream,
+                               owner->jobs_dev, j_lo, L, seq);
+        if (j_hi > split)
+            hipLaunchKernelGGL(rgh_k_frame_step, dim3((unsigned)gx, (unsigned)(j_hi - split)), dim3(64), 0,
+                               owner->stream, owner->jobs_dev, split, L, seq);
         for (int j = j_lo; j < j_hi; j++) last[(size_t)j] = L;
         L++;
     }
