@@ -55,26 +55,34 @@ def profile_record(kernel):
         return None
 
 
-def measured_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` (FETCH_SIZE / WRITE_SIZE, copy-calibrated), or None."""
-    k = profile_record(kernel)
-    if not k or "fetch_bytes" not in k:
+def measured_traffic():
+    """HBM bytes per launch (the P and IDR kernels of one launch; FETCH_SIZE / WRITE_SIZE,
+    copy-calibrated) from the committed profile, or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return round(json.load(f)["bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
         return None
-    return round(k["fetch_bytes"] + k["write_bytes"])
 
 
-def valu_figure(kernel, mbs_per_launch, avg_launch_s):
-    """Integer VALU throughput of `kernel`: the counted INT32 + INT64 VALU instructions per
-    macroblock (SQ_INSTS_VALU_INT32 / _INT64 over the macroblocks of the profiled run, every
-    deferred attempt included) x 64 lanes x macroblocks per launch / launch time, against the
-    integer VALU peak; None without the committed counters."""
-    k = profile_record(kernel)
-    if not k or "valu_int_per_mb" not in k:
-        return None
-    tops = k["valu_int_per_mb"] * 64 * mbs_per_launch / avg_launch_s / 1e12
+def valu_figure(mbs, device_s):
+    """Integer VALU throughput of the job: the counted INT32 + INT64 VALU instructions per
+    macroblock of each kernel (SQ_INSTS_VALU_INT32 / _INT64 over the macroblocks of the profiled
+    run, every deferred attempt included) x 64 lanes x the job's macroblocks of that kernel, over
+    the job's device time, against the integer VALU peak; None without the committed counters.
+    mbs: {kernel: macroblocks per job}."""
+    ops = 0.0
+    per = {}
+    for kernel, n in mbs.items():
+        k = profile_record(kernel)
+        if not k or "valu_int_per_mb" not in k:
+            return None
+        ops += k["valu_int_per_mb"] * 64 * n
+        per[kernel] = {"valu_insts_per_mb": k.get("valu_per_mb"), "valu_int_insts_per_mb": k["valu_int_per_mb"],
+                       "valu_busy": k.get("valu_busy")}
+    tops = ops / device_s / 1e12
     return {"achieved": round(tops, 3), "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s (int32 lane ops)",
-            "frac": tops / VALU_PEAK_TOPS, "valu_insts_per_mb": k.get("valu_per_mb"),
-            "valu_int_insts_per_mb": k["valu_int_per_mb"], "valu_issue_util": k.get("valu_issue_util")}
+            "frac": tops / VALU_PEAK_TOPS, "kernels": per}
 
 
 def calibrate_copy(mib, repeats=4):
@@ -291,13 +299,14 @@ def run_workload(gops, args):
                    "device_ms_per_step": round(dev_ms / args.steps, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("rgh_k_frame_step") if headline else None,
-                     "kernel": "rgh_k_frame_step", "avg_launch_us": round(avg_launch_s * 1e6, 3),
+                     "traffic": measured_traffic() if headline else None,
+                     "kernel": "rgh_k_frame_step (+ rgh_k_intra_step in launches with IDR frames)",
+                     "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "mbs_per_launch": round(mbs_per_launch, 2), "bytes_per_p_mb": MB_BYTES,
                      "bytes_per_idr_mb": IDR_MB_BYTES},
     }
     if headline:
-        out["valu"] = valu_figure("rgh_k_frame_step", mbs_per_launch, avg_launch_s)
+        out["valu"] = valu_figure({"rgh_k_frame_step": total_mbs, "rgh_k_intra_step": idr_mbs}, dev_ms / 1e3 / args.steps)
     for a in ans:
         a.close()
     for p in [q for q, _ in uploaded.values()] + [q for d in decs for q in d] + [q for l in lvls for q in l]:
