@@ -835,13 +835,23 @@ struct ora_enc {
     int is_idr, skip_run, cur_qp;
     skip_bounds_t sb;
     hpel_win_t win;
+    /* Compact preset (enc/compact.rs, enc/inter.rs): chroma QP offset 1, fine inter rounding,
+     * half-sample planes of the reference, the 16x16 search vectors of this and the previous
+     * picture (mvr), the Compact motion vector costs. */
+    int compact, cqo;
+    plane_t hh, hv, hc;
+    int16_t (*mvr)[2], (*prev_mvr)[2];
+    uint16_t *mv_costs_c;
+    int mv_costs_c_qp;
 };
 
 int ora_mb_w(const ora_enc *e) { return e->mb_w; }
 int ora_mb_h(const ora_enc *e) { return e->mb_h; }
 int ora_last_was_idr(const ora_enc *e) { return e->last_idr; }
 
-static int chroma_qp(int qp) { return T_CHROMA_QP[iclamp(qp, 0, 51)]; } /* enc.rs:540, offset 0 */
+/* enc.rs:540 chroma_qp with the picture parameter set's chroma_qp_index_offset */
+static int chroma_qp_o(const ora_enc *e, int qp) { return T_CHROMA_QP[iclamp(qp + e->cqo, 0, 51)]; }
+#define chroma_qp(qp) chroma_qp_o(e, (qp))
 static uint32_t se_bits(int d) { /* enc.rs:546 */
     uint32_t v = (uint32_t)iabs(d) * 2 - (uint32_t)(d > 0);
     return 2 * (uint32_t)(32 - clz32(v + 1)) - 1;
@@ -892,7 +902,26 @@ ora_enc *ora_create(int width, int height, int fps_num, int fps_den, int qp, int
     e->pre_y = malloc((size_t)W * H);
     e->pre_u = malloc((size_t)W * H / 4);
     e->pre_v = malloc((size_t)W * H / 4);
+    e->mvr = calloc((size_t)n, sizeof(*e->mvr));
+    e->prev_mvr = calloc((size_t)n, sizeof(*e->prev_mvr));
+    e->mv_costs_c = (uint16_t *)malloc(sizeof(uint16_t) * (2 * MV_COST_RANGE + 1));
+    e->mv_costs_c_qp = -1;
     return e;
+}
+
+/* H264Encoder::with_speed(Compact) (h264.rs:3066-3098): chroma QP offset 1; with CAVLC no B
+ * pictures. Returns -1 for the CABAC profiles, whose Compact configuration codes B pictures. */
+int ora_set_compact(ora_enc *e, int on) {
+    if (on && e->cabac) return -1;
+    e->compact = on != 0;
+    e->cqo = on ? 1 : 0;
+    if (on && !e->hh.buf) {
+        int W = e->mb_w * 16, H = e->mb_h * 16;
+        plane_alloc(&e->hh, W, H, PAD_Y);
+        plane_alloc(&e->hv, W, H, PAD_Y);
+        plane_alloc(&e->hc, W, H, PAD_Y);
+    }
+    return 0;
 }
 
 void ora_destroy(ora_enc *e) {
@@ -923,6 +952,12 @@ void ora_destroy(ora_enc *e) {
     free(e->pre_y);
     free(e->pre_u);
     free(e->pre_v);
+    free(e->mvr);
+    free(e->prev_mvr);
+    free(e->mv_costs_c);
+    free(e->hh.buf);
+    free(e->hv.buf);
+    free(e->hc.buf);
     free(e);
 }
 
@@ -964,9 +999,11 @@ static int16_t median3(int16_t a, int16_t b, int16_t c) { /* h264.rs:3570 */
     int16_t t = mn > c ? mn : c;
     return mx < t ? mx : t;
 }
-/* h264.rs:3577 pred_motion_partition (16x16: part_w4 = 4, ref 0) */
-static void pred_mv(const ora_enc *e, int gx, int gy, int16_t mvp[2]) {
-    int gs = e->mb_w * 4, pw = 4;
+/* h264.rs:3577 pred_motion_partition / enc/inter.rs:104 neighbours + :143 mvp_from, for the
+ * partition of reference 0 whose top-left 4x4 block is (gx, gy), pw blocks wide, directional
+ * rule 0 (16x16 and 8x8 partitions) */
+static void pred_mv_part(const ora_enc *e, int gx, int gy, int pw, int16_t mvp[2]) {
+    int gs = e->mb_w * 4;
     int lref = -2, tref = -2, dref = -2;
     int16_t a[2] = {0, 0}, b[2] = {0, 0}, c[2] = {0, 0};
     if (gx > 0) {
@@ -1012,6 +1049,7 @@ static void pred_mv(const ora_enc *e, int gx, int gy, int16_t mvp[2]) {
         mvp[1] = median3(a[1], b[1], c[1]);
     }
 }
+static void pred_mv(const ora_enc *e, int gx, int gy, int16_t mvp[2]) { pred_mv_part(e, gx, gy, 4, mvp); }
 
 /* enc.rs:389 MbSrc::load + h264.rs:3660 extract_mb_planes (edge replication) */
 typedef struct {
@@ -1087,8 +1125,8 @@ static uint32_t zero_ssd_bound(const quant4_t *q) {
     }
     return m;
 }
-static void skip_bounds_init(skip_bounds_t *s, int qp, int qpc) {
-    const quant4_t *q = quant4(qp, 0), *qc = quant4(qpc, 0);
+static void skip_bounds_init(skip_bounds_t *s, int qp, int qpc, int fine) {
+    const quant4_t *q = quant4(qp, fine), *qc = quant4(qpc, fine);
     uint32_t dc_shift = 16 + (uint32_t)qpc / 6;
     uint32_t dc_room = (1u << dc_shift) - (1u << dc_shift) / 3 - 1;
     uint32_t dc = dc_room / (uint32_t)T_QUANT4_MF[qpc % 6][0];
@@ -1111,10 +1149,10 @@ static void dct_luma(const uint8_t *src, const uint8_t *pred, int16_t out[16][16
 }
 
 /* enc.rs:2163 probe_skip_luma */
-static int probe_skip_luma(const mbsrc_t *s, const uint8_t *py, int qp) {
+static int probe_skip_luma(const mbsrc_t *s, const uint8_t *py, int qp, int fine) {
     int16_t c[16][16];
     dct_luma(s->y, py, c);
-    const quant4_t *q = quant4(qp, 0);
+    const quant4_t *q = quant4(qp, fine);
     uint32_t total = 0;
     for (int i8 = 0; i8 < 4; i8++) {
         uint32_t nz = quant_4x4_x4(&c[i8 * 4], q);
@@ -1182,9 +1220,9 @@ static uint32_t chroma_ac_score(const chroma_levels_t *c, int p) {
     return s;
 }
 /* enc.rs:2186 probe_skip_chroma */
-static int probe_skip_chroma(const mbsrc_t *s, const uint8_t *pu, const uint8_t *pv, int qpc) {
+static int probe_skip_chroma(const mbsrc_t *s, const uint8_t *pu, const uint8_t *pv, int qpc, int fine) {
     chroma_levels_t c;
-    quant_chroma(s, pu, pv, qpc, 0, &c);
+    quant_chroma(s, pu, pv, qpc, fine, &c);
     for (int p = 0; p < 2; p++)
         for (int b = 0; b < 4; b++)
             if (c.dc[p][b]) return 0;
@@ -1192,8 +1230,8 @@ static int probe_skip_chroma(const mbsrc_t *s, const uint8_t *pu, const uint8_t 
 }
 /* enc.rs:2084 quant_chroma_inter */
 static void quant_chroma_inter(const mbsrc_t *s, const uint8_t *pu, const uint8_t *pv, int qpc,
-                               chroma_levels_t *c) {
-    quant_chroma(s, pu, pv, qpc, 0, c);
+                               chroma_levels_t *c, int fine) {
+    quant_chroma(s, pu, pv, qpc, fine, c);
     for (int p = 0; p < 2; p++) {
         uint32_t any = c->ac_masks[p][0] | c->ac_masks[p][1] | c->ac_masks[p][2] | c->ac_masks[p][3];
         if (any && chroma_ac_score(c, p) < 7) {
@@ -1204,9 +1242,9 @@ static void quant_chroma_inter(const mbsrc_t *s, const uint8_t *pu, const uint8_
     }
 }
 /* enc.rs:1810 quant_luma_4x4_inter */
-static int quant_luma_4x4_inter(const uint8_t *src, const uint8_t *pred, int qp, int16_t q4[16][16]) {
+static int quant_luma_4x4_inter(const uint8_t *src, const uint8_t *pred, int qp, int16_t q4[16][16], int fine) {
     dct_luma(src, pred, q4);
-    const quant4_t *q = quant4(qp, 0);
+    const quant4_t *q = quant4(qp, fine);
     int cbp = 0;
     uint32_t total = 0;
     for (int i8 = 0; i8 < 4; i8++) {
@@ -2181,7 +2219,7 @@ static int try_p_skip(ora_enc *e, int mb_x, int mb_y, int qp, int qpc, const mbs
                     (int_y ? ssd_n(s->y, 16, ry->buf + yo, ry->stride, 16) : ssd_n(s->y, 16, py, 16, 16)) <= z->luma_ssd;
     if (!luma_zero) {
         if (int_y) load_block(ry, yo, py, 16);
-        luma_zero = probe_skip_luma(s, py, qp);
+        luma_zero = probe_skip_luma(s, py, qp, e->compact);
     }
     if (!luma_zero) return 0;
     if (!int_c) mc_chroma_mb(&e->refp, mb_x * 8, mb_y * 8, pskip, pu, pv);
@@ -2198,7 +2236,7 @@ static int try_p_skip(ora_enc *e, int mb_x, int mb_y, int qp, int qpc, const mbs
             load_block(ru, co, pu, 8);
             load_block(rv, co, pv, 8);
         }
-        chroma_zero = probe_skip_chroma(s, pu, pv, qpc);
+        chroma_zero = probe_skip_chroma(s, pu, pv, qpc, e->compact);
     }
     if (!chroma_zero) return 0;
     code_skip(e, mb_x, mb_y, pskip);
@@ -2287,7 +2325,7 @@ static void code_p_mb(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const m
     d->mvp[1] = mvp[1];
     d->qp = (uint8_t)qp;
     uint8_t py[256], pu[64], pv[64];
-    if (e->sb.qp != qp) skip_bounds_init(&e->sb, qp, qpc);
+    if (e->sb.qp != qp) skip_bounds_init(&e->sb, qp, qpc, e->compact);
     if (try_p_skip(e, mb_x, mb_y, qp, qpc, s, pskip, mvb_contains(&b, pskip[0], pskip[1]), py, pu, pv)) {
         d->mb_type = 0;
         d->flags = 1;
@@ -2345,7 +2383,7 @@ static void code_p_mb(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const m
     }
     mc_chroma_mb(&e->refp, mb_x * 8, mb_y * 8, mv, pu, pv);
     int16_t q4[16][16], q8[4][64];
-    int cbpl = quant_luma_4x4_inter(s->y, py, qp, q4);
+    int cbpl = quant_luma_4x4_inter(s->y, py, qp, q4, 0);
     int use8 = 0;
     if (e->t8x8 && e->cabac && cbpl) {
         int cbp8 = quant_luma_8x8_inter(s->y, py, qp, q8);
@@ -2360,7 +2398,7 @@ static void code_p_mb(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const m
         }
     }
     chroma_levels_t c;
-    quant_chroma_inter(s, pu, pv, qpc, &c);
+    quant_chroma_inter(s, pu, pv, qpc, &c, 0);
     int cbpc = chroma_cbp(&c);
     d->mv[0] = mv[0];
     d->mv[1] = mv[1];
@@ -2540,6 +2578,641 @@ static void deblock(ora_enc *e) {
 #undef MVEQ
 }
 
+
+/* ------------------------------------------------------------------------------------------ */
+/* Compact P macroblock (enc/inter.rs:498-759, enc/compact.rs).                                 */
+/* ------------------------------------------------------------------------------------------ */
+/* enc.rs:73 LAMBDA_COMPACT: four fifths of LAMBDA, at least 1 */
+static uint32_t lambda_compact(int qp) {
+    uint32_t l = (uint32_t)T_LAMBDA[qp] * 4 / 5;
+    return l > 1 ? l : 1;
+}
+/* enc.rs:282 mv_costs(qp, compact) */
+static void ensure_mv_costs_c(ora_enc *e, int qp) {
+    if (e->mv_costs_c_qp == qp) return;
+    uint32_t lambda = lambda_compact(qp);
+    for (int d = -MV_COST_RANGE; d <= MV_COST_RANGE; d++)
+        e->mv_costs_c[d + MV_COST_RANGE] = (uint16_t)(lambda * se_bits(d));
+    e->mv_costs_c_qp = qp;
+}
+
+/* compact.rs:24 HpelPlanes::build with simd/h264_enc.rs:905 hpel_planes_scalar: b, h and j of
+ * the reference luma over [-pad + 2, w + pad - 11) x [-pad + 2, h + pad - 3), replicated
+ * outward (compact.rs:55 extend_from) */
+static void plane_extend_from(plane_t *p, int x0, int x1, int y0, int y1) {
+    int s = p->stride, rows = p->h + 2 * p->pad;
+    long bx0 = pat(p, x0, 0) % s, bx1 = pat(p, x1 - 1, 0) % s;
+    long by0 = pat(p, 0, y0) / s, by1 = pat(p, 0, y1 - 1) / s;
+    for (long y = by0; y <= by1; y++) {
+        uint8_t *row = p->buf + y * s;
+        uint8_t l = row[bx0], r = row[bx1];
+        memset(row, l, (size_t)bx0);
+        memset(row + bx1 + 1, r, (size_t)(s - bx1 - 1));
+    }
+    for (long y = 0; y < by0; y++) memcpy(p->buf + y * s, p->buf + by0 * s, (size_t)s);
+    for (long y = by1 + 1; y < rows; y++) memcpy(p->buf + y * s, p->buf + by1 * s, (size_t)s);
+}
+static void hpel_planes_build(ora_enc *e) {
+    const plane_t *f = &e->refp.y;
+    int pad = f->pad, s = f->stride;
+    int x_lo = -pad + 2, y_lo = -pad + 2, x_hi = f->w + pad - 11, y_hi = f->h + pad - 3;
+    int *tv = (int *)malloc(sizeof(int) * (size_t)(x_hi - x_lo + 5));
+    for (int y = y_lo; y < y_hi; y++) {
+        for (int k = 0; k < x_hi - x_lo + 5; k++) {
+            const uint8_t *r = f->buf + pat(f, x_lo + k - 2, y);
+            tv[k] = tap6(r[-2 * s], r[-s], r[0], r[s], r[2 * s], r[3 * s]);
+        }
+        for (int x = x_lo; x < x_hi; x++) {
+            const uint8_t *r = f->buf + pat(f, x, y);
+            const int *t = tv + (x - x_lo);
+            e->hh.buf[pat(&e->hh, x, y)] = clip_u8((tap6(r[-2], r[-1], r[0], r[1], r[2], r[3]) + 16) >> 5);
+            e->hv.buf[pat(&e->hv, x, y)] = clip_u8((t[2] + 16) >> 5);
+            e->hc.buf[pat(&e->hc, x, y)] = clip_u8((tap6(t[0], t[1], t[2], t[3], t[4], t[5]) + 512) >> 10);
+        }
+    }
+    free(tv);
+    plane_extend_from(&e->hh, x_lo, x_hi, y_lo, y_hi);
+    plane_extend_from(&e->hv, x_lo, x_hi, y_lo, y_hi);
+    plane_extend_from(&e->hc, x_lo, x_hi, y_lo, y_hi);
+}
+
+/* compact.rs:87 RefLuma::src: the block (or the two blocks whose rounded average is the
+ * prediction) of the w x h block at (x0, y0) displaced by quarter-pel (mx, my) */
+static void rl_blocks(const ora_enc *e, int x0, int y0, int mx, int my, const uint8_t **pa, const uint8_t **pb) {
+    const plane_t *f = &e->refp.y;
+    long o = pat(f, x0 + (mx >> 2), y0 + (my >> 2)), s = f->stride;
+    const uint8_t *F = f->buf, *Hh = e->hh.buf, *V = e->hv.buf, *C = e->hc.buf;
+    const uint8_t *a = NULL, *b = NULL;
+    switch ((mx & 3) | ((my & 3) << 2)) {
+    case 0: a = F + o; break;
+    case 2: a = Hh + o; break;
+    case 8: a = V + o; break;
+    case 10: a = C + o; break;
+    case 1: a = F + o; b = Hh + o; break;
+    case 3: a = Hh + o; b = F + o + 1; break;
+    case 4: a = F + o; b = V + o; break;
+    case 12: a = V + o; b = F + o + s; break;
+    case 5: a = Hh + o; b = V + o; break;
+    case 7: a = Hh + o; b = V + o + 1; break;
+    case 13: a = V + o; b = Hh + o + s; break;
+    case 15: a = V + o + 1; b = Hh + o + s; break;
+    case 6: a = Hh + o; b = C + o; break;
+    case 14: a = C + o; b = Hh + o + s; break;
+    case 9: a = V + o; b = C + o; break;
+    default: a = C + o; b = V + o + 1; break; /* (3, 2) */
+    }
+    *pa = a;
+    *pb = b;
+}
+/* compact.rs:130 RefLuma::pred into dst (stride ds) */
+static void rl_pred(const ora_enc *e, int x0, int y0, int w, int h, int mx, int my, uint8_t *dst, int ds) {
+    const uint8_t *a, *b;
+    rl_blocks(e, x0, y0, mx, my, &a, &b);
+    long s = e->refp.y.stride;
+    for (int r = 0; r < h; r++)
+        for (int c = 0; c < w; c++)
+            dst[r * ds + c] = b ? (uint8_t)((a[r * s + c] + b[r * s + c] + 1) >> 1) : a[r * s + c];
+}
+/* simd/h264_me.rs:163 pixel_satd_wxh_scalar: 4x4 Hadamard sums over the block, halved at the end */
+static uint32_t satd_wh(const uint8_t *a, int sa, const uint8_t *b, int sb, int w, int h) {
+    uint32_t sum = 0;
+    for (int by = 0; by < h; by += 4)
+        for (int bx = 0; bx < w; bx += 4) {
+            int d[4][4];
+            for (int y = 0; y < 4; y++)
+                for (int x = 0; x < 4; x++) d[y][x] = a[(by + y) * sa + bx + x] - b[(by + y) * sb + bx + x];
+            for (int y = 0; y < 4; y++) {
+                int s01 = d[y][0] + d[y][1], d01 = d[y][0] - d[y][1];
+                int s23 = d[y][2] + d[y][3], d23 = d[y][2] - d[y][3];
+                d[y][0] = s01 + s23;
+                d[y][1] = s01 - s23;
+                d[y][2] = d01 + d23;
+                d[y][3] = d01 - d23;
+            }
+            for (int c = 0; c < 4; c++) {
+                int s01 = d[0][c] + d[1][c], d01 = d[0][c] - d[1][c];
+                int s23 = d[2][c] + d[3][c], d23 = d[2][c] - d[3][c];
+                sum += (uint32_t)(iabs(s01 + s23) + iabs(s01 - s23) + iabs(d01 + d23) + iabs(d01 - d23));
+            }
+        }
+    return sum >> 1;
+}
+
+/* compact.rs:267 PartSearch + :453 FullPel */
+typedef struct {
+    ora_enc *e;
+    const uint8_t *src; /* the partition's source samples, stride 16 */
+    int x0, y0, w, h;   /* the partition in the picture */
+    int mvp[2];
+    uint32_t lambda;
+    mvb_t b;            /* bounds of the macroblock's vectors */
+    int lo[2], hi[2];   /* full-pel bounds */
+    int best[2];
+    uint32_t best_cost;
+} ps_t;
+typedef struct {
+    int mv[2];
+    uint32_t satd, mv_cost;
+} me_t;
+static uint32_t me_cost(const me_t *m) { return m->satd + m->mv_cost; }
+static void ps_init(ps_t *p, ora_enc *e, const uint8_t *src, int x0, int y0, int w, int h, const int16_t mvp[2],
+                    uint32_t lambda, const mvb_t *b) {
+    p->e = e;
+    p->src = src;
+    p->x0 = x0;
+    p->y0 = y0;
+    p->w = w;
+    p->h = h;
+    p->mvp[0] = mvp[0];
+    p->mvp[1] = mvp[1];
+    p->lambda = lambda;
+    p->b = *b;
+    p->lo[0] = (b->min[0] + 3) >> 2;
+    p->lo[1] = (b->min[1] + 3) >> 2;
+    p->hi[0] = b->max[0] >> 2;
+    p->hi[1] = b->max[1] >> 2;
+    p->best[0] = p->best[1] = 0;
+    p->best_cost = UINT32_MAX;
+}
+static uint32_t ps_mvd_cost(const ps_t *p, int d) { /* compact.rs:543 */
+    if (d >= -MV_COST_RANGE && d <= MV_COST_RANGE) return p->e->mv_costs_c[d + MV_COST_RANGE];
+    return p->lambda * se_bits(d);
+}
+static uint32_t ps_mv_cost(const ps_t *p, int mx, int my) { return ps_mvd_cost(p, mx - p->mvp[0]) + ps_mvd_cost(p, my - p->mvp[1]); }
+/* compact.rs:557 cost: SAD or SATD against the prediction at quarter-pel (mx, my) */
+static uint32_t ps_cost(const ps_t *p, int mx, int my, int satd) {
+    uint8_t pred[256];
+    rl_pred(p->e, p->x0, p->y0, p->w, p->h, mx, my, pred, 16);
+    return satd ? satd_wh(p->src, 16, pred, 16, p->w, p->h) : sad_wh(p->src, 16, pred, 16, p->w, p->h);
+}
+/* compact.rs:492 FullPel::at */
+static void fp_at(const ps_t *p, const int from[2], int dx, int dy, int out[2]) {
+    int m0 = from[0] + dx, m1 = from[1] + dy;
+    int ok = (uint32_t)(m0 - p->lo[0]) <= (uint32_t)(p->hi[0] - p->lo[0]) && (uint32_t)(m1 - p->lo[1]) <= (uint32_t)(p->hi[1] - p->lo[1]);
+    out[0] = ok ? m0 : from[0];
+    out[1] = ok ? m1 : from[1];
+}
+/* compact.rs:516 FullPel::take: four points in order, the first of equal costs winning;
+ * returns the last that improved, or -1 */
+static int fp_take(ps_t *p, int pts[4][2]) {
+    const plane_t *f = &p->e->refp.y;
+    int moved = -1;
+    for (int k = 0; k < 4; k++) {
+        long o = pat(f, p->x0 + pts[k][0], p->y0 + pts[k][1]);
+        uint32_t c = sad_wh(p->src, 16, f->buf + o, f->stride, p->w, p->h) + ps_mv_cost(p, 4 * pts[k][0], 4 * pts[k][1]);
+        if (c < p->best_cost) {
+            p->best_cost = c;
+            p->best[0] = pts[k][0];
+            p->best[1] = pts[k][1];
+            moved = k;
+        }
+    }
+    return moved;
+}
+/* compact.rs:723 refine_qpel_k: quarter-pel diamonds by SATD, at most iters */
+static me_t ps_refine_qpel(const ps_t *p, me_t m, int iters) {
+    static const int D[4][2] = {{0, -1}, {0, 1}, {-1, 0}, {1, 0}};
+    int best[2] = {m.mv[0], m.mv[1]}, prev[2] = {0, 0}, has_prev = 0;
+    uint32_t best_satd = m.satd, best_cost = m.satd + ps_mv_cost(p, best[0], best[1]);
+    for (int it = 0; it < iters; it++) {
+        int from[2] = {best[0], best[1]};
+        for (int k = 0; k < 4; k++) {
+            int mx = from[0] + D[k][0], my = from[1] + D[k][1];
+            if ((has_prev && mx == prev[0] && my == prev[1]) || !mvb_contains(&p->b, mx, my)) continue;
+            uint32_t satd = ps_cost(p, mx, my, 1);
+            uint32_t cost = satd + ps_mv_cost(p, mx, my);
+            if (cost < best_cost) {
+                best_cost = cost;
+                best_satd = satd;
+                best[0] = mx;
+                best[1] = my;
+            }
+        }
+        if (best[0] == from[0] && best[1] == from[1]) break;
+        prev[0] = from[0];
+        prev[1] = from[1];
+        has_prev = 1;
+    }
+    me_t r = {{best[0], best[1]}, best_satd, ps_mv_cost(p, best[0], best[1])};
+    return r;
+}
+/* compact.rs:651 refine: the predictor at its sub-pel position, a half-pel diamond by SAD,
+ * then quarter-pel diamonds by SATD */
+static me_t ps_refine(const ps_t *p, int sx, int sy, uint32_t start_cost, int qpel) {
+    int best[2] = {sx, sy};
+    uint32_t best_cost = start_cost;
+    if ((p->mvp[0] != best[0] || p->mvp[1] != best[1]) && mvb_contains(&p->b, p->mvp[0], p->mvp[1])) {
+        uint32_t c = ps_cost(p, p->mvp[0], p->mvp[1], 0) + ps_mv_cost(p, p->mvp[0], p->mvp[1]);
+        if (c < best_cost) {
+            best_cost = c;
+            best[0] = p->mvp[0];
+            best[1] = p->mvp[1];
+        }
+    }
+    static const int DIRS[4][2] = {{0, -2}, {-2, 0}, {2, 0}, {0, 2}};
+    int from[2] = {best[0], best[1]};
+    for (int k = 0; k < 4; k++) {
+        int mx = from[0] + DIRS[k][0], my = from[1] + DIRS[k][1];
+        if (!mvb_contains(&p->b, mx, my)) continue;
+        uint32_t c = ps_cost(p, mx, my, 0) + ps_mv_cost(p, mx, my);
+        if (c < best_cost) {
+            best_cost = c;
+            best[0] = mx;
+            best[1] = my;
+        }
+    }
+    me_t m = {{best[0], best[1]}, ps_cost(p, best[0], best[1], 1), ps_mv_cost(p, best[0], best[1])};
+    return ps_refine_qpel(p, m, qpel);
+}
+/* compact.rs:583 search_qpel: zero and the candidates (rounded to full-pel, clamped, distinct),
+ * a radius-2 hexagon for up to range / 2 steps, a square of eight, then sub-pel refinement */
+static me_t ps_search(ps_t *p, const int16_t (*cands)[2], int nc, int range, int qpel) {
+    int pts[12][2] = {{0, 0}};
+    int n = 1;
+    for (int i = 0; i < nc; i++) {
+        int m0 = iclamp((cands[i][0] + 2) >> 2, p->lo[0], p->hi[0]);
+        int m1 = iclamp((cands[i][1] + 2) >> 2, p->lo[1], p->hi[1]);
+        int dup = 0;
+        for (int j = 0; j < n; j++)
+            if (pts[j][0] == m0 && pts[j][1] == m1) dup = 1;
+        if (n < 12 && !dup) {
+            pts[n][0] = m0;
+            pts[n][1] = m1;
+            n++;
+        }
+    }
+    for (int c = 0; c < n; c += 4) {
+        int q[4][2];
+        for (int k = 0; k < 4; k++) {
+            int i = c + k < n ? c + k : 0;
+            q[k][0] = pts[i][0];
+            q[k][1] = pts[i][1];
+        }
+        fp_take(p, q);
+    }
+    static const int HEX[6][2] = {{-2, 0}, {-1, 2}, {1, 2}, {2, 0}, {1, -2}, {-1, -2}};
+    static const int NEXT[6][3] = {{0, 1, 5}, {0, 1, 2}, {1, 2, 3}, {2, 3, 4}, {3, 4, 5}, {0, 4, 5}};
+    int dir = -1;
+    if (range >= 2) {
+        int from[2] = {p->best[0], p->best[1]}, h[6][2];
+        for (int k = 0; k < 6; k++) fp_at(p, from, HEX[k][0], HEX[k][1], h[k]);
+        int qa[4][2] = {{h[0][0], h[0][1]}, {h[1][0], h[1][1]}, {h[2][0], h[2][1]}, {h[3][0], h[3][1]}};
+        int qb[4][2] = {{h[4][0], h[4][1]}, {h[5][0], h[5][1]}, {from[0], from[1]}, {from[0], from[1]}};
+        int a = fp_take(p, qa), b = fp_take(p, qb);
+        dir = b >= 0 ? b + 4 : a;
+    }
+    for (int it = 1; it < range / 2 && dir >= 0; it++) {
+        int from[2] = {p->best[0], p->best[1]}, q[4][2];
+        for (int k = 0; k < 3; k++) fp_at(p, from, HEX[NEXT[dir][k]][0], HEX[NEXT[dir][k]][1], q[k]);
+        q[3][0] = from[0];
+        q[3][1] = from[1];
+        int t = fp_take(p, q);
+        dir = t >= 0 ? NEXT[dir][t] : -1;
+    }
+    {
+        static const int SQ[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {-1, 0}, {1, 0}, {-1, 1}, {0, 1}, {1, 1}};
+        int from[2] = {p->best[0], p->best[1]}, qa[4][2], qb[4][2];
+        for (int k = 0; k < 4; k++) {
+            fp_at(p, from, SQ[k][0], SQ[k][1], qa[k]);
+            fp_at(p, from, SQ[k + 4][0], SQ[k + 4][1], qb[k]);
+        }
+        fp_take(p, qa);
+        fp_take(p, qb);
+    }
+    return ps_refine(p, p->best[0] * 4, p->best[1] * 4, p->best_cost, qpel);
+}
+
+#define COMPACT_ME_RANGE 16  /* compact.rs:758 ME_RANGE */
+#define COMPACT_QPEL_REFINE 2 /* compact.rs:760 QPEL_REFINE */
+#define P8X8_GATE 40         /* inter.rs:10 */
+
+static void fill_part(ora_enc *e, int gx, int gy, int w4, int h4, int8_t r, int mx, int my) { /* h264.rs:3554 */
+    int gs = e->mb_w * 4;
+    for (int y = gy; y < gy + h4; y++)
+        for (int x = gx; x < gx + w4; x++) {
+            e->ref_grid[y * gs + x] = r;
+            e->mv_grid[y * gs + x][0] = (int16_t)mx;
+            e->mv_grid[y * gs + x][1] = (int16_t)my;
+        }
+}
+/* The Compact decision record: words 2-5 hold the vectors of the four 8x8 quadrants (raster
+ * order), word 6 the 16x16 search vector (mvr) the next picture takes as a candidate. */
+static void set_compact_record(ora_mb_decision *d, const int q[4][2], const int16_t mvr[2]) {
+    int16_t w[5][2];
+    for (int k = 0; k < 4; k++) {
+        w[k][0] = (int16_t)q[k][0];
+        w[k][1] = (int16_t)q[k][1];
+    }
+    w[4][0] = mvr[0];
+    w[4][1] = mvr[1];
+    memcpy(d->mv, w[0], 4);
+    memcpy(d->mvp, w[1], 4);
+    memcpy(&d->cost, w[2], 4);
+    memcpy(&d->n_sad, w[3], 4);
+    memcpy(&d->n_satd, w[4], 4);
+}
+
+/* inter.rs:293 code_inter_p_body (CAVLC): P_L0_16x16 or P_8x8 (four P_L0_8x8) */
+static void code_inter_p(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, int p8x8, const int q[4][2], int cbpl,
+                         int cbpc, int16_t q4[16][16], const chroma_levels_t *c, uint8_t *py, uint8_t *pu, uint8_t *pv) {
+    int idx = mb_y * e->mb_w + mb_x, gx0 = mb_x * 4, gy0 = mb_y * 4;
+    int cbp = cbpl | (cbpc << 4);
+    int qp_delta = cbp ? qp - e->cur_qp : 0, eff_qp = iclamp(e->cur_qp + qp_delta, 0, 51);
+    int eff_qpc = chroma_qp(eff_qp);
+    if (w) {
+        put_ue(w, p8x8 ? 3 : 0);
+        if (p8x8)
+            for (int k = 0; k < 4; k++) put_ue(w, 0);
+    }
+    for (int part = 0; part < (p8x8 ? 4 : 1); part++) {
+        int bx = p8x8 ? (part & 1) * 2 : 0, by = p8x8 ? (part >> 1) * 2 : 0, w4 = p8x8 ? 2 : 4;
+        int16_t mvp[2];
+        pred_mv_part(e, gx0 + bx, gy0 + by, w4, mvp);
+        if (w) {
+            put_se(w, q[part][0] - mvp[0]);
+            put_se(w, q[part][1] - mvp[1]);
+        }
+        fill_part(e, gx0 + bx, gy0 + by, w4, w4, 0, q[part][0], q[part][1]);
+    }
+    if (w) put_ue(w, inter_cbp_golomb(cbp));
+    if (cbp) {
+        if (w) put_se(w, qp_delta);
+        write_cavlc_residuals(e, w, mb_x, mb_y, cbpl, cbpc, q4, c);
+    }
+    e->mb_cbp_luma[idx] = (uint8_t)cbpl;
+    e->mb_cbp_chroma[idx] = (uint8_t)cbpc;
+    e->cur_qp = eff_qp;
+    e->mb_qps[idx] = (uint8_t)eff_qp;
+    recon_luma_4x4(py, q4, cbpl, eff_qp);
+    recon_chroma(pu, pv, c, cbpc, eff_qpc);
+    store_mb(e, mb_x, mb_y, py, pu, pv);
+}
+
+/* inter.rs:229 predict_inter (list 0 only), per 8x8 quadrant: luma from the reference and its
+ * half-sample planes, chroma by the 1/8-sample bilinear filter. Motion compensation is per
+ * sample, so quadrants give the prediction of any partitioning of equal motion. */
+static void predict_quads(const ora_enc *e, int mb_x, int mb_y, const int q[4][2], uint8_t *py, uint8_t *pu, uint8_t *pv) {
+    for (int k = 0; k < 4; k++) {
+        int bx = (k & 1) * 8, by = (k >> 1) * 8;
+        rl_pred(e, mb_x * 16 + bx, mb_y * 16 + by, 8, 8, q[k][0], q[k][1], py + by * 16 + bx, 16);
+        int cx = mb_x * 8 + bx / 2, cy = mb_y * 8 + by / 2, fx = q[k][0] & 7, fy = q[k][1] & 7;
+        int wa = (8 - fx) * (8 - fy), wb = fx * (8 - fy), wc = (8 - fx) * fy, wd = fx * fy;
+        for (int pl = 0; pl < 2; pl++) {
+            const plane_t *cp = pl ? &e->refp.v : &e->refp.u;
+            uint8_t *dst = (pl ? pv : pu) + (by / 2) * 8 + bx / 2;
+            long off = pat(cp, cx + (q[k][0] >> 3), cy + (q[k][1] >> 3));
+            for (int r = 0; r < 4; r++)
+                for (int c = 0; c < 4; c++) {
+                    const uint8_t *s0 = cp->buf + off + (long)r * cp->stride + c, *s1 = s0 + cp->stride;
+                    dst[r * 8 + c] = (uint8_t)((wa * s0[0] + wb * s0[1] + wc * s1[0] + wd * s1[1] + 32) >> 6);
+                }
+        }
+    }
+}
+
+/* inter.rs:498 code_p_mb_body (P pictures, CAVLC) */
+static void code_p_mb_compact(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const mbsrc_t *s) {
+    int mb_w = e->mb_w, gs = mb_w * 4, gx0 = mb_x * 4, gy0 = mb_y * 4, x0 = mb_x * 16, y0 = mb_y * 16;
+    int idx = mb_y * mb_w + mb_x;
+    ora_mb_decision *d = &e->dec[idx];
+    int16_t *lv = e->levels + (long)idx * ORA_LEVELS_PER_MB;
+    memset(lv, 0, ORA_LEVELS_PER_MB * sizeof(int16_t));
+    int qpc = chroma_qp(qp);
+    mvb_t b;
+    mvb_init(&b, x0, y0, e->refp.y.w, e->refp.y.h);
+    uint32_t lambda = lambda_compact(qp);
+    int16_t mvp[2];
+    pred_mv_part(e, gx0, gy0, 4, mvp);
+    int16_t pskip[2] = {mvp[0], mvp[1]};
+    {
+        int li = gy0 * gs + gx0 - 1, ti = (gy0 - 1) * gs + gx0;
+        if (gx0 == 0 || gy0 == 0 || (e->ref_grid[li] == 0 && e->mv_grid[li][0] == 0 && e->mv_grid[li][1] == 0) ||
+            (e->ref_grid[ti] == 0 && e->mv_grid[ti][0] == 0 && e->mv_grid[ti][1] == 0))
+            pskip[0] = pskip[1] = 0;
+    }
+    d->qp = (uint8_t)qp;
+    uint8_t py[256], pu[64], pv[64];
+    if (e->sb.qp != qp) skip_bounds_init(&e->sb, qp, qpc, 1);
+    /* The skip probe only next to a skipped macroblock (x264 fast P_Skip detection). */
+    const uint8_t *sk = e->mb_is_skip;
+    int probe = mvb_contains(&b, pskip[0], pskip[1]) &&
+                ((mb_x > 0 && sk[idx - 1]) ||
+                 (mb_y > 0 && (sk[idx - mb_w] || (mb_x > 0 && sk[idx - mb_w - 1]) || (mb_x + 1 < mb_w && sk[idx - mb_w + 1]))));
+    const int skipq[4][2] = {{pskip[0], pskip[1]}, {pskip[0], pskip[1]}, {pskip[0], pskip[1]}, {pskip[0], pskip[1]}};
+    if (try_p_skip(e, mb_x, mb_y, qp, qpc, s, pskip, probe, py, pu, pv)) {
+        e->mvr[idx][0] = pskip[0];
+        e->mvr[idx][1] = pskip[1];
+        d->mb_type = 0;
+        d->flags = 1;
+        set_compact_record(d, skipq, pskip);
+        return;
+    }
+    /* 1. 16x16 from the predictor, zero, the neighbours' and the previous picture's search vectors. */
+    int16_t cands[11][2];
+    int n = 0;
+#define PUSH(v) do { cands[n][0] = (v)[0]; cands[n][1] = (v)[1]; n++; } while (0)
+    PUSH(mvp);
+    if (mb_x > 0) PUSH(e->mvr[idx - 1]);
+    if (mb_y > 0) {
+        PUSH(e->mvr[idx - mb_w]);
+        if (mb_x > 0) PUSH(e->mvr[idx - mb_w - 1]);
+        if (mb_x + 1 < mb_w) PUSH(e->mvr[idx - mb_w + 1]);
+    }
+    PUSH(e->prev_mvr[idx]);
+    if (mb_x + 1 < mb_w) PUSH(e->prev_mvr[idx + 1]);
+    if (mb_y + 1 < e->mb_h) PUSH(e->prev_mvr[idx + mb_w]);
+#undef PUSH
+    ensure_mv_costs_c(e, qp);
+    ps_t ps;
+    ps_init(&ps, e, s->y, x0, y0, 16, 16, mvp, lambda, &b);
+    me_t m16 = ps_search(&ps, cands, n, COMPACT_ME_RANGE, 1);
+    int16_t mvr16[2] = {(int16_t)m16.mv[0], (int16_t)m16.mv[1]};
+    e->mvr[idx][0] = mvr16[0];
+    e->mvr[idx][1] = mvr16[1];
+    /* A 16x16 vector at the skip vector whose residual decimates away is P_Skip. */
+    if (!probe && m16.mv[0] == pskip[0] && m16.mv[1] == pskip[1] && mvb_contains(&b, pskip[0], pskip[1]) &&
+        try_p_skip(e, mb_x, mb_y, qp, qpc, s, pskip, 1, py, pu, pv)) {
+        d->mb_type = 0;
+        set_compact_record(d, skipq, mvr16);
+        return;
+    }
+    int p8x8 = 0, q[4][2];
+    for (int k = 0; k < 4; k++) {
+        q[k][0] = m16.mv[0];
+        q[k][1] = m16.mv[1];
+    }
+    uint32_t best_cost = me_cost(&m16);
+    me_t best_me[4] = {m16, m16, m16, m16};
+    /* 2. Four 8x8 searches seeded by the 16x16 vector and the partitions before them. */
+    if (best_cost > lambda * P8X8_GATE) {
+        me_t m8[4];
+        int16_t cands8[5][2];
+        cands8[0][0] = (int16_t)m16.mv[0];
+        cands8[0][1] = (int16_t)m16.mv[1];
+        for (int i = 0; i < 4; i++) {
+            int bx = (i & 1) * 2, by = (i >> 1) * 2;
+            int16_t mvp8[2];
+            pred_mv_part(e, gx0 + bx, gy0 + by, 2, mvp8);
+            ps_t p8;
+            ps_init(&p8, e, s->y + by * 4 * 16 + bx * 4, x0 + bx * 4, y0 + by * 4, 8, 8, mvp8, lambda, &b);
+            m8[i] = ps_search(&p8, cands8, i + 1, COMPACT_ME_RANGE, 1);
+            cands8[i + 1][0] = (int16_t)m8[i].mv[0];
+            cands8[i + 1][1] = (int16_t)m8[i].mv[1];
+            fill_part(e, gx0 + bx, gy0 + by, 2, 2, 0, m8[i].mv[0], m8[i].mv[1]);
+        }
+        uint32_t cost8 = 0;
+        for (int i = 0; i < 4; i++) cost8 += me_cost(&m8[i]);
+        if (cost8 < best_cost) {
+            p8x8 = 1;
+            for (int i = 0; i < 4; i++) {
+                q[i][0] = m8[i].mv[0];
+                q[i][1] = m8[i].mv[1];
+                best_me[i] = m8[i];
+            }
+        }
+    }
+    /* Quarter-pel refinement of the partitions chosen, each from the final vectors before it. */
+    best_cost = 0;
+    for (int part = 0; part < (p8x8 ? 4 : 1); part++) {
+        int bx = p8x8 ? (part & 1) * 2 : 0, by = p8x8 ? (part >> 1) * 2 : 0, w4 = p8x8 ? 2 : 4;
+        int16_t mvpp[2];
+        pred_mv_part(e, gx0 + bx, gy0 + by, w4, mvpp);
+        ps_t pr;
+        ps_init(&pr, e, s->y + by * 4 * 16 + bx * 4, x0 + bx * 4, y0 + by * 4, w4 * 4, w4 * 4, mvpp, lambda, &b);
+        me_t r = ps_refine_qpel(&pr, best_me[part], COMPACT_QPEL_REFINE);
+        best_cost += me_cost(&r);
+        fill_part(e, gx0 + bx, gy0 + by, w4, w4, 0, r.mv[0], r.mv[1]);
+        for (int k = 0; k < 4; k++) {
+            int kx = (k & 1) * 2, ky = (k >> 1) * 2;
+            if (kx >= bx && kx < bx + w4 && ky >= by && ky < by + w4) {
+                q[k][0] = r.mv[0];
+                q[k][1] = r.mv[1];
+            }
+        }
+    }
+    /* 3. Intra 16x16 and 4x4 against the best inter cost. */
+    edges_t ed;
+    get_edges(&e->cur.y, mb_x, mb_y, 16, &ed);
+    uint32_t i16_satd;
+    int i16_mode;
+    analyse_i16(&ed, s->y, &i16_satd, &i16_mode, NULL);
+    uint32_t i16_cost = i16_satd + lambda * 4;
+    static const int zq[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    if (i16_cost < best_cost) {
+        begin_coded_mb(e, w);
+        code_i16(e, w, mb_x, mb_y, qp, s, i16_mode);
+        set_compact_record(d, zq, mvr16);
+        return;
+    }
+    uint32_t lo = i16_cost < best_cost ? i16_cost : best_cost;
+    if (lo > lambda * 40 && (uint64_t)i16_cost * 2 < (uint64_t)best_cost * 3) {
+        i4_choice_t ch;
+        if (analyse_i4(e, mb_x, mb_y, qp, s, lo, &ch)) {
+            begin_coded_mb(e, w);
+            code_i4(e, w, mb_x, mb_y, qp, s, &ch);
+            uint64_t packed;
+            memcpy(&packed, d->mv, 8);
+            set_compact_record(d, zq, mvr16);
+            memcpy(d->mv, &packed, 8);
+            return;
+        }
+    }
+    /* 4. Prediction, residual and syntax. */
+    predict_quads(e, mb_x, mb_y, (const int(*)[2])q, py, pu, pv);
+    int16_t q4[16][16];
+    int cbpl = quant_luma_4x4_inter(s->y, py, qp, q4, 1);
+    chroma_levels_t c;
+    quant_chroma_inter(s, pu, pv, qpc, &c, 1);
+    int cbpc = chroma_cbp(&c);
+    if (cbpl == 0 && cbpc == 0 && !p8x8 && q[0][0] == pskip[0] && q[0][1] == pskip[1]) {
+        code_skip(e, mb_x, mb_y, pskip);
+        store_mb(e, mb_x, mb_y, py, pu, pv);
+        d->mb_type = 0;
+        set_compact_record(d, skipq, mvr16);
+        return;
+    }
+    begin_coded_mb(e, w);
+    code_inter_p(e, w, mb_x, mb_y, qp, p8x8, (const int(*)[2])q, cbpl, cbpc, q4, &c, py, pu, pv);
+    d->mb_type = p8x8 ? 5 : 1;
+    d->cbp_luma = (uint8_t)cbpl;
+    d->cbp_chroma = (uint8_t)cbpc;
+    d->qp = e->mb_qps[idx];
+    set_compact_record(d, (const int(*)[2])q, mvr16);
+    rec_luma4(lv, q4);
+    rec_chroma(lv, &c);
+}
+
+/* inter.rs:781 deblock_parts (P pictures): the boundary strength of each edge segment from the
+ * coefficients and the motion of the two 4x4 blocks beside it */
+static int uniform_inter(const ora_enc *e, long g0) { /* inter.rs:768 */
+    int gs = e->mb_w * 4;
+    for (int r = 0; r < 4; r++)
+        for (int c = 0; c < 4; c++) {
+            long g = g0 + r * gs + c;
+            if (e->luma_nnz[g] || e->ref_grid[g] != e->ref_grid[g0] || e->mv_grid[g][0] != e->mv_grid[g0][0] ||
+                e->mv_grid[g][1] != e->mv_grid[g0][1])
+                return 0;
+        }
+    return 1;
+}
+static void deblock_parts(ora_enc *e) {
+    int mb_w = e->mb_w, mb_h = e->mb_h, gs = mb_w * 4;
+    for (int mb_y = 0; mb_y < mb_h; mb_y++)
+        for (int mb_x = 0; mb_x < mb_w; mb_x++) {
+            int m = mb_y * mb_w + mb_x;
+            long g0 = (long)mb_y * 4 * gs + mb_x * 4;
+            int intra = e->mb_is_intra[m], t8 = e->mb_8x8[m];
+            int qp_q = e->mb_qps[m], qpc_q = chroma_qp(qp_q);
+            int edges = (!intra && uniform_inter(e, g0)) ? 1 : 4;
+            for (int dir = 0; dir < 2; dir++)
+                for (int ed = 0; ed < edges; ed++) {
+                    int pm;
+                    if (ed == 0) {
+                        if (dir == 0 && mb_x > 0)
+                            pm = m - 1;
+                        else if (dir == 1 && mb_y > 0)
+                            pm = m - mb_w;
+                        else
+                            continue;
+                    } else {
+                        if ((ed & 1) && t8) continue;
+                        pm = m;
+                    }
+                    int bs[4], any = 0;
+                    for (int k = 0; k < 4; k++) {
+                        if (intra || e->mb_is_intra[pm]) {
+                            bs[k] = ed == 0 ? 4 : 3;
+                        } else {
+                            long q = dir == 0 ? g0 + k * gs + ed : g0 + ed * gs + k;
+                            long pp = dir == 0 ? q - 1 : q - gs;
+                            if (e->luma_nnz[q] || e->luma_nnz[pp])
+                                bs[k] = 2;
+                            else
+                                bs[k] = e->ref_grid[q] != e->ref_grid[pp] || iabs(e->mv_grid[q][0] - e->mv_grid[pp][0]) >= 4 ||
+                                        iabs(e->mv_grid[q][1] - e->mv_grid[pp][1]) >= 4;
+                        }
+                        any |= bs[k];
+                    }
+                    if (!any) continue;
+                    int qp, qpc;
+                    if (pm == m) {
+                        qp = qp_q;
+                        qpc = qpc_q;
+                    } else {
+                        int qp_p = e->mb_qps[pm];
+                        qp = (qp_q + qp_p + 1) >> 1;
+                        qpc = (qpc_q + chroma_qp(qp_p) + 1) >> 1;
+                    }
+                    int x = dir == 0 ? mb_x * 16 + ed * 4 : mb_x * 16, y = dir == 0 ? mb_y * 16 : mb_y * 16 + ed * 4;
+                    edge_filter(&e->cur.y, x, y, dir, 0, qp, bs);
+                    if (!(ed & 1)) {
+                        edge_filter(&e->cur.u, x / 2, y / 2, dir, 1, qpc, bs);
+                        edge_filter(&e->cur.v, x / 2, y / 2, dir, 1, qpc, bs);
+                    }
+                }
+        }
+}
+
 /* ------------------------------------------------------------------------------------------ */
 /* Headers and frame driver (h264.rs:239-1127, 3032-3165; enc.rs:2234-2514, 2643-2672).        */
 /* ------------------------------------------------------------------------------------------ */
@@ -2628,14 +3301,14 @@ static void write_pps(const ora_enc *e, bits_t *w) {
     put32(w, 0, 2);
     put_se(w, e->qp - 26);
     put_se(w, 0);
-    put_se(w, 0);
+    put_se(w, e->cqo); /* chroma_qp_index_offset (h264.rs:3087 sync_reorder) */
     put32(w, 1, 1);
     put32(w, 0, 1);
     put32(w, 0, 1);
     if (e->t8x8) {
         put32(w, 1, 1);
         put32(w, 0, 1);
-        put_se(w, 0);
+        put_se(w, e->cqo);
     }
     bits_trailing(w);
 }
@@ -2719,13 +3392,15 @@ long ora_encode_frame(ora_enc *e, const uint8_t *y, const uint8_t *u, const uint
     e->is_idr = is_idr;
     e->skip_run = 0;
     e->cur_qp = slice_qp;
-    skip_bounds_init(&e->sb, slice_qp, chroma_qp(slice_qp));
+    skip_bounds_init(&e->sb, slice_qp, chroma_qp(slice_qp), e->compact);
     mbsrc_t s;
     for (int mb_y = 0; mb_y < mb_h; mb_y++)
         for (int mb_x = 0; mb_x < mb_w; mb_x++) {
             load_mb(&f, mb_x, mb_y, &s);
             if (is_idr)
                 code_intra(e, wp, mb_x, mb_y, slice_qp, &s);
+            else if (e->compact)
+                code_p_mb_compact(e, wp, mb_x, mb_y, slice_qp, &s);
             else
                 code_p_mb(e, wp, mb_x, mb_y, slice_qp, &s);
             uint16_t nnz = 0;
@@ -2741,7 +3416,12 @@ long ora_encode_frame(ora_enc *e, const uint8_t *y, const uint8_t *u, const uint
     copy_plane(&e->cur.y, e->pre_y);
     copy_plane(&e->cur.u, e->pre_u);
     copy_plane(&e->cur.v, e->pre_v);
-    if (slice_qp != 0) deblock(e);
+    if (slice_qp != 0) {
+        if (e->compact)
+            deblock_parts(e);
+        else
+            deblock(e);
+    }
     plane_extend(&e->cur.y);
     plane_extend(&e->cur.u);
     plane_extend(&e->cur.v);
@@ -2750,6 +3430,12 @@ long ora_encode_frame(ora_enc *e, const uint8_t *y, const uint8_t *u, const uint
     e->cur = e->refp;
     e->refp = t;
     e->has_ref = 1;
+    if (e->compact) {
+        hpel_planes_build(e);
+        int16_t(*t)[2] = e->prev_mvr;
+        e->prev_mvr = e->mvr;
+        e->mvr = t;
+    }
     int gs = mb_w * 4;
     for (int my = 0; my < mb_h; my++)
         for (int mx = 0; mx < mb_w; mx++) {
@@ -2834,6 +3520,22 @@ int ora_cavlc_block(const int16_t *scanned, int n, int nc, uint8_t *out, int cap
     if (w.len <= cap) memcpy(out, w.buf, (size_t)w.len);
     bits_free(&w);
     return nb;
+}
+/* Test hook: the Compact prediction (RefLuma::pred over the half-sample planes) of the bw x bh
+ * block at (x0, y0) of a w x h picture displaced by quarter-pel (mx, my). */
+int ora_compact_pred(const uint8_t *pic, int w, int h, int x0, int y0, int bw, int bh, int mx, int my, uint8_t *dst) {
+    ora_enc *e = ora_create(w, h, 30, 1, 26, 0, 30);
+    if (!e || e->mb_w * 16 != w || e->mb_h * 16 != h) {
+        ora_destroy(e);
+        return -1;
+    }
+    ora_set_compact(e, 1);
+    for (int r = 0; r < h; r++) memcpy(e->refp.y.buf + pat(&e->refp.y, 0, r), pic + (long)r * w, (size_t)w);
+    plane_extend(&e->refp.y);
+    hpel_planes_build(e);
+    rl_pred(e, x0, y0, bw, bh, mx, my, dst, bw);
+    ora_destroy(e);
+    return 0;
 }
 void ora_predict_intra4x4(int mode, const uint8_t *top8, const uint8_t *left4, int tl, int has_top, int has_left,
                           uint8_t *dst) {

@@ -34,7 +34,7 @@ extern "C" {
 
 /* Per-macroblock decision record, 32 bytes (layout shared with the product's rgh_mb_decision). */
 typedef struct ora_mb_decision {
-    uint8_t mb_type;      /* 0 P_Skip, 1 P_L0_16x16, 2 I_16x16, 3 I_4x4, 4 I_PCM */
+    uint8_t mb_type;      /* 0 P_Skip, 1 P_L0_16x16, 2 I_16x16, 3 I_4x4, 4 I_PCM, 5 P_8x8 */
     uint8_t i16_mode;     /* I_16x16 luma mode (0 V, 1 H, 2 DC, 3 plane) */
     uint8_t chroma_mode;  /* intra chroma mode (0 DC, 1 H, 2 V, 3 plane) */
     uint8_t cbp_luma;
@@ -68,6 +68,8 @@ ora_enc *ora_create(int width, int height, int fps_num, int fps_den, int qp, int
 void ora_destroy(ora_enc *e);
 /* Diamond iterations of the integer search (16 = the reference's value). */
 void ora_set_me_range(ora_enc *e, int iters);
+/* H264Encoder::with_speed(H264Speed::Compact) for Baseline (CAVLC) streams; -1 for CABAC. */
+int ora_set_compact(ora_enc *e, int on);
 /* Encodes one I420 frame. Writes the Annex-B access unit (SPS+PPS before an IDR slice) to out
  * and returns its length; returns -1 on a too small buffer. For CABAC profiles no bitstream is
  * written (returns 0) but every decision, level and reconstruction is produced. */

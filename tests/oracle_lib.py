@@ -32,6 +32,10 @@ def lib():
         L.ora_create.argtypes = [I, I, I, I, I, I, I]
         L.ora_destroy.argtypes = [P]
         L.ora_set_me_range.argtypes = [P, I]
+        L.ora_compact_pred.restype = I
+        L.ora_compact_pred.argtypes = [P, I, I, I, I, I, I, I, I, P]
+        L.ora_set_compact.restype = I
+        L.ora_set_compact.argtypes = [P, I]
         L.ora_encode_frame.restype = L64
         L.ora_encode_frame.argtypes = [P, P, P, P, I, I, I, P, L64]
         L.ora_headers.restype = L64
@@ -70,12 +74,14 @@ PROFILES = {"baseline": 0, "main": 1, "high": 2}
 class OracleEncoder:
     """Mirror of `H264Encoder::new(w, h, fps, qp).with_profile(p)` with `keyint` (the oracle)."""
 
-    def __init__(self, width, height, qp, fps=(30, 1), profile="baseline", keyint=30, me_range=16):
+    def __init__(self, width, height, qp, fps=(30, 1), profile="baseline", keyint=30, me_range=16, speed="balanced"):
         self.L = lib()
         self.h = self.L.ora_create(width, height, fps[0], fps[1], qp, PROFILES[profile], keyint)
         if not self.h:
             raise ValueError("unsupported dimensions %dx%d" % (width, height))
         self.L.ora_set_me_range(self.h, me_range)
+        if speed == "compact" and self.L.ora_set_compact(self.h, 1) != 0:
+            raise ValueError("the Compact preset codes B pictures with CABAC")
         self.width, self.height, self.qp = width, height, qp
         self.mb_w = self.L.ora_mb_w(self.h)
         self.mb_h = self.L.ora_mb_h(self.h)

@@ -205,3 +205,25 @@ def test_intra4x4_predictors_known_answers():
             k = x + y
             want = (t[6] + 3 * t[7] + 2) >> 2 if k == 6 else (t[k] + 2 * t[k + 1] + t[k + 2] + 2) >> 2
             assert out[y * 4 + x] == want
+
+
+def test_compact_half_sample_planes_give_the_spec_luma_mc():
+    # compact.rs:87 RefLuma::src pairs the b / h / j planes (compact.rs:24 HpelPlanes::build)
+    # into every quarter-sample position; the result must be the standard's luma MC
+    # (8.4.2.2.1), including blocks reaching into the replicated padding.
+    W, H, PAD = 48, 32, 48
+    rng = np.random.default_rng(7)
+    for it in range(3):
+        pic = np.ascontiguousarray(rng.integers(0, 256, (H, W)).astype(np.uint8))
+        padded = np.pad(pic, PAD, mode="edge")
+        ss = padded.shape[1]
+        flat = np.ascontiguousarray(padded).reshape(-1)
+        for (x0, y0, bw, bh) in ((0, 0, 16, 16), (16, 8, 8, 8), (32, 16, 16, 16), (8, 24, 8, 8)):
+            for mx in range(-4 * (x0 + 28), 4 * (W - bw - x0 + 28) + 1, 37):
+                for my in (-4 * (y0 + 28), -5, 0, 3, 6, 4 * (H - bh - y0 + 28)):
+                    got = np.zeros((bh, bw), np.uint8)
+                    assert lib().ora_compact_pred(ptr(pic), W, H, x0, y0, bw, bh, mx, my, ptr(got)) == 0
+                    want = np.zeros((16, 16), np.uint8)
+                    off = (PAD + y0 + (my >> 2)) * ss + PAD + x0 + (mx >> 2)
+                    lib().ora_luma_mc(ptr(want), 16, ptr(flat[off:]), ss, bw, bh, mx & 3, my & 3)
+                    assert (got == want[:bh, :bw]).all(), (x0, y0, mx, my)
