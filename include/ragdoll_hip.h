@@ -68,6 +68,8 @@ typedef struct rgh_config {
 /* rgh_config.flags: skip the device copy of each frame's reconstruction before deblocking
  * (rgh_get_outputs with which_recon = 0 then fails with RGH_ERR_INVALID_ARGUMENT). */
 #define RGH_FLAG_NO_UNDEBLOCKED_COPY 1
+/* rgh_config.flags: the Compact preset (rgh_speed RGH_SPEED_COMPACT); Baseline profile only. */
+#define RGH_FLAG_COMPACT 2
 
 /* An I420 picture: three planes with their strides; width/height are display sizes.
  * Chroma planes are (width+1)/2 x (height+1)/2. */
@@ -98,8 +100,17 @@ enum {
     RGH_MB_P_L0_16X16 = 1,
     RGH_MB_I16X16 = 2,
     RGH_MB_I4X4 = 3,
-    RGH_MB_I_PCM = 4
+    RGH_MB_I_PCM = 4,
+    RGH_MB_P_8X8 = 5 /* four P_L0_8x8 partitions (Compact preset) */
 };
+
+/* Encoder presets (H264Speed, h264.rs). */
+typedef enum rgh_speed {
+    RGH_SPEED_BALANCED = 0, /* diamond search of 16x16 vectors (enc.rs:1182-1339) */
+    RGH_SPEED_COMPACT = 1   /* hexagon search over half-sample planes, 8x8 partitions, I_4x4 in P,
+                               fine inter rounding, chroma QP offset 1 (enc/inter.rs, enc/compact.rs);
+                               Constrained Baseline only (with CABAC the preset codes B pictures) */
+} rgh_speed;
 
 /* Fixed 32-byte decision record per macroblock, raster order. */
 typedef struct rgh_mb_decision {
@@ -124,6 +135,12 @@ typedef struct rgh_mb_decision {
     uint16_t nnz;          /* 4x4 luma blocks holding levels, bit 4*by+bx */
     uint16_t reserved;
 } rgh_mb_decision;
+/* Compact-preset P pictures use the words from mv on differently: the 32-bit words at byte
+ * offsets 8, 12, 16 and 20 hold the quarter-pel vectors (x, y as int16) of the four 8x8
+ * quadrants in raster order (all four equal for P_Skip and P_L0_16x16, zero for intra; I_4x4
+ * modes as above in the first two), the word at offset 24 the macroblock's 16x16 search vector
+ * (the next picture's candidate, enc/inter.rs:549/:610 mvr). No predictor or counters are
+ * recorded: the syntax writer derives the predictors (8.4.1.3) from the quadrant vectors. */
 
 /* Quantized levels per macroblock (int16 elements), the reference's scan layout:
  *   [0, 256)   luma: 16 blocks (decoding order) x 16 zig-zag levels; I_16x16 AC blocks keep
@@ -206,6 +223,7 @@ typedef struct rgh_slice_config {
     int32_t qp;
     int32_t frame_num;
     int32_t poc_lsb;
+    int32_t speed; /* rgh_speed of the records */
 } rgh_slice_config;
 
 /* Entropy-codes one analysed P picture (decisions and levels in host memory, as
@@ -234,6 +252,7 @@ typedef struct rgh_stream_config {
     int32_t profile;       /* rgh_profile */
     int32_t qp;            /* the PPS's initial QP (the fixed QP) */
     int32_t fps_num, fps_den;
+    int32_t speed;         /* rgh_speed: chroma_qp_index_offset 1 for Compact */
 } rgh_stream_config;
 
 /* The stream's SPS and PPS as two Annex B NAL units (types 7 and 8), as the encoder emits them

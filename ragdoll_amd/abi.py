@@ -8,7 +8,9 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libragdoll_hip.so")
 LEVELS_PER_MB = 408
 PROFILES = {"baseline": 0, "main": 1, "high": 2}
+SPEEDS = {"balanced": 0, "compact": 1}
 FLAG_NO_UNDEBLOCKED_COPY = 1
+FLAG_COMPACT = 2
 
 DECISION_DTYPE = np.dtype([
     ("mb_type", "u1"), ("i16_mode", "u1"), ("chroma_mode", "u1"), ("cbp_luma", "u1"),
@@ -32,12 +34,13 @@ class RgConfig(ctypes.Structure):
 
 class RgSliceConfig(ctypes.Structure):
     _fields_ = [("mb_w", ctypes.c_int32), ("mb_h", ctypes.c_int32), ("qp", ctypes.c_int32),
-                ("frame_num", ctypes.c_int32), ("poc_lsb", ctypes.c_int32)]
+                ("frame_num", ctypes.c_int32), ("poc_lsb", ctypes.c_int32), ("speed", ctypes.c_int32)]
 
 
 class RgStreamConfig(ctypes.Structure):
     _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("profile", ctypes.c_int32),
-                ("qp", ctypes.c_int32), ("fps_num", ctypes.c_int32), ("fps_den", ctypes.c_int32)]
+                ("qp", ctypes.c_int32), ("fps_num", ctypes.c_int32), ("fps_den", ctypes.c_int32),
+                ("speed", ctypes.c_int32)]
 
 
 class RgI420(ctypes.Structure):
@@ -248,14 +251,14 @@ def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0,
            "rgh_analyse_frames")
 
 
-def write_p_slice_cavlc(decisions, levels, mb_w, mb_h, qp, frame_num, poc_lsb):
+def write_p_slice_cavlc(decisions, levels, mb_w, mb_h, qp, frame_num, poc_lsb, speed="balanced"):
     """The Annex B NAL unit (bytes) of one analysed P picture, Constrained Baseline CAVLC."""
     L = load_library()
     dec = np.ascontiguousarray(decisions, DECISION_DTYPE)
     lv = np.ascontiguousarray(levels, np.int16)
     if dec.shape != (mb_w * mb_h,) or lv.shape != (mb_w * mb_h, LEVELS_PER_MB):
         raise ValueError("decisions/levels do not match the macroblock grid")
-    cfg = RgSliceConfig(mb_w, mb_h, qp, frame_num, poc_lsb)
+    cfg = RgSliceConfig(mb_w, mb_h, qp, frame_num, poc_lsb, SPEEDS[speed])
     cap = mb_w * mb_h * 1024 + 64
     out = np.zeros(cap, np.uint8)
     n = ctypes.c_int64()
@@ -271,7 +274,7 @@ def write_idr_slice_cavlc(decisions, levels, mb_w, mb_h, qp, idr_pic_id):
     lv = np.ascontiguousarray(levels, np.int16)
     if dec.shape != (mb_w * mb_h,) or lv.shape != (mb_w * mb_h, LEVELS_PER_MB):
         raise ValueError("decisions/levels do not match the macroblock grid")
-    cfg = RgSliceConfig(mb_w, mb_h, qp, 0, 0)
+    cfg = RgSliceConfig(mb_w, mb_h, qp, 0, 0, 0)
     cap = mb_w * mb_h * 1024 + 64
     out = np.zeros(cap, np.uint8)
     n = ctypes.c_int64()
@@ -280,10 +283,10 @@ def write_idr_slice_cavlc(decisions, levels, mb_w, mb_h, qp, idr_pic_id):
     return bytes(out[:n.value])
 
 
-def write_headers(width, height, qp, profile="baseline", fps=(30, 1)):
+def write_headers(width, height, qp, profile="baseline", fps=(30, 1), speed="balanced"):
     """SPS and PPS NAL units (bytes, Annex B) of a stream."""
     L = load_library()
-    cfg = RgStreamConfig(width, height, PROFILES[profile], qp, fps[0], fps[1])
+    cfg = RgStreamConfig(width, height, PROFILES[profile], qp, fps[0], fps[1], SPEEDS[speed])
     out = np.zeros(1024, np.uint8)
     n = ctypes.c_int64()
     _check(L.rgh_write_headers(ctypes.byref(cfg), _ptr(out), 1024, ctypes.byref(n)), "rgh_write_headers")

@@ -247,6 +247,108 @@ void write_i16_body(BitWriter &w, NnzGrid &lg, NnzGrid *cg, int mb_x, int mb_y, 
     write_chroma(w, cg, mb_x, mb_y, d.cbp_chroma, lv);
 }
 
+// Motion of the picture's 4x4 blocks as coded so far (reference index: -1 intra, 0 list 0), for
+// the motion vector predictors of Compact records: h264.rs:3577 pred_motion_partition /
+// enc/inter.rs:104 neighbours and :143 mvp_from, reference 0, directional rule 0 (16x16 and
+// 8x8 partitions).
+struct MotionGrid {
+    int gs, gh;
+    std::vector<int8_t> ref;
+    std::vector<int16_t> mv;  // x, y per block
+    MotionGrid(int w4, int h4) : gs(w4), gh(h4), ref((size_t)w4 * h4, -2), mv((size_t)w4 * h4 * 2, 0) {}
+    void fill(int gx, int gy, int w4, int h4, int r, int mx, int my) {
+        for (int y = gy; y < gy + h4; y++)
+            for (int x = gx; x < gx + w4; x++) {
+                const size_t i = (size_t)y * gs + x;
+                ref[i] = (int8_t)r;
+                mv[2 * i] = (int16_t)mx;
+                mv[2 * i + 1] = (int16_t)my;
+            }
+    }
+    static int median3(int a, int b, int c) {
+        const int mx = a > b ? a : b, mn = a < b ? a : b, t = mn > c ? mn : c;
+        return mx < t ? mx : t;
+    }
+    void pred(int gx, int gy, int w4, int out[2]) const {
+        int ra = -2, rb = -2, rc = -2, a[2] = {0, 0}, b[2] = {0, 0}, c[2] = {0, 0};
+        auto at = [&](int x, int y, int &r, int *v) {
+            const size_t i = (size_t)y * gs + x;
+            r = ref[i];
+            v[0] = mv[2 * i];
+            v[1] = mv[2 * i + 1];
+        };
+        if (gx > 0) at(gx - 1, gy, ra, a);
+        if (gy > 0) at(gx, gy - 1, rb, b);
+        const int bx = gx & 3, by = gy & 3, trx = gx + w4;
+        const bool tr = gy > 0 && trx < gs &&
+                        !((by == 1 && (bx + w4 == 2 || bx + w4 == 4)) || (by == 2 && bx + w4 == 4) ||
+                          (by == 3 && (bx + w4 == 2 || bx + w4 == 4)));
+        if (tr && ref[(size_t)(gy - 1) * gs + trx] != -2)
+            at(trx, gy - 1, rc, c);
+        else if (gy > 0 && gx > 0)
+            at(gx - 1, gy - 1, rc, c);
+        const int match = (ra == 0) + (rb == 0) + (rc == 0);
+        if (match == 1) {
+            const int *v = ra == 0 ? a : (rb == 0 ? b : c);
+            out[0] = v[0];
+            out[1] = v[1];
+        } else if (match == 0 && rb == -2 && rc == -2 && ra != -2) {
+            out[0] = a[0];
+            out[1] = a[1];
+        } else {
+            out[0] = median3(a[0], b[0], c[0]);
+            out[1] = median3(a[1], b[1], c[1]);
+        }
+    }
+};
+
+// The quadrant vectors of a Compact record (raster order).
+void compact_quads(const rgh_mb_decision &d, int q[4][2]) {
+    const uint8_t *p = (const uint8_t *)&d + 8;
+    for (int k = 0; k < 4; k++) {
+        int16_t v[2];
+        memcpy(v, p + 4 * k, 4);
+        q[k][0] = v[0];
+        q[k][1] = v[1];
+    }
+}
+
+// An I_4x4 macroblock (enc.rs:3441 code_i4) after its mb_type: the 16 prediction modes as
+// predicted-mode flags and 3-bit remainders (8.3.1.1: the smaller of the left and top modes, DC
+// at the picture's edges; `modes` holds the coded modes, DC elsewhere), the chroma mode, the
+// intra CBP and the residual.
+rgh_status write_i4_body(BitWriter &w, NnzGrid &lg, NnzGrid *cg, std::vector<uint8_t> &modes, int mb_w, int mb_x,
+                         int mb_y, const rgh_mb_decision &d, const int16_t *lv) {
+    const int cbpl = d.cbp_luma, cbp = cbpl | (d.cbp_chroma << 4), gs = 4 * mb_w;
+    uint64_t packed;
+    memcpy(&packed, d.mv, sizeof(packed));
+    for (int b = 0; b < 16; b++) {
+        const int gx = 4 * mb_x + blk_x(b), gy = 4 * mb_y + blk_y(b);
+        const int m = (int)((packed >> (4 * b)) & 15);
+        if (m > 8) return RGH_ERR_INVALID_ARGUMENT;
+        int pm = 2;
+        if (gx > 0 && gy > 0) {
+            const int a = modes[(size_t)gy * gs + gx - 1], t = modes[(size_t)(gy - 1) * gs + gx];
+            pm = a < t ? a : t;
+        }
+        if (m == pm) {
+            w.put(1, 1);
+        } else {
+            w.put(0, 1);
+            w.put((uint32_t)(m < pm ? m : m - 1), 3);
+        }
+        modes[(size_t)gy * gs + gx] = (uint8_t)m;
+    }
+    w.ue(d.chroma_mode);
+    w.ue(intra4_cbp_code(cbp));
+    if (cbp) {
+        w.se(0);  // mb_qp_delta: fixed QP
+        write_luma4x4(w, lg, mb_x, mb_y, cbpl, lv);
+        write_chroma(w, cg, mb_x, mb_y, d.cbp_chroma, lv);
+    }
+    return RGH_OK;
+}
+
 }  // namespace
 
 extern "C" rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const rgh_mb_decision *decisions,
@@ -254,7 +356,10 @@ extern "C" rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const
     if (!cfg || !decisions || !levels || !out || !out_len || cap < 0) return RGH_ERR_INVALID_ARGUMENT;
     const int mb_w = cfg->mb_w, mb_h = cfg->mb_h;
     if (mb_w <= 0 || mb_h <= 0 || mb_w > 512 || mb_h > 512) return RGH_ERR_INVALID_DIMENSIONS;
-    if (cfg->qp < 0 || cfg->qp > 51 || cfg->frame_num < 0 || cfg->poc_lsb < 0) return RGH_ERR_INVALID_ARGUMENT;
+    if (cfg->qp < 0 || cfg->qp > 51 || cfg->frame_num < 0 || cfg->poc_lsb < 0 ||
+        (cfg->speed != RGH_SPEED_BALANCED && cfg->speed != RGH_SPEED_COMPACT))
+        return RGH_ERR_INVALID_ARGUMENT;
+    const bool compact = cfg->speed == RGH_SPEED_COMPACT;
     BitWriter w;
     // Slice header (h264.rs:1071 SliceHeader::write): first_mb_in_slice, slice_type P (5),
     // pic_parameter_set_id, frame_num (4 bits), pic_order_cnt_lsb (6 bits),
@@ -278,12 +383,21 @@ extern "C" rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const
     }
     NnzGrid lg(4 * mb_w, 4 * mb_h);
     NnzGrid cg[2] = {NnzGrid(2 * mb_w, 2 * mb_h), NnzGrid(2 * mb_w, 2 * mb_h)};
+    MotionGrid mg(compact ? 4 * mb_w : 0, compact ? 4 * mb_h : 0);
+    std::vector<uint8_t> modes((size_t)16 * mb_w * mb_h, 2);
     uint32_t skip_run = 0;
     for (int mb_y = 0; mb_y < mb_h; mb_y++)
         for (int mb_x = 0; mb_x < mb_w; mb_x++) {
             const int idx = mb_y * mb_w + mb_x;
             const rgh_mb_decision &d = decisions[idx];
             const int16_t *lv = levels + (size_t)idx * RGH_LEVELS_PER_MB;
+            int q[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+            if (compact) {
+                const bool intra = d.mb_type == RGH_MB_I16X16 || d.mb_type == RGH_MB_I4X4;
+                if (!intra) compact_quads(d, q);
+                if (d.mb_type != RGH_MB_P_8X8)  // the partitions fill their blocks as they are coded
+                    mg.fill(4 * mb_x, 4 * mb_y, 4, 4, intra ? -1 : 0, q[0][0], q[0][1]);
+            }
             if (d.mb_type == RGH_MB_P_SKIP) {
                 skip_run++;
                 continue;
@@ -293,11 +407,32 @@ extern "C" rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const
             if (cbpc > 2 || (cbpl & ~15)) return RGH_ERR_INVALID_ARGUMENT;
             w.ue(skip_run);
             skip_run = 0;
-            if (d.mb_type == RGH_MB_P_L0_16X16) {
+            if (d.mb_type == RGH_MB_P_L0_16X16 || (compact && d.mb_type == RGH_MB_P_8X8)) {
                 const int cbp = cbpl | (cbpc << 4);
-                w.ue(0);
-                w.se(d.mv[0] - d.mvp[0]);
-                w.se(d.mv[1] - d.mvp[1]);
+                if (!compact) {
+                    w.ue(0);
+                    w.se(d.mv[0] - d.mvp[0]);
+                    w.se(d.mv[1] - d.mvp[1]);
+                } else if (d.mb_type == RGH_MB_P_L0_16X16) {
+                    int mvp[2];
+                    mg.pred(4 * mb_x, 4 * mb_y, 4, mvp);
+                    w.ue(0);
+                    w.se(q[0][0] - mvp[0]);
+                    w.se(q[0][1] - mvp[1]);
+                } else {
+                    // P_8x8 of four P_L0_8x8 (enc/inter.rs:331-358): each partition's predictor
+                    // sees the partitions coded before it.
+                    w.ue(3);
+                    for (int k = 0; k < 4; k++) w.ue(0);
+                    for (int k = 0; k < 4; k++) {
+                        const int gx = 4 * mb_x + (k & 1) * 2, gy = 4 * mb_y + (k >> 1) * 2;
+                        int mvp[2];
+                        mg.pred(gx, gy, 2, mvp);
+                        w.se(q[k][0] - mvp[0]);
+                        w.se(q[k][1] - mvp[1]);
+                        mg.fill(gx, gy, 2, 2, 0, q[k][0], q[k][1]);
+                    }
+                }
                 w.ue(inter_cbp_code(cbp));
                 if (cbp) {
                     w.se(0);  // mb_qp_delta: fixed QP
@@ -308,8 +443,13 @@ extern "C" rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const
                 if (d.i16_mode > 3 || d.chroma_mode > 3 || (cbpl != 0 && cbpl != 15)) return RGH_ERR_INVALID_ARGUMENT;
                 w.ue(5u + 1u + d.i16_mode + 4u * (uint32_t)cbpc + (cbpl ? 12u : 0u));
                 write_i16_body(w, lg, cg, mb_x, mb_y, d, lv);
+            } else if (compact && d.mb_type == RGH_MB_I4X4) {
+                if (d.chroma_mode > 3) return RGH_ERR_INVALID_ARGUMENT;
+                w.ue(5);  // I_NxN in a P slice
+                rgh_status st = write_i4_body(w, lg, cg, modes, mb_w, mb_x, mb_y, d, lv);
+                if (st != RGH_OK) return st;
             } else {
-                return RGH_ERR_UNSUPPORTED;  // I_4x4 / I_PCM are not produced by the P analysis
+                return RGH_ERR_UNSUPPORTED;  // I_PCM; I_4x4 is produced by the Compact analysis only
             }
         }
     if (skip_run) w.ue(skip_run);
@@ -349,7 +489,6 @@ extern "C" rgh_status rgh_write_idr_slice_cavlc(const rgh_slice_config *cfg, int
     NnzGrid cg[2] = {NnzGrid(2 * mb_w, 2 * mb_h), NnzGrid(2 * mb_w, 2 * mb_h)};
     // Intra 4x4 prediction modes per 4x4 block (DC for I_16x16) for the predicted mode.
     std::vector<uint8_t> modes((size_t)16 * mb_w * mb_h, 2);
-    const int gs = 4 * mb_w;
     for (int mb_y = 0; mb_y < mb_h; mb_y++)
         for (int mb_x = 0; mb_x < mb_w; mb_x++) {
             const int idx = mb_y * mb_w + mb_x;
@@ -357,36 +496,9 @@ extern "C" rgh_status rgh_write_idr_slice_cavlc(const rgh_slice_config *cfg, int
             const int16_t *lv = levels + (size_t)idx * RGH_LEVELS_PER_MB;
             if (d.qp != cfg->qp || d.transform_8x8 || d.cbp_chroma > 2 || d.chroma_mode > 3) return RGH_ERR_UNSUPPORTED;
             if (d.mb_type == RGH_MB_I4X4) {
-                const int cbpl = d.cbp_luma, cbp = cbpl | (d.cbp_chroma << 4);
-                uint64_t packed;
-                memcpy(&packed, d.mv, sizeof(packed));
                 w.ue(0);
-                for (int b = 0; b < 16; b++) {
-                    const int gx = 4 * mb_x + blk_x(b), gy = 4 * mb_y + blk_y(b);
-                    const int m = (int)((packed >> (4 * b)) & 15);
-                    if (m > 8) return RGH_ERR_INVALID_ARGUMENT;
-                    // 8.3.1.1: the predicted mode is the smaller of the left and top modes, DC at
-                    // the picture's edges.
-                    int pm = 2;
-                    if (gx > 0 && gy > 0) {
-                        const int a = modes[(size_t)gy * gs + gx - 1], t = modes[(size_t)(gy - 1) * gs + gx];
-                        pm = a < t ? a : t;
-                    }
-                    if (m == pm) {
-                        w.put(1, 1);
-                    } else {
-                        w.put(0, 1);
-                        w.put((uint32_t)(m < pm ? m : m - 1), 3);
-                    }
-                    modes[(size_t)gy * gs + gx] = (uint8_t)m;
-                }
-                w.ue(d.chroma_mode);
-                w.ue(intra4_cbp_code(cbp));
-                if (cbp) {
-                    w.se(0);  // mb_qp_delta: fixed QP
-                    write_luma4x4(w, lg, mb_x, mb_y, cbpl, lv);
-                    write_chroma(w, cg, mb_x, mb_y, d.cbp_chroma, lv);
-                }
+                rgh_status st = write_i4_body(w, lg, cg, modes, mb_w, mb_x, mb_y, d, lv);
+                if (st != RGH_OK) return st;
             } else if (d.mb_type == RGH_MB_I16X16) {
                 const int cbpl = d.cbp_luma;
                 if (d.i16_mode > 3 || (cbpl != 0 && cbpl != 15)) return RGH_ERR_INVALID_ARGUMENT;
@@ -408,8 +520,9 @@ extern "C" rgh_status rgh_write_headers(const rgh_stream_config *cfg, uint8_t *o
         cfg->height > 8192)
         return RGH_ERR_INVALID_DIMENSIONS;
     if (cfg->profile < RGH_PROFILE_BASELINE || cfg->profile > RGH_PROFILE_HIGH || cfg->qp < 0 || cfg->qp > 51 ||
-        cfg->fps_num <= 0 || cfg->fps_den <= 0)
+        cfg->fps_num <= 0 || cfg->fps_den <= 0 || (cfg->speed != RGH_SPEED_BALANCED && cfg->speed != RGH_SPEED_COMPACT))
         return RGH_ERR_INVALID_ARGUMENT;
+    const int cqo = cfg->speed == RGH_SPEED_COMPACT ? 1 : 0;  // h264.rs:3087 sync_reorder
     const int mb_w = (cfg->width + 15) / 16, mb_h = (cfg->height + 15) / 16;
     const int profile_idc = cfg->profile == RGH_PROFILE_BASELINE ? 66 : (cfg->profile == RGH_PROFILE_MAIN ? 77 : 100);
     std::vector<uint8_t> nal;
@@ -483,14 +596,14 @@ extern "C" rgh_status rgh_write_headers(const rgh_stream_config *cfg, uint8_t *o
         w.put(0, 2);
         w.se(cfg->qp - 26);
         w.se(0);
-        w.se(0);
+        w.se(cqo);  // chroma_qp_index_offset
         w.put(1, 1);
         w.put(0, 1);
         w.put(0, 1);
         if (cfg->profile == RGH_PROFILE_HIGH) {
             w.put(1, 1);
             w.put(0, 1);
-            w.se(0);
+            w.se(cqo);
         }
         w.trailing();
         emit_nal(nal, 3, 8, w.bytes());

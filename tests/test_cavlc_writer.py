@@ -117,3 +117,26 @@ def test_writer_matches_oracle_idr_slices_and_headers(qp, profile):
         assert split_annexb(ora.headers()) == nals[:2]
         got = split_annexb(write_idr_slice_cavlc(ora.decisions(), ora.levels(), ora.mb_w, ora.mb_h, qp, k // 2))
         assert got == nals[2:], "qp %d frame %d" % (qp, k)
+
+
+@pytest.mark.parametrize("qp", [4, 14, 22, 28, 36, 46])
+def test_writer_matches_oracle_compact_slices(qp):
+    # The Compact preset's P slices (P_8x8 partitions with their own predictors, I_4x4 in P) and
+    # parameter sets (chroma_qp_index_offset 1), from the oracle's records.
+    kw = dict(CONFIGS["cif"])
+    w, h = kw.pop("width"), kw.pop("height")
+    g = SceneGenerator(w, h, seed=config_seed(3, qp), **kw)
+    ora = OracleEncoder(w, h, qp, keyint=1 << 20, speed="compact")
+    types = set()
+    for k in range(5):
+        nals = split_annexb(ora.encode(*g.next_frame()))
+        if k == 0:
+            assert split_annexb(write_headers(w, h, qp, speed="compact")) == nals[:2]
+            got = split_annexb(write_idr_slice_cavlc(ora.decisions(), ora.levels(), ora.mb_w, ora.mb_h, qp, 0))
+            assert got == nals[2:]
+            continue
+        nal = write_p_slice_cavlc(ora.decisions(), ora.levels(), ora.mb_w, ora.mb_h, qp, k & 15, (2 * k) & 63,
+                                  speed="compact")
+        assert split_annexb(nal) == nals[-1:], "qp %d frame %d" % (qp, k)
+        types |= set(np.unique(ora.decisions()["mb_type"]).tolist())
+    assert 5 in types and 1 in types
