@@ -68,7 +68,10 @@ typedef struct rgh_config {
 /* rgh_config.flags: skip the device copy of each frame's reconstruction before deblocking
  * (rgh_get_outputs with which_recon = 0 then fails with RGH_ERR_INVALID_ARGUMENT). */
 #define RGH_FLAG_NO_UNDEBLOCKED_COPY 1
-/* rgh_config.flags: the Compact preset (rgh_speed RGH_SPEED_COMPACT); Baseline profile only. */
+/* rgh_config.flags: the Compact preset (rgh_speed RGH_SPEED_COMPACT); Baseline profile only.
+ * A context's stream starts at its IDR (GopRingEncoder's encoder per GOP): the P picture after
+ * an IDR takes zero 16x16 search vectors of the previous picture as candidates, where the
+ * reference, continuing across a mid-stream IDR, takes those of the picture two before it. */
 #define RGH_FLAG_COMPACT 2
 
 /* An I420 picture: three planes with their strides; width/height are display sizes.

@@ -135,10 +135,10 @@ class StreamAnalyzer:
     """One stream's P-frame analysis context (mirror of an H264Encoder's P path)."""
 
     def __init__(self, width, height, qp, profile="baseline", me_range=0, qpel_iters=0, device=0,
-                 keep_undeblocked=True, ring=0):
+                 keep_undeblocked=True, ring=0, speed="balanced"):
         self.L = load_library()
-        cfg = RgConfig(width, height, PROFILES[profile], qp, me_range, qpel_iters, device,
-                       0 if keep_undeblocked else FLAG_NO_UNDEBLOCKED_COPY, ring)
+        flags = (0 if keep_undeblocked else FLAG_NO_UNDEBLOCKED_COPY) | (FLAG_COMPACT if speed == "compact" else 0)
+        cfg = RgConfig(width, height, PROFILES[profile], qp, me_range, qpel_iters, device, flags, ring)
         h = ctypes.c_void_p()
         _check(self.L.rgh_create(ctypes.byref(cfg), ctypes.byref(h)), "rgh_create")
         self.h = h

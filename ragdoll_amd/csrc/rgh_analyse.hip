@@ -986,7 +986,7 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
     const int rr = lane >> 2, c0 = (lane & 3) * 4;
     const int mb_w = J.mb_w, idx = mb_y * mb_w + mb_x;
     const int x0 = mb_x * 16, y0 = mb_y * 16, cx0 = mb_x * 8, cy0 = mb_y * 8;
-    const int qp = J.qp, qpc = c_chroma_qp[qp];
+    const int qp = J.qp, qpc = chroma_qp_of(qp, J.cqo);
     const uint32_t lambda = c_lambda[qp];
     const int W = mb_w * 16, H = J.mb_h * 16;
 
@@ -1779,7 +1779,7 @@ __device__ void analyse_intra_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S)
     const int lane = threadIdx.x, rr = lane >> 2, c0 = (lane & 3) * 4;
     const int mb_w = J.mb_w, idx = mb_y * mb_w + mb_x;
     const int x0 = mb_x * 16, y0 = mb_y * 16, cx0 = mb_x * 8, cy0 = mb_y * 8;
-    const int qp = J.qp, qpc = c_chroma_qp[qp];
+    const int qp = J.qp, qpc = chroma_qp_of(qp, J.cqo);
     const uint32_t lambda = c_lambda[qp];
     const int W = mb_w * 16;
     const bool has_l = mb_x > 0, has_t = mb_y > 0;
@@ -1955,9 +1955,23 @@ __device__ void analyse_intra_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S)
     }
 }
 
+#include "rgh_compact.h"
+
 union StepLds {
     MbLds a;
     DbLds d;
+};
+union CompactStepLds {
+    CompactLds a;
+    DbLds d;
+};
+template <int MODE>
+struct StepLdsOf {
+    using T = StepLds;
+};
+template <>
+struct StepLdsOf<2> {
+    using T = CompactStepLds;
 };
 
 // One launch of a frame schedule. Block row blockIdx.y is frame job j0 + blockIdx.y; the frame's
@@ -1973,9 +1987,10 @@ union StepLds {
 // and the whole frame repeats the step then (items already done skip themselves by their
 // stamps). Nothing waits inside a launch.
 //
-// IDR jobs run the same schedule in their own kernel (rgh_k_intra_step), so that the intra 4x4
-// search's registers do not lower the occupancy of the P analysis.
-template <bool INTRA>
+// IDR jobs run the same schedule in their own kernel (rgh_k_intra_step), and Compact-preset P
+// pictures in theirs (rgh_k_compact_step), so that each analysis has its own register and LDS
+// budget. MODE: 0 Balanced P, 1 IDR, 2 Compact P.
+template <int MODE>
 __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int j0, int launch, uint32_t seq) {
     const RghJobD J = device_view(jobs[j0 + (int)blockIdx.y]);
     RGH_G RghFrameCtl *ctl = J.ctl;
@@ -1986,7 +2001,7 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
     if (blockIdx.x == 0 && lane == 0 && delay > 0) atomicMax(next_delay, delay);
     const int na = min(J.mb_h, (J.mb_w + 1) >> 1);
     const int b = (int)blockIdx.x;
-    __shared__ StepLds S;
+    __shared__ typename StepLdsOf<MODE>::T S;
     if (b < na) {
         if (s < 0 || s >= J.steps) return;
         const int a = s - J.mb_w + 1;
@@ -1997,8 +2012,10 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
         bool ok = true;
         if (s == 0 && J.wait_analysed) ok = __atomic_load_n(J.wait_analysed, __ATOMIC_RELAXED) >= J.wait_total;
         if (ok) {
-            if constexpr (INTRA)
+            if constexpr (MODE == 1)
                 analyse_intra_mb(J, x, y, S.a);
+            else if constexpr (MODE == 2)
+                ok = analyse_compact_mb(J, x, y, S.a, seq);
             else
                 ok = analyse_mb(J, x, y, S.a, seq);
         }
@@ -2026,12 +2043,17 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
 
 extern "C" __global__ __launch_bounds__(64) void rgh_k_frame_step(const RghJob *__restrict__ jobs, int j0, int launch,
                                                                    uint32_t seq) {
-    frame_step<false>(jobs, j0, launch, seq);
+    frame_step<0>(jobs, j0, launch, seq);
 }
 
 extern "C" __global__ __launch_bounds__(64) void rgh_k_intra_step(const RghJob *__restrict__ jobs, int j0, int launch,
                                                                    uint32_t seq) {
-    frame_step<true>(jobs, j0, launch, seq);
+    frame_step<1>(jobs, j0, launch, seq);
+}
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_compact_step(const RghJob *__restrict__ jobs, int j0, int launch,
+                                                                     uint32_t seq) {
+    frame_step<2>(jobs, j0, launch, seq);
 }
 
 // Fills n 64-bit words (stamps) with v.

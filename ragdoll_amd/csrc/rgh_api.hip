@@ -14,6 +14,7 @@
 
 extern "C" __global__ void rgh_k_frame_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_intra_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
+extern "C" __global__ void rgh_k_compact_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_extend(const RghJob *jobs);
 extern "C" __global__ void rgh_k_fill_u64(uint64_t *p, int n, uint64_t v);
 
@@ -32,9 +33,10 @@ int pos_type(int i) {
 }
 
 // enc.rs:1784 zero_sad_bound / :2122 zero_ssd_bound for the inter quantizer at qp.
-void zero_bounds(int qp, uint32_t *sad, uint32_t *ssd, uint32_t *probe) {
+// fine: the Compact preset's intra-style rounding of inter residuals (enc.rs:2135 with fine).
+void zero_bounds(int qp, bool fine, uint32_t *sad, uint32_t *ssd, uint32_t *probe) {
     static const uint32_t Wt[4] = {1, 2, 1, 2}, Nn[4] = {4, 10, 4, 10};
-    uint32_t shift = 15 + (uint32_t)(qp / 6), bias = (1u << shift) / 6u;
+    uint32_t shift = 15 + (uint32_t)(qp / 6), bias = (1u << shift) / (fine ? 3u : 6u);
     uint32_t room = (1u << shift) - bias - 1, ms = UINT32_MAX, mq = UINT32_MAX;
     for (int i = 0; i < 16; i++) {
         uint32_t mf = (uint32_t)kQuant4Mf[qp % 6][pos_type(i)];
@@ -63,6 +65,8 @@ struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int width = 0, height = 0, mb_w = 0, mb_h = 0, profile = 0, qp = 0, me_iters = 16, qpel_iters = 2;
+    bool compact = false;  // RGH_FLAG_COMPACT
+    int cqo = 0;           // chroma_qp_index_offset
     int sy = 0, sc = 0, rows_y = 0, rows_c = 0;
     std::vector<Picture> pic;  // ring of reconstructed pictures
     int ref = 0;               // slot of the reference picture (the last frame analysed)
@@ -178,6 +182,8 @@ RghJob make_job(Ctx *c, int k, const rgh_i420 &src, rgh_mb_decision *dec, const 
     j.sb_probe = c->sb[4];
     j.steps = frame_steps(c);
     j.deblock = c->qp != 0;
+    j.compact = c->compact ? 1 : 0;
+    j.cqo = c->cqo;
     j.base = base;
     j.serial = c->serial + 1 + (uint32_t)k;
     j.ref_serial = c->serial + (uint32_t)k;
@@ -208,6 +214,7 @@ rgh_status ensure_jobs(Ctx *c, int n);
 struct Schedule {
     Ctx *owner = nullptr;
     int n = 0, n_frames = 0, lag = 1, nj = 0, n_intra = 0;
+    bool compact = false;
     std::vector<RghJob> jobs;
     std::vector<int> span, last;
     int gx = 1, max_span = 0, planned = 0, L = 0, extra_total = 0;
@@ -217,6 +224,13 @@ struct Schedule {
         // IDR jobs lead (frame-major order, IDR only as a stream's first frame).
         n_intra = 0;
         while (n_intra < nj && jobs[(size_t)n_intra].intra) n_intra++;
+        // One P kernel per schedule: every job of it uses one preset.
+        compact = nj > 0 && jobs[0].compact;
+        for (int j = 0; j < nj; j++)
+            if ((jobs[(size_t)j].compact != 0) != compact) {
+                set_error("run_schedule", "contexts of one call must use one preset", __LINE__);
+                return RGH_ERR_INVALID_ARGUMENT;
+            }
         for (int j = n_intra; j < nj; j++)
             if (jobs[(size_t)j].intra) {
                 set_error("run_schedule", "IDR jobs must lead the schedule", __LINE__);
@@ -247,8 +261,8 @@ struct Schedule {
             hipLaunchKernelGGL(rgh_k_intra_step, dim3((unsigned)gx, (unsigned)(split - j_lo)), dim3(64), 0, owner->stream,
                                owner->jobs_dev, j_lo, L, seq);
         if (j_hi > split)
-            hipLaunchKernelGGL(rgh_k_frame_step, dim3((unsigned)gx, (unsigned)(j_hi - split)), dim3(64), 0,
-                               owner->stream, owner->jobs_dev, split, L, seq);
+            hipLaunchKernelGGL(compact ? rgh_k_compact_step : rgh_k_frame_step, dim3((unsigned)gx, (unsigned)(j_hi - split)),
+                               dim3(64), 0, owner->stream, owner->jobs_dev, split, L, seq);
         for (int j = j_lo; j < j_hi; j++) last[(size_t)j] = L;
         L++;
     }
@@ -413,17 +427,24 @@ rgh_status rgh_create(const rgh_config *cfg, void **ctx_out) {
     c->me_iters = cfg->me_range ? cfg->me_range : 16;
     c->qpel_iters = cfg->qpel_iters ? cfg->qpel_iters : 2;
     c->keep_pre = !(cfg->flags & RGH_FLAG_NO_UNDEBLOCKED_COPY);
+    // The Compact preset: with CABAC it codes B pictures (h264.rs:3075), not analysed here.
+    c->compact = (cfg->flags & RGH_FLAG_COMPACT) != 0;
+    if (c->compact && cfg->profile != RGH_PROFILE_BASELINE) {
+        delete c;
+        return RGH_ERR_UNSUPPORTED;
+    }
+    c->cqo = c->compact ? 1 : 0;
     const int W = c->mb_w * 16, H = c->mb_h * 16;
     c->sy = (W + 2 * RGH_PAD_Y + 31) & ~31;
     c->sc = (W / 2 + 2 * RGH_PAD_C + 31) & ~31;
     c->rows_y = H + 2 * RGH_PAD_Y;
     c->rows_c = H / 2 + 2 * RGH_PAD_C;
-    zero_bounds(c->qp, &c->sb[0], &c->sb[1], &c->sb[4]);
+    zero_bounds(c->qp, c->compact, &c->sb[0], &c->sb[1], &c->sb[4]);
     {
         // enc.rs:2131 SkipBounds::new: the chroma bounds also keep the 2x2 DC level zero.
-        int qpc = kChromaQp[c->qp];
+        int qpc = kChromaQp[std::min(c->qp + c->cqo, 51)];
         uint32_t cs, css;
-        zero_bounds(qpc, &cs, &css, nullptr);
+        zero_bounds(qpc, c->compact, &cs, &css, nullptr);
         uint32_t dc_shift = 16 + (uint32_t)qpc / 6;
         uint32_t dc_room = (1u << dc_shift) - (1u << dc_shift) / 3 - 1;
         uint32_t dc = dc_room / (uint32_t)kQuant4Mf[qpc % 6][0];

@@ -105,10 +105,26 @@ __device__ void chroma_line(uint8_t *b, int q0, int s, const Edge &E, int k) {
     }
 }
 
-__device__ __forceinline__ int is_intra(const rgh_mb_decision &d) { return d.mb_type >= 2; }
+__device__ __forceinline__ int is_intra(const rgh_mb_decision &d) {
+    return d.mb_type == RGH_MB_I16X16 || d.mb_type == RGH_MB_I4X4 || d.mb_type == RGH_MB_I_PCM;
+}
 __device__ __forceinline__ int quiet(const rgh_mb_decision &d) { return !is_intra(d) && d.cbp_luma == 0; }
 __device__ __forceinline__ uint32_t nz_col(uint32_t nnz, int c) {
     return ((nnz >> c) & 1) | (((nnz >> (4 + c)) & 1) << 1) | (((nnz >> (8 + c)) & 1) << 2) | (((nnz >> (12 + c)) & 1) << 3);
+}
+
+// Vector of quadrant k (raster) of a Compact-preset record (words 2-5); zero for intra.
+__device__ __forceinline__ uint32_t quad_word(const rgh_mb_decision &d, int k) {
+    const uint32_t w0 = (uint32_t)(uint16_t)d.mv[0] | ((uint32_t)(uint16_t)d.mv[1] << 16);
+    const uint32_t w1 = (uint32_t)(uint16_t)d.mvp[0] | ((uint32_t)(uint16_t)d.mvp[1] << 16);
+    const uint32_t w2 = d.cost, w3 = (uint32_t)d.n_sad | ((uint32_t)d.n_subpel << 16);
+    return is_intra(d) ? 0u : (k == 0 ? w0 : (k == 1 ? w1 : (k == 2 ? w2 : w3)));
+}
+// Boundary strength 0 / 1 of two inter 4x4 blocks without coefficients: their vectors (packed
+// x, y) four or more quarter samples apart in a component (8.7.2.1, one reference).
+__device__ __forceinline__ int motion_bs(uint32_t a, uint32_t b) {
+    return iabs((int)(int16_t)(a & 0xFFFF) - (int)(int16_t)(b & 0xFFFF)) >= 4 ||
+           iabs((int)(int16_t)(a >> 16) - (int)(int16_t)(b >> 16)) >= 4;
 }
 
 __device__ void deblock_mb(const RghJobD &J, int mb_x, int mb_y, DbLds &S) {
@@ -118,7 +134,13 @@ __device__ void deblock_mb(const RghJobD &J, int mb_x, int mb_y, DbLds &S) {
     if (mb_x > 0) L = load_dec(&J.dec[m - 1]);
     if (mb_y > 0) T = load_dec(&J.dec[m - mb_w]);
     auto mveq = [](const rgh_mb_decision &a, const rgh_mb_decision &b) { return a.mv[0] == b.mv[0] && a.mv[1] == b.mv[1]; };
-    if (quiet(Q) && (mb_x == 0 || (quiet(L) && mveq(L, Q))) && (mb_y == 0 || (quiet(T) && mveq(T, Q)))) return;
+    // Compact pictures: every 4x4 block carries its quadrant's vector (enc/inter.rs:781
+    // deblock_parts); the internal edges of an inter macroblock without coefficients and with
+    // one vector throughout are not filtered.
+    const bool parts = J.compact != 0;
+    const bool uniform = !is_intra(Q) && Q.nnz == 0 && quad_word(Q, 0) == quad_word(Q, 1) &&
+                         quad_word(Q, 0) == quad_word(Q, 2) && quad_word(Q, 0) == quad_word(Q, 3);
+    if (!parts && quiet(Q) && (mb_x == 0 || (quiet(L) && mveq(L, Q))) && (mb_y == 0 || (quiet(T) && mveq(T, Q)))) return;
 
     const int x0 = mb_x * 16, y0 = mb_y * 16, cx0 = mb_x * 8, cy0 = mb_y * 8;
     // Stage luma rows y0-4 .. y0+15, columns x0-4 .. x0+15 (5 aligned dwords a row) and chroma
@@ -146,8 +168,8 @@ __device__ void deblock_mb(const RghJobD &J, int mb_x, int mb_y, DbLds &S) {
     }
     __syncthreads();
 
-    const int intra = is_intra(Q), internal = intra || Q.cbp_luma != 0, skip_odd = Q.transform_8x8;
-    const int qp_q = Q.qp, qpc_q = c_chroma_qp[iclamp(qp_q, 0, 51)];
+    const int intra = is_intra(Q), internal = parts ? !uniform : (intra || Q.cbp_luma != 0), skip_odd = Q.transform_8x8;
+    const int qp_q = Q.qp, qpc_q = chroma_qp_of(qp_q, J.cqo);
     for (int dir = 0; dir < 2; dir++) {
         for (int e = 0; e < 4; e++) {
             // The macroblock on the p side of the edge: the left or top neighbour for edge 0, else
@@ -168,6 +190,22 @@ __device__ void deblock_mb(const RghJobD &J, int mb_x, int mb_y, DbLds &S) {
             if (intra || p_intra) {
 #pragma unroll
                 for (int k = 0; k < 4; k++) bs[k] = ext ? 4 : 3;
+            } else if (parts) {
+                // Segment k: q block (e, k) (vertical edges) or (k, e) of this macroblock, p
+                // block one to the left / above, in the neighbour for edge 0.
+                int any = 0;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int qx = dir == 0 ? e : k, qy = dir == 0 ? k : e;
+                    const int px = dir == 0 ? (e ? e - 1 : 3) : k, py = dir == 0 ? k : (e ? e - 1 : 3);
+                    const int qn = (Q.nnz >> (4 * qy + qx)) & 1, pn = (p_nnz >> (4 * py + px)) & 1;
+                    const uint32_t qv = quad_word(Q, (qx >> 1) + (qy & 2));
+                    const int pk = (px >> 1) + (py & 2);
+                    const uint32_t pv = ext ? (dir == 0 ? quad_word(L, pk) : quad_word(T, pk)) : quad_word(Q, pk);
+                    bs[k] = (qn | pn) ? 2 : motion_bs(qv, pv);
+                    any |= bs[k];
+                }
+                if (!any) continue;
             } else {
                 uint32_t coded;
                 if (dir == 0 && ext)
@@ -186,7 +224,7 @@ __device__ void deblock_mb(const RghJobD &J, int mb_x, int mb_y, DbLds &S) {
             int qp = qp_q, qpc = qpc_q;
             if (ext) {
                 qp = (qp_q + qp_p + 1) >> 1;
-                qpc = (qpc_q + c_chroma_qp[iclamp(qp_p, 0, 51)] + 1) >> 1;
+                qpc = (qpc_q + chroma_qp_of(qp_p, J.cqo) + 1) >> 1;
             }
             Edge EL = make_edge(qp, bs);
             if (EL.on && lane < 16) {

@@ -44,6 +44,8 @@ struct RghJob {
     int32_t steps;             // wavefront steps of the frame: (mb_w - 1) + 2 (mb_h - 1) + 1
     int32_t deblock;           // 0 when the slice disables the loop filter
     int32_t intra;             // an IDR picture (no reference)
+    int32_t compact;           // the Compact preset (enc/inter.rs, enc/compact.rs)
+    int32_t cqo;               // chroma_qp_index_offset
     // Schedule.
     int32_t base;              // launch of the frame's first wavefront step
     uint32_t serial, ref_serial;  // this frame's and its reference's serial numbers
@@ -72,7 +74,7 @@ struct RghJobD {
     const RGH_G rgh_mb_decision *prev;
     RGH_G int16_t *levels;
     uint32_t sb_luma, sb_luma_ssd, sb_chroma, sb_chroma_ssd, sb_probe;
-    int32_t steps, deblock, intra, base;
+    int32_t steps, deblock, intra, compact, cqo, base;
     uint32_t serial, ref_serial;
     RGH_G uint64_t *ana;
     RGH_G uint64_t *fin;
@@ -125,6 +127,8 @@ __device__ __forceinline__ RghJobD device_view(const RghJob &j) {
     d.steps = j.steps;
     d.deblock = j.deblock;
     d.intra = j.intra;
+    d.compact = j.compact;
+    d.cqo = j.cqo;
     d.base = j.base;
     d.serial = j.serial;
     d.ref_serial = j.ref_serial;
@@ -191,6 +195,7 @@ __device__ __forceinline__ void store_dec(RGH_G rgh_mb_decision *p, const rgh_mb
 }
 
 __device__ __forceinline__ int clip255(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
+
 
 // clip255(v >> S) (arithmetic shift), computed as a clamp before a logical shift. Written as
 // shift-then-clip, two such results packed into one dword are selected by the gfx950 backend as

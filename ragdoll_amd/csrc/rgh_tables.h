@@ -42,6 +42,12 @@ RGH_TABLE_QUALIFIER uint8_t c_chroma_qp[52] = {
     18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 29, 30, 31, 32, 32, 33,
     34, 34, 35, 35, 36, 36, 37, 37, 37, 38, 38, 38, 39, 39, 39, 39};
 
+// QP'c of luma QP qp under the picture parameter set's chroma_qp_index_offset (enc.rs:540).
+__device__ __forceinline__ int chroma_qp_of(int qp, int cqo) {
+    const int q = qp + cqo;
+    return c_chroma_qp[q < 0 ? 0 : (q > 51 ? 51 : q)];
+}
+
 RGH_TABLE_QUALIFIER uint8_t c_alpha[52] = {
     0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,  0,   0,   4,   4,
     5,  6,  7,  8,  9,  10, 12, 13, 15, 17, 20, 22, 25, 28, 32,  36,  40,  45,

@@ -41,14 +41,14 @@ def check_frame(gpu, ora, dec, lv, rec, label, i):
 
 
 def run_parity(frames, width, height, qp, profile="baseline", me_range=16, label="", keyint=1 << 20,
-               gpu_idr=False):
+               gpu_idr=False, speed="balanced"):
     """Codes frames with the oracle and the GPU context side by side. IDR pictures are set as
     the GPU's reference from the oracle's reconstruction, or, with gpu_idr, analysed on the GPU
     (rgh_analyse_idr) and checked like P pictures."""
     from ragdoll_amd import StreamAnalyzer
-    ora = OracleEncoder(width, height, qp, profile=profile, keyint=keyint, me_range=me_range)
-    gpu = StreamAnalyzer(width, height, qp, profile=profile, me_range=me_range)
-    stats = {"frames": 0, "mbs": 0, "skip": 0, "inter": 0, "intra": 0, "idr": 0, "i4": 0}
+    ora = OracleEncoder(width, height, qp, profile=profile, keyint=keyint, me_range=me_range, speed=speed)
+    gpu = StreamAnalyzer(width, height, qp, profile=profile, me_range=me_range, speed=speed)
+    stats = {"frames": 0, "mbs": 0, "skip": 0, "inter": 0, "intra": 0, "idr": 0, "i4": 0, "p8x8": 0}
     try:
         for i, (y, u, v) in enumerate(frames):
             ora.encode(y, u, v)
@@ -70,6 +70,8 @@ def run_parity(frames, width, height, qp, profile="baseline", me_range=16, label
             stats["skip"] += int((dec["mb_type"] == 0).sum())
             stats["inter"] += int((dec["mb_type"] == 1).sum())
             stats["intra"] += int((dec["mb_type"] == 2).sum())
+            stats["i4"] += int((dec["mb_type"] == 3).sum())
+            stats["p8x8"] += int((dec["mb_type"] == 5).sum())
     finally:
         gpu.close()
     return stats
