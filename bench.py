@@ -163,8 +163,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--lag", type=int, default=0, help="frame lag in wavefront steps (0: autotune)")
     ap.add_argument("--groups", type=int, default=0, help="concurrent schedules (0: autotune)")
-    ap.add_argument("--workload", choices=["headline", "batched"], default="headline",
-                    help="headline: the 1080p metric (default); batched: the 24-stream uneven-size set")
+    ap.add_argument("--workload", choices=["headline", "batched", "compact"], default="headline",
+                    help="headline: the 1080p metric (default); batched: the 24-stream uneven-size set; "
+                         "compact: the headline streams with the Compact preset (Baseline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -179,6 +180,8 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         gops, label = batched_gops(), "batched"
+    elif args.workload == "compact":
+        gops, label = [dict(g, profile="baseline", speed="compact") for g in headline_gops()], "compact"
     else:
         gops, label = headline_gops(), "headline"
     out = run_workload(gops, args)
@@ -226,8 +229,8 @@ def run_workload(gops, args):
 
     ans, srcs, decs, lvls = [], [], [], []
     for gp in gops:
-        a = StreamAnalyzer(gp["w"], gp["h"], gp["qp"], profile="high", me_range=gp["me"], keep_undeblocked=False,
-                           ring=len(gp["frames"]) + 1)
+        a = StreamAnalyzer(gp["w"], gp["h"], gp["qp"], profile=gp.get("profile", "high"), me_range=gp["me"],
+                           keep_undeblocked=False, ring=len(gp["frames"]) + 1, speed=gp.get("speed", "balanced"))
         ans.append(a)
         srcs.append([(dev(y), dev(u), dev(v), y.shape[1], u.shape[1], v.shape[1]) for y, u, v in gp["frames"]])
         n = a.mb_w * a.mb_h
@@ -269,10 +272,23 @@ def run_workload(gops, args):
     avg_launch_s = dev_ms / 1e3 / launches
     mbs_per_launch = (total_mbs + idr_mbs) * args.steps / launches
     achieved = step_bytes * args.steps / launches / avg_launch_s / 1e9
-    headline = args.workload != "batched"
+    headline = args.workload == "headline"
+    metric = {"headline": "1080p ME macroblocks/sec, 1 GPU: plain vs learned-proposal at <=1% BD-rate",
+              "batched": "batched 24-stream ME macroblocks/sec, 1 GPU, plain form",
+              "compact": "1080p ME macroblocks/sec, 1 GPU, Compact preset"}[args.workload]
+    workload = {
+        "headline": "plain GPU form: 1920x1080 analysis, High decisions, QP 24/30/36, keyint 30, 12 GOP streams x "
+                    "(1 IDR + 29 P frames) per step in one pipelined schedule, bit-exact with the CPU oracle; value "
+                    "counts P macroblocks",
+        "batched": "plain GPU form: 24 streams (8 x 640x360, 8 x 1280x720, 8 x 1920x1080), 60 frames each (2 GOPs, "
+                   "keyint 30), seeded QP 22-38, search range scaled by width, High decisions, all 48 GOPs (IDR + P "
+                   "frames) in one pipelined schedule; value counts P macroblocks",
+        "compact": "Compact preset (hexagon search, 8x8 partitions, I_4x4 in P; Constrained Baseline decisions): "
+                   "1920x1080, QP 24/30/36, keyint 30, 12 GOP streams x (1 IDR + 29 P frames) per step in one "
+                   "pipelined schedule, bit-exact with the CPU oracle; value counts P macroblocks"}[args.workload]
+    kernel = "rgh_k_compact_step" if args.workload == "compact" else "rgh_k_frame_step"
     out = {
-        "metric": "1080p ME macroblocks/sec, 1 GPU: plain vs learned-proposal at <=1% BD-rate" if headline else
-                  "batched 24-stream ME macroblocks/sec, 1 GPU, plain form",
+        "metric": metric,
         "value": round(value, 1),
         "unit": "macroblocks/s",
         "n_gpus": 1,
@@ -284,13 +300,7 @@ def run_workload(gops, args):
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": "plain GPU form: 1920x1080 analysis, High decisions, QP 24/30/36, keyint 30, "
-                               "12 GOP streams x (1 IDR + 29 P frames) per step in one pipelined schedule, "
-                               "bit-exact with the CPU oracle; value counts P macroblocks" if headline else
-                               "plain GPU form: 24 streams (8 x 640x360, 8 x 1280x720, 8 x 1920x1080), 60 frames "
-                               "each (2 GOPs, keyint 30), seeded QP 22-38, search range scaled by width, High "
-                               "decisions, all 48 GOPs (IDR + P frames) in one pipelined schedule; value counts "
-                               "P macroblocks",
+        "config": {"workload": workload,
                    "form": "plain", "gops": len(gops), "p_macroblocks_per_step": total_mbs,
                    "idr_macroblocks_per_step": idr_mbs,
                    "all_macroblocks_per_s": round((total_mbs + idr_mbs) * args.steps / el, 1),
@@ -300,7 +310,7 @@ def run_workload(gops, args):
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic() if headline else None,
-                     "kernel": "rgh_k_frame_step (+ rgh_k_intra_step in launches with IDR frames)",
+                     "kernel": kernel + " (+ rgh_k_intra_step in launches with IDR frames)",
                      "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "mbs_per_launch": round(mbs_per_launch, 2), "bytes_per_p_mb": MB_BYTES,
                      "bytes_per_idr_mb": IDR_MB_BYTES},
