@@ -412,6 +412,8 @@ rgh_status rgh_create(const rgh_config *cfg, void **ctx_out) {
     if (cfg->me_range < 0 || cfg->me_range > 1024 || cfg->qpel_iters < 0 || cfg->qpel_iters > 8)
         return RGH_ERR_INVALID_ARGUMENT;
     if (cfg->ring < 0 || cfg->ring == 1 || cfg->ring > 64) return RGH_ERR_INVALID_ARGUMENT;
+    // The Compact preset: with CABAC it codes B pictures (h264.rs:3075), not analysed here.
+    if ((cfg->flags & RGH_FLAG_COMPACT) && cfg->profile != RGH_PROFILE_BASELINE) return RGH_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RGH_ERR_DEVICE;
     if (cfg->device < 0 || cfg->device >= ndev) return RGH_ERR_INVALID_ARGUMENT;
@@ -427,12 +429,7 @@ rgh_status rgh_create(const rgh_config *cfg, void **ctx_out) {
     c->me_iters = cfg->me_range ? cfg->me_range : 16;
     c->qpel_iters = cfg->qpel_iters ? cfg->qpel_iters : 2;
     c->keep_pre = !(cfg->flags & RGH_FLAG_NO_UNDEBLOCKED_COPY);
-    // The Compact preset: with CABAC it codes B pictures (h264.rs:3075), not analysed here.
     c->compact = (cfg->flags & RGH_FLAG_COMPACT) != 0;
-    if (c->compact && cfg->profile != RGH_PROFILE_BASELINE) {
-        delete c;
-        return RGH_ERR_UNSUPPORTED;
-    }
     c->cqo = c->compact ? 1 : 0;
     const int W = c->mb_w * 16, H = c->mb_h * 16;
     c->sy = (W + 2 * RGH_PAD_Y + 31) & ~31;

@@ -50,3 +50,28 @@ def test_record_layouts_match_between_product_and_oracle():
     assert DECISION_DTYPE == abi.DECISION_DTYPE
     assert LEVELS_PER_MB == abi.LEVELS_PER_MB
     assert ctypes.sizeof(abi.RgConfig) == 36  # nine int32 fields of rgh_config
+    assert ctypes.sizeof(abi.RgSliceConfig) == 24  # six int32 fields of rgh_slice_config
+    assert ctypes.sizeof(abi.RgStreamConfig) == 28  # seven int32 fields of rgh_stream_config
+
+
+def test_compact_preset_is_baseline_only():
+    # With CABAC the reference's Compact preset codes B pictures: refused before any device use.
+    L = abi.load_library()
+    h = ctypes.c_void_p()
+    for prof in (1, 2):
+        cfg = abi.RgConfig(64, 48, prof, 26, 0, 0, 0, abi.FLAG_COMPACT, 0)
+        assert L.rgh_create(ctypes.byref(cfg), ctypes.byref(h)) == 2
+
+
+def test_writers_refuse_unknown_presets():
+    import numpy as np
+    L = abi.load_library()
+    dec = np.zeros(4, abi.DECISION_DTYPE)
+    lv = np.zeros((4, abi.LEVELS_PER_MB), np.int16)
+    out = np.zeros(4096, np.uint8)
+    n = ctypes.c_int64()
+    cfg = abi.RgSliceConfig(2, 2, 26, 1, 2, 7)
+    assert L.rgh_write_p_slice_cavlc(ctypes.byref(cfg), abi._ptr(dec), abi._ptr(lv), abi._ptr(out), 4096,
+                                     ctypes.byref(n)) == 1
+    scfg = abi.RgStreamConfig(32, 32, 0, 26, 30, 1, 9)
+    assert L.rgh_write_headers(ctypes.byref(scfg), abi._ptr(out), 4096, ctypes.byref(n)) == 1
