@@ -23,7 +23,12 @@ static_assert(CW16_S * CW16_N <= 704 && CW8_S * CW8_N <= 352, "window sizes");
 struct CompactLds {
     MbLds a;
     alignas(16) uint8_t c[CP_BYTES];
+    // Partition state, uniform across the wave: results of the 8x8 searches (slots 0-3) and the
+    // refined quadrant vectors (slots 4-7), 8 ints each (x, y, satd, mv cost, region base,
+    // stride, x, y); the 16x16 candidates (x at ST_C16X, y at ST_C16Y) and the 8x8 ones.
+    alignas(16) int st[128];
 };
+constexpr int ST_C16X = 64, ST_C16Y = 80, ST_C8X = 96, ST_C8Y = 104;
 
 // A staged region: byte offset in CompactLds::c, stride, picture coordinates of its first
 // sample.
@@ -267,10 +272,16 @@ __device__ MeR refine_qpel(const CompactLds &S, const CpMb &M, int bx, int by, i
 // then a quarter-sample diamond by SATD. cx / cy: nc candidate vectors. Regions: search region
 // at rbase (stride rst), predictor window at wbase (stride wst). Returns false when a reference
 // macroblock it reads is not final yet.
+// cxo / cyo: offsets of the nc candidate vectors in S.st. outer (when has_outer): an already staged
+// and checked region with picture box [ox0, ox1] x [oy0, oy1]; whatever lies inside it is read
+// from it instead of being staged again.
 template <int W, int RPER, int WPER>
-__device__ bool part_search(const RghJobD &J, CompactLds &S, const CpMb &M, int px, int py, int mvpx, int mvpy,
-                            const int *cx, const int *cy, int nc, int rbase, int rst, int wbase, int wst, uint32_t seq,
-                            MeR &out) {
+__device__ bool part_search(const RghJobD &J, CompactLds &S, const CpMb &M, int px, int py, int mvpx, int mvpy, int cxo,
+                            int cyo, int nc, int rbase, int rst, int wbase, int wst, bool has_outer, const Reg &outer,
+                            int ox0, int ox1, int oy0, int oy1, uint32_t seq, MeR &out, Reg &sreg, int &sx0, int &sy0) {
+    auto inside = [&](int x0_, int x1_, int y0_, int y1_) {
+        return has_outer && x0_ >= ox0 && x1_ <= ox1 && y0_ >= oy0 && y1_ <= oy1;
+    };
     const int lane = threadIdx.x, bx = M.x0 + px, by = M.y0 + py;
     const int lo0 = (M.bmin0 + 3) >> 2, lo1 = (M.bmin1 + 3) >> 2, hi0 = M.bmax0 >> 2, hi1 = M.bmax1 >> 2;
     int ptx[12], pty[12], n = 1;
@@ -280,7 +291,8 @@ __device__ bool part_search(const RghJobD &J, CompactLds &S, const CpMb &M, int 
 #pragma unroll
     for (int i = 0; i < 11; i++) {
         if (i < nc) {
-            const int m0 = iclamp((cx[i] + 2) >> 2, lo0, hi0), m1 = iclamp((cy[i] + 2) >> 2, lo1, hi1);
+            const int cvx = __builtin_amdgcn_readfirstlane(S.st[cxo + i]), cvy = __builtin_amdgcn_readfirstlane(S.st[cyo + i]);
+            const int m0 = iclamp((cvx + 2) >> 2, lo0, hi0), m1 = iclamp((cvy + 2) >> 2, lo1, hi1);
             bool dup = false;
 #pragma unroll
             for (int k = 0; k < 12; k++)
@@ -297,51 +309,83 @@ __device__ bool part_search(const RghJobD &J, CompactLds &S, const CpMb &M, int 
             }
         }
     }
-    // Round trip: the candidate blocks, and the window around the predictor.
+    // Round trip: the candidate blocks, and the window around the predictor (both read from the
+    // outer region instead when they lie inside it).
     const bool mvp_in = cp_inb(M, mvpx, mvpy);
-    constexpr int NB = W / 4;
-    const int blk = lane >> 2, j = lane & 3, bc = blk % NB, br = blk / NB;
-    const bool on = W == 16 || lane < 16;
-    uint32_t cw[12];
+    const int wx = bx + (mvpx >> 2) - 4, wy = by + (mvpy >> 2) - 4;
+    bool cand_in = true;
 #pragma unroll
     for (int k = 0; k < 12; k++)
-        cw[k] = (k < n && on) ? ld4u(J.ref_y + (long)(by + pty[k] + 4 * br + j) * J.sy + bx + ptx[k] + 4 * bc) : 0u;
-    int wait = 0;
-    // Candidate blocks span at most two macroblocks each way: their corners name them all.
-    if (lane < 4 * n) {
-        const int k = lane >> 2, kx = lane & 1, ky = (lane >> 1) & 1;
-        int qx = 0, qy = 0;
-#pragma unroll
-        for (int i = 0; i < 12; i++)
-            if (i == k) {
-                qx = ptx[i];
-                qy = pty[i];
-            }
-        const int X = bx + qx + kx * (W - 1), Y = by + qy + ky * (W - 1);
-        const int a = iclamp(X >> 4, 0, J.mb_w - 1), b = iclamp(Y >> 4, 0, J.mb_h - 1);
-        wait |= !stamp_final(J.ref_fin[(b + 1) * (J.mb_w + 1) + a + 1], J.ref_serial, seq);
-    }
-    Reg wreg{wbase, wst, 0, 0};
-    if (mvp_in) {
-        const int X = bx + (mvpx >> 2) - 4, Y = by + (mvpy >> 2) - 4;
-        wait |= box_waits(J, X, X + W + 8, Y, Y + W + 8, seq);
-        wreg = stage_ref<WPER>(J, S, wbase, wst, X, Y, W + 9, W + 9);
-    }
-    if (__ballot(wait)) return false;
-    // Candidates in order: the first strictly cheapest wins.
+        if (k < n) cand_in = cand_in && inside(bx + ptx[k], bx + ptx[k] + W - 1, by + pty[k], by + pty[k] + W - 1);
+    const bool win_in = !mvp_in || inside(wx, wx + W + 8, wy, wy + W + 8);
     int best0 = 0, best1 = 0;
     uint32_t best_cost = 0xFFFFFFFFu;
+    Reg wreg = outer;
+    if (cand_in && win_in) {
+        // Zero and the candidates, four at a time (padding repeats the zero vector, which cannot
+        // win again).
 #pragma unroll
-    for (int k = 0; k < 12; k++) {
-        if (k >= n) continue;
-        const uint32_t s = wave_sum(on ? __builtin_amdgcn_sad_u8(*(const uint32_t *)&S.a.src[(py + 4 * br + j) * 16 + px + 4 * bc],
-                                                                 cw[k], 0)
-                                       : 0u);
-        const uint32_t c = s + cp_mvc(M, mvpx, mvpy, 4 * ptx[k], 4 * pty[k]);
-        if (c < best_cost) {
-            best_cost = c;
-            best0 = ptx[k];
-            best1 = pty[k];
+        for (int c = 0; c < 12; c += 4) {
+            if (c >= n) continue;
+            int qx[4], qy[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                qx[k] = c + k < n ? ptx[c + k] : 0;
+                qy[k] = c + k < n ? pty[c + k] : 0;
+            }
+            uint32_t sads[4];
+            take4_sads<W>(S, outer, bx, by, px, py, qx, qy, sads);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint32_t cst = sads[k] + cp_mvc(M, mvpx, mvpy, 4 * qx[k], 4 * qy[k]);
+                if (cst < best_cost) {
+                    best_cost = cst;
+                    best0 = qx[k];
+                    best1 = qy[k];
+                }
+            }
+        }
+    } else {
+        constexpr int NB = W / 4;
+        const int blk = lane >> 2, j = lane & 3, bc = blk % NB, br = blk / NB;
+        const bool on = W == 16 || lane < 16;
+        uint32_t cw[12];
+#pragma unroll
+        for (int k = 0; k < 12; k++)
+            cw[k] = (k < n && on) ? ld4u(J.ref_y + (long)(by + pty[k] + 4 * br + j) * J.sy + bx + ptx[k] + 4 * bc) : 0u;
+        int wait = 0;
+        // Candidate blocks span at most two macroblocks each way: their corners name them all.
+        if (lane < 4 * n) {
+            const int k = lane >> 2, kx = lane & 1, ky = (lane >> 1) & 1;
+            int qx = 0, qy = 0;
+#pragma unroll
+            for (int i = 0; i < 12; i++)
+                if (i == k) {
+                    qx = ptx[i];
+                    qy = pty[i];
+                }
+            const int X = bx + qx + kx * (W - 1), Y = by + qy + ky * (W - 1);
+            const int a = iclamp(X >> 4, 0, J.mb_w - 1), b = iclamp(Y >> 4, 0, J.mb_h - 1);
+            wait |= !stamp_final(J.ref_fin[(b + 1) * (J.mb_w + 1) + a + 1], J.ref_serial, seq);
+        }
+        if (mvp_in) {
+            wait |= box_waits(J, wx, wx + W + 8, wy, wy + W + 8, seq);
+            wreg = stage_ref<WPER>(J, S, wbase, wst, wx, wy, W + 9, W + 9);
+        }
+        if (__ballot(wait)) return false;
+        // Candidates in order: the first strictly cheapest wins.
+#pragma unroll
+        for (int k = 0; k < 12; k++) {
+            if (k >= n) continue;
+            const uint32_t sd = wave_sum(on ? __builtin_amdgcn_sad_u8(*(const uint32_t *)&S.a.src[(py + 4 * br + j) * 16 + px + 4 * bc],
+                                                                      cw[k], 0)
+                                            : 0u);
+            const uint32_t c = sd + cp_mvc(M, mvpx, mvpy, 4 * ptx[k], 4 * pty[k]);
+            if (c < best_cost) {
+                best_cost = c;
+                best0 = ptx[k];
+                best1 = pty[k];
+            }
         }
     }
     // Round trip: the search region, 21 samples around the best candidate's block (the hexagon
@@ -353,9 +397,15 @@ __device__ bool part_search(const RghJobD &J, CompactLds &S, const CpMb &M, int 
         // region moved inward still covers them.
         const int X = iclamp(bx + best0 - 21, -(RGH_PAD_Y - 4), J.mb_w * 16 + RGH_PAD_Y - (W + 42));
         const int Y = iclamp(by + best1 - 21, -(RGH_PAD_Y - 4), J.mb_h * 16 + RGH_PAD_Y - (W + 42));
-        const int w2 = box_waits(J, X, X + W + 41, Y, Y + W + 41, seq);
-        rreg = stage_ref<RPER>(J, S, rbase, rst, X, Y, W + 42, W + 42);
-        if (__ballot(w2)) return false;
+        sx0 = X;
+        sy0 = Y;
+        if (inside(X, X + W + 41, Y, Y + W + 41)) {
+            rreg = outer;
+        } else {
+            const int w2 = box_waits(J, X, X + W + 41, Y, Y + W + 41, seq);
+            rreg = stage_ref<RPER>(J, S, rbase, rst, X, Y, W + 42, W + 42);
+            if (__ballot(w2)) return false;
+        }
     }
     // Takes of four points in order (compact.rs:516): returns the last that improved, or -1.
     auto take = [&](const int *qx, const int *qy) {
@@ -383,13 +433,15 @@ __device__ bool part_search(const RghJobD &J, CompactLds &S, const CpMb &M, int 
     };
     // Hexagon of radius 2 in circular order; after a step toward point d only the three points
     // around it are new (x264 hex2).
-    const int HX[6] = {-2, -1, 1, 2, 1, -1}, HY[6] = {0, 2, 2, 0, -2, -2};
+    // Point k of the hexagon: (-2,0) (-1,2) (1,2) (2,0) (1,-2) (-1,-2).
+    auto hx = [](int k) { return k == 0 ? -2 : (k == 3 ? 2 : (k == 1 || k == 5 ? -1 : 1)); };
+    auto hy = [](int k) { return k == 0 || k == 3 ? 0 : (k < 3 ? 2 : -2); };
     int dir;
     {
         const int fx = best0, fy = best1;
         int qx[8], qy[8];
 #pragma unroll
-        for (int k = 0; k < 6; k++) at(fx, fy, HX[k], HY[k], qx[k], qy[k]);
+        for (int k = 0; k < 6; k++) at(fx, fy, hx(k), hy(k), qx[k], qy[k]);
         qx[6] = qx[7] = fx;
         qy[6] = qy[7] = fy;
         const int a = take(qx, qy), b = take(qx + 4, qy + 4);
@@ -399,15 +451,16 @@ __device__ bool part_search(const RghJobD &J, CompactLds &S, const CpMb &M, int 
         const int fx = best0, fy = best1;
         // The three points around direction d, in compact.rs NEXT order: {0,1,5} {0,1,2} {1,2,3}
         // {2,3,4} {3,4,5} {0,4,5}.
-        const int ks[3] = {dir == 0 ? 0 : (dir == 5 ? 0 : dir - 1), dir == 0 ? 1 : (dir == 5 ? 4 : dir),
-                           dir == 0 ? 5 : (dir == 5 ? 5 : dir + 1)};
+        const int k0 = dir == 0 ? 0 : (dir == 5 ? 0 : dir - 1), k1 = dir == 0 ? 1 : (dir == 5 ? 4 : dir);
+        const int k2 = dir == 0 ? 5 : (dir == 5 ? 5 : dir + 1);
         int qx[4], qy[4];
-#pragma unroll
-        for (int k = 0; k < 3; k++) at(fx, fy, HX[ks[k]], HY[ks[k]], qx[k], qy[k]);
+        at(fx, fy, hx(k0), hy(k0), qx[0], qy[0]);
+        at(fx, fy, hx(k1), hy(k1), qx[1], qy[1]);
+        at(fx, fy, hx(k2), hy(k2), qx[2], qy[2]);
         qx[3] = fx;
         qy[3] = fy;
         const int t = take(qx, qy);
-        dir = t >= 0 ? ks[t] : -1;
+        dir = t == 0 ? k0 : (t == 1 ? k1 : (t == 2 ? k2 : -1));
     }
     {
         const int fx = best0, fy = best1;
@@ -451,16 +504,49 @@ __device__ bool part_search(const RghJobD &J, CompactLds &S, const CpMb &M, int 
     m.satd = part_cost<W>(S, m.reg, bx, by, px, py, m.x, m.y, true);
     m.mvc = cp_mvc(M, mvpx, mvpy, m.x, m.y);
     out = refine_qpel<W>(S, M, bx, by, px, py, mvpx, mvpy, m, 1);
+    sreg = rreg;
     return true;
 }
 
-// The prediction of the 16x16 macroblock into S.a.pred from per-quadrant vectors and regions
-// (quadrant k of the lane: 8x8 raster). Lane layout: row lane >> 2, columns 4 (lane & 3)..+3.
-__device__ uint32_t cp_predict(CompactLds &S, const CpMb &M, const Reg *regs, const int *qx, const int *qy) {
+// Partition state in LDS (written by every lane with the same values, read back as scalars).
+__device__ __forceinline__ void put_mer(CompactLds &S, int slot, const MeR &m) {
+    int *p = &S.st[8 * slot];
+    p[0] = m.x;
+    p[1] = m.y;
+    p[2] = (int)m.satd;
+    p[3] = (int)m.mvc;
+    p[4] = m.reg.base;
+    p[5] = m.reg.st;
+    p[6] = m.reg.ox;
+    p[7] = m.reg.oy;
+}
+__device__ __forceinline__ MeR get_mer(const CompactLds &S, int slot) {
+    const int *p = &S.st[8 * slot];
+    MeR m;
+    m.x = __builtin_amdgcn_readfirstlane(p[0]);
+    m.y = __builtin_amdgcn_readfirstlane(p[1]);
+    m.satd = (uint32_t)__builtin_amdgcn_readfirstlane(p[2]);
+    m.mvc = (uint32_t)__builtin_amdgcn_readfirstlane(p[3]);
+    m.reg.base = __builtin_amdgcn_readfirstlane(p[4]);
+    m.reg.st = __builtin_amdgcn_readfirstlane(p[5]);
+    m.reg.ox = __builtin_amdgcn_readfirstlane(p[6]);
+    m.reg.oy = __builtin_amdgcn_readfirstlane(p[7]);
+    return m;
+}
+
+// The lane's four luma prediction samples of the 16x16 macroblock (row lane >> 2, columns
+// 4 (lane & 3)..+3): from one vector and region, or (quads) from the refined quadrant vectors and
+// regions in LDS slots 4-7.
+__device__ uint32_t cp_predict(CompactLds &S, const CpMb &M, const Reg &R0, int mx0, int my0, bool quads) {
     const int lane = threadIdx.x, r = lane >> 2, c0 = (lane & 3) * 4, k = (c0 >> 3) + 2 * (r >> 3);
-    const Reg R = k == 0 ? regs[0] : (k == 1 ? regs[1] : (k == 2 ? regs[2] : regs[3]));
-    const int mx = k == 0 ? qx[0] : (k == 1 ? qx[1] : (k == 2 ? qx[2] : qx[3]));
-    const int my = k == 0 ? qy[0] : (k == 1 ? qy[1] : (k == 2 ? qy[2] : qy[3]));
+    Reg R = R0;
+    int mx = mx0, my = my0;
+    if (quads) {
+        const int *p = &S.st[8 * (4 + k)];
+        mx = p[0];
+        my = p[1];
+        R = Reg{p[4], p[5], p[6], p[7]};
+    }
     const int X = M.x0 + c0 + (mx >> 2), Y = M.y0 + r + (my >> 2), f = (mx & 3) | ((my & 3) << 2);
     uint32_t w = 0;
 #pragma unroll
@@ -656,8 +742,6 @@ __device__ bool analyse_compact_mb(const RghJobD &J, int mb_x, int mb_y, Compact
     memset(&D, 0, sizeof(D));
     D.qp = (uint8_t)qp;
     int mb_type = -1, mvrx = pskx, mvry = psky;
-    int qx[4], qy[4];
-    Reg qreg[4];
     uint32_t predw = 0, pu = 0, pv = 0;
     int cbpl = 0, cbpc = 0, i16_mode = 0, cmode = 0;
     uint64_t modes = 0;
@@ -668,9 +752,7 @@ __device__ bool analyse_compact_mb(const RghJobD &J, int mb_x, int mb_y, Compact
         const int w = box_waits(J, X, X + 24, Y, Y + 24, seq);
         const Reg R = stage_ref<3>(J, S, CP_W16, CW16_S, X, Y, 25, 25);
         if (__ballot(w)) return false;
-        const int rq[4] = {pskx, pskx, pskx, pskx}, cq[4] = {psky, psky, psky, psky};
-        const Reg rs[4] = {R, R, R, R};
-        predw = cp_predict(S, M, rs, rq, cq);
+        predw = cp_predict(S, M, R, pskx, psky, false);
         const ChromaTaps ct = chroma_taps(J, mb_x, mb_y, pskx, psky);
         chroma_mc_combine(ct, pskx, psky, &pu, &pv);
         if (cp_skip_check(J, S, srcw, predw, src_u, src_v, pu, pv, qp, qpc)) {
@@ -681,142 +763,126 @@ __device__ bool analyse_compact_mb(const RghJobD &J, int mb_x, int mb_y, Compact
 
     if (mb_type < 0) {
         // ---- 16x16: the predictor, the neighbours' and the previous picture's search vectors. ----
-        int cx[11], cy[11], nc = 0;
-        cx[nc] = mvpx;
-        cy[nc++] = mvpy;
-        if (has_l) {
-            mvr_of(0, cx[nc], cy[nc]);
+        int nc = 0;
+        auto push = [&](int x, int y) {
+            S.st[ST_C16X + nc] = x;
+            S.st[ST_C16Y + nc] = y;
             nc++;
+        };
+        int vx, vy;
+        push(mvpx, mvpy);
+        if (has_l) {
+            mvr_of(0, vx, vy);
+            push(vx, vy);
         }
         if (has_t) {
-            mvr_of(1, cx[nc], cy[nc]);
-            nc++;
+            mvr_of(1, vx, vy);
+            push(vx, vy);
             if (has_l) {
-                mvr_of(2, cx[nc], cy[nc]);
-                nc++;
+                mvr_of(2, vx, vy);
+                push(vx, vy);
             }
             if (mb_x + 1 < mb_w) {
-                mvr_of(3, cx[nc], cy[nc]);
-                nc++;
+                mvr_of(3, vx, vy);
+                push(vx, vy);
             }
         }
-        mvr_of(4, cx[nc], cy[nc]);
-        nc++;
+        mvr_of(4, vx, vy);
+        push(vx, vy);
         if (mb_x + 1 < mb_w) {
-            mvr_of(5, cx[nc], cy[nc]);
-            nc++;
+            mvr_of(5, vx, vy);
+            push(vx, vy);
         }
         if (mb_y + 1 < mb_h) {
-            mvr_of(6, cx[nc], cy[nc]);
-            nc++;
+            mvr_of(6, vx, vy);
+            push(vx, vy);
         }
         MeR m16;
-        if (!part_search<16, 15, 3>(J, S, M, 0, 0, mvpx, mvpy, cx, cy, nc, CP_R16, CR16_S, CP_W16, CW16_S, seq, m16))
+        Reg r16;
+        int r16x = 0, r16y = 0;
+        if (!part_search<16, 15, 3>(J, S, M, 0, 0, mvpx, mvpy, ST_C16X, ST_C16Y, nc, CP_R16, CR16_S, CP_W16, CW16_S, false,
+                                    Reg{0, 0, 0, 0}, 0, 0, 0, 0, seq, m16, r16, r16x, r16y))
             return false;
         mvrx = m16.x;
         mvry = m16.y;
         // A 16x16 vector at the skip vector whose residual decimates away is P_Skip.
         if (!probe && m16.x == pskx && m16.y == psky && cp_inb(M, pskx, psky)) {
-            const int rq[4] = {pskx, pskx, pskx, pskx}, cq[4] = {psky, psky, psky, psky};
-            const Reg rs[4] = {m16.reg, m16.reg, m16.reg, m16.reg};
-            predw = cp_predict(S, M, rs, rq, cq);
+            predw = cp_predict(S, M, m16.reg, pskx, psky, false);
             const ChromaTaps ct = chroma_taps(J, mb_x, mb_y, pskx, psky);
             chroma_mc_combine(ct, pskx, psky, &pu, &pv);
             if (cp_skip_check(J, S, srcw, predw, src_u, src_v, pu, pv, qp, qpc)) mb_type = 0;
         }
         if (mb_type < 0) {
             bool p8 = false;
-            MeR best[4] = {m16, m16, m16, m16};
             uint32_t best_cost = m16.satd + m16.mvc;
             // ---- Four 8x8 searches seeded by the 16x16 vector and the partitions before them. ----
+            // Predictor of partition i (8.4.1.3, enc/inter.rs:104): A left, B top, C top-right
+            // else D top-left; inside the macroblock from the partitions before it (LDS slots
+            // base .. base + 2).
+            auto part_mvp = [&](int i, int base, int &ox, int &oy) {
+                auto own = [&](int k) {
+                    const MeR t = get_mer(S, base + k);
+                    return Nb{0, t.x, t.y};
+                };
+                Nb A, B, C;
+                if (i == 0) {
+                    A = Lq1;
+                    B = Tq2;
+                    C = has_t ? Tq3 : Nb{-2, 0, 0};
+                } else if (i == 1) {
+                    A = own(0);
+                    B = Tq3;
+                    C = has_tr ? TRq2 : (has_t ? Tq2 : Nb{-2, 0, 0});
+                } else if (i == 2) {
+                    A = Lq3;
+                    B = own(0);
+                    C = own(1);
+                } else {
+                    A = own(2);
+                    B = own(1);
+                    C = own(0);
+                }
+                mvp_from(A, B, C, ox, oy);
+            };
             if (best_cost > lambda * 40u) {
-                MeR m8[4];
-                int ccx[5], ccy[5];
-                ccx[0] = m16.x;
-                ccy[0] = m16.y;
+                S.st[ST_C8X] = m16.x;
+                S.st[ST_C8Y] = m16.y;
+                uint32_t cost8 = 0;
 #pragma unroll 1
                 for (int i = 0; i < 4; i++) {
-                    // Predictor of partition i (8.4.1.3, enc/inter.rs:104): A left, B top, C
-                    // top-right else D top-left, inside the macroblock from the partitions before.
-                    Nb A, B, C;
-                    auto own = [&](int k) {
-                        return Nb{0, k == 0 ? m8[0].x : (k == 1 ? m8[1].x : m8[2].x), k == 0 ? m8[0].y : (k == 1 ? m8[1].y : m8[2].y)};
-                    };
-                    if (i == 0) {
-                        A = Lq1;
-                        B = Tq2;
-                        C = has_t ? Tq3 : Nb{-2, 0, 0};
-                    } else if (i == 1) {
-                        A = own(0);
-                        B = Tq3;
-                        C = has_tr ? TRq2 : (has_t ? Tq2 : Nb{-2, 0, 0});
-                    } else if (i == 2) {
-                        A = Lq3;
-                        B = own(0);
-                        C = own(1);
-                    } else {
-                        A = own(2);
-                        B = own(1);
-                        C = own(0);
-                    }
                     int p8x, p8y;
-                    mvp_from(A, B, C, p8x, p8y);
+                    part_mvp(i, 0, p8x, p8y);
                     const int px = (i & 1) * 8, py = (i >> 1) * 8;
-                    if (!part_search<8, 11, 2>(J, S, M, px, py, p8x, p8y, ccx, ccy, i + 1, CP_R8 + i * CR8_S * CR8_N, CR8_S,
-                                               CP_W8 + i * 352, CW8_S, seq, m8[i]))
+                    MeR m8;
+                    Reg dummy;
+                    int dx, dy;
+                    if (!part_search<8, 11, 2>(J, S, M, px, py, p8x, p8y, ST_C8X, ST_C8Y, i + 1, CP_R8 + i * CR8_S * CR8_N, CR8_S,
+                                               CP_W8 + i * 352, CW8_S, true, r16, r16x, r16x + 57, r16y, r16y + 57, seq, m8, dummy,
+                                               dx, dy))
                         return false;
-                    ccx[i + 1] = m8[i].x;
-                    ccy[i + 1] = m8[i].y;
+                    put_mer(S, i, m8);
+                    S.st[ST_C8X + i + 1] = m8.x;
+                    S.st[ST_C8Y + i + 1] = m8.y;
+                    cost8 += m8.satd + m8.mvc;
                 }
-                const uint32_t cost8 = m8[0].satd + m8[0].mvc + m8[1].satd + m8[1].mvc + m8[2].satd + m8[2].mvc +
-                                       m8[3].satd + m8[3].mvc;
-                if (cost8 < best_cost) {
-                    p8 = true;
-                    for (int i = 0; i < 4; i++) best[i] = m8[i];
-                }
+                p8 = cost8 < best_cost;
             }
-            // ---- Quarter-pel refinement of the partitions chosen (enc/inter.rs:655-678). ----
+            // ---- Quarter-pel refinement of the partitions chosen (enc/inter.rs:655-678), into
+            // slots 4-7. ----
             best_cost = 0;
             if (!p8) {
-                best[0] = refine_qpel<16>(S, M, x0, y0, 0, 0, mvpx, mvpy, best[0], 2);
-                best_cost = best[0].satd + best[0].mvc;
-                for (int k = 0; k < 4; k++) {
-                    qx[k] = best[0].x;
-                    qy[k] = best[0].y;
-                    qreg[k] = best[0].reg;
-                }
+                const MeR r = refine_qpel<16>(S, M, x0, y0, 0, 0, mvpx, mvpy, m16, 2);
+                best_cost = r.satd + r.mvc;
+                for (int k = 0; k < 4; k++) put_mer(S, 4 + k, r);
             } else {
 #pragma unroll 1
                 for (int i = 0; i < 4; i++) {
-                    Nb A, B, C;
-                    auto own = [&](int k) {
-                        return Nb{0, k == 0 ? qx[0] : (k == 1 ? qx[1] : qx[2]), k == 0 ? qy[0] : (k == 1 ? qy[1] : qy[2])};
-                    };
-                    if (i == 0) {
-                        A = Lq1;
-                        B = Tq2;
-                        C = has_t ? Tq3 : Nb{-2, 0, 0};
-                    } else if (i == 1) {
-                        A = own(0);
-                        B = Tq3;
-                        C = has_tr ? TRq2 : (has_t ? Tq2 : Nb{-2, 0, 0});
-                    } else if (i == 2) {
-                        A = Lq3;
-                        B = own(0);
-                        C = own(1);
-                    } else {
-                        A = own(2);
-                        B = own(1);
-                        C = own(0);
-                    }
                     int p8x, p8y;
-                    mvp_from(A, B, C, p8x, p8y);
+                    part_mvp(i, 4, p8x, p8y);
                     const MeR r = refine_qpel<8>(S, M, x0 + (i & 1) * 8, y0 + (i >> 1) * 8, (i & 1) * 8, (i >> 1) * 8, p8x,
-                                                 p8y, best[i], 2);
+                                                 p8y, get_mer(S, i), 2);
                     best_cost += r.satd + r.mvc;
-                    qx[i] = r.x;
-                    qy[i] = r.y;
-                    qreg[i] = r.reg;
+                    put_mer(S, 4 + i, r);
                 }
             }
             // ---- Intra 16x16 and 4x4 against the best inter cost (enc/inter.rs:681-700). ----
@@ -870,12 +936,11 @@ __device__ bool analyse_compact_mb(const RghJobD &J, int mb_x, int mb_y, Compact
             }
             if (mb_type < 0) {
                 // ---- Prediction, residual (intra-style rounding) and reconstruction. ----
-                predw = cp_predict(S, M, qreg, qx, qy);
+                predw = cp_predict(S, M, Reg{0, 0, 0, 0}, 0, 0, true);
                 *(uint32_t *)&S.a.pred[lane * 4] = predw;
                 {
                     const int k = ((lane & 7) >> 2) + 2 * ((lane >> 3) >> 2);
-                    const int mx = k == 0 ? qx[0] : (k == 1 ? qx[1] : (k == 2 ? qx[2] : qx[3]));
-                    const int my = k == 0 ? qy[0] : (k == 1 ? qy[1] : (k == 2 ? qy[2] : qy[3]));
+                    const int mx = S.st[8 * (4 + k)], my = S.st[8 * (4 + k) + 1];
                     const ChromaTaps ct = chroma_taps(J, mb_x, mb_y, mx, my);
                     chroma_mc_combine(ct, mx, my, &pu, &pv);
                 }
@@ -937,7 +1002,7 @@ __device__ bool analyse_compact_mb(const RghJobD &J, int mb_x, int mb_y, Compact
                     }
                 }
                 cbpc = chroma_cbp(S.a);
-                if (cbpl == 0 && cbpc == 0 && !p8 && qx[0] == pskx && qy[0] == psky) {
+                if (cbpl == 0 && cbpc == 0 && !p8 && S.st[32] == pskx && S.st[33] == psky) {
                     mb_type = 0;
                 } else {
                     mb_type = p8 ? RGH_MB_P_8X8 : 1;
@@ -956,18 +1021,19 @@ __device__ bool analyse_compact_mb(const RghJobD &J, int mb_x, int mb_y, Compact
     }
 
     // ---- Outputs: reconstruction, decision record and levels. ----
+    uint32_t qw[4];
     if (mb_type == 0) {
         // P_Skip: the prediction at the skip vector is the reconstruction.
         *(uint32_t *)&S.a.pred[lane * 4] = predw;
         S.a.predu[lane] = (uint8_t)pu;
         S.a.predv[lane] = (uint8_t)pv;
         cbpl = cbpc = 0;
-        for (int k = 0; k < 4; k++) {
-            qx[k] = pskx;
-            qy[k] = psky;
-        }
+        for (int k = 0; k < 4; k++) qw[k] = (uint32_t)(uint16_t)pskx | ((uint32_t)(uint16_t)psky << 16);
     } else if (mb_type == 2 || mb_type == 3) {
-        for (int k = 0; k < 4; k++) qx[k] = qy[k] = 0;
+        for (int k = 0; k < 4; k++) qw[k] = 0;
+    } else {
+        for (int k = 0; k < 4; k++)
+            qw[k] = (uint32_t)(uint16_t)S.st[8 * (4 + k)] | ((uint32_t)(uint16_t)S.st[8 * (4 + k) + 1] << 16);
     }
     __syncthreads();
     *(RGH_G uint32_t *)&J.cur_y[(long)(y0 + rr) * J.sy + x0 + c0] = *(const uint32_t *)&S.a.pred[lane * 4];
@@ -1013,9 +1079,7 @@ __device__ bool analyse_compact_mb(const RghJobD &J, int mb_x, int mb_y, Compact
         D.cbp_luma = (uint8_t)cbpl;
         D.cbp_chroma = (uint8_t)cbpc;
         D.nnz = (uint16_t)nnz;
-        uint32_t w[5];
-        for (int k = 0; k < 4; k++) w[k] = ((uint32_t)(uint16_t)qx[k]) | ((uint32_t)(uint16_t)qy[k] << 16);
-        w[4] = ((uint32_t)(uint16_t)mvrx) | ((uint32_t)(uint16_t)mvry << 16);
+        const uint32_t w[5] = {qw[0], qw[1], qw[2], qw[3], ((uint32_t)(uint16_t)mvrx) | ((uint32_t)(uint16_t)mvry << 16)};
         __builtin_memcpy((uint8_t *)&D + 8, w, 20);
         if (mb_type == 3) __builtin_memcpy(D.mv, &modes, sizeof(modes));
         store_dec(&J.dec[idx], D);
