@@ -16,7 +16,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libragdoll_hip.so")
-SOURCES = ["rgh_analyse.hip", "rgh_api.hip"]
+SOURCES = ["rgh_analyse.hip", "rgh_api.hip", "rgh_learned.hip"]
 HOST_SOURCES = ["rgh_cavlc.cpp"]
 ARCH = os.environ.get("RGH_OFFLOAD_ARCH", "gfx950")
 # Device instructions no kernel may contain. v_ashr_pk_{u8,i8}_i32: the backend assumes they

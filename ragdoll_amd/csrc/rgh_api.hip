@@ -383,6 +383,11 @@ rgh_status check_i420(const rgh_i420 *p, int w, int h) {
 
 }  // namespace
 
+// Records the last failure for rgh_last_error from the library's other translation units.
+void rgh_note_error(const char *what, const char *detail, const char *file, int line) {
+    std::snprintf(g_last_error, sizeof(g_last_error), "%s: %s (%s:%d)", what, detail, file, line);
+}
+
 extern "C" {
 
 const char *rgh_last_error(void) { return g_last_error; }

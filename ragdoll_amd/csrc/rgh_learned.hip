@@ -1,0 +1,814 @@
+// Learned motion proposals: feature extraction, bfloat16 training and FP8 inference of the
+// proposal network on MFMA (gfx950), behind the C ABI of include/ragdoll_learned.h.
+//
+// The network (192 -> 128 -> 128 -> 176, ReLU) is three GEMMs. Training keeps fp32 master
+// parameters and Adam state; every GEMM takes bfloat16 operands on v_mfma_f32_16x16x32_bf16
+// with fp32 accumulation, and activations are stored in bfloat16 between layers. Inference
+// keeps the whole quantized network (62 KB of E4M3 weights) in LDS, runs persistent
+// workgroups of four waves over 64-macroblock tiles on v_mfma_f32_16x16x32_fp8_fp8, and
+// reduces each macroblock's 169 displacement logits to its four best cells in the same kernel.
+//
+// No counterpart in the reference (its motion search, crates/ragdoll-codec/src/enc.rs:1182 and
+// :3081-3095, has no learned component).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "ragdoll_learned.h"
+
+void rgh_note_error(const char *what, const char *detail, const char *file, int line);
+
+namespace {
+
+constexpr int F_IN = RGH_PNET_FEATURES, HID = RGH_PNET_HIDDEN, OUT = RGH_PNET_OUTPUTS;
+constexpr int NCLS = RGH_PNET_CELLS * RGH_PNET_CELLS;  // 169 displacement classes
+constexpr int MODE0 = RGH_PNET_MODE_OFFSET;
+// Offsets into the flat parameter vector.
+constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + F_IN * HID, OFF_W2 = OFF_B1 + HID, OFF_B2 = OFF_W2 + HID * HID,
+              OFF_W3 = OFF_B2 + HID, OFF_B3 = OFF_W3 + HID * OUT, NPARAM = OFF_B3 + OUT;
+static_assert(NPARAM == RGH_PNET_PARAMS, "parameter layout");
+// Quantized weights: [out][in] per layer.
+constexpr int Q_W1 = 0, Q_W2 = Q_W1 + HID * F_IN, Q_W3 = Q_W2 + HID * HID, Q_BYTES = Q_W3 + OUT * HID;
+static_assert(Q_BYTES == RGH_PNET_W8_BYTES, "quantized layout");
+constexpr float FP8_MAX = 448.0f;
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even (no NaN reaches here)
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// E4M3 code of x, |x| <= 448, round to nearest even (v_cvt_pk_fp8_f32).
+__device__ __forceinline__ uint32_t fp8_code(float x) {
+    return (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false) & 0xFFu;
+}
+
+// ---------------------------------------------------------------------------------------
+// bfloat16 GEMM (training): 64x64 tile per workgroup of four waves, each wave 32x32 as 2x2
+// MFMA 16x16x32 blocks, k in steps of 32 through LDS. Operands are read with arbitrary element
+// strides (the backward products read transposed activations) and zero-filled past the edges.
+struct GemmArgs {
+    const uint16_t *A;
+    long sam, sak;
+    const uint16_t *B;
+    long sbk, sbn;
+    float *C;
+    uint16_t *Cb;
+    long ldc;
+    const float *bias;
+    const uint16_t *mask;
+    int M, N, K, relu, accumulate;
+};
+
+constexpr int GT = 64, GK = 32, GLD = GK + 8;  // LDS row: 32 operands + 8 pad (80 B, 16-B aligned)
+
+__global__ __launch_bounds__(256) void k_gemm_bf16(GemmArgs g) {
+    __shared__ __align__(16) uint16_t As[GT * GLD];  // [m][k]
+    __shared__ __align__(16) uint16_t Bs[GT * GLD];  // [n][k]
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int m0 = blockIdx.y * GT, n0 = blockIdx.x * GT;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < g.K; k0 += GK) {
+#pragma unroll
+        for (int e = 0; e < GT * GK / 256; e++) {
+            const int i = tid + 256 * e, r = i >> 5, c = i & 31, k = k0 + c;
+            const int m = m0 + r, n = n0 + r;
+            As[r * GLD + c] = (m < g.M && k < g.K) ? g.A[(long)m * g.sam + (long)k * g.sak] : (uint16_t)0;
+            Bs[r * GLD + c] = (n < g.N && k < g.K) ? g.B[(long)k * g.sbk + (long)n * g.sbn] : (uint16_t)0;
+        }
+        __syncthreads();
+        // Operand layout of v_mfma_f32_16x16x32_bf16: lane l holds row (A) / column (B)
+        // l % 16 at k = 8 * (l / 16) + [0, 8).
+        const int kq = 8 * (lane >> 4);
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            a[i] = __builtin_bit_cast(bf16x8, *(const uint4 *)&As[(wm + i * 16 + (lane & 15)) * GLD + kq]);
+            b[i] = __builtin_bit_cast(bf16x8, *(const uint4 *)&Bs[(wn + i * 16 + (lane & 15)) * GLD + kq]);
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        __syncthreads();
+    }
+    // Result layout: lane l holds column l % 16, rows 4 * (l / 16) + [0, 4).
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int m = m0 + wm + i * 16 + 4 * (lane >> 4) + q, n = n0 + wn + j * 16 + (lane & 15);
+                if (m >= g.M || n >= g.N) continue;
+                const long o = (long)m * g.ldc + n;
+                float v = acc[i][j][q];
+                if (g.accumulate) v += g.C[o];
+                if (g.bias) v += g.bias[n];
+                if (g.relu) v = fmaxf(v, 0.0f);
+                if (g.mask && g.mask[o] == 0) v = 0.0f;
+                if (g.C) g.C[o] = v;
+                if (g.Cb) g.Cb[o] = f2bf(v);
+            }
+}
+
+// Softmax cross-entropy of both heads; dZ = (softmax - onehot) / batch (fp32 and bfloat16).
+__global__ void k_xent(const float *Z, const rgh_pnet_label *lab, int B, float inv_b, float *dZ, uint16_t *dZb,
+                       float *loss) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= B) return;
+    const float *z = Z + (long)r * OUT;
+    float *gz = dZ + (long)r * OUT;
+    uint16_t *gb = dZb + (long)r * OUT;
+    const int d = lab[r].disp, md = lab[r].mode;
+    float ld = 0.0f, lm = 0.0f;
+    if (d >= 0 && d < NCLS) {
+        float mx = z[0];
+        for (int i = 1; i < NCLS; i++) mx = fmaxf(mx, z[i]);
+        float s = 0.0f;
+        for (int i = 0; i < NCLS; i++) s += expf(z[i] - mx);
+        const float inv = 1.0f / s;
+        ld = mx + logf(s) - z[d];
+        for (int i = 0; i < NCLS; i++) gz[i] = (expf(z[i] - mx) * inv - (i == d ? 1.0f : 0.0f)) * inv_b;
+    } else {
+        for (int i = 0; i < NCLS; i++) gz[i] = 0.0f;
+    }
+    {
+        const float *zm = z + MODE0;
+        const float mx = fmaxf(zm[0], fmaxf(zm[1], zm[2]));
+        const float e0 = expf(zm[0] - mx), e1 = expf(zm[1] - mx), e2 = expf(zm[2] - mx), s = e0 + e1 + e2;
+        lm = mx + logf(s) - zm[md];
+        const float e[3] = {e0, e1, e2};
+        for (int i = 0; i < 3; i++) gz[MODE0 + i] = (e[i] / s - (i == md ? 1.0f : 0.0f)) * inv_b;
+    }
+    for (int i = MODE0 + 3; i < OUT; i++) gz[i] = 0.0f;
+    for (int i = 0; i < OUT; i++) gb[i] = f2bf(gz[i]);
+    atomicAdd(&loss[0], ld);
+    atomicAdd(&loss[1], lm);
+}
+
+// Column sums in a fixed order: partial sums over 64-row chunks, then over the chunks.
+__global__ void k_colsum_part(const float *G, int M, int N, float *part) {
+    const int n = threadIdx.x, c = blockIdx.x;
+    if (n >= N) return;
+    float s = 0.0f;
+    const int r1 = min(M, (c + 1) * 64);
+    for (int r = c * 64; r < r1; r++) s += G[(long)r * N + n];
+    part[(long)c * N + n] = s;
+}
+__global__ void k_colsum_final(const float *part, int chunks, int N, float *out) {
+    const int n = threadIdx.x;
+    if (n >= N) return;
+    float s = 0.0f;
+    for (int c = 0; c < chunks; c++) s += part[(long)c * N + n];
+    out[n] = s;
+}
+
+__global__ void k_adam(float *p, const float *g, float *m, float *v, uint16_t *pb, int n, float lr, float c1,
+                       float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = 0.9f * m[i] + 0.1f * gi, vi = 0.999f * v[i] + 0.001f * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+    p[i] = pi;
+    pb[i] = f2bf(pi);
+}
+
+__global__ void k_to_bf16(const float *x, uint16_t *y, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = f2bf(x[i]);
+}
+
+// ---------------------------------------------------------------------------------------
+// Features: one wave per macroblock.
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__global__ __launch_bounds__(64) void k_features(const uint8_t *cur, int cs, const uint8_t *prev, int ps, int W,
+                                                 int H, int mb_w, uint16_t *F) {
+    __shared__ int rc[16 * 16];
+    __shared__ int sc[16];
+    __shared__ int red[2][64];
+    const int mb = blockIdx.x, lane = threadIdx.x;
+    const int x0 = (mb % mb_w) * 16, y0 = (mb / mb_w) * 16;
+    // The previous picture's 4x4 means over the 64x64 window at (x0 - 24, y0 - 24).
+    for (int c = lane; c < 256; c += 64) {
+        const int bx = x0 - 24 + 4 * (c & 15), by = y0 - 24 + 4 * (c >> 4);
+        int s = 0;
+        for (int j = 0; j < 4; j++) {
+            const uint8_t *row = prev + (long)clampi(by + j, 0, H - 1) * ps;
+            for (int i = 0; i < 4; i++) s += row[clampi(bx + i, 0, W - 1)];
+        }
+        rc[c] = (s + 8) >> 4;
+    }
+    // The macroblock: four samples per lane (row lane / 4, columns 4 * (lane % 4) + [0, 4)).
+    const int r = lane >> 2, c4 = (lane & 3) * 4;
+    const uint8_t *row = cur + (long)clampi(y0 + r, 0, H - 1) * cs;
+    int px[4], sum = 0;
+    for (int i = 0; i < 4; i++) {
+        px[i] = row[clampi(x0 + c4 + i, 0, W - 1)];
+        sum += px[i];
+    }
+    red[0][lane] = sum;
+    __syncthreads();
+    if (lane < 16) {
+        // Cell (cx, cy) = (lane % 4, lane / 4) covers rows 4cy..4cy+3 of lanes 4*row + cx.
+        const int cx = lane & 3, cy = lane >> 2;
+        int s = 0;
+        for (int j = 0; j < 4; j++) s += red[0][(4 * cy + j) * 4 + cx];
+        sc[lane] = (s + 8) >> 4;
+    }
+    int total = 0;
+    for (int i = 0; i < 64; i++) total += red[0][i];
+    const int mean = (total + 128) >> 8;
+    int dev = 0;
+    for (int i = 0; i < 4; i++) dev += abs(px[i] - mean);
+    red[1][lane] = dev;
+    __syncthreads();
+    uint16_t *f = F + (long)mb * F_IN;
+    for (int e = lane; e < NCLS; e += 64) {
+        const int dy = e / 13, dx = e % 13;  // cells displaced by (dx - 6, dy - 6)
+        int s = 0;
+        for (int j = 0; j < 4; j++)
+            for (int i = 0; i < 4; i++) s += abs(sc[j * 4 + i] - rc[(dy + j) * 16 + dx + i]);
+        f[e] = f2bf((float)min(s, 2047) * (1.0f / 256.0f));
+    }
+    if (lane < 16) f[NCLS + lane] = f2bf((float)sc[lane] * (1.0f / 64.0f));
+    if (lane == 16) {
+        int t = 0;
+        for (int i = 0; i < 64; i++) t += red[1][i];
+        f[NCLS + 16] = f2bf((float)((t + 128) >> 8) * (1.0f / 16.0f));
+    }
+    if (lane >= 17 && lane < 17 + (F_IN - NCLS - 17)) f[NCLS + lane] = 0;
+}
+
+__global__ void k_labels(const rgh_mb_decision *d, int n, rgh_pnet_label *l) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int t = d[i].mb_type;
+    rgh_pnet_label o;
+    if (t == 2 || t == 3 || t == 4) {
+        o.disp = -1;
+        o.mode = 2;
+    } else {
+        const int cx = clampi((d[i].mv[0] + 8) >> 4, -6, 6), cy = clampi((d[i].mv[1] + 8) >> 4, -6, 6);
+        o.disp = (int16_t)((cy + 6) * 13 + cx + 6);
+        o.mode = t == 0 ? 0 : 1;
+    }
+    l[i] = o;
+}
+
+__global__ void k_gather(const uint16_t *F, const rgh_pnet_label *L, const int *idx, int n, uint16_t *Fo,
+                         rgh_pnet_label *Lo) {
+    const int i = blockIdx.x, t = threadIdx.x;
+    if (i >= n) return;
+    const long s = idx[i];
+    for (int k = t; k < F_IN; k += blockDim.x) Fo[(long)i * F_IN + k] = F[s * F_IN + k];
+    if (t == 0) Lo[i] = L[s];
+}
+
+__global__ void k_fp8_encode(const float *x, int n, uint8_t *o) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = (uint8_t)fp8_code(x[i]);
+}
+
+// ---------------------------------------------------------------------------------------
+// FP8 inference. Workgroup: four waves, 16 macroblocks each per 64-macroblock tile.
+constexpr int IT = 64;
+struct InferArgs {
+    const uint16_t *X;
+    int M;
+    const uint8_t *w8;
+    const float *qs;    // [3][176] column scales, [3] input multipliers
+    const float *bias;  // flat parameter vector (biases read at their offsets)
+    rgh_proposal *out;
+    float *logits;
+};
+struct InferLds {  // 138 KB: one workgroup per CU
+    uint8_t w[Q_BYTES];
+    uint8_t x[4][16 * F_IN];
+    uint8_t h[4][2][16 * HID];
+    float z[4][16 * OUT];
+    float scale[3][OUT];
+    float bias[3][OUT];
+    float mul[3];
+};
+static_assert(Q_BYTES % 16 == 0 && (16 * F_IN) % 16 == 0 && (16 * HID) % 16 == 0, "LDS alignment");
+
+// One wave: rows [16] x columns [nblk * 16] of act(in[16][K] . w[N][K]^T) with the layer's
+// scale/bias/ReLU and coding into out8 (E4M3, [16][N]), or fp32 logits into outf.
+template <int K, int NBLK>
+__device__ __forceinline__ void fp8_layer(const uint8_t *in, const uint8_t *w, const float *scale, const float *bias,
+                                          float mul_next, uint8_t *out8, float *outf, int lane) {
+    const int kq = 8 * (lane >> 4), rl = lane & 15;
+#pragma unroll 1
+    for (int nb = 0; nb < NBLK; nb++) {
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int k0 = 0; k0 < K; k0 += 32) {
+            const long a = *(const long *)&in[rl * K + k0 + kq];
+            const long b = *(const long *)&w[(nb * 16 + rl) * K + k0 + kq];
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, acc, 0, 0, 0);
+        }
+        const int n = nb * 16 + rl;
+        const float sc = scale[n], bi = bias[n];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int m = 4 * (lane >> 4) + q;
+            const float v = acc[q] * sc + bi;
+            if (out8)
+                out8[m * (NBLK * 16) + n] = (uint8_t)fp8_code(fminf(fmaxf(v, 0.0f) * mul_next, FP8_MAX));
+            else
+                outf[m * OUT + n] = v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_infer(InferArgs g) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    InferLds &S = *(InferLds *)lds_raw;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int i = tid; i < Q_BYTES / 16; i += 256) ((uint4 *)S.w)[i] = ((const uint4 *)g.w8)[i];
+    for (int i = tid; i < 3 * OUT; i += 256) {
+        const int l = i / OUT, n = i % OUT;
+        S.scale[l][n] = g.qs[i];
+        const int boff = l == 0 ? OFF_B1 : (l == 1 ? OFF_B2 : OFF_B3);
+        S.bias[l][n] = n < (l == 2 ? OUT : HID) ? g.bias[boff + n] : 0.0f;
+    }
+    if (tid < 3) S.mul[tid] = g.qs[3 * OUT + tid];
+    __syncthreads();
+    const int ntiles = (g.M + IT - 1) / IT;
+    uint8_t *x8 = S.x[wave], *h1 = S.h[wave][0], *h2 = S.h[wave][1];
+    float *z = S.z[wave];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int r0 = t * IT + wave * 16;
+        // Inputs: 16 rows x 192 bfloat16 -> E4M3 (24 groups of 8 per row).
+        const float mul0 = S.mul[0];
+        for (int e = lane; e < 16 * (F_IN / 8); e += 64) {
+            const int rr = e / (F_IN / 8), c8 = (e % (F_IN / 8)) * 8, row = r0 + rr;
+            uint32_t lo = 0, hi = 0;
+            if (row < g.M) {
+                const uint4 v = *(const uint4 *)&g.X[(long)row * F_IN + c8];
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const float a = fminf(fmaxf(bf2f((uint16_t)(w[j] & 0xFFFF)) * mul0, -FP8_MAX), FP8_MAX);
+                    const float b = fminf(fmaxf(bf2f((uint16_t)(w[j] >> 16)) * mul0, -FP8_MAX), FP8_MAX);
+                    const uint32_t p = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false) & 0xFFFFu;
+                    if (j < 2)
+                        lo |= p << (16 * j);
+                    else
+                        hi |= p << (16 * (j - 2));
+                }
+            }
+            *(uint2 *)&x8[rr * F_IN + c8] = make_uint2(lo, hi);
+        }
+        __syncthreads();
+        fp8_layer<F_IN, HID / 16>(x8, S.w + Q_W1, S.scale[0], S.bias[0], S.mul[1], h1, nullptr, lane);
+        __syncthreads();
+        fp8_layer<HID, HID / 16>(h1, S.w + Q_W2, S.scale[1], S.bias[1], S.mul[2], h2, nullptr, lane);
+        __syncthreads();
+        fp8_layer<HID, OUT / 16>(h2, S.w + Q_W3, S.scale[2], S.bias[2], 0.0f, nullptr, z, lane);
+        __syncthreads();
+        if (g.logits)
+            for (int e = lane; e < 16 * OUT; e += 64) {
+                const int row = r0 + e / OUT;
+                if (row < g.M) g.logits[(long)row * OUT + e % OUT] = z[e];
+            }
+        if (lane < 16 && r0 + lane < g.M) {
+            const float *zz = z + lane * OUT;
+            // The four largest displacement logits, earliest index first among equals (an
+            // insertion keeps v0 >= v1 >= v2 >= v3; a later equal logit never moves up).
+            float v0 = -INFINITY, v1 = -INFINITY, v2 = -INFINITY, v3 = -INFINITY;
+            int i0 = 84, i1 = 84, i2 = 84, i3 = 84;  // the zero cell until four logits are seen
+            for (int i = 0; i < NCLS; i++) {
+                const float v = zz[i];
+                if (!(v > v3)) continue;
+                v3 = v;
+                i3 = i;
+                if (v3 > v2) {
+                    v3 = v2; i3 = i2; v2 = v; i2 = i;
+                    if (v2 > v1) {
+                        v2 = v1; i2 = i1; v1 = v; i1 = i;
+                        if (v1 > v0) {
+                            v1 = v0; i1 = i0; v0 = v; i0 = i;
+                        }
+                    }
+                }
+            }
+            rgh_proposal o;
+            o.cell[0][0] = (int8_t)(i0 % 13 - 6);
+            o.cell[0][1] = (int8_t)(i0 / 13 - 6);
+            o.cell[1][0] = (int8_t)(i1 % 13 - 6);
+            o.cell[1][1] = (int8_t)(i1 / 13 - 6);
+            o.cell[2][0] = (int8_t)(i2 % 13 - 6);
+            o.cell[2][1] = (int8_t)(i2 / 13 - 6);
+            o.cell[3][0] = (int8_t)(i3 % 13 - 6);
+            o.cell[3][1] = (int8_t)(i3 / 13 - 6);
+            const float m0 = zz[MODE0], m1 = zz[MODE0 + 1], m2 = zz[MODE0 + 2], mx = fmaxf(m0, fmaxf(m1, m2));
+            const float e0 = expf(m0 - mx), e1 = expf(m1 - mx), e2 = expf(m2 - mx), inv = 255.0f / (e0 + e1 + e2);
+            o.mode_prob[0] = (uint8_t)(e0 * inv + 0.5f);
+            o.mode_prob[1] = (uint8_t)(e1 * inv + 0.5f);
+            o.mode_prob[2] = (uint8_t)(e2 * inv + 0.5f);
+            for (int k = 0; k < 5; k++) o.reserved[k] = 0;
+            g.out[r0 + lane] = o;
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Host side.
+rgh_status hip_status(hipError_t e) {
+    if (e == hipSuccess) return RGH_OK;
+    if (e == hipErrorOutOfMemory) return RGH_ERR_OOM;
+    return RGH_ERR_DEVICE;
+}
+#define TRY(expr)                                                             \
+    do {                                                                      \
+        hipError_t e__ = (expr);                                              \
+        if (e__ != hipSuccess) {                                              \
+            rgh_note_error(#expr, hipGetErrorString(e__), "rgh_learned.hip", __LINE__); \
+            return hip_status(e__);                                           \
+        }                                                                     \
+    } while (0)
+#define FAIL(code, what)                                                      \
+    do {                                                                      \
+        rgh_note_error(__func__, what, "rgh_learned.hip", __LINE__);          \
+        return code;                                                          \
+    } while (0)
+
+// E4M3 (OCP, no infinities) code of x, round to nearest even; |x| >= 448 saturates to 448.
+uint8_t host_fp8(float x) {
+    const uint8_t sign = std::signbit(x) ? 0x80 : 0;
+    const float a = std::fabs(x);
+    if (a >= FP8_MAX) return sign | 0x7E;
+    if (a < 0.015625f) return sign | (uint8_t)std::nearbyint(a * 512.0f);  // subnormal (8 -> 2^-6, code 8)
+    int e = std::ilogb(a);
+    int q = (int)std::nearbyint((std::ldexp(a, -e) - 1.0f) * 8.0f);
+    if (q == 8) {
+        e++;
+        q = 0;
+    }
+    const int code = ((e + 7) << 3) | q;
+    return sign | (uint8_t)std::min(code, 0x7E);
+}
+
+struct Pnet {
+    int device = 0, cap = 0, step = 0, quantized = 0;
+    hipStream_t stream = nullptr;
+    float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr, *qs = nullptr, *loss = nullptr;
+    uint16_t *pb = nullptr;
+    uint8_t *w8 = nullptr;
+    uint16_t *h1b = nullptr, *h2b = nullptr, *dzb = nullptr, *dh2b = nullptr, *dh1b = nullptr;
+    float *zf = nullptr, *dzf = nullptr, *dh2f = nullptr, *dh1f = nullptr, *part = nullptr;
+    std::vector<void *> owned;
+};
+
+template <typename T>
+hipError_t dalloc(Pnet *p, T **out, size_t n) {
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+    if (e != hipSuccess) return e;
+    p->owned.push_back(q);
+    *out = (T *)q;
+    return hipMemsetAsync(q, 0, std::max<size_t>(n, 1) * sizeof(T), p->stream);
+}
+
+void free_net(Pnet *p) {
+    if (!p) return;
+    (void)hipSetDevice(p->device);
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    for (void *q : p->owned) (void)hipFree(q);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+hipError_t gemm(Pnet *p, const uint16_t *A, long sam, long sak, const uint16_t *B, long sbk, long sbn, float *C,
+                uint16_t *Cb, long ldc, const float *bias, const uint16_t *mask, int M, int N, int K, int relu,
+                int accumulate) {
+    GemmArgs g{A, sam, sak, B, sbk, sbn, C, Cb, ldc, bias, mask, M, N, K, relu, accumulate};
+    dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT);
+    hipLaunchKernelGGL(k_gemm_bf16, grid, dim3(256), 0, p->stream, g);
+    return hipGetLastError();
+}
+
+hipError_t colsum(Pnet *p, const float *G, int M, int N, float *out) {
+    const int chunks = (M + 63) / 64;
+    hipLaunchKernelGGL(k_colsum_part, dim3(chunks), dim3(256), 0, p->stream, G, M, N, p->part);
+    hipLaunchKernelGGL(k_colsum_final, dim3(1), dim3(256), 0, p->stream, (const float *)p->part, chunks, N, out);
+    return hipGetLastError();
+}
+
+// Forward pass over n <= cap rows: H1b, H2b (bfloat16) and logits Z (fp32).
+hipError_t forward(Pnet *p, const uint16_t *X, int n, float *Z) {
+    hipError_t e;
+    if ((e = gemm(p, X, F_IN, 1, p->pb + OFF_W1, HID, 1, nullptr, p->h1b, HID, p->params + OFF_B1, nullptr, n, HID,
+                  F_IN, 1, 0)) != hipSuccess)
+        return e;
+    if ((e = gemm(p, p->h1b, HID, 1, p->pb + OFF_W2, HID, 1, nullptr, p->h2b, HID, p->params + OFF_B2, nullptr, n,
+                  HID, HID, 1, 0)) != hipSuccess)
+        return e;
+    return gemm(p, p->h2b, HID, 1, p->pb + OFF_W3, OUT, 1, Z, nullptr, OUT, p->params + OFF_B3, nullptr, n, OUT, HID,
+                0, 0);
+}
+
+// Loss gradients of a batch into p->grads (and the summed losses into p->loss).
+hipError_t backward(Pnet *p, const uint16_t *X, const rgh_pnet_label *L, int B) {
+    hipError_t e;
+    if ((e = hipMemsetAsync(p->loss, 0, 2 * sizeof(float), p->stream)) != hipSuccess) return e;
+    if ((e = forward(p, X, B, p->zf)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_xent, dim3((B + 127) / 128), dim3(128), 0, p->stream, (const float *)p->zf, L, B,
+                       1.0f / (float)B, p->dzf, p->dzb, p->loss);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // Layer 3: dW3 = H2^T dZ, db3 = sum dZ, dH2 = (dZ W3^T) * [H2 > 0].
+    if ((e = gemm(p, p->h2b, 1, HID, p->dzb, OUT, 1, p->grads + OFF_W3, nullptr, OUT, nullptr, nullptr, HID, OUT, B, 0,
+                  0)) != hipSuccess)
+        return e;
+    if ((e = colsum(p, p->dzf, B, OUT, p->grads + OFF_B3)) != hipSuccess) return e;
+    if ((e = gemm(p, p->dzb, OUT, 1, p->pb + OFF_W3, 1, OUT, p->dh2f, p->dh2b, HID, nullptr, p->h2b, B, HID, OUT, 0,
+                  0)) != hipSuccess)
+        return e;
+    // Layer 2.
+    if ((e = gemm(p, p->h1b, 1, HID, p->dh2b, HID, 1, p->grads + OFF_W2, nullptr, HID, nullptr, nullptr, HID, HID, B, 0,
+                  0)) != hipSuccess)
+        return e;
+    if ((e = colsum(p, p->dh2f, B, HID, p->grads + OFF_B2)) != hipSuccess) return e;
+    if ((e = gemm(p, p->dh2b, HID, 1, p->pb + OFF_W2, 1, HID, p->dh1f, p->dh1b, HID, nullptr, p->h1b, B, HID, HID, 0,
+                  0)) != hipSuccess)
+        return e;
+    // Layer 1.
+    if ((e = gemm(p, X, 1, F_IN, p->dh1b, HID, 1, p->grads + OFF_W1, nullptr, HID, nullptr, nullptr, F_IN, HID, B, 0,
+                  0)) != hipSuccess)
+        return e;
+    return colsum(p, p->dh1f, B, HID, p->grads + OFF_B1);
+}
+
+}  // namespace
+
+extern "C" {
+
+rgh_status rgh_pnet_create(int32_t device, int32_t max_batch, void **net) {
+    if (!net || max_batch < 1 || max_batch > (1 << 20)) FAIL(RGH_ERR_INVALID_ARGUMENT, "bad net pointer or max_batch");
+    *net = nullptr;
+    int count = 0;
+    TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) FAIL(RGH_ERR_INVALID_ARGUMENT, "no such device");
+    TRY(hipSetDevice(device));
+    Pnet *p = new Pnet;
+    p->device = device;
+    p->cap = max_batch;
+    hipError_t e = hipStreamCreate(&p->stream);
+    const size_t B = (size_t)max_batch;
+    if (e == hipSuccess) e = dalloc(p, &p->params, NPARAM);
+    if (e == hipSuccess) e = dalloc(p, &p->grads, NPARAM);
+    if (e == hipSuccess) e = dalloc(p, &p->m, NPARAM);
+    if (e == hipSuccess) e = dalloc(p, &p->v, NPARAM);
+    if (e == hipSuccess) e = dalloc(p, &p->pb, NPARAM);
+    if (e == hipSuccess) e = dalloc(p, &p->w8, Q_BYTES);
+    if (e == hipSuccess) e = dalloc(p, &p->qs, RGH_PNET_SCALES);
+    if (e == hipSuccess) e = dalloc(p, &p->loss, 2);
+    if (e == hipSuccess) e = dalloc(p, &p->h1b, B * HID);
+    if (e == hipSuccess) e = dalloc(p, &p->h2b, B * HID);
+    if (e == hipSuccess) e = dalloc(p, &p->dh2b, B * HID);
+    if (e == hipSuccess) e = dalloc(p, &p->dh1b, B * HID);
+    if (e == hipSuccess) e = dalloc(p, &p->dzb, B * OUT);
+    if (e == hipSuccess) e = dalloc(p, &p->zf, B * OUT);
+    if (e == hipSuccess) e = dalloc(p, &p->dzf, B * OUT);
+    if (e == hipSuccess) e = dalloc(p, &p->dh2f, B * HID);
+    if (e == hipSuccess) e = dalloc(p, &p->dh1f, B * HID);
+    if (e == hipSuccess) e = dalloc(p, &p->part, ((B + 63) / 64) * OUT);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) {
+        rgh_note_error("rgh_pnet_create", hipGetErrorString(e), "rgh_learned.hip", __LINE__);
+        free_net(p);
+        return hip_status(e);
+    }
+    *net = p;
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_destroy(void *net) {
+    free_net((Pnet *)net);
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_set_params(void *net, const float *params, int64_t n) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !params || n != NPARAM) FAIL(RGH_ERR_INVALID_ARGUMENT, "expected RGH_PNET_PARAMS parameters");
+    TRY(hipSetDevice(p->device));
+    TRY(hipMemcpyAsync(p->params, params, NPARAM * sizeof(float), hipMemcpyHostToDevice, p->stream));
+    TRY(hipMemsetAsync(p->m, 0, NPARAM * sizeof(float), p->stream));
+    TRY(hipMemsetAsync(p->v, 0, NPARAM * sizeof(float), p->stream));
+    hipLaunchKernelGGL(k_to_bf16, dim3((NPARAM + 255) / 256), dim3(256), 0, p->stream, (const float *)p->params, p->pb,
+                       NPARAM);
+    TRY(hipGetLastError());
+    TRY(hipStreamSynchronize(p->stream));
+    p->step = 0;
+    p->quantized = 0;
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_get_params(void *net, float *params, int64_t n) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !params || n != NPARAM) FAIL(RGH_ERR_INVALID_ARGUMENT, "expected RGH_PNET_PARAMS parameters");
+    TRY(hipSetDevice(p->device));
+    TRY(hipMemcpyAsync(params, p->params, NPARAM * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    TRY(hipStreamSynchronize(p->stream));
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_features(void *net, const uint8_t *cur, int32_t cur_stride, const uint8_t *prev,
+                             int32_t prev_stride, int32_t width, int32_t height, uint16_t *features) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !cur || !prev || !features) FAIL(RGH_ERR_INVALID_ARGUMENT, "null pointer");
+    if (width < 1 || height < 1 || width > 16384 || height > 16384 || cur_stride < width || prev_stride < width)
+        FAIL(RGH_ERR_INVALID_DIMENSIONS, "bad picture size or stride");
+    TRY(hipSetDevice(p->device));
+    const int mb_w = (width + 15) / 16, mb_h = (height + 15) / 16;
+    hipLaunchKernelGGL(k_features, dim3(mb_w * mb_h), dim3(64), 0, p->stream, cur, (int)cur_stride, prev,
+                       (int)prev_stride, (int)width, (int)height, mb_w, features);
+    TRY(hipGetLastError());
+    TRY(hipStreamSynchronize(p->stream));
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_labels(void *net, const rgh_mb_decision *dec, int32_t n, rgh_pnet_label *labels) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !dec || !labels || n < 0) FAIL(RGH_ERR_INVALID_ARGUMENT, "null pointer or negative count");
+    if (n == 0) return RGH_OK;
+    TRY(hipSetDevice(p->device));
+    hipLaunchKernelGGL(k_labels, dim3((n + 255) / 256), dim3(256), 0, p->stream, dec, (int)n, labels);
+    TRY(hipGetLastError());
+    TRY(hipStreamSynchronize(p->stream));
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_gather(void *net, const uint16_t *F, const rgh_pnet_label *L, const int32_t *idx, int32_t n,
+                           uint16_t *Fo, rgh_pnet_label *Lo) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !F || !L || !idx || !Fo || !Lo || n < 0) FAIL(RGH_ERR_INVALID_ARGUMENT, "null pointer or negative count");
+    if (n == 0) return RGH_OK;
+    TRY(hipSetDevice(p->device));
+    hipLaunchKernelGGL(k_gather, dim3(n), dim3(64), 0, p->stream, F, L, (const int *)idx, (int)n, Fo, Lo);
+    TRY(hipGetLastError());
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_train_step(void *net, const uint16_t *X, const rgh_pnet_label *L, int32_t batch, float lr,
+                               float *loss) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !X || !L || batch < 1 || batch > p->cap || !(lr > 0.0f)) FAIL(RGH_ERR_INVALID_ARGUMENT, "bad batch or rate");
+    TRY(hipSetDevice(p->device));
+    TRY(backward(p, X, L, batch));
+    p->step++;
+    const float c1 = 1.0f - std::pow(0.9f, (float)p->step), c2 = 1.0f - std::pow(0.999f, (float)p->step);
+    hipLaunchKernelGGL(k_adam, dim3((NPARAM + 255) / 256), dim3(256), 0, p->stream, p->params, (const float *)p->grads,
+                       p->m, p->v, p->pb, NPARAM, lr, c1, c2);
+    TRY(hipGetLastError());
+    p->quantized = 0;
+    if (loss) {
+        TRY(hipMemcpyAsync(loss, p->loss, 2 * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+        TRY(hipStreamSynchronize(p->stream));
+    }
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_gradients(void *net, const uint16_t *X, const rgh_pnet_label *L, int32_t batch, float *grads,
+                              float *logits) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !X || !L || batch < 1 || batch > p->cap) FAIL(RGH_ERR_INVALID_ARGUMENT, "bad batch");
+    TRY(hipSetDevice(p->device));
+    TRY(backward(p, X, L, batch));
+    if (grads) TRY(hipMemcpyAsync(grads, p->grads, NPARAM * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    if (logits)
+        TRY(hipMemcpyAsync(logits, p->zf, (size_t)batch * OUT * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    TRY(hipStreamSynchronize(p->stream));
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_forward(void *net, const uint16_t *X, int32_t n, float *logits) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !X || !logits || n < 0) FAIL(RGH_ERR_INVALID_ARGUMENT, "null pointer or negative count");
+    TRY(hipSetDevice(p->device));
+    for (int r = 0; r < n; r += p->cap) {
+        const int k = std::min(p->cap, n - r);
+        TRY(forward(p, X + (size_t)r * F_IN, k, logits + (size_t)r * OUT));
+    }
+    TRY(hipStreamSynchronize(p->stream));
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_quantize(void *net, const uint16_t *calib, int32_t n) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !calib || n < 1) FAIL(RGH_ERR_INVALID_ARGUMENT, "calibration rows required");
+    TRY(hipSetDevice(p->device));
+    const int k = std::min<int>(n, p->cap);
+    TRY(forward(p, calib, k, p->zf));
+    std::vector<float> P(NPARAM);
+    std::vector<uint16_t> h1((size_t)k * HID), h2((size_t)k * HID);
+    TRY(hipMemcpyAsync(P.data(), p->params, NPARAM * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    TRY(hipMemcpyAsync(h1.data(), p->h1b, h1.size() * 2, hipMemcpyDeviceToHost, p->stream));
+    TRY(hipMemcpyAsync(h2.data(), p->h2b, h2.size() * 2, hipMemcpyDeviceToHost, p->stream));
+    TRY(hipStreamSynchronize(p->stream));
+    auto bf = [](uint16_t h) {
+        uint32_t u = (uint32_t)h << 16;
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f;
+    };
+    float amax[3] = {0.0f, 0.0f, 0.0f};
+    for (uint16_t h : h1) amax[1] = std::max(amax[1], bf(h));
+    for (uint16_t h : h2) amax[2] = std::max(amax[2], bf(h));
+    // Layer inputs: features as they are (all below 8); hidden activations scaled so that the
+    // calibration maximum maps to 448.
+    float mul[3] = {1.0f, amax[1] > 0.0f ? FP8_MAX / amax[1] : 1.0f, amax[2] > 0.0f ? FP8_MAX / amax[2] : 1.0f};
+    std::vector<uint8_t> w8(Q_BYTES);
+    std::vector<float> qs(RGH_PNET_SCALES, 0.0f);
+    const int offs[3] = {OFF_W1, OFF_W2, OFF_W3}, qoff[3] = {Q_W1, Q_W2, Q_W3}, kin[3] = {F_IN, HID, HID},
+              nout[3] = {HID, HID, OUT};
+    for (int l = 0; l < 3; l++) {
+        for (int c = 0; c < nout[l]; c++) {
+            float wmax = 0.0f;
+            for (int r = 0; r < kin[l]; r++) wmax = std::max(wmax, std::fabs(P[offs[l] + r * nout[l] + c]));
+            const float ws = wmax > 0.0f ? wmax / FP8_MAX : 1.0f, inv = 1.0f / ws;
+            for (int r = 0; r < kin[l]; r++) w8[qoff[l] + c * kin[l] + r] = host_fp8(P[offs[l] + r * nout[l] + c] * inv);
+            qs[l * OUT + c] = ws / mul[l];
+        }
+        qs[3 * OUT + l] = mul[l];
+    }
+    TRY(hipMemcpyAsync(p->w8, w8.data(), Q_BYTES, hipMemcpyHostToDevice, p->stream));
+    TRY(hipMemcpyAsync(p->qs, qs.data(), qs.size() * sizeof(float), hipMemcpyHostToDevice, p->stream));
+    TRY(hipStreamSynchronize(p->stream));
+    p->quantized = 1;
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_infer(void *net, const uint16_t *X, int32_t n, rgh_proposal *out, float *logits) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !X || !out || n < 0) FAIL(RGH_ERR_INVALID_ARGUMENT, "null pointer or negative count");
+    if (!p->quantized) FAIL(RGH_ERR_INVALID_ARGUMENT, "rgh_pnet_quantize has not run since the parameters changed");
+    if (n == 0) return RGH_OK;
+    TRY(hipSetDevice(p->device));
+    int cus = 0;
+    TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
+    const int tiles = (n + IT - 1) / IT, grid = std::max(1, std::min(tiles, 2 * cus));
+    InferArgs g{X, (int)n, p->w8, p->qs, p->params, out, logits};
+    TRY(hipFuncSetAttribute((const void *)k_infer, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(InferLds)));
+    hipLaunchKernelGGL(k_infer, dim3(grid), dim3(256), sizeof(InferLds), p->stream, g);
+    TRY(hipGetLastError());
+    TRY(hipStreamSynchronize(p->stream));
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_get_quantized(void *net, uint8_t *w8, float *scales) {
+    Pnet *p = (Pnet *)net;
+    if (!p) FAIL(RGH_ERR_INVALID_ARGUMENT, "null net");
+    if (!p->quantized) FAIL(RGH_ERR_INVALID_ARGUMENT, "rgh_pnet_quantize has not run since the parameters changed");
+    TRY(hipSetDevice(p->device));
+    if (w8) TRY(hipMemcpyAsync(w8, p->w8, Q_BYTES, hipMemcpyDeviceToHost, p->stream));
+    if (scales) TRY(hipMemcpyAsync(scales, p->qs, RGH_PNET_SCALES * sizeof(float), hipMemcpyDeviceToHost, p->stream));
+    TRY(hipStreamSynchronize(p->stream));
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_fp8_encode(void *net, const float *x, int32_t n, uint8_t *out) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !x || !out || n < 0) FAIL(RGH_ERR_INVALID_ARGUMENT, "null pointer or negative count");
+    if (n == 0) return RGH_OK;
+    TRY(hipSetDevice(p->device));
+    hipLaunchKernelGGL(k_fp8_encode, dim3((n + 255) / 256), dim3(256), 0, p->stream, x, (int)n, out);
+    TRY(hipGetLastError());
+    TRY(hipStreamSynchronize(p->stream));
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_gemm_bf16(void *net, const uint16_t *A, int64_t sam, int64_t sak, const uint16_t *B, int64_t sbk,
+                              int64_t sbn, float *C, uint16_t *Cb, int64_t ldc, const float *bias, const uint16_t *mask,
+                              int32_t M, int32_t N, int32_t K, int32_t relu, int32_t accumulate) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !A || !B || (!C && !Cb) || M < 0 || N < 0 || K < 0 || ldc < N || (accumulate && !C))
+        FAIL(RGH_ERR_INVALID_ARGUMENT, "bad GEMM arguments");
+    if (M == 0 || N == 0) return RGH_OK;
+    TRY(hipSetDevice(p->device));
+    TRY(gemm(p, A, sam, sak, B, sbk, sbn, C, Cb, ldc, bias, mask, M, N, K, relu, accumulate));
+    TRY(hipStreamSynchronize(p->stream));
+    return RGH_OK;
+}
+
+}  // extern "C"
