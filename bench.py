@@ -167,6 +167,12 @@ def main():
                     help="headline: the 1080p metric (default); batched: the 24-stream uneven-size set; "
                          "compact: the headline streams with the Compact preset (Baseline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal form")
+    ap.add_argument("--learned-iters", type=int, default=4)
+    ap.add_argument("--skip-gate", type=int, default=0)
+    ap.add_argument("--intra-gate", type=int, default=0)
+    ap.add_argument("--fallback-pct", type=int, default=105)
+    ap.add_argument("--bd-frames", type=int, default=30, help="frames per QP of the BD-rate sample (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration)")
@@ -185,9 +191,44 @@ def main():
     else:
         gops, label = headline_gops(), "headline"
     out = run_workload(gops, args)
+    if label == "headline" and not args.no_learned and args.bd_frames:
+        out["learned"]["bd_rate"] = bd_rate_sample(args)
     if label == "headline" and not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(render_frames(120), args.cpu_seconds)
     print(json.dumps(out), flush=True)
+
+
+def learned_config(args):
+    return dict(iters=args.learned_iters, skip_gate=args.skip_gate, intra_gate=args.intra_gate,
+                fallback_pct=args.fallback_pct)
+
+
+def bd_rate_sample(args):
+    """BD-rate of the learned form against the plain one: the first --bd-frames frames of the
+    headline content coded at QP 22/27/32/37 by both (Constrained Baseline CAVLC bits, luma PSNR)."""
+    from ragdoll_amd.learned import Proposer, load_trained
+    from ragdoll_amd.rd import bd_rate, code_gop
+    frames = render_frames(120)[:args.bd_frames]
+    net = load_trained()
+    proposer = Proposer(net, ((W + 15) // 16) * ((H + 15) // 16))
+    t0 = time.time()
+    curves = {"plain": [], "learned": []}
+    for qp in (22, 27, 32, 37):
+        curves["plain"].append(code_gop(frames, W, H, qp))
+        curves["learned"].append(code_gop(frames, W, H, qp, proposer=proposer, learned=learned_config(args)))
+    proposer.close()
+    net.close()
+    rate = {k: [c["bytes"] for c in v] for k, v in curves.items()}
+    q = {k: [c["psnr_y"] for c in v] for k, v in curves.items()}
+    out = {"bd_rate_pct": round(bd_rate(rate["plain"], q["plain"], rate["learned"], q["learned"]), 3),
+           "sample": "first %d frames of the 1080p content (1 IDR + %d P), QP 22/27/32/37, Constrained Baseline "
+                     "CAVLC bits vs luma PSNR" % (len(frames), len(frames) - 1),
+           "qps": [22, 27, 32, 37], "plain_bytes": rate["plain"], "learned_bytes": rate["learned"],
+           "plain_psnr_y": [round(v, 4) for v in q["plain"]], "learned_psnr_y": [round(v, 4) for v in q["learned"]],
+           "learned_plain_search": [c["plain_search"] for c in curves["learned"]],
+           "p_macroblocks_per_qp": curves["learned"][0]["p_macroblocks"], "seconds": round(time.time() - t0, 1)}
+    print("bd-rate sample: %s" % json.dumps(out), file=sys.stderr, flush=True)
+    return out
 
 
 def headline_gops():
@@ -317,11 +358,77 @@ def run_workload(gops, args):
     }
     if headline:
         out["valu"] = valu_figure({"rgh_k_frame_step": total_mbs, "rgh_k_intra_step": idr_mbs}, dev_ms / 1e3 / args.steps)
+        if not args.no_learned:
+            out["learned"] = run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, value, hip)
     for a in ans:
         a.close()
     for p in [q for q, _ in uploaded.values()] + [q for d in decs for q in d] + [q for l in lvls for q in l]:
         hip.free(p)
     return out
+
+
+def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value, hip):
+    """The learned-proposal form on the same streams and schedule: each step computes the
+    proposals of every P frame (features from the frame and the previous source frame, FP8
+    inference) and analyses the GOPs from them (rgh_analyse_frames_learned); both are timed."""
+    from ragdoll_amd import DECISION_DTYPE, analyse_frames
+    from ragdoll_amd.learned import Proposer, load_trained
+    net = load_trained()
+    pics = []
+    for i, gp in enumerate(gops):
+        for k in range(1, len(gp["frames"])):
+            pics.append((srcs[i][k][0], srcs[i][k][3], srcs[i][k - 1][0], srcs[i][k - 1][3], gp["w"], gp["h"], gp["qp"]))
+    proposer = Proposer(net, total_mbs)
+    cfg = learned_config(args)
+
+    def step():
+        t = time.perf_counter()
+        props = iter(proposer.run(pics))
+        t1 = time.perf_counter()
+        P = [[None] + [next(props) for _ in gp["frames"][1:]] for gp in gops]
+        analyse_frames(ans, srcs, decs, lvls, lag=sched[0], groups=sched[1], first_idr=True, proposals=P,
+                       learned=cfg)
+        return time.perf_counter() - t, t1 - t
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    el = prop = 0.0
+    for _ in range(args.steps):
+        e, p = step()
+        el += e
+        prop += p
+    # FP8 inference alone (one launch over every P macroblock of the step), for its roofline.
+    t = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        net.infer(proposer.feat, total_mbs, proposer.props)
+    infer_s = (time.perf_counter() - t) / reps
+    flops = total_mbs * 2 * (192 * 128 + 128 * 128 + 128 * 176)
+    # Decision statistics of the last step's P frames.
+    counts = np.zeros(4, np.int64)
+    types = np.zeros(8, np.int64)
+    for i, gp in enumerate(gops):
+        n = ans[i].mb_w * ans[i].mb_h
+        for k in range(1, len(gp["frames"])):
+            d = hip.download(decs[i][k], np.zeros(n, DECISION_DTYPE))
+            types += np.bincount(d["mb_type"], minlength=8)[:8]
+            for b in range(4):
+                counts[b] += int(np.count_nonzero(d["flags"] & (1 << b)))
+    proposer.close()
+    net.close()
+    value = total_mbs * args.steps / el
+    return {"value": round(value, 1), "unit": "macroblocks/s", "vs_plain": round(value / plain_value, 4),
+            "ms_per_step": round(el / args.steps * 1e3, 3),
+            "proposal_ms_per_step": round(prop / args.steps * 1e3, 3),
+            "config": cfg, "lag": sched[0], "groups": sched[1],
+            "plain_search_pct": round(100.0 * counts[1] / total_mbs, 3),
+            "skip_probe_gated_pct": round(100.0 * counts[2] / total_mbs, 3),
+            "intra_gated_pct": round(100.0 * counts[3] / total_mbs, 3),
+            "mb_types": types.tolist(),
+            "mfma_roofline": {"bound": "mfma", "kernel": "k_infer (FP8 proposal network)",
+                              "achieved": round(flops / infer_s / 1e12, 3), "peak": 5000.0, "unit": "TFLOP/s",
+                              "frac": flops / infer_s / 1e12 / 5000.0, "flops_per_mb": 2 * (192 * 128 + 128 * 128 + 128 * 176),
+                              "launch_ms": round(infer_s * 1e3, 3)}}
 
 
 if __name__ == "__main__":

@@ -35,7 +35,8 @@ extern "C" {
  *               window centred on the macroblock (samples clamped to the picture).
  *   [169, 185)  the macroblock's 4x4-mean cells / 64, raster order.
  *   [185]       mean absolute deviation of the macroblock's luma from its mean / 16.
- *   [186, 192)  zero. */
+ *   [186]       the picture's QP / 32.
+ *   [187, 192)  zero. */
 #define RGH_PNET_FEATURES 192
 #define RGH_PNET_HIDDEN 128
 #define RGH_PNET_OUTPUTS 176 /* 169 displacement logits, 3 mode logits, 4 unused */
@@ -72,7 +73,18 @@ rgh_status rgh_pnet_get_params(void *net, float *params, int64_t n);
  * (device luma planes). */
 rgh_status rgh_pnet_features(void *net, const uint8_t *cur_y_device, int32_t cur_stride,
                              const uint8_t *prev_y_device, int32_t prev_stride, int32_t width,
-                             int32_t height, uint16_t *features_device);
+                             int32_t height, int32_t qp, uint16_t *features_device);
+
+/* One picture of rgh_pnet_features_batch. */
+typedef struct rgh_pnet_picture {
+    const uint8_t *cur_y;   /* device luma plane of the picture */
+    const uint8_t *prev_y;  /* device luma plane of the previous picture */
+    uint16_t *features;     /* device rows, ceil(width/16) * ceil(height/16) of them */
+    int32_t cur_stride, prev_stride, width, height, qp, reserved;
+} rgh_pnet_picture;
+
+/* rgh_pnet_features for n pictures in one launch (pics: host array). */
+rgh_status rgh_pnet_features_batch(void *net, const rgh_pnet_picture *pics, int32_t n);
 
 /* Labels from analysed macroblocks (rgh_mb_decision records of ragdoll_hip.h): the nearest
  * cell to the chosen vector (clamped to the grid), and the mode. */
@@ -102,6 +114,10 @@ rgh_status rgh_pnet_forward(void *net, const uint16_t *features_device, int32_t 
  * scales from the largest activation over the calibration features (n rows). */
 rgh_status rgh_pnet_quantize(void *net, const uint16_t *calib_features_device, int32_t n);
 
+/* rgh_pnet_quantize with the two hidden layers' activation maxima given (as a previous
+ * calibration found them: rgh_pnet_get_quantized's multipliers m give max = 448 / m). */
+rgh_status rgh_pnet_quantize_ranges(void *net, float h1_max, float h2_max);
+
 /* FP8 inference of n macroblocks: proposals_device [n]; logits_device [n][176] fp32 may be NULL. */
 rgh_status rgh_pnet_infer(void *net, const uint16_t *features_device, int32_t n, rgh_proposal *proposals_device,
                           float *logits_device);
@@ -118,6 +134,30 @@ rgh_status rgh_pnet_get_quantized(void *net, uint8_t *w8, float *scales);
 /* The device's float -> E4M3 conversion (round to nearest even), as inference codes
  * activations, exported for its tests: |x| <= 448 (device pointers). */
 rgh_status rgh_pnet_fp8_encode(void *net, const float *x_device, int32_t n, uint8_t *out_device);
+
+/* Search effort of rgh_analyse_frames_learned (zero fields select the defaults). */
+typedef struct rgh_learned_config {
+    int32_t iters;        /* diamond iterations after the proposals (default 4) */
+    int32_t skip_gate;    /* the skip probe runs when mode_prob[0] >= skip_gate (default 0: always) */
+    int32_t intra_gate;   /* the intra 16x16 check runs when mode_prob[2] >= intra_gate (default 0: always) */
+    int32_t fallback_pct; /* the plain search runs when the cheapest proposal costs more than
+                             fallback_pct percent of the predictor's cost (default 105) */
+} rgh_learned_config;
+
+/* rgh_analyse_frames (ragdoll_hip.h) with learned proposals: proposals[c * n_frames + k] holds
+ * mb_w * mb_h records for frame k of context c (device memory; NULL only for an IDR frame 0
+ * under RGH_FRAMES_FIRST_IDR). Every P macroblock's integer search starts from the predictor
+ * and the four proposed cells; when the cheapest proposal costs more than fallback_pct percent
+ * of the predictor's cost, the reference's candidates join and the diamond runs its full count
+ * (the plain search), otherwise at most `iters` iterations; the skip probe and the intra check
+ * run as gated above. SAD, SATD and rate-distortion arithmetic, sub-pel refinement, residual
+ * coding, reconstruction and deblocking are the plain path's. Decision flags: bit 1 the plain
+ * search ran, bit 2 the skip probe was gated off, bit 3 the intra check was gated off.
+ * Balanced contexts only (RGH_ERR_UNSUPPORTED for RGH_FLAG_COMPACT). */
+rgh_status rgh_analyse_frames_learned(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src_device,
+                                      const rgh_proposal *const *proposals, const rgh_learned_config *config,
+                                      rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag,
+                                      int32_t groups, int32_t flags);
 
 /* The bfloat16 GEMM behind training, exported for its tests: C[m][n] = sum_k A(m,k) * B(k,n)
  * with A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn] (bfloat16 bit patterns), then

@@ -86,6 +86,11 @@ def load_library(path=LIB_PATH):
     L.rgh_analyse_idr.argtypes = [P, ctypes.POINTER(RgI420), P, P, ctypes.POINTER(RgI420Out)]
     L.rgh_analyse_batch.restype = I32
     L.rgh_analyse_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), I32, ctypes.POINTER(RgI420)]
+    L.rgh_analyse_frames_learned.restype = I32
+    L.rgh_analyse_frames_learned.argtypes = [ctypes.POINTER(ctypes.c_void_p), I32, I32, ctypes.POINTER(RgI420),
+                                             ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
+                                             ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), I32, I32,
+                                             I32]
     L.rgh_analyse_frames.restype = I32
     L.rgh_analyse_frames.argtypes = [ctypes.POINTER(ctypes.c_void_p), I32, I32, ctypes.POINTER(RgI420),
                                      ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), I32, I32, I32]
@@ -142,7 +147,7 @@ class StreamAnalyzer:
         h = ctypes.c_void_p()
         _check(self.L.rgh_create(ctypes.byref(cfg), ctypes.byref(h)), "rgh_create")
         self.h = h
-        self.width, self.height = width, height
+        self.width, self.height, self.qp = width, height, qp
         mw, mh = ctypes.c_int32(), ctypes.c_int32()
         _check(self.L.rgh_get_mb_dims(self.h, ctypes.byref(mw), ctypes.byref(mh)), "rgh_get_mb_dims")
         self.mb_w, self.mb_h = mw.value, mh.value
@@ -227,12 +232,20 @@ def analyse_batch(analyzers, device_frames):
 FRAMES_FIRST_IDR = 1
 
 
-def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0, first_idr=False):
+class RgLearnedConfig(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int32), ("skip_gate", ctypes.c_int32), ("intra_gate", ctypes.c_int32),
+                ("fallback_pct", ctypes.c_int32)]
+
+
+def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0, first_idr=False, proposals=None,
+                   learned=None):
     """n_frames consecutive P frames of every analyzer in one pipelined schedule.
 
     device_frames[i][k]: (y_ptr, u_ptr, v_ptr, stride_y, stride_u, stride_v) of frame k of
     analyzer i in device memory; decisions[i][k] / levels[i][k]: device addresses receiving its
-    mb_w*mb_h decision records / levels."""
+    mb_w*mb_h decision records / levels. With proposals[i][k] (device addresses of rgh_proposal
+    records; None for an IDR frame 0) the search runs from learned proposals
+    (rgh_analyse_frames_learned) with learned = dict(iters, skip_gate, intra_gate, fallback_pct)."""
     L = load_library()
     n = len(analyzers)
     nf = len(device_frames[0])
@@ -247,8 +260,17 @@ def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0,
             srcs[i * nf + k] = RgI420(f[0], f[1], f[2], f[3], f[4], f[5], a.width, a.height)
             dp[i * nf + k] = decisions[i][k]
             lp[i * nf + k] = levels[i][k]
-    _check(L.rgh_analyse_frames(arr, n, nf, srcs, dp, lp, lag, groups, FRAMES_FIRST_IDR if first_idr else 0),
-           "rgh_analyse_frames")
+    flags = FRAMES_FIRST_IDR if first_idr else 0
+    if proposals is None:
+        _check(L.rgh_analyse_frames(arr, n, nf, srcs, dp, lp, lag, groups, flags), "rgh_analyse_frames")
+        return
+    pp = (ctypes.c_void_p * (n * nf))()
+    for i in range(n):
+        for k in range(nf):
+            pp[i * nf + k] = proposals[i][k]
+    cfg = RgLearnedConfig(**(learned or {}))
+    _check(L.rgh_analyse_frames_learned(arr, n, nf, srcs, pp, ctypes.byref(cfg), dp, lp, lag, groups, flags),
+           "rgh_analyse_frames_learned")
 
 
 def write_p_slice_cavlc(decisions, levels, mb_w, mb_h, qp, frame_num, poc_lsb, speed="balanced"):

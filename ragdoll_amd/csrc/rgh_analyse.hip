@@ -981,6 +981,15 @@ __device__ I16Code code_i16(MbLds &S, int mb_x, int mb_y, int qp, int qpc, const
 //
 // Returns false, having written nothing to global memory, when a reference sample the
 // macroblock may read is not final yet for a reader in launch `seq`.
+// LEARNED: the search starts from the macroblock's learned proposal (rgh_proposal: four cells
+// of 4 samples and mode probabilities) instead of the reference's candidate list. Candidates
+// [mvp, four proposals]; when the cheapest proposal costs more than learn_fallback percent of
+// the predictor's cost, the reference's candidates (0, left, top, co-located) join and the
+// diamond runs its full iteration count (the plain search), otherwise at most learn_iters
+// iterations. The skip probe runs only when the skip probability reaches learn_skip_gate / 255,
+// the intra 16x16 check only when the intra probability reaches learn_intra_gate / 255. Costs,
+// sub-pel refinement, residual coding and reconstruction are the plain path's.
+template <bool LEARNED>
 __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint32_t seq) {
     const int lane = threadIdx.x;
     const int rr = lane >> 2, c0 = (lane & 3) * 4;
@@ -1010,6 +1019,14 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
         const int csy = min(cy0 + (lane >> 3), ch - 1), csx = min(cx0 + (lane & 7), cw - 1);
         src_u = J.src_u[(long)csy * J.src_su + csx];
         src_v = J.src_v[(long)csy * J.src_sv + csx];
+    }
+    // The learned proposal (16 B, uniform): words 0-1 the four cells, word 2 the probabilities.
+    uint32_t pw0 = 0, pw1 = 0, pw2 = 0;
+    if constexpr (LEARNED) {
+        const RGH_G uint32_t *pp = (const RGH_G uint32_t *)&J.prop[idx];
+        pw0 = pp[0];
+        pw1 = pp[1];
+        pw2 = pp[2];
     }
     // Neighbour motion (left, top, top-right else top-left decisions of this frame, and the
     // co-located decision of the reference frame) and intra edges (enc.rs:1457 i16_edges, :1476:
@@ -1120,13 +1137,41 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
 
     // Integer-search candidates (enc.rs:1185 full_pel, :3081-3095): [mvp, 0, left, top,
     // co-located], each rounded to full-pel, clamped and deduplicated. Fixed slots keep the
-    // reference's list order; absent neighbours are not live.
+    // reference's list order; absent neighbours are not live. Learned: [mvp, proposals 0-3],
+    // then the reference's list (its fallback) in slots 5-8.
     const int imn0 = (bmin0 + 3) >> 2, imn1 = (bmin1 + 3) >> 2, imx0 = bmax0 >> 2, imx1 = bmax1 >> 2;
-    const int cvx[5] = {mvpx, 0, ax, bx_, colx}, cvy[5] = {mvpy, 0, ay, by_, coly};
-    const int present[5] = {1, 1, mb_x > 0, mb_y > 0, 1};
-    int fxs[5], fys[5], live[5];
+    constexpr int NC = LEARNED ? 9 : 5;
+    auto pcell = [&](int k, int comp) { return (int)(int8_t)(((k < 2 ? pw0 : pw1) >> (16 * (k & 1) + 8 * comp)) & 0xFF) * 16; };
+    int cvx[NC], cvy[NC], present[NC];
+    if constexpr (LEARNED) {
+        cvx[0] = mvpx;
+        cvy[0] = mvpy;
+        present[0] = 1;
 #pragma unroll
-    for (int k = 0; k < 5; k++) {
+        for (int k = 0; k < 4; k++) {
+            cvx[1 + k] = pcell(k, 0);
+            cvy[1 + k] = pcell(k, 1);
+            present[1 + k] = 1;
+        }
+        const int ex[4] = {0, ax, bx_, colx}, ey[4] = {0, ay, by_, coly}, ep[4] = {1, mb_x > 0, mb_y > 0, 1};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            cvx[5 + k] = ex[k];
+            cvy[5 + k] = ey[k];
+            present[5 + k] = ep[k];
+        }
+    } else {
+        const int ex[5] = {mvpx, 0, ax, bx_, colx}, ey[5] = {mvpy, 0, ay, by_, coly}, ep[5] = {1, 1, mb_x > 0, mb_y > 0, 1};
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            cvx[k] = ex[k];
+            cvy[k] = ey[k];
+            present[k] = ep[k];
+        }
+    }
+    int fxs[NC], fys[NC], live[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
         fxs[k] = iclamp((cvx[k] + 2) >> 2, imn0, imx0);
         fys[k] = iclamp((cvy[k] + 2) >> 2, imn1, imx1);
         live[k] = present[k];
@@ -1147,7 +1192,7 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
     if (J.ref_fin) {
         int lx = fxs[0], hx = fxs[0], ly = fys[0], hy = fys[0];
 #pragma unroll
-        for (int k = 1; k < 5; k++) {
+        for (int k = 1; k < NC; k++) {
             lx = min(lx, fxs[k]);
             hx = max(hx, fxs[k]);
             ly = min(ly, fys[k]);
@@ -1162,13 +1207,14 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
             wait |= !stamp_final(J.ref_fin[(b + 1) * (mb_w + 1) + a + 1], J.ref_serial, seq);
         }
     }
-    uint32_t rw[5];
+    uint32_t rw[NC];
     {
         const RGH_G uint8_t *rb = J.ref_y + (long)(y0 + rr) * J.sy + x0 + c0;
 #pragma unroll
-        for (int k = 0; k < 5; k++) rw[k] = live[k] ? ld4u(rb + (long)fys[k] * J.sy + fxs[k]) : 0;
+        for (int k = 0; k < NC; k++) rw[k] = live[k] ? ld4u(rb + (long)fys[k] * J.sy + fxs[k]) : 0;
     }
-    const bool probe = in_bounds(pskx, psky);
+    const int p_skip = (int)(pw2 & 0xFF), p_intra = (int)((pw2 >> 16) & 0xFF);
+    const bool probe = in_bounds(pskx, psky) && (!LEARNED || p_skip >= J.learn_skip_gate);
     const int int_y = ((pskx | psky) & 3) == 0;
     uint32_t predw = 0, pu = 0, pv = 0;
     ChromaTaps ct{};
@@ -1237,7 +1283,7 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
                 }
                 if (chroma_zero) {
                     mb_type = 0;
-                    D.flags = 1;
+                    D.flags |= 1;
                     mvx = pskx;
                     mvy = psky;
                 }
@@ -1245,14 +1291,17 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
         }
     }
 
+    if constexpr (LEARNED) {
+        if (in_bounds(pskx, psky) && !probe) D.flags |= 4;
+    }
     if (mb_type < 0) {
         // ---- Integer candidates: the first strictly cheapest wins. ----
         const RGH_G uint8_t *rb = J.ref_y + (long)(y0 + rr) * J.sy + x0 + c0;
         int bestx = 0, besty = 0;
         uint32_t best_cost = 0xFFFFFFFFu;
-#pragma unroll
-        for (int k = 0; k < 5; k++) {
-            if (!live[k]) continue;
+        int iters = J.me_iters;
+        auto take = [&](int k) {
+            if (!live[k]) return;
             const uint32_t cst = wave_sum(__builtin_amdgcn_sad_u8(srcw, rw[k], 0)) + mv_cost(fxs[k] * 4, fys[k] * 4);
             D.n_sad++;
             if (cst < best_cost) {
@@ -1260,6 +1309,30 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
                 bestx = fxs[k];
                 besty = fys[k];
             }
+        };
+        if constexpr (LEARNED) {
+            // The predictor (slot 0) is always live; a proposal equal to it is not evaluated twice.
+            take(0);
+            const uint32_t c_mvp = best_cost;
+            best_cost = 0xFFFFFFFFu;
+#pragma unroll
+            for (int k = 1; k < 5; k++) take(k);
+            const uint32_t c_prop = best_cost == 0xFFFFFFFFu ? c_mvp : best_cost;  // all equal to the predictor
+            if (c_mvp <= best_cost) {
+                best_cost = c_mvp;
+                bestx = fxs[0];
+                besty = fys[0];
+            }
+            if ((uint64_t)c_prop * 100u > (uint64_t)c_mvp * (uint32_t)J.learn_fallback) {
+                D.flags |= 2;
+#pragma unroll
+                for (int k = 5; k < NC; k++) take(k);
+            } else {
+                iters = min(iters, J.learn_iters);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 5; k++) take(k);
         }
 
         // ---- Round trip 4: the reference region around the best candidate, in LDS. ----
@@ -1280,7 +1353,7 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
         };
         // Small diamond (enc.rs:1207-1272): the first of the cheapest in-bounds points below the
         // best moves, until none does or the iterations run out.
-        for (int it = 0; it < J.me_iters; it++) {
+        for (int it = 0; it < iters; it++) {
             const int px[4] = {bestx, bestx - 1, bestx + 1, bestx};
             const int py[4] = {besty - 1, besty, besty, besty + 1};
             int ok[4];
@@ -1366,7 +1439,14 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
         const uint32_t inter_satd = satd16(S.src, S.pred);
         D.n_satd++;
         const uint32_t inter_cost = inter_satd + lambda * (se_bits(mvx - mvpx) + se_bits(mvy - mvpy) + 1);
-        if (inter_satd > 24 * lambda) {
+        bool try_i16 = inter_satd > 24 * lambda;
+        if constexpr (LEARNED) {
+            if (try_i16 && p_intra < J.learn_intra_gate) {
+                try_i16 = false;
+                D.flags |= 8;
+            }
+        }
+        if (try_i16) {
             const I16Analysis ia = analyse_i16(S, mb_x, mb_y, D.n_satd);
             if (ia.cost + lambda * 8 < inter_cost) {
                 // enc.rs:3587 code_i16 in a P slice.
@@ -1989,7 +2069,8 @@ struct StepLdsOf<2> {
 //
 // IDR jobs run the same schedule in their own kernel (rgh_k_intra_step), and Compact-preset P
 // pictures in theirs (rgh_k_compact_step), so that each analysis has its own register and LDS
-// budget. MODE: 0 Balanced P, 1 IDR, 2 Compact P.
+// budget; Balanced P pictures searched from learned proposals run rgh_k_learned_step. MODE: 0
+// Balanced P, 1 IDR, 2 Compact P, 3 Balanced P with learned proposals.
 template <int MODE>
 __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int j0, int launch, uint32_t seq) {
     const RghJobD J = device_view(jobs[j0 + (int)blockIdx.y]);
@@ -2017,7 +2098,7 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
             else if constexpr (MODE == 2)
                 ok = analyse_compact_mb(J, x, y, S.a, seq);
             else
-                ok = analyse_mb(J, x, y, S.a, seq);
+                ok = analyse_mb<MODE == 3>(J, x, y, S.a, seq);
         }
         if (lane == 0) {
             if (ok) {
@@ -2054,6 +2135,11 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_intra_step(const RghJob *
 extern "C" __global__ __launch_bounds__(64) void rgh_k_compact_step(const RghJob *__restrict__ jobs, int j0, int launch,
                                                                      uint32_t seq) {
     frame_step<2>(jobs, j0, launch, seq);
+}
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_learned_step(const RghJob *__restrict__ jobs, int j0, int launch,
+                                                                     uint32_t seq) {
+    frame_step<3>(jobs, j0, launch, seq);
 }
 
 // Fills n 64-bit words (stamps) with v.

@@ -15,6 +15,7 @@
 extern "C" __global__ void rgh_k_frame_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_intra_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_compact_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
+extern "C" __global__ void rgh_k_learned_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_extend(const RghJob *jobs);
 extern "C" __global__ void rgh_k_fill_u64(uint64_t *p, int n, uint64_t v);
 
@@ -214,7 +215,7 @@ rgh_status ensure_jobs(Ctx *c, int n);
 struct Schedule {
     Ctx *owner = nullptr;
     int n = 0, n_frames = 0, lag = 1, nj = 0, n_intra = 0;
-    bool compact = false;
+    bool compact = false, learned = false;
     std::vector<RghJob> jobs;
     std::vector<int> span, last;
     int gx = 1, max_span = 0, planned = 0, L = 0, extra_total = 0;
@@ -229,6 +230,12 @@ struct Schedule {
         for (int j = 0; j < nj; j++)
             if ((jobs[(size_t)j].compact != 0) != compact) {
                 set_error("run_schedule", "contexts of one call must use one preset", __LINE__);
+                return RGH_ERR_INVALID_ARGUMENT;
+            }
+        learned = n_intra < nj && jobs[(size_t)n_intra].prop != nullptr;
+        for (int j = n_intra; j < nj; j++)
+            if ((jobs[(size_t)j].prop != nullptr) != learned) {
+                set_error("run_schedule", "every P frame of a learned call needs proposals", __LINE__);
                 return RGH_ERR_INVALID_ARGUMENT;
             }
         for (int j = n_intra; j < nj; j++)
@@ -261,7 +268,8 @@ struct Schedule {
             hipLaunchKernelGGL(rgh_k_intra_step, dim3((unsigned)gx, (unsigned)(split - j_lo)), dim3(64), 0, owner->stream,
                                owner->jobs_dev, j_lo, L, seq);
         if (j_hi > split)
-            hipLaunchKernelGGL(compact ? rgh_k_compact_step : rgh_k_frame_step, dim3((unsigned)gx, (unsigned)(j_hi - split)),
+            hipLaunchKernelGGL(learned ? rgh_k_learned_step : (compact ? rgh_k_compact_step : rgh_k_frame_step),
+                               dim3((unsigned)gx, (unsigned)(j_hi - split)),
                                dim3(64), 0, owner->stream, owner->jobs_dev, split, L, seq);
         for (int j = j_lo; j < j_hi; j++) last[(size_t)j] = L;
         L++;
@@ -600,13 +608,35 @@ rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src) 
     return analyse_single(cs.data(), n, src);
 }
 
-rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src,
-                              rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag, int32_t groups,
-                              int32_t flags) {
+static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src,
+                               const rgh_proposal *const *props, const rgh_learned_config *lcfg,
+                               rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag, int32_t groups,
+                               int32_t flags) {
     if (!ctxs || n <= 0 || n_frames <= 0 || !src || !decisions || !levels || lag < 0 || groups < 0 ||
         (flags & ~RGH_FRAMES_FIRST_IDR))
         return RGH_ERR_INVALID_ARGUMENT;
     const bool first_idr = (flags & RGH_FRAMES_FIRST_IDR) != 0;
+    rgh_learned_config lc{4, 0, 0, 105};
+    if (props) {
+        if (lcfg) {
+            if (lcfg->iters < 0 || lcfg->iters > 64 || lcfg->skip_gate < 0 || lcfg->skip_gate > 256 ||
+                lcfg->intra_gate < 0 || lcfg->intra_gate > 256 || lcfg->fallback_pct < 0 || lcfg->fallback_pct > 100000)
+                return RGH_ERR_INVALID_ARGUMENT;
+            if (lcfg->iters) lc.iters = lcfg->iters;
+            lc.skip_gate = lcfg->skip_gate;
+            lc.intra_gate = lcfg->intra_gate;
+            if (lcfg->fallback_pct) lc.fallback_pct = lcfg->fallback_pct;
+        }
+        for (int i = 0; i < n; i++) {
+            const Ctx *c = static_cast<Ctx *>(ctxs[i]);
+            if (c && c->compact) {
+                set_error("rgh_analyse_frames_learned", "learned proposals serve the Balanced preset", __LINE__);
+                return RGH_ERR_UNSUPPORTED;
+            }
+            for (int k = first_idr ? 1 : 0; k < n_frames; k++)
+                if (!props[(size_t)i * n_frames + k]) return RGH_ERR_INVALID_ARGUMENT;
+        }
+    }
     if ((int64_t)n * n_frames > (1 << 20)) return RGH_ERR_INVALID_ARGUMENT;
     std::vector<Ctx *> cs((size_t)n);
     for (int i = 0; i < n; i++) {
@@ -652,8 +682,16 @@ rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, co
                 const int wk = k + 1 - R;
                 const RghFrameCtl *wait = wk >= 0 ? ctl + (size_t)wk * m + t : nullptr;
                 const rgh_mb_decision *prev = k ? decisions[q - 1] : c->prev;
-                S.jobs[(size_t)k * m + t] = make_job(c, k, src[q], decisions[q], prev, levels[q], k * lag,
-                                                     ctl + (size_t)k * m + t, wait, first_idr && k == 0);
+                RghJob &j = S.jobs[(size_t)k * m + t];
+                j = make_job(c, k, src[q], decisions[q], prev, levels[q], k * lag, ctl + (size_t)k * m + t, wait,
+                             first_idr && k == 0);
+                if (props && !j.intra) {
+                    j.prop = props[q];
+                    j.learn_iters = std::min(lc.iters, c->me_iters);
+                    j.learn_skip_gate = lc.skip_gate;
+                    j.learn_intra_gate = lc.intra_gate;
+                    j.learn_fallback = lc.fallback_pct;
+                }
             }
     }
     // The other groups' streams start after everything queued on the first one.
@@ -680,6 +718,20 @@ rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, co
     }
     HIP_TRY(hipStreamSynchronize(cs[0]->stream));
     return RGH_OK;
+}
+
+rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src,
+                              rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag, int32_t groups,
+                              int32_t flags) {
+    return analyse_frames_impl(ctxs, n, n_frames, src, nullptr, nullptr, decisions, levels, lag, groups, flags);
+}
+
+rgh_status rgh_analyse_frames_learned(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src,
+                                      const rgh_proposal *const *proposals, const rgh_learned_config *config,
+                                      rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag,
+                                      int32_t groups, int32_t flags) {
+    if (!proposals) return RGH_ERR_INVALID_ARGUMENT;
+    return analyse_frames_impl(ctxs, n, n_frames, src, proposals, config, decisions, levels, lag, groups, flags);
 }
 
 rgh_status analyse_host_frame(void *ctx, const rgh_i420 *src, rgh_mb_decision *decisions, int16_t *levels,

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include "../../include/ragdoll_hip.h"
+#include "../../include/ragdoll_learned.h"
 
 #define RGH_PAD_Y 48      // luma padding of reconstructed pictures (enc.rs:56)
 #define RGH_PAD_C 16      // chroma padding (enc.rs:58)
@@ -46,6 +47,9 @@ struct RghJob {
     int32_t intra;             // an IDR picture (no reference)
     int32_t compact;           // the Compact preset (enc/inter.rs, enc/compact.rs)
     int32_t cqo;               // chroma_qp_index_offset
+    // Learned proposals (rgh_k_learned_step): one record per macroblock, and the search effort.
+    const rgh_proposal *prop;
+    int32_t learn_iters, learn_skip_gate, learn_intra_gate, learn_fallback;
     // Schedule.
     int32_t base;              // launch of the frame's first wavefront step
     uint32_t serial, ref_serial;  // this frame's and its reference's serial numbers
@@ -75,6 +79,8 @@ struct RghJobD {
     RGH_G int16_t *levels;
     uint32_t sb_luma, sb_luma_ssd, sb_chroma, sb_chroma_ssd, sb_probe;
     int32_t steps, deblock, intra, compact, cqo, base;
+    const RGH_G rgh_proposal *prop;
+    int32_t learn_iters, learn_skip_gate, learn_intra_gate, learn_fallback;
     uint32_t serial, ref_serial;
     RGH_G uint64_t *ana;
     RGH_G uint64_t *fin;
@@ -129,6 +135,11 @@ __device__ __forceinline__ RghJobD device_view(const RghJob &j) {
     d.intra = j.intra;
     d.compact = j.compact;
     d.cqo = j.cqo;
+    d.prop = gp(j.prop);
+    d.learn_iters = j.learn_iters;
+    d.learn_skip_gate = j.learn_skip_gate;
+    d.learn_intra_gate = j.learn_intra_gate;
+    d.learn_fallback = j.learn_fallback;
     d.base = j.base;
     d.serial = j.serial;
     d.ref_serial = j.ref_serial;

@@ -199,12 +199,16 @@ __global__ void k_to_bf16(const float *x, uint16_t *y, int n) {
 // Features: one wave per macroblock.
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-__global__ __launch_bounds__(64) void k_features(const uint8_t *cur, int cs, const uint8_t *prev, int ps, int W,
-                                                 int H, int mb_w, uint16_t *F) {
+__global__ __launch_bounds__(64) void k_features(const rgh_pnet_picture *pics) {
     __shared__ int rc[16 * 16];
     __shared__ int sc[16];
     __shared__ int red[2][64];
-    const int mb = blockIdx.x, lane = threadIdx.x;
+    const rgh_pnet_picture &P = pics[blockIdx.y];
+    const uint8_t *cur = P.cur_y, *prev = P.prev_y;
+    const int cs = P.cur_stride, ps = P.prev_stride, W = P.width, H = P.height, qp = P.qp;
+    const int mb_w = (W + 15) >> 4, mb = blockIdx.x, lane = threadIdx.x;
+    if (mb >= mb_w * ((H + 15) >> 4)) return;
+    uint16_t *F = P.features;
     const int x0 = (mb % mb_w) * 16, y0 = (mb / mb_w) * 16;
     // The previous picture's 4x4 means over the 64x64 window at (x0 - 24, y0 - 24).
     for (int c = lane; c < 256; c += 64) {
@@ -254,7 +258,8 @@ __global__ __launch_bounds__(64) void k_features(const uint8_t *cur, int cs, con
         for (int i = 0; i < 64; i++) t += red[1][i];
         f[NCLS + 16] = f2bf((float)((t + 128) >> 8) * (1.0f / 16.0f));
     }
-    if (lane >= 17 && lane < 17 + (F_IN - NCLS - 17)) f[NCLS + lane] = 0;
+    if (lane == 17) f[NCLS + 17] = f2bf((float)qp * (1.0f / 32.0f));
+    if (lane >= 18 && lane < F_IN - NCLS) f[NCLS + lane] = 0;
 }
 
 __global__ void k_labels(const rgh_mb_decision *d, int n, rgh_pnet_label *l) {
@@ -470,7 +475,8 @@ uint8_t host_fp8(float x) {
 }
 
 struct Pnet {
-    int device = 0, cap = 0, step = 0, quantized = 0;
+    int device = 0, cap = 0, step = 0, quantized = 0, pics_cap = 0;
+    rgh_pnet_picture *pics = nullptr;  // device copy of a features batch
     hipStream_t stream = nullptr;
     float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr, *qs = nullptr, *loss = nullptr;
     uint16_t *pb = nullptr;
@@ -495,6 +501,7 @@ void free_net(Pnet *p) {
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     for (void *q : p->owned) (void)hipFree(q);
+    if (p->pics) (void)hipFree(p->pics);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -633,19 +640,39 @@ rgh_status rgh_pnet_get_params(void *net, float *params, int64_t n) {
     return RGH_OK;
 }
 
-rgh_status rgh_pnet_features(void *net, const uint8_t *cur, int32_t cur_stride, const uint8_t *prev,
-                             int32_t prev_stride, int32_t width, int32_t height, uint16_t *features) {
+rgh_status rgh_pnet_features_batch(void *net, const rgh_pnet_picture *pics, int32_t n) {
     Pnet *p = (Pnet *)net;
-    if (!p || !cur || !prev || !features) FAIL(RGH_ERR_INVALID_ARGUMENT, "null pointer");
-    if (width < 1 || height < 1 || width > 16384 || height > 16384 || cur_stride < width || prev_stride < width)
-        FAIL(RGH_ERR_INVALID_DIMENSIONS, "bad picture size or stride");
+    if (!p || !pics || n < 0 || n > 65535) FAIL(RGH_ERR_INVALID_ARGUMENT, "null pointer or bad picture count");
+    int max_mbs = 0;
+    for (int i = 0; i < n; i++) {
+        const rgh_pnet_picture &q = pics[i];
+        if (!q.cur_y || !q.prev_y || !q.features) FAIL(RGH_ERR_INVALID_ARGUMENT, "null pointer");
+        if (q.width < 1 || q.height < 1 || q.width > 16384 || q.height > 16384 || q.cur_stride < q.width ||
+            q.prev_stride < q.width)
+            FAIL(RGH_ERR_INVALID_DIMENSIONS, "bad picture size or stride");
+        if (q.qp < 0 || q.qp > 51) FAIL(RGH_ERR_INVALID_ARGUMENT, "qp outside 0..51");
+        max_mbs = std::max(max_mbs, ((q.width + 15) / 16) * ((q.height + 15) / 16));
+    }
+    if (n == 0) return RGH_OK;
     TRY(hipSetDevice(p->device));
-    const int mb_w = (width + 15) / 16, mb_h = (height + 15) / 16;
-    hipLaunchKernelGGL(k_features, dim3(mb_w * mb_h), dim3(64), 0, p->stream, cur, (int)cur_stride, prev,
-                       (int)prev_stride, (int)width, (int)height, mb_w, features);
+    if (p->pics_cap < n) {
+        if (p->pics) TRY(hipFree(p->pics));
+        p->pics = nullptr;
+        p->pics_cap = 0;
+        TRY(hipMalloc(&p->pics, sizeof(rgh_pnet_picture) * (size_t)n));
+        p->pics_cap = n;
+    }
+    TRY(hipMemcpyAsync(p->pics, pics, sizeof(rgh_pnet_picture) * (size_t)n, hipMemcpyHostToDevice, p->stream));
+    hipLaunchKernelGGL(k_features, dim3(max_mbs, n), dim3(64), 0, p->stream, (const rgh_pnet_picture *)p->pics);
     TRY(hipGetLastError());
     TRY(hipStreamSynchronize(p->stream));
     return RGH_OK;
+}
+
+rgh_status rgh_pnet_features(void *net, const uint8_t *cur, int32_t cur_stride, const uint8_t *prev,
+                             int32_t prev_stride, int32_t width, int32_t height, int32_t qp, uint16_t *features) {
+    const rgh_pnet_picture q{cur, prev, features, cur_stride, prev_stride, width, height, qp, 0};
+    return rgh_pnet_features_batch(net, &q, 1);
 }
 
 rgh_status rgh_pnet_labels(void *net, const rgh_mb_decision *dec, int32_t n, rgh_pnet_label *labels) {
@@ -714,30 +741,14 @@ rgh_status rgh_pnet_forward(void *net, const uint16_t *X, int32_t n, float *logi
     return RGH_OK;
 }
 
-rgh_status rgh_pnet_quantize(void *net, const uint16_t *calib, int32_t n) {
-    Pnet *p = (Pnet *)net;
-    if (!p || !calib || n < 1) FAIL(RGH_ERR_INVALID_ARGUMENT, "calibration rows required");
-    TRY(hipSetDevice(p->device));
-    const int k = std::min<int>(n, p->cap);
-    TRY(forward(p, calib, k, p->zf));
+// Quantizes the current parameters with the given hidden-activation maxima (see the header).
+static rgh_status quantize_ranges(Pnet *p, float h1max, float h2max) {
     std::vector<float> P(NPARAM);
-    std::vector<uint16_t> h1((size_t)k * HID), h2((size_t)k * HID);
     TRY(hipMemcpyAsync(P.data(), p->params, NPARAM * sizeof(float), hipMemcpyDeviceToHost, p->stream));
-    TRY(hipMemcpyAsync(h1.data(), p->h1b, h1.size() * 2, hipMemcpyDeviceToHost, p->stream));
-    TRY(hipMemcpyAsync(h2.data(), p->h2b, h2.size() * 2, hipMemcpyDeviceToHost, p->stream));
     TRY(hipStreamSynchronize(p->stream));
-    auto bf = [](uint16_t h) {
-        uint32_t u = (uint32_t)h << 16;
-        float f;
-        std::memcpy(&f, &u, 4);
-        return f;
-    };
-    float amax[3] = {0.0f, 0.0f, 0.0f};
-    for (uint16_t h : h1) amax[1] = std::max(amax[1], bf(h));
-    for (uint16_t h : h2) amax[2] = std::max(amax[2], bf(h));
-    // Layer inputs: features as they are (all below 8); hidden activations scaled so that the
-    // calibration maximum maps to 448.
-    float mul[3] = {1.0f, amax[1] > 0.0f ? FP8_MAX / amax[1] : 1.0f, amax[2] > 0.0f ? FP8_MAX / amax[2] : 1.0f};
+    // Layer inputs: features as they are (all below 8); hidden activations scaled so that their
+    // maximum maps to 448.
+    const float mul[3] = {1.0f, h1max > 0.0f ? FP8_MAX / h1max : 1.0f, h2max > 0.0f ? FP8_MAX / h2max : 1.0f};
     std::vector<uint8_t> w8(Q_BYTES);
     std::vector<float> qs(RGH_PNET_SCALES, 0.0f);
     const int offs[3] = {OFF_W1, OFF_W2, OFF_W3}, qoff[3] = {Q_W1, Q_W2, Q_W3}, kin[3] = {F_IN, HID, HID},
@@ -757,6 +768,35 @@ rgh_status rgh_pnet_quantize(void *net, const uint16_t *calib, int32_t n) {
     TRY(hipStreamSynchronize(p->stream));
     p->quantized = 1;
     return RGH_OK;
+}
+
+rgh_status rgh_pnet_quantize(void *net, const uint16_t *calib, int32_t n) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !calib || n < 1) FAIL(RGH_ERR_INVALID_ARGUMENT, "calibration rows required");
+    TRY(hipSetDevice(p->device));
+    const int k = std::min<int>(n, p->cap);
+    TRY(forward(p, calib, k, p->zf));
+    std::vector<uint16_t> h1((size_t)k * HID), h2((size_t)k * HID);
+    TRY(hipMemcpyAsync(h1.data(), p->h1b, h1.size() * 2, hipMemcpyDeviceToHost, p->stream));
+    TRY(hipMemcpyAsync(h2.data(), p->h2b, h2.size() * 2, hipMemcpyDeviceToHost, p->stream));
+    TRY(hipStreamSynchronize(p->stream));
+    auto bf = [](uint16_t h) {
+        uint32_t u = (uint32_t)h << 16;
+        float f;
+        std::memcpy(&f, &u, 4);
+        return f;
+    };
+    float m1 = 0.0f, m2 = 0.0f;
+    for (uint16_t h : h1) m1 = std::max(m1, bf(h));
+    for (uint16_t h : h2) m2 = std::max(m2, bf(h));
+    return quantize_ranges(p, m1, m2);
+}
+
+rgh_status rgh_pnet_quantize_ranges(void *net, float h1_max, float h2_max) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !(h1_max >= 0.0f) || !(h2_max >= 0.0f)) FAIL(RGH_ERR_INVALID_ARGUMENT, "bad activation maxima");
+    TRY(hipSetDevice(p->device));
+    return quantize_ranges(p, h1_max, h2_max);
 }
 
 rgh_status rgh_pnet_infer(void *net, const uint16_t *X, int32_t n, rgh_proposal *out, float *logits) {

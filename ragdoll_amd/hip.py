@@ -40,6 +40,11 @@ class Hip:
         self.check(self.L.hipMemcpy(p, arr.ctypes.data, arr.nbytes, 1), "hipMemcpy")  # host to device
         return p
 
+    def upload_to(self, ptr, arr):
+        """Copies the contiguous array arr into device memory at ptr."""
+        arr = np.ascontiguousarray(arr)
+        self.check(self.L.hipMemcpy(ptr, arr.ctypes.data, arr.nbytes, 1), "hipMemcpy")  # host to device
+
     def download(self, ptr, out):
         """Copies out.nbytes from device address ptr into the contiguous array out."""
         self.check(self.L.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2), "hipMemcpy")  # device to host

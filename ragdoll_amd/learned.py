@@ -7,6 +7,8 @@ analysed decisions, bfloat16 training steps, FP8 quantization and inference into
 from the HIP runtime the library links (ragdoll_amd.hip); nothing here computes on the CPU.
 """
 import ctypes
+import json
+import os
 
 import numpy as np
 
@@ -30,11 +32,17 @@ PROPOSAL_DTYPE = np.dtype([("cell", "i1", (4, 2)), ("mode_prob", "u1", (3,)), ("
 assert PROPOSAL_DTYPE.itemsize == 16
 
 EXPORTS = ["rgh_pnet_create", "rgh_pnet_destroy", "rgh_pnet_set_params", "rgh_pnet_get_params",
-           "rgh_pnet_features", "rgh_pnet_labels", "rgh_pnet_gather", "rgh_pnet_train_step",
-           "rgh_pnet_gradients", "rgh_pnet_forward", "rgh_pnet_quantize", "rgh_pnet_infer",
-           "rgh_pnet_get_quantized", "rgh_pnet_fp8_encode", "rgh_pnet_gemm_bf16"]
+           "rgh_pnet_features", "rgh_pnet_features_batch", "rgh_pnet_labels", "rgh_pnet_gather", "rgh_pnet_train_step",
+           "rgh_pnet_gradients", "rgh_pnet_forward", "rgh_pnet_quantize", "rgh_pnet_quantize_ranges", "rgh_pnet_infer",
+           "rgh_pnet_get_quantized", "rgh_pnet_fp8_encode", "rgh_pnet_gemm_bf16", "rgh_analyse_frames_learned"]
 
 _bound = False
+
+
+class RgPnetPicture(ctypes.Structure):
+    _fields_ = [("cur_y", ctypes.c_void_p), ("prev_y", ctypes.c_void_p), ("features", ctypes.c_void_p),
+                ("cur_stride", ctypes.c_int32), ("prev_stride", ctypes.c_int32), ("width", ctypes.c_int32),
+                ("height", ctypes.c_int32), ("qp", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 def _bind():
@@ -48,13 +56,15 @@ def _bind():
         "rgh_pnet_destroy": [P],
         "rgh_pnet_set_params": [P, P, I64],
         "rgh_pnet_get_params": [P, P, I64],
-        "rgh_pnet_features": [P, P, I32, P, I32, I32, I32, P],
+        "rgh_pnet_features": [P, P, I32, P, I32, I32, I32, I32, P],
+        "rgh_pnet_features_batch": [P, P, I32],
         "rgh_pnet_labels": [P, P, I32, P],
         "rgh_pnet_gather": [P, P, P, P, I32, P, P],
         "rgh_pnet_train_step": [P, P, P, I32, F32, P],
         "rgh_pnet_gradients": [P, P, P, I32, P, P],
         "rgh_pnet_forward": [P, P, I32, P],
         "rgh_pnet_quantize": [P, P, I32],
+        "rgh_pnet_quantize_ranges": [P, F32, F32],
         "rgh_pnet_infer": [P, P, I32, P, P],
         "rgh_pnet_get_quantized": [P, P, P],
         "rgh_pnet_fp8_encode": [P, P, I32, P],
@@ -112,9 +122,16 @@ class ProposalNet:
         _check(self.L.rgh_pnet_get_params(self.h, p.ctypes.data, PARAMS), "rgh_pnet_get_params")
         return p
 
-    def features(self, cur_y_dev, cur_stride, prev_y_dev, prev_stride, width, height, out_dev):
-        _check(self.L.rgh_pnet_features(self.h, cur_y_dev, cur_stride, prev_y_dev, prev_stride, width, height,
+    def features(self, cur_y_dev, cur_stride, prev_y_dev, prev_stride, width, height, qp, out_dev):
+        _check(self.L.rgh_pnet_features(self.h, cur_y_dev, cur_stride, prev_y_dev, prev_stride, width, height, qp,
                                         out_dev), "rgh_pnet_features")
+
+    def features_batch(self, pictures):
+        """pictures: (cur_y_dev, cur_stride, prev_y_dev, prev_stride, width, height, qp, out_dev) tuples."""
+        arr = (RgPnetPicture * max(1, len(pictures)))()
+        for i, (c, cs, p, ps, w, h, qp, out) in enumerate(pictures):
+            arr[i] = RgPnetPicture(c, p, out, cs, ps, w, h, qp, 0)
+        _check(self.L.rgh_pnet_features_batch(self.h, arr, len(pictures)), "rgh_pnet_features_batch")
 
     def labels(self, decisions_dev, n, out_dev):
         _check(self.L.rgh_pnet_labels(self.h, decisions_dev, n, out_dev), "rgh_pnet_labels")
@@ -142,6 +159,14 @@ class ProposalNet:
     def quantize(self, calib_dev, n):
         _check(self.L.rgh_pnet_quantize(self.h, calib_dev, n), "rgh_pnet_quantize")
 
+    def quantize_ranges(self, h1_max, h2_max):
+        _check(self.L.rgh_pnet_quantize_ranges(self.h, h1_max, h2_max), "rgh_pnet_quantize_ranges")
+
+    def activation_maxima(self):
+        """The hidden-activation maxima the FP8 copy was built with."""
+        _, sc = self.get_quantized()
+        return [float(448.0 / sc[3 * 176 + 1]), float(448.0 / sc[3 * 176 + 2])]
+
     def infer(self, feat_dev, n, proposals_dev, logits_dev=None):
         _check(self.L.rgh_pnet_infer(self.h, feat_dev, n, proposals_dev, logits_dev), "rgh_pnet_infer")
 
@@ -159,7 +184,7 @@ class ProposalNet:
                                          accumulate), "rgh_pnet_gemm_bf16")
 
     # ---- host-array conveniences (upload, run, download) ----
-    def features_host(self, cur_y, prev_y):
+    def features_host(self, cur_y, prev_y, qp=26):
         """Features of every macroblock of two host luma planes (same size)."""
         cur_y = np.ascontiguousarray(cur_y, np.uint8)
         prev_y = np.ascontiguousarray(prev_y, np.uint8)
@@ -170,7 +195,7 @@ class ProposalNet:
         hip = self.hip
         c, p, f = hip.upload(cur_y), hip.upload(prev_y), hip.alloc(n * FEATURES * 2)
         try:
-            self.features(c, W, p, W, W, H, f)
+            self.features(c, W, p, W, W, H, qp, f)
             return hip.download(f, np.zeros((n, FEATURES), np.uint16))
         finally:
             for q in (c, p, f):
@@ -213,3 +238,53 @@ class ProposalNet:
         finally:
             hip.free(dd)
             hip.free(ld)
+
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "weights", "proposal_net.npz")
+
+
+def trained(path=WEIGHTS):
+    """(parameters, [h1_max, h2_max], training record) of the trained network shipped with the
+    package (written by ragdoll_amd.train_proposals)."""
+    with np.load(path, allow_pickle=False) as d:
+        return d["params"].astype(np.float32), [float(v) for v in d["act_max"]], json.loads(str(d["meta"]))
+
+
+def load_trained(device=0, max_batch=4096, path=WEIGHTS):
+    """A ProposalNet holding the shipped parameters, quantized to FP8 as it was evaluated."""
+    params, act_max, _ = trained(path)
+    net = ProposalNet(max_batch=max_batch, device=device, params=params)
+    net.quantize_ranges(*act_max)
+    return net
+
+
+class Proposer:
+    """Learned proposals for batches of device pictures: their features in one launch, then FP8
+    inference over all their macroblocks in one launch, into buffers kept for reuse."""
+
+    def __init__(self, net, max_rows):
+        self.net = net
+        self.max_rows = max_rows
+        self.feat = net.hip.alloc(max_rows * FEATURES * 2)
+        self.props = net.hip.alloc(max_rows * 16)
+
+    def close(self):
+        if self.feat:
+            self.net.hip.free(self.feat)
+            self.net.hip.free(self.props)
+            self.feat = self.props = None
+
+    def run(self, pictures):
+        """pictures: (cur_y_dev, cur_stride, prev_y_dev, prev_stride, width, height, qp) tuples.
+        Returns the device address of each picture's proposal records (valid until the next run)."""
+        rows, jobs, out = 0, [], []
+        for (c, cs, p, ps, w, h, qp) in pictures:
+            n = ((w + 15) // 16) * ((h + 15) // 16)
+            jobs.append((c, cs, p, ps, w, h, qp, self.feat + rows * FEATURES * 2))
+            out.append(self.props + rows * 16)
+            rows += n
+        if rows > self.max_rows:
+            raise ValueError("%d macroblocks exceed the proposer's %d rows" % (rows, self.max_rows))
+        self.net.features_batch(jobs)
+        self.net.infer(self.feat, rows, self.props)
+        return out

@@ -64,7 +64,7 @@ def e4m3_code(x):
 
 
 # ---- features ----
-def features(cur, prev):
+def features(cur, prev, qp=26):
     """Feature rows ([n][192] bfloat16 bits) of every macroblock."""
     cur = np.asarray(cur, np.int64)
     prev = np.asarray(prev, np.int64)
@@ -91,6 +91,7 @@ def features(cur, prev):
         blk = cur[np.ix_(ys, xs)]
         mean = (int(blk.sum()) + 128) >> 8
         out[m, NCLS + 16] = ((int(np.abs(blk - mean).sum()) + 128) >> 8) / 16.0
+        out[m, NCLS + 17] = qp / 32.0
     return bf16_bits(out)
 
 

@@ -72,7 +72,7 @@ def test_features_match_a_brute_force_loop():
     H, W = 21, 37  # 3 x 2 macroblocks, partial ones, clamped borders everywhere
     cur = rng.integers(0, 256, (H, W))
     prev = rng.integers(0, 256, (H, W))
-    got = R.bf16_value(R.features(cur, prev))
+    got = R.bf16_value(R.features(cur, prev, 37))
 
     def px(img, x, y):
         return int(img[min(max(y, 0), H - 1), min(max(x, 0), W - 1)])
@@ -96,7 +96,8 @@ def test_features_match_a_brute_force_loop():
         mean = (sum(samples) + 128) >> 8
         mad = (sum(abs(s - mean) for s in samples) + 128) >> 8
         assert got[m, 185] == R.bf16_round(np.float32(mad / 16.0))
-        assert not got[m, 186:].any()
+        assert got[m, 186] == R.bf16_round(np.float32(37 / 32.0))
+        assert not got[m, 187:].any()
 
 
 def test_labels_follow_the_nearest_cell():
@@ -149,3 +150,17 @@ def test_quantize_reference_reconstructs_the_weights():
         back = R.e4m3_value(codes[l]).T * (scales[l, :w.shape[1]] * mul[l])[None, :]
         # E4M3 keeps 3 mantissa bits: within 1/16 relative, plus the subnormal step.
         assert np.all(np.abs(back - w) <= np.abs(w) / 16 + np.abs(w).max(0) / 448 * 2.0 ** -9 + 1e-12)
+
+
+def test_bd_rate_of_a_scaled_curve_is_the_scale():
+    from ragdoll_amd.rd import bd_rate, psnr
+    rate = np.array([1000.0, 1800.0, 3500.0, 7000.0])
+    q = np.array([30.0, 33.0, 36.5, 40.0])
+    assert abs(bd_rate(rate, q, rate, q)) < 1e-9
+    assert abs(bd_rate(rate, q, rate * 0.9, q) + 10.0) < 1e-6
+    assert abs(bd_rate(rate, q, rate * 1.01, q) - 1.0) < 1e-6
+    # A curve shifted up in PSNR reaches the same quality with fewer bits.
+    assert bd_rate(rate, q, rate, q + 0.5) < 0
+    a = np.full((4, 4), 10, np.uint8)
+    assert psnr(a, a) == 100.0
+    assert abs(psnr(a, a + 1) - 10 * np.log10(255.0 ** 2)) < 1e-9
