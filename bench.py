@@ -172,6 +172,7 @@ def main():
     ap.add_argument("--skip-gate", type=int, default=0)
     ap.add_argument("--intra-gate", type=int, default=0)
     ap.add_argument("--fallback-pct", type=int, default=105)
+    ap.add_argument("--intra-first", type=int, default=0)
     ap.add_argument("--bd-frames", type=int, default=30, help="frames per QP of the BD-rate sample (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -200,7 +201,7 @@ def main():
 
 def learned_config(args):
     return dict(iters=args.learned_iters, skip_gate=args.skip_gate, intra_gate=args.intra_gate,
-                fallback_pct=args.fallback_pct)
+                fallback_pct=args.fallback_pct, intra_first=args.intra_first)
 
 
 def bd_rate_sample(args):
@@ -368,9 +369,12 @@ def run_workload(gops, args):
 
 
 def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value, hip):
-    """The learned-proposal form on the same streams and schedule: each step computes the
-    proposals of every P frame (features from the frame and the previous source frame, FP8
-    inference) and analyses the GOPs from them (rgh_analyse_frames_learned); both are timed."""
+    """The learned-proposal form on the same streams and schedule. A step analyses every GOP from
+    its proposals (rgh_analyse_frames_learned) while the next step's proposals (features from
+    each P frame and its previous source frame, then FP8 inference over all their macroblocks)
+    are computed on the network's own HIP stream: a stream of GOP waves pipelines them so, and
+    the proposals of the first timed step come from the warm-up. The serial form (proposals,
+    then analysis) is timed too."""
     from ragdoll_amd import DECISION_DTYPE, analyse_frames
     from ragdoll_amd.learned import Proposer, load_trained
     net = load_trained()
@@ -378,56 +382,84 @@ def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value
     for i, gp in enumerate(gops):
         for k in range(1, len(gp["frames"])):
             pics.append((srcs[i][k][0], srcs[i][k][3], srcs[i][k - 1][0], srcs[i][k - 1][3], gp["w"], gp["h"], gp["qp"]))
-    proposer = Proposer(net, total_mbs)
+    proposers = [Proposer(net, total_mbs), Proposer(net, total_mbs)]
     cfg = learned_config(args)
 
-    def step():
-        t = time.perf_counter()
-        props = iter(proposer.run(pics))
-        t1 = time.perf_counter()
-        P = [[None] + [next(props) for _ in gp["frames"][1:]] for gp in gops]
+    def analyse(props):
+        it = iter(props)
+        P = [[None] + [next(it) for _ in gp["frames"][1:]] for gp in gops]
         analyse_frames(ans, srcs, decs, lvls, lag=sched[0], groups=sched[1], first_idr=True, proposals=P,
                        learned=cfg)
+
+    def serial_step():
+        t = time.perf_counter()
+        props = proposers[0].run(pics)
+        t1 = time.perf_counter()
+        analyse(props)
         return time.perf_counter() - t, t1 - t
 
+    def pipelined_step(b, props):
+        box = {}
+        th = threading.Thread(target=lambda: box.setdefault("p", proposers[b].run(pics)))
+        t = time.perf_counter()
+        th.start()
+        analyse(props)
+        th.join()
+        return time.perf_counter() - t, box["p"]
+
     for _ in range(max(1, args.warmup)):
-        step()
-    el = prop = 0.0
+        serial_step()
+    el_serial = prop = 0.0
     for _ in range(args.steps):
-        e, p = step()
-        el += e
+        e, p = serial_step()
+        el_serial += e
         prop += p
+    props = proposers[0].run(pics)
+    b = 1
+    for _ in range(max(1, args.warmup)):
+        _, props = pipelined_step(b, props)
+        b ^= 1
+    el = 0.0
+    for _ in range(args.steps):
+        e, props = pipelined_step(b, props)
+        el += e
+        b ^= 1
     # FP8 inference alone (one launch over every P macroblock of the step), for its roofline.
     t = time.perf_counter()
     reps = 3
     for _ in range(reps):
-        net.infer(proposer.feat, total_mbs, proposer.props)
+        net.infer(proposers[0].feat, total_mbs, proposers[0].props)
     infer_s = (time.perf_counter() - t) / reps
-    flops = total_mbs * 2 * (192 * 128 + 128 * 128 + 128 * 176)
+    flops_mb = 2 * (192 * 128 + 128 * 128 + 128 * 176)
+    flops = total_mbs * flops_mb
     # Decision statistics of the last step's P frames.
-    counts = np.zeros(4, np.int64)
+    counts = np.zeros(5, np.int64)
     types = np.zeros(8, np.int64)
     for i, gp in enumerate(gops):
         n = ans[i].mb_w * ans[i].mb_h
         for k in range(1, len(gp["frames"])):
             d = hip.download(decs[i][k], np.zeros(n, DECISION_DTYPE))
             types += np.bincount(d["mb_type"], minlength=8)[:8]
-            for b in range(4):
-                counts[b] += int(np.count_nonzero(d["flags"] & (1 << b)))
-    proposer.close()
+            for bit in range(5):
+                counts[bit] += int(np.count_nonzero(d["flags"] & (1 << bit)))
+    for q in proposers:
+        q.close()
     net.close()
     value = total_mbs * args.steps / el
     return {"value": round(value, 1), "unit": "macroblocks/s", "vs_plain": round(value / plain_value, 4),
             "ms_per_step": round(el / args.steps * 1e3, 3),
+            "serial_value": round(total_mbs * args.steps / el_serial, 1),
+            "serial_ms_per_step": round(el_serial / args.steps * 1e3, 3),
             "proposal_ms_per_step": round(prop / args.steps * 1e3, 3),
             "config": cfg, "lag": sched[0], "groups": sched[1],
             "plain_search_pct": round(100.0 * counts[1] / total_mbs, 3),
             "skip_probe_gated_pct": round(100.0 * counts[2] / total_mbs, 3),
             "intra_gated_pct": round(100.0 * counts[3] / total_mbs, 3),
+            "full_pel_only_pct": round(100.0 * counts[4] / total_mbs, 3),
             "mb_types": types.tolist(),
             "mfma_roofline": {"bound": "mfma", "kernel": "k_infer (FP8 proposal network)",
                               "achieved": round(flops / infer_s / 1e12, 3), "peak": 5000.0, "unit": "TFLOP/s",
-                              "frac": flops / infer_s / 1e12 / 5000.0, "flops_per_mb": 2 * (192 * 128 + 128 * 128 + 128 * 176),
+                              "frac": flops / infer_s / 1e12 / 5000.0, "flops_per_mb": flops_mb,
                               "launch_ms": round(infer_s * 1e3, 3)}}
 
 

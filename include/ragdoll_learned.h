@@ -142,6 +142,9 @@ typedef struct rgh_learned_config {
     int32_t intra_gate;   /* the intra 16x16 check runs when mode_prob[2] >= intra_gate (default 0: always) */
     int32_t fallback_pct; /* the plain search runs when the cheapest proposal costs more than
                              fallback_pct percent of the predictor's cost (default 105) */
+    int32_t intra_first;  /* when mode_prob[2] >= intra_first, the macroblock keeps its cheapest
+                             full-pel candidate (no diamond, no sub-pel refinement) before the
+                             intra 16x16 check (default 0: never) */
 } rgh_learned_config;
 
 /* rgh_analyse_frames (ragdoll_hip.h) with learned proposals: proposals[c * n_frames + k] holds
@@ -152,7 +155,8 @@ typedef struct rgh_learned_config {
  * (the plain search), otherwise at most `iters` iterations; the skip probe and the intra check
  * run as gated above. SAD, SATD and rate-distortion arithmetic, sub-pel refinement, residual
  * coding, reconstruction and deblocking are the plain path's. Decision flags: bit 1 the plain
- * search ran, bit 2 the skip probe was gated off, bit 3 the intra check was gated off.
+ * search ran, bit 2 the skip probe was gated off, bit 3 the intra check was gated off, bit 4
+ * the search stopped at full-pel (intra_first).
  * Balanced contexts only (RGH_ERR_UNSUPPORTED for RGH_FLAG_COMPACT). */
 rgh_status rgh_analyse_frames_learned(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src_device,
                                       const rgh_proposal *const *proposals, const rgh_learned_config *config,

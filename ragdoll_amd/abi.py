@@ -234,7 +234,7 @@ FRAMES_FIRST_IDR = 1
 
 class RgLearnedConfig(ctypes.Structure):
     _fields_ = [("iters", ctypes.c_int32), ("skip_gate", ctypes.c_int32), ("intra_gate", ctypes.c_int32),
-                ("fallback_pct", ctypes.c_int32)]
+                ("fallback_pct", ctypes.c_int32), ("intra_first", ctypes.c_int32)]
 
 
 def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0, first_idr=False, proposals=None,

@@ -1298,8 +1298,9 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
         // ---- Integer candidates: the first strictly cheapest wins. ----
         const RGH_G uint8_t *rb = J.ref_y + (long)(y0 + rr) * J.sy + x0 + c0;
         int bestx = 0, besty = 0;
-        uint32_t best_cost = 0xFFFFFFFFu;
+        uint32_t best_cost = 0xFFFFFFFFu, bestw = 0;
         int iters = J.me_iters;
+        bool quick = false;  // learned: an intra-likely macroblock keeps its full-pel candidate
         auto take = [&](int k) {
             if (!live[k]) return;
             const uint32_t cst = wave_sum(__builtin_amdgcn_sad_u8(srcw, rw[k], 0)) + mv_cost(fxs[k] * 4, fys[k] * 4);
@@ -1308,6 +1309,7 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
                 best_cost = cst;
                 bestx = fxs[k];
                 besty = fys[k];
+                bestw = rw[k];
             }
         };
         if constexpr (LEARNED) {
@@ -1322,8 +1324,12 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
                 best_cost = c_mvp;
                 bestx = fxs[0];
                 besty = fys[0];
+                bestw = rw[0];
             }
-            if ((uint64_t)c_prop * 100u > (uint64_t)c_mvp * (uint32_t)J.learn_fallback) {
+            quick = p_intra >= J.learn_intra_first;
+            if (quick) {
+                D.flags |= 16;
+            } else if ((uint64_t)c_prop * 100u > (uint64_t)c_mvp * (uint32_t)J.learn_fallback) {
                 D.flags |= 2;
 #pragma unroll
                 for (int k = 5; k < NC; k++) take(k);
@@ -1335,6 +1341,12 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
             for (int k = 0; k < 5; k++) take(k);
         }
 
+        uint32_t me_cost = best_cost;
+        if (quick) {
+            predw = bestw;
+            mvx = bestx * 4;
+            mvy = besty * 4;
+        } else {
         // ---- Round trip 4: the reference region around the best candidate, in LDS. ----
         // Columns [bx - 19 (aligned down), +68) and rows [by - 19, +55) hold every 16x16 block
         // the diamond can reach from the candidate in 16 steps and the half-pel window of any of
@@ -1383,7 +1395,6 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
 
         // Sub-pel refinement unless the full-pel residual already quantizes to nothing
         // (enc.rs:3097-3117; enc.rs:1283 sub_pel: half-pel, then quarter-pel diamonds).
-        uint32_t me_cost = best_cost;
         const int fbx = x0 + bestx, fby = y0 + besty;
         int rox = ox, roy = oy;
         if (fbx - 3 < rox || fbx + 29 > rox + SR_COLS || fby - 3 < roy || fby + 20 > roy + SR_H) {
@@ -1428,6 +1439,7 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
             predw = window_pred4(x0, y0, fbx, fby, rox, roy, bmx, bmy, S);
             mvx = bmx;
             mvy = bmy;
+        }
         }
         D.cost = me_cost;
         // ---- Round trip 5 (overlapped with the SATD): chroma taps at the final vector. ----

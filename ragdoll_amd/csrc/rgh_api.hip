@@ -616,16 +616,18 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
         (flags & ~RGH_FRAMES_FIRST_IDR))
         return RGH_ERR_INVALID_ARGUMENT;
     const bool first_idr = (flags & RGH_FRAMES_FIRST_IDR) != 0;
-    rgh_learned_config lc{4, 0, 0, 105};
+    rgh_learned_config lc{4, 0, 0, 105, 256};
     if (props) {
         if (lcfg) {
             if (lcfg->iters < 0 || lcfg->iters > 64 || lcfg->skip_gate < 0 || lcfg->skip_gate > 256 ||
-                lcfg->intra_gate < 0 || lcfg->intra_gate > 256 || lcfg->fallback_pct < 0 || lcfg->fallback_pct > 100000)
+                lcfg->intra_gate < 0 || lcfg->intra_gate > 256 || lcfg->fallback_pct < 0 ||
+                lcfg->fallback_pct > 100000 || lcfg->intra_first < 0 || lcfg->intra_first > 256)
                 return RGH_ERR_INVALID_ARGUMENT;
             if (lcfg->iters) lc.iters = lcfg->iters;
             lc.skip_gate = lcfg->skip_gate;
             lc.intra_gate = lcfg->intra_gate;
             if (lcfg->fallback_pct) lc.fallback_pct = lcfg->fallback_pct;
+            if (lcfg->intra_first) lc.intra_first = lcfg->intra_first;
         }
         for (int i = 0; i < n; i++) {
             const Ctx *c = static_cast<Ctx *>(ctxs[i]);
@@ -691,6 +693,7 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
                     j.learn_skip_gate = lc.skip_gate;
                     j.learn_intra_gate = lc.intra_gate;
                     j.learn_fallback = lc.fallback_pct;
+                    j.learn_intra_first = lc.intra_first;
                 }
             }
     }

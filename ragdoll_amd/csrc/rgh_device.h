@@ -49,7 +49,7 @@ struct RghJob {
     int32_t cqo;               // chroma_qp_index_offset
     // Learned proposals (rgh_k_learned_step): one record per macroblock, and the search effort.
     const rgh_proposal *prop;
-    int32_t learn_iters, learn_skip_gate, learn_intra_gate, learn_fallback;
+    int32_t learn_iters, learn_skip_gate, learn_intra_gate, learn_fallback, learn_intra_first;
     // Schedule.
     int32_t base;              // launch of the frame's first wavefront step
     uint32_t serial, ref_serial;  // this frame's and its reference's serial numbers
@@ -80,7 +80,7 @@ struct RghJobD {
     uint32_t sb_luma, sb_luma_ssd, sb_chroma, sb_chroma_ssd, sb_probe;
     int32_t steps, deblock, intra, compact, cqo, base;
     const RGH_G rgh_proposal *prop;
-    int32_t learn_iters, learn_skip_gate, learn_intra_gate, learn_fallback;
+    int32_t learn_iters, learn_skip_gate, learn_intra_gate, learn_fallback, learn_intra_first;
     uint32_t serial, ref_serial;
     RGH_G uint64_t *ana;
     RGH_G uint64_t *fin;
@@ -140,6 +140,7 @@ __device__ __forceinline__ RghJobD device_view(const RghJob &j) {
     d.learn_skip_gate = j.learn_skip_gate;
     d.learn_intra_gate = j.learn_intra_gate;
     d.learn_fallback = j.learn_fallback;
+    d.learn_intra_first = j.learn_intra_first;
     d.base = j.base;
     d.serial = j.serial;
     d.ref_serial = j.ref_serial;

@@ -199,52 +199,68 @@ __global__ void k_to_bf16(const float *x, uint16_t *y, int n) {
 // Features: one wave per macroblock.
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
-__global__ __launch_bounds__(64) void k_features(const rgh_pnet_picture *pics) {
+// 4x4-mean images: cell (cx, cy) of a picture, -6 <= cx < 4 * mb_w + 6 and likewise cy, is
+// the rounded mean of the samples (4cx + i, 4cy + j), coordinates clamped to the picture. One
+// image per distinct plane of a features batch; every macroblock's 64x64 window and its own 4x4
+// cells are then read from it instead of from the samples.
+constexpr int MPAD = 6;
+struct MeanPlane {
+    const uint8_t *y;
+    uint8_t *m;
+    int stride, W, H, mw, mh;  // mw = 4 * mb_w + 12, mh likewise
+};
+__global__ __launch_bounds__(256) void k_means(const MeanPlane *planes) {
+    const MeanPlane &P = planes[blockIdx.y];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= P.mw * P.mh) return;
+    const int cx = i % P.mw - MPAD, cy = i / P.mw - MPAD;
+    int s = 0;
+    for (int j = 0; j < 4; j++) {
+        const uint8_t *row = P.y + (long)min(max(4 * cy + j, 0), P.H - 1) * P.stride;
+        for (int k = 0; k < 4; k++) s += row[min(max(4 * cx + k, 0), P.W - 1)];
+    }
+    P.m[i] = (uint8_t)((s + 8) >> 4);
+}
+
+struct FeatPic {
+    const uint8_t *cur;       // luma samples (mean absolute deviation)
+    const uint8_t *mcur, *mprev;  // mean images of the picture and the previous picture
+    uint16_t *features;
+    int cs, W, H, mw, qp;
+};
+
+// Features: one wave per macroblock.
+__global__ __launch_bounds__(64) void k_features(const FeatPic *pics) {
     __shared__ int rc[16 * 16];
     __shared__ int sc[16];
-    __shared__ int red[2][64];
-    const rgh_pnet_picture &P = pics[blockIdx.y];
-    const uint8_t *cur = P.cur_y, *prev = P.prev_y;
-    const int cs = P.cur_stride, ps = P.prev_stride, W = P.width, H = P.height, qp = P.qp;
-    const int mb_w = (W + 15) >> 4, mb = blockIdx.x, lane = threadIdx.x;
-    if (mb >= mb_w * ((H + 15) >> 4)) return;
-    uint16_t *F = P.features;
-    const int x0 = (mb % mb_w) * 16, y0 = (mb / mb_w) * 16;
-    // The previous picture's 4x4 means over the 64x64 window at (x0 - 24, y0 - 24).
-    for (int c = lane; c < 256; c += 64) {
-        const int bx = x0 - 24 + 4 * (c & 15), by = y0 - 24 + 4 * (c >> 4);
-        int s = 0;
-        for (int j = 0; j < 4; j++) {
-            const uint8_t *row = prev + (long)clampi(by + j, 0, H - 1) * ps;
-            for (int i = 0; i < 4; i++) s += row[clampi(bx + i, 0, W - 1)];
-        }
-        rc[c] = (s + 8) >> 4;
-    }
-    // The macroblock: four samples per lane (row lane / 4, columns 4 * (lane % 4) + [0, 4)).
+    __shared__ int red[64];
+    const FeatPic &P = pics[blockIdx.y];
+    const int mb_w = (P.W + 15) >> 4, mb = blockIdx.x, lane = threadIdx.x;
+    if (mb >= mb_w * ((P.H + 15) >> 4)) return;
+    const int bx = mb % mb_w, by = mb / mb_w, x0 = bx * 16, y0 = by * 16;
+    // The previous picture's cells of the 64x64 window at (x0 - 24, y0 - 24): mean-image cells
+    // (4 bx - 6 + c, 4 by - 6 + r), i.e. indices (4 bx + c, 4 by + r) with the image's offset.
+    for (int c = lane; c < 256; c += 64) rc[c] = P.mprev[(long)(4 * by + (c >> 4)) * P.mw + 4 * bx + (c & 15)];
+    if (lane < 16) sc[lane] = P.mcur[(long)(4 * by + MPAD + (lane >> 2)) * P.mw + 4 * bx + MPAD + (lane & 3)];
+    // The macroblock's samples: four per lane (row lane / 4, columns 4 * (lane % 4) + [0, 4)).
     const int r = lane >> 2, c4 = (lane & 3) * 4;
-    const uint8_t *row = cur + (long)clampi(y0 + r, 0, H - 1) * cs;
+    const uint8_t *row = P.cur + (long)clampi(y0 + r, 0, P.H - 1) * P.cs;
     int px[4], sum = 0;
     for (int i = 0; i < 4; i++) {
-        px[i] = row[clampi(x0 + c4 + i, 0, W - 1)];
+        px[i] = row[clampi(x0 + c4 + i, 0, P.W - 1)];
         sum += px[i];
     }
-    red[0][lane] = sum;
+    red[lane] = sum;
     __syncthreads();
-    if (lane < 16) {
-        // Cell (cx, cy) = (lane % 4, lane / 4) covers rows 4cy..4cy+3 of lanes 4*row + cx.
-        const int cx = lane & 3, cy = lane >> 2;
-        int s = 0;
-        for (int j = 0; j < 4; j++) s += red[0][(4 * cy + j) * 4 + cx];
-        sc[lane] = (s + 8) >> 4;
-    }
     int total = 0;
-    for (int i = 0; i < 64; i++) total += red[0][i];
+    for (int i = 0; i < 64; i++) total += red[i];
     const int mean = (total + 128) >> 8;
     int dev = 0;
     for (int i = 0; i < 4; i++) dev += abs(px[i] - mean);
-    red[1][lane] = dev;
     __syncthreads();
-    uint16_t *f = F + (long)mb * F_IN;
+    red[lane] = dev;
+    __syncthreads();
+    uint16_t *f = P.features + (long)mb * F_IN;
     for (int e = lane; e < NCLS; e += 64) {
         const int dy = e / 13, dx = e % 13;  // cells displaced by (dx - 6, dy - 6)
         int s = 0;
@@ -255,10 +271,10 @@ __global__ __launch_bounds__(64) void k_features(const rgh_pnet_picture *pics) {
     if (lane < 16) f[NCLS + lane] = f2bf((float)sc[lane] * (1.0f / 64.0f));
     if (lane == 16) {
         int t = 0;
-        for (int i = 0; i < 64; i++) t += red[1][i];
+        for (int i = 0; i < 64; i++) t += red[i];
         f[NCLS + 16] = f2bf((float)((t + 128) >> 8) * (1.0f / 16.0f));
     }
-    if (lane == 17) f[NCLS + 17] = f2bf((float)qp * (1.0f / 32.0f));
+    if (lane == 17) f[NCLS + 17] = f2bf((float)P.qp * (1.0f / 32.0f));
     if (lane >= 18 && lane < F_IN - NCLS) f[NCLS + lane] = 0;
 }
 
@@ -475,8 +491,11 @@ uint8_t host_fp8(float x) {
 }
 
 struct Pnet {
-    int device = 0, cap = 0, step = 0, quantized = 0, pics_cap = 0;
-    rgh_pnet_picture *pics = nullptr;  // device copy of a features batch
+    int device = 0, cap = 0, step = 0, quantized = 0;
+    // Features batch state: device copies of the descriptors and the mean images.
+    void *fpics = nullptr, *mplanes = nullptr;
+    uint8_t *means = nullptr;
+    size_t fpics_cap = 0, mplanes_cap = 0, means_cap = 0;
     hipStream_t stream = nullptr;
     float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr, *qs = nullptr, *loss = nullptr;
     uint16_t *pb = nullptr;
@@ -501,7 +520,9 @@ void free_net(Pnet *p) {
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     for (void *q : p->owned) (void)hipFree(q);
-    if (p->pics) (void)hipFree(p->pics);
+    if (p->fpics) (void)hipFree(p->fpics);
+    if (p->mplanes) (void)hipFree(p->mplanes);
+    if (p->means) (void)hipFree(p->means);
     if (p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
 }
@@ -640,6 +661,20 @@ rgh_status rgh_pnet_get_params(void *net, float *params, int64_t n) {
     return RGH_OK;
 }
 
+// Grows a device buffer to at least `bytes`.
+static hipError_t grow(void **buf, size_t *cap, size_t bytes) {
+    if (*cap >= bytes) return hipSuccess;
+    if (*buf) {
+        hipError_t e = hipFree(*buf);
+        if (e != hipSuccess) return e;
+    }
+    *buf = nullptr;
+    *cap = 0;
+    hipError_t e = hipMalloc(buf, bytes);
+    if (e == hipSuccess) *cap = bytes;
+    return e;
+}
+
 rgh_status rgh_pnet_features_batch(void *net, const rgh_pnet_picture *pics, int32_t n) {
     Pnet *p = (Pnet *)net;
     if (!p || !pics || n < 0 || n > 65535) FAIL(RGH_ERR_INVALID_ARGUMENT, "null pointer or bad picture count");
@@ -655,15 +690,44 @@ rgh_status rgh_pnet_features_batch(void *net, const rgh_pnet_picture *pics, int3
     }
     if (n == 0) return RGH_OK;
     TRY(hipSetDevice(p->device));
-    if (p->pics_cap < n) {
-        if (p->pics) TRY(hipFree(p->pics));
-        p->pics = nullptr;
-        p->pics_cap = 0;
-        TRY(hipMalloc(&p->pics, sizeof(rgh_pnet_picture) * (size_t)n));
-        p->pics_cap = n;
+    // One mean image per distinct (plane, stride, size).
+    std::vector<MeanPlane> planes;
+    std::vector<size_t> offs;
+    size_t total = 0;
+    int max_cells = 0;
+    auto plane_of = [&](const uint8_t *y, int stride, int W, int H) {
+        for (size_t k = 0; k < planes.size(); k++)
+            if (planes[k].y == y && planes[k].stride == stride && planes[k].W == W && planes[k].H == H) return (int)k;
+        // Cells -6 .. 4 * mb + 6 in each direction: every macroblock's window and cells.
+        MeanPlane m{y, nullptr, stride, W, H, 4 * ((W + 15) / 16) + 2 * MPAD, 4 * ((H + 15) / 16) + 2 * MPAD};
+        planes.push_back(m);
+        offs.push_back(total);
+        total += (size_t)m.mw * m.mh;
+        max_cells = std::max(max_cells, m.mw * m.mh);
+        return (int)planes.size() - 1;
+    };
+    std::vector<std::pair<int, int>> pair_of(n);
+    for (int i = 0; i < n; i++)
+        pair_of[i] = {plane_of(pics[i].cur_y, pics[i].cur_stride, pics[i].width, pics[i].height),
+                      plane_of(pics[i].prev_y, pics[i].prev_stride, pics[i].width, pics[i].height)};
+    if (planes.size() > 65535) FAIL(RGH_ERR_INVALID_ARGUMENT, "too many distinct planes");
+    TRY(grow((void **)&p->means, &p->means_cap, total));
+    for (size_t k = 0; k < planes.size(); k++) planes[k].m = p->means + offs[k];
+    std::vector<FeatPic> fp(n);
+    for (int i = 0; i < n; i++) {
+        const MeanPlane &c = planes[(size_t)pair_of[i].first], &q = planes[(size_t)pair_of[i].second];
+        fp[i] = FeatPic{pics[i].cur_y, c.m, q.m, pics[i].features, pics[i].cur_stride, pics[i].width, pics[i].height,
+                        c.mw, pics[i].qp};
     }
-    TRY(hipMemcpyAsync(p->pics, pics, sizeof(rgh_pnet_picture) * (size_t)n, hipMemcpyHostToDevice, p->stream));
-    hipLaunchKernelGGL(k_features, dim3(max_mbs, n), dim3(64), 0, p->stream, (const rgh_pnet_picture *)p->pics);
+    TRY(grow(&p->mplanes, &p->mplanes_cap, sizeof(MeanPlane) * planes.size()));
+    TRY(grow(&p->fpics, &p->fpics_cap, sizeof(FeatPic) * (size_t)n));
+    TRY(hipMemcpyAsync(p->mplanes, planes.data(), sizeof(MeanPlane) * planes.size(), hipMemcpyHostToDevice,
+                       p->stream));
+    TRY(hipMemcpyAsync(p->fpics, fp.data(), sizeof(FeatPic) * (size_t)n, hipMemcpyHostToDevice, p->stream));
+    hipLaunchKernelGGL(k_means, dim3((max_cells + 255) / 256, (unsigned)planes.size()), dim3(256), 0, p->stream,
+                       (const MeanPlane *)p->mplanes);
+    TRY(hipGetLastError());
+    hipLaunchKernelGGL(k_features, dim3(max_mbs, n), dim3(64), 0, p->stream, (const FeatPic *)p->fpics);
     TRY(hipGetLastError());
     TRY(hipStreamSynchronize(p->stream));
     return RGH_OK;

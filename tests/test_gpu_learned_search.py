@@ -1,12 +1,12 @@
 """The learned-proposal search on the GPU (rgh_analyse_frames_learned) with the shipped network.
 
 What the network gains over the same search without it: the learned form seeds the integer
-search with the predictor and four proposed cells and stops the diamond early; with proposals
-that carry no information (every cell the zero vector, uniform mode probabilities) the same
-search falls back to the plain search far more often and, where it does not, ends at higher
-motion costs. The trained network must beat that on held-out content (seeds the training never
-saw) on both counts, and its rate at each QP must stay within 3% of the plain search's at no
-worse PSNR (the benchmark's BD-rate sample measures the full curve).
+search with the predictor and four proposed cells and stops the diamond early. Fed proposals
+that carry no information (every cell the zero vector, uniform mode probabilities), the same
+search ends at higher motion costs and spends more bits for the same PSNR. On held-out content
+(seeds the training never saw), coded at QP 22/27/32/37, the trained network must beat that
+by at least 3% BD-rate with lower inter costs at every QP, and stay within 1% BD-rate of the
+plain search (the target of the learned form; the benchmark measures it at 1080p).
 """
 import numpy as np
 import pytest
@@ -40,6 +40,9 @@ class NullProposer:
         self.hip.free(self.buf)
 
 
+QPS = (22, 27, 32, 37)
+
+
 @pytest.fixture(scope="module")
 def results(frames):
     from ragdoll_amd.hip import Hip
@@ -51,7 +54,7 @@ def results(frames):
     null = NullProposer(Hip(), n)
     cfg = dict(iters=4, skip_gate=0, intra_gate=0, fallback_pct=105)
     out = {}
-    for qp in (27, 32):
+    for qp in QPS:
         print("coding QP %d" % qp, flush=True)
         out[qp] = {"plain": code_gop(frames, W, H, qp),
                    "learned": code_gop(frames, W, H, qp, proposer=prop, learned=cfg),
@@ -63,6 +66,12 @@ def results(frames):
     null.close()
     net.close()
     return out
+
+
+def bd(results, a, b):
+    from ragdoll_amd.rd import bd_rate
+    return bd_rate([results[q][a]["bytes"] for q in QPS], [results[q][a]["psnr_y"] for q in QPS],
+                   [results[q][b]["bytes"] for q in QPS], [results[q][b]["psnr_y"] for q in QPS])
 
 
 def test_learned_search_is_deterministic(frames):
@@ -78,12 +87,14 @@ def test_learned_search_is_deterministic(frames):
 
 
 def test_trained_network_beats_proposals_without_information(results):
+    gain = bd(results, "null", "learned")
+    print("BD-rate learned vs no-information proposals: %.2f%%" % gain, flush=True)
+    assert gain < -3.0
     for qp, r in results.items():
-        assert r["learned"]["plain_search"] < 0.5 * r["null"]["plain_search"], qp
-        assert r["learned"]["inter_cost_per_mb"] <= r["null"]["inter_cost_per_mb"], qp
+        assert r["learned"]["inter_cost_per_mb"] < r["null"]["inter_cost_per_mb"], qp
 
 
 def test_learned_rate_stays_near_the_plain_search(results):
-    for qp, r in results.items():
-        assert r["learned"]["bytes"] <= 1.03 * r["plain"]["bytes"], qp
-        assert r["learned"]["psnr_y"] >= r["plain"]["psnr_y"] - 0.05, qp
+    loss = bd(results, "plain", "learned")
+    print("BD-rate learned vs plain: %.2f%%" % loss, flush=True)
+    assert loss <= 1.0
