@@ -168,11 +168,13 @@ def main():
                          "compact: the headline streams with the Compact preset (Baseline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal form")
-    ap.add_argument("--learned-iters", type=int, default=4)
-    ap.add_argument("--skip-gate", type=int, default=0)
+    ap.add_argument("--learned-iters", type=int, default=2)
+    ap.add_argument("--skip-gate", type=int, default=26)
     ap.add_argument("--intra-gate", type=int, default=0)
     ap.add_argument("--fallback-pct", type=int, default=105)
-    ap.add_argument("--intra-first", type=int, default=0)
+    ap.add_argument("--intra-first", type=int, default=153)
+    ap.add_argument("--proposal-groups", type=int, default=0,
+                    help="workgroup cap of the proposal launches beside the analysis (0: none)")
     ap.add_argument("--bd-frames", type=int, default=30, help="frames per QP of the BD-rate sample (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -414,6 +416,7 @@ def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value
         e, p = serial_step()
         el_serial += e
         prop += p
+    net.set_launch_limit(args.proposal_groups)
     props = proposers[0].run(pics)
     b = 1
     for _ in range(max(1, args.warmup)):
@@ -424,7 +427,9 @@ def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value
         e, props = pipelined_step(b, props)
         el += e
         b ^= 1
-    # FP8 inference alone (one launch over every P macroblock of the step), for its roofline.
+    # FP8 inference alone (one launch over every P macroblock of the step, the whole device), for
+    # its roofline.
+    net.set_launch_limit(0)
     t = time.perf_counter()
     reps = 3
     for _ in range(reps):
@@ -451,7 +456,7 @@ def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value
             "serial_value": round(total_mbs * args.steps / el_serial, 1),
             "serial_ms_per_step": round(el_serial / args.steps * 1e3, 3),
             "proposal_ms_per_step": round(prop / args.steps * 1e3, 3),
-            "config": cfg, "lag": sched[0], "groups": sched[1],
+            "config": cfg, "lag": sched[0], "groups": sched[1], "proposal_groups": args.proposal_groups,
             "plain_search_pct": round(100.0 * counts[1] / total_mbs, 3),
             "skip_probe_gated_pct": round(100.0 * counts[2] / total_mbs, 3),
             "intra_gated_pct": round(100.0 * counts[3] / total_mbs, 3),

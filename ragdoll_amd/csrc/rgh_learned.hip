@@ -223,59 +223,75 @@ __global__ __launch_bounds__(256) void k_means(const MeanPlane *planes) {
 }
 
 struct FeatPic {
-    const uint8_t *cur;       // luma samples (mean absolute deviation)
+    const uint8_t *cur;           // luma samples (mean absolute deviation)
     const uint8_t *mcur, *mprev;  // mean images of the picture and the previous picture
     uint16_t *features;
     int cs, W, H, mw, qp;
 };
 
-// Features: one wave per macroblock.
-__global__ __launch_bounds__(64) void k_features(const FeatPic *pics) {
-    __shared__ int rc[16 * 16];
-    __shared__ int sc[16];
-    __shared__ int red[64];
-    const FeatPic &P = pics[blockIdx.y];
-    const int mb_w = (P.W + 15) >> 4, mb = blockIdx.x, lane = threadIdx.x;
-    if (mb >= mb_w * ((P.H + 15) >> 4)) return;
-    const int bx = mb % mb_w, by = mb / mb_w, x0 = bx * 16, y0 = by * 16;
-    // The previous picture's cells of the 64x64 window at (x0 - 24, y0 - 24): mean-image cells
-    // (4 bx - 6 + c, 4 by - 6 + r), i.e. indices (4 bx + c, 4 by + r) with the image's offset.
-    for (int c = lane; c < 256; c += 64) rc[c] = P.mprev[(long)(4 * by + (c >> 4)) * P.mw + 4 * bx + (c & 15)];
-    if (lane < 16) sc[lane] = P.mcur[(long)(4 * by + MPAD + (lane >> 2)) * P.mw + 4 * bx + MPAD + (lane & 3)];
-    // The macroblock's samples: four per lane (row lane / 4, columns 4 * (lane % 4) + [0, 4)).
-    const int r = lane >> 2, c4 = (lane & 3) * 4;
-    const uint8_t *row = P.cur + (long)clampi(y0 + r, 0, P.H - 1) * P.cs;
-    int px[4], sum = 0;
-    for (int i = 0; i < 4; i++) {
-        px[i] = row[clampi(x0 + c4 + i, 0, P.W - 1)];
-        sum += px[i];
+// Features: one wave per macroblock, four waves per workgroup, over the batch's (picture,
+// macroblock) items (item = picture * max_mbs + macroblock). The loop runs the same number of
+// rounds in every wave of a workgroup (its barriers are workgroup-wide); a wave whose item is
+// past the end or past its picture's macroblocks only joins the barriers.
+__device__ __forceinline__ int wave_total(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_features(const FeatPic *pics, int max_mbs, long items) {
+    __shared__ uint32_t rc_all[4][16 * 5];  // window cells, 16 bytes per row + a pad dword
+    __shared__ uint32_t sc_all[4][4];       // the macroblock's cells, one row of 4 per dword
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    uint32_t *rc = rc_all[wave], *sc = sc_all[wave];
+    for (long base = (long)blockIdx.x * 4; base < items; base += (long)gridDim.x * 4) {
+        const long it = base + wave;
+        const FeatPic *P = it < items ? &pics[it / max_mbs] : nullptr;
+        const int mb = (int)(it % max_mbs), mb_w = P ? (P->W + 15) >> 4 : 0;
+        const bool valid = P && mb < mb_w * ((P->H + 15) >> 4);
+        if (valid) {
+            const int bx = mb % mb_w, by = mb / mb_w, x0 = bx * 16, y0 = by * 16;
+            // The previous picture's window cells (4 bx - 6 + c, 4 by - 6 + r) are cells
+            // (4 bx + c, 4 by + r) of the offset mean image (rows of 4 * mb_w + 12 cells, so
+            // every dword below is aligned): lane = row * 4 + dword.
+            rc[(lane >> 2) * 5 + (lane & 3)] =
+                *(const uint32_t *)(P->mprev + (long)(4 * by + (lane >> 2)) * P->mw + 4 * bx + 4 * (lane & 3));
+            if (lane < 16) rc[lane * 5 + 4] = 0;
+            if (lane < 4) {  // column 4 bx + 6 is not dword-aligned: bytes
+                const uint8_t *q = P->mcur + (long)(4 * by + MPAD + lane) * P->mw + 4 * bx + MPAD;
+                sc[lane] = (uint32_t)q[0] | ((uint32_t)q[1] << 8) | ((uint32_t)q[2] << 16) | ((uint32_t)q[3] << 24);
+            }
+            // The macroblock's samples: four per lane (row lane / 4, columns 4 * (lane % 4) + [0, 4)).
+            const uint8_t *row = P->cur + (long)clampi(y0 + (lane >> 2), 0, P->H - 1) * P->cs;
+            int px[4];
+            for (int i = 0; i < 4; i++) px[i] = row[clampi(x0 + (lane & 3) * 4 + i, 0, P->W - 1)];
+            const int mean = (wave_total(px[0] + px[1] + px[2] + px[3]) + 128) >> 8;
+            const int mad = (wave_total(abs(px[0] - mean) + abs(px[1] - mean) + abs(px[2] - mean) + abs(px[3] - mean)) +
+                             128) >> 8;
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            uint16_t *f = P->features + (long)mb * F_IN;
+            const uint32_t s0 = sc[0], s1 = sc[1], s2 = sc[2], s3 = sc[3];
+            for (int e = lane; e < NCLS; e += 64) {
+                const int dy = e / 13, dx = e % 13;  // cells displaced by (dx - 6, dy - 6)
+                const int d = dx >> 2, sh = 8 * (dx & 3);
+                const uint32_t *rw = rc + dy * 5 + d;
+                uint32_t acc = 0;
+                acc = __builtin_amdgcn_sad_u8(s0, (uint32_t)((((uint64_t)rw[1] << 32) | rw[0]) >> sh), acc);
+                acc = __builtin_amdgcn_sad_u8(s1, (uint32_t)((((uint64_t)rw[6] << 32) | rw[5]) >> sh), acc);
+                acc = __builtin_amdgcn_sad_u8(s2, (uint32_t)((((uint64_t)rw[11] << 32) | rw[10]) >> sh), acc);
+                acc = __builtin_amdgcn_sad_u8(s3, (uint32_t)((((uint64_t)rw[16] << 32) | rw[15]) >> sh), acc);
+                f[e] = f2bf((float)min((int)acc, 2047) * (1.0f / 256.0f));
+            }
+            if (lane < 16) f[NCLS + lane] = f2bf((float)((sc[lane >> 2] >> (8 * (lane & 3))) & 0xFF) * (1.0f / 64.0f));
+            if (lane == 16) f[NCLS + 16] = f2bf((float)mad * (1.0f / 16.0f));
+            if (lane == 17) f[NCLS + 17] = f2bf((float)P->qp * (1.0f / 32.0f));
+            if (lane >= 18 && lane < F_IN - NCLS) f[NCLS + lane] = 0;
+        }
+        // The wave reuses its LDS rows next round: every lane's reads end before the next writes.
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
-    red[lane] = sum;
-    __syncthreads();
-    int total = 0;
-    for (int i = 0; i < 64; i++) total += red[i];
-    const int mean = (total + 128) >> 8;
-    int dev = 0;
-    for (int i = 0; i < 4; i++) dev += abs(px[i] - mean);
-    __syncthreads();
-    red[lane] = dev;
-    __syncthreads();
-    uint16_t *f = P.features + (long)mb * F_IN;
-    for (int e = lane; e < NCLS; e += 64) {
-        const int dy = e / 13, dx = e % 13;  // cells displaced by (dx - 6, dy - 6)
-        int s = 0;
-        for (int j = 0; j < 4; j++)
-            for (int i = 0; i < 4; i++) s += abs(sc[j * 4 + i] - rc[(dy + j) * 16 + dx + i]);
-        f[e] = f2bf((float)min(s, 2047) * (1.0f / 256.0f));
-    }
-    if (lane < 16) f[NCLS + lane] = f2bf((float)sc[lane] * (1.0f / 64.0f));
-    if (lane == 16) {
-        int t = 0;
-        for (int i = 0; i < 64; i++) t += red[i];
-        f[NCLS + 16] = f2bf((float)((t + 128) >> 8) * (1.0f / 16.0f));
-    }
-    if (lane == 17) f[NCLS + 17] = f2bf((float)P.qp * (1.0f / 32.0f));
-    if (lane >= 18 && lane < F_IN - NCLS) f[NCLS + lane] = 0;
 }
 
 __global__ void k_labels(const rgh_mb_decision *d, int n, rgh_pnet_label *l) {
@@ -309,8 +325,14 @@ __global__ void k_fp8_encode(const float *x, int n, uint8_t *o) {
 }
 
 // ---------------------------------------------------------------------------------------
-// FP8 inference. Workgroup: four waves, 16 macroblocks each per 64-macroblock tile.
+// FP8 inference. A workgroup of four waves takes 64 macroblocks at a time; every layer is one
+// 64 x N product in which wave w owns a slice of the output columns for all 64 rows, so each
+// k-step issues 4 x (columns / 16) independent MFMAs from shared A fragments. Weights stay in
+// LDS for the workgroup's lifetime (rows padded by 8 bytes against bank conflicts); the
+// activations of the tile pass between layers through LDS as E4M3, the logits as fp32.
 constexpr int IT = 64;
+constexpr int XS = F_IN + 8, HS = HID + 8, ZS = OUT + 1;
+constexpr int QL_W1 = 0, QL_W2 = QL_W1 + HID * XS, QL_W3 = QL_W2 + HID * HS, QL_BYTES = QL_W3 + OUT * HS;
 struct InferArgs {
     const uint16_t *X;
     int M;
@@ -320,43 +342,60 @@ struct InferArgs {
     rgh_proposal *out;
     float *logits;
 };
-struct InferLds {  // 138 KB: one workgroup per CU
-    uint8_t w[Q_BYTES];
-    uint8_t x[4][16 * F_IN];
-    uint8_t h[4][2][16 * HID];
-    float z[4][16 * OUT];
+struct InferLds {
+    uint8_t w[QL_BYTES];
+    uint8_t x[IT * XS];
+    uint8_t h1[IT * HS];
+    uint8_t h2[IT * HS];
+    float z[IT * ZS];
     float scale[3][OUT];
     float bias[3][OUT];
     float mul[3];
+    float tv[IT][4][4];
+    int ti[IT][4][4];
 };
-static_assert(Q_BYTES % 16 == 0 && (16 * F_IN) % 16 == 0 && (16 * HID) % 16 == 0, "LDS alignment");
+static_assert(sizeof(InferLds) <= 160 * 1024, "LDS budget");
+static_assert(XS % 8 == 0 && HS % 8 == 0 && QL_W2 % 8 == 0 && QL_W3 % 8 == 0, "8-byte operand alignment");
 
-// One wave: rows [16] x columns [nblk * 16] of act(in[16][K] . w[N][K]^T) with the layer's
-// scale/bias/ReLU and coding into out8 (E4M3, [16][N]), or fp32 logits into outf.
-template <int K, int NBLK>
-__device__ __forceinline__ void fp8_layer(const uint8_t *in, const uint8_t *w, const float *scale, const float *bias,
-                                          float mul_next, uint8_t *out8, float *outf, int lane) {
+// Columns [16 nb0, 16 (nb0 + NB)) of act(in[64][K] . w[N][K]^T) for the tile's 64 rows: scale,
+// bias and ReLU, then E4M3 into out8 (row stride OS) or fp32 into outf (row stride ZS).
+template <int K, int IS, int NB, int OS>
+__device__ __forceinline__ void fp8_layer(const uint8_t *in, const uint8_t *w, int nb0, int nbn, const float *scale,
+                                          const float *bias, float mul_next, uint8_t *out8, float *outf, int lane) {
     const int kq = 8 * (lane >> 4), rl = lane & 15;
-#pragma unroll 1
-    for (int nb = 0; nb < NBLK; nb++) {
-        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    f32x4 acc[4][NB];
 #pragma unroll
-        for (int k0 = 0; k0 < K; k0 += 32) {
-            const long a = *(const long *)&in[rl * K + k0 + kq];
-            const long b = *(const long *)&w[(nb * 16 + rl) * K + k0 + kq];
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a, b, acc, 0, 0, 0);
-        }
-        const int n = nb * 16 + rl;
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int j = 0; j < NB; j++) acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        long a[4], b[NB];
+#pragma unroll
+        for (int r = 0; r < 4; r++) a[r] = *(const long *)&in[(r * 16 + rl) * IS + k0 + kq];
+#pragma unroll
+        for (int j = 0; j < NB; j++) b[j] = j < nbn ? *(const long *)&w[((nb0 + j) * 16 + rl) * IS + k0 + kq] : 0L;
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int j = 0; j < NB; j++) acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[r], b[j], acc[r][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        if (j >= nbn) continue;
+        const int n = (nb0 + j) * 16 + rl;
         const float sc = scale[n], bi = bias[n];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int m = 4 * (lane >> 4) + q;
-            const float v = acc[q] * sc + bi;
-            if (out8)
-                out8[m * (NBLK * 16) + n] = (uint8_t)fp8_code(fminf(fmaxf(v, 0.0f) * mul_next, FP8_MAX));
-            else
-                outf[m * OUT + n] = v;
-        }
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int m = r * 16 + 4 * (lane >> 4) + q;
+                const float v = acc[r][j][q] * sc + bi;
+                if (out8)
+                    out8[m * OS + n] = (uint8_t)fp8_code(fminf(fmaxf(v, 0.0f) * mul_next, FP8_MAX));
+                else
+                    outf[m * ZS + n] = v;
+            }
     }
 }
 
@@ -364,7 +403,18 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs g) {
     extern __shared__ __align__(16) uint8_t lds_raw[];
     InferLds &S = *(InferLds *)lds_raw;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    for (int i = tid; i < Q_BYTES / 16; i += 256) ((uint4 *)S.w)[i] = ((const uint4 *)g.w8)[i];
+    // Weights into padded rows: 8-byte chunks.
+    {
+        const int rows[3] = {HID, HID, OUT}, kin[3] = {F_IN, HID, HID}, src[3] = {Q_W1, Q_W2, Q_W3},
+                  dst[3] = {QL_W1, QL_W2, QL_W3}, str[3] = {XS, HS, HS};
+        for (int l = 0; l < 3; l++) {
+            const int per = kin[l] / 8, total = rows[l] * per;
+            for (int i = tid; i < total; i += 256) {
+                const int r = i / per, c = (i % per) * 8;
+                *(uint2 *)&S.w[dst[l] + r * str[l] + c] = *(const uint2 *)&g.w8[src[l] + r * kin[l] + c];
+            }
+        }
+    }
     for (int i = tid; i < 3 * OUT; i += 256) {
         const int l = i / OUT, n = i % OUT;
         S.scale[l][n] = g.qs[i];
@@ -374,13 +424,11 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs g) {
     if (tid < 3) S.mul[tid] = g.qs[3 * OUT + tid];
     __syncthreads();
     const int ntiles = (g.M + IT - 1) / IT;
-    uint8_t *x8 = S.x[wave], *h1 = S.h[wave][0], *h2 = S.h[wave][1];
-    float *z = S.z[wave];
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        const int r0 = t * IT + wave * 16;
-        // Inputs: 16 rows x 192 bfloat16 -> E4M3 (24 groups of 8 per row).
+        const int r0 = t * IT;
+        // Inputs: 64 rows x 192 bfloat16 -> E4M3, 8 per item.
         const float mul0 = S.mul[0];
-        for (int e = lane; e < 16 * (F_IN / 8); e += 64) {
+        for (int e = tid; e < IT * (F_IN / 8); e += 256) {
             const int rr = e / (F_IN / 8), c8 = (e % (F_IN / 8)) * 8, row = r0 + rr;
             uint32_t lo = 0, hi = 0;
             if (row < g.M) {
@@ -397,29 +445,32 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs g) {
                         hi |= p << (16 * (j - 2));
                 }
             }
-            *(uint2 *)&x8[rr * F_IN + c8] = make_uint2(lo, hi);
+            *(uint2 *)&S.x[rr * XS + c8] = make_uint2(lo, hi);
         }
         __syncthreads();
-        fp8_layer<F_IN, HID / 16>(x8, S.w + Q_W1, S.scale[0], S.bias[0], S.mul[1], h1, nullptr, lane);
+        fp8_layer<F_IN, XS, 2, HS>(S.x, S.w + QL_W1, 2 * wave, 2, S.scale[0], S.bias[0], S.mul[1], S.h1, nullptr, lane);
         __syncthreads();
-        fp8_layer<HID, HID / 16>(h1, S.w + Q_W2, S.scale[1], S.bias[1], S.mul[2], h2, nullptr, lane);
+        fp8_layer<HID, HS, 2, HS>(S.h1, S.w + QL_W2, 2 * wave, 2, S.scale[1], S.bias[1], S.mul[2], S.h2, nullptr, lane);
         __syncthreads();
-        fp8_layer<HID, OUT / 16>(h2, S.w + Q_W3, S.scale[2], S.bias[2], 0.0f, nullptr, z, lane);
+        // 11 column blocks: waves 0-2 take three, wave 3 two.
+        fp8_layer<HID, HS, 3, HS>(S.h2, S.w + QL_W3, 3 * wave, wave == 3 ? 2 : 3, S.scale[2], S.bias[2], 0.0f, nullptr,
+                                  S.z, lane);
         __syncthreads();
         if (g.logits)
-            for (int e = lane; e < 16 * OUT; e += 64) {
-                const int row = r0 + e / OUT;
-                if (row < g.M) g.logits[(long)row * OUT + e % OUT] = z[e];
+            for (int e = tid; e < IT * OUT; e += 256) {
+                const int rr = e / OUT, row = r0 + rr;
+                if (row < g.M) g.logits[(long)row * OUT + e % OUT] = S.z[rr * ZS + e % OUT];
             }
-        if (lane < 16 && r0 + lane < g.M) {
-            const float *zz = z + lane * OUT;
-            // The four largest displacement logits, earliest index first among equals (an
-            // insertion keeps v0 >= v1 >= v2 >= v3; a later equal logit never moves up).
+        // The four largest displacement logits of each row, earliest index first among equals:
+        // four threads per row scan a quarter each (an insertion keeps v0 >= v1 >= v2 >= v3 and
+        // a later equal logit never moves up), then the first merges the quarters in order.
+        {
+            const int rr = tid >> 2, q = tid & 3;
+            const float *zz = S.z + rr * ZS;
             float v0 = -INFINITY, v1 = -INFINITY, v2 = -INFINITY, v3 = -INFINITY;
-            int i0 = 84, i1 = 84, i2 = 84, i3 = 84;  // the zero cell until four logits are seen
-            for (int i = 0; i < NCLS; i++) {
-                const float v = zz[i];
-                if (!(v > v3)) continue;
+            int i0 = -1, i1 = -1, i2 = -1, i3 = -1;
+            auto insert = [&](float v, int i) {
+                if (!(v > v3)) return;
                 v3 = v;
                 i3 = i;
                 if (v3 > v2) {
@@ -431,23 +482,36 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs g) {
                         }
                     }
                 }
+            };
+            const int lo = q * 43, hi = min(NCLS, lo + 43);
+            for (int i = lo; i < hi; i++) insert(zz[i], i);
+            S.tv[rr][q][0] = v0; S.tv[rr][q][1] = v1; S.tv[rr][q][2] = v2; S.tv[rr][q][3] = v3;
+            S.ti[rr][q][0] = i0; S.ti[rr][q][1] = i1; S.ti[rr][q][2] = i2; S.ti[rr][q][3] = i3;
+            __syncthreads();
+            if (q == 0 && r0 + rr < g.M) {
+                v0 = v1 = v2 = v3 = -INFINITY;
+                i0 = i1 = i2 = i3 = -1;
+                for (int qq = 0; qq < 4; qq++)
+                    for (int k = 0; k < 4; k++)
+                        if (S.ti[rr][qq][k] >= 0) insert(S.tv[rr][qq][k], S.ti[rr][qq][k]);
+                const int c0 = i0 < 0 ? 84 : i0, c1 = i1 < 0 ? 84 : i1, c2 = i2 < 0 ? 84 : i2, c3 = i3 < 0 ? 84 : i3;
+                rgh_proposal o;
+                o.cell[0][0] = (int8_t)(c0 % 13 - 6);
+                o.cell[0][1] = (int8_t)(c0 / 13 - 6);
+                o.cell[1][0] = (int8_t)(c1 % 13 - 6);
+                o.cell[1][1] = (int8_t)(c1 / 13 - 6);
+                o.cell[2][0] = (int8_t)(c2 % 13 - 6);
+                o.cell[2][1] = (int8_t)(c2 / 13 - 6);
+                o.cell[3][0] = (int8_t)(c3 % 13 - 6);
+                o.cell[3][1] = (int8_t)(c3 / 13 - 6);
+                const float m0 = zz[MODE0], m1 = zz[MODE0 + 1], m2 = zz[MODE0 + 2], mx = fmaxf(m0, fmaxf(m1, m2));
+                const float e0 = expf(m0 - mx), e1 = expf(m1 - mx), e2 = expf(m2 - mx), inv = 255.0f / (e0 + e1 + e2);
+                o.mode_prob[0] = (uint8_t)(e0 * inv + 0.5f);
+                o.mode_prob[1] = (uint8_t)(e1 * inv + 0.5f);
+                o.mode_prob[2] = (uint8_t)(e2 * inv + 0.5f);
+                for (int k = 0; k < 5; k++) o.reserved[k] = 0;
+                g.out[r0 + rr] = o;
             }
-            rgh_proposal o;
-            o.cell[0][0] = (int8_t)(i0 % 13 - 6);
-            o.cell[0][1] = (int8_t)(i0 / 13 - 6);
-            o.cell[1][0] = (int8_t)(i1 % 13 - 6);
-            o.cell[1][1] = (int8_t)(i1 / 13 - 6);
-            o.cell[2][0] = (int8_t)(i2 % 13 - 6);
-            o.cell[2][1] = (int8_t)(i2 / 13 - 6);
-            o.cell[3][0] = (int8_t)(i3 % 13 - 6);
-            o.cell[3][1] = (int8_t)(i3 / 13 - 6);
-            const float m0 = zz[MODE0], m1 = zz[MODE0 + 1], m2 = zz[MODE0 + 2], mx = fmaxf(m0, fmaxf(m1, m2));
-            const float e0 = expf(m0 - mx), e1 = expf(m1 - mx), e2 = expf(m2 - mx), inv = 255.0f / (e0 + e1 + e2);
-            o.mode_prob[0] = (uint8_t)(e0 * inv + 0.5f);
-            o.mode_prob[1] = (uint8_t)(e1 * inv + 0.5f);
-            o.mode_prob[2] = (uint8_t)(e2 * inv + 0.5f);
-            for (int k = 0; k < 5; k++) o.reserved[k] = 0;
-            g.out[r0 + lane] = o;
         }
         __syncthreads();
     }
@@ -492,6 +556,7 @@ uint8_t host_fp8(float x) {
 
 struct Pnet {
     int device = 0, cap = 0, step = 0, quantized = 0;
+    int launch_limit = 0;  // most workgroups a features or inference launch uses (0: no limit)
     // Features batch state: device copies of the descriptors and the mean images.
     void *fpics = nullptr, *mplanes = nullptr;
     uint8_t *means = nullptr;
@@ -636,6 +701,13 @@ rgh_status rgh_pnet_destroy(void *net) {
     return RGH_OK;
 }
 
+rgh_status rgh_pnet_set_launch_limit(void *net, int32_t max_workgroups) {
+    Pnet *p = (Pnet *)net;
+    if (!p || max_workgroups < 0) FAIL(RGH_ERR_INVALID_ARGUMENT, "null net or negative limit");
+    p->launch_limit = max_workgroups;
+    return RGH_OK;
+}
+
 rgh_status rgh_pnet_set_params(void *net, const float *params, int64_t n) {
     Pnet *p = (Pnet *)net;
     if (!p || !params || n != NPARAM) FAIL(RGH_ERR_INVALID_ARGUMENT, "expected RGH_PNET_PARAMS parameters");
@@ -727,7 +799,11 @@ rgh_status rgh_pnet_features_batch(void *net, const rgh_pnet_picture *pics, int3
     hipLaunchKernelGGL(k_means, dim3((max_cells + 255) / 256, (unsigned)planes.size()), dim3(256), 0, p->stream,
                        (const MeanPlane *)p->mplanes);
     TRY(hipGetLastError());
-    hipLaunchKernelGGL(k_features, dim3(max_mbs, n), dim3(64), 0, p->stream, (const FeatPic *)p->fpics);
+    const long items = (long)n * max_mbs;
+    long groups = (items + 3) / 4;
+    if (p->launch_limit > 0) groups = std::min<long>(groups, p->launch_limit);
+    hipLaunchKernelGGL(k_features, dim3((unsigned)std::min<long>(groups, 1L << 30)), dim3(256), 0, p->stream,
+                       (const FeatPic *)p->fpics, max_mbs, items);
     TRY(hipGetLastError());
     TRY(hipStreamSynchronize(p->stream));
     return RGH_OK;
@@ -871,7 +947,8 @@ rgh_status rgh_pnet_infer(void *net, const uint16_t *X, int32_t n, rgh_proposal 
     TRY(hipSetDevice(p->device));
     int cus = 0;
     TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
-    const int tiles = (n + IT - 1) / IT, grid = std::max(1, std::min(tiles, 2 * cus));
+    int grid = std::max(1, std::min((n + IT - 1) / IT, cus));
+    if (p->launch_limit > 0) grid = std::min(grid, p->launch_limit);
     InferArgs g{X, (int)n, p->w8, p->qs, p->params, out, logits};
     TRY(hipFuncSetAttribute((const void *)k_infer, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sizeof(InferLds)));
     hipLaunchKernelGGL(k_infer, dim3(grid), dim3(256), sizeof(InferLds), p->stream, g);

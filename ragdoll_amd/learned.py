@@ -31,7 +31,7 @@ LABEL_DTYPE = np.dtype([("disp", "<i2"), ("mode", "<i2")])
 PROPOSAL_DTYPE = np.dtype([("cell", "i1", (4, 2)), ("mode_prob", "u1", (3,)), ("reserved", "u1", (5,))])
 assert PROPOSAL_DTYPE.itemsize == 16
 
-EXPORTS = ["rgh_pnet_create", "rgh_pnet_destroy", "rgh_pnet_set_params", "rgh_pnet_get_params",
+EXPORTS = ["rgh_pnet_create", "rgh_pnet_destroy", "rgh_pnet_set_launch_limit", "rgh_pnet_set_params", "rgh_pnet_get_params",
            "rgh_pnet_features", "rgh_pnet_features_batch", "rgh_pnet_labels", "rgh_pnet_gather", "rgh_pnet_train_step",
            "rgh_pnet_gradients", "rgh_pnet_forward", "rgh_pnet_quantize", "rgh_pnet_quantize_ranges", "rgh_pnet_infer",
            "rgh_pnet_get_quantized", "rgh_pnet_fp8_encode", "rgh_pnet_gemm_bf16", "rgh_analyse_frames_learned"]
@@ -54,6 +54,7 @@ def _bind():
     sig = {
         "rgh_pnet_create": [I32, I32, ctypes.POINTER(ctypes.c_void_p)],
         "rgh_pnet_destroy": [P],
+        "rgh_pnet_set_launch_limit": [P, I32],
         "rgh_pnet_set_params": [P, P, I64],
         "rgh_pnet_get_params": [P, P, I64],
         "rgh_pnet_features": [P, P, I32, P, I32, I32, I32, I32, P],
@@ -110,6 +111,9 @@ class ProposalNet:
 
     def __del__(self):
         self.close()
+
+    def set_launch_limit(self, max_workgroups):
+        _check(self.L.rgh_pnet_set_launch_limit(self.h, max_workgroups), "rgh_pnet_set_launch_limit")
 
     def set_params(self, params):
         p = np.ascontiguousarray(params, np.float32)
