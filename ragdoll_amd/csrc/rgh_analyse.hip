@@ -1395,6 +1395,7 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
 
         // Sub-pel refinement unless the full-pel residual already quantizes to nothing
         // (enc.rs:3097-3117; enc.rs:1283 sub_pel: half-pel, then quarter-pel diamonds).
+        me_cost = best_cost;
         const int fbx = x0 + bestx, fby = y0 + besty;
         int rox = ox, roy = oy;
         if (fbx - 3 < rox || fbx + 29 > rox + SR_COLS || fby - 3 < roy || fby + 20 > roy + SR_H) {
