@@ -1707,7 +1707,12 @@ __device__ __forceinline__ int mode_at(uint64_t packed, int bx, int by) {
 // Value of row r (0..3) of the lane's quad (DPP quad broadcast).
 template <int R>
 __device__ __forceinline__ int quad_row(int v) {
-    return __builtin_amdgcn_update_dpp(0, v, R * 0x55, 0xF, 0xF, false);
+    // The empty asm keeps the broadcast a v_mov_b32_dpp of its own: folded into the arithmetic
+    // that consumes it (v_add_u32_dpp / v_subrev_u32_dpp with quad_perm:[R,R,R,R]), the result
+    // was wrong on MI355X (the intra 4x4 transforms lost parity with the oracle).
+    int r = __builtin_amdgcn_update_dpp(0, v, R * 0x55, 0xF, 0xF, false);
+    __asm__ volatile("" : "+v"(r));
+    return r;
 }
 
 // Canvas offset, relative to the block's top-left edge sample (row py, column px), of edge-line
@@ -1732,6 +1737,12 @@ __device__ __forceinline__ int g4_offset(int i, bool tr) {
 // levels are kept and its reconstruction written to the canvas before the next block. Returns
 // false as soon as the running cost (from 24 lambda) reaches budget. lmodes / tmodes: the left
 // and top macroblocks' packed modes (all DC when they are not I_4x4).
+// The y-th of four values (y in 0..3, per lane) by masks: a ternary chain on a lane-varying
+// index compiles to divergent branches.
+__device__ __forceinline__ int sel4(int y, int a, int b, int c, int d) {
+    return (a & -(int)(y == 0)) | (b & -(int)(y == 1)) | (c & -(int)(y == 2)) | (d & -(int)(y == 3));
+}
+
 __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint32_t lambda, uint32_t budget,
                            uint64_t lmodes, uint64_t tmodes, uint64_t &modes, int &cbpl) {
     const int lane = threadIdx.x, m = lane >> 2, y = lane & 3;
@@ -1786,7 +1797,10 @@ __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint3
                 const uint32_t w = tr ? off_tr[slot >> 2] : off_nt[slot >> 2];
                 g[j] = cb[(w >> (8 * (slot & 3))) & 0xFF];
             }
-            const int v = k == 0 ? g[1] : (k == 1 ? (g[1] + g[2] + 1) >> 1 : (k == 2 ? (g[0] + 2 * g[1] + g[2] + 2) >> 2 : dc));
+            // Every lane computes the three predictor forms and picks its own with masks: a
+            // ternary chain here compiles to divergent branches around the loads.
+            const int f1 = (g[1] + g[2] + 1) >> 1, f2 = (g[0] + 2 * g[1] + g[2] + 2) >> 2;
+            const int v = (g[1] & -(int)(k == 0)) | (f1 & -(int)(k == 1)) | (f2 & -(int)(k == 2)) | (dc & -(int)(k > 2));
             p[x] = v;
             d[x] = (int)byte_of(srow, x) - v;
         }
@@ -1818,7 +1832,7 @@ __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint3
             for (int c = 0; c < 4; c++) {
                 const int v0 = quad_row<0>(t[c]), v1 = quad_row<1>(t[c]), v2 = quad_row<2>(t[c]), v3 = quad_row<3>(t[c]);
                 const int cs03 = v0 + v3, cs12 = v1 + v2, cd03 = v0 - v3, cd12 = v1 - v2;
-                const int o = y == 0 ? cs03 + cs12 : (y == 1 ? (cd03 << 1) + cd12 : (y == 2 ? cs03 - cs12 : cd03 - (cd12 << 1)));
+                const int o = sel4(y, cs03 + cs12, (cd03 << 1) + cd12, cs03 - cs12, cd03 - (cd12 << 1));
                 const int cv = (int16_t)o;
                 uint32_t q = ((uint32_t)iabs(cv) * ((c & 1) ? mfo : mfe) + qbias) >> qshift;
                 q = q > 32767u ? 32767u : q;
@@ -1838,7 +1852,7 @@ __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint3
             for (int k = 0; k < 4; k++) {
                 const int f0 = quad_row<0>(t[k]), f1 = quad_row<1>(t[k]), f2 = quad_row<2>(t[k]), f3 = quad_row<3>(t[k]);
                 const int g0 = f0 + f2, g1 = f0 - f2, g2 = (f1 >> 1) - f3, g3 = f1 + (f3 >> 1);
-                const int o = y == 0 ? g0 + g3 : (y == 1 ? g1 + g2 : (y == 2 ? g1 - g2 : g0 - g3));
+                const int o = sel4(y, g0 + g3, g1 + g2, g1 - g2, g0 - g3);
                 rec |= (uint32_t)clip255(p[k] + (int16_t)((o + 32) >> 6)) << (8 * k);
             }
         }
