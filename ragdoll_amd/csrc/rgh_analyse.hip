@@ -1364,16 +1364,27 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
             return __builtin_amdgcn_sad_u8(srcw, w, 0);
         };
         // Small diamond (enc.rs:1207-1272): the first of the cheapest in-bounds points below the
-        // best moves, until none does or the iterations run out.
+        // best moves, until none does or the iterations run out. With at most 16 iterations every
+        // point evaluated lies within 17 samples of the candidate, inside the staged region
+        // (columns 2..39 + 16, rows 2..36 + 16): the four SADs then read LDS without a branch, so
+        // their loads issue together; out-of-bounds points are computed and ignored.
+        const bool in_region = J.me_iters <= 16;
         for (int it = 0; it < iters; it++) {
             const int px[4] = {bestx, bestx - 1, bestx + 1, bestx};
             const int py[4] = {besty - 1, besty, besty, besty + 1};
             int ok[4];
             uint32_t part[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                ok[k] = px[k] >= imn0 && px[k] <= imx0 && py[k] >= imn1 && py[k] <= imx1;
-                part[k] = ok[k] ? sad_part(px[k], py[k]) : 0u;
+            for (int k = 0; k < 4; k++) ok[k] = px[k] >= imn0 && px[k] <= imx0 && py[k] >= imn1 && py[k] <= imx1;
+            if (in_region) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int lx = x0 + px[k] - ox, ly = y0 + py[k] - oy;
+                    part[k] = __builtin_amdgcn_sad_u8(srcw, lds4u(&S.sr[(ly + rr) * SR_W], lx + c0), 0);
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++) part[k] = ok[k] ? sad_part(px[k], py[k]) : 0u;
             }
             uint32_t sums[4];
             wave_sum4(part, sums);
@@ -1710,7 +1721,7 @@ __device__ __forceinline__ int quad_row(int v) {
     // The empty asm keeps the broadcast a v_mov_b32_dpp of its own: folded into the arithmetic
     // that consumes it (v_add_u32_dpp / v_subrev_u32_dpp with quad_perm:[R,R,R,R]), the result
     // was wrong on MI355X (the intra 4x4 transforms lost parity with the oracle).
-    int r = __builtin_amdgcn_update_dpp(0, v, R * 0x55, 0xF, 0xF, false);
+    int r = __builtin_amdgcn_mov_dpp(v, R * 0x55, 0xF, 0xF, true);  // every source lane is valid
     __asm__ volatile("" : "+v"(r));
     return r;
 }
