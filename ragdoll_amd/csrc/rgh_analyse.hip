@@ -67,7 +67,9 @@ struct MbLds {
 };
 
 __device__ __forceinline__ uint32_t se_bits(int d) {  // enc.rs:546
-    uint32_t v = (uint32_t)iabs(d) * 2 - (uint32_t)(d > 0);
+    // se(v) code number 2|d| - (d > 0), as a select (the form `- (d > 0)` costs a VALU round trip
+    // when d is uniform).
+    const uint32_t v = d > 0 ? (uint32_t)(2 * d - 1) : (uint32_t)(-2 * d);
     return 2u * (uint32_t)(32 - __clz((int)(v + 1))) - 1u;
 }
 
@@ -1426,6 +1428,8 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
             // quarter samples of the centre: all 49 SADs at once, then the diamonds' sequential
             // order over them (costs, bounds and strict-< ties as enc.rs:1283 sub_pel).
             const uint32_t sads = subpel_sads(x0, y0, fbx, fby, rox, roy, cx, cy, S);
+            // Lane q < 49 holds the cost of offset (q % 7 - 3, q / 7 - 3) quarter samples.
+            const uint32_t costs = sads + mv_cost(cx + lane % 7 - 3, cy + lane / 7 - 3);
             int bmx = cx, bmy = cy;
             for (int pass = 0; pass < 2; pass++) {
                 const int step = pass == 0 ? 2 : 1, iters = pass == 0 ? 1 : J.qpel_iters;
@@ -1437,7 +1441,7 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
                         const int mx = fx + dx[k], my = fy + dy[k];
                         if (iabs(mx - cx) > 3 || iabs(my - cy) > 3 || !in_bounds(mx, my)) continue;
                         const int qq = (my - cy + 3) * 7 + (mx - cx + 3);
-                        const uint32_t cst = (uint32_t)__builtin_amdgcn_readlane((int)sads, qq) + mv_cost(mx, my);
+                        const uint32_t cst = (uint32_t)__builtin_amdgcn_readlane((int)costs, qq);
                         D.n_subpel++;
                         if (cst < me_cost) {
                             me_cost = cst;
