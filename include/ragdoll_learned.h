@@ -63,9 +63,8 @@ typedef struct rgh_proposal {
  * rgh_pnet_train_step / rgh_pnet_gradients. */
 rgh_status rgh_pnet_create(int32_t device, int32_t max_batch, void **net);
 rgh_status rgh_pnet_destroy(void *net);
-/* Caps the workgroups of the features and inference launches (0: one per CU for inference, one
- * per four macroblocks for features), so that proposals computed on the net's stream beside a
- * running analysis leave it most of the device. */
+/* Caps the workgroups of the inference launch (0: one per CU). Each holds 151 KB of LDS, so a
+ * cap leaves the other CUs to an analysis running beside the proposals on another stream. */
 rgh_status rgh_pnet_set_launch_limit(void *net, int32_t max_workgroups);
 /* Parameters in the flat fp32 layout above (host memory); setting them resets the Adam state
  * and the quantized copy. */

@@ -233,10 +233,14 @@ struct FeatPic {
 // macroblock) items (item = picture * max_mbs + macroblock). The loop runs the same number of
 // rounds in every wave of a workgroup (its barriers are workgroup-wide); a wave whose item is
 // past the end or past its picture's macroblocks only joins the barriers.
+// Sum over the wave: DPP within rows of 16, then the four row sums by readlane.
 __device__ __forceinline__ int wave_total(int v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+    v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    v += __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    v += __builtin_amdgcn_update_dpp(0, v, 0x140, 0xF, 0xF, false);  // row_mirror
+    return __builtin_amdgcn_readlane(v, 0) + __builtin_amdgcn_readlane(v, 16) + __builtin_amdgcn_readlane(v, 32) +
+           __builtin_amdgcn_readlane(v, 48);
 }
 
 __global__ __launch_bounds__(256) void k_features(const FeatPic *pics, int max_mbs, long items) {
@@ -263,8 +267,14 @@ __global__ __launch_bounds__(256) void k_features(const FeatPic *pics, int max_m
             }
             // The macroblock's samples: four per lane (row lane / 4, columns 4 * (lane % 4) + [0, 4)).
             const uint8_t *row = P->cur + (long)clampi(y0 + (lane >> 2), 0, P->H - 1) * P->cs;
+            const int sx = x0 + (lane & 3) * 4;
             int px[4];
-            for (int i = 0; i < 4; i++) px[i] = row[clampi(x0 + (lane & 3) * 4 + i, 0, P->W - 1)];
+            if (sx + 3 < P->W && (((uintptr_t)(row + sx)) & 3) == 0) {
+                const uint32_t w = *(const uint32_t *)(row + sx);
+                for (int i = 0; i < 4; i++) px[i] = (int)((w >> (8 * i)) & 0xFF);
+            } else {
+                for (int i = 0; i < 4; i++) px[i] = row[min(sx + i, P->W - 1)];
+            }
             const int mean = (wave_total(px[0] + px[1] + px[2] + px[3]) + 128) >> 8;
             const int mad = (wave_total(abs(px[0] - mean) + abs(px[1] - mean) + abs(px[2] - mean) + abs(px[3] - mean)) +
                              128) >> 8;
@@ -276,11 +286,12 @@ __global__ __launch_bounds__(256) void k_features(const FeatPic *pics, int max_m
                 const int dy = e / 13, dx = e % 13;  // cells displaced by (dx - 6, dy - 6)
                 const int d = dx >> 2, sh = 8 * (dx & 3);
                 const uint32_t *rw = rc + dy * 5 + d;
+                const uint32_t b = (uint32_t)sh >> 3;  // byte offset of the window in its dword pair
                 uint32_t acc = 0;
-                acc = __builtin_amdgcn_sad_u8(s0, (uint32_t)((((uint64_t)rw[1] << 32) | rw[0]) >> sh), acc);
-                acc = __builtin_amdgcn_sad_u8(s1, (uint32_t)((((uint64_t)rw[6] << 32) | rw[5]) >> sh), acc);
-                acc = __builtin_amdgcn_sad_u8(s2, (uint32_t)((((uint64_t)rw[11] << 32) | rw[10]) >> sh), acc);
-                acc = __builtin_amdgcn_sad_u8(s3, (uint32_t)((((uint64_t)rw[16] << 32) | rw[15]) >> sh), acc);
+                acc = __builtin_amdgcn_sad_u8(s0, __builtin_amdgcn_alignbyte(rw[1], rw[0], b), acc);
+                acc = __builtin_amdgcn_sad_u8(s1, __builtin_amdgcn_alignbyte(rw[6], rw[5], b), acc);
+                acc = __builtin_amdgcn_sad_u8(s2, __builtin_amdgcn_alignbyte(rw[11], rw[10], b), acc);
+                acc = __builtin_amdgcn_sad_u8(s3, __builtin_amdgcn_alignbyte(rw[16], rw[15], b), acc);
                 f[e] = f2bf((float)min((int)acc, 2047) * (1.0f / 256.0f));
             }
             if (lane < 16) f[NCLS + lane] = f2bf((float)((sc[lane >> 2] >> (8 * (lane & 3))) & 0xFF) * (1.0f / 64.0f));
@@ -800,8 +811,7 @@ rgh_status rgh_pnet_features_batch(void *net, const rgh_pnet_picture *pics, int3
                        (const MeanPlane *)p->mplanes);
     TRY(hipGetLastError());
     const long items = (long)n * max_mbs;
-    long groups = (items + 3) / 4;
-    if (p->launch_limit > 0) groups = std::min<long>(groups, p->launch_limit);
+    const long groups = (items + 3) / 4;
     hipLaunchKernelGGL(k_features, dim3((unsigned)std::min<long>(groups, 1L << 30)), dim3(256), 0, p->stream,
                        (const FeatPic *)p->fpics, max_mbs, items);
     TRY(hipGetLastError());
