@@ -115,6 +115,89 @@ __device__ __forceinline__ uint32_t decimate_mask64(uint64_t mask, int big) {
     return score;
 }
 
+// Value of row r (0..3) of the lane's quad (DPP quad broadcast).
+template <int R>
+__device__ __forceinline__ int quad_row(int v) {
+    // The empty asm keeps the broadcast a v_mov_b32_dpp of its own: folded into the arithmetic
+    // that consumes it (v_add_u32_dpp / v_subrev_u32_dpp with quad_perm:[R,R,R,R]), the result
+    // was wrong on MI355X (the intra 4x4 transforms lost parity with the oracle).
+    int r = __builtin_amdgcn_mov_dpp(v, R * 0x55, 0xF, 0xF, true);  // every source lane is valid
+    __asm__ volatile("" : "+v"(r));
+    return r;
+}
+
+// The y-th of four values (y in 0..3, per lane) by masks: a ternary chain on a lane-varying
+// index compiles to divergent branches.
+__device__ __forceinline__ int sel4(int y, int a, int b, int c, int d) {
+    return (a & -(int)(y == 0)) | (b & -(int)(y == 1)) | (c & -(int)(y == 2)) | (d & -(int)(y == 3));
+}
+
+// ---------------------------------------------------------------------------------------
+// 4x4 transforms, one row per lane: lanes 4k..4k+3 hold rows 0..3 of block k; the column
+// butterflies cross the quad with quad_row broadcasts. Same arithmetic as the block forms below.
+// ---------------------------------------------------------------------------------------
+// Coefficient row `row` of forward_dct4x4 (simd/fixed.rs:104) of src - pred, block at (x, y0).
+__device__ __forceinline__ void fdct4_row(const uint8_t *src, const uint8_t *pred, int s, int x, int y0, int row,
+                                          int *o) {
+    const uint32_t a = *(const uint32_t *)&src[(y0 + row) * s + x], b = *(const uint32_t *)&pred[(y0 + row) * s + x];
+    const int d0 = (int)byte_of(a, 0) - (int)byte_of(b, 0), d1 = (int)byte_of(a, 1) - (int)byte_of(b, 1);
+    const int d2 = (int)byte_of(a, 2) - (int)byte_of(b, 2), d3 = (int)byte_of(a, 3) - (int)byte_of(b, 3);
+    const int s03 = d0 + d3, s12 = d1 + d2, d03 = d0 - d3, d12 = d1 - d2;
+    const int t[4] = {s03 + s12, (d03 << 1) + d12, s03 - s12, d03 - (d12 << 1)};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int v0 = quad_row<0>(t[c]), v1 = quad_row<1>(t[c]), v2 = quad_row<2>(t[c]), v3 = quad_row<3>(t[c]);
+        const int cs03 = v0 + v3, cs12 = v1 + v2, cd03 = v0 - v3, cd12 = v1 - v2;
+        o[c] = (int16_t)sel4(row, cs03 + cs12, (cd03 << 1) + cd12, cs03 - cs12, cd03 - (cd12 << 1));
+    }
+}
+// quant4 on coefficient row `row`: position classes 0 / 2 (even rows), 2 / 1 (odd rows) by column.
+__device__ __forceinline__ void quant4_row(int *o, int row, int qp, int intra) {
+    const int shift = 15 + qp / 6, rem = qp % 6;
+    const uint32_t bias = (1u << shift) / (intra ? 3u : 6u);
+    const uint32_t mfe = (uint32_t)c_quant4_mf[rem][(row & 1) ? 2 : 0], mfo = (uint32_t)c_quant4_mf[rem][(row & 1) ? 1 : 2];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int v = o[c];
+        uint32_t lv = ((uint32_t)iabs(v) * ((c & 1) ? mfo : mfe) + bias) >> shift;
+        lv = lv > 32767u ? 32767u : lv;
+        o[c] = v < 0 ? -(int)lv : (int)lv;
+    }
+}
+// dequant4 on coefficient row `row`.
+__device__ __forceinline__ void dequant4_row(int *c, int row, int qp) {
+    const int sh = qp / 6, rem = qp % 6;
+    const int se = c_dequant4_init[rem][(row & 1) ? 2 : 0] << sh, so = c_dequant4_init[rem][(row & 1) ? 1 : 2] << sh;
+#pragma unroll
+    for (int k = 0; k < 4; k++) c[k] = iclamp(c[k] * ((k & 1) ? so : se), -32768, 32767);
+}
+// Row `row` of inverse_idct4x4 (simd/fixed.rs:185) of the block whose lane holds coefficient
+// row c, added onto row y0 + row of a stride-s LDS block at column x with clipping.
+__device__ __forceinline__ void add_idct4_row(uint8_t *dst, int s, int x, int y0, int row, const int *c) {
+    const int e0 = c[0] + c[2], e1 = c[0] - c[2], e2 = (c[1] >> 1) - c[3], e3 = c[1] + (c[3] >> 1);
+    const int t[4] = {e0 + e3, e1 + e2, e1 - e2, e0 - e3};
+    uint32_t *p = (uint32_t *)&dst[(y0 + row) * s + x];
+    const uint32_t w = *p;
+    uint32_t n = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int f0 = quad_row<0>(t[k]), f1 = quad_row<1>(t[k]), f2 = quad_row<2>(t[k]), f3 = quad_row<3>(t[k]);
+        const int g0 = f0 + f2, g1 = f0 - f2, g2 = (f1 >> 1) - f3, g3 = f1 + (f3 >> 1);
+        const int o = (int16_t)((sel4(row, g0 + g3, g1 + g2, g1 - g2, g0 - g3) + 32) >> 6);
+        n |= (uint32_t)clip255((int)byte_of(w, k) + o) << (8 * k);
+    }
+    *p = n;
+}
+// OR of a value over the lane's quad.
+__device__ __forceinline__ int quad_or(int v) {
+    int a = __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
+    __asm__ volatile("" : "+v"(a));  // kept a v_mov_b32_dpp of its own, as in quad_row
+    v |= a;
+    int b = __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, true);  // quad_perm [2,3,0,1]
+    __asm__ volatile("" : "+v"(b));
+    return v | b;
+}
+
 // ---------------------------------------------------------------------------------------
 // 4x4 transforms and quantization, one block per lane, in registers.
 // ---------------------------------------------------------------------------------------
@@ -500,29 +583,36 @@ __device__ __forceinline__ void chroma_mc_combine(const ChromaTaps &t, int mx, i
     *pv = (uint32_t)((wa * (int)t.v[0] + wb * (int)t.v[1] + wc * (int)t.v[2] + wd * (int)t.v[3] + 32) >> 6);
 }
 
-// Chroma transform + quantization of both planes (enc.rs:1897 quant_chroma), lanes 0..7 one
-// block each (plane lane >> 2, raster 2x2 block lane & 3), lanes 0..1 the 2x2 DC of a plane
-// (enc.rs:1939 quant_chroma_dc). Leaves raster levels, scanned AC masks and decimation scores
-// in LDS. intra selects the rounding of quantize_block4x4.
+// Chroma transform + quantization of both planes (enc.rs:1897 quant_chroma), lanes 0..31 one
+// block row each (plane lane >> 4, raster 2x2 block (lane >> 2) & 3, row lane & 3), lanes 0..1
+// the 2x2 DC of a plane (enc.rs:1939 quant_chroma_dc). Leaves raster levels, scanned AC masks
+// and decimation scores in LDS. intra selects the rounding of quantize_block4x4.
 __device__ void quant_chroma(MbLds &S, int qpc, int intra) {
     const int lane = threadIdx.x;
-    if (lane < 8) {
-        int p = lane >> 2, b = lane & 3;
-        int c[16];
-        sub_fdct4(p ? S.srcv : S.srcu, p ? S.predv : S.predu, 8, (b & 1) * 4, (b >> 1) * 4, c);
-        S.cdc_raw[p][b] = (int16_t)c[0];
-        quant4(c, qpc, intra);
+    if (lane < 32) {
+        const int p = lane >> 4, b = (lane >> 2) & 3, row = lane & 3;
+        int c[4];
+        fdct4_row(p ? S.srcv : S.srcu, p ? S.predv : S.predu, 8, (b & 1) * 4, (b >> 1) * 4, row, c);
+        if (row == 0) S.cdc_raw[p][b] = (int16_t)c[0];
+        quant4_row(c, row, qpc, intra);
+        *(uint2 *)&S.craster[p][b][row * 4] =
+            make_uint2(((uint32_t)c[0] & 0xFFFFu) | ((uint32_t)c[1] << 16), ((uint32_t)c[2] & 0xFFFFu) | ((uint32_t)c[3] << 16));
+        // Scan positions of this row's four coefficients (byte `row` of each word), the DC's
+        // bit (scan position 0) dropped by the final shift.
+        const uint32_t zz[4] = {0x09030200u, 0x0A080401u, 0x0E0B0705u, 0x0F0D0C06u};
         uint32_t m = 0;
         int big = 0;
 #pragma unroll
-        for (int i = 1; i < 16; i++) {
-            m |= (uint32_t)(c[ZZ4[i]] != 0) << (i - 1);
-            big |= iabs(c[ZZ4[i]]) > 1;
+        for (int k = 0; k < 4; k++) {
+            m |= (uint32_t)(c[k] != 0) << ((zz[k] >> (8 * row)) & 15u);
+            big |= (iabs(c[k]) > 1) & (k != 0 || row != 0);
         }
-#pragma unroll
-        for (int i = 0; i < 16; i++) S.craster[p][b][i] = (int16_t)c[i];
-        S.cmask[lane] = (int32_t)m;
-        S.cscore[lane] = (int32_t)decimate_mask(m, big);
+        m = (uint32_t)quad_or((int)m) >> 1;
+        big = quad_or(big);
+        if (row == 0) {
+            S.cmask[lane >> 2] = (int32_t)m;
+            S.cscore[lane >> 2] = (int32_t)decimate_mask(m, big);
+        }
     }
     __syncthreads();
     if (lane < 2) {
@@ -551,14 +641,19 @@ __device__ void recon_chroma(MbLds &S, int cbpc, int qpc) {
         S.cdcr[p][3] = (int16_t)(((a - b - c + d) * qmul) >> 7);
     }
     __syncthreads();
-    if (lane < 8) {
-        int p = lane >> 2, b = lane & 3;
-        int blk[16];
-#pragma unroll
-        for (int i = 0; i < 16; i++) blk[i] = cbpc >= 2 ? S.craster[p][b][i] : 0;
-        if (cbpc >= 2) dequant4(blk, qpc);
-        blk[0] = S.cdcr[p][b];
-        add_idct4(p ? S.predv : S.predu, 8, (b & 1) * 4, (b >> 1) * 4, blk);
+    if (lane < 32) {
+        const int p = lane >> 4, b = (lane >> 2) & 3, row = lane & 3;
+        int blk[4] = {0, 0, 0, 0};
+        if (cbpc >= 2) {
+            const uint2 w = *(const uint2 *)&S.craster[p][b][row * 4];
+            blk[0] = (int16_t)w.x;
+            blk[1] = (int)w.x >> 16;
+            blk[2] = (int16_t)w.y;
+            blk[3] = (int)w.y >> 16;
+            dequant4_row(blk, row, qpc);
+        }
+        if (row == 0) blk[0] = S.cdcr[p][b];
+        add_idct4_row(p ? S.predv : S.predu, 8, (b & 1) * 4, (b >> 1) * 4, row, blk);
     }
     __syncthreads();
 }
@@ -895,82 +990,64 @@ __device__ I16Code code_i16(MbLds &S, int mb_x, int mb_y, int qp, int qpc, const
     S.predv[lane] = (uint8_t)pv;
     __syncthreads();
     // Luma: 4x4 transforms, AC with intra rounding, DC through the 4x4 Hadamard
-    // (enc.rs:3630-3660; quant_luma_dc :1959).
-    int acnz = 0;
-    if (lane < 16) {
-        int c[16];
-        sub_fdct4(S.src, S.pred, 16, blk_x(lane) * 4, blk_y(lane) * 4, c);
-        S.dc_raw[Z2R[lane]] = (int16_t)c[0];
-        quant4(c, qp, 1);
-        c[0] = 0;
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            S.coef[lane][i] = (int16_t)c[i];
-            acnz |= c[i] != 0;
-        }
+    // (enc.rs:3630-3660; quant_luma_dc :1959). One block row per lane: block lane >> 2 (decoding
+    // order), row lane & 3.
+    const int lb = lane >> 2, lrow = lane & 3, lbx = blk_x(lb) * 4, lby = blk_y(lb) * 4;
+    const int lraster = (lby & ~3) + (lbx >> 2);  // raster index of block lb
+    {
+        int c[4];
+        fdct4_row(S.src, S.pred, 16, lbx, lby, lrow, c);
+        if (lrow == 0) S.dc_raw[lraster] = (int16_t)c[0];
+        quant4_row(c, lrow, qp, 1);
+        c[0] &= -(int)(lrow != 0);
+        *(uint2 *)&S.coef[lb][lrow * 4] =
+            make_uint2(((uint32_t)c[0] & 0xFFFFu) | ((uint32_t)c[1] << 16), ((uint32_t)c[2] & 0xFFFFu) | ((uint32_t)c[3] << 16));
+        cbpl = __ballot((c[0] | c[1] | c[2] | c[3]) != 0) ? 15 : 0;
     }
-    cbpl = __ballot(acnz) ? 15 : 0;
     __syncthreads();
-    if (lane == 0) {
+    {
         // simd/fixed.rs:318 hadamard4x4_scalar (with >> 1), then the DC quantizer;
-        // h264.rs:1645 dequantize_idct_luma_dc4x4 for the reconstruction.
-        int t[16], had[16];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const int i = r * 4;
-            const int s03 = S.dc_raw[i] + S.dc_raw[i + 3], s12 = S.dc_raw[i + 1] + S.dc_raw[i + 2];
-            const int d03 = S.dc_raw[i] - S.dc_raw[i + 3], d12 = S.dc_raw[i + 1] - S.dc_raw[i + 2];
-            t[i] = s03 + s12;
-            t[i + 1] = d03 + d12;
-            t[i + 2] = s03 - s12;
-            t[i + 3] = d03 - d12;
-        }
+        // h264.rs:1645 dequantize_idct_luma_dc4x4 for the reconstruction. Row lane & 3 of the
+        // 4x4 DC matrix per lane (every quad computes it; lanes 0..3 store).
+        const uint2 w = *(const uint2 *)&S.dc_raw[lrow * 4];
+        const int r0 = (int16_t)w.x, r1 = (int)w.x >> 16, r2 = (int16_t)w.y, r3 = (int)w.y >> 16;
+        const int s03 = r0 + r3, s12 = r1 + r2, d03 = r0 - r3, d12 = r1 - r2;
+        const int t[4] = {s03 + s12, d03 + d12, s03 - s12, d03 - d12};
+        const int qbits = 16 + qp / 6, mf = c_quant4_mf[qp % 6][0], f = (1 << qbits) / 3;
+        int dq[4];
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            const int s03 = t[c] + t[c + 12], s12 = t[c + 4] + t[c + 8], d03 = t[c] - t[c + 12], d12 = t[c + 4] - t[c + 8];
-            had[c] = (int16_t)((s03 + s12) >> 1);
-            had[c + 4] = (int16_t)((d03 + d12) >> 1);
-            had[c + 8] = (int16_t)((s03 - s12) >> 1);
-            had[c + 12] = (int16_t)((d03 - d12) >> 1);
-        }
-        const int qbits = 16 + qp / 6, mf = c_quant4_mf[qp % 6][0], f = (1 << qbits) / 3;
-        int dq[16];
-#pragma unroll
-        for (int i = 0; i < 16; i++) {
-            const int l = (iabs(had[i]) * mf + f) >> qbits;
-            dq[i] = had[i] < 0 ? -l : l;
-            S.dcq[i] = (int16_t)dq[i];
+            const int v0 = quad_row<0>(t[c]), v1 = quad_row<1>(t[c]), v2 = quad_row<2>(t[c]), v3 = quad_row<3>(t[c]);
+            const int cs03 = v0 + v3, cs12 = v1 + v2, cd03 = v0 - v3, cd12 = v1 - v2;
+            const int had = (int16_t)(sel4(lrow, cs03 + cs12, cd03 + cd12, cs03 - cs12, cd03 - cd12) >> 1);
+            const int l = (iabs(had) * mf + f) >> qbits;
+            dq[c] = had < 0 ? -l : l;
         }
         const int qmul = (c_dequant4_init[qp % 6][0] * 16) << (qp / 6 + 2);
+        const int z0 = dq[0] + dq[1], z1 = dq[0] - dq[1], z2 = dq[2] - dq[3], z3 = dq[2] + dq[3];
+        const int u[4] = {z0 + z3, z0 - z3, z1 - z2, z1 + z2};
+        int dr[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int z0 = dq[4 * i] + dq[4 * i + 1], z1 = dq[4 * i] - dq[4 * i + 1];
-            const int z2 = dq[4 * i + 2] - dq[4 * i + 3], z3 = dq[4 * i + 2] + dq[4 * i + 3];
-            t[4 * i] = z0 + z3;
-            t[4 * i + 1] = z0 - z3;
-            t[4 * i + 2] = z1 - z2;
-            t[4 * i + 3] = z1 + z2;
+        for (int c = 0; c < 4; c++) {
+            const int f0 = quad_row<0>(u[c]), f1 = quad_row<1>(u[c]), f2 = quad_row<2>(u[c]), f3 = quad_row<3>(u[c]);
+            const int y0 = f0 + f2, y1 = f0 - f2, y2 = f1 - f3, y3 = f1 + f3;
+            dr[c] = (int16_t)((sel4(lrow, y0 + y3, y1 + y2, y1 - y2, y0 - y3) * qmul + 128) >> 8);
         }
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int z0 = t[i] + t[8 + i], z1 = t[i] - t[8 + i], z2 = t[4 + i] - t[12 + i], z3 = t[4 + i] + t[12 + i];
-            S.dcr[i] = (int16_t)(((z0 + z3) * qmul + 128) >> 8);
-            S.dcr[4 + i] = (int16_t)(((z1 + z2) * qmul + 128) >> 8);
-            S.dcr[8 + i] = (int16_t)(((z1 - z2) * qmul + 128) >> 8);
-            S.dcr[12 + i] = (int16_t)(((z0 - z3) * qmul + 128) >> 8);
+        if (lane < 4) {
+            *(uint2 *)&S.dcq[lrow * 4] = make_uint2(((uint32_t)dq[0] & 0xFFFFu) | ((uint32_t)dq[1] << 16),
+                                                    ((uint32_t)dq[2] & 0xFFFFu) | ((uint32_t)dq[3] << 16));
+            *(uint2 *)&S.dcr[lrow * 4] = make_uint2(((uint32_t)dr[0] & 0xFFFFu) | ((uint32_t)dr[1] << 16),
+                                                    ((uint32_t)dr[2] & 0xFFFFu) | ((uint32_t)dr[3] << 16));
         }
     }
     quant_chroma(S, qpc, 1);
     cbpc = chroma_cbp(S);
-    if (lane < 16) {
-        int blk[16], any = 0;
-#pragma unroll
-        for (int i = 0; i < 16; i++) blk[i] = S.coef[lane][i];
-        if (cbpl) dequant4(blk, qp);
-        blk[0] = S.dcr[Z2R[lane]];
-#pragma unroll
-        for (int i = 0; i < 16; i++) any |= blk[i];
-        if (any) add_idct4(S.pred, 16, blk_x(lane) * 4, blk_y(lane) * 4, blk);
+    {
+        const uint2 w = *(const uint2 *)&S.coef[lb][lrow * 4];
+        int blk[4] = {(int16_t)w.x, (int)w.x >> 16, (int16_t)w.y, (int)w.y >> 16};
+        if (cbpl) dequant4_row(blk, lrow, qp);
+        if (lrow == 0) blk[0] = S.dcr[lraster];
+        add_idct4_row(S.pred, 16, lbx, lby, lrow, blk);
     }
     __syncthreads();
     recon_chroma(S, cbpc, qpc);
@@ -1719,17 +1796,6 @@ __device__ __forceinline__ int mode_at(uint64_t packed, int bx, int by) {
     return (int)((packed >> (4 * b)) & 15);
 }
 
-// Value of row r (0..3) of the lane's quad (DPP quad broadcast).
-template <int R>
-__device__ __forceinline__ int quad_row(int v) {
-    // The empty asm keeps the broadcast a v_mov_b32_dpp of its own: folded into the arithmetic
-    // that consumes it (v_add_u32_dpp / v_subrev_u32_dpp with quad_perm:[R,R,R,R]), the result
-    // was wrong on MI355X (the intra 4x4 transforms lost parity with the oracle).
-    int r = __builtin_amdgcn_mov_dpp(v, R * 0x55, 0xF, 0xF, true);  // every source lane is valid
-    __asm__ volatile("" : "+v"(r));
-    return r;
-}
-
 // Canvas offset, relative to the block's top-left edge sample (row py, column px), of edge-line
 // entry i (see c_i4_taps), with (tr) or without the top-right samples.
 __device__ __forceinline__ int g4_offset(int i, bool tr) {
@@ -1752,12 +1818,6 @@ __device__ __forceinline__ int g4_offset(int i, bool tr) {
 // levels are kept and its reconstruction written to the canvas before the next block. Returns
 // false as soon as the running cost (from 24 lambda) reaches budget. lmodes / tmodes: the left
 // and top macroblocks' packed modes (all DC when they are not I_4x4).
-// The y-th of four values (y in 0..3, per lane) by masks: a ternary chain on a lane-varying
-// index compiles to divergent branches.
-__device__ __forceinline__ int sel4(int y, int a, int b, int c, int d) {
-    return (a & -(int)(y == 0)) | (b & -(int)(y == 1)) | (c & -(int)(y == 2)) | (d & -(int)(y == 3));
-}
-
 __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint32_t lambda, uint32_t budget,
                            uint64_t lmodes, uint64_t tmodes, uint64_t &modes, int &cbpl) {
     const int lane = threadIdx.x, m = lane >> 2, y = lane & 3;
