@@ -21,7 +21,6 @@
 namespace {
 
 constexpr int ZZ4[16] = {0, 1, 4, 8, 5, 2, 3, 6, 9, 12, 13, 10, 7, 11, 14, 15};
-constexpr int Z2R[16] = {0, 1, 4, 5, 2, 3, 6, 7, 8, 9, 12, 13, 10, 11, 14, 15};
 constexpr int ZZ8[64] = {0,  1,  8,  16, 9,  2,  3,  10, 17, 24, 32, 25, 18, 11, 4,  5,  12, 19, 26, 33, 40, 48,
                          41, 34, 27, 20, 13, 6,  7,  14, 21, 28, 35, 42, 49, 56, 57, 50, 43, 36, 29, 22, 15, 23,
                          30, 37, 44, 51, 58, 59, 52, 45, 38, 31, 39, 46, 53, 60, 61, 54, 47, 55, 62, 63};
@@ -52,7 +51,6 @@ struct MbLds {
             alignas(16) int16_t q4i[16][16];   // intra 4x4 levels per block (decoding order), raster
         };
         struct {
-            alignas(16) uint8_t tmp[256];
             alignas(16) int16_t coef[16][16];  // luma levels, decoding order, raster positions
             alignas(16) int16_t q8[4][64];     // 8x8 levels, raster positions
             alignas(16) int32_t t8[4][64];     // 8x8 transform intermediate
@@ -67,10 +65,10 @@ struct MbLds {
 };
 
 __device__ __forceinline__ uint32_t se_bits(int d) {  // enc.rs:546
-    // se(v) code number 2|d| - (d > 0), as a select (the form `- (d > 0)` costs a VALU round trip
-    // when d is uniform).
-    const uint32_t v = d > 0 ? (uint32_t)(2 * d - 1) : (uint32_t)(-2 * d);
-    return 2u * (uint32_t)(32 - __clz((int)(v + 1))) - 1u;
+    // Length of se(v): 2 bit_length(k + 1) - 1 for code number k = 2|d| - (d > 0), where
+    // k + 1 = 2|d| + (d <= 0) is formed without a select.
+    const uint32_t k1 = ((uint32_t)iabs(d) << 1) | (uint32_t)(d <= 0);
+    return 2u * (uint32_t)(32 - __clz((int)k1)) - 1u;
 }
 
 __device__ __forceinline__ int tap6(int a, int b, int c, int d, int e, int f) {
@@ -83,36 +81,28 @@ __device__ __forceinline__ uint32_t lds4u(const uint8_t *row, int x) {
     return __builtin_amdgcn_alignbyte(q[1], q[0], (uint32_t)(x & 3));
 }
 
-// x264 decimation weight of a zero run (enc.rs:88 DECIMATE4): 3, 2, 2, 1, 1, 1, 0...
-__device__ __forceinline__ uint32_t dec4_weight(int run) { return run == 0 ? 3u : (run <= 2 ? 2u : (run <= 5 ? 1u : 0u)); }
-// DECIMATE8: 3 x4, 2 x8, 1 x12, 0...
-__device__ __forceinline__ uint32_t dec8_weight(int run) { return run < 4 ? 3u : (run < 12 ? 2u : (run < 24 ? 1u : 0u)); }
-
 // enc.rs:583 decimate_masked from the non-zero mask of the scanned levels and whether one of
-// them exceeds 1 in magnitude (then the score is 9).
+// them exceeds 1 in magnitude (then the score is 9). The reference walks the set bits from the
+// top, adding DECIMATE4[r] (enc.rs:88: 3, 2, 2, 1, 1, 1, 0...) for the run r of zeros below
+// each (down to the next set bit, or to the start). Branch-free form: the weight is [r <= 5] +
+// [r <= 2] + [r == 0], and [r <= k] holds for set bit p exactly when a set bit lies in
+// p-k-1 .. p-1 or p <= k, so each term is a popcount of the mask against itself smeared k+1
+// bits upwards (with the k+1 lowest bits set).
 __device__ __forceinline__ uint32_t decimate_mask(uint32_t mask, int big) {
-    if (big) return 9;
-    uint32_t score = 0, m = mask;
-    while (m) {
-        int p = 31 - __clz((int)m);
-        m ^= 1u << p;
-        int below = m ? 31 - __clz((int)m) : -1;
-        score += dec4_weight(p - below - 1);
-    }
-    return score;
+    const uint32_t t3 = mask | (mask << 1) | (mask << 2), t6 = t3 | (t3 << 3);
+    const uint32_t s = (uint32_t)__builtin_popcount(mask & ((mask << 1) | 1u)) +
+                       (uint32_t)__builtin_popcount(mask & ((t3 << 1) | 7u)) +
+                       (uint32_t)__builtin_popcount(mask & ((t6 << 1) | 63u));
+    return big ? 9u : s;
 }
-// enc.rs:557 decimate_score of 64 levels (8x8 table) from their zig-zag non-zero mask.
+// enc.rs:557 decimate_score of 64 levels (8x8 table) from their zig-zag non-zero mask, in the
+// same form: DECIMATE8[r] (3 x4, 2 x8, 1 x12, 0...) = [r <= 23] + [r <= 11] + [r <= 3].
 __device__ __forceinline__ uint32_t decimate_mask64(uint64_t mask, int big) {
-    if (big) return 9;
-    uint32_t score = 0;
-    uint64_t m = mask;
-    while (m) {
-        int p = 63 - __clzll((long long)m);
-        m ^= 1ull << p;
-        int below = m ? 63 - __clzll((long long)m) : -1;
-        score += dec8_weight(p - below - 1);
-    }
-    return score;
+    const uint64_t t2 = mask | (mask << 1), t4 = t2 | (t2 << 2), t12 = t4 | (t4 << 4) | (t4 << 8), t24 = t12 | (t12 << 12);
+    const uint32_t s = (uint32_t)__builtin_popcountll(mask & ((t4 << 1) | 0xFull)) +
+                       (uint32_t)__builtin_popcountll(mask & ((t12 << 1) | 0xFFFull)) +
+                       (uint32_t)__builtin_popcountll(mask & ((t24 << 1) | 0xFFFFFFull));
+    return big ? 9u : s;
 }
 
 // Value of row r (0..3) of the lane's quad (DPP quad broadcast).
@@ -413,6 +403,10 @@ __device__ void compute_window(MbLds &S, int wx0, int wy0) {
     __syncthreads();
 }
 
+// Rounded byte-wise average (a + b + 1) >> 1 of four packed bytes: v_lerp_u8 with the
+// round-up bit set in every byte.
+__device__ __forceinline__ uint32_t avg4(uint32_t a, uint32_t b) { return __builtin_amdgcn_lerp(a, b, 0x01010101u); }
+
 // The lane's four prediction bytes at quarter-pel mv (relative to the macroblock at x0, y0)
 // from the window anchored at full-pel (bx, by): one plane or the rounded average of two
 // (enc.rs:1415 window_src, the x264 get_ref table). Integer samples come from the staged
@@ -448,8 +442,7 @@ __device__ uint32_t window_pred4(int x0, int y0, int bx, int by, int ox, int oy,
     const uint32_t a = fetch4(ka, dxa, 0);
     if (kb < 0) return a;
     const uint32_t b = fetch4(kb, dxb, dyb);
-    // Rounded byte-wise average (a + b + 1) >> 1 without carries between bytes.
-    return (a | b) - (((a ^ b) >> 1) & 0x7F7F7F7Fu);
+    return avg4(a, b);
 }
 
 // The quarter-pel phase table of window_pred4 packed one byte per phase (my & 3) * 4 + (mx & 3):
@@ -503,7 +496,7 @@ __device__ uint32_t subpel_sads(int x0, int y0, int bx, int by, int ox, int oy, 
         for (int c = 0; c < 4; c++) {
             const uint32_t a = __builtin_amdgcn_alignbyte(da[c + 1], da[c], fa);
             const uint32_t b = __builtin_amdgcn_alignbyte(db[c + 1], db[c], fb);
-            const uint32_t p = (a | b) - (((a ^ b) >> 1) & 0x7F7F7F7Fu);
+            const uint32_t p = avg4(a, b);
             sad = __builtin_amdgcn_sad_u8(src[r * 4 + c], p, sad);
         }
     }
@@ -520,29 +513,24 @@ __device__ __forceinline__ uint32_t ssd4(uint32_t a, uint32_t b) {
     return s;
 }
 
-// pixel_satd_16x16 (simd/h264_enc.rs:167) of two 16-stride LDS blocks: lanes 0..15 each
-// sum one 4x4 block's unhalved Hadamard magnitudes; the total is halved once.
+// pixel_satd_16x16 (simd/h264_enc.rs:167) of two 16-stride LDS blocks: the sum of the 4x4
+// blocks' unhalved Hadamard magnitudes, halved once. One block row per lane (block lane >> 2 in
+// raster order, row lane & 3): the row transform in the lane, the column transform across the
+// quad, each lane summing its output row.
 __device__ uint32_t satd16(const uint8_t *a, const uint8_t *b) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x, bx = (lane >> 2) & 3, by = lane >> 4, row = lane & 3;
+    const int o = (by * 4 + row) * 16 + bx * 4;
+    const uint32_t wa = *(const uint32_t *)&a[o], wb = *(const uint32_t *)&b[o];
+    const int d0 = (int)byte_of(wa, 0) - (int)byte_of(wb, 0), d1 = (int)byte_of(wa, 1) - (int)byte_of(wb, 1);
+    const int d2 = (int)byte_of(wa, 2) - (int)byte_of(wb, 2), d3 = (int)byte_of(wa, 3) - (int)byte_of(wb, 3);
+    const int s01 = d0 + d1, d01 = d0 - d1, s23 = d2 + d3, d23 = d2 - d3;
+    const int t[4] = {s01 + s23, s01 - s23, d01 + d23, d01 - d23};
     uint32_t part = 0;
-    if (lane < 16) {
-        int bx = (lane & 3) * 4, by = (lane >> 2) * 4, d[4][4];
 #pragma unroll
-        for (int y = 0; y < 4; y++) {
-            uint32_t wa = *(const uint32_t *)&a[(by + y) * 16 + bx], wb = *(const uint32_t *)&b[(by + y) * 16 + bx];
-            int d0 = (int)byte_of(wa, 0) - (int)byte_of(wb, 0), d1 = (int)byte_of(wa, 1) - (int)byte_of(wb, 1);
-            int d2 = (int)byte_of(wa, 2) - (int)byte_of(wb, 2), d3 = (int)byte_of(wa, 3) - (int)byte_of(wb, 3);
-            int s01 = d0 + d1, d01 = d0 - d1, s23 = d2 + d3, d23 = d2 - d3;
-            d[y][0] = s01 + s23;
-            d[y][1] = s01 - s23;
-            d[y][2] = d01 + d23;
-            d[y][3] = d01 - d23;
-        }
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            int s01 = d[0][c] + d[1][c], d01 = d[0][c] - d[1][c], s23 = d[2][c] + d[3][c], d23 = d[2][c] - d[3][c];
-            part += (uint32_t)(iabs(s01 + s23) + iabs(s01 - s23) + iabs(d01 + d23) + iabs(d01 - d23));
-        }
+    for (int c = 0; c < 4; c++) {
+        const int v0 = quad_row<0>(t[c]), v1 = quad_row<1>(t[c]), v2 = quad_row<2>(t[c]), v3 = quad_row<3>(t[c]);
+        const int p01 = v0 + v1, q01 = v0 - v1, p23 = v2 + v3, q23 = v2 - v3;
+        part += (uint32_t)iabs(sel4(row, p01 + p23, p01 - p23, q01 + q23, q01 - q23));
     }
     return wave_sum(part) >> 1;
 }
@@ -793,9 +781,10 @@ struct I16Analysis {
 };
 
 // V, H and DC SATDs from one source Hadamard, plane SATD by prediction; the intra edges and
-// the source are in S.
+// the source are in S. One 4x4 block row per lane (block (lane >> 2) & 3, lane >> 4 in raster
+// order, row lane & 3): rows in the lane, columns across the quad.
 __device__ I16Analysis analyse_i16(MbLds &S, int mb_x, int mb_y, uint16_t &n_satd) {
-    const int lane = threadIdx.x, rr = lane >> 2, c0 = (lane & 3) * 4;
+    const int lane = threadIdx.x;
     const int has_t = mb_y > 0, has_l = mb_x > 0, tl = S.etl;
     uint32_t st = 0, sl = 0;
 #pragma unroll
@@ -805,57 +794,6 @@ __device__ I16Analysis analyse_i16(MbLds &S, int mb_x, int mb_y, uint16_t &n_sat
     }
     const int dc = (has_t && has_l) ? (int)((st + sl + 16) >> 5)
                                     : (has_t ? (int)((st + 8) >> 4) : (has_l ? (int)((sl + 8) >> 4) : 128));
-    // simd/h264_enc.rs:628 intra_satd_x3_16x16: V, H, DC SATDs from one source Hadamard
-    // (unavailable edges read as zero, as the reference passes zero rows).
-    uint32_t sv = 0, sh = 0, sd = 0;
-    if (lane < 16) {
-        const int bx = lane & 3, by = lane >> 2;
-        int rows[4][4];
-#pragma unroll
-        for (int y = 0; y < 4; y++) {
-            const uint32_t w = *(const uint32_t *)&S.src[(by * 4 + y) * 16 + bx * 4];
-            const int x0_ = byte_of(w, 0), x1 = byte_of(w, 1), x2 = byte_of(w, 2), x3 = byte_of(w, 3);
-            const int s01 = x0_ + x1, d01 = x0_ - x1, s23 = x2 + x3, d23 = x2 - x3;
-            rows[y][0] = s01 + s23;
-            rows[y][1] = s01 - s23;
-            rows[y][2] = d01 + d23;
-            rows[y][3] = d01 - d23;
-        }
-        int tv[4], th[4];
-        {
-            const int a0 = S.etop[bx * 4], a1 = S.etop[bx * 4 + 1], a2 = S.etop[bx * 4 + 2], a3 = S.etop[bx * 4 + 3];
-            tv[0] = 4 * (a0 + a1 + a2 + a3);
-            tv[1] = 4 * (a0 + a1 - a2 - a3);
-            tv[2] = 4 * (a0 - a1 + a2 - a3);
-            tv[3] = 4 * (a0 - a1 - a2 + a3);
-            const int l0 = S.eleft[by * 4], l1 = S.eleft[by * 4 + 1], l2 = S.eleft[by * 4 + 2], l3 = S.eleft[by * 4 + 3];
-            th[0] = 4 * (l0 + l1 + l2 + l3);
-            th[1] = 4 * (l0 + l1 - l2 - l3);
-            th[2] = 4 * (l0 - l1 + l2 - l3);
-            th[3] = 4 * (l0 - l1 - l2 + l3);
-        }
-#pragma unroll
-        for (int h = 0; h < 4; h++) {
-            const int s01 = rows[0][h] + rows[1][h], d01 = rows[0][h] - rows[1][h];
-            const int s23 = rows[2][h] + rows[3][h], d23 = rows[2][h] - rows[3][h];
-            const int o[4] = {(int16_t)(s01 + s23), (int16_t)(s01 - s23), (int16_t)(d01 + d23), (int16_t)(d01 - d23)};
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int c = o[v];
-                sv += (uint32_t)iabs(c - (v == 0 ? tv[h] : 0));
-                sh += (uint32_t)iabs(c - (h == 0 ? th[v] : 0));
-                sd += (uint32_t)iabs(c - ((v == 0 && h == 0) ? 16 * dc : 0));
-            }
-        }
-    }
-    sv = wave_sum(sv) >> 1;
-    sh = wave_sum(sh) >> 1;
-    sd = wave_sum(sd) >> 1;
-    // enc.rs:1637 analyse_i16_edges: DC, then V, then H, then plane, strict improvements.
-    uint32_t ib = sd;
-    int im = 2;
-    if (has_t && sv < ib) { ib = sv; im = 0; }
-    if (has_l && sh < ib) { ib = sh; im = 1; }
     int pa = 0, pb = 0, pcc = 0;
     if (has_t && has_l) {
         int Hh = 0, Vv = 0;
@@ -868,12 +806,63 @@ __device__ I16Analysis analyse_i16(MbLds &S, int mb_x, int mb_y, uint16_t &n_sat
         pa = 16 * (S.etop[15] + S.eleft[15]);
         pb = (5 * Hh + 32) >> 6;
         pcc = (5 * Vv + 32) >> 6;
-        uint32_t w = 0;
+    }
+    // simd/h264_enc.rs:628 intra_satd_x3_16x16: V, H, DC SATDs from one source Hadamard
+    // (unavailable edges read as zero, as the reference passes zero rows); the plane SATD is
+    // pixel_satd_16x16 (simd/h264_enc.rs:167) of the plane prediction.
+    const int bx = (lane >> 2) & 3, by = lane >> 4, row = lane & 3;
+    const uint32_t w = *(const uint32_t *)&S.src[(by * 4 + row) * 16 + bx * 4];
+    int hs[4], hp[4];
+    {
+        const int x0_ = byte_of(w, 0), x1 = byte_of(w, 1), x2 = byte_of(w, 2), x3 = byte_of(w, 3);
+        const int s01 = x0_ + x1, d01 = x0_ - x1, s23 = x2 + x3, d23 = x2 - x3;
+        hs[0] = s01 + s23;
+        hs[1] = s01 - s23;
+        hs[2] = d01 + d23;
+        hs[3] = d01 - d23;
+        int e[4];
+        const int xy = pa + pcc * (by * 4 + row - 7) + 16;
 #pragma unroll
-        for (int i = 0; i < 4; i++) w |= sat_shr<5>(pa + pb * (c0 + i - 7) + pcc * (rr - 7) + 16) << (8 * i);
-        *(uint32_t *)&S.tmp[lane * 4] = w;
-        __syncthreads();
-        const uint32_t ps = satd16(S.src, S.tmp);
+        for (int i = 0; i < 4; i++) e[i] = (int)byte_of(w, i) - (int)sat_shr<5>(xy + pb * (bx * 4 + i - 7));
+        const int p01 = e[0] + e[1], q01 = e[0] - e[1], p23 = e[2] + e[3], q23 = e[2] - e[3];
+        hp[0] = p01 + p23;
+        hp[1] = p01 - p23;
+        hp[2] = q01 + q23;
+        hp[3] = q01 - q23;
+    }
+    int tv[4], thr;
+    {
+        const int a0 = S.etop[bx * 4], a1 = S.etop[bx * 4 + 1], a2 = S.etop[bx * 4 + 2], a3 = S.etop[bx * 4 + 3];
+        tv[0] = 4 * (a0 + a1 + a2 + a3);
+        tv[1] = 4 * (a0 + a1 - a2 - a3);
+        tv[2] = 4 * (a0 - a1 + a2 - a3);
+        tv[3] = 4 * (a0 - a1 - a2 + a3);
+        const int l0 = S.eleft[by * 4], l1 = S.eleft[by * 4 + 1], l2 = S.eleft[by * 4 + 2], l3 = S.eleft[by * 4 + 3];
+        thr = 4 * sel4(row, l0 + l1 + l2 + l3, l0 + l1 - l2 - l3, l0 - l1 + l2 - l3, l0 - l1 - l2 + l3);
+    }
+    uint32_t part[4] = {0, 0, 0, 0};  // V, H, DC, plane
+#pragma unroll
+    for (int h = 0; h < 4; h++) {
+        const int v0 = quad_row<0>(hs[h]), v1 = quad_row<1>(hs[h]), v2 = quad_row<2>(hs[h]), v3 = quad_row<3>(hs[h]);
+        const int s01 = v0 + v1, d01 = v0 - v1, s23 = v2 + v3, d23 = v2 - v3;
+        const int c = (int16_t)sel4(row, s01 + s23, s01 - s23, d01 + d23, d01 - d23);
+        part[0] += (uint32_t)iabs(c - (row == 0 ? tv[h] : 0));
+        part[1] += (uint32_t)iabs(c - (h == 0 ? thr : 0));
+        part[2] += (uint32_t)iabs(c - ((row == 0 && h == 0) ? 16 * dc : 0));
+        const int u0 = quad_row<0>(hp[h]), u1 = quad_row<1>(hp[h]), u2 = quad_row<2>(hp[h]), u3 = quad_row<3>(hp[h]);
+        const int p01 = u0 + u1, q01 = u0 - u1, p23 = u2 + u3, q23 = u2 - u3;
+        part[3] += (uint32_t)iabs(sel4(row, p01 + p23, p01 - p23, q01 + q23, q01 - q23));
+    }
+    uint32_t sums[4];
+    wave_sum4(part, sums);
+    const uint32_t sv = sums[0] >> 1, sh = sums[1] >> 1, sd = sums[2] >> 1;
+    // enc.rs:1637 analyse_i16_edges: DC, then V, then H, then plane, strict improvements.
+    uint32_t ib = sd;
+    int im = 2;
+    if (has_t && sv < ib) { ib = sv; im = 0; }
+    if (has_l && sh < ib) { ib = sh; im = 1; }
+    if (has_t && has_l) {
+        const uint32_t ps = sums[3] >> 1;
         n_satd++;
         if (ps < ib) { ib = ps; im = 3; }
     }
@@ -887,73 +876,71 @@ struct ChromaIntra {
 
 // enc.rs:1523 analyse_chroma_intra: the chroma mode by SAD + 8 for non-DC modes (predictors
 // h264.rs:1967 and enc.rs:1588) from the chroma edges in S; the lane's predicted U and V samples.
+// All four modes are predicted and their SADs reduced together; unavailable modes are computed
+// from whatever the edge arrays hold and then ignored.
 __device__ ChromaIntra chroma_intra(MbLds &S, int mb_x, int mb_y, uint32_t src_u, uint32_t src_v) {
     const int lane = threadIdx.x;
     const int has_t = mb_y > 0, has_l = mb_x > 0;
-    int cmode = 0;
     const int cr = lane >> 3, cc = lane & 7;
-    uint32_t cbest = 0xFFFFFFFFu, bpu = 0, bpv = 0;
-    for (int mode = 0; mode < 4; mode++) {
-        const int ok = mode == 1 ? has_l : (mode == 2 ? has_t : (mode == 3 ? (has_t && has_l) : 1));
-        if (!ok) continue;
-        uint32_t smp[2];
+    int smp[4][2];
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
-            const uint8_t *t = S.ctop[p], *l = S.cleft[p];
-            int v;
-            if (mode == 0) {
-                uint32_t t0 = 0, t1 = 0, l0 = 0, l1 = 0;
+    for (int p = 0; p < 2; p++) {
+        const uint8_t *t = S.ctop[p], *l = S.cleft[p];
+        uint32_t t0 = 0, t1 = 0, l0 = 0, l1 = 0;
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    t0 += t[i];
-                    t1 += t[4 + i];
-                    l0 += l[i];
-                    l1 += l[4 + i];
-                }
-                int d00, d01, d10, d11;
-                if (has_t && has_l) {
-                    d00 = (int)((t0 + l0 + 4) >> 3);
-                    d01 = (int)((t1 + 2) >> 2);
-                    d10 = (int)((l1 + 2) >> 2);
-                    d11 = (int)((t1 + l1 + 4) >> 3);
-                } else if (has_t) {
-                    d00 = d10 = (int)((t0 + 2) >> 2);
-                    d01 = d11 = (int)((t1 + 2) >> 2);
-                } else if (has_l) {
-                    d00 = d01 = (int)((l0 + 2) >> 2);
-                    d10 = d11 = (int)((l1 + 2) >> 2);
-                } else {
-                    d00 = d01 = d10 = d11 = 128;
-                }
-                v = cr < 4 ? (cc < 4 ? d00 : d01) : (cc < 4 ? d10 : d11);
-            } else if (mode == 1) {
-                v = l[cr];
-            } else if (mode == 2) {
-                v = t[cc];
-            } else {
-                int Hh = 0, Vv = 0;
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const int ptop = i == 3 ? S.ctl[p] : t[2 - i], pl = i == 3 ? S.ctl[p] : l[2 - i];
-                    Hh += (i + 1) * (t[4 + i] - ptop);
-                    Vv += (i + 1) * (l[4 + i] - pl);
-                }
-                const int a = 16 * (t[7] + l[7]), b = (17 * Hh + 16) >> 5, c = (17 * Vv + 16) >> 5;
-                v = (int)sat_shr<5>(a + b * (cc - 3) + c * (cr - 3) + 16);
-            }
-            smp[p] = (uint32_t)v;
+        for (int i = 0; i < 4; i++) {
+            t0 += t[i];
+            t1 += t[4 + i];
+            l0 += l[i];
+            l1 += l[4 + i];
         }
-        const uint32_t cost =
-            wave_sum((uint32_t)iabs((int)src_u - (int)smp[0]) + (uint32_t)iabs((int)src_v - (int)smp[1])) +
-            (mode ? 8u : 0u);
-        if (cost < cbest) {
-            cbest = cost;
-            cmode = mode;
-            bpu = smp[0];
-            bpv = smp[1];
+        int d00, d01, d10, d11;
+        if (has_t && has_l) {
+            d00 = (int)((t0 + l0 + 4) >> 3);
+            d01 = (int)((t1 + 2) >> 2);
+            d10 = (int)((l1 + 2) >> 2);
+            d11 = (int)((t1 + l1 + 4) >> 3);
+        } else if (has_t) {
+            d00 = d10 = (int)((t0 + 2) >> 2);
+            d01 = d11 = (int)((t1 + 2) >> 2);
+        } else if (has_l) {
+            d00 = d01 = (int)((l0 + 2) >> 2);
+            d10 = d11 = (int)((l1 + 2) >> 2);
+        } else {
+            d00 = d01 = d10 = d11 = 128;
         }
+        smp[0][p] = cr < 4 ? (cc < 4 ? d00 : d01) : (cc < 4 ? d10 : d11);
+        smp[1][p] = l[cr];
+        smp[2][p] = t[cc];
+        int Hh = 0, Vv = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int ptop = i == 3 ? S.ctl[p] : t[2 - i], pl = i == 3 ? S.ctl[p] : l[2 - i];
+            Hh += (i + 1) * (t[4 + i] - ptop);
+            Vv += (i + 1) * (l[4 + i] - pl);
+        }
+        const int a = 16 * (t[7] + l[7]), b = (17 * Hh + 16) >> 5, c = (17 * Vv + 16) >> 5;
+        smp[3][p] = (int)sat_shr<5>(a + b * (cc - 3) + c * (cr - 3) + 16);
     }
-    return {cmode, bpu, bpv};
+    uint32_t part[4], cost[4];
+#pragma unroll
+    for (int m = 0; m < 4; m++) part[m] = (uint32_t)(iabs((int)src_u - smp[m][0]) + iabs((int)src_v - smp[m][1]));
+    wave_sum4(part, cost);
+    // First strictly cheapest available mode, DC (always available) first.
+    int cmode = 0;
+    uint32_t cbest = cost[0];
+    if (has_l && cost[1] + 8u < cbest) {
+        cbest = cost[1] + 8u;
+        cmode = 1;
+    }
+    if (has_t && cost[2] + 8u < cbest) {
+        cbest = cost[2] + 8u;
+        cmode = 2;
+    }
+    if (has_t && has_l && cost[3] + 8u < cbest) cmode = 3;
+    const int pu = cmode == 0 ? smp[0][0] : (cmode == 1 ? smp[1][0] : (cmode == 2 ? smp[2][0] : smp[3][0]));
+    const int pv = cmode == 0 ? smp[0][1] : (cmode == 1 ? smp[1][1] : (cmode == 2 ? smp[2][1] : smp[3][1]));
+    return {cmode, (uint32_t)pu, (uint32_t)pv};
 }
 
 struct I16Code {
@@ -1211,7 +1198,7 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
     // Vector bounds (enc.rs:1104 MvBounds::new).
     const int bmin0 = -4 * (x0 + RGH_MV_MARGIN), bmin1 = max(-4 * (y0 + RGH_MV_MARGIN), -2048);
     const int bmax0 = 4 * (W - 16 - x0 + RGH_MV_MARGIN), bmax1 = min(4 * (H - 16 - y0 + RGH_MV_MARGIN), 2044);
-    auto in_bounds = [&](int mx, int my) { return mx >= bmin0 && mx <= bmax0 && my >= bmin1 && my <= bmax1; };
+    auto in_bounds = [&](int mx, int my) { return (int)(mx >= bmin0) & (int)(mx <= bmax0) & (int)(my >= bmin1) & (int)(my <= bmax1); };
     auto mv_cost = [&](int mx, int my) { return lambda * (se_bits(mx - mvpx) + se_bits(my - mvpy)); };
 
     // Integer-search candidates (enc.rs:1185 full_pel, :3081-3095): [mvp, 0, left, top,
@@ -1330,15 +1317,22 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
                 // enc.rs:2163 probe_skip_luma: no level above 1 and a total score below 6.
                 *(uint32_t *)&S.pred[lane * 4] = predw;
                 __syncthreads();
+                // One block row per lane (block lane >> 2, row lane & 3).
+                const int lb = lane >> 2, lrow = lane & 3;
+                int c[4];
+                fdct4_row(S.src, S.pred, 16, blk_x(lb) * 4, blk_y(lb) * 4, lrow, c);
+                quant4_row(c, lrow, qp, 0);
+                const uint32_t zz[4] = {0x09030200u, 0x0A080401u, 0x0E0B0705u, 0x0F0D0C06u};
+                uint32_t m = 0;
                 int big = 0;
-                uint32_t score = 0;
-                if (lane < 16) {
-                    int c[16];
-                    sub_fdct4(S.src, S.pred, 16, blk_x(lane) * 4, blk_y(lane) * 4, c);
-                    quant4(c, qp, 0);
-                    uint32_t m = zz_mask(c, &big);
-                    score = decimate_mask(m, big);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    m |= (uint32_t)(c[k] != 0) << ((zz[k] >> (8 * lrow)) & 15u);
+                    big |= iabs(c[k]) > 1;
                 }
+                m = (uint32_t)quad_or((int)m);
+                const int bigq = quad_or(big);
+                const uint32_t score = lrow == 0 ? decimate_mask(m, bigq) : 0u;
                 luma_zero = __ballot(big) == 0 && wave_sum(score) < 6;
             }
             if (luma_zero) {
@@ -1513,20 +1507,25 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
                 for (int it = 0; it < iters; it++) {
                     const int fx = bmx, fy = bmy;
                     const int dx[4] = {0, -step, step, 0}, dy[4] = {-step, 0, 0, step};
+                    // The first strictly cheapest point below me_cost: the least (cost, k) key
+                    // over the in-range points (costs stay below 2^30), without branches.
+                    uint32_t key = 0xFFFFFFFFu;
+                    int nok = 0;
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
                         const int mx = fx + dx[k], my = fy + dy[k];
-                        if (iabs(mx - cx) > 3 || iabs(my - cy) > 3 || !in_bounds(mx, my)) continue;
-                        const int qq = (my - cy + 3) * 7 + (mx - cx + 3);
-                        const uint32_t cst = (uint32_t)__builtin_amdgcn_readlane((int)costs, qq);
-                        D.n_subpel++;
-                        if (cst < me_cost) {
-                            me_cost = cst;
-                            bmx = mx;
-                            bmy = my;
-                        }
+                        const int ok = (int)(iabs(mx - cx) <= 3) & (int)(iabs(my - cy) <= 3) & (int)in_bounds(mx, my);
+                        const int qq = ok ? (my - cy + 3) * 7 + (mx - cx + 3) : 0;
+                        const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)costs, qq);
+                        nok += ok;
+                        key = min(key, ok ? (c << 2) | (uint32_t)k : 0xFFFFFFFFu);
                     }
-                    if (bmx == fx && bmy == fy) break;
+                    D.n_subpel += nok;
+                    if ((key >> 2) >= me_cost) break;
+                    me_cost = key >> 2;
+                    const int kb = (int)(key & 3u);
+                    bmx = fx + (kb == 1 ? -step : (kb == 2 ? step : 0));
+                    bmy = fy + (kb == 0 ? -step : (kb == 3 ? step : 0));
                 }
             }
             predw = window_pred4(x0, y0, fbx, fby, rox, roy, bmx, bmy, S);
@@ -1572,17 +1571,29 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
             chroma_mc_combine(ct, mvx, mvy, &pu, &pv);
             S.predu[lane] = (uint8_t)pu;
             S.predv[lane] = (uint8_t)pv;
-            // enc.rs:1810 quant_luma_4x4_inter: decimated 8x8 quadrants, then the whole MB.
-            if (lane < 16) {
-                int c[16], big;
-                sub_fdct4(S.src, S.pred, 16, blk_x(lane) * 4, blk_y(lane) * 4, c);
-                quant4(c, qp, 0);
-                const uint32_t m = zz_mask(c, &big);
+            // enc.rs:1810 quant_luma_4x4_inter: decimated 8x8 quadrants, then the whole MB. One
+            // block row per lane: block lane >> 2 (decoding order), row lane & 3.
+            const int lb = lane >> 2, lrow = lane & 3;
+            int q4[4];
+            {
+                fdct4_row(S.src, S.pred, 16, blk_x(lb) * 4, blk_y(lb) * 4, lrow, q4);
+                quant4_row(q4, lrow, qp, 0);
+                // Scan positions of the row's coefficients (byte lrow of each word).
+                const uint32_t zz[4] = {0x09030200u, 0x0A080401u, 0x0E0B0705u, 0x0F0D0C06u};
+                uint32_t m = 0;
+                int big = 0;
 #pragma unroll
-                for (int i = 0; i < 16; i++) S.coef[lane][i] = (int16_t)c[i];
-                S.bnz[lane] = m != 0;
-                S.bbig[lane] = big;
-                S.bscore[lane] = (int32_t)decimate_mask(m, big);
+                for (int k = 0; k < 4; k++) {
+                    m |= (uint32_t)(q4[k] != 0) << ((zz[k] >> (8 * lrow)) & 15u);
+                    big |= iabs(q4[k]) > 1;
+                }
+                m = (uint32_t)quad_or((int)m);
+                big = quad_or(big);
+                if (lrow == 0) {
+                    S.bnz[lb] = m != 0;
+                    S.bbig[lb] = big;
+                    S.bscore[lb] = (int32_t)decimate_mask(m, big);
+                }
             }
             __syncthreads();
             int keep = 0;
@@ -1605,9 +1616,13 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
                 if (total < 6) keep = 0;
             }
             cbpl = keep;
-            if (lane < 16 && !((keep >> (lane >> 2)) & 1))
+            {
+                const int kept = -(int)((keep >> (lb >> 2)) & 1);
 #pragma unroll
-                for (int i = 0; i < 16; i++) S.coef[lane][i] = 0;
+                for (int k = 0; k < 4; k++) q4[k] &= kept;
+                *(uint2 *)&S.coef[lb][lrow * 4] = make_uint2(((uint32_t)q4[0] & 0xFFFFu) | ((uint32_t)q4[1] << 16),
+                                                             ((uint32_t)q4[2] & 0xFFFFu) | ((uint32_t)q4[3] << 16));
+            }
             __syncthreads();
             if (J.t8x8 && cbpl) {
                 // enc.rs:2041 quant_luma_8x8_inter; the 8x8 transform wins when it leaves no
@@ -1646,11 +1661,7 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
                     if (S.b8score[b] >= 4) cbp8 |= 1 << b;
                 }
                 if (total < 6) cbp8 = 0;
-                uint32_t n4 = 0, n8 = 0;
-                if (lane < 16) {
-#pragma unroll
-                    for (int i = 0; i < 16; i++) n4 += S.coef[lane][i] != 0;
-                }
+                uint32_t n4 = (uint32_t)((q4[0] != 0) + (q4[1] != 0) + (q4[2] != 0) + (q4[3] != 0)), n8 = 0;
                 if (lane < 32 && ((cbp8 >> (lane >> 3)) & 1)) {
 #pragma unroll
                     for (int j = 0; j < 8; j++) n8 += S.q8[lane >> 3][j * 8 + (lane & 7)] != 0;
@@ -1692,13 +1703,11 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
                 if (use8) {
                     recon8(S, cbpl, qp);
                 } else {
-                    if (lane < 16 && ((cbpl >> (lane >> 2)) & 1)) {
-                        int blk[16];
-#pragma unroll
-                        for (int i = 0; i < 16; i++) blk[i] = S.coef[lane][i];
-                        dequant4(blk, qp);
-                        add_idct4(S.pred, 16, blk_x(lane) * 4, blk_y(lane) * 4, blk);
-                    }
+                    // Blocks outside the coded quadrants hold zeros and add nothing.
+                    const uint2 w = *(const uint2 *)&S.coef[lb][lrow * 4];
+                    int blk[4] = {(int16_t)w.x, (int)w.x >> 16, (int16_t)w.y, (int)w.y >> 16};
+                    dequant4_row(blk, lrow, qp);
+                    add_idct4_row(S.pred, 16, blk_x(lb) * 4, blk_y(lb) * 4, lrow, blk);
                     __syncthreads();
                 }
                 recon_chroma(S, cbpc, qpc);
