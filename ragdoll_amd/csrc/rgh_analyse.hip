@@ -178,6 +178,15 @@ __device__ __forceinline__ void add_idct4_row(uint8_t *dst, int s, int x, int y0
     }
     *p = n;
 }
+// Sum of a value over the lane's row of 16 lanes, in every lane of the row (the first half of
+// wave_sum).
+__device__ __forceinline__ uint32_t row16_sum(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);  // row_mirror
+    return v;
+}
 // OR of a value over the lane's quad.
 __device__ __forceinline__ int quad_or(int v) {
     int a = __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, true);  // quad_perm [1,0,3,2]
@@ -1442,37 +1451,51 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
         // (columns 2..39 + 16, rows 2..36 + 16): the four SADs then read LDS without a branch, so
         // their loads issue together; out-of-bounds points are computed and ignored.
         const bool in_region = J.me_iters <= 16;
+        // In the region, lane 16 k + r computes row r of point k's SAD (k: up, left, right,
+        // down) from the source row in registers, and each row of 16 lanes sums one point.
+        const int dk = lane >> 4, drow = lane & 15;
+        const int ddx = dk == 1 ? -1 : (dk == 2 ? 1 : 0), ddy = dk == 0 ? -1 : (dk == 3 ? 1 : 0);
+        const uint4 srow = *(const uint4 *)&S.src[drow * 16];
         for (int it = 0; it < iters; it++) {
             const int px[4] = {bestx, bestx - 1, bestx + 1, bestx};
             const int py[4] = {besty - 1, besty, besty, besty + 1};
             int ok[4];
-            uint32_t part[4];
 #pragma unroll
-            for (int k = 0; k < 4; k++) ok[k] = px[k] >= imn0 && px[k] <= imx0 && py[k] >= imn1 && py[k] <= imx1;
+            for (int k = 0; k < 4; k++)
+                ok[k] = (int)(px[k] >= imn0) & (int)(px[k] <= imx0) & (int)(py[k] >= imn1) & (int)(py[k] <= imx1);
+            uint32_t sums[4];
             if (in_region) {
+                const int lx = x0 + bestx + ddx - ox, ly = y0 + besty + ddy - oy + drow;
+                const uint32_t *q = (const uint32_t *)&S.sr[ly * SR_W + (lx & ~3)];
+                const uint32_t f = (uint32_t)(lx & 3);
+                const uint32_t w0 = q[0], w1 = q[1], w2 = q[2], w3 = q[3], w4 = q[4];
+                uint32_t sad = __builtin_amdgcn_sad_u8(srow.x, __builtin_amdgcn_alignbyte(w1, w0, f), 0u);
+                sad = __builtin_amdgcn_sad_u8(srow.y, __builtin_amdgcn_alignbyte(w2, w1, f), sad);
+                sad = __builtin_amdgcn_sad_u8(srow.z, __builtin_amdgcn_alignbyte(w3, w2, f), sad);
+                sad = __builtin_amdgcn_sad_u8(srow.w, __builtin_amdgcn_alignbyte(w4, w3, f), sad);
+                sad = row16_sum(sad);
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int lx = x0 + px[k] - ox, ly = y0 + py[k] - oy;
-                    part[k] = __builtin_amdgcn_sad_u8(srcw, lds4u(&S.sr[(ly + rr) * SR_W], lx + c0), 0);
-                }
+                for (int k = 0; k < 4; k++) sums[k] = (uint32_t)__builtin_amdgcn_readlane((int)sad, 16 * k);
             } else {
+                uint32_t part[4];
 #pragma unroll
                 for (int k = 0; k < 4; k++) part[k] = ok[k] ? sad_part(px[k], py[k]) : 0u;
+                wave_sum4(part, sums);
             }
-            uint32_t sums[4];
-            wave_sum4(part, sums);
-            int moved = -1;
+            // The first strictly cheapest in-range point: the least (cost, k) key (costs stay
+            // below 2^30).
+            uint32_t key = 0xFFFFFFFFu;
+            int nok = 0;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                if (!ok[k]) continue;
                 const uint32_t cst = sums[k] + mv_cost(px[k] * 4, py[k] * 4);
-                D.n_sad++;
-                if (cst < best_cost) {
-                    best_cost = cst;
-                    moved = k;
-                }
+                nok += ok[k];
+                key = min(key, ok[k] ? (cst << 2) | (uint32_t)k : 0xFFFFFFFFu);
             }
-            if (moved < 0) break;
+            D.n_sad += nok;
+            if ((key >> 2) >= best_cost) break;
+            best_cost = key >> 2;
+            const int moved = (int)(key & 3u);
             bestx += moved == 1 ? -1 : (moved == 2 ? 1 : 0);
             besty += moved == 0 ? -1 : (moved == 3 ? 1 : 0);
         }
