@@ -65,8 +65,14 @@ struct MbLds {
 };
 
 __device__ __forceinline__ uint32_t se_bits(int d) {  // enc.rs:546
-    // Length of se(v): 2 bit_length(k + 1) - 1 for code number k = 2|d| - (d > 0), where
-    // k + 1 = 2|d| + (d <= 0) is formed without a select.
+    // Length of se(v): 2 bit_length(k + 1) - 1 for code number k = 2|d| - (d > 0), with
+    // k + 1 = 2d (d > 0) or 1 - 2d as a select: for a uniform d it stays a scalar select (a
+    // comparison turned into an integer costs a VALU round trip there).
+    const uint32_t k1 = d > 0 ? (uint32_t)(2 * d) : (uint32_t)(1 - 2 * d);
+    return 2u * (uint32_t)(32 - __clz((int)k1)) - 1u;
+}
+// The same length for a lane-varying d, without a select (k + 1 = 2|d| + (d <= 0)).
+__device__ __forceinline__ uint32_t se_bits_v(int d) {
     const uint32_t k1 = ((uint32_t)iabs(d) << 1) | (uint32_t)(d <= 0);
     return 2u * (uint32_t)(32 - __clz((int)k1)) - 1u;
 }
@@ -795,11 +801,16 @@ struct I16Analysis {
 __device__ I16Analysis analyse_i16(MbLds &S, int mb_x, int mb_y, uint16_t &n_satd) {
     const int lane = threadIdx.x;
     const int has_t = mb_y > 0, has_l = mb_x > 0, tl = S.etl;
+    // The 16 top and left edge samples as four words each (byte i: byte i & 3 of word i >> 2).
+    const uint4 te = *(const uint4 *)S.etop, le = *(const uint4 *)S.eleft;
+    const uint32_t tw[4] = {te.x, te.y, te.z, te.w}, lw[4] = {le.x, le.y, le.z, le.w};
+    auto et = [&](int i) { return (int)byte_of(tw[i >> 2], i & 3); };
+    auto el = [&](int i) { return (int)byte_of(lw[i >> 2], i & 3); };
     uint32_t st = 0, sl = 0;
 #pragma unroll
-    for (int i = 0; i < 16; i++) {
-        st += S.etop[i];
-        sl += S.eleft[i];
+    for (int k = 0; k < 4; k++) {
+        st = __builtin_amdgcn_sad_u8(tw[k], 0u, st);
+        sl = __builtin_amdgcn_sad_u8(lw[k], 0u, sl);
     }
     const int dc = (has_t && has_l) ? (int)((st + sl + 16) >> 5)
                                     : (has_t ? (int)((st + 8) >> 4) : (has_l ? (int)((sl + 8) >> 4) : 128));
@@ -808,11 +819,11 @@ __device__ I16Analysis analyse_i16(MbLds &S, int mb_x, int mb_y, uint16_t &n_sat
         int Hh = 0, Vv = 0;
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            const int ptop = i == 7 ? tl : S.etop[6 - i], pl = i == 7 ? tl : S.eleft[6 - i];
-            Hh += (i + 1) * (S.etop[8 + i] - ptop);
-            Vv += (i + 1) * (S.eleft[8 + i] - pl);
+            const int ptop = i == 7 ? tl : et(6 - i), pl = i == 7 ? tl : el(6 - i);
+            Hh += (i + 1) * (et(8 + i) - ptop);
+            Vv += (i + 1) * (el(8 + i) - pl);
         }
-        pa = 16 * (S.etop[15] + S.eleft[15]);
+        pa = 16 * (et(15) + el(15));
         pb = (5 * Hh + 32) >> 6;
         pcc = (5 * Vv + 32) >> 6;
     }
@@ -841,12 +852,14 @@ __device__ I16Analysis analyse_i16(MbLds &S, int mb_x, int mb_y, uint16_t &n_sat
     }
     int tv[4], thr;
     {
-        const int a0 = S.etop[bx * 4], a1 = S.etop[bx * 4 + 1], a2 = S.etop[bx * 4 + 2], a3 = S.etop[bx * 4 + 3];
+        const uint32_t ta = bx == 0 ? te.x : (bx == 1 ? te.y : (bx == 2 ? te.z : te.w));
+        const uint32_t la = by == 0 ? le.x : (by == 1 ? le.y : (by == 2 ? le.z : le.w));
+        const int a0 = (int)byte_of(ta, 0), a1 = (int)byte_of(ta, 1), a2 = (int)byte_of(ta, 2), a3 = (int)byte_of(ta, 3);
         tv[0] = 4 * (a0 + a1 + a2 + a3);
         tv[1] = 4 * (a0 + a1 - a2 - a3);
         tv[2] = 4 * (a0 - a1 + a2 - a3);
         tv[3] = 4 * (a0 - a1 - a2 + a3);
-        const int l0 = S.eleft[by * 4], l1 = S.eleft[by * 4 + 1], l2 = S.eleft[by * 4 + 2], l3 = S.eleft[by * 4 + 3];
+        const int l0 = (int)byte_of(la, 0), l1 = (int)byte_of(la, 1), l2 = (int)byte_of(la, 2), l3 = (int)byte_of(la, 3);
         thr = 4 * sel4(row, l0 + l1 + l2 + l3, l0 + l1 - l2 - l3, l0 - l1 + l2 - l3, l0 - l1 - l2 + l3);
     }
     uint32_t part[4] = {0, 0, 0, 0};  // V, H, DC, plane
@@ -894,15 +907,13 @@ __device__ ChromaIntra chroma_intra(MbLds &S, int mb_x, int mb_y, uint32_t src_u
     int smp[4][2];
 #pragma unroll
     for (int p = 0; p < 2; p++) {
-        const uint8_t *t = S.ctop[p], *l = S.cleft[p];
-        uint32_t t0 = 0, t1 = 0, l0 = 0, l1 = 0;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            t0 += t[i];
-            t1 += t[4 + i];
-            l0 += l[i];
-            l1 += l[4 + i];
-        }
+        // The eight top and left edge samples as two words each (byte i of the edge: byte i & 3
+        // of word i >> 2); v_sad_u8 against zero sums four bytes.
+        const uint2 tw = *(const uint2 *)S.ctop[p], lw = *(const uint2 *)S.cleft[p];
+        auto t = [&](int i) { return (int)byte_of(i < 4 ? tw.x : tw.y, i & 3); };
+        auto l = [&](int i) { return (int)byte_of(i < 4 ? lw.x : lw.y, i & 3); };
+        const uint32_t t0 = __builtin_amdgcn_sad_u8(tw.x, 0u, 0u), t1 = __builtin_amdgcn_sad_u8(tw.y, 0u, 0u);
+        const uint32_t l0 = __builtin_amdgcn_sad_u8(lw.x, 0u, 0u), l1 = __builtin_amdgcn_sad_u8(lw.y, 0u, 0u);
         int d00, d01, d10, d11;
         if (has_t && has_l) {
             d00 = (int)((t0 + l0 + 4) >> 3);
@@ -919,16 +930,16 @@ __device__ ChromaIntra chroma_intra(MbLds &S, int mb_x, int mb_y, uint32_t src_u
             d00 = d01 = d10 = d11 = 128;
         }
         smp[0][p] = cr < 4 ? (cc < 4 ? d00 : d01) : (cc < 4 ? d10 : d11);
-        smp[1][p] = l[cr];
-        smp[2][p] = t[cc];
+        smp[1][p] = (int)byte_of(cr < 4 ? lw.x : lw.y, cr & 3);
+        smp[2][p] = (int)byte_of(cc < 4 ? tw.x : tw.y, cc & 3);
         int Hh = 0, Vv = 0;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int ptop = i == 3 ? S.ctl[p] : t[2 - i], pl = i == 3 ? S.ctl[p] : l[2 - i];
-            Hh += (i + 1) * (t[4 + i] - ptop);
-            Vv += (i + 1) * (l[4 + i] - pl);
+            const int ptop = i == 3 ? S.ctl[p] : t(2 - i), pl = i == 3 ? S.ctl[p] : l(2 - i);
+            Hh += (i + 1) * (t(4 + i) - ptop);
+            Vv += (i + 1) * (l(4 + i) - pl);
         }
-        const int a = 16 * (t[7] + l[7]), b = (17 * Hh + 16) >> 5, c = (17 * Vv + 16) >> 5;
+        const int a = 16 * (t(7) + l(7)), b = (17 * Hh + 16) >> 5, c = (17 * Vv + 16) >> 5;
         smp[3][p] = (int)sat_shr<5>(a + b * (cc - 3) + c * (cr - 3) + 16);
     }
     uint32_t part[4], cost[4];
@@ -968,13 +979,18 @@ __device__ I16Code code_i16(MbLds &S, int mb_x, int mb_y, int qp, int qpc, const
     int cbpl = 0, cbpc = 0, cmode = 0;
     uint32_t pu = 0, pv = 0;
     {
-        uint32_t w = 0;
+        // The lane's four predicted samples (row rr, columns c0..c0 + 3); im is uniform.
+        uint32_t w;
+        if (im == 0) {
+            w = has_t ? *(const uint32_t *)&S.etop[c0] : 0x80808080u;
+        } else if (im == 1) {
+            w = has_l ? (uint32_t)S.eleft[rr] * 0x01010101u : 0x80808080u;
+        } else if (im == 2) {
+            w = (uint32_t)dc * 0x01010101u;
+        } else {
+            w = 0;
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            int v = im == 0 ? (has_t ? S.etop[c0 + i] : 128)
-                            : (im == 1 ? (has_l ? S.eleft[rr] : 128)
-                                       : (im == 2 ? dc : (int)sat_shr<5>(pa + pb * (c0 + i - 7) + pcc * (rr - 7) + 16)));
-            w |= (uint32_t)v << (8 * i);
+            for (int i = 0; i < 4; i++) w |= sat_shr<5>(pa + pb * (c0 + i - 7) + pcc * (rr - 7) + 16) << (8 * i);
         }
         *(uint32_t *)&S.pred[lane * 4] = w;
     }
@@ -1209,6 +1225,7 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
     const int bmax0 = 4 * (W - 16 - x0 + RGH_MV_MARGIN), bmax1 = min(4 * (H - 16 - y0 + RGH_MV_MARGIN), 2044);
     auto in_bounds = [&](int mx, int my) { return (int)(mx >= bmin0) & (int)(mx <= bmax0) & (int)(my >= bmin1) & (int)(my <= bmax1); };
     auto mv_cost = [&](int mx, int my) { return lambda * (se_bits(mx - mvpx) + se_bits(my - mvpy)); };
+    auto mv_cost_v = [&](int mx, int my) { return lambda * (se_bits_v(mx - mvpx) + se_bits_v(my - mvpy)); };
 
     // Integer-search candidates (enc.rs:1185 full_pel, :3081-3095): [mvp, 0, left, top,
     // co-located], each rounded to full-pel, clamped and deduplicated. Fixed slots keep the
@@ -1319,7 +1336,13 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
             predw = window_pred4(x0, y0, fbx, fby, ox, oy, pskx, psky, S);
         }
         chroma_mc_combine(ct, pskx, psky, &pu, &pv);
-        const uint32_t sad = wave_sum(__builtin_amdgcn_sad_u8(srcw, predw, 0));
+        // The luma SAD (below 2^16) with the U SAD (below 2^14) in its upper half, and the V SAD,
+        // reduced together; the SSDs only when a test needs them.
+        const uint32_t du = (uint32_t)iabs((int)src_u - (int)pu), dv = (uint32_t)iabs((int)src_v - (int)pv);
+        const uint32_t in2[2] = {__builtin_amdgcn_sad_u8(srcw, predw, 0) | (du << 16), dv};
+        uint32_t ps[2];
+        wave_sums<2>(in2, ps);
+        const uint32_t sad = ps[0] & 0xFFFFu;
         if (sad < J.sb_probe) {
             int luma_zero = sad <= J.sb_luma || wave_sum(ssd4(srcw, predw)) <= J.sb_luma_ssd;
             if (!luma_zero) {
@@ -1345,9 +1368,8 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
                 luma_zero = __ballot(big) == 0 && wave_sum(score) < 6;
             }
             if (luma_zero) {
-                const uint32_t du = (uint32_t)iabs((int)src_u - (int)pu), dv = (uint32_t)iabs((int)src_v - (int)pv);
-                int chroma_zero = wave_sum(du) <= J.sb_chroma || wave_sum(du * du) <= J.sb_chroma_ssd;
-                if (chroma_zero) chroma_zero = wave_sum(dv) <= J.sb_chroma || wave_sum(dv * dv) <= J.sb_chroma_ssd;
+                int chroma_zero = (ps[0] >> 16) <= J.sb_chroma || wave_sum(du * du) <= J.sb_chroma_ssd;
+                if (chroma_zero) chroma_zero = ps[1] <= J.sb_chroma || wave_sum(dv * dv) <= J.sb_chroma_ssd;
                 if (!chroma_zero) {
                     // enc.rs:2186 probe_skip_chroma: no DC level and AC scores below 7.
                     S.predu[lane] = (uint8_t)pu;
@@ -1383,9 +1405,22 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
         uint32_t best_cost = 0xFFFFFFFFu, bestw = 0;
         int iters = J.me_iters;
         bool quick = false;  // learned: an intra-likely macroblock keeps its full-pel candidate
+        // Every candidate's SAD, two to a reduction (each below 2^16).
+        uint32_t csad[NC];
+        {
+            constexpr int NP = (NC + 1) / 2;
+            uint32_t in[NP], out[NP];
+#pragma unroll
+            for (int k = 0; k < NP; k++)
+                in[k] = __builtin_amdgcn_sad_u8(srcw, rw[2 * k], 0) |
+                        (2 * k + 1 < NC ? __builtin_amdgcn_sad_u8(srcw, rw[2 * k + 1 < NC ? 2 * k + 1 : 0], 0) << 16 : 0u);
+            wave_sums<NP>(in, out);
+#pragma unroll
+            for (int k = 0; k < NC; k++) csad[k] = (k & 1) ? out[k >> 1] >> 16 : out[k >> 1] & 0xFFFFu;
+        }
         auto take = [&](int k) {
             if (!live[k]) return;
-            const uint32_t cst = wave_sum(__builtin_amdgcn_sad_u8(srcw, rw[k], 0)) + mv_cost(fxs[k] * 4, fys[k] * 4);
+            const uint32_t cst = csad[k] + mv_cost(fxs[k] * 4, fys[k] * 4);
             D.n_sad++;
             if (cst < best_cost) {
                 best_cost = cst;
@@ -1523,7 +1558,7 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
             // order over them (costs, bounds and strict-< ties as enc.rs:1283 sub_pel).
             const uint32_t sads = subpel_sads(x0, y0, fbx, fby, rox, roy, cx, cy, S);
             // Lane q < 49 holds the cost of offset (q % 7 - 3, q / 7 - 3) quarter samples.
-            const uint32_t costs = sads + mv_cost(cx + lane % 7 - 3, cy + lane / 7 - 3);
+            const uint32_t costs = sads + mv_cost_v(cx + lane % 7 - 3, cy + lane / 7 - 3);
             int bmx = cx, bmy = cy;
             for (int pass = 0; pass < 2; pass++) {
                 const int step = pass == 0 ? 2 : 1, iters = pass == 0 ? 1 : J.qpel_iters;

@@ -165,22 +165,26 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
            (uint32_t)__builtin_amdgcn_readlane((int)v, 32) + (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
 }
 
-// Four independent wave sums, interleaved so their DPP chains overlap.
-__device__ __forceinline__ void wave_sum4(const uint32_t *in, uint32_t *out) {
-    uint32_t v[4] = {in[0], in[1], in[2], in[3]};
+// N independent wave sums, interleaved so their DPP chains overlap.
+template <int N>
+__device__ __forceinline__ void wave_sums(const uint32_t *in, uint32_t *out) {
+    uint32_t v[N];
 #pragma unroll
-    for (int k = 0; k < 4; k++) v[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[k], 0xB1, 0xF, 0xF, false);
+    for (int k = 0; k < N; k++) v[k] = in[k];
 #pragma unroll
-    for (int k = 0; k < 4; k++) v[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[k], 0x4E, 0xF, 0xF, false);
+    for (int k = 0; k < N; k++) v[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[k], 0xB1, 0xF, 0xF, false);
 #pragma unroll
-    for (int k = 0; k < 4; k++) v[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[k], 0x141, 0xF, 0xF, false);
+    for (int k = 0; k < N; k++) v[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[k], 0x4E, 0xF, 0xF, false);
 #pragma unroll
-    for (int k = 0; k < 4; k++) v[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[k], 0x140, 0xF, 0xF, false);
+    for (int k = 0; k < N; k++) v[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[k], 0x141, 0xF, 0xF, false);
 #pragma unroll
-    for (int k = 0; k < 4; k++)
+    for (int k = 0; k < N; k++) v[k] += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v[k], 0x140, 0xF, 0xF, false);
+#pragma unroll
+    for (int k = 0; k < N; k++)
         out[k] = (uint32_t)__builtin_amdgcn_readlane((int)v[k], 0) + (uint32_t)__builtin_amdgcn_readlane((int)v[k], 16) +
                  (uint32_t)__builtin_amdgcn_readlane((int)v[k], 32) + (uint32_t)__builtin_amdgcn_readlane((int)v[k], 48);
 }
+__device__ __forceinline__ void wave_sum4(const uint32_t *in, uint32_t *out) { wave_sums<4>(in, out); }
 
 // Four bytes at any address of global memory, from the two aligned dwords that hold them. The
 // second dword must lie inside the allocation (true for every padded-plane read of the kernels).
