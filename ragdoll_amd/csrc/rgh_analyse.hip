@@ -71,12 +71,6 @@ __device__ __forceinline__ uint32_t se_bits(int d) {  // enc.rs:546
     const uint32_t k1 = d > 0 ? (uint32_t)(2 * d) : (uint32_t)(1 - 2 * d);
     return 2u * (uint32_t)(32 - __clz((int)k1)) - 1u;
 }
-// The same length for a lane-varying d, without a select (k + 1 = 2|d| + (d <= 0)).
-__device__ __forceinline__ uint32_t se_bits_v(int d) {
-    const uint32_t k1 = ((uint32_t)iabs(d) << 1) | (uint32_t)(d <= 0);
-    return 2u * (uint32_t)(32 - __clz((int)k1)) - 1u;
-}
-
 __device__ __forceinline__ int tap6(int a, int b, int c, int d, int e, int f) {
     return a - 5 * b + 20 * c + 20 * d - 5 * e + f;
 }
@@ -480,42 +474,40 @@ __device__ __forceinline__ int plane_offset(int k, int x0, int y0, int bx, int b
     return po + (iy - by + 1) * WP_COLS + ix - bx + 1;
 }
 
-// SADs of the 49 quarter-pel positions within +-3 quarter samples of the full-pel centre
-// (cx, cy) through the window anchored at (bx, by), one position per lane: lane q holds the SAD
-// at (cx + q % 7 - 3, cy + q / 7 - 3) (lanes 49-63 repeat position 48). Each lane walks the
-// 16x16 block through its own plane pair (window_pred4's table), five aligned dwords a row per
-// plane; the source rows are read at one address by all lanes.
-__device__ uint32_t subpel_sads(int x0, int y0, int bx, int by, int ox, int oy, int cx, int cy, MbLds &S) {
-    const int q = min((int)threadIdx.x, 48);
-    const int mx = cx + q % 7 - 3, my = cy + q / 7 - 3;
-    const int ph = (my & 3) * 4 + (mx & 3);
+// SADs of four quarter-pel vectors (mx[k], my[k]) (relative to the macroblock at x0, y0), each
+// within +-3 quarter samples of the window anchor (bx, by), through the window: lane 16 k + r
+// sums row r of vector k from its plane pair (window_pred4's table; five aligned dwords a plane),
+// and each row of 16 lanes adds its vector's rows. Returns the four sums in out.
+__device__ void subpel_sad4(int x0, int y0, int bx, int by, int ox, int oy, const int *mx, const int *my, MbLds &S,
+                            uint32_t *out) {
+    const int lane = threadIdx.x, k = lane >> 4, r = lane & 15;
+    const int qx = sel4(k, mx[0], mx[1], mx[2], mx[3]), qy = sel4(k, my[0], my[1], my[2], my[3]);
+    const int ph = (qy & 3) * 4 + (qx & 3);
     const uint32_t e = (uint32_t)(((ph < 8 ? kPhaseLo : kPhaseHi) >> (8 * (ph & 7))) & 0xFF);
     int sa, sb;
-    const int oa = plane_offset(e & 3, x0, y0, bx, by, ox, oy, mx, my, (e >> 4) & 1, 0, sa);
-    const int ob = plane_offset((e >> 2) & 3, x0, y0, bx, by, ox, oy, mx, my, (e >> 5) & 1, (e >> 6) & 1, sb);
+    const int oa = plane_offset(e & 3, x0, y0, bx, by, ox, oy, qx, qy, (e >> 4) & 1, 0, sa);
+    const int ob = plane_offset((e >> 2) & 3, x0, y0, bx, by, ox, oy, qx, qy, (e >> 5) & 1, (e >> 6) & 1, sb);
     const uint8_t *base = (const uint8_t *)&S;
-    const uint32_t *qa = (const uint32_t *)(base + (oa & ~3)), *qb = (const uint32_t *)(base + (ob & ~3));
+    const uint32_t *qa = (const uint32_t *)(base + (oa & ~3) + r * sa), *qb = (const uint32_t *)(base + (ob & ~3) + r * sb);
     const uint32_t fa = (uint32_t)(oa & 3), fb = (uint32_t)(ob & 3);
-    const int wa = sa >> 2, wb = sb >> 2;
-    const uint32_t *src = (const uint32_t *)S.src;
-    uint32_t sad = 0;
-#pragma unroll 4
-    for (int r = 0; r < 16; r++) {
-        uint32_t da[5], db[5];
+    uint32_t da[5], db[5];
 #pragma unroll
-        for (int c = 0; c < 5; c++) {
-            da[c] = qa[r * wa + c];
-            db[c] = qb[r * wb + c];
-        }
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const uint32_t a = __builtin_amdgcn_alignbyte(da[c + 1], da[c], fa);
-            const uint32_t b = __builtin_amdgcn_alignbyte(db[c + 1], db[c], fb);
-            const uint32_t p = avg4(a, b);
-            sad = __builtin_amdgcn_sad_u8(src[r * 4 + c], p, sad);
-        }
+    for (int c = 0; c < 5; c++) {
+        da[c] = qa[c];
+        db[c] = qb[c];
     }
-    return sad;
+    const uint4 sw = *(const uint4 *)&S.src[r * 16];
+    const uint32_t src[4] = {sw.x, sw.y, sw.z, sw.w};
+    uint32_t sad = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const uint32_t a = __builtin_amdgcn_alignbyte(da[c + 1], da[c], fa);
+        const uint32_t b = __builtin_amdgcn_alignbyte(db[c + 1], db[c], fb);
+        sad = __builtin_amdgcn_sad_u8(src[c], avg4(a, b), sad);
+    }
+    sad = row16_sum(sad);
+#pragma unroll
+    for (int q = 0; q < 4; q++) out[q] = (uint32_t)__builtin_amdgcn_readlane((int)sad, 16 * q);
 }
 
 __device__ __forceinline__ uint32_t ssd4(uint32_t a, uint32_t b) {
@@ -1225,7 +1217,6 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
     const int bmax0 = 4 * (W - 16 - x0 + RGH_MV_MARGIN), bmax1 = min(4 * (H - 16 - y0 + RGH_MV_MARGIN), 2044);
     auto in_bounds = [&](int mx, int my) { return (int)(mx >= bmin0) & (int)(mx <= bmax0) & (int)(my >= bmin1) & (int)(my <= bmax1); };
     auto mv_cost = [&](int mx, int my) { return lambda * (se_bits(mx - mvpx) + se_bits(my - mvpy)); };
-    auto mv_cost_v = [&](int mx, int my) { return lambda * (se_bits_v(mx - mvpx) + se_bits_v(my - mvpy)); };
 
     // Integer-search candidates (enc.rs:1185 full_pel, :3081-3095): [mvp, 0, left, top,
     // co-located], each rounded to full-pel, clamped and deduplicated. Fixed slots keep the
@@ -1553,30 +1544,40 @@ __device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint3
             const int cx = bestx * 4, cy = besty * 4;
             compute_window(S, fbx - 1 - rox, fby - 1 - roy);
             D.n_hpel++;
-            // Every position the half-pel and quarter-pel diamonds can reach lies within +-3
-            // quarter samples of the centre: all 49 SADs at once, then the diamonds' sequential
-            // order over them (costs, bounds and strict-< ties as enc.rs:1283 sub_pel).
-            const uint32_t sads = subpel_sads(x0, y0, fbx, fby, rox, roy, cx, cy, S);
-            // Lane q < 49 holds the cost of offset (q % 7 - 3, q / 7 - 3) quarter samples.
-            const uint32_t costs = sads + mv_cost_v(cx + lane % 7 - 3, cy + lane / 7 - 3);
+            // The half-pel diamond, then the quarter-pel diamond (costs, bounds and strict-< ties as
+            // enc.rs:1283 sub_pel), each step's four SADs computed together. Every point lies
+            // within +-3 quarter samples of the centre (cx, cy), inside the window; points out of
+            // range are computed at the centre and ignored.
             int bmx = cx, bmy = cy;
             for (int pass = 0; pass < 2; pass++) {
                 const int step = pass == 0 ? 2 : 1, iters = pass == 0 ? 1 : J.qpel_iters;
                 for (int it = 0; it < iters; it++) {
                     const int fx = bmx, fy = bmy;
                     const int dx[4] = {0, -step, step, 0}, dy[4] = {-step, 0, 0, step};
+                    int mx[4], my[4], ok[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        mx[k] = fx + dx[k];
+                        my[k] = fy + dy[k];
+                        ok[k] = (int)(iabs(mx[k] - cx) <= 3) & (int)(iabs(my[k] - cy) <= 3) & (int)in_bounds(mx[k], my[k]);
+                    }
+                    int qx[4], qy[4];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        qx[k] = ok[k] ? mx[k] : cx;
+                        qy[k] = ok[k] ? my[k] : cy;
+                    }
+                    uint32_t sads[4];
+                    subpel_sad4(x0, y0, fbx, fby, rox, roy, qx, qy, S, sads);
                     // The first strictly cheapest point below me_cost: the least (cost, k) key
                     // over the in-range points (costs stay below 2^30), without branches.
                     uint32_t key = 0xFFFFFFFFu;
                     int nok = 0;
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
-                        const int mx = fx + dx[k], my = fy + dy[k];
-                        const int ok = (int)(iabs(mx - cx) <= 3) & (int)(iabs(my - cy) <= 3) & (int)in_bounds(mx, my);
-                        const int qq = ok ? (my - cy + 3) * 7 + (mx - cx + 3) : 0;
-                        const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)costs, qq);
-                        nok += ok;
-                        key = min(key, ok ? (c << 2) | (uint32_t)k : 0xFFFFFFFFu);
+                        const uint32_t c = sads[k] + mv_cost(mx[k], my[k]);
+                        nok += ok[k];
+                        key = min(key, ok[k] ? (c << 2) | (uint32_t)k : 0xFFFFFFFFu);
                     }
                     D.n_subpel += nok;
                     if ((key >> 2) >= me_cost) break;
