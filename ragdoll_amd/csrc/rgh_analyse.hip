@@ -1999,16 +1999,16 @@ __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint3
                 rec |= (uint32_t)clip255(p[k] + (int16_t)((o + 32) >> 6)) << (8 * k);
             }
         }
-        uint32_t best = 0xFFFFFFFFu;
-        int bm = 2;
-#pragma unroll
-        for (int k = 0; k < 9; k++) {
-            const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)cost, 4 * k);
-            if (c < best) {
-                best = c;
-                bm = k;
-            }
-        }
+        // The first cheapest mode: the least (cost, mode) key over the quads (costs stay below
+        // 2^27; unavailable modes and lanes 36-63 hold the largest key), reduced by DPP within
+        // each row of 16 lanes (four quads), then over the three rows that hold modes.
+        uint32_t key = cost == 0xFFFFFFFFu ? 0xFFFFFFFFu : (cost << 4) | (uint32_t)m;
+        key = min(key, (uint32_t)__builtin_amdgcn_mov_dpp((int)key, 0x141, 0xF, 0xF, true));  // row_half_mirror
+        key = min(key, (uint32_t)__builtin_amdgcn_mov_dpp((int)key, 0x140, 0xF, 0xF, true));  // row_mirror
+        key = min(min((uint32_t)__builtin_amdgcn_readlane((int)key, 0), (uint32_t)__builtin_amdgcn_readlane((int)key, 16)),
+                  (uint32_t)__builtin_amdgcn_readlane((int)key, 32));
+        const uint32_t best = key == 0xFFFFFFFFu ? 0xFFFFFFFFu : key >> 4;
+        const int bm = key == 0xFFFFFFFFu ? 2 : (int)(key & 15u);
         total += best;
         if (total >= budget) return false;
         modes |= (uint64_t)bm << (4 * b);
