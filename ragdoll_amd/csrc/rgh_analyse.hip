@@ -347,7 +347,7 @@ __device__ __forceinline__ void tap6x4(const uint32_t *w, int o, int *out) {
 // and :820 hpel_window_centre_scalar, the centre computed up front). Work items produce four
 // adjacent outputs from dword LDS reads: pass 1 the unrounded horizontal taps of rows -2 .. 20
 // with the b plane and the h plane; pass 2 the j plane from the kept taps.
-__device__ void compute_window(MbLds &S, int wx0, int wy0) {
+__device__ __forceinline__ void compute_window(MbLds &S, int wx0, int wy0) {
     const int lane = threadIdx.x;
     constexpr int NA = HR_ROWS * 6, NB = WP_ROWS * 6;
 #pragma unroll
@@ -582,7 +582,7 @@ __device__ __forceinline__ void chroma_mc_combine(const ChromaTaps &t, int mx, i
 // block row each (plane lane >> 4, raster 2x2 block (lane >> 2) & 3, row lane & 3), lanes 0..1
 // the 2x2 DC of a plane (enc.rs:1939 quant_chroma_dc). Leaves raster levels, scanned AC masks
 // and decimation scores in LDS. intra selects the rounding of quantize_block4x4.
-__device__ void quant_chroma(MbLds &S, int qpc, int intra) {
+__device__ __forceinline__ void quant_chroma(MbLds &S, int qpc, int intra) {
     const int lane = threadIdx.x;
     if (lane < 32) {
         const int p = lane >> 4, b = (lane >> 2) & 3, row = lane & 3;
@@ -1073,7 +1073,7 @@ __device__ I16Code code_i16(MbLds &S, int mb_x, int mb_y, int qp, int qpc, const
 // the intra 16x16 check only when the intra probability reaches learn_intra_gate / 255. Costs,
 // sub-pel refinement, residual coding and reconstruction are the plain path's.
 template <bool LEARNED>
-__device__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint32_t seq) {
+__device__ __forceinline__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint32_t seq) {
     const int lane = threadIdx.x;
     const int rr = lane >> 2, c0 = (lane & 3) * 4;
     const int mb_w = J.mb_w, idx = mb_y * mb_w + mb_x;
@@ -2025,7 +2025,7 @@ __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint3
 }
 
 // One IDR macroblock: decision record, levels and reconstruction.
-__device__ void analyse_intra_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S) {
+__device__ __forceinline__ void analyse_intra_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S) {
     const int lane = threadIdx.x, rr = lane >> 2, c0 = (lane & 3) * 4;
     const int mb_w = J.mb_w, idx = mb_y * mb_w + mb_x;
     const int x0 = mb_x * 16, y0 = mb_y * 16, cx0 = mb_x * 8, cy0 = mb_y * 8;
@@ -2239,8 +2239,10 @@ struct StepLdsOf<2> {
 //
 // IDR jobs run the same schedule in their own kernel (rgh_k_intra_step), and Compact-preset P
 // pictures in theirs (rgh_k_compact_step), so that each analysis has its own register and LDS
-// budget; Balanced P pictures searched from learned proposals run rgh_k_learned_step. MODE: 0
-// Balanced P, 1 IDR, 2 Compact P, 3 Balanced P with learned proposals.
+// budget; Balanced P pictures searched from learned proposals run rgh_k_learned_step. A launch
+// that holds both IDR and Balanced P pictures runs them in one kernel (rgh_k_mixed_step), each
+// block taking its picture's analysis, so that neither waits for the other. MODE: 0 Balanced P,
+// 1 IDR, 2 Compact P, 3 Balanced P with learned proposals, 4 IDR and Balanced P.
 template <int MODE>
 __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int j0, int launch, uint32_t seq) {
     const RghJobD J = device_view(jobs[j0 + (int)blockIdx.y]);
@@ -2263,9 +2265,14 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
         bool ok = true;
         if (s == 0 && J.wait_analysed) ok = __atomic_load_n(J.wait_analysed, __ATOMIC_RELAXED) >= J.wait_total;
         if (ok) {
-            if constexpr (MODE == 1)
+            if constexpr (MODE == 1) {
                 analyse_intra_mb(J, x, y, S.a);
-            else if constexpr (MODE == 2)
+            } else if constexpr (MODE == 4) {
+                if (J.intra)
+                    analyse_intra_mb(J, x, y, S.a);
+                else
+                    ok = analyse_mb<false>(J, x, y, S.a, seq);
+            } else if constexpr (MODE == 2)
                 ok = analyse_compact_mb(J, x, y, S.a, seq);
             else
                 ok = analyse_mb<MODE == 3>(J, x, y, S.a, seq);
@@ -2286,8 +2293,14 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
         if (y > min(J.mb_h, sd >> 1)) return;
         const int x = sd - 2 * y, m = y * (J.mb_w + 1) + x;
         if ((uint32_t)(J.fin[m] >> 32) == J.serial) return;  // done in an earlier launch
-        if (x < J.mb_w && y < J.mb_h && J.deblock) deblock_mb(J, x, y, S.d);
-        if (x > 0 && y > 0) pad_mb(J, x - 1, y - 1);
+        if constexpr (MODE == 4) {
+            // The mixed kernel inlines both analyses; the deblock and pad items stay inline too.
+            if (x < J.mb_w && y < J.mb_h && J.deblock) deblock_mb_inline(J, x, y, S.d);
+            if (x > 0 && y > 0) pad_mb_inline(J, x - 1, y - 1);
+        } else {
+            if (x < J.mb_w && y < J.mb_h && J.deblock) deblock_mb(J, x, y, S.d);
+            if (x > 0 && y > 0) pad_mb(J, x - 1, y - 1);
+        }
         if (lane == 0) J.fin[m] = make_stamp(J.serial, seq);
     }
 }
@@ -2300,6 +2313,11 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_frame_step(const RghJob *
 extern "C" __global__ __launch_bounds__(64) void rgh_k_intra_step(const RghJob *__restrict__ jobs, int j0, int launch,
                                                                    uint32_t seq) {
     frame_step<1>(jobs, j0, launch, seq);
+}
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_mixed_step(const RghJob *__restrict__ jobs, int j0, int launch,
+                                                                   uint32_t seq) {
+    frame_step<4>(jobs, j0, launch, seq);
 }
 
 extern "C" __global__ __launch_bounds__(64) void rgh_k_compact_step(const RghJob *__restrict__ jobs, int j0, int launch,

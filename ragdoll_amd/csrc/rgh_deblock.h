@@ -127,7 +127,7 @@ __device__ __forceinline__ int motion_bs(uint32_t a, uint32_t b) {
            iabs((int)(int16_t)(a >> 16) - (int)(int16_t)(b >> 16)) >= 4;
 }
 
-__device__ void deblock_mb(const RghJobD &J, int mb_x, int mb_y, DbLds &S) {
+__device__ __forceinline__ void deblock_mb_inline(const RghJobD &J, int mb_x, int mb_y, DbLds &S) {
     const int lane = threadIdx.x, mb_w = J.mb_w, m = mb_y * mb_w + mb_x;
     const rgh_mb_decision Q = load_dec(&J.dec[m]);
     rgh_mb_decision L{}, T{};
@@ -265,7 +265,7 @@ __device__ void deblock_mb(const RghJobD &J, int mb_x, int mb_y, DbLds &S) {
 // enc.rs:164 extend_borders, restricted to the padding samples whose nearest interior sample
 // lies in border macroblock (X, Y) of the current picture, all three planes. Rows and columns
 // are dword aligned (strides are multiples of 32, paddings of 16).
-__device__ void pad_mb(const RghJobD &J, int X, int Y) {
+__device__ __forceinline__ void pad_mb_inline(const RghJobD &J, int X, int Y) {
     const int mb_w = J.mb_w, mb_h = J.mb_h;
     if (X != 0 && X != mb_w - 1 && Y != 0 && Y != mb_h - 1) return;
     const int lane = threadIdx.x;
@@ -310,3 +310,7 @@ extern "C" __global__ __launch_bounds__(256) void rgh_k_extend(const RghJob *__r
         base[(long)y * s + x] = base[(long)sy * s + iclamp(x, 0, w - 1)];
     }
 }
+
+// Out-of-line forms, for kernels whose inliner budget is better spent elsewhere.
+__device__ void deblock_mb(const RghJobD &J, int mb_x, int mb_y, DbLds &S) { deblock_mb_inline(J, mb_x, mb_y, S); }
+__device__ void pad_mb(const RghJobD &J, int X, int Y) { pad_mb_inline(J, X, Y); }
