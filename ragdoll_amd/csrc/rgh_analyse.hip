@@ -2240,9 +2240,10 @@ struct StepLdsOf<2> {
 // IDR jobs run the same schedule in their own kernel (rgh_k_intra_step), and Compact-preset P
 // pictures in theirs (rgh_k_compact_step), so that each analysis has its own register and LDS
 // budget; Balanced P pictures searched from learned proposals run rgh_k_learned_step. A launch
-// that holds both IDR and Balanced P pictures runs them in one kernel (rgh_k_mixed_step), each
-// block taking its picture's analysis, so that neither waits for the other. MODE: 0 Balanced P,
-// 1 IDR, 2 Compact P, 3 Balanced P with learned proposals, 4 IDR and Balanced P.
+// that holds both IDR and Balanced P pictures runs them in one kernel (rgh_k_mixed_step, or
+// rgh_k_mixed_learned_step with learned proposals), each block taking its picture's analysis,
+// so that neither waits for the other. MODE: 0 Balanced P, 1 IDR, 2 Compact P, 3 Balanced P with
+// learned proposals, 4 IDR and Balanced P, 5 IDR and Balanced P with learned proposals.
 template <int MODE>
 __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int j0, int launch, uint32_t seq) {
     const RghJobD J = device_view(jobs[j0 + (int)blockIdx.y]);
@@ -2267,11 +2268,11 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
         if (ok) {
             if constexpr (MODE == 1) {
                 analyse_intra_mb(J, x, y, S.a);
-            } else if constexpr (MODE == 4) {
+            } else if constexpr (MODE == 4 || MODE == 5) {
                 if (J.intra)
                     analyse_intra_mb(J, x, y, S.a);
                 else
-                    ok = analyse_mb<false>(J, x, y, S.a, seq);
+                    ok = analyse_mb<MODE == 5>(J, x, y, S.a, seq);
             } else if constexpr (MODE == 2)
                 ok = analyse_compact_mb(J, x, y, S.a, seq);
             else
@@ -2293,8 +2294,8 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
         if (y > min(J.mb_h, sd >> 1)) return;
         const int x = sd - 2 * y, m = y * (J.mb_w + 1) + x;
         if ((uint32_t)(J.fin[m] >> 32) == J.serial) return;  // done in an earlier launch
-        if constexpr (MODE == 4) {
-            // The mixed kernel inlines both analyses; the deblock and pad items stay inline too.
+        if constexpr (MODE >= 4) {
+            // The mixed kernels inline both analyses; the deblock and pad items stay inline too.
             if (x < J.mb_w && y < J.mb_h && J.deblock) deblock_mb_inline(J, x, y, S.d);
             if (x > 0 && y > 0) pad_mb_inline(J, x - 1, y - 1);
         } else {
@@ -2318,6 +2319,11 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_intra_step(const RghJob *
 extern "C" __global__ __launch_bounds__(64) void rgh_k_mixed_step(const RghJob *__restrict__ jobs, int j0, int launch,
                                                                    uint32_t seq) {
     frame_step<4>(jobs, j0, launch, seq);
+}
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_mixed_learned_step(const RghJob *__restrict__ jobs, int j0,
+                                                                           int launch, uint32_t seq) {
+    frame_step<5>(jobs, j0, launch, seq);
 }
 
 extern "C" __global__ __launch_bounds__(64) void rgh_k_compact_step(const RghJob *__restrict__ jobs, int j0, int launch,

@@ -17,6 +17,7 @@ extern "C" __global__ void rgh_k_intra_step(const RghJob *jobs, int j0, int laun
 extern "C" __global__ void rgh_k_compact_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_learned_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_mixed_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
+extern "C" __global__ void rgh_k_mixed_learned_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_extend(const RghJob *jobs);
 extern "C" __global__ void rgh_k_fill_u64(uint64_t *p, int n, uint64_t v);
 
@@ -265,10 +266,11 @@ struct Schedule {
         // IDR jobs (the leading n_intra) and P jobs of the launch, in one sequence number: a
         // stamp of one is not final for the other within the launch.
         const int split = std::min(std::max(n_intra, j_lo), j_hi);
-        if (split > j_lo && j_hi > split && !learned && !compact) {
+        if (split > j_lo && j_hi > split && !compact) {
             // IDR and Balanced P jobs together: one kernel, so that neither waits for the other.
-            hipLaunchKernelGGL(rgh_k_mixed_step, dim3((unsigned)gx, (unsigned)(j_hi - j_lo)), dim3(64), 0, owner->stream,
-                               owner->jobs_dev, j_lo, L, seq);
+            hipLaunchKernelGGL(learned ? rgh_k_mixed_learned_step : rgh_k_mixed_step,
+                               dim3((unsigned)gx, (unsigned)(j_hi - j_lo)), dim3(64), 0, owner->stream, owner->jobs_dev,
+                               j_lo, L, seq);
             for (int j = j_lo; j < j_hi; j++) last[(size_t)j] = L;
             L++;
             return;

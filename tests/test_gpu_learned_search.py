@@ -98,3 +98,63 @@ def test_learned_rate_stays_near_the_plain_search(results):
     loss = bd(results, "plain", "learned")
     print("BD-rate learned vs plain: %.2f%%" % loss, flush=True)
     assert loss <= 1.0
+
+
+def test_pipelined_learned_schedule_matches_single_frames(frames):
+    # Two streams' IDR and learned-P pictures in one pipelined call (IDR and P pictures share
+    # launches there) against the same pictures analysed one call at a time, with the same
+    # proposals: decisions and levels byte for byte.
+    from ragdoll_amd import DECISION_DTYPE, LEVELS_PER_MB, StreamAnalyzer, analyse_frames
+    from ragdoll_amd.hip import Hip
+    from ragdoll_amd.learned import Proposer, load_trained
+    hip = Hip()
+    net = load_trained(max_batch=1024)
+    n, nf, qp, cfg = (W // 16) * (H // 16), 4, 30, dict(iters=2)
+    streams = [frames[:nf], frames[nf:2 * nf]]
+    prop = Proposer(net, 2 * (nf - 1) * n)
+    bufs = []
+    try:
+        srcs = []
+        for fr in streams:
+            row = []
+            for y, u, v in fr:
+                p = (hip.upload(y), hip.upload(u), hip.upload(v))
+                bufs += list(p)
+                row.append((p[0], p[1], p[2], y.shape[1], u.shape[1], v.shape[1]))
+            srcs.append(row)
+        addrs = prop.run([(srcs[s][k][0], W, srcs[s][k - 1][0], W, W, H, qp) for s in range(2) for k in range(1, nf)])
+        props = [[None] + addrs[s * (nf - 1):(s + 1) * (nf - 1)] for s in range(2)]
+
+        def run(pipelined):
+            ans = [StreamAnalyzer(W, H, qp, ring=nf + 2) for _ in range(2)]
+            decs = [[hip.alloc(n * 32) for _ in range(nf)] for _ in range(2)]
+            lvs = [[hip.alloc(n * LEVELS_PER_MB * 2) for _ in range(nf)] for _ in range(2)]
+            try:
+                print("learned schedule, pipelined=%s" % pipelined, flush=True)
+                if pipelined:
+                    analyse_frames(ans, srcs, decs, lvs, lag=3, first_idr=True, proposals=props, learned=cfg)
+                else:
+                    for s in range(2):
+                        analyse_frames([ans[s]], [[srcs[s][0]]], [[decs[s][0]]], [[lvs[s][0]]], first_idr=True)
+                        for k in range(1, nf):
+                            analyse_frames([ans[s]], [[srcs[s][k]]], [[decs[s][k]]], [[lvs[s][k]]],
+                                           proposals=[[props[s][k]]], learned=cfg)
+                return [(hip.download(decs[s][k], np.zeros(n, DECISION_DTYPE)),
+                         hip.download(lvs[s][k], np.zeros((n, LEVELS_PER_MB), np.int16)))
+                        for s in range(2) for k in range(nf)]
+            finally:
+                for a in ans:
+                    a.close()
+                for q in [q for d in decs for q in d] + [q for l in lvs for q in l]:
+                    hip.free(q)
+
+        piped, single = run(True), run(False)
+        for i, ((da, la), (db, lb)) in enumerate(zip(piped, single)):
+            assert np.array_equal(da.view(np.uint8), db.view(np.uint8)), "picture %d: decisions differ" % i
+            assert np.array_equal(la, lb), "picture %d: levels differ" % i
+        assert (piped[1][0]["mb_type"] != 3).all()  # P pictures were searched, not intra-coded as IDR
+    finally:
+        for q in bufs:
+            hip.free(q)
+        prop.close()
+        net.close()
