@@ -201,7 +201,8 @@ rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src_d
  * of a stream are in flight at once; the results are identical to analysing the frames one by
  * one. src_device[c * n_frames + k] is frame k of context c; decisions[c * n_frames + k]
  * receives its mb_w*mb_h records and levels[c * n_frames + k] its mb_w*mb_h*RGH_LEVELS_PER_MB
- * levels (device pointers, kept until the call returns). groups > 1 splits the contexts into that
+ * levels (device pointers, kept until the call returns; both 8-byte aligned, as hipMalloc
+ * returns them). groups > 1 splits the contexts into that
  * many schedules (context i in group i % groups) whose launches run concurrently on their own
  * streams, so one schedule's slowest macroblocks do not leave the device idle (0 selects 1).
  * With RGH_FRAMES_FIRST_IDR in flags, frame 0 of every context is analysed as an IDR picture

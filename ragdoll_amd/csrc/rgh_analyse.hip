@@ -105,6 +105,10 @@ __device__ __forceinline__ uint32_t decimate_mask64(uint64_t mask, int big) {
     return big ? 9u : s;
 }
 
+// Raster position of zig-zag scan index k (0..15) of a 4x4 block (the c_zigzag4 table, one
+// nibble per index), without a memory lookup for a lane-varying k.
+__device__ __forceinline__ int zz4(int k) { return (int)((0xFEB7ADC963258410ull >> (4 * k)) & 15u); }
+
 // Value of row r (0..3) of the lane's quad (DPP quad broadcast).
 template <int R>
 __device__ __forceinline__ int quad_row(int v) {
@@ -1799,7 +1803,7 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y,
             for (int i = lane; i < 256; i += 64) lv[i] = S.q8[i >> 6][c_zigzag8[i & 63]];
             nnz = (cbpl & 1 ? 0x0033u : 0) | (cbpl & 2 ? 0x00CCu : 0) | (cbpl & 4 ? 0x3300u : 0) | (cbpl & 8 ? 0xCC00u : 0);
         } else {
-            for (int i = lane; i < 256; i += 64) lv[i] = S.coef[i >> 4][c_zigzag4[i & 15]];
+            for (int i = lane; i < 256; i += 64) lv[i] = S.coef[i >> 4][zz4(i & 15)];
             int nzb = 0;
             if (lane < 16 && !(mb_type == 2 && !cbpl)) {
 #pragma unroll
@@ -1810,14 +1814,17 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y,
             for (int b = 0; b < 16; b++)
                 if ((bl >> b) & 1) nnz |= 1u << (blk_y(b) * 4 + blk_x(b));
         }
-        if (lane < 16) lv[256 + lane] = mb_type == 2 ? S.dcq[c_zigzag4[lane]] : 0;
+        if (lane < 16) lv[256 + lane] = mb_type == 2 ? S.dcq[zz4(lane)] : 0;
         if (lane < 8) lv[272 + lane] = S.cdc[lane >> 2][lane & 3];
         for (int i = lane; i < 128; i += 64) {
             const int p = i >> 6, b = (i >> 4) & 3, k = i & 15;
-            lv[280 + i] = (k == 0 || cbpc < 2) ? 0 : S.craster[p][b][c_zigzag4[k]];
+            lv[280 + i] = (k == 0 || cbpc < 2) ? 0 : S.craster[p][b][zz4(k)];
         }
     } else {
-        for (int i = lane; i < RGH_LEVELS_PER_MB; i += 64) lv[i] = 0;
+        if (lane < RGH_LEVELS_PER_MB / 8) {  // 816 bytes, 16 a lane
+            ((RGH_G uint64_t *)lv)[2 * lane] = 0;
+            ((RGH_G uint64_t *)lv)[2 * lane + 1] = 0;
+        }
     }
     if (lane == 0) {
         D.mb_type = (uint8_t)mb_type;
@@ -2169,12 +2176,12 @@ __device__ __forceinline__ void analyse_intra_mb(const RghJobD &J, int mb_x, int
     {
         int nzb = 0;
         if (mb_type == 3) {
-            for (int i = lane; i < 256; i += 64) lv[i] = S.q4i[i >> 4][c_zigzag4[i & 15]];
+            for (int i = lane; i < 256; i += 64) lv[i] = S.q4i[i >> 4][zz4(i & 15)];
             if (lane < 16)
 #pragma unroll
                 for (int i = 0; i < 16; i++) nzb |= S.q4i[lane][i] != 0;
         } else {
-            for (int i = lane; i < 256; i += 64) lv[i] = S.coef[i >> 4][c_zigzag4[i & 15]];
+            for (int i = lane; i < 256; i += 64) lv[i] = S.coef[i >> 4][zz4(i & 15)];
             if (lane < 16 && cbpl)
 #pragma unroll
                 for (int i = 0; i < 16; i++) nzb |= S.coef[lane][i] != 0;
@@ -2184,11 +2191,11 @@ __device__ __forceinline__ void analyse_intra_mb(const RghJobD &J, int mb_x, int
         for (int b = 0; b < 16; b++)
             if ((bl >> b) & 1) nnz |= 1u << (blk_y(b) * 4 + blk_x(b));
     }
-    if (lane < 16) lv[256 + lane] = mb_type == 2 ? S.dcq[c_zigzag4[lane]] : 0;
+    if (lane < 16) lv[256 + lane] = mb_type == 2 ? S.dcq[zz4(lane)] : 0;
     if (lane < 8) lv[272 + lane] = S.cdc[lane >> 2][lane & 3];
     for (int i = lane; i < 128; i += 64) {
         const int p = i >> 6, b = (i >> 4) & 3, k = i & 15;
-        lv[280 + i] = (k == 0 || cbpc < 2) ? 0 : S.craster[p][b][c_zigzag4[k]];
+        lv[280 + i] = (k == 0 || cbpc < 2) ? 0 : S.craster[p][b][zz4(k)];
     }
     if (lane == 0) {
         rgh_mb_decision D;

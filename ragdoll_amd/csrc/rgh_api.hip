@@ -664,6 +664,8 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
             rgh_status s = check_i420(&src[q], c->width, c->height);
             if (s != RGH_OK) return s;
             if (!decisions[q] || !levels[q]) return RGH_ERR_INVALID_ARGUMENT;
+            // The kernels store records and clear levels as 64-bit words.
+            if (((uintptr_t)decisions[q] & 7) || ((uintptr_t)levels[q] & 7)) return RGH_ERR_INVALID_ARGUMENT;
         }
         cs[(size_t)i] = c;
     }

@@ -1049,12 +1049,12 @@ __device__ bool analyse_compact_mb(const RghJobD &J, int mb_x, int mb_y, Compact
     if (mb_type != 0) {
         int nzb = 0;
         if (mb_type == 3) {
-            for (int i = lane; i < 256; i += 64) lv[i] = S.a.q4i[i >> 4][c_zigzag4[i & 15]];
+            for (int i = lane; i < 256; i += 64) lv[i] = S.a.q4i[i >> 4][zz4(i & 15)];
             if (lane < 16)
 #pragma unroll
                 for (int i = 0; i < 16; i++) nzb |= S.a.q4i[lane][i] != 0;
         } else {
-            for (int i = lane; i < 256; i += 64) lv[i] = S.a.coef[i >> 4][c_zigzag4[i & 15]];
+            for (int i = lane; i < 256; i += 64) lv[i] = S.a.coef[i >> 4][zz4(i & 15)];
             if (lane < 16 && !(mb_type == 2 && !cbpl))
 #pragma unroll
                 for (int i = 0; i < 16; i++) nzb |= S.a.coef[lane][i] != 0;
@@ -1063,14 +1063,17 @@ __device__ bool analyse_compact_mb(const RghJobD &J, int mb_x, int mb_y, Compact
 #pragma unroll
         for (int b = 0; b < 16; b++)
             if ((bl >> b) & 1) nnz |= 1u << (blk_y(b) * 4 + blk_x(b));
-        if (lane < 16) lv[256 + lane] = mb_type == 2 ? S.a.dcq[c_zigzag4[lane]] : 0;
+        if (lane < 16) lv[256 + lane] = mb_type == 2 ? S.a.dcq[zz4(lane)] : 0;
         if (lane < 8) lv[272 + lane] = S.a.cdc[lane >> 2][lane & 3];
         for (int i = lane; i < 128; i += 64) {
             const int p = i >> 6, b = (i >> 4) & 3, k = i & 15;
-            lv[280 + i] = (k == 0 || cbpc < 2) ? 0 : S.a.craster[p][b][c_zigzag4[k]];
+            lv[280 + i] = (k == 0 || cbpc < 2) ? 0 : S.a.craster[p][b][zz4(k)];
         }
     } else {
-        for (int i = lane; i < RGH_LEVELS_PER_MB; i += 64) lv[i] = 0;
+        if (lane < RGH_LEVELS_PER_MB / 8) {  // 816 bytes, 16 a lane
+            ((RGH_G uint64_t *)lv)[2 * lane] = 0;
+            ((RGH_G uint64_t *)lv)[2 * lane + 1] = 0;
+        }
     }
     if (lane == 0) {
         D.mb_type = (uint8_t)mb_type;
