@@ -126,6 +126,41 @@ __device__ __forceinline__ int sel4(int y, int a, int b, int c, int d) {
     return (a & -(int)(y == 0)) | (b & -(int)(y == 1)) | (c & -(int)(y == 2)) | (d & -(int)(y == 3));
 }
 
+// Column pass of a 4x4 transform across the quad: output row y = m0 v0 + m1 v1 + m2 v2 + m3 v3
+// over the quad's row values v0..v3 (DPP broadcasts) with the lane's multipliers, as 24-bit
+// multiply-adds (every operand here stays within 24 bits).
+struct QuadMix {
+    int m0, m1, m2, m3;
+};
+__device__ __forceinline__ int quad_mix(int t, const QuadMix &m) {
+    const int v0 = quad_row<0>(t), v1 = quad_row<1>(t), v2 = quad_row<2>(t), v3 = quad_row<3>(t);
+    return __mul24(v0, m.m0) + __mul24(v1, m.m1) + __mul24(v2, m.m2) + __mul24(v3, m.m3);
+}
+// forward_dct4x4 (simd/fixed.rs:104) columns: rows (1,1,1,1), (2,1,-1,-2), (1,-1,-1,1), (1,-2,2,-1).
+__device__ __forceinline__ QuadMix fdct_mix(int y) {
+    return {y == 1 ? 2 : 1, sel4(y, 1, 1, -1, -2), sel4(y, 1, -1, -1, 2), sel4(y, 1, -2, 1, -1)};
+}
+// 4-point Hadamard with outputs (1,1,1,1), (1,1,-1,-1), (1,-1,-1,1), (1,-1,1,-1) (the luma DC
+// transforms, simd/fixed.rs:318 and h264.rs:1645).
+__device__ __forceinline__ QuadMix had_mix(int y) { return {1, y < 2 ? 1 : -1, (y == 0 || y == 3) ? 1 : -1, sel4(y, 1, -1, 1, -1)}; }
+// The SATD's Hadamard output order (simd/h264_enc.rs:167): (1,1,1,1), (1,1,-1,-1), (1,-1,1,-1),
+// (1,-1,-1,1).
+__device__ __forceinline__ QuadMix satd_mix(int y) { return {1, y < 2 ? 1 : -1, (y & 1) ? -1 : 1, (y == 0 || y == 3) ? 1 : -1}; }
+// inverse_idct4x4 (simd/fixed.rs:185) columns with their halved odd inputs: rows f0 + f2 + f1 +
+// (f3 >> 1), f0 - f2 + (f1 >> 1) - f3, f0 - f2 - (f1 >> 1) + f3, f0 + f2 - f1 - (f3 >> 1).
+struct IdctMix {
+    int outer, s1, s2, s3;  // outer: rows 0 and 3
+};
+__device__ __forceinline__ IdctMix idct_mix(int y) {
+    const int outer = y == 0 || y == 3;
+    return {outer, y < 2 ? 1 : -1, outer ? 1 : -1, (y & 1) ? -1 : 1};
+}
+__device__ __forceinline__ int quad_idct(int t, const IdctMix &m) {
+    const int f0 = quad_row<0>(t), f1 = quad_row<1>(t), f2 = quad_row<2>(t), f3 = quad_row<3>(t);
+    const int a = m.outer ? f1 : f1 >> 1, b = m.outer ? f3 >> 1 : f3;
+    return f0 + __mul24(f2, m.s2) + __mul24(a, m.s1) + __mul24(b, m.s3);
+}
+
 // ---------------------------------------------------------------------------------------
 // 4x4 transforms, one row per lane: lanes 4k..4k+3 hold rows 0..3 of block k; the column
 // butterflies cross the quad with quad_row broadcasts. Same arithmetic as the block forms below.
@@ -138,12 +173,9 @@ __device__ __forceinline__ void fdct4_row(const uint8_t *src, const uint8_t *pre
     const int d2 = (int)byte_of(a, 2) - (int)byte_of(b, 2), d3 = (int)byte_of(a, 3) - (int)byte_of(b, 3);
     const int s03 = d0 + d3, s12 = d1 + d2, d03 = d0 - d3, d12 = d1 - d2;
     const int t[4] = {s03 + s12, (d03 << 1) + d12, s03 - s12, d03 - (d12 << 1)};
+    const QuadMix m = fdct_mix(row);
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int v0 = quad_row<0>(t[c]), v1 = quad_row<1>(t[c]), v2 = quad_row<2>(t[c]), v3 = quad_row<3>(t[c]);
-        const int cs03 = v0 + v3, cs12 = v1 + v2, cd03 = v0 - v3, cd12 = v1 - v2;
-        o[c] = (int16_t)sel4(row, cs03 + cs12, (cd03 << 1) + cd12, cs03 - cs12, cd03 - (cd12 << 1));
-    }
+    for (int c = 0; c < 4; c++) o[c] = (int16_t)quad_mix(t[c], m);
 }
 // quant4 on coefficient row `row`: position classes 0 / 2 (even rows), 2 / 1 (odd rows) by column.
 __device__ __forceinline__ void quant4_row(int *o, int row, int qp, int intra) {
@@ -173,11 +205,10 @@ __device__ __forceinline__ void add_idct4_row(uint8_t *dst, int s, int x, int y0
     uint32_t *p = (uint32_t *)&dst[(y0 + row) * s + x];
     const uint32_t w = *p;
     uint32_t n = 0;
+    const IdctMix m = idct_mix(row);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-        const int f0 = quad_row<0>(t[k]), f1 = quad_row<1>(t[k]), f2 = quad_row<2>(t[k]), f3 = quad_row<3>(t[k]);
-        const int g0 = f0 + f2, g1 = f0 - f2, g2 = (f1 >> 1) - f3, g3 = f1 + (f3 >> 1);
-        const int o = (int16_t)((sel4(row, g0 + g3, g1 + g2, g1 - g2, g0 - g3) + 32) >> 6);
+        const int o = (int16_t)((quad_idct(t[k], m) + 32) >> 6);
         n |= (uint32_t)clip255((int)byte_of(w, k) + o) << (8 * k);
     }
     *p = n;
@@ -536,13 +567,10 @@ __device__ uint32_t satd16(const uint8_t *a, const uint8_t *b) {
     const int d2 = (int)byte_of(wa, 2) - (int)byte_of(wb, 2), d3 = (int)byte_of(wa, 3) - (int)byte_of(wb, 3);
     const int s01 = d0 + d1, d01 = d0 - d1, s23 = d2 + d3, d23 = d2 - d3;
     const int t[4] = {s01 + s23, s01 - s23, d01 + d23, d01 - d23};
+    const QuadMix m = satd_mix(row);
     uint32_t part = 0;
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int v0 = quad_row<0>(t[c]), v1 = quad_row<1>(t[c]), v2 = quad_row<2>(t[c]), v3 = quad_row<3>(t[c]);
-        const int p01 = v0 + v1, q01 = v0 - v1, p23 = v2 + v3, q23 = v2 - v3;
-        part += (uint32_t)iabs(sel4(row, p01 + p23, p01 - p23, q01 + q23, q01 - q23));
-    }
+    for (int c = 0; c < 4; c++) part += (uint32_t)iabs(quad_mix(t[c], m));
     return wave_sum(part) >> 1;
 }
 
@@ -859,17 +887,14 @@ __device__ I16Analysis analyse_i16(MbLds &S, int mb_x, int mb_y, uint16_t &n_sat
         thr = 4 * sel4(row, l0 + l1 + l2 + l3, l0 + l1 - l2 - l3, l0 - l1 + l2 - l3, l0 - l1 - l2 + l3);
     }
     uint32_t part[4] = {0, 0, 0, 0};  // V, H, DC, plane
+    const QuadMix sm = satd_mix(row);
 #pragma unroll
     for (int h = 0; h < 4; h++) {
-        const int v0 = quad_row<0>(hs[h]), v1 = quad_row<1>(hs[h]), v2 = quad_row<2>(hs[h]), v3 = quad_row<3>(hs[h]);
-        const int s01 = v0 + v1, d01 = v0 - v1, s23 = v2 + v3, d23 = v2 - v3;
-        const int c = (int16_t)sel4(row, s01 + s23, s01 - s23, d01 + d23, d01 - d23);
+        const int c = (int16_t)quad_mix(hs[h], sm);
         part[0] += (uint32_t)iabs(c - (row == 0 ? tv[h] : 0));
         part[1] += (uint32_t)iabs(c - (h == 0 ? thr : 0));
         part[2] += (uint32_t)iabs(c - ((row == 0 && h == 0) ? 16 * dc : 0));
-        const int u0 = quad_row<0>(hp[h]), u1 = quad_row<1>(hp[h]), u2 = quad_row<2>(hp[h]), u3 = quad_row<3>(hp[h]);
-        const int p01 = u0 + u1, q01 = u0 - u1, p23 = u2 + u3, q23 = u2 - u3;
-        part[3] += (uint32_t)iabs(sel4(row, p01 + p23, p01 - p23, q01 + q23, q01 - q23));
+        part[3] += (uint32_t)iabs(quad_mix(hp[h], sm));
     }
     uint32_t sums[4];
     wave_sum4(part, sums);
@@ -1023,11 +1048,10 @@ __device__ I16Code code_i16(MbLds &S, int mb_x, int mb_y, int qp, int qpc, const
         const int t[4] = {s03 + s12, d03 + d12, s03 - s12, d03 - d12};
         const int qbits = 16 + qp / 6, mf = c_quant4_mf[qp % 6][0], f = (1 << qbits) / 3;
         int dq[4];
+        const QuadMix hm = had_mix(lrow);
 #pragma unroll
         for (int c = 0; c < 4; c++) {
-            const int v0 = quad_row<0>(t[c]), v1 = quad_row<1>(t[c]), v2 = quad_row<2>(t[c]), v3 = quad_row<3>(t[c]);
-            const int cs03 = v0 + v3, cs12 = v1 + v2, cd03 = v0 - v3, cd12 = v1 - v2;
-            const int had = (int16_t)(sel4(lrow, cs03 + cs12, cd03 + cd12, cs03 - cs12, cd03 - cd12) >> 1);
+            const int had = (int16_t)(quad_mix(t[c], hm) >> 1);
             const int l = (iabs(had) * mf + f) >> qbits;
             dq[c] = had < 0 ? -l : l;
         }
@@ -1036,11 +1060,7 @@ __device__ I16Code code_i16(MbLds &S, int mb_x, int mb_y, int qp, int qpc, const
         const int u[4] = {z0 + z3, z0 - z3, z1 - z2, z1 + z2};
         int dr[4];
 #pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const int f0 = quad_row<0>(u[c]), f1 = quad_row<1>(u[c]), f2 = quad_row<2>(u[c]), f3 = quad_row<3>(u[c]);
-            const int y0 = f0 + f2, y1 = f0 - f2, y2 = f1 - f3, y3 = f1 + f3;
-            dr[c] = (int16_t)((sel4(lrow, y0 + y3, y1 + y2, y1 - y2, y0 - y3) * qmul + 128) >> 8);
-        }
+        for (int c = 0; c < 4; c++) dr[c] = (int16_t)((quad_mix(u[c], hm) * qmul + 128) >> 8);
         if (lane < 4) {
             *(uint2 *)&S.dcq[lrow * 4] = make_uint2(((uint32_t)dq[0] & 0xFFFFu) | ((uint32_t)dq[1] << 16),
                                                     ((uint32_t)dq[2] & 0xFFFFu) | ((uint32_t)dq[3] << 16));
@@ -1916,6 +1936,8 @@ __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint3
     const uint32_t qbias = (1u << qshift) / 3u;
     const uint32_t mfe = (uint32_t)c_quant4_mf[qrem][(y & 1) ? 2 : 0], mfo = (uint32_t)c_quant4_mf[qrem][(y & 1) ? 1 : 2];
     const int dqe = c_dequant4_init[qrem][(y & 1) ? 2 : 0] << dsh, dqo = c_dequant4_init[qrem][(y & 1) ? 1 : 2] << dsh;
+    const QuadMix fm = fdct_mix(y);
+    const IdctMix im = idct_mix(y);
     uint32_t total = 24 * lambda;
     modes = 0;
     cbpl = 0;
@@ -1980,10 +2002,7 @@ __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint3
             const int t[4] = {s03 + s12, (d03 << 1) + d12, s03 - s12, d03 - (d12 << 1)};
 #pragma unroll
             for (int c = 0; c < 4; c++) {
-                const int v0 = quad_row<0>(t[c]), v1 = quad_row<1>(t[c]), v2 = quad_row<2>(t[c]), v3 = quad_row<3>(t[c]);
-                const int cs03 = v0 + v3, cs12 = v1 + v2, cd03 = v0 - v3, cd12 = v1 - v2;
-                const int o = sel4(y, cs03 + cs12, (cd03 << 1) + cd12, cs03 - cs12, cd03 - (cd12 << 1));
-                const int cv = (int16_t)o;
+                const int cv = (int16_t)quad_mix(t[c], fm);
                 uint32_t q = ((uint32_t)iabs(cv) * ((c & 1) ? mfo : mfe) + qbias) >> qshift;
                 q = q > 32767u ? 32767u : q;
                 lv[c] = cv < 0 ? -(int)q : (int)q;
@@ -1999,12 +2018,7 @@ __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint3
             const int e0 = c[0] + c[2], e1 = c[0] - c[2], e2 = (c[1] >> 1) - c[3], e3 = c[1] + (c[3] >> 1);
             const int t[4] = {e0 + e3, e1 + e2, e1 - e2, e0 - e3};
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const int f0 = quad_row<0>(t[k]), f1 = quad_row<1>(t[k]), f2 = quad_row<2>(t[k]), f3 = quad_row<3>(t[k]);
-                const int g0 = f0 + f2, g1 = f0 - f2, g2 = (f1 >> 1) - f3, g3 = f1 + (f3 >> 1);
-                const int o = sel4(y, g0 + g3, g1 + g2, g1 - g2, g0 - g3);
-                rec |= (uint32_t)clip255(p[k] + (int16_t)((o + 32) >> 6)) << (8 * k);
-            }
+            for (int k = 0; k < 4; k++) rec |= (uint32_t)clip255(p[k] + (int16_t)((quad_idct(t[k], im) + 32) >> 6)) << (8 * k);
         }
         // The first cheapest mode: the least (cost, mode) key over the quads (costs stay below
         // 2^27; unavailable modes and lanes 36-63 hold the largest key), reduced by DPP within
