@@ -128,13 +128,22 @@ __device__ __forceinline__ int sel4(int y, int a, int b, int c, int d) {
 
 // Column pass of a 4x4 transform across the quad: output row y = m0 v0 + m1 v1 + m2 v2 + m3 v3
 // over the quad's row values v0..v3 (DPP broadcasts) with the lane's multipliers, as 24-bit
-// multiply-adds (every operand here stays within 24 bits).
+// multiply-adds (every operand here stays within 24 bits: row sums of 8-bit residuals, of
+// quantized levels times their scales, or of luma DC values).
+// a * b + c on 24-bit signed a, b (v_mad_i32_i24). Written out because the backend cannot see the
+// range of a broadcast kept behind quad_row's barrier and would pick the quarter-rate 32-bit
+// multiply.
+__device__ __forceinline__ int mad24(int a, int b, int c) {
+    int r;
+    __asm__("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
 struct QuadMix {
     int m0, m1, m2, m3;
 };
 __device__ __forceinline__ int quad_mix(int t, const QuadMix &m) {
     const int v0 = quad_row<0>(t), v1 = quad_row<1>(t), v2 = quad_row<2>(t), v3 = quad_row<3>(t);
-    return __mul24(v0, m.m0) + __mul24(v1, m.m1) + __mul24(v2, m.m2) + __mul24(v3, m.m3);
+    return mad24(v3, m.m3, mad24(v2, m.m2, mad24(v1, m.m1, mad24(v0, m.m0, 0))));
 }
 // forward_dct4x4 (simd/fixed.rs:104) columns: rows (1,1,1,1), (2,1,-1,-2), (1,-1,-1,1), (1,-2,2,-1).
 __device__ __forceinline__ QuadMix fdct_mix(int y) {
@@ -158,7 +167,7 @@ __device__ __forceinline__ IdctMix idct_mix(int y) {
 __device__ __forceinline__ int quad_idct(int t, const IdctMix &m) {
     const int f0 = quad_row<0>(t), f1 = quad_row<1>(t), f2 = quad_row<2>(t), f3 = quad_row<3>(t);
     const int a = m.outer ? f1 : f1 >> 1, b = m.outer ? f3 >> 1 : f3;
-    return f0 + __mul24(f2, m.s2) + __mul24(a, m.s1) + __mul24(b, m.s3);
+    return mad24(b, m.s3, mad24(a, m.s1, mad24(f2, m.s2, f0)));
 }
 
 // ---------------------------------------------------------------------------------------
@@ -185,7 +194,7 @@ __device__ __forceinline__ void quant4_row(int *o, int row, int qp, int intra) {
 #pragma unroll
     for (int c = 0; c < 4; c++) {
         const int v = o[c];
-        uint32_t lv = ((uint32_t)iabs(v) * ((c & 1) ? mfo : mfe) + bias) >> shift;
+        uint32_t lv = (__umul24((uint32_t)iabs(v), (c & 1) ? mfo : mfe) + bias) >> shift;
         lv = lv > 32767u ? 32767u : lv;
         o[c] = v < 0 ? -(int)lv : (int)lv;
     }
@@ -195,7 +204,7 @@ __device__ __forceinline__ void dequant4_row(int *c, int row, int qp) {
     const int sh = qp / 6, rem = qp % 6;
     const int se = c_dequant4_init[rem][(row & 1) ? 2 : 0] << sh, so = c_dequant4_init[rem][(row & 1) ? 1 : 2] << sh;
 #pragma unroll
-    for (int k = 0; k < 4; k++) c[k] = iclamp(c[k] * ((k & 1) ? so : se), -32768, 32767);
+    for (int k = 0; k < 4; k++) c[k] = iclamp(__mul24(c[k], (k & 1) ? so : se), -32768, 32767);
 }
 // Row `row` of inverse_idct4x4 (simd/fixed.rs:185) of the block whose lane holds coefficient
 // row c, added onto row y0 + row of a stride-s LDS block at column x with clipping.
@@ -274,7 +283,7 @@ __device__ __forceinline__ void quant4(int *c, int qp, int intra) {
     for (int i = 0; i < 16; i++) {
         int v = c[i];
         uint32_t mf = pos4(i) == 0 ? mf0 : (pos4(i) == 1 ? mf1 : mf2);
-        uint32_t lv = ((uint32_t)iabs(v) * mf + bias) >> shift;
+        uint32_t lv = (__umul24((uint32_t)iabs(v), mf) + bias) >> shift;
         lv = lv > 32767u ? 32767u : lv;
         c[i] = v < 0 ? -(int)lv : (int)lv;
     }
@@ -296,7 +305,7 @@ __device__ __forceinline__ void dequant4(int *c, int qp) {
     const int sh = qp / 6, rem = qp % 6;
     const int s0 = c_dequant4_init[rem][0] << sh, s1 = c_dequant4_init[rem][1] << sh, s2 = c_dequant4_init[rem][2] << sh;
 #pragma unroll
-    for (int i = 0; i < 16; i++) c[i] = iclamp(c[i] * (pos4(i) == 0 ? s0 : (pos4(i) == 1 ? s1 : s2)), -32768, 32767);
+    for (int i = 0; i < 16; i++) c[i] = iclamp(__mul24(c[i], pos4(i) == 0 ? s0 : (pos4(i) == 1 ? s1 : s2)), -32768, 32767);
 }
 // simd/fixed.rs:185 inverse_idct4x4_scalar added onto a stride-s LDS block with clipping
 // (simd/h264_enc.rs:497 add_idct4x4_8x8_scalar for one block); x a multiple of 4.
@@ -648,7 +657,7 @@ __device__ __forceinline__ void quant_chroma(MbLds &S, int qpc, int intra) {
         int qbits = 16 + qpc / 6, mf = c_quant4_mf[qpc % 6][0], f = (1 << qbits) / 3;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            int l = (iabs(had[i]) * mf + f) >> qbits;
+            int l = (int)((__umul24((uint32_t)iabs(had[i]), (uint32_t)mf) + (uint32_t)f) >> qbits);
             S.cdc[p][i] = (int16_t)(had[i] < 0 ? -l : l);
         }
     }
@@ -1052,7 +1061,7 @@ __device__ I16Code code_i16(MbLds &S, int mb_x, int mb_y, int qp, int qpc, const
 #pragma unroll
         for (int c = 0; c < 4; c++) {
             const int had = (int16_t)(quad_mix(t[c], hm) >> 1);
-            const int l = (iabs(had) * mf + f) >> qbits;
+            const int l = (int)((__umul24((uint32_t)iabs(had), (uint32_t)mf) + (uint32_t)f) >> qbits);
             dq[c] = had < 0 ? -l : l;
         }
         const int qmul = (c_dequant4_init[qp % 6][0] * 16) << (qp / 6 + 2);
@@ -2003,7 +2012,7 @@ __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint3
 #pragma unroll
             for (int c = 0; c < 4; c++) {
                 const int cv = (int16_t)quad_mix(t[c], fm);
-                uint32_t q = ((uint32_t)iabs(cv) * ((c & 1) ? mfo : mfe) + qbias) >> qshift;
+                uint32_t q = (__umul24((uint32_t)iabs(cv), (c & 1) ? mfo : mfe) + qbias) >> qshift;
                 q = q > 32767u ? 32767u : q;
                 lv[c] = cv < 0 ? -(int)q : (int)q;
             }
@@ -2014,7 +2023,7 @@ __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint3
         {
             int c[4];
 #pragma unroll
-            for (int x = 0; x < 4; x++) c[x] = iclamp(lv[x] * ((x & 1) ? dqo : dqe), -32768, 32767);
+            for (int x = 0; x < 4; x++) c[x] = iclamp(__mul24(lv[x], (x & 1) ? dqo : dqe), -32768, 32767);
             const int e0 = c[0] + c[2], e1 = c[0] - c[2], e2 = (c[1] >> 1) - c[3], e3 = c[1] + (c[3] >> 1);
             const int t[4] = {e0 + e3, e1 + e2, e1 - e2, e0 - e3};
 #pragma unroll
