@@ -993,6 +993,50 @@ __device__ ChromaIntra chroma_intra(MbLds &S, int mb_x, int mb_y, uint32_t src_u
     return {cmode, (uint32_t)pu, (uint32_t)pv};
 }
 
+// Output stores of a macroblock, as dwords and 64-bit words (the level buffer is 8-byte aligned,
+// every macroblock's levels start at a multiple of 816 bytes).
+// Chroma reconstruction from S.predu / S.predv: lanes 0-31, one 4-byte half row each (plane
+// lane >> 4, row (lane >> 1) & 7).
+__device__ __forceinline__ void store_chroma(const MbLds &S, RGH_G uint8_t *u, RGH_G uint8_t *v, long stride, int cx0,
+                                             int cy0) {
+    const int lane = threadIdx.x;
+    if (lane < 32) {
+        const int p = lane >> 4, row = (lane >> 1) & 7, half = lane & 1;
+        const uint32_t w = *(const uint32_t *)&(p ? S.predv : S.predu)[row * 8 + half * 4];
+        *(RGH_G uint32_t *)&(p ? v : u)[(long)(cy0 + row) * stride + cx0 + half * 4] = w;
+    }
+}
+__device__ __forceinline__ uint64_t pack4x16(int a, int b, int c, int d) {
+    return (uint64_t)((uint32_t)(uint16_t)a | ((uint32_t)(uint16_t)b << 16)) |
+           ((uint64_t)((uint32_t)(uint16_t)c | ((uint32_t)(uint16_t)d << 16)) << 32);
+}
+// Luma levels [0, 256): 16 blocks (decoding order) of 16 zig-zag levels from raster-ordered LDS
+// blocks; lane l stores levels 4l .. 4l + 3.
+__device__ __forceinline__ void store_luma_levels(RGH_G int16_t *lv, const int16_t (*blk)[16]) {
+    const int lane = threadIdx.x, b = lane >> 2, k = (lane & 3) * 4;
+    ((RGH_G uint64_t *)lv)[lane] = pack4x16(blk[b][zz4(k)], blk[b][zz4(k + 1)], blk[b][zz4(k + 2)], blk[b][zz4(k + 3)]);
+}
+// Levels [256, 408): the luma DC (I_16x16, zig-zag), the chroma DC (U then V), and the chroma AC
+// (U then V, 4 blocks of 16 zig-zag levels, slot 0 empty; zero without chroma AC).
+__device__ __forceinline__ void store_dc_chroma_levels(RGH_G int16_t *lv, const MbLds &S, bool i16, int cbpc) {
+    const int lane = threadIdx.x;
+    if (lane < 4) {
+        const int k = 4 * lane;
+        ((RGH_G uint64_t *)(lv + 256))[lane] =
+            i16 ? pack4x16(S.dcq[zz4(k)], S.dcq[zz4(k + 1)], S.dcq[zz4(k + 2)], S.dcq[zz4(k + 3)]) : 0ull;
+    } else if (lane < 6) {
+        const int p = lane - 4;
+        ((RGH_G uint64_t *)(lv + 272))[p] = pack4x16(S.cdc[p][0], S.cdc[p][1], S.cdc[p][2], S.cdc[p][3]);
+    } else if (lane >= 32) {
+        const int l = lane - 32, p = l >> 4, b = (l >> 2) & 3, k = (l & 3) * 4;
+        uint64_t w = 0;
+        if (cbpc >= 2)
+            w = pack4x16(k == 0 ? 0 : S.craster[p][b][zz4(k)], S.craster[p][b][zz4(k + 1)], S.craster[p][b][zz4(k + 2)],
+                         S.craster[p][b][zz4(k + 3)]);
+        ((RGH_G uint64_t *)(lv + 280))[l] = w;
+    }
+}
+
 struct I16Code {
     int cbpl, cbpc, cmode;
     uint32_t pu, pv;
@@ -1817,12 +1861,10 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y,
     }
     __syncthreads();
     *(uint32_t *)&J.cur_y[(long)(y0 + rr) * J.sy + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
-    J.cur_u[(long)(cy0 + (lane >> 3)) * J.sc + cx0 + (lane & 7)] = S.predu[lane];
-    J.cur_v[(long)(cy0 + (lane >> 3)) * J.sc + cx0 + (lane & 7)] = S.predv[lane];
+    store_chroma(S, J.cur_u, J.cur_v, J.sc, cx0, cy0);
     if (J.pre_y) {
         *(uint32_t *)&J.pre_y[(long)(y0 + rr) * W + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
-        J.pre_u[(long)(cy0 + (lane >> 3)) * (W / 2) + cx0 + (lane & 7)] = S.predu[lane];
-        J.pre_v[(long)(cy0 + (lane >> 3)) * (W / 2) + cx0 + (lane & 7)] = S.predv[lane];
+        store_chroma(S, J.pre_u, J.pre_v, W / 2, cx0, cy0);
     }
 
     RGH_G int16_t *lv = J.levels + (long)idx * RGH_LEVELS_PER_MB;
@@ -1832,7 +1874,7 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y,
             for (int i = lane; i < 256; i += 64) lv[i] = S.q8[i >> 6][c_zigzag8[i & 63]];
             nnz = (cbpl & 1 ? 0x0033u : 0) | (cbpl & 2 ? 0x00CCu : 0) | (cbpl & 4 ? 0x3300u : 0) | (cbpl & 8 ? 0xCC00u : 0);
         } else {
-            for (int i = lane; i < 256; i += 64) lv[i] = S.coef[i >> 4][zz4(i & 15)];
+            store_luma_levels(lv, S.coef);
             int nzb = 0;
             if (lane < 16 && !(mb_type == 2 && !cbpl)) {
 #pragma unroll
@@ -1843,12 +1885,7 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y,
             for (int b = 0; b < 16; b++)
                 if ((bl >> b) & 1) nnz |= 1u << (blk_y(b) * 4 + blk_x(b));
         }
-        if (lane < 16) lv[256 + lane] = mb_type == 2 ? S.dcq[zz4(lane)] : 0;
-        if (lane < 8) lv[272 + lane] = S.cdc[lane >> 2][lane & 3];
-        for (int i = lane; i < 128; i += 64) {
-            const int p = i >> 6, b = (i >> 4) & 3, k = i & 15;
-            lv[280 + i] = (k == 0 || cbpc < 2) ? 0 : S.craster[p][b][zz4(k)];
-        }
+        store_dc_chroma_levels(lv, S, mb_type == 2, cbpc);
     } else {
         if (lane < RGH_LEVELS_PER_MB / 8) {  // 816 bytes, 16 a lane
             ((RGH_G uint64_t *)lv)[2 * lane] = 0;
@@ -2187,24 +2224,22 @@ __device__ __forceinline__ void analyse_intra_mb(const RghJobD &J, int mb_x, int
     }
     __syncthreads();
     *(RGH_G uint32_t *)&J.cur_y[(long)(y0 + rr) * J.sy + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
-    J.cur_u[(long)(cy0 + (lane >> 3)) * J.sc + cx0 + (lane & 7)] = S.predu[lane];
-    J.cur_v[(long)(cy0 + (lane >> 3)) * J.sc + cx0 + (lane & 7)] = S.predv[lane];
+    store_chroma(S, J.cur_u, J.cur_v, J.sc, cx0, cy0);
     if (J.pre_y) {
         *(RGH_G uint32_t *)&J.pre_y[(long)(y0 + rr) * W + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
-        J.pre_u[(long)(cy0 + (lane >> 3)) * (W / 2) + cx0 + (lane & 7)] = S.predu[lane];
-        J.pre_v[(long)(cy0 + (lane >> 3)) * (W / 2) + cx0 + (lane & 7)] = S.predv[lane];
+        store_chroma(S, J.pre_u, J.pre_v, W / 2, cx0, cy0);
     }
     RGH_G int16_t *lv = J.levels + (long)idx * RGH_LEVELS_PER_MB;
     uint32_t nnz = 0;
     {
         int nzb = 0;
         if (mb_type == 3) {
-            for (int i = lane; i < 256; i += 64) lv[i] = S.q4i[i >> 4][zz4(i & 15)];
+            store_luma_levels(lv, S.q4i);
             if (lane < 16)
 #pragma unroll
                 for (int i = 0; i < 16; i++) nzb |= S.q4i[lane][i] != 0;
         } else {
-            for (int i = lane; i < 256; i += 64) lv[i] = S.coef[i >> 4][zz4(i & 15)];
+            store_luma_levels(lv, S.coef);
             if (lane < 16 && cbpl)
 #pragma unroll
                 for (int i = 0; i < 16; i++) nzb |= S.coef[lane][i] != 0;
@@ -2214,12 +2249,7 @@ __device__ __forceinline__ void analyse_intra_mb(const RghJobD &J, int mb_x, int
         for (int b = 0; b < 16; b++)
             if ((bl >> b) & 1) nnz |= 1u << (blk_y(b) * 4 + blk_x(b));
     }
-    if (lane < 16) lv[256 + lane] = mb_type == 2 ? S.dcq[zz4(lane)] : 0;
-    if (lane < 8) lv[272 + lane] = S.cdc[lane >> 2][lane & 3];
-    for (int i = lane; i < 128; i += 64) {
-        const int p = i >> 6, b = (i >> 4) & 3, k = i & 15;
-        lv[280 + i] = (k == 0 || cbpc < 2) ? 0 : S.craster[p][b][zz4(k)];
-    }
+    store_dc_chroma_levels(lv, S, mb_type == 2, cbpc);
     if (lane == 0) {
         rgh_mb_decision D;
         memset(&D, 0, sizeof(D));
