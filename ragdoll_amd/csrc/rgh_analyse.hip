@@ -1037,6 +1037,20 @@ __device__ __forceinline__ void store_dc_chroma_levels(RGH_G int16_t *lv, const 
     }
 }
 
+// The output buffers of a job, read again from its descriptor when a macroblock stores its
+// results: the barrier on the descriptor's address makes the reads happen there, so these
+// pointers do not hold scalar registers through the analysis.
+struct JobOut {
+    RGH_G uint8_t *cur_y, *cur_u, *cur_v, *pre_y, *pre_u, *pre_v;
+    RGH_G int16_t *levels;
+    RGH_G rgh_mb_decision *dec;
+};
+__device__ __forceinline__ JobOut late_outputs(const RghJob *jp) {
+    const RGH_G RghJob *g = (const RGH_G RghJob *)jp;
+    __asm__ volatile("" : "+s"(g));
+    return {gp(g->cur_y), gp(g->cur_u), gp(g->cur_v), gp(g->pre_y), gp(g->pre_u), gp(g->pre_v), gp(g->levels), gp(g->dec)};
+}
+
 struct I16Code {
     int cbpl, cbpc, cmode;
     uint32_t pu, pv;
@@ -1150,7 +1164,7 @@ __device__ I16Code code_i16(MbLds &S, int mb_x, int mb_y, int qp, int qpc, const
 // the intra 16x16 check only when the intra probability reaches learn_intra_gate / 255. Costs,
 // sub-pel refinement, residual coding and reconstruction are the plain path's.
 template <bool LEARNED>
-__device__ __forceinline__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S, uint32_t seq) {
+__device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, int mb_x, int mb_y, MbLds &S, uint32_t seq) {
     const int lane = threadIdx.x;
     const int rr = lane >> 2, c0 = (lane & 3) * 4;
     const int mb_w = J.mb_w, idx = mb_y * mb_w + mb_x;
@@ -1860,14 +1874,15 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y,
         cbpl = cbpc = 0;
     }
     __syncthreads();
-    *(uint32_t *)&J.cur_y[(long)(y0 + rr) * J.sy + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
-    store_chroma(S, J.cur_u, J.cur_v, J.sc, cx0, cy0);
-    if (J.pre_y) {
-        *(uint32_t *)&J.pre_y[(long)(y0 + rr) * W + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
-        store_chroma(S, J.pre_u, J.pre_v, W / 2, cx0, cy0);
+    const JobOut O = late_outputs(jp);
+    *(uint32_t *)&O.cur_y[(long)(y0 + rr) * J.sy + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
+    store_chroma(S, O.cur_u, O.cur_v, J.sc, cx0, cy0);
+    if (O.pre_y) {
+        *(uint32_t *)&O.pre_y[(long)(y0 + rr) * W + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
+        store_chroma(S, O.pre_u, O.pre_v, W / 2, cx0, cy0);
     }
 
-    RGH_G int16_t *lv = J.levels + (long)idx * RGH_LEVELS_PER_MB;
+    RGH_G int16_t *lv = O.levels + (long)idx * RGH_LEVELS_PER_MB;
     uint32_t nnz = 0;
     if (mb_type == 1 || mb_type == 2) {
         if (mb_type == 1 && use8) {
@@ -1902,7 +1917,7 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, int mb_x, int mb_y,
         D.mv[0] = (int16_t)mvx;
         D.mv[1] = (int16_t)mvy;
         D.nnz = (uint16_t)nnz;
-        store_dec(&J.dec[idx], D);
+        store_dec(&O.dec[idx], D);
     }
     return true;
 }
@@ -2092,7 +2107,7 @@ __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint3
 }
 
 // One IDR macroblock: decision record, levels and reconstruction.
-__device__ __forceinline__ void analyse_intra_mb(const RghJobD &J, int mb_x, int mb_y, MbLds &S) {
+__device__ __forceinline__ void analyse_intra_mb(const RghJobD &J, const RghJob *jp, int mb_x, int mb_y, MbLds &S) {
     const int lane = threadIdx.x, rr = lane >> 2, c0 = (lane & 3) * 4;
     const int mb_w = J.mb_w, idx = mb_y * mb_w + mb_x;
     const int x0 = mb_x * 16, y0 = mb_y * 16, cx0 = mb_x * 8, cy0 = mb_y * 8;
@@ -2223,13 +2238,14 @@ __device__ __forceinline__ void analyse_intra_mb(const RghJobD &J, int mb_x, int
         cmode = ic.cmode;
     }
     __syncthreads();
-    *(RGH_G uint32_t *)&J.cur_y[(long)(y0 + rr) * J.sy + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
-    store_chroma(S, J.cur_u, J.cur_v, J.sc, cx0, cy0);
-    if (J.pre_y) {
-        *(RGH_G uint32_t *)&J.pre_y[(long)(y0 + rr) * W + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
-        store_chroma(S, J.pre_u, J.pre_v, W / 2, cx0, cy0);
+    const JobOut O = late_outputs(jp);
+    *(RGH_G uint32_t *)&O.cur_y[(long)(y0 + rr) * J.sy + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
+    store_chroma(S, O.cur_u, O.cur_v, J.sc, cx0, cy0);
+    if (O.pre_y) {
+        *(RGH_G uint32_t *)&O.pre_y[(long)(y0 + rr) * W + x0 + c0] = *(const uint32_t *)&S.pred[lane * 4];
+        store_chroma(S, O.pre_u, O.pre_v, W / 2, cx0, cy0);
     }
-    RGH_G int16_t *lv = J.levels + (long)idx * RGH_LEVELS_PER_MB;
+    RGH_G int16_t *lv = O.levels + (long)idx * RGH_LEVELS_PER_MB;
     uint32_t nnz = 0;
     {
         int nzb = 0;
@@ -2261,7 +2277,7 @@ __device__ __forceinline__ void analyse_intra_mb(const RghJobD &J, int mb_x, int
         D.qp = (uint8_t)qp;
         if (mb_type == 3) __builtin_memcpy(D.mv, &modes, sizeof(modes));
         D.nnz = (uint16_t)nnz;
-        store_dec(&J.dec[idx], D);
+        store_dec(&O.dec[idx], D);
     }
 }
 
@@ -2326,17 +2342,18 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
         bool ok = true;
         if (s == 0 && J.wait_analysed) ok = __atomic_load_n(J.wait_analysed, __ATOMIC_RELAXED) >= J.wait_total;
         if (ok) {
+            const RghJob *jp = jobs + j0 + (int)blockIdx.y;
             if constexpr (MODE == 1) {
-                analyse_intra_mb(J, x, y, S.a);
+                analyse_intra_mb(J, jp, x, y, S.a);
             } else if constexpr (MODE == 4 || MODE == 5) {
                 if (J.intra)
-                    analyse_intra_mb(J, x, y, S.a);
+                    analyse_intra_mb(J, jp, x, y, S.a);
                 else
-                    ok = analyse_mb<MODE == 5>(J, x, y, S.a, seq);
+                    ok = analyse_mb<MODE == 5>(J, jp, x, y, S.a, seq);
             } else if constexpr (MODE == 2)
                 ok = analyse_compact_mb(J, x, y, S.a, seq);
             else
-                ok = analyse_mb<MODE == 3>(J, x, y, S.a, seq);
+                ok = analyse_mb<MODE == 3>(J, jp, x, y, S.a, seq);
         }
         if (lane == 0) {
             if (ok) {
