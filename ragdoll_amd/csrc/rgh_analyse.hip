@@ -2356,11 +2356,15 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
                 ok = analyse_mb<MODE == 3>(J, jp, x, y, S.a, seq);
         }
         if (lane == 0) {
+            // The stamp and progress buffers, read again from the descriptor (as late_outputs).
+            const RGH_G RghJob *g = (const RGH_G RghJob *)(jobs + j0 + (int)blockIdx.y);
+            __asm__ volatile("" : "+s"(g));
+            RGH_G RghFrameCtl *lctl = gp(g->ctl);
             if (ok) {
-                J.ana[m] = make_stamp(J.serial, seq);
-                atomicAdd((int *)&ctl->analysed, 1);
+                gp(g->ana)[m] = make_stamp(g->serial, seq);
+                atomicAdd((int *)&lctl->analysed, 1);
             } else {
-                atomicMax(next_delay, delay + 1);
+                atomicMax((int *)&lctl->delay[(launch + 1) & 1], delay + 1);
             }
         }
     } else {
