@@ -138,3 +138,30 @@ def test_pipelined_gops_from_idr(lag, groups):
             a.close()
         for b in bufs:
             hip.free(b)
+
+
+def test_misaligned_output_buffers_are_rejected():
+    # rgh_analyse_frames stores records and clears levels as 64-bit words: buffers that are not
+    # 8-byte aligned are refused before any work starts; aligned ones are analysed.
+    from ragdoll_amd import RghError
+    hip = Hip()
+    frames, w, h = cif(1, config_seed(4, 0))
+    y, u, v = frames[0]
+    a = StreamAnalyzer(w, h, 27, ring=3)
+    n = a.mb_w * a.mb_h
+    src = (hip.upload(y), hip.upload(u), hip.upload(v))
+    d, l = hip.alloc(n * 32 + 16), hip.alloc(n * LEVELS_PER_MB * 2 + 16)
+    frame = [[(src[0], src[1], src[2], y.shape[1], u.shape[1], v.shape[1])]]
+    try:
+        print("misaligned output buffers", flush=True)
+        with pytest.raises(RghError):
+            analyse_frames([a], frame, [[d]], [[l + 2]], first_idr=True)
+        with pytest.raises(RghError):
+            analyse_frames([a], frame, [[d + 4]], [[l]], first_idr=True)
+        analyse_frames([a], frame, [[d]], [[l]], first_idr=True)
+        dec = hip.download(d, np.zeros(n, DECISION_DTYPE))
+        assert (dec["mb_type"] >= 2).all()  # an IDR picture: I_16x16 or I_4x4 throughout
+    finally:
+        a.close()
+        for q in list(src) + [d, l]:
+            hip.free(q)
