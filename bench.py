@@ -2,7 +2,8 @@
 
 Workload (the plain GPU form at scale): 1920x1080 I420, 120 frames of seeded synthetic content
 (20 objects, |v| <= 32 px/frame, 10% scene cuts, noise sigma 3), High profile decisions
-(adaptive 8x8 transform), QP 24/30/36, keyint 30: 3 QPs x 4 GOPs = 12 independent GOP streams,
+(adaptive 8x8 transform), QP 24/30/36, search range +-64 (64 diamond iterations, quarter-pel
+refinement), keyint 30: 3 QPs x 4 GOPs = 12 independent GOP streams,
 each an IDR then 29 P frames. A step is the whole job: all 30 frames of the 12 streams (12 IDR
 and 348 P frames) analysed by one pipelined schedule (rgh_analyse_frames with each stream's first
 frame its IDR): every macroblock's decision, quantized levels and reconstruction, and the
@@ -42,6 +43,7 @@ HBM_PEAK_GBS = 8000.0
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 GOP = 30
 QPS = (24, 30, 36)
+ME_RANGE = 64  # the headline's search range: +-64 (64 diamond iterations)
 W, H = 1920, 1080
 
 
@@ -112,27 +114,46 @@ def render_frames(n_frames):
     return [g.next_frame() for _ in range(n_frames)]
 
 
-def cpu_baseline(frames, seconds=12.0):
-    """The CPU oracle (the reference's Balanced encoder restated in C, built -O3) on a bounded
-    sample of the same workload: the 12 GOP streams on the host cores, one thread per stream
-    (like the reference's parallel_gop_map), the streams repeated until `seconds` pass. Each
-    thread times its IDR and P frames; the value is the sum of the threads' P-macroblock rates
-    over that time (the GPU figure's definition)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(frames, seconds=12.0, single_seconds=5.0, max_threads=16):
+    """The CPU oracle (the reference's Balanced encoder restated in C, built -O3 for x86-64-v3, the
+    reference's AVX2 level) on a bounded sample of the same workload (1080p, High decisions, QP
+    24/30/36, +-64 search), timed on this host's cores:
+    * single_thread: one thread coding the first 8 frames of each GOP stream in turn for
+      `single_seconds`;
+    * all_cores: one thread per usable core (up to `max_threads`, the host share of one GPU), each
+      coding its own GOP streams (the 12 streams dealt round-robin, repeated), like the
+      reference's parallel_gop_map, for `seconds`.
+    Each thread times its IDR and P frames; a figure is the sum of the threads' P-macroblock
+    rates over that time (the GPU figure's definition)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_lib import OracleEncoder
-    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     streams = [(qp, g) for qp in QPS for g in range(len(frames) // GOP)]
-    threads = min(cores, len(streams))
-    done = [0] * threads
-    busy = [0.0] * threads
-    deadline = time.perf_counter() + seconds
+    mbf = (W // 16) * ((H + 15) // 16)
 
-    def work(k):
-        while True:
-            for s in range(k, len(streams), threads):
-                qp, g = streams[s]
-                enc = OracleEncoder(W, H, qp, profile="high", keyint=1 << 20)
-                for i in range(GOP):
+    def run(threads, secs, frames_per_gop=GOP):
+        done = [0] * threads
+        busy = [0.0] * threads
+        deadline = time.perf_counter() + secs
+
+        def work(k):
+            s = k
+            while True:
+                qp, g = streams[s % len(streams)]
+                s += threads
+                enc = OracleEncoder(W, H, qp, profile="high", keyint=1 << 20, me_range=ME_RANGE)
+                for i in range(frames_per_gop):
                     t = time.perf_counter()
                     if t > deadline:
                         return
@@ -140,20 +161,29 @@ def cpu_baseline(frames, seconds=12.0):
                     busy[k] += time.perf_counter() - t
                     done[k] += i > 0
 
-    t0 = time.perf_counter()
-    ts = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
-    for t in ts:
-        t.start()
-    for t in ts:
-        t.join()
-    el = time.perf_counter() - t0
-    mbf = (W // 16) * ((H + 15) // 16)
-    value = sum(done[k] * mbf / busy[k] for k in range(threads) if busy[k] > 0)
-    return {"value": value, "unit": "macroblocks/s", "cores": threads, "kind": "port",
-            "sample": "%d P frames (and their GOPs' IDR frames) of the 12 streams (1080p, High decisions, "
-                      "QP 24/30/36), one thread per stream, %.1f s wall, by the C oracle (Balanced encoder "
-                      "restated: IDR and P analysis, reconstruction, deblocking; no entropy coding for High)"
-                      % (sum(done), el)}
+        t0 = time.perf_counter()
+        ts = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        el = time.perf_counter() - t0
+        return sum(done[k] * mbf / busy[k] for k in range(threads) if busy[k] > 0), sum(done), el
+
+    # One thread visits every stream (the first 8 frames of each GOP, QP 24/30/36 alike), so the
+    # single-thread sample holds the same content mix as the all-cores one.
+    single, n1, el1 = run(1, single_seconds, frames_per_gop=8)
+    threads = max(1, min(usable, max_threads))
+    allc, n, el = run(threads, seconds)
+    return {"value": round(allc, 1), "unit": "macroblocks/s", "cores": threads, "kind": "port",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "usable_cores": usable,
+            "single_thread": round(single, 1), "all_cores": round(allc, 1),
+            "sample": "C oracle (Balanced encoder restated: IDR and P analysis, reconstruction, deblocking; "
+                      "-O3 -march=x86-64-v3) on the headline workload (1080p, High decisions, QP 24/30/36, "
+                      "+-64 search): single_thread = %d P frames (the first 8 frames of each GOP in turn) on 1 thread "
+                      "in %.1f s; all_cores = "
+                      "%d P frames of the 12 GOP streams on %d threads (one stream at a time each) in %.1f s"
+                      % (n1, el1, n, threads, el)}
 
 
 def main():
@@ -239,7 +269,7 @@ def headline_gops():
     t0 = time.time()
     frames = render_frames(120)
     print("rendered 120 frames in %.1f s" % (time.time() - t0), file=sys.stderr, flush=True)
-    return [dict(w=W, h=H, qp=qp, me=16, frames=frames[g * GOP:(g + 1) * GOP]) for qp in QPS for g in range(4)]
+    return [dict(w=W, h=H, qp=qp, me=ME_RANGE, frames=frames[g * GOP:(g + 1) * GOP]) for qp in QPS for g in range(4)]
 
 
 def batched_gops():
@@ -321,14 +351,15 @@ def run_workload(gops, args):
               "batched": "batched 24-stream ME macroblocks/sec, 1 GPU, plain form",
               "compact": "1080p ME macroblocks/sec, 1 GPU, Compact preset"}[args.workload]
     workload = {
-        "headline": "plain GPU form: 1920x1080 analysis, High decisions, QP 24/30/36, keyint 30, 12 GOP streams x "
-                    "(1 IDR + 29 P frames) per step in one pipelined schedule, bit-exact with the CPU oracle; value "
-                    "counts P macroblocks",
+        "headline": "plain GPU form: 1920x1080 analysis, High decisions, QP 24/30/36, search range +-64 (64 "
+                    "diamond iterations) with quarter-pel refinement, keyint 30, 12 GOP streams x (1 IDR + 29 P "
+                    "frames) per step in one pipelined schedule, bit-exact with the CPU oracle; value counts P "
+                    "macroblocks",
         "batched": "plain GPU form: 24 streams (8 x 640x360, 8 x 1280x720, 8 x 1920x1080), 60 frames each (2 GOPs, "
                    "keyint 30), seeded QP 22-38, search range scaled by width, High decisions, all 48 GOPs (IDR + P "
                    "frames) in one pipelined schedule; value counts P macroblocks",
-        "compact": "Compact preset (hexagon search, 8x8 partitions, I_4x4 in P; Constrained Baseline decisions): "
-                   "1920x1080, QP 24/30/36, keyint 30, 12 GOP streams x (1 IDR + 29 P frames) per step in one "
+        "compact": "Compact preset (hexagon search, range 16, 8x8 partitions, I_4x4 in P; Constrained Baseline "
+                   "decisions): 1920x1080, QP 24/30/36, keyint 30, 12 GOP streams x (1 IDR + 29 P frames) per step in one "
                    "pipelined schedule, bit-exact with the CPU oracle; value counts P macroblocks"}[args.workload]
     kernel = "rgh_k_compact_step" if args.workload == "compact" else "rgh_k_frame_step"
     out = {

@@ -267,7 +267,13 @@ rgh_status rgh_write_headers(const rgh_stream_config *cfg, uint8_t *out, int64_t
 /* Copies the last analysed frame's results to host memory (any pointer may be NULL). After
  * rgh_analyse_frames the levels are read from that call's last levels buffer, which must still
  * be allocated.
- * which_recon: 0 = before deblocking, 1 = deblocked reference. */
+ * which_recon: 0 = before deblocking, 1 = deblocked reference. The undeblocked copy exists only
+ * after the single-frame entry points (rgh_analyse_p_frame, rgh_analyse_idr, rgh_analyse_batch)
+ * of a context created without RGH_FLAG_NO_UNDEBLOCKED_COPY; otherwise which_recon = 0 returns
+ * RGH_ERR_INVALID_ARGUMENT.
+ * An analysis call that fails after its launches may have started leaves its contexts without a
+ * reference (the next P frame returns RGH_ERR_NO_REFERENCE until rgh_set_reference or an IDR) and
+ * without outputs to read (RGH_ERR_NO_REFERENCE here). */
 rgh_status rgh_get_outputs(void *ctx, rgh_mb_decision *decisions, int16_t *levels,
                            const rgh_i420_out *recon, int32_t which_recon);
 

@@ -11,6 +11,7 @@ import shutil
 import subprocess
 import sys
 import tempfile
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -39,31 +40,47 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+def _run(cmd, verbose, quiet=False):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True, stderr=subprocess.DEVNULL if quiet else None)
+
+
+def _jobs():
+    return max(1, min(8, os.cpu_count() or 1))
+
+
 def check_device_code(verbose=False):
     """Compiles every source's device code to assembly and rejects forbidden instructions."""
     with tempfile.TemporaryDirectory() as tmp:
-        for src in SOURCES:
+        def one(src):
             out = os.path.join(tmp, src + ".s")
-            cmd = [hipcc()] + FLAGS + ["--cuda-device-only", "-S", "-I", os.path.join(ROOT, "include"),
-                                       "-o", out, os.path.join(CSRC, src)]
-            if verbose:
-                print(" ".join(cmd), flush=True)
-            subprocess.run(cmd, check=True, stderr=subprocess.DEVNULL)
+            _run([hipcc()] + FLAGS + ["--cuda-device-only", "-S", "-I", os.path.join(ROOT, "include"),
+                                      "-o", out, os.path.join(CSRC, src)], verbose, quiet=True)
             with open(out) as f:
                 text = f.read()
             bad = sorted({m.group(1) for m in FORBIDDEN.finditer(text)} |
                          {m.group(1) + " (broadcast)" for m in FORBIDDEN_DPP.finditer(text)})
             if bad:
                 raise RuntimeError("%s: device code contains %s" % (src, ", ".join(bad)))
+        with ThreadPoolExecutor(_jobs()) as ex:
+            list(ex.map(one, SOURCES))
 
 
 def build_library(verbose=False):
-    cmd = [hipcc()] + FLAGS + ["-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
-                               "-o", LIB + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES + HOST_SOURCES]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    subprocess.run(cmd, check=True)
-    check_device_code(verbose)
+    """Compiles each source to an object (in parallel, with the device-code check beside them),
+    then links the shared library."""
+    inc = ["-I", os.path.join(ROOT, "include")]
+    with tempfile.TemporaryDirectory() as tmp:
+        def obj(src):
+            out = os.path.join(tmp, src + ".o")
+            _run([hipcc()] + FLAGS + ["-fPIC", "-c"] + inc + ["-o", out, os.path.join(CSRC, src)], verbose)
+            return out
+        with ThreadPoolExecutor(_jobs()) as ex:
+            check = ex.submit(check_device_code, verbose)
+            objs = list(ex.map(obj, SOURCES + HOST_SOURCES))
+            check.result()
+        _run([hipcc()] + FLAGS + ["-fPIC", "-shared", "-o", LIB + ".tmp"] + objs, verbose)
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -1149,6 +1149,22 @@ __device__ I16Code code_i16(MbLds &S, int mb_x, int mb_y, int qp, int qpc, const
     return {cbpl, cbpc, cmode, pu, pv};
 }
 
+// Non-zero in some lane when a reference macroblock holding a sample of [px0, px1] x [py0, py1]
+// (picture coordinates, clamped to the picture) is not final for a reader in launch `seq`.
+// Macroblock (a, b) of the reference is final, padding included, once its deblock/pad item
+// (a + 1, b + 1) is stamped by an earlier launch.
+__device__ __forceinline__ int ref_box_pending(const RghJobD &J, int px0, int px1, int py0, int py1, uint32_t seq) {
+    const int ax0 = iclamp(px0 >> 4, 0, J.mb_w - 1), ax1 = iclamp(px1 >> 4, 0, J.mb_w - 1);
+    const int ay0 = iclamp(py0 >> 4, 0, J.mb_h - 1), ay1 = iclamp(py1 >> 4, 0, J.mb_h - 1);
+    const int nx = ax1 - ax0 + 1, n = nx * (ay1 - ay0 + 1);
+    int wait = 0;
+    for (int i = (int)threadIdx.x; i < n; i += 64) {
+        const int a = ax0 + i % nx, b = ay0 + i / nx;
+        wait |= !stamp_final(J.ref_fin[(b + 1) * (J.mb_w + 1) + a + 1], J.ref_serial, seq);
+    }
+    return wait;
+}
+
 // ---------------------------------------------------------------------------------------
 // One macroblock (enc.rs:3011 code_p_mb and everything it calls).
 // ---------------------------------------------------------------------------------------
@@ -1354,14 +1370,14 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
     }
 
     // ---- Round trip 2: readiness stamps, candidate rows and the skip probe's samples. ----
-    // Readiness of the reference (when it may still be in flight): every reference read stays
-    // within [c - 22 - r, c + 68 + r] x [c - 19 - r, c + 36 + r] of some candidate c (the
-    // staged 68x55 search region, the skip-probe window and the chroma taps around a vector at
-    // most 16 + r samples from c), r = 0 for at most 16 diamond iterations, else the iteration
-    // count. The zero candidate keeps this macroblock's co-located decision inside. Macroblock
-    // (a, b) of the reference is final, padding included, once its item (a + 1, b + 1) is
-    // stamped. The candidate rows and probe samples are loaded before the verdict (loads only;
-    // nothing is written before it).
+    // Readiness of the reference (when it may still be in flight): while the diamond stays
+    // within 16 samples of the centre c of the staged region (a candidate, or the point the walk
+    // re-staged at), every reference read stays within [c - 22, c + 68] x [c - 19, c + 36] (the
+    // staged 68x55 search region, the skip-probe window, the half-pel window and the chroma taps
+    // around a vector at most 16 samples from c). This box around every candidate is checked
+    // here; a walk that re-stages checks the box of its new centre then. The zero candidate keeps
+    // this macroblock's co-located decision inside. The candidate rows and probe samples are
+    // loaded before the verdict (loads only; nothing is written before it).
     int wait = 0;
     if (J.ref_fin) {
         int lx = fxs[0], hx = fxs[0], ly = fys[0], hy = fys[0];
@@ -1372,14 +1388,7 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
             ly = min(ly, fys[k]);
             hy = max(hy, fys[k]);
         }
-        const int r = J.me_iters > 16 ? J.me_iters : 0;
-        const int ax0 = iclamp((x0 + lx - 22 - r) >> 4, 0, mb_w - 1), ax1 = iclamp((x0 + hx + 68 + r) >> 4, 0, mb_w - 1);
-        const int ay0 = iclamp((y0 + ly - 19 - r) >> 4, 0, J.mb_h - 1), ay1 = iclamp((y0 + hy + 36 + r) >> 4, 0, J.mb_h - 1);
-        const int nx = ax1 - ax0 + 1, n = nx * (ay1 - ay0 + 1);
-        for (int i = lane; i < n; i += 64) {
-            const int a = ax0 + i % nx, b = ay0 + i / nx;
-            wait |= !stamp_final(J.ref_fin[(b + 1) * (mb_w + 1) + a + 1], J.ref_serial, seq);
-        }
+        wait = ref_box_pending(J, x0 + lx - 22, x0 + hx + 68, y0 + ly - 19, y0 + hy + 36, seq);
     }
     uint32_t rw[NC];
     {
@@ -1482,7 +1491,6 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
     }
     if (mb_type < 0) {
         // ---- Integer candidates: the first strictly cheapest wins. ----
-        const RGH_G uint8_t *rb = J.ref_y + (long)(y0 + rr) * J.sy + x0 + c0;
         int bestx = 0, besty = 0;
         uint32_t best_cost = 0xFFFFFFFFu, bestw = 0;
         int iters = J.me_iters;
@@ -1548,32 +1556,35 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
         } else {
         // ---- Round trip 4: the reference region around the best candidate, in LDS. ----
         // Columns [bx - 19 (aligned down), +68) and rows [by - 19, +55) hold every 16x16 block
-        // the diamond can reach from the candidate in 16 steps and the half-pel window of any of
-        // them; positions beyond it (me_range > 16) read the reference directly.
-        const int bxa = x0 + bestx, bya = y0 + besty;
-        const int ox = (bxa - 19) & ~3, oy = bya - 19;
+        // within 17 samples of the centre (bx, by) and the half-pel window of any of them.
+        int scx = bestx, scy = besty;  // centre of the staged region (full-pel vector)
+        int ox = (x0 + scx - 19) & ~3, oy = y0 + scy - 19;
         stage_region<SR_H>(J, ox, oy, S);
-        auto sad_part = [&](int fx, int fy) {
-            const int px = x0 + fx - ox, py = y0 + fy - oy;
-            uint32_t w;
-            if (px >= 0 && px + 16 <= SR_COLS && py >= 0 && py + 16 <= SR_H)
-                w = lds4u(&S.sr[(py + rr) * SR_W], px + c0);
-            else
-                w = ld4u(rb + (long)fy * J.sy + fx);
-            return __builtin_amdgcn_sad_u8(srcw, w, 0);
-        };
         // Small diamond (enc.rs:1207-1272): the first of the cheapest in-bounds points below the
-        // best moves, until none does or the iterations run out. With at most 16 iterations every
-        // point evaluated lies within 17 samples of the candidate, inside the staged region
-        // (columns 2..39 + 16, rows 2..36 + 16): the four SADs then read LDS without a branch, so
-        // their loads issue together; out-of-bounds points are computed and ignored.
-        const bool in_region = J.me_iters <= 16;
-        // In the region, lane 16 k + r computes row r of point k's SAD (k: up, left, right,
-        // down) from the source row in registers, and each row of 16 lanes sums one point.
+        // best moves, until none does or the iterations run out. An iteration starts at most 15
+        // samples from the staged centre, so its points lie within 16 of it, inside the region
+        // (columns 3..38 + 16, rows 3..35 + 16), and the walk ends within 16 of it: the four SADs
+        // read LDS without a branch, so their loads issue together; out-of-bounds points are
+        // computed and ignored. The reference's 16 iterations never leave the first region; a
+        // longer walk (me_range > 16) stages a new region around its best point once that point is
+        // 16 samples from the centre, after checking that the reference macroblocks its box
+        // reaches are final (else the macroblock defers, having written nothing).
+        // Lane 16 k + r computes row r of point k's SAD (k: up, left, right, down) from the source
+        // row in registers, and each row of 16 lanes sums one point.
         const int dk = lane >> 4, drow = lane & 15;
         const int ddx = dk == 1 ? -1 : (dk == 2 ? 1 : 0), ddy = dk == 0 ? -1 : (dk == 3 ? 1 : 0);
         const uint4 srow = *(const uint4 *)&S.src[drow * 16];
         for (int it = 0; it < iters; it++) {
+            if (iabs(bestx - scx) >= 16 || iabs(besty - scy) >= 16) {
+                if (J.ref_fin && __ballot(ref_box_pending(J, x0 + bestx - 22, x0 + bestx + 68, y0 + besty - 19,
+                                                          y0 + besty + 36, seq)))
+                    return false;
+                scx = bestx;
+                scy = besty;
+                ox = (x0 + scx - 19) & ~3;
+                oy = y0 + scy - 19;
+                stage_region<SR_H>(J, ox, oy, S);
+            }
             const int px[4] = {bestx, bestx - 1, bestx + 1, bestx};
             const int py[4] = {besty - 1, besty, besty, besty + 1};
             int ok[4];
@@ -1581,7 +1592,7 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
             for (int k = 0; k < 4; k++)
                 ok[k] = (int)(px[k] >= imn0) & (int)(px[k] <= imx0) & (int)(py[k] >= imn1) & (int)(py[k] <= imx1);
             uint32_t sums[4];
-            if (in_region) {
+            {
                 const int lx = x0 + bestx + ddx - ox, ly = y0 + besty + ddy - oy + drow;
                 const uint32_t *q = (const uint32_t *)&S.sr[ly * SR_W + (lx & ~3)];
                 const uint32_t f = (uint32_t)(lx & 3);
@@ -1593,11 +1604,6 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
                 sad = row16_sum(sad);
 #pragma unroll
                 for (int k = 0; k < 4; k++) sums[k] = (uint32_t)__builtin_amdgcn_readlane((int)sad, 16 * k);
-            } else {
-                uint32_t part[4];
-#pragma unroll
-                for (int k = 0; k < 4; k++) part[k] = ok[k] ? sad_part(px[k], py[k]) : 0u;
-                wave_sum4(part, sums);
             }
             // The first strictly cheapest in-range point: the least (cost, k) key (costs stay
             // below 2^30).

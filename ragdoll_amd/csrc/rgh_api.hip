@@ -88,6 +88,7 @@ struct Ctx {
     bool has_ref = false;
     bool analysed = false;
     bool keep_pre = true;
+    bool pre_valid = false;  // pre[] holds the last analysed frame's undeblocked reconstruction
     uint32_t sb[5] = {0, 0, 0, 0, 0};
     hipEvent_t ev[2] = {nullptr, nullptr};  // schedule start, schedule end
     bool timing = false, timed = false;
@@ -366,6 +367,17 @@ rgh_status run_schedule(Ctx *const *ctxs, int n, int n_frames, std::vector<RghJo
     return run_schedules(sch, ctxs[0]);
 }
 
+// After a call failed once its launches may have started: stamps of this call's serials may be
+// written and its picture slots (the reference's included, when the ring is short) overwritten,
+// so the context takes no P frame until a new reference or IDR, and later calls use serials
+// beyond every one this call could have stamped.
+void poison(Ctx *c, int n_frames) {
+    c->serial += (uint32_t)n_frames + 1;
+    c->has_ref = false;
+    c->analysed = false;
+    c->pre_valid = false;
+}
+
 // After a call analysed n_frames frames of c: the last one is the reference.
 void advance(Ctx *c, int n_frames, const rgh_mb_decision *last_dec) {
     const int R = (int)c->pic.size();
@@ -591,13 +603,17 @@ rgh_status analyse_single(Ctx *const *cs, int n, const rgh_i420 *src, bool intra
         }
     }
     s = run_schedule(cs, n, 1, jobs, kDefaultLag);
-    if (s != RGH_OK) return s;
+    if (s != RGH_OK) {
+        for (int i = 0; i < n; i++) poison(cs[i], 1);
+        return s;
+    }
     for (int i = 0; i < n; i++) {
         Ctx *c = cs[i];
         advance(c, 1, c->dec[c->dcur]);
         c->last_levels = c->levels;
         c->dcur = 1 - c->dcur;
         c->has_ref = true;
+        c->pre_valid = c->keep_pre;
     }
     return RGH_OK;
 }
@@ -719,7 +735,10 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
     }
     rgh_status s = run_schedules(sch, cs[0]);
     if (go) (void)hipEventDestroy(go);
-    if (s != RGH_OK) return s;
+    if (s != RGH_OK) {
+        for (int i = 0; i < n; i++) poison(cs[(size_t)i], n_frames);
+        return s;
+    }
     // The last frame's decisions carry the co-located vectors into the next call.
     for (int i = 0; i < n; i++) {
         Ctx *c = cs[(size_t)i];
@@ -731,6 +750,7 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
         c->last_levels = levels[(size_t)i * n_frames + n_frames - 1];
         c->dcur = 1 - c->dcur;
         c->has_ref = true;
+        c->pre_valid = false;  // this form keeps no undeblocked copy
     }
     HIP_TRY(hipStreamSynchronize(cs[0]->stream));
     return RGH_OK;
@@ -800,7 +820,11 @@ rgh_status rgh_get_outputs(void *ctx, rgh_mb_decision *decisions, int16_t *level
         const uint8_t *sp[3];
         size_t ss[3];
         if (which == 0) {
-            if (!c->keep_pre) return RGH_ERR_INVALID_ARGUMENT;
+            if (!c->pre_valid) {
+                set_error("rgh_get_outputs", "no undeblocked copy of the last frame (disabled, or the last call was "
+                          "rgh_analyse_frames)", __LINE__);
+                return RGH_ERR_INVALID_ARGUMENT;
+            }
             for (int k = 0; k < 3; k++) sp[k] = c->pre[k];
             ss[0] = (size_t)W;
             ss[1] = ss[2] = (size_t)W / 2;

@@ -649,6 +649,13 @@ __device__ bool analyse_compact_mb(const RghJobD &J, int mb_x, int mb_y, Compact
         const RGH_G uint32_t *dp = (const RGH_G uint32_t *)(cur ? &J.dec[ri] : &J.prev[ri]);
         recw = dp[w];
     }
+    // Records 4-6 are final once the previous picture's deblock/pad item (x + 1, y + 1) is
+    // stamped: it runs after every analysis of that picture up to 6 wavefront steps past this
+    // macroblock's, which include (x, y), (x + 1, y) and (x, y + 1). The candidates' own boxes do
+    // not imply it when every candidate points away from this macroblock.
+    int own_pending = 0;
+    if (lane == 56 && J.ref_fin)
+        own_pending = !stamp_final(J.ref_fin[(mb_y + 1) * (mb_w + 1) + mb_x + 1], J.ref_serial, seq);
     uint32_t ew = 0;
     int eshift = 0;
     if (lane < 44) {
@@ -737,6 +744,8 @@ __device__ bool analyse_compact_mb(const RghJobD &J, int mb_x, int mb_y, Compact
     auto is_skip = [&](int rec) { return (word(rec, 0) & 0xFF) == RGH_MB_P_SKIP; };
     const bool probe = cp_inb(M, pskx, psky) &&
                        ((has_l && is_skip(0)) || (has_t && (is_skip(1) || (has_tl && is_skip(2)) || (has_tr && is_skip(3)))));
+
+    if (__ballot(own_pending)) return false;
 
     rgh_mb_decision D;
     memset(&D, 0, sizeof(D));
