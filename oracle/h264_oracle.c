@@ -1359,12 +1359,11 @@ static void recon_luma_4x4(uint8_t *pred, int16_t q4[16][16], int cbp, int qp) {
         add_idct4_8x8(pred + (i8 >> 1) * 128 + (i8 & 1) * 8, 16, g);
     }
 }
-/* enc.rs:2017 recon_8x8 with h264.rs:1403 dequantize_block8x8 */
-static void recon_8x8(uint8_t *dst, const int16_t *q8, int qp) {
+/* h264.rs:1403 dequantize_block8x8 (flat scaling lists: LevelScale8x8 = 16 x normAdjust8x8) */
+static void dequant8(const int16_t *q8, int qp, int32_t *d) {
     qp = imin(qp, 51);
     int qd = qp / 6;
     const int32_t *scale = DEQUANT8_POS[qp % 6];
-    int32_t d[64];
     for (int i = 0; i < 64; i++) {
         int v;
         if (qd >= 2)
@@ -1375,6 +1374,11 @@ static void recon_8x8(uint8_t *dst, const int16_t *q8, int qp) {
         }
         d[i] = iclamp(v, -32768, 32767);
     }
+}
+/* enc.rs:2017 recon_8x8 */
+static void recon_8x8(uint8_t *dst, const int16_t *q8, int qp) {
+    int32_t d[64];
+    dequant8(q8, qp, d);
     idct8_add(dst, 16, d);
 }
 /* enc.rs:1990 recon_chroma */
@@ -3511,6 +3515,28 @@ void ora_window_pred(uint8_t *dst, const uint8_t *src, int ss, int dx, int dy) {
     winsrc_pred(&s, dst);
 }
 uint32_t ora_decimate_score(const int16_t *scanned, int n) { return decimate_score(scanned, n); }
+void ora_dequant8(const int16_t *levels, int qp, int32_t *out) {
+    init_tables();
+    dequant8(levels, qp, out);
+}
+void ora_idct8(const int32_t *in, int32_t *out) {
+    /* The residual idct8_add adds: rows, then columns, then (x + 32) >> 6. */
+    int32_t g[64], col[8], o[8];
+    for (int i = 0; i < 8; i++) idct8_1d(in + i * 8, g + i * 8);
+    for (int j = 0; j < 8; j++) {
+        for (int i = 0; i < 8; i++) col[i] = g[i * 8 + j];
+        idct8_1d(col, o);
+        for (int i = 0; i < 8; i++) out[i * 8 + j] = (int32_t)((uint32_t)o[i] + 32u) >> 6;
+    }
+}
+int ora_quant8_mf(int rem, int pos) {
+    init_tables();
+    return QUANT8_MF16[rem][pos];
+}
+int ora_dequant8_norm(int rem, int pos) {
+    init_tables();
+    return DEQUANT8_POS[rem][pos];
+}
 int ora_cavlc_block(const int16_t *scanned, int n, int nc, uint8_t *out, int cap) {
     bits_t w;
     bits_init(&w);

@@ -99,6 +99,13 @@ void ora_chroma_mc8(uint8_t *dst, const uint8_t *src, int ss, int fx, int fy);
  * plane, stride ss) at full-pel search centre (0,0) and quarter-pel offset (dx,dy), |d| <= 3. */
 void ora_window_pred(uint8_t *dst, const uint8_t *src, int ss, int dx, int dy);
 uint32_t ora_decimate_score(const int16_t *scanned, int n);
+/* High-profile 8x8 pieces: dequantization of 64 raster-order levels at qp (h264.rs:1403), the
+ * inverse 8x8 transform's residual ((x + 32) >> 6 of the 2-D transform, simd/h264_idct.rs:268),
+ * and the quantizer multiplier / normAdjust8x8 entry of raster position pos for qp % 6 = rem. */
+void ora_dequant8(const int16_t *levels, int qp, int32_t *out);
+void ora_idct8(const int32_t *in, int32_t *out);
+int ora_quant8_mf(int rem, int pos);
+int ora_dequant8_norm(int rem, int pos);
 /* CAVLC residual_block of n (16, 15 or 4) scanned levels at nC; returns bit count and writes
  * the bits MSB-first into out. */
 int ora_cavlc_block(const int16_t *scanned, int n, int nc, uint8_t *out, int cap);

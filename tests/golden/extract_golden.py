@@ -10,6 +10,10 @@ This script reads those files (run it where the reference checkout is present) a
 clip, `<name>.nals`: the SPS, the PPS and every slice NAL unit in decoding order, each as a
 4-byte big-endian length followed by the escaped NAL unit, plus `golden.json` with the clip
 parameters. The tests compare the oracle's output against these bytes.
+
+It also copies qa/corpus/synth/clip_64x36_30fps.y4m (manifest.toml:23-29; raw
+synth_video_frame(64, 36, 0..5), the reference's content generator) as `clip_64x36_30fps.y4m`,
+which pins the test-content generator byte for byte.
 """
 import hashlib
 import json
@@ -180,6 +184,12 @@ def main():
             "sample_sizes": sizes,
         }
         print(name, "frames", len(nals), "bytes", sum(len(n) for n in nals))
+    y4m = open(os.path.join(REF, "qa", "corpus", "synth", "clip_64x36_30fps.y4m"), "rb").read()
+    with open(os.path.join(HERE, "clip_64x36_30fps.y4m"), "wb") as f:
+        f.write(y4m)
+    meta["clip_64x36_30fps"] = {"source": "qa/corpus/synth/clip_64x36_30fps.y4m",
+                                "source_sha256": hashlib.sha256(y4m).hexdigest(), "width": 64, "height": 36,
+                                "frames": 6}
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(meta, f, indent=1)
 

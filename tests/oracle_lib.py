@@ -57,6 +57,10 @@ def lib():
         L.ora_window_pred.argtypes = [P, P, I, I, I]
         L.ora_decimate_score.restype = ctypes.c_uint32
         L.ora_decimate_score.argtypes = [P, I]
+        L.ora_dequant8.argtypes = [P, I, P]
+        L.ora_idct8.argtypes = [P, P]
+        L.ora_quant8_mf.argtypes = [I, I]
+        L.ora_dequant8_norm.argtypes = [I, I]
         L.ora_cavlc_block.restype = I
         L.ora_cavlc_block.argtypes = [P, I, I, P, I]
         L.ora_predict_intra4x4.argtypes = [I, P, P, I, I, I, P]

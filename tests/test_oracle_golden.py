@@ -10,16 +10,19 @@ tests/golden/extract_golden.py.
 import json
 import os
 
+import numpy as np
 import pytest
 
 from oracle_lib import OracleEncoder, read_nals, split_annexb
 from ragdoll_amd.synth import fixture_frame
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-META = json.load(open(os.path.join(HERE, "golden", "golden.json")))
+GOLDEN = os.path.join(HERE, "golden")
+META = json.load(open(os.path.join(GOLDEN, "golden.json")))
+CLIPS = sorted(k for k, m in META.items() if "qp" in m)  # the encoded clips (the y4m is raw content)
 
 
-@pytest.mark.parametrize("name", sorted(META))
+@pytest.mark.parametrize("name", CLIPS)
 def test_oracle_reproduces_golden_clip(name):
     m = META[name]
     gold = read_nals(os.path.join(HERE, "golden", name + ".nals"))
@@ -46,8 +49,30 @@ def test_golden_sample_sizes_match_probe_snapshot():
     assert META["clip_160x90_30fps"]["sample_sizes"] == sizes
 
 
-def test_fixture_generator_matches_reference_y4m_sizes():
-    # qa/corpus/synth/clip_64x36_30fps.y4m holds synth_video_frame(64, 36, 0..5); its frame
-    # size pins the plane geometry (64*36 + 2*32*18 bytes).
-    y, u, v = fixture_frame(64, 36, 3)
-    assert y.shape == (36, 64) and u.shape == (18, 32) and v.shape == (18, 32)
+def read_y4m(path):
+    """Frames (y, u, v) of a 4:2:0 YUV4MPEG2 file."""
+    data = open(path, "rb").read()
+    head, _, rest = data.partition(b"\n")
+    assert head.startswith(b"YUV4MPEG2 ")
+    tags = {t[:1]: t[1:] for t in head.split()[1:]}
+    w, h = int(tags[b"W"]), int(tags[b"H"])
+    cw, ch = (w + 1) // 2, (h + 1) // 2
+    frames = []
+    while rest:
+        fh, _, rest = rest.partition(b"\n")
+        assert fh.startswith(b"FRAME")
+        buf = np.frombuffer(rest[:w * h + 2 * cw * ch], np.uint8)
+        rest = rest[w * h + 2 * cw * ch:]
+        frames.append((buf[:w * h].reshape(h, w), buf[w * h:w * h + cw * ch].reshape(ch, cw),
+                       buf[w * h + cw * ch:].reshape(ch, cw)))
+    return frames
+
+
+def test_fixture_generator_matches_reference_y4m_bytes():
+    # qa/corpus/synth/clip_64x36_30fps.y4m holds synth_video_frame(64, 36, 0..5), the content of
+    # the golden clips; the restated generator must give the same bytes.
+    frames = read_y4m(os.path.join(GOLDEN, "clip_64x36_30fps.y4m"))
+    assert len(frames) == META["clip_64x36_30fps"]["frames"] == 6
+    for i, planes in enumerate(frames):
+        for p, (want, got) in enumerate(zip(planes, fixture_frame(64, 36, i))):
+            assert np.array_equal(want, got), "frame %d plane %d differs" % (i, p)
