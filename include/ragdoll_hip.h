@@ -217,6 +217,20 @@ rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, co
                               rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag,
                               int32_t groups, int32_t flags);
 
+/* rgh_analyse_frames without any allocation: no counterpart in the reference. For callers that
+ * cannot allocate device or pinned memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Same arguments, results and errors as rgh_analyse_frames;
+ * the call's job descriptors and progress records live in a fixed scratch array of the library
+ * on each device (about 0.5 MB, held until the process ends) instead of buffers it allocates,
+ * so one call takes at most RGH_NOALLOC_MAX_FRAMES frames in total (n * n_frames; more returns
+ * RGH_ERR_INVALID_ARGUMENT). All calls on one device share that array: calls that overlap on
+ * one device race and must be serialised by the caller. The contexts must exist (rgh_create
+ * allocates their buffers once). */
+#define RGH_NOALLOC_MAX_FRAMES 2048
+rgh_status rgh_analyse_frames_noalloc(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src_device,
+                                      rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag,
+                                      int32_t groups, int32_t flags);
+
 /* Slice parameters of rgh_write_p_slice_cavlc: the picture's macroblock grid, the slice QP
  * (equal to the PPS's initial QP and to every macroblock's), frame_num and pic_order_cnt_lsb as
  * the encoder numbers them inside a GOP (k & 15 and 2k & 63 for the k-th picture after the

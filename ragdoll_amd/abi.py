@@ -21,7 +21,7 @@ DECISION_DTYPE = np.dtype([
 assert DECISION_DTYPE.itemsize == 32
 
 EXPORTS = ["rgh_create", "rgh_destroy", "rgh_set_reference", "rgh_analyse_p_frame", "rgh_analyse_idr",
-           "rgh_analyse_batch", "rgh_analyse_frames", "rgh_write_p_slice_cavlc",
+           "rgh_analyse_batch", "rgh_analyse_frames", "rgh_analyse_frames_noalloc", "rgh_write_p_slice_cavlc",
            "rgh_write_idr_slice_cavlc", "rgh_write_headers", "rgh_get_outputs", "rgh_get_mb_dims", "rgh_synchronize",
            "rgh_get_stream", "rgh_enable_timing", "rgh_get_timing", "rgh_status_str", "rgh_last_error"]
 
@@ -91,8 +91,9 @@ def load_library(path=LIB_PATH):
                                              ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p,
                                              ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), I32, I32,
                                              I32]
-    L.rgh_analyse_frames.restype = I32
-    L.rgh_analyse_frames.argtypes = [ctypes.POINTER(ctypes.c_void_p), I32, I32, ctypes.POINTER(RgI420),
+    for name in ("rgh_analyse_frames", "rgh_analyse_frames_noalloc"):
+        getattr(L, name).restype = I32
+        getattr(L, name).argtypes = [ctypes.POINTER(ctypes.c_void_p), I32, I32, ctypes.POINTER(RgI420),
                                      ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p), I32, I32, I32]
     L.rgh_write_p_slice_cavlc.restype = I32
     L.rgh_write_p_slice_cavlc.argtypes = [ctypes.POINTER(RgSliceConfig), P, P, P, ctypes.c_int64,
@@ -238,14 +239,15 @@ class RgLearnedConfig(ctypes.Structure):
 
 
 def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0, first_idr=False, proposals=None,
-                   learned=None):
+                   learned=None, noalloc=False):
     """n_frames consecutive P frames of every analyzer in one pipelined schedule.
 
     device_frames[i][k]: (y_ptr, u_ptr, v_ptr, stride_y, stride_u, stride_v) of frame k of
     analyzer i in device memory; decisions[i][k] / levels[i][k]: device addresses receiving its
     mb_w*mb_h decision records / levels. With proposals[i][k] (device addresses of rgh_proposal
     records; None for an IDR frame 0) the search runs from learned proposals
-    (rgh_analyse_frames_learned) with learned = dict(iters, skip_gate, intra_gate, fallback_pct)."""
+    (rgh_analyse_frames_learned) with learned = dict(iters, skip_gate, intra_gate, fallback_pct).
+    noalloc: the plain form through rgh_analyse_frames_noalloc (no allocation by the library)."""
     L = load_library()
     n = len(analyzers)
     nf = len(device_frames[0])
@@ -262,8 +264,11 @@ def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0,
             lp[i * nf + k] = levels[i][k]
     flags = FRAMES_FIRST_IDR if first_idr else 0
     if proposals is None:
-        _check(L.rgh_analyse_frames(arr, n, nf, srcs, dp, lp, lag, groups, flags), "rgh_analyse_frames")
+        entry = "rgh_analyse_frames_noalloc" if noalloc else "rgh_analyse_frames"
+        _check(getattr(L, entry)(arr, n, nf, srcs, dp, lp, lag, groups, flags), entry)
         return
+    if noalloc:
+        raise ValueError("the no-allocation entry point serves the plain form only")
     pp = (ctypes.c_void_p * (n * nf))()
     for i in range(n):
         for k in range(nf):

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -91,6 +92,7 @@ struct Ctx {
     bool pre_valid = false;  // pre[] holds the last analysed frame's undeblocked reconstruction
     uint32_t sb[5] = {0, 0, 0, 0, 0};
     hipEvent_t ev[2] = {nullptr, nullptr};  // schedule start, schedule end
+    hipEvent_t go = nullptr;                // start of a multi-group call (other groups wait on it)
     bool timing = false, timed = false;
     int timed_launches = 0;
 };
@@ -140,6 +142,7 @@ void free_ctx(Ctx *c) {
     if (c->ctl_host) (void)hipHostFree(c->ctl_host);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    if (c->go) (void)hipEventDestroy(c->go);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -217,6 +220,11 @@ rgh_status ensure_jobs(Ctx *c, int n);
 // Several schedules (disjoint contexts) can interleave their launches on their own streams.
 struct Schedule {
     Ctx *owner = nullptr;
+    // Device job descriptors and progress records of this schedule (nj each), and the host
+    // buffer the progress records are read back into: the owner context's, or scratch memory.
+    RghJob *jobs_dev = nullptr;
+    RghFrameCtl *ctl_dev = nullptr;
+    RghFrameCtl *ctl_host = nullptr;
     int n = 0, n_frames = 0, lag = 1, nj = 0, n_intra = 0;
     bool compact = false, learned = false;
     std::vector<RghJob> jobs;
@@ -257,9 +265,15 @@ struct Schedule {
             gx = std::max(gx, std::min(mb_h, (mb_w + 1) / 2) + std::min(mb_h + 1, (mb_w + 2) / 2));
         }
         planned = (n_frames - 1) * lag + max_span;
-        HIP_TRY(hipMemcpyAsync(owner->jobs_dev, jobs.data(), sizeof(RghJob) * (size_t)nj, hipMemcpyHostToDevice,
+        HIP_TRY(hipMemcpyAsync(jobs_dev, jobs.data(), sizeof(RghJob) * (size_t)nj, hipMemcpyHostToDevice,
                                owner->stream));
-        HIP_TRY(hipMemsetAsync(owner->ctl_dev, 0, sizeof(RghFrameCtl) * (size_t)nj, owner->stream));
+        // The progress records start at zero (a fill kernel, not a memset node, so that the
+        // sequence stays correct under graph capture).
+        static_assert(sizeof(RghFrameCtl) == 16, "RghFrameCtl is two 64-bit words");
+        const int words = 2 * nj;
+        hipLaunchKernelGGL(rgh_k_fill_u64, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, owner->stream,
+                           (uint64_t *)ctl_dev, words, (uint64_t)0);
+        HIP_TRY(hipGetLastError());
         return RGH_OK;
     }
     void launch(int j_lo, int j_hi) {
@@ -270,7 +284,7 @@ struct Schedule {
         if (split > j_lo && j_hi > split && !compact) {
             // IDR and Balanced P jobs together: one kernel, so that neither waits for the other.
             hipLaunchKernelGGL(learned ? rgh_k_mixed_learned_step : rgh_k_mixed_step,
-                               dim3((unsigned)gx, (unsigned)(j_hi - j_lo)), dim3(64), 0, owner->stream, owner->jobs_dev,
+                               dim3((unsigned)gx, (unsigned)(j_hi - j_lo)), dim3(64), 0, owner->stream, jobs_dev,
                                j_lo, L, seq);
             for (int j = j_lo; j < j_hi; j++) last[(size_t)j] = L;
             L++;
@@ -278,11 +292,11 @@ struct Schedule {
         }
         if (split > j_lo)
             hipLaunchKernelGGL(rgh_k_intra_step, dim3((unsigned)gx, (unsigned)(split - j_lo)), dim3(64), 0, owner->stream,
-                               owner->jobs_dev, j_lo, L, seq);
+                               jobs_dev, j_lo, L, seq);
         if (j_hi > split)
             hipLaunchKernelGGL(learned ? rgh_k_learned_step : (compact ? rgh_k_compact_step : rgh_k_frame_step),
                                dim3((unsigned)gx, (unsigned)(j_hi - split)),
-                               dim3(64), 0, owner->stream, owner->jobs_dev, split, L, seq);
+                               dim3(64), 0, owner->stream, jobs_dev, split, L, seq);
         for (int j = j_lo; j < j_hi; j++) last[(size_t)j] = L;
         L++;
     }
@@ -302,18 +316,18 @@ struct Schedule {
     // L - base - next.
     rgh_status complete() {
         for (;;) {
-            HIP_TRY(hipMemcpyAsync(owner->ctl_host, owner->ctl_dev, sizeof(RghFrameCtl) * (size_t)nj,
-                                   hipMemcpyDeviceToHost, owner->stream));
+            HIP_TRY(hipMemcpyAsync(ctl_host, ctl_dev, sizeof(RghFrameCtl) * (size_t)nj, hipMemcpyDeviceToHost,
+                                   owner->stream));
             HIP_TRY(hipStreamSynchronize(owner->stream));
             int lo = nj, hi = -1, need = 0;
             for (int j = 0; j < nj; j++) {
                 const int base = jobs[(size_t)j].base, P = last[(size_t)j];
-                const int next = P < 0 ? 0 : P + 1 - base - owner->ctl_host[j].delay[(P + 1) & 1];
+                const int next = P < 0 ? 0 : P + 1 - base - ctl_host[j].delay[(P + 1) & 1];
                 if (next >= span[(size_t)j]) continue;
                 lo = std::min(lo, j);
                 hi = std::max(hi, j);
                 need = std::max(need, span[(size_t)j] - next);
-                owner->ctl_host[j].delay[0] = owner->ctl_host[j].delay[1] = L - base - next;
+                ctl_host[j].delay[0] = ctl_host[j].delay[1] = L - base - next;
             }
             if (hi < 0) return RGH_OK;
             extra_total += need;
@@ -321,7 +335,7 @@ struct Schedule {
                 set_error("run_schedule", "frames stopped making progress", __LINE__);
                 return RGH_ERR_DEVICE;
             }
-            HIP_TRY(hipMemcpyAsync(owner->ctl_dev + lo, owner->ctl_host + lo, sizeof(RghFrameCtl) * (size_t)(hi - lo + 1),
+            HIP_TRY(hipMemcpyAsync(ctl_dev + lo, ctl_host + lo, sizeof(RghFrameCtl) * (size_t)(hi - lo + 1),
                                    hipMemcpyHostToDevice, owner->stream));
             for (int e = 0; e < need; e++) launch(lo, hi + 1);
             HIP_TRY(hipGetLastError());
@@ -360,6 +374,9 @@ rgh_status run_schedules(std::vector<Schedule> &sch, Ctx *timer) {
 rgh_status run_schedule(Ctx *const *ctxs, int n, int n_frames, std::vector<RghJob> &jobs, int lag) {
     std::vector<Schedule> sch(1);
     sch[0].owner = ctxs[0];
+    sch[0].jobs_dev = ctxs[0]->jobs_dev;
+    sch[0].ctl_dev = ctxs[0]->ctl_dev;
+    sch[0].ctl_host = ctxs[0]->ctl_host;
     sch[0].n = n;
     sch[0].n_frames = n_frames;
     sch[0].lag = lag;
@@ -402,6 +419,51 @@ rgh_status ensure_jobs(Ctx *c, int n) {
     HIP_TRY(hipMalloc(&c->ctl_dev, sizeof(RghFrameCtl) * (size_t)cap));
     HIP_TRY(hipHostMalloc(&c->ctl_host, sizeof(RghFrameCtl) * (size_t)cap, hipHostMallocDefault));
     c->jobs_cap = cap;
+    return RGH_OK;
+}
+
+// ---- Scratch memory of rgh_analyse_frames_noalloc ----
+// The job descriptors and progress records of one call live in this file-scope device array
+// instead of buffers the library allocates: RGH_NOALLOC_MAX_FRAMES of each (about 0.5 MB per
+// device, held until the process ends). Host code never passes the array's name to the device
+// (on the host it is only a stand-in): the address comes from hipGetSymbolAddress, once per
+// device with that device current, and every launch, copy and fill uses that address. Every
+// call on a device shares its array (and the matching host readback buffer below), so calls
+// that overlap on one device race; each call rewrites the descriptors and clears the progress
+// records it uses before its first launch.
+constexpr int kScratchDevices = 16;
+constexpr size_t kScratchJobBytes = (sizeof(RghJob) * RGH_NOALLOC_MAX_FRAMES + 255) & ~(size_t)255;
+constexpr size_t kScratchBytes = kScratchJobBytes + sizeof(RghFrameCtl) * RGH_NOALLOC_MAX_FRAMES;
+
+struct ScratchDevice {
+    std::once_flag once;
+    void *ptr = nullptr;
+    hipError_t err = hipSuccess;
+};
+ScratchDevice g_scratch_dev[kScratchDevices];
+// Host readback of the progress records, per device (ordinary host memory: read with a copy
+// that the stream synchronizes).
+RghFrameCtl g_scratch_ctl_host[kScratchDevices][RGH_NOALLOC_MAX_FRAMES];
+
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+// The device address of `scratch` on `device` (the current device), looked up on the first call.
+rgh_status scratch_address(int device, uint8_t **out) {
+    if (device < 0 || device >= kScratchDevices) {
+        set_error("rgh_analyse_frames_noalloc", "device ordinal beyond the scratch table", __LINE__);
+        return RGH_ERR_UNSUPPORTED;
+    }
+    ScratchDevice &d = g_scratch_dev[device];
+    std::call_once(d.once, [&] { d.err = hipGetSymbolAddress(&d.ptr, HIP_SYMBOL(scratch)); });
+    if (d.err != hipSuccess || !d.ptr) {
+        set_error("hipGetSymbolAddress(scratch)", hipGetErrorString(d.err), __LINE__);
+        return RGH_ERR_DEVICE;
+    }
+    *out = static_cast<uint8_t *>(d.ptr);
     return RGH_OK;
 }
 
@@ -490,6 +552,7 @@ rgh_status rgh_create(const rgh_config *cfg, void **ctx_out) {
     };
     if (hipSetDevice(c->device) != hipSuccess) return fail(RGH_ERR_DEVICE);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return fail(RGH_ERR_DEVICE);
+    if (hipEventCreateWithFlags(&c->go, hipEventDisableTiming) != hipSuccess) return fail(RGH_ERR_DEVICE);
     const size_t ny = (size_t)c->sy * c->rows_y + 64, nc = (size_t)c->sc * c->rows_c + 64;
     const size_t nmb = (size_t)c->mb_w * c->mb_h, nit = (size_t)(c->mb_w + 1) * (c->mb_h + 1);
     c->pic.resize((size_t)(cfg->ring ? cfg->ring : 2));
@@ -635,10 +698,12 @@ rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src) 
     return analyse_single(cs.data(), n, src);
 }
 
+// noalloc: the schedules' job descriptors and progress records come from the scratch array (no
+// device or pinned allocation, no event creation); otherwise from the group owners' buffers.
 static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src,
                                const rgh_proposal *const *props, const rgh_learned_config *lcfg,
                                rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag, int32_t groups,
-                               int32_t flags) {
+                               int32_t flags, bool noalloc = false) {
     if (!ctxs || n <= 0 || n_frames <= 0 || !src || !decisions || !levels || lag < 0 || groups < 0 ||
         (flags & ~RGH_FRAMES_FIRST_IDR))
         return RGH_ERR_INVALID_ARGUMENT;
@@ -685,9 +750,19 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
         }
         cs[(size_t)i] = c;
     }
+    if (noalloc && (int64_t)n * n_frames > RGH_NOALLOC_MAX_FRAMES) {
+        set_error("rgh_analyse_frames_noalloc", "more frames than RGH_NOALLOC_MAX_FRAMES in one call", __LINE__);
+        return RGH_ERR_INVALID_ARGUMENT;
+    }
     if (lag == 0) lag = kDefaultLag;
     const int G = std::max(1, std::min(groups ? groups : 1, n));
     HIP_TRY(hipSetDevice(cs[0]->device));
+    uint8_t *scr = nullptr;
+    if (noalloc) {
+        rgh_status st = scratch_address(cs[0]->device, &scr);
+        if (st != RGH_OK) return st;
+    }
+    size_t scr_used = 0;  // frames of the call placed in the scratch array so far
     // Context i joins group i % G; each group runs on its first context's stream and buffers.
     std::vector<Schedule> sch((size_t)G);
     for (int g = 0; g < G; g++) {
@@ -699,9 +774,19 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
         S.n = m;
         S.n_frames = n_frames;
         S.lag = lag;
-        rgh_status s = ensure_jobs(S.owner, m * n_frames);
-        if (s != RGH_OK) return s;
-        RghFrameCtl *ctl = S.owner->ctl_dev;
+        if (noalloc) {
+            S.jobs_dev = reinterpret_cast<RghJob *>(scr) + scr_used;
+            S.ctl_dev = reinterpret_cast<RghFrameCtl *>(scr + kScratchJobBytes) + scr_used;
+            S.ctl_host = g_scratch_ctl_host[cs[0]->device] + scr_used;
+            scr_used += (size_t)m * n_frames;
+        } else {
+            rgh_status s = ensure_jobs(S.owner, m * n_frames);
+            if (s != RGH_OK) return s;
+            S.jobs_dev = S.owner->jobs_dev;
+            S.ctl_dev = S.owner->ctl_dev;
+            S.ctl_host = S.owner->ctl_host;
+        }
+        RghFrameCtl *ctl = S.ctl_dev;
         S.jobs.resize((size_t)m * n_frames);
         for (int k = 0; k < n_frames; k++)
             for (int t = 0; t < m; t++) {
@@ -727,14 +812,11 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
             }
     }
     // The other groups' streams start after everything queued on the first one.
-    hipEvent_t go = nullptr;
     if (G > 1) {
-        HIP_TRY(hipEventCreateWithFlags(&go, hipEventDisableTiming));
-        HIP_TRY(hipEventRecord(go, cs[0]->stream));
-        for (int g = 1; g < G; g++) HIP_TRY(hipStreamWaitEvent(sch[(size_t)g].owner->stream, go, 0));
+        HIP_TRY(hipEventRecord(cs[0]->go, cs[0]->stream));
+        for (int g = 1; g < G; g++) HIP_TRY(hipStreamWaitEvent(sch[(size_t)g].owner->stream, cs[0]->go, 0));
     }
     rgh_status s = run_schedules(sch, cs[0]);
-    if (go) (void)hipEventDestroy(go);
     if (s != RGH_OK) {
         for (int i = 0; i < n; i++) poison(cs[(size_t)i], n_frames);
         return s;
@@ -760,6 +842,12 @@ rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, co
                               rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag, int32_t groups,
                               int32_t flags) {
     return analyse_frames_impl(ctxs, n, n_frames, src, nullptr, nullptr, decisions, levels, lag, groups, flags);
+}
+
+rgh_status rgh_analyse_frames_noalloc(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src,
+                                      rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag,
+                                      int32_t groups, int32_t flags) {
+    return analyse_frames_impl(ctxs, n, n_frames, src, nullptr, nullptr, decisions, levels, lag, groups, flags, true);
 }
 
 rgh_status rgh_analyse_frames_learned(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src,
