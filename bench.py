@@ -14,7 +14,8 @@ the whole job, IDR pictures included.
 
 The schedule's frame lag (wavefront steps between consecutive frames of a stream) and its number
 of concurrent groups (schedules on separate HIP streams) are picked by a small autotuner during
-warm-up unless --lag / --groups are given.
+warm-up unless --lag / --groups are given. The persistent schedule (one launch per job) is timed
+beside it on the same job (--compare-persistent 0 skips it).
 
 Prints one JSON line (see the keys below). --gpus above 1 is refused: this build measures one
 GPU.
@@ -197,6 +198,8 @@ def main():
                     help="headline: the 1080p metric (default); batched: the 24-stream uneven-size set; "
                          "compact: the headline streams with the Compact preset (Baseline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--compare-persistent", type=int, default=1,
+                    help="also time the persistent schedule (one launch per job) on the same job (1: yes)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal form")
     ap.add_argument("--learned-iters", type=int, default=2)
     ap.add_argument("--skip-gate", type=int, default=26)
@@ -316,15 +319,15 @@ def run_workload(gops, args):
     idr_mbs = sum(a.mb_w * a.mb_h for a in ans)
     step_bytes = total_mbs * MB_BYTES + idr_mbs * IDR_MB_BYTES
 
-    def step(sched):
+    def step(sched, persistent=False):
         lag, groups = sched
         t = time.perf_counter()
-        analyse_frames(ans, srcs, decs, lvls, lag=lag, groups=groups, first_idr=True)
+        analyse_frames(ans, srcs, decs, lvls, lag=lag, groups=groups, first_idr=True, persistent=persistent)
         el = time.perf_counter() - t
         ms, _, launches = ans[0].timing()
         return el, ms, launches
 
-    # Warm-up, with the schedule (frame lag, concurrent groups) autotuned unless given.
+    # Warm-up, with the per-step schedule's frame lag and concurrent groups autotuned unless given.
     lags = [args.lag] if args.lag else [12, 16, 20, 24]
     groups = [args.groups] if args.groups else [1, 2, 3, 4]
     tune = {}
@@ -332,6 +335,21 @@ def run_workload(gops, args):
         for g in groups:
             tune[(lag, g)] = step((lag, g))[0]
     sched = min(tune, key=tune.get)
+    # The persistent schedule (one launch per job) on the same job, for comparison: its lag tuned,
+    # then timed over the same number of steps.
+    persist = None
+    if args.compare_persistent:
+        pt = {lag: step((lag, 1), True)[0] for lag in (16, 20, 24, 32)}
+        plag = min(pt, key=pt.get)
+        el_p = 0.0
+        launches_p = 0
+        for _ in range(args.steps):
+            e, _, nl = step((plag, 1), True)
+            el_p += e
+            launches_p += nl
+        persist = {"value": round(total_mbs * args.steps / el_p, 1), "ms_per_step": round(el_p / args.steps * 1e3, 3),
+                   "lag": plag, "launches_per_step": launches_p / args.steps,
+                   "tuning_s": {"lag %d" % k: round(v, 4) for k, v in pt.items()}}
     for _ in range(args.warmup):
         step(sched)
     el = dev_ms = 0.0
@@ -379,8 +397,10 @@ def run_workload(gops, args):
                    "form": "plain", "gops": len(gops), "p_macroblocks_per_step": total_mbs,
                    "idr_macroblocks_per_step": idr_mbs,
                    "all_macroblocks_per_s": round((total_mbs + idr_mbs) * args.steps / el, 1),
-                   "lag": sched[0], "groups": sched[1],
+                   "schedule": "per-step launches", "lag": sched[0], "groups": sched[1],
+                   "launches_per_step": launches / args.steps,
                    "schedule_tuning_s": {"%d/%d" % k: round(v, 4) for k, v in tune.items()},
+                   "persistent_schedule": persist,
                    "device_ms_per_step": round(dev_ms / args.steps, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

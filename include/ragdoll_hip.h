@@ -213,6 +213,16 @@ rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src_d
  * which encodes a GOP's frames one after another on one thread
  * (crates/ragdoll-pipeline/src/lib.rs:575-590); x264's frame threads are the analogue. */
 #define RGH_FRAMES_FIRST_IDR 1 /* rgh_analyse_frames: frame 0 of every context is an IDR picture */
+/* rgh_analyse_frames: run the call as one persistent launch instead of the per-step launch
+ * schedule (one kernel launch per wavefront step, in `groups` concurrent schedules). Each stream
+ * is homed on one XCD whose waves take its items (macroblock analyses, deblock/pad items) in
+ * planned order and wait for the items they read; if an item cannot wait (it needs one planned
+ * after it) or the launch gives up, the per-step schedule finishes the call. Results are
+ * identical. It serves Balanced and IDR pictures: calls with Compact contexts, and
+ * rgh_analyse_frames_noalloc, use the per-step schedule. On MI355X it is slower than the per-step
+ * schedule on the measured workloads (the waits and their cache invalidations cost more than the
+ * launches they save), so it is not the default. */
+#define RGH_FRAMES_PERSISTENT 2
 rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src_device,
                               rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag,
                               int32_t groups, int32_t flags);

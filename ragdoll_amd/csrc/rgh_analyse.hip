@@ -1166,6 +1166,87 @@ __device__ __forceinline__ int ref_box_pending(const RghJobD &J, int px0, int px
 }
 
 // ---------------------------------------------------------------------------------------
+// Waiting in the persistent schedule.
+// ---------------------------------------------------------------------------------------
+// A producer publishes an item with its data stores completed in its XCD's L2 (s_waitcnt
+// vmcnt(0)) before the stamp store; a consumer on the same XCD reads the stamp coherently (an
+// agent-scope atomic load misses the L1), then invalidates its CU's L1 before reading the data.
+__device__ __forceinline__ uint64_t load_stamp(const RGH_G uint64_t *p) {
+    return __hip_atomic_load((const uint64_t *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void publish_stamp(RGH_G uint64_t *p, uint64_t v) {
+    __asm__ volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __hip_atomic_store((uint64_t *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// The CU's L1 is invalidated at agent scope (buffer_inv sc1): the workgroup-scope form (sc0) does
+// not invalidate it on MI355X, in or out of thread-group-split mode (stale lines of neighbouring
+// macroblocks were read through it).
+__device__ __forceinline__ void invalidate_l1() { __asm__ volatile("s_waitcnt vmcnt(0)\n\tbuffer_inv sc1" ::: "memory"); }
+
+// How a macroblock's reads of the reference are made safe. Per-step launches (pw null): the
+// reference's stamps must come from an earlier launch `seq`, otherwise the macroblock defers.
+// Persistent schedule: it waits for stamps of items planned before its own time T; it defers
+// when one is not (or its frame deferred, or the schedule gave up).
+struct Gate {
+    uint32_t seq;
+    const RghPersist *pw;
+    int T;        // this item's time
+    int ref_job;  // the reference frame's job in this launch (-1: complete before the launch)
+    int ref_base; // its base time
+};
+
+// Waits until every lane's pending stamp *p carries `serial`. T_dep: the time of the item the lane
+// waits for, dep_job its job. False when a pending item's frame has deferred at or before T_dep,
+// or the schedule has given up (every wait then ends: the bounded sleep count sets the flag).
+__device__ bool wait_stamps(bool pend, const RGH_G uint64_t *p, uint32_t serial, int T_dep, int dep_job,
+                            const RghPersist *P) {
+    RGH_G int32_t *deferred = gp(P->deferred);
+    RGH_G uint32_t *status = gp(P->status);
+    uint32_t spins = 0;
+    for (;;) {
+        if (pend) pend = (uint32_t)(load_stamp(p) >> 32) != serial;
+        if (!__ballot(pend)) {
+            if (P->debug && spins && threadIdx.x == 0) atomicAdd((uint32_t *)&status[3], spins);
+            return true;
+        }
+        if ((spins & 7) == 7) {
+            int gone = 0;
+            if (pend) gone = __hip_atomic_load((int *)&deferred[dep_job], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >=
+                             0x7fffffff - T_dep;
+            if (__ballot(gone)) return false;
+            if (__hip_atomic_load((uint32_t *)&status[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+        }
+        if (++spins > P->spin_limit) {
+            if (threadIdx.x == 0) __hip_atomic_store((uint32_t *)&status[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+}
+
+// Persistent schedule: waits until every reference macroblock holding a sample of
+// [px0, px1] x [py0, py1] is final (deblock/pad item (a + 1, b + 1) of the reference stamped),
+// then invalidates the L1; false when one cannot be waited for.
+__device__ bool ref_box_wait(const RghJobD &J, int px0, int px1, int py0, int py1, const Gate &G) {
+    const int ax0 = iclamp(px0 >> 4, 0, J.mb_w - 1), ax1 = iclamp(px1 >> 4, 0, J.mb_w - 1);
+    const int ay0 = iclamp(py0 >> 4, 0, J.mb_h - 1), ay1 = iclamp(py1 >> 4, 0, J.mb_h - 1);
+    const int nx = ax1 - ax0 + 1, n = nx * (ay1 - ay0 + 1);
+    for (int i0 = 0; i0 < n; i0 += 64) {
+        const int i = i0 + (int)threadIdx.x;
+        const int a = ax0 + i % nx, b = ay0 + i / nx;
+        const RGH_G uint64_t *p = J.ref_fin + (i < n ? (b + 1) * (J.mb_w + 1) + a + 1 : 0);
+        const int T_dep = G.ref_base + (a + 1) + 2 * (b + 1) + 4;
+        bool pend = i < n && (uint32_t)(load_stamp(p) >> 32) != J.ref_serial;
+        if (!__ballot(pend)) continue;
+        // Only items of this launch planned before this one are waited for.
+        if (__ballot(pend && (G.ref_job < 0 || T_dep >= G.T))) return false;
+        if (!wait_stamps(pend, p, J.ref_serial, T_dep, G.ref_job, G.pw)) return false;
+    }
+    invalidate_l1();
+    return true;
+}
+
+// ---------------------------------------------------------------------------------------
 // One macroblock (enc.rs:3011 code_p_mb and everything it calls).
 // ---------------------------------------------------------------------------------------
 //
@@ -1179,8 +1260,13 @@ __device__ __forceinline__ int ref_box_pending(const RghJobD &J, int px0, int px
 // iterations. The skip probe runs only when the skip probability reaches learn_skip_gate / 255,
 // the intra 16x16 check only when the intra probability reaches learn_intra_gate / 255. Costs,
 // sub-pel refinement, residual coding and reconstruction are the plain path's.
-template <bool LEARNED>
-__device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, int mb_x, int mb_y, MbLds &S, uint32_t seq) {
+// PERSIST: the persistent schedule (the reference is waited for through G before any read of it;
+// the neighbours, the co-located decision and the reference item of this macroblock were waited
+// for by the caller); otherwise per-step launches (reads issued before the readiness verdict are
+// discarded when the macroblock defers).
+template <bool LEARNED, bool PERSIST = false>
+__device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, int mb_x, int mb_y, MbLds &S,
+                                           const Gate &G) {
     const int lane = threadIdx.x;
     const int rr = lane >> 2, c0 = (lane & 3) * 4;
     const int mb_w = J.mb_w, idx = mb_y * mb_w + mb_x;
@@ -1388,7 +1474,11 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
             ly = min(ly, fys[k]);
             hy = max(hy, fys[k]);
         }
-        wait = ref_box_pending(J, x0 + lx - 22, x0 + hx + 68, y0 + ly - 19, y0 + hy + 36, seq);
+        if constexpr (PERSIST) {
+            if (!ref_box_wait(J, x0 + lx - 22, x0 + hx + 68, y0 + ly - 19, y0 + hy + 36, G)) return false;
+        } else {
+            wait = ref_box_pending(J, x0 + lx - 22, x0 + hx + 68, y0 + ly - 19, y0 + hy + 36, G.seq);
+        }
     }
     uint32_t rw[NC];
     {
@@ -1576,9 +1666,15 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
         const uint4 srow = *(const uint4 *)&S.src[drow * 16];
         for (int it = 0; it < iters; it++) {
             if (iabs(bestx - scx) >= 16 || iabs(besty - scy) >= 16) {
-                if (J.ref_fin && __ballot(ref_box_pending(J, x0 + bestx - 22, x0 + bestx + 68, y0 + besty - 19,
-                                                          y0 + besty + 36, seq)))
-                    return false;
+                if (J.ref_fin) {
+                    if constexpr (PERSIST) {
+                        if (!ref_box_wait(J, x0 + bestx - 22, x0 + bestx + 68, y0 + besty - 19, y0 + besty + 36, G))
+                            return false;
+                    } else if (__ballot(ref_box_pending(J, x0 + bestx - 22, x0 + bestx + 68, y0 + besty - 19,
+                                                        y0 + besty + 36, G.seq))) {
+                        return false;
+                    }
+                }
                 scx = bestx;
                 scy = besty;
                 ox = (x0 + scx - 19) & ~3;
@@ -2355,11 +2451,11 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
                 if (J.intra)
                     analyse_intra_mb(J, jp, x, y, S.a);
                 else
-                    ok = analyse_mb<MODE == 5>(J, jp, x, y, S.a, seq);
+                    ok = analyse_mb<MODE == 5>(J, jp, x, y, S.a, Gate{seq, nullptr, 0, -1, 0});
             } else if constexpr (MODE == 2)
                 ok = analyse_compact_mb(J, x, y, S.a, seq);
             else
-                ok = analyse_mb<MODE == 3>(J, jp, x, y, S.a, seq);
+                ok = analyse_mb<MODE == 3>(J, jp, x, y, S.a, Gate{seq, nullptr, 0, -1, 0});
         }
         if (lane == 0) {
             // The stamp and progress buffers, read again from the descriptor (as late_outputs).
@@ -2427,4 +2523,273 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_learned_step(const RghJob
 extern "C" __global__ __launch_bounds__(256) void rgh_k_fill_u64(uint64_t *p, int n, uint64_t v) {
     const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------------------
+// The persistent schedule: one launch runs every item of a call.
+// ---------------------------------------------------------------------------------------
+
+// The XCC (XCD) the calling wave runs on.
+__device__ __forceinline__ int xcc_id() {
+    uint32_t v;
+    __asm__ volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(v));
+    return (int)(v & 15u);
+}
+
+// Sets bit xcc_id() of *mask for every workgroup: which XCC ids the device's waves report.
+extern "C" __global__ __launch_bounds__(64) void rgh_k_xcc_probe(uint32_t *mask) {
+    if (threadIdx.x == 0) atomicOr(mask, 1u << xcc_id());
+}
+
+// Recounts the analysed macroblocks of job blockIdx.x (stamps of its serial) into its progress
+// record, for the per-step schedule's picture-slot gate after a persistent launch.
+extern "C" __global__ __launch_bounds__(64) void rgh_k_recount(const RghJob *__restrict__ jobs) {
+    const RghJobD J = device_view(jobs[blockIdx.x]);
+    const int n = J.mb_w * J.mb_h;
+    uint32_t c = 0;
+    for (int m = (int)threadIdx.x; m < n; m += 64) c += (uint32_t)(J.ana[m] >> 32) == J.serial;
+    c = wave_sum(c);
+    if (threadIdx.x == 0) J.ctl->analysed = (int32_t)c;
+}
+
+// Analysis macroblocks of a frame at wavefront step s: rows [y0, y0 + n) (n <= 0: none).
+__device__ __forceinline__ void step_rows(int s, int mb_w, int mb_h, int steps, int &y0, int &n) {
+    const int a = s - mb_w + 1;
+    y0 = a > 0 ? (a + 1) >> 1 : 0;
+    n = (s < 0 || s >= steps) ? 0 : min(mb_h - 1, s >> 1) - y0 + 1;
+}
+// Deblock/pad items ((mb_w + 1) x (mb_h + 1) grid) at step sd: rows [y0, y0 + n).
+__device__ __forceinline__ void item_rows(int sd, int mb_w, int mb_h, int steps, int &y0, int &n) {
+    const int a = sd - mb_w;
+    y0 = a > 0 ? (a + 1) >> 1 : 0;
+    n = (sd < 0 || sd > steps + 2) ? 0 : min(mb_h, sd >> 1) - y0 + 1;
+}
+
+__device__ __forceinline__ void defer_item(const RghPersist *P, int j, int T) {
+    if (threadIdx.x == 0) {
+        atomicMax((int *)&gp(P->deferred)[j], 0x7fffffff - T);
+        atomicAdd((uint32_t *)&gp(P->status)[1], 1u);
+    }
+}
+
+// One item (job j, time T, index r among the job's items of that time): an analysis macroblock,
+// else a deblock/pad item. It waits for the items it reads (all planned before T): an analysis
+// for its left, top, top-right and top-left neighbours (decisions and undeblocked edge samples)
+// and, in a P picture, for the reference's item (x + 1, y + 1) (the co-located decision and the
+// previous picture's neighbouring decisions); a deblock/pad item (x, y) for the analyses that read
+// samples it modifies ((x, y), (x + 1, y), (x - 1, y + 1), (x, y + 1), (x + 1, y + 1)) and the
+// items (x - 1, y), (x, y - 1), (x + 1, y - 1) that modify samples it reads (the reference's
+// raster deblocking order). The first step of a frame also waits until the frame that last read
+// its picture slot has been analysed.
+template <int MODE>
+__device__ __forceinline__ void persist_item(const RghJob *__restrict__ jobs, const RghPersist *P, int j, int T, int r,
+                                             uint32_t seq, StepLds &S) {
+    const uint64_t t_item0 = __builtin_amdgcn_s_memtime();
+    const RghJobD J = device_view(jobs[j]);
+    const RghJob *jp = jobs + j;
+    const int lane = threadIdx.x, s = T - J.base;
+    int ya, na;
+    step_rows(s, J.mb_w, J.mb_h, J.steps, ya, na);
+    if (r < na) {
+        const int y = ya + r, x = s - 2 * y, m = y * J.mb_w + x;
+        if ((uint32_t)(load_stamp(&J.ana[m]) >> 32) == J.serial) return;  // done by an earlier launch
+        const int ref_base = J.ref_job >= 0 ? jobs[J.ref_job].base : 0;
+        bool has = false;
+        const RGH_G uint64_t *p = J.ana;
+        int T_dep = 0, dj = j;
+        uint32_t ser = J.serial;
+        if (lane == 0 && x > 0) {
+            p = &J.ana[m - 1];
+            T_dep = T - 1;
+            has = true;
+        } else if (lane == 1 && y > 0) {
+            p = &J.ana[m - J.mb_w];
+            T_dep = T - 2;
+            has = true;
+        } else if (lane == 2 && y > 0 && x + 1 < J.mb_w) {
+            p = &J.ana[m - J.mb_w + 1];
+            T_dep = T - 1;
+            has = true;
+        } else if (lane == 3 && y > 0 && x > 0) {
+            p = &J.ana[m - J.mb_w - 1];
+            T_dep = T - 3;
+            has = true;
+        } else if (lane == 4 && !J.intra) {
+            p = &J.ref_fin[(y + 1) * (J.mb_w + 1) + x + 1];
+            T_dep = ref_base + (x + 1) + 2 * (y + 1) + 4;
+            ser = J.ref_serial;
+            dj = J.ref_job;
+            has = true;
+        }
+        bool pend = has && (uint32_t)(load_stamp(p) >> 32) != ser;
+        if (__ballot(pend && (dj < 0 || T_dep >= T)) || !wait_stamps(pend, p, ser, T_dep, dj, P)) {
+            defer_item(P, j, T);
+            return;
+        }
+        if (s == 0 && J.wait_job >= 0) {
+            // The picture slot is free once the frame that last read it is analysed (planned to
+            // finish before this frame starts): its last macroblock's analysis waited, through its
+            // neighbours, for every other one. (A gating frame outside this launch is complete.)
+            const RghJob *wjp = jobs + J.wait_job;
+            const RGH_G uint64_t *wp = gp(wjp->ana) + (wjp->mb_w * wjp->mb_h - 1);
+            const int wT = wjp->base + wjp->steps - 1;
+            const uint32_t wser = wjp->serial;
+            const bool wpend = lane == 0 && (uint32_t)(load_stamp(wp) >> 32) != wser;
+            if (!wait_stamps(wpend, wp, wser, wT, J.wait_job, P)) {
+                defer_item(P, j, T);
+                return;
+            }
+        }
+        invalidate_l1();
+        const uint64_t t_run0 = __builtin_amdgcn_s_memtime();
+        bool ok = true;
+        if (J.intra)
+            analyse_intra_mb(J, jp, x, y, S.a);
+        else
+            ok = analyse_mb<MODE == 1, true>(J, jp, x, y, S.a, Gate{seq, P, T, J.ref_job, ref_base});
+        if (!ok) {
+            defer_item(P, j, T);
+            return;
+        }
+        if (lane == 0) {
+            const RGH_G RghJob *g = (const RGH_G RghJob *)jp;
+            __asm__ volatile("" : "+s"(g));
+            publish_stamp(&gp(g->ana)[m], make_stamp(g->serial, seq));
+            if (P->debug) {
+                const uint64_t t_run1 = __builtin_amdgcn_s_memtime();
+                atomicAdd((uint32_t *)&gp(P->status)[2], 1u);
+                atomicAdd((uint32_t *)&gp(P->status)[4], (uint32_t)((t_run0 - t_item0) >> 8));
+                atomicAdd((uint32_t *)&gp(P->status)[5], (uint32_t)((t_run1 - t_run0) >> 8));
+                atomicAdd((uint32_t *)&gp(P->status)[6], (uint32_t)((t_run1 - t_run0) >> 8));
+                atomicAdd((uint32_t *)&gp(P->status)[7], 1u);
+            }
+        }
+        return;
+    }
+    const int sd = s - 4;
+    int yd, nd;
+    item_rows(sd, J.mb_w, J.mb_h, J.steps, yd, nd);
+    const int y = yd + (r - na), x = sd - 2 * y, iw = J.mb_w + 1, m = y * iw + x;
+    if ((uint32_t)(load_stamp(&J.fin[m]) >> 32) == J.serial) return;
+    bool has = false;
+    const RGH_G uint64_t *p = J.ana;
+    int T_dep = 0;
+    const bool in_x = x < J.mb_w, in_y = y < J.mb_h, nx_ = x + 1 < J.mb_w, ny_ = y + 1 < J.mb_h;
+    if (lane == 0 && in_x && in_y) {
+        p = &J.ana[y * J.mb_w + x];
+        T_dep = T - 4;
+        has = true;
+    } else if (lane == 1 && nx_ && in_y) {
+        p = &J.ana[y * J.mb_w + x + 1];
+        T_dep = T - 3;
+        has = true;
+    } else if (lane == 2 && x > 0 && ny_) {
+        p = &J.ana[(y + 1) * J.mb_w + x - 1];
+        T_dep = T - 3;
+        has = true;
+    } else if (lane == 3 && in_x && ny_) {
+        p = &J.ana[(y + 1) * J.mb_w + x];
+        T_dep = T - 2;
+        has = true;
+    } else if (lane == 4 && nx_ && ny_) {
+        p = &J.ana[(y + 1) * J.mb_w + x + 1];
+        T_dep = T - 1;
+        has = true;
+    } else if (lane == 5 && x > 0) {
+        p = &J.fin[m - 1];
+        T_dep = T - 1;
+        has = true;
+    } else if (lane == 6 && y > 0) {
+        p = &J.fin[m - iw];
+        T_dep = T - 2;
+        has = true;
+    } else if (lane == 7 && y > 0 && x + 1 <= J.mb_w) {
+        p = &J.fin[m - iw + 1];
+        T_dep = T - 1;
+        has = true;
+    }
+    bool pend = has && (uint32_t)(load_stamp(p) >> 32) != J.serial;
+    if (!wait_stamps(pend, p, J.serial, T_dep, j, P)) {
+        defer_item(P, j, T);
+        return;
+    }
+    invalidate_l1();
+    const uint64_t t_run0 = __builtin_amdgcn_s_memtime();
+    if (in_x && in_y && J.deblock) deblock_mb_inline(J, x, y, S.d);
+    if (x > 0 && y > 0) pad_mb_inline(J, x - 1, y - 1);
+    if (lane == 0) {
+        publish_stamp(&J.fin[m], make_stamp(J.serial, seq));
+        if (P->debug) {
+            const uint64_t t_run1 = __builtin_amdgcn_s_memtime();
+            atomicAdd((uint32_t *)&gp(P->status)[2], 1u);
+            atomicAdd((uint32_t *)&gp(P->status)[4], (uint32_t)((t_run0 - t_item0) >> 8));
+            atomicAdd((uint32_t *)&gp(P->status)[5], (uint32_t)((t_run1 - t_run0) >> 8));
+        }
+    }
+}
+
+// MODE 0: Balanced P and IDR pictures; 1: Balanced P searched from learned proposals, and IDR.
+// Each workgroup (one wave) runs one ticket of the queue of the XCD it runs on. The grid is dealt
+// round-robin over the n_xcc XCDs in workgroup order (from whichever XCD the dispatcher's turn
+// starts at), so each group of n_xcc consecutive workgroups puts one on every XCD and workgroup b
+// takes ticket b / n_xcc there, in dispatch order: no shared counter (a device-scope atomic on one
+// address costs about a microsecond per operation in series, more than an item). The workgroup
+// claims its ticket in a word of its own; a ticket claimed twice (a dispatch that is not
+// round-robin) flags the launch (status 2) and the host finishes the call with the per-step
+// schedule. Every item an item waits for (an earlier ticket of its queue) belongs to a workgroup
+// dispatched before it and still resident: the earliest unfinished item never waits, and the
+// schedule always advances. (A loop over tickets inside one workgroup would keep the analysis's
+// loop-invariant lane constants live across the whole body, several times the registers of a
+// single item.) Then the time slot by binary search over the queue's slot table, and the job by
+// binary search over the slot's segments.
+template <int MODE>
+__device__ __forceinline__ void persist(const RghJob *__restrict__ jobs, const RghPersist *__restrict__ P, uint32_t seq) {
+    __shared__ StepLds S;
+    const int lane = threadIdx.x;
+    const int q = P->queue_of_xcc[xcc_id()];
+    if (q < 0) return;
+    const RghQueue Q = P->q[q];
+    const uint32_t t = blockIdx.x / (uint32_t)P->n_xcc;
+    if (t >= Q.total) return;
+    // Every lane takes part (lane 0 sets the word): no divergent branch around the atomic.
+    if (__builtin_amdgcn_readfirstlane(atomicOr(P->claim + Q.claim0 + t, lane == 0 ? 1u : 0u))) {
+        if (lane == 0) __hip_atomic_store(&P->status[0], 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    int lo = 0, hi = (int)Q.n_slots - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (P->slot_tl[Q.slot0 + mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    int e0 = (int)P->slot_seg[Q.slot0 + lo], e1 = (int)P->slot_seg[Q.slot0 + lo + 1] - 1;
+    while (e0 < e1) {
+        const int mid = (e0 + e1 + 1) >> 1;
+        if (P->seg_tl[mid] <= t)
+            e0 = mid;
+        else
+            e1 = mid - 1;
+    }
+    if (P->jitter) {
+        // Test hook: a pseudo-random delay, so that items finish out of ticket order.
+        const uint32_t h = (t * 2654435761u) ^ (t >> 7);
+        for (uint32_t k = h % P->jitter; k > 0; k--) __builtin_amdgcn_s_sleep(1);
+    }
+    // The decoded values are uniform; readfirstlane states it (they address the job descriptor
+    // through scalar registers).
+    const int j = __builtin_amdgcn_readfirstlane((int)P->seg_job[e0]);
+    const int r = __builtin_amdgcn_readfirstlane((int)(t - P->seg_tl[e0]));
+    persist_item<MODE>(jobs, P, j, __builtin_amdgcn_readfirstlane(Q.t0 + lo), r, seq, S);
+}
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_persist(const RghJob *__restrict__ jobs,
+                                                                const RghPersist *__restrict__ P, uint32_t seq) {
+    persist<0>(jobs, P, seq);
+}
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_persist_learned(const RghJob *__restrict__ jobs,
+                                                                        const RghPersist *__restrict__ P, uint32_t seq) {
+    persist<1>(jobs, P, seq);
 }

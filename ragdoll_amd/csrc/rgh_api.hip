@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,6 +20,10 @@ extern "C" __global__ void rgh_k_compact_step(const RghJob *jobs, int j0, int la
 extern "C" __global__ void rgh_k_learned_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_mixed_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_mixed_learned_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
+extern "C" __global__ void rgh_k_persist(const RghJob *jobs, const RghPersist *P, uint32_t seq);
+extern "C" __global__ void rgh_k_persist_learned(const RghJob *jobs, const RghPersist *P, uint32_t seq);
+extern "C" __global__ void rgh_k_xcc_probe(uint32_t *mask);
+extern "C" __global__ void rgh_k_recount(const RghJob *jobs);
 extern "C" __global__ void rgh_k_extend(const RghJob *jobs);
 extern "C" __global__ void rgh_k_fill_u64(uint64_t *p, int n, uint64_t v);
 
@@ -86,6 +91,9 @@ struct Ctx {
     RghFrameCtl *ctl_dev = nullptr;
     RghFrameCtl *ctl_host = nullptr;  // pinned readback
     int jobs_cap = 0;
+    uint8_t *pbuf = nullptr;   // persistent schedule: plan tables, counters, deferral marks, status
+    uint8_t *phost = nullptr;  // pinned readback of the deferral marks and status
+    size_t pbuf_cap = 0, phost_cap = 0;
     bool has_ref = false;
     bool analysed = false;
     bool keep_pre = true;
@@ -96,6 +104,12 @@ struct Ctx {
     bool timing = false, timed = false;
     int timed_launches = 0;
 };
+
+// Numeric settings read from the environment (test hooks and debugging), dflt when unset.
+uint32_t env_u32(const char *name, uint32_t dflt) {
+    const char *v = std::getenv(name);
+    return v && *v ? (uint32_t)std::strtoul(v, nullptr, 10) : dflt;
+}
 
 // Launch numbers stamped into completion stamps; strictly increasing over the process.
 std::atomic<uint32_t> g_seq{1};
@@ -140,6 +154,8 @@ void free_ctx(Ctx *c) {
     if (c->jobs_dev) (void)hipFree(c->jobs_dev);
     if (c->ctl_dev) (void)hipFree(c->ctl_dev);
     if (c->ctl_host) (void)hipHostFree(c->ctl_host);
+    if (c->pbuf) (void)hipFree(c->pbuf);
+    if (c->phost) (void)hipHostFree(c->phost);
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->go) (void)hipEventDestroy(c->go);
@@ -197,6 +213,7 @@ RghJob make_job(Ctx *c, int k, const rgh_i420 &src, rgh_mb_decision *dec, const 
     j.ana = w.ana;
     j.fin = w.fin;
     j.ref_fin = r.fin;
+    j.ref_job = j.wait_job = -1;
     if (intra) {  // an IDR picture reads no reference
         j.intra = 1;
         j.ref_fin = nullptr;
@@ -227,6 +244,7 @@ struct Schedule {
     RghFrameCtl *ctl_host = nullptr;
     int n = 0, n_frames = 0, lag = 1, nj = 0, n_intra = 0;
     bool compact = false, learned = false;
+    bool recount = false;  // recount analysed macroblocks from their stamps (after a persistent launch)
     std::vector<RghJob> jobs;
     std::vector<int> span, last;
     int gx = 1, max_span = 0, planned = 0, L = 0, extra_total = 0;
@@ -273,6 +291,7 @@ struct Schedule {
         const int words = 2 * nj;
         hipLaunchKernelGGL(rgh_k_fill_u64, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, owner->stream,
                            (uint64_t *)ctl_dev, words, (uint64_t)0);
+        if (recount) hipLaunchKernelGGL(rgh_k_recount, dim3((unsigned)nj), dim3(64), 0, owner->stream, jobs_dev);
         HIP_TRY(hipGetLastError());
         return RGH_OK;
     }
@@ -345,8 +364,8 @@ struct Schedule {
 
 // Runs schedules concurrently (their launches interleaved across their streams) and waits for
 // all of them. Timing, when enabled on `timer`, spans the whole run.
-rgh_status run_schedules(std::vector<Schedule> &sch, Ctx *timer) {
-    if (timer->timing) HIP_TRY(hipEventRecord(timer->ev[0], timer->stream));
+rgh_status run_schedules(std::vector<Schedule> &sch, Ctx *timer, bool record = true) {
+    if (record && timer->timing) HIP_TRY(hipEventRecord(timer->ev[0], timer->stream));
     for (auto &s : sch) {
         rgh_status st = s.begin();
         if (st != RGH_OK) return st;
@@ -357,13 +376,19 @@ rgh_status run_schedules(std::vector<Schedule> &sch, Ctx *timer) {
         for (auto &s : sch)
             if (L < s.planned) s.planned_launch();
     HIP_TRY(hipGetLastError());
+    if (env_u32("RGH_TEST_FAIL_AFTER_LAUNCH", 0)) {
+        // Test hook: a failure once launches have started (their work completes first).
+        for (auto &s : sch) HIP_TRY(hipStreamSynchronize(s.owner->stream));
+        set_error("run_schedule", "failure forced after the launches started (RGH_TEST_FAIL_AFTER_LAUNCH)", __LINE__);
+        return RGH_ERR_DEVICE;
+    }
     int launches = 0;
     for (auto &s : sch) {
         rgh_status st = s.complete();
         if (st != RGH_OK) return st;
         launches = std::max(launches, s.L);
     }
-    if (timer->timing) {
+    if (record && timer->timing) {
         HIP_TRY(hipEventRecord(timer->ev[1], timer->stream));
         timer->timed = true;
         timer->timed_launches = launches;
@@ -382,6 +407,313 @@ rgh_status run_schedule(Ctx *const *ctxs, int n, int n_frames, std::vector<RghJo
     sch[0].lag = lag;
     sch[0].jobs = jobs;
     return run_schedules(sch, ctxs[0]);
+}
+
+// ---- The persistent schedule (rgh_k_persist) ----
+// One launch runs every item of a call. Each stream (context) is homed on one XCD; the items of
+// an XCD's streams form its queue of tickets ordered by planned time. A frame's items span
+// times [base, base + steps + 7): analysis step s at base + s, deblock/pad step sd at base + sd +
+// 4. Pipelined plan: frame k of a stream starts at k * lag, and not before the frame that last
+// read its picture slot has finished its analysis. An item that would have to wait for a later
+// item defers (its frame is marked); frames with a mark are then finished by a second launch
+// whose plan runs each stream's remaining frames one after another, where nothing can defer.
+
+constexpr int kMaxDevices = 16;
+constexpr int kFrameSpanExtra = 7;  // deblock/pad steps reach 3 past the analysis, 4 behind it
+
+struct XccMap {
+    std::once_flag once;
+    rgh_status st = RGH_OK;
+    int n = 0;                    // XCC ids seen
+    int xcc[RGH_MAX_QUEUES] = {}; // queue q is served by the waves of XCC xcc[q]
+};
+XccMap g_xcc[kMaxDevices];
+
+// The XCC ids the device's waves report (probed once per device with a grid large enough to
+// reach every XCD).
+rgh_status xcc_map(Ctx *c, const XccMap **out) {
+    if (c->device < 0 || c->device >= kMaxDevices) {
+        set_error("persistent schedule", "device ordinal beyond the XCC table", __LINE__);
+        return RGH_ERR_UNSUPPORTED;
+    }
+    XccMap &m = g_xcc[c->device];
+    std::call_once(m.once, [&] {
+        uint32_t *d = nullptr, h = 0;
+        if (hipMalloc(&d, sizeof(uint32_t)) != hipSuccess) {
+            m.st = RGH_ERR_OOM;
+            return;
+        }
+        bool ok = hipMemsetAsync(d, 0, sizeof(uint32_t), c->stream) == hipSuccess;
+        if (ok) {
+            hipLaunchKernelGGL(rgh_k_xcc_probe, dim3(4096), dim3(64), 0, c->stream, d);
+            ok = hipGetLastError() == hipSuccess;
+        }
+        ok = ok && hipMemcpyAsync(&h, d, sizeof(uint32_t), hipMemcpyDeviceToHost, c->stream) == hipSuccess &&
+             hipStreamSynchronize(c->stream) == hipSuccess;
+        (void)hipFree(d);
+        if (!ok || !h) {
+            m.st = RGH_ERR_DEVICE;
+            return;
+        }
+        for (int x = 0; x < RGH_MAX_QUEUES; x++)
+            if ((h >> x) & 1) m.xcc[m.n++] = x;
+    });
+    if (m.st != RGH_OK) {
+        set_error("persistent schedule", "XCC probe failed", __LINE__);
+        return m.st;
+    }
+    *out = &m;
+    return RGH_OK;
+}
+
+int n_analysis_at(int s, int mb_w, int mb_h, int steps) {
+    if (s < 0 || s >= steps) return 0;
+    const int a = s - mb_w + 1, y0 = a > 0 ? (a + 1) >> 1 : 0;
+    return std::min(mb_h - 1, s >> 1) - y0 + 1;
+}
+int n_items_at(int sd, int mb_w, int mb_h, int steps) {
+    if (sd < 0 || sd > steps + 2) return 0;
+    const int a = sd - mb_w, y0 = a > 0 ? (a + 1) >> 1 : 0;
+    return std::min(mb_h, sd >> 1) - y0 + 1;
+}
+int items_at(const RghJob &j, int T) {
+    const int s = T - j.base;
+    return n_analysis_at(s, j.mb_w, j.mb_h, j.steps) + n_items_at(s - 4, j.mb_w, j.mb_h, j.steps);
+}
+
+// The device image of a plan: RghPersist, then its tables; after them (not uploaded, cleared on
+// the device) the ticket counters, the deferral marks and the status words.
+struct PersistPlan {
+    std::vector<uint8_t> img;
+    size_t mut_off = 0, mut_bytes = 0, deferred_off = 0, total = 0;
+    uint32_t max_queue = 0;
+};
+
+size_t align8(size_t v) { return (v + 7) & ~(size_t)7; }
+
+void build_plan(const std::vector<RghJob> &lj, const std::vector<int> &queue_of_job, const XccMap &xm, int nq,
+                uint8_t *dev, uint32_t spin_limit, uint32_t jitter, PersistPlan &pl) {
+    const int nj = (int)lj.size();
+    std::vector<int> t0((size_t)nq, INT32_MAX), t1((size_t)nq, INT32_MIN);
+    for (int j = 0; j < nj; j++) {
+        const int q = queue_of_job[(size_t)j];
+        t0[(size_t)q] = std::min(t0[(size_t)q], lj[(size_t)j].base);
+        t1[(size_t)q] = std::max(t1[(size_t)q], lj[(size_t)j].base + lj[(size_t)j].steps + kFrameSpanExtra - 1);
+    }
+    std::vector<uint32_t> slot0((size_t)nq, 0), nslots((size_t)nq, 0);
+    size_t S = 0;
+    for (int q = 0; q < nq; q++) {
+        slot0[(size_t)q] = (uint32_t)S;
+        nslots[(size_t)q] = t1[(size_t)q] >= t0[(size_t)q] ? (uint32_t)(t1[(size_t)q] - t0[(size_t)q] + 1) : 0;
+        S += nslots[(size_t)q] + 1;
+    }
+    // Segments (job, time) per slot, in job order within a slot.
+    std::vector<uint32_t> slot_seg(S + 1, 0), slot_tl(S, 0);
+    for (int j = 0; j < nj; j++) {
+        const RghJob &J = lj[(size_t)j];
+        const int q = queue_of_job[(size_t)j];
+        for (int T = J.base; T < J.base + J.steps + kFrameSpanExtra; T++)
+            if (items_at(J, T) > 0) slot_seg[slot0[(size_t)q] + (size_t)(T - t0[(size_t)q]) + 1]++;
+    }
+    for (size_t i = 0; i < S; i++) slot_seg[i + 1] += slot_seg[i];
+    const size_t E = slot_seg[S];
+    std::vector<uint32_t> seg_job(E), seg_cnt(E), fillp(slot_seg.begin(), slot_seg.end() - 1);
+    for (int j = 0; j < nj; j++) {
+        const RghJob &J = lj[(size_t)j];
+        const int q = queue_of_job[(size_t)j];
+        for (int T = J.base; T < J.base + J.steps + kFrameSpanExtra; T++) {
+            const int c = items_at(J, T);
+            if (c <= 0) continue;
+            const size_t e = fillp[slot0[(size_t)q] + (size_t)(T - t0[(size_t)q])]++;
+            seg_job[e] = (uint32_t)j;
+            seg_cnt[e] = (uint32_t)c;
+        }
+    }
+    // Tickets: each queue numbers its slots' segments in order.
+    std::vector<uint32_t> seg_tl(E);
+    RghPersist P;
+    std::memset(&P, 0, sizeof(P));
+    P.n_queues = nq;
+    P.n_xcc = xm.n;
+    for (int x = 0; x < RGH_MAX_QUEUES; x++) P.queue_of_xcc[x] = -1;
+    for (int q = 0; q < nq; q++) P.queue_of_xcc[xm.xcc[q]] = q;
+    P.spin_limit = spin_limit;
+    P.jitter = jitter;
+    P.debug = env_u32("RGH_PERSIST_DEBUG", 0);
+    pl.max_queue = 0;
+    uint32_t claims = 0;
+    for (int q = 0; q < nq; q++) {
+        uint32_t tk = 0;
+        const size_t s0 = slot0[(size_t)q];
+        for (uint32_t i = 0; i <= nslots[(size_t)q]; i++) {
+            slot_tl[s0 + i] = tk;
+            if (i == nslots[(size_t)q]) break;
+            for (size_t e = slot_seg[s0 + i]; e < slot_seg[s0 + i + 1]; e++) {
+                seg_tl[e] = tk;
+                tk += seg_cnt[e];
+            }
+        }
+        P.q[q].total = tk;
+        P.q[q].t0 = t0[(size_t)q];
+        P.q[q].slot0 = (uint32_t)s0;
+        P.q[q].n_slots = nslots[(size_t)q];
+        P.q[q].claim0 = claims;
+        claims += tk;
+        pl.max_queue = std::max(pl.max_queue, tk);
+    }
+    // Layout.
+    size_t off = align8(sizeof(RghPersist));
+    const size_t o_tl = off;
+    off = align8(off + 4 * S);
+    const size_t o_sseg = off;
+    off = align8(off + 4 * (S + 1));
+    const size_t o_sjob = off;
+    off = align8(off + 4 * E);
+    const size_t o_stl = off;
+    off = align8(off + 4 * E);
+    pl.mut_off = off;
+    off = (off + 255) & ~(size_t)255;
+    pl.mut_off = off;
+    const size_t o_claim = off;
+    off = align8(off + 4 * (size_t)claims);
+    pl.deferred_off = off;
+    off = align8(off + 4 * (size_t)nj);
+    const size_t o_status = off;
+    off = align8(off + 4 * 8);
+    pl.mut_bytes = off - pl.mut_off;
+    pl.total = off;
+    P.slot_tl = (const uint32_t *)(dev + o_tl);
+    P.slot_seg = (const uint32_t *)(dev + o_sseg);
+    P.seg_job = (const uint32_t *)(dev + o_sjob);
+    P.seg_tl = (const uint32_t *)(dev + o_stl);
+    P.claim = (uint32_t *)(dev + o_claim);
+    P.deferred = (int32_t *)(dev + pl.deferred_off);
+    P.status = (uint32_t *)(dev + o_status);
+    pl.img.assign(pl.mut_off, 0);
+    std::memcpy(pl.img.data(), &P, sizeof(P));
+    std::memcpy(pl.img.data() + o_tl, slot_tl.data(), 4 * S);
+    std::memcpy(pl.img.data() + o_sseg, slot_seg.data(), 4 * (S + 1));
+    std::memcpy(pl.img.data() + o_sjob, seg_job.data(), 4 * E);
+    std::memcpy(pl.img.data() + o_stl, seg_tl.data(), 4 * E);
+}
+
+// One persistent launch over lj (bases, ref_job and wait_job already in lj's numbering) on the
+// owner's stream; returns after it finishes, with the deferral marks (per job), the items
+// deferred and the give-up status (1: a wait reached its bound, 2: a workgroup ran off its XCD).
+rgh_status persistent_launch(Ctx *owner, const XccMap &xm, std::vector<RghJob> &lj, const std::vector<int> &queue_of_job,
+                             int nq, bool learned, std::vector<int32_t> &marks, uint32_t &deferred, uint32_t &gave_up) {
+    const int nj = (int)lj.size();
+    rgh_status st = ensure_jobs(owner, nj);
+    if (st != RGH_OK) return st;
+    PersistPlan pl;
+    // A wait that sleeps this many times (about a second) means the schedule cannot advance.
+    const uint32_t spin = env_u32("RGH_PERSIST_SPIN_LIMIT", 1u << 20), jitter = env_u32("RGH_PERSIST_JITTER", 0);
+    build_plan(lj, queue_of_job, xm, nq, nullptr, spin, jitter, pl);
+    if (owner->pbuf_cap < pl.total) {
+        if (owner->pbuf) HIP_TRY(hipFree(owner->pbuf));
+        owner->pbuf = nullptr;
+        owner->pbuf_cap = 0;
+        const size_t cap = pl.total + pl.total / 4 + 4096;
+        HIP_TRY(hipMalloc(&owner->pbuf, cap));
+        owner->pbuf_cap = cap;
+    }
+    const size_t rb = pl.total - pl.deferred_off;  // deferral marks, status
+    if (owner->phost_cap < rb) {
+        if (owner->phost) HIP_TRY(hipHostFree(owner->phost));
+        owner->phost = nullptr;
+        owner->phost_cap = 0;
+        HIP_TRY(hipHostMalloc(&owner->phost, rb + 4096, hipHostMallocDefault));
+        owner->phost_cap = rb + 4096;
+    }
+    build_plan(lj, queue_of_job, xm, nq, owner->pbuf, spin, jitter, pl);  // again, with device addresses
+    HIP_TRY(hipMemcpyAsync(owner->jobs_dev, lj.data(), sizeof(RghJob) * (size_t)nj, hipMemcpyHostToDevice,
+                           owner->stream));
+    HIP_TRY(hipMemcpyAsync(owner->pbuf, pl.img.data(), pl.img.size(), hipMemcpyHostToDevice, owner->stream));
+    const int words = (int)(pl.mut_bytes / 8);
+    hipLaunchKernelGGL(rgh_k_fill_u64, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, owner->stream,
+                       (uint64_t *)(owner->pbuf + pl.mut_off), words, (uint64_t)0);
+    const uint32_t seq = g_seq.fetch_add(1);
+    // At least max_queue workgroups per XCD (dispatched round-robin over the XCDs).
+    const uint64_t grid = (uint64_t)xm.n * pl.max_queue;
+    if (grid > 0x7fffffffu) {
+        set_error("persistent schedule", "too many items in one call", __LINE__);
+        return RGH_ERR_INVALID_ARGUMENT;
+    }
+    if (grid)
+        hipLaunchKernelGGL(learned ? rgh_k_persist_learned : rgh_k_persist, dim3((unsigned)grid), dim3(64), 0,
+                           owner->stream, owner->jobs_dev, (const RghPersist *)owner->pbuf, seq);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(owner->phost, owner->pbuf + pl.deferred_off, rb, hipMemcpyDeviceToHost, owner->stream));
+    HIP_TRY(hipStreamSynchronize(owner->stream));
+    const int32_t *m = (const int32_t *)owner->phost;
+    marks.assign(m, m + nj);
+    const uint32_t *status = (const uint32_t *)(owner->phost + (rb - 32));
+    deferred = status[1];
+    gave_up = status[0];
+    if (env_u32("RGH_PERSIST_DEBUG", 0)) {
+        RghPersist P;
+        std::memcpy(&P, pl.img.data(), sizeof(P));
+        uint64_t items = 0;
+        for (int q = 0; q < nq; q++) items += P.q[q].total;
+        std::fprintf(stderr, "persist: xcc ids %d:", xm.n);
+        for (int q = 0; q < xm.n; q++) std::fprintf(stderr, " %d", xm.xcc[q]);
+        std::fprintf(stderr, "; queues %d:", nq);
+        for (int q = 0; q < nq; q++) std::fprintf(stderr, " %u", P.q[q].total);
+        std::fprintf(stderr, "; jobs %d; items %llu run %u deferred %u gave_up %u spins %u; grid %llu; wait/item %.1f us "
+                     "run/item %.1f us (analysis %.1f us)\n", nj,
+                     (unsigned long long)items, status[2], status[1], status[0], status[3],
+                     (unsigned long long)xm.n * pl.max_queue, status[4] * 256.0 / 2400.0 / std::max(1u, status[2]),
+                     status[5] * 256.0 / 2400.0 / std::max(1u, status[2]), status[6] * 256.0 / 2400.0 / std::max(1u, status[7]));
+    }
+    return RGH_OK;
+}
+
+// Runs a call's schedule S (jobs frame-major: job k * n + i is frame k of context i, ctl pointers
+// into the owner's progress records) with one persistent launch. When an item deferred, or the
+// launch gave up, the per-step schedule finishes the call (items already done skip themselves;
+// the analysed counts of its picture-slot gate are recounted from the stamps first).
+// `ring[i]`: context i's ring size.
+rgh_status run_persistent(Schedule &S, int lag, bool learned, const std::vector<int> &ring, Ctx *timer) {
+    Ctx *owner = S.owner;
+    const int n = S.n, n_frames = S.n_frames, nj = n * n_frames;
+    const XccMap *xm = nullptr;
+    rgh_status st = xcc_map(owner, &xm);
+    if (st != RGH_OK) return st;
+    const int nq = std::min(xm->n, n);
+    std::vector<RghJob> jobs = S.jobs;
+    // Streams homed on queues (XCDs) by a greedy balance of their macroblocks.
+    std::vector<int> order((size_t)n), queue_of_stream((size_t)n, 0);
+    std::vector<double> load((size_t)nq, 0.0);
+    for (int i = 0; i < n; i++) order[(size_t)i] = i;
+    auto work = [&](int i) { return (double)jobs[(size_t)i].mb_w * jobs[(size_t)i].mb_h; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+    for (int i : order) {
+        const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        queue_of_stream[(size_t)i] = q;
+        load[(size_t)q] += work(i);
+    }
+    // The pipelined plan: frame k at k * lag, after the frame gating its picture slot.
+    std::vector<int> qj((size_t)nj);
+    for (int k = 0; k < n_frames; k++)
+        for (int i = 0; i < n; i++) {
+            RghJob &J = jobs[(size_t)k * n + i];
+            const int wk = k + 1 - ring[(size_t)i];
+            J.ref_job = (k > 0 && !J.intra) ? (k - 1) * n + i : -1;
+            J.wait_job = wk >= 0 ? wk * n + i : -1;
+            J.base = k * lag;
+            if (wk >= 0) J.base = std::max(J.base, jobs[(size_t)wk * n + i].base + J.steps);
+            qj[(size_t)k * n + i] = queue_of_stream[(size_t)i];
+        }
+    std::vector<int32_t> marks;
+    uint32_t deferred = 0, gave_up = 0;
+    st = persistent_launch(owner, *xm, jobs, qj, nq, learned, marks, deferred, gave_up);
+    timer->timed_launches = 1;
+    if (st != RGH_OK || (deferred == 0 && gave_up == 0)) return st;
+    std::vector<Schedule> sch(1, S);
+    sch[0].recount = true;
+    st = run_schedules(sch, timer, false);
+    timer->timed_launches = 1 + sch[0].L;
+    return st;
 }
 
 // After a call failed once its launches may have started: stamps of this call's serials may be
@@ -705,7 +1037,7 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
                                rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag, int32_t groups,
                                int32_t flags, bool noalloc = false) {
     if (!ctxs || n <= 0 || n_frames <= 0 || !src || !decisions || !levels || lag < 0 || groups < 0 ||
-        (flags & ~RGH_FRAMES_FIRST_IDR))
+        (flags & ~(RGH_FRAMES_FIRST_IDR | RGH_FRAMES_PERSISTENT)))
         return RGH_ERR_INVALID_ARGUMENT;
     const bool first_idr = (flags & RGH_FRAMES_FIRST_IDR) != 0;
     rgh_learned_config lc{4, 0, 0, 105, 256};
@@ -755,7 +1087,12 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
         return RGH_ERR_INVALID_ARGUMENT;
     }
     if (lag == 0) lag = kDefaultLag;
-    const int G = std::max(1, std::min(groups ? groups : 1, n));
+    // The persistent schedule (RGH_FRAMES_PERSISTENT) serves Balanced and IDR pictures; Compact
+    // contexts and the no-allocation entry point use the per-step launch schedule.
+    bool compact_any = false;
+    for (int i = 0; i < n; i++) compact_any |= cs[(size_t)i]->compact;
+    const bool persistent = !noalloc && !compact_any && (flags & RGH_FRAMES_PERSISTENT);
+    const int G = persistent ? 1 : std::max(1, std::min(groups ? groups : 1, n));
     HIP_TRY(hipSetDevice(cs[0]->device));
     uint8_t *scr = nullptr;
     if (noalloc) {
@@ -811,12 +1148,25 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
                 }
             }
     }
-    // The other groups' streams start after everything queued on the first one.
-    if (G > 1) {
-        HIP_TRY(hipEventRecord(cs[0]->go, cs[0]->stream));
-        for (int g = 1; g < G; g++) HIP_TRY(hipStreamWaitEvent(sch[(size_t)g].owner->stream, cs[0]->go, 0));
+    rgh_status s;
+    if (persistent) {
+        std::vector<int> ring((size_t)n);
+        for (int i = 0; i < n; i++) ring[(size_t)i] = (int)cs[(size_t)i]->pic.size();
+        Ctx *timer = cs[0];
+        if (timer->timing) HIP_TRY(hipEventRecord(timer->ev[0], timer->stream));
+        s = run_persistent(sch[0], lag, props != nullptr, ring, timer);
+        if (s == RGH_OK && timer->timing) {
+            HIP_TRY(hipEventRecord(timer->ev[1], timer->stream));
+            timer->timed = true;
+        }
+    } else {
+        // The other groups' streams start after everything queued on the first one.
+        if (G > 1) {
+            HIP_TRY(hipEventRecord(cs[0]->go, cs[0]->stream));
+            for (int g = 1; g < G; g++) HIP_TRY(hipStreamWaitEvent(sch[(size_t)g].owner->stream, cs[0]->go, 0));
+        }
+        s = run_schedules(sch, cs[0]);
     }
-    rgh_status s = run_schedules(sch, cs[0]);
     if (s != RGH_OK) {
         for (int i = 0; i < n; i++) poison(cs[(size_t)i], n_frames);
         return s;

@@ -59,6 +59,9 @@ struct RghJob {
     RghFrameCtl *ctl;
     const int32_t *wait_analysed;  // analysis count of the frame that last reads the picture slot
     int32_t wait_total;        // ... which must reach this value before the first write
+    // Persistent schedule: the jobs of this launch that hold the reference frame and the frame
+    // gating the picture slot (-1: not in this launch, so complete before it started).
+    int32_t ref_job, wait_job;
     int32_t pad_;
 };
 
@@ -88,6 +91,7 @@ struct RghJobD {
     RGH_G RghFrameCtl *ctl;
     const RGH_G int32_t *wait_analysed;
     int32_t wait_total;
+    int32_t ref_job, wait_job;
 };
 
 template <typename T>
@@ -150,8 +154,45 @@ __device__ __forceinline__ RghJobD device_view(const RghJob &j) {
     d.ctl = gp(j.ctl);
     d.wait_analysed = gp(j.wait_analysed);
     d.wait_total = j.wait_total;
+    d.ref_job = j.ref_job;
+    d.wait_job = j.wait_job;
     return d;
 }
+
+// ---- Persistent schedule (rgh_k_persist) ----
+// One launch runs every item of a call (analysis macroblocks and deblock/pad items of every
+// frame). Each stream is homed on one XCD; the items of the streams homed on an XCD form that
+// XCD's queue of tickets, ordered by planned time (a frame's base time plus the wavefront step
+// x + 2y, deblock/pad items 4 steps later), and only waves running on that XCD take its tickets,
+// so every item that reads another's results shares its L2 with it. An item waits only for items
+// of earlier time (held by running waves); one that would need a later item defers, and its frame
+// is finished by a later launch.
+constexpr int RGH_MAX_QUEUES = 16;
+struct RghQueue {
+    uint32_t total;    // tickets (items) in the queue
+    int32_t t0;        // time of the queue's first slot
+    uint32_t slot0;    // index of its first entry in the slot tables (n_slots + 1 entries)
+    uint32_t n_slots;  // time slots
+    uint32_t claim0;   // index of its first ticket's claim word
+    uint32_t pad_[3];
+};
+struct RghPersist {
+    int32_t n_queues;
+    int32_t n_xcc;                         // XCDs the grid is dealt over (round-robin by workgroup)
+    int32_t queue_of_xcc[RGH_MAX_QUEUES];  // queue of the waves on each XCC id (-1: none)
+    uint32_t spin_limit;  // sleeps a wait may take before the schedule gives up
+    uint32_t jitter;      // test hook: a pseudo-random delay before each item (0: none)
+    uint32_t debug;       // count items, spins and cycles into status[2..7]
+    RghQueue q[RGH_MAX_QUEUES];
+    const uint32_t *slot_tl;   // per slot: its first ticket (the queue's total after the last)
+    const uint32_t *slot_seg;  // per slot: its first segment (the end after the last)
+    const uint32_t *seg_job;   // per segment: the job whose items it holds
+    const uint32_t *seg_tl;    // per segment: its first ticket
+    uint32_t *claim;           // per ticket: set by the workgroup that runs it (zero at the launch)
+    int32_t *deferred;         // per job: 0x7fffffff - the earliest time an item deferred (0: none)
+    uint32_t *status;          // [0] gave up (1: a wait's bound, 2: a ticket claimed twice),
+                               // [1] items deferred, [2] items run, [3..7] debug counters
+};
 
 // Sum of a 32-bit value over the 64 lanes of the wave, broadcast as a scalar. DPP quad and row
 // mirrors reduce each row of 16 lanes; the four row sums are read out with readlane. Every lane

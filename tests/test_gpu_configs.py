@@ -43,6 +43,15 @@ def test_1080p_pipelined_me64():
                  label="1080p me64")
 
 
+def test_1080p_persistent_me64():
+    # The same streams through the persistent schedule.
+    w, h, frames = headline_frames()
+    f = frames[CUT_SLICE]
+    print("persistent 1080p me64", flush=True)
+    run_pipeline([(f, w, h, 24, "high", 64), (f, w, h, 30, "high", 64), (f, w, h, 36, "high", 64)], lag=20, ring=5,
+                 label="persistent 1080p me64", persistent=True)
+
+
 def test_720p_high_qp26_me32():
     kw = dict(CONFIGS["720p"])
     w, h = kw.pop("width"), kw.pop("height")

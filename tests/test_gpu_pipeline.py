@@ -36,10 +36,10 @@ def oracle_run(frames, w, h, qp, profile, me_range=16):
     return idr, out, ora.recon(deblocked=True)
 
 
-def run_pipeline(streams, lag, ring, calls=1, groups=0, label=""):
+def run_pipeline(streams, lag, ring, calls=1, groups=0, label="", persistent=False):
     """streams: list of (frames, w, h, qp, profile[, me_range]). Runs frames 1.. of every stream
-    through rgh_analyse_frames (split into `calls` consecutive calls) and checks them against the
-    oracle."""
+    through rgh_analyse_frames (split into `calls` consecutive calls; the per-step launch schedule,
+    or one persistent launch per call) and checks them against the oracle."""
     hip = Hip()
     ans, refs, bufs = [], [], []
     try:
@@ -67,7 +67,7 @@ def run_pipeline(streams, lag, ring, calls=1, groups=0, label=""):
         for c in range(calls):
             sl = slice(cuts[c], cuts[c + 1])
             analyse_frames(ans, [r[2][sl] for r in refs], [r[3][sl] for r in refs], [r[4][sl] for r in refs], lag=lag,
-                           groups=groups)
+                           groups=groups, persistent=persistent)
         for s, (a, (want, final, srcs, decs, lvs, n)) in enumerate(zip(ans, refs)):
             for k in range(nf):
                 dec = hip.download(decs[k], np.zeros(n, DECISION_DTYPE))
@@ -122,3 +122,74 @@ def test_batched_uneven_streams():
         g = SceneGenerator(w, h, seed=seed, **kw)
         streams.append(([g.next_frame() for _ in range(4)], w, h, qp, "high", me_range))
     run_pipeline(streams, lag=12, ring=4, groups=2, label="batched")
+
+
+# The persistent schedule (RGH_FRAMES_PERSISTENT): one launch whose items wait for each other.
+# Small lags make macroblocks need reference items planned after them (they defer, and the
+# per-step schedule finishes the call); ring 2 exercises the picture-slot gate.
+@pytest.mark.parametrize("lag,ring", [(20, 8), (3, 8), (1, 2), (12, 2)])
+def test_cif_persistent(lag, ring):
+    w, h, frames = scene("cif", 8)
+    print("persistent cif lag %d ring %d" % (lag, ring), flush=True)
+    run_pipeline([(frames, w, h, 28, "baseline")], lag, ring, label="persistent cif lag %d ring %d" % (lag, ring),
+                 persistent=True)
+
+
+def test_multi_stream_persistent():
+    print("persistent multi-stream", flush=True)
+    w, h, cif = scene("cif", 6, 3)
+    fx = [fixture_frame(160, 90, i) for i in range(6)]
+    w7, h7, f720 = scene("720p", 6, 5)
+    run_pipeline([(cif, w, h, 26, "high"), (fx, 160, 90, 20, "baseline"), (f720, w7, h7, 30, "main")],
+                 lag=6, ring=6, calls=2, label="persistent multi", persistent=True)
+
+
+@pytest.mark.parametrize("env", [{"RGH_PERSIST_JITTER": "48"}, {"RGH_PERSIST_SPIN_LIMIT": "0"}],
+                         ids=["jitter", "gives-up"])
+def test_persistent_under_stress(env, monkeypatch):
+    # Jitter: items finish far out of ticket order. A wait bound of zero: the first wait gives up
+    # and the per-step schedule finishes the call. Both must give the oracle's results.
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    w, h, cif = scene("cif", 6, 11)
+    w7, h7, f720 = scene("720p", 4, 12)
+    print("persistent stress %s" % env, flush=True)
+    run_pipeline([(cif, w, h, 30, "high"), (cif, w, h, 24, "baseline")], lag=16, ring=6,
+                 label="persistent %s" % env, persistent=True)
+    run_pipeline([(f720, w7, h7, 26, "high", 32)], lag=20, ring=4, label="persistent 720p %s" % env, persistent=True)
+
+
+def test_failed_call_leaves_context_without_reference(monkeypatch):
+    # A call that fails after its launches started (forced from the host) leaves its contexts
+    # without a reference: the next P frame and the outputs are refused until a new reference.
+    from ragdoll_amd.abi import RghError
+    w, h, frames = scene("cif", 3, 21)
+    idr, _, _ = oracle_run(frames[:1], w, h, 28, "baseline")
+    hip = Hip()
+    a = StreamAnalyzer(w, h, 28, keep_undeblocked=False, ring=4)
+    bufs = []
+    try:
+        a.set_reference(*idr)
+        n = a.mb_w * a.mb_h
+        y, u, v = frames[1]
+        dy, du, dv = hip.upload(y), hip.upload(u), hip.upload(v)
+        d, l = hip.alloc(n * 32), hip.alloc(n * LEVELS_PER_MB * 2)
+        bufs = [dy, du, dv, d, l]
+        monkeypatch.setenv("RGH_TEST_FAIL_AFTER_LAUNCH", "1")
+        with pytest.raises(RghError) as e:
+            analyse_frames([a], [[(dy, du, dv, y.shape[1], u.shape[1], v.shape[1])]], [[d]], [[l]])
+        assert e.value.status == 5  # RGH_ERR_DEVICE
+        monkeypatch.delenv("RGH_TEST_FAIL_AFTER_LAUNCH")
+        with pytest.raises(RghError) as e:
+            a.analyse(*frames[2])
+        assert e.value.status == 4  # RGH_ERR_NO_REFERENCE
+        with pytest.raises(RghError) as e:
+            a.outputs()
+        assert e.value.status == 4
+        # A new reference makes it usable again.
+        a.set_reference(*idr)
+        a.analyse(*frames[1])
+    finally:
+        a.close()
+        for b in bufs:
+            hip.free(b)
