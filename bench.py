@@ -59,11 +59,15 @@ def profile_record(kernel):
 
 
 def measured_traffic():
-    """HBM bytes per launch (the P and IDR kernels of one launch; FETCH_SIZE / WRITE_SIZE,
-    copy-calibrated) from the committed profile, or None."""
+    """HBM-side bytes of the analysis kernels from the committed counter profile of this workload
+    (raw L2-to-fabric read and write requests by size): per round of launches (one launch per
+    group, the unit of avg_launch_us), per macroblock, and their ratio to the algorithmic bytes;
+    None without the profile."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            return round(json.load(f)["bytes_per_launch"])
+            t = json.load(f)
+        return {"per_round": t["bytes_per_round"], "per_macroblock": t["bytes_per_macroblock"],
+                "ratio_to_algorithmic": t["traffic_ratio"]}
     except (OSError, KeyError, ValueError):
         return None
 
@@ -89,8 +93,9 @@ def valu_figure(mbs, device_s):
 
 
 def calibrate_copy(mib, repeats=4):
-    """A plain device-to-device copy of known size: reading FETCH_SIZE / WRITE_SIZE for it gives
-    the factor between those counters and the bytes a kernel moves."""
+    """A plain device-to-device copy of known size, for checking what the HBM-side counters
+    (the L2's read/write requests by size, FETCH_SIZE / WRITE_SIZE) read for bytes a kernel
+    must move."""
     hip = Hip()
     n = mib << 20
     a, b = ctypes.c_void_p(), ctypes.c_void_p()
@@ -103,6 +108,21 @@ def calibrate_copy(mib, repeats=4):
     hip.free(a.value)
     hip.free(b.value)
     print(json.dumps({"copy_bytes": n, "repeats": repeats}), flush=True)
+
+
+def calibrate_gather(mib, repeats=4):
+    """A gather of one 4-byte word from every 128-byte line of a buffer of known size (torch's
+    index_select, in a process of its own): what the read counters give for it tells their unit
+    for scattered reads like the analysis's, beside the streaming copy."""
+    import torch
+    n = mib << 20
+    src = torch.ones(n // 4, dtype=torch.int32, device="cuda")
+    idx = torch.arange(0, n // 4, 32, device="cuda")
+    for _ in range(repeats):
+        out = torch.index_select(src, 0, idx)
+    torch.cuda.synchronize()
+    print(json.dumps({"gather_bytes_spanned": n, "lines": int(idx.numel()), "repeats": repeats,
+                      "checksum": int(out.sum().item())}), flush=True)
 
 
 @functools.lru_cache(maxsize=1)
@@ -198,6 +218,7 @@ def main():
                     help="headline: the 1080p metric (default); batched: the 24-stream uneven-size set; "
                          "compact: the headline streams with the Compact preset (Baseline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--end-to-end", type=int, default=1, help="also time the step with its host transfers (1: yes)")
     ap.add_argument("--compare-persistent", type=int, default=1,
                     help="also time the persistent schedule (one launch per job) on the same job (1: yes)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal form")
@@ -212,9 +233,14 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--calibrate-gather", type=int, default=0, metavar="MIB",
+                    help="only run a one-word-per-line gather over MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.calibrate_gather:
+        calibrate_gather(args.calibrate_gather)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (independent GOP streams would "
@@ -361,10 +387,12 @@ def run_workload(gops, args):
         launches += nl
 
     value = total_mbs * args.steps / el
+    e2e = end_to_end(hip, gops, ans, srcs, decs, lvls, uploaded, sched, total_mbs, args) if args.end_to_end else None
     avg_launch_s = dev_ms / 1e3 / launches
     mbs_per_launch = (total_mbs + idr_mbs) * args.steps / launches
     achieved = step_bytes * args.steps / launches / avg_launch_s / 1e9
     headline = args.workload == "headline"
+    traffic = measured_traffic() if headline else None
     metric = {"headline": "1080p ME macroblocks/sec, 1 GPU: plain vs learned-proposal at <=1% BD-rate",
               "batched": "batched 24-stream ME macroblocks/sec, 1 GPU, plain form",
               "compact": "1080p ME macroblocks/sec, 1 GPU, Compact preset"}[args.workload]
@@ -401,10 +429,13 @@ def run_workload(gops, args):
                    "launches_per_step": launches / args.steps,
                    "schedule_tuning_s": {"%d/%d" % k: round(v, 4) for k, v in tune.items()},
                    "persistent_schedule": persist,
-                   "device_ms_per_step": round(dev_ms / args.steps, 3)},
+                   "device_ms_per_step": round(dev_ms / args.steps, 3),
+                   "end_to_end": e2e},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic() if headline else None,
+                     "traffic": traffic["per_round"] if traffic else None,
+                     "traffic_per_macroblock": traffic["per_macroblock"] if traffic else None,
+                     "traffic_ratio": traffic["ratio_to_algorithmic"] if traffic else None,
                      "kernel": kernel + (" (+ rgh_k_intra_step in launches with IDR frames)" if args.workload == "compact"
                                          else " (rgh_k_mixed_step in launches with IDR frames)"),
                      "avg_launch_us": round(avg_launch_s * 1e6, 3),
@@ -420,6 +451,63 @@ def run_workload(gops, args):
     for p in [q for q, _ in uploaded.values()] + [q for d in decs for q in d] + [q for l in lvls for q in l]:
         hip.free(p)
     return out
+
+
+def end_to_end(hip, gops, ans, srcs, decs, lvls, uploaded, sched, total_mbs, args):
+    """The step with its host transfers: every distinct source frame uploaded from pinned host
+    memory (the 12 streams share the 120 frames: 4 GOPs x 3 QPs), the analysis, and every frame's
+    decision records and levels downloaded into pinned host memory, each copy synchronous."""
+    from ragdoll_amd import LEVELS_PER_MB
+    planes = [(dp, arr) for dp, arr in uploaded.values()]
+    src_bytes = sum(a.nbytes for _, a in planes)
+    outs = []
+    for a, d, l in zip(ans, decs, lvls):
+        n = a.mb_w * a.mb_h
+        outs += [(p, n * 32) for p in d] + [(p, n * LEVELS_PER_MB * 2) for p in l]
+    out_bytes = sum(b for _, b in outs)
+    hs, hsrc = hip.host_alloc(src_bytes)
+    ho, hout = hip.host_alloc(out_bytes)
+    off = 0
+    offs = []
+    for _, arr in planes:
+        hsrc[off:off + arr.nbytes] = np.ascontiguousarray(arr).reshape(-1).view(np.uint8)
+        offs.append(off)
+        off += arr.nbytes
+
+    def one():
+        t = time.perf_counter()
+        for (dp, arr), o in zip(planes, offs):
+            hip.memcpy(dp, hs + o, arr.nbytes, 1)
+        t1 = time.perf_counter()
+        analyse_frames_e2e(sched)
+        t2 = time.perf_counter()
+        o = 0
+        for p, b in outs:
+            hip.memcpy(ho + o, p, b, 2)
+            o += b
+        t3 = time.perf_counter()
+        return t3 - t, t1 - t, t2 - t1, t3 - t2
+
+    def analyse_frames_e2e(s):
+        from ragdoll_amd import analyse_frames
+        analyse_frames(ans, srcs, decs, lvls, lag=s[0], groups=s[1], first_idr=True)
+
+    one()
+    tot = h2d = ana = d2h = 0.0
+    for _ in range(args.steps):
+        a, b, c, d = one()
+        tot += a
+        h2d += b
+        ana += c
+        d2h += d
+    hip.host_free(hs)
+    hip.host_free(ho)
+    k = args.steps
+    return {"value": round(total_mbs * k / tot, 1), "unit": "macroblocks/s", "ms_per_step": round(tot / k * 1e3, 3),
+            "h2d_ms": round(h2d / k * 1e3, 3), "analysis_ms": round(ana / k * 1e3, 3), "d2h_ms": round(d2h / k * 1e3, 3),
+            "h2d_bytes": src_bytes, "d2h_bytes": out_bytes,
+            "note": "distinct source frames up from pinned host memory, then every frame's decisions and levels "
+                    "(408 int16 per macroblock, stored densely) down to pinned host memory; synchronous copies"}
 
 
 def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value, hip):

@@ -24,6 +24,8 @@ class Hip:
         self.L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
         self.L.hipMemset.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t]
         self.L.hipFree.argtypes = [ctypes.c_void_p]
+        self.L.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+        self.L.hipHostFree.argtypes = [ctypes.c_void_p]
 
     def check(self, rc, what):
         if rc != 0:
@@ -58,6 +60,20 @@ class Hip:
 
     def free(self, ptr):
         self.L.hipFree(ctypes.c_void_p(ptr))
+
+    def host_alloc(self, nbytes):
+        """Pinned host memory of nbytes, as a writable uint8 numpy array (release with host_free)."""
+        p = ctypes.c_void_p()
+        self.check(self.L.hipHostMalloc(ctypes.byref(p), max(int(nbytes), 1), 0), "hipHostMalloc")
+        arr = np.ctypeslib.as_array((ctypes.c_uint8 * max(int(nbytes), 1)).from_address(p.value))
+        return p.value, arr
+
+    def host_free(self, ptr):
+        self.L.hipHostFree(ctypes.c_void_p(ptr))
+
+    def memcpy(self, dst, src, nbytes, kind):
+        """Raw copy between addresses (kind 1 host to device, 2 device to host, 3 device to device)."""
+        self.check(self.L.hipMemcpy(ctypes.c_void_p(dst), ctypes.c_void_p(src), ctypes.c_size_t(nbytes), kind), "hipMemcpy")
 
     def synchronize(self):
         self.check(self.L.hipDeviceSynchronize(), "hipDeviceSynchronize")
