@@ -274,6 +274,27 @@ rgh_status rgh_write_idr_slice_cavlc(const rgh_slice_config *cfg, int32_t idr_pi
                                      const rgh_mb_decision *decisions, const int16_t *levels,
                                      uint8_t *out, int64_t cap, int64_t *out_len);
 
+/* Entropy-codes one analysed P picture with CABAC as the Main (RGH_PROFILE_MAIN) or High
+ * (RGH_PROFILE_HIGH: with transform_size_8x8_flag) slice the reference writes: one Annex B NAL
+ * unit (nal_ref_idc 3, type 1) holding the slice header with cabac_init_idc 0, the
+ * cabac_alignment_one_bits and the arithmetic-coded macroblocks, end_of_slice_flag after each.
+ * Records as rgh_write_p_slice_cavlc (P_Skip, P_L0_16x16 with or without the 8x8 transform,
+ * I_16x16; one QP); the Compact preset is RGH_ERR_UNSUPPORTED (with CABAC it codes B pictures).
+ * *out_len receives the size; RGH_ERR_INVALID_ARGUMENT if it exceeds cap.
+ * Replaces: the P slice of encode_frame_body with CABAC (enc.rs:2346-2440; begin_coded_mb /
+ * code_skip / code_inter / code_i16 syntax :2779-3306, :3587-3720; h264.rs:3327-3552
+ * encode_cabac_cbp_and_residuals; cabac.rs:504-1130 CabacEncoder). */
+rgh_status rgh_write_p_slice_cabac(const rgh_slice_config *cfg, int32_t profile,
+                                   const rgh_mb_decision *decisions, const int16_t *levels,
+                                   uint8_t *out, int64_t cap, int64_t *out_len);
+
+/* The IDR counterpart of rgh_write_p_slice_cabac: one NAL unit of type 5 with idr_pic_id, the
+ * I_4x4 (transform_size_8x8_flag 0 under High) and I_16x16 records of rgh_analyse_idr.
+ * Replaces: the IDR slice of encode_frame_body with CABAC (code_i4 :3441, code_i16 :3587). */
+rgh_status rgh_write_idr_slice_cabac(const rgh_slice_config *cfg, int32_t profile, int32_t idr_pic_id,
+                                     const rgh_mb_decision *decisions, const int16_t *levels,
+                                     uint8_t *out, int64_t cap, int64_t *out_len);
+
 /* Stream parameters of rgh_write_headers. */
 typedef struct rgh_stream_config {
     int32_t width, height; /* display size */

@@ -3338,6 +3338,421 @@ static void copy_plane(const plane_t *p, uint8_t *dst) {
     for (int r = 0; r < p->h; r++) memcpy(dst + (long)r * p->w, p->buf + pat(p, 0, r), (size_t)p->w);
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* CABAC slice data (Main/High): the syntax of code_skip / begin_coded_mb / code_inter /        */
+/* code_i4 / code_i16 (enc.rs:2779-3306, :3441-3720) with encode_cabac_cbp_and_residuals and   */
+/* encode_cabac_chroma_residuals (h264.rs:3327-3552), written after the frame's analysis from   */
+/* its decision records and levels (entropy coding feeds nothing back into Balanced decisions). */
+/* The arithmetic coder is ITU-T H.264 9.3.4.2 in its bit-serial form (PutBit with              */
+/* bitsOutstanding); the reference (cabac.rs:504-611) writes the same bits byte by byte.        */
+/* ------------------------------------------------------------------------------------------ */
+#include "h264_cabac_tables.h"
+
+typedef struct {
+    bits_t *w;
+    uint32_t low, range;
+    int first, outstanding;
+    uint8_t st[460]; /* pStateIdx << 1 | valMPS */
+} cabac_t;
+
+/* 9.3.1.1: context variables from (m, n) and SliceQPY; 9.3.4.1: encoder registers. */
+static void cabac_init(cabac_t *c, bits_t *w, int slice_qp, int table) {
+    c->w = w;
+    c->low = 0;
+    c->range = 510;
+    c->first = 1;
+    c->outstanding = 0;
+    int q = iclamp(slice_qp, 0, 51);
+    for (int i = 0; i < 460; i++) {
+        int m = CABAC_INIT_MN[table][i][0], n = CABAC_INIT_MN[table][i][1];
+        int pre = iclamp(((m * q) >> 4) + n, 1, 126);
+        c->st[i] = (uint8_t)(pre <= 63 ? (63 - pre) << 1 : ((pre - 64) << 1) | 1);
+    }
+}
+static void cabac_put_bit(cabac_t *c, int b) { /* 9.3.4.2 PutBit */
+    if (c->first)
+        c->first = 0;
+    else
+        put32(c->w, (uint32_t)b, 1);
+    for (; c->outstanding > 0; c->outstanding--) put32(c->w, (uint32_t)(1 - b), 1);
+}
+static void cabac_renorm(cabac_t *c) { /* 9.3.4.2 RenormE */
+    while (c->range < 256) {
+        if (c->low < 256) {
+            cabac_put_bit(c, 0);
+        } else if (c->low >= 512) {
+            c->low -= 512;
+            cabac_put_bit(c, 1);
+        } else {
+            c->low -= 256;
+            c->outstanding++;
+        }
+        c->range <<= 1;
+        c->low <<= 1;
+    }
+}
+static void cabac_bin(cabac_t *c, int ctx, int bin) { /* 9.3.4.2 EncodeDecision */
+    int s = c->st[ctx] >> 1, mps = c->st[ctx] & 1;
+    uint32_t rlps = CABAC_RANGE_LPS[s][(c->range >> 6) & 3];
+    c->range -= rlps;
+    if (bin != mps) {
+        c->low += c->range;
+        c->range = rlps;
+        if (s == 0) mps = 1 - mps;
+        s = CABAC_NEXT_LPS[s];
+    } else if (s < 62) {
+        s++;
+    }
+    c->st[ctx] = (uint8_t)((s << 1) | mps);
+    cabac_renorm(c);
+}
+static void cabac_bypass(cabac_t *c, int bin) { /* 9.3.4.4 */
+    c->low <<= 1;
+    if (bin) c->low += c->range;
+    if (c->low >= 1024) {
+        cabac_put_bit(c, 1);
+        c->low -= 1024;
+    } else if (c->low < 512) {
+        cabac_put_bit(c, 0);
+    } else {
+        c->low -= 512;
+        c->outstanding++;
+    }
+}
+static void cabac_terminate(cabac_t *c, int bin) { /* 9.3.4.5, EncodeFlush writes rbsp_stop_one_bit */
+    c->range -= 2;
+    if (bin) {
+        c->low += c->range;
+        c->range = 2;
+        cabac_renorm(c);
+        cabac_put_bit(c, (int)((c->low >> 9) & 1));
+        put32(c->w, ((c->low >> 7) & 3) | 1, 2);
+    } else {
+        cabac_renorm(c);
+    }
+}
+
+/* I-slice mb_type (cabac.rs:738 encode_intra_mb_type; Table 9-36 binarization, ctxIdx 3-10,
+ * or 17-20 after the P-slice prefix): 0 I_NxN, 1..24 I_16x16. */
+static void cabac_intra_type(cabac_t *c, int in_p, int inc, int t) {
+    int b0 = in_p ? 17 : 3 + inc;
+    cabac_bin(c, b0, t != 0);
+    if (!t) return;
+    cabac_terminate(c, 0);
+    int k = t - 1, mode = k & 3, cc = (k >> 2) % 3, luma = k >= 12;
+    cabac_bin(c, in_p ? 18 : 6, luma);
+    cabac_bin(c, in_p ? 19 : 7, cc != 0);
+    if (cc) cabac_bin(c, in_p ? 19 : 8, cc == 2);
+    cabac_bin(c, in_p ? 20 : 9, mode >> 1);
+    cabac_bin(c, in_p ? 20 : 10, mode & 1);
+}
+/* Exp-Golomb suffix of a UEGk binarization in bypass bins (9.3.2.3). */
+static void cabac_eg(cabac_t *c, uint32_t v, int k) {
+    while (v >= (1u << k)) {
+        cabac_bypass(c, 1);
+        v -= 1u << k;
+        k++;
+    }
+    cabac_bypass(c, 0);
+    while (k--) cabac_bypass(c, (int)((v >> k) & 1));
+}
+/* mvd_l0 component (cabac.rs:940; UEG3, uCoff 9, signed; ctxIdx 40-46 / 47-53). */
+static void cabac_mvd(cabac_t *c, int comp, int amvd, int v) {
+    int base = comp ? 47 : 40, a = iabs(v);
+    cabac_bin(c, base + (amvd < 3 ? 0 : amvd <= 32 ? 1 : 2), a != 0);
+    if (!a) return;
+    for (int i = 1; i < 9 && i < a; i++) cabac_bin(c, base + 3 + imin(i - 1, 3), 1);
+    if (a < 9)
+        cabac_bin(c, base + 3 + imin(a - 1, 3), 0);
+    else
+        cabac_eg(c, (uint32_t)(a - 9), 3);
+    cabac_bypass(c, v < 0);
+}
+
+/* One residual block (cabac.rs:981 encode_residual_block_cat, 9.3.2.3 / 9.3.3.1.3): coded_block_flag
+ * (not for 8x8 blocks), the significance map, then levels from the last one down. */
+static const int CB_CBF[5] = {85, 89, 93, 97, 101};
+static const int CB_SIG[6] = {105, 120, 134, 149, 152, 402};
+static const int CB_LAST[6] = {166, 181, 195, 210, 213, 417};
+static const int CB_ABS[6] = {227, 237, 247, 257, 266, 426};
+/* Table 9-43, frame-coded 8x8 blocks: ctxIdxInc of significant_coeff_flag and of
+ * last_significant_coeff_flag by scan position. */
+static const uint8_t SIG8_INC[63] = {0,  1,  2,  3,  4,  5,  5,  4,  4,  3,  3,  4,  4,  4,  5,  5,
+                                     4,  4,  4,  4,  3,  3,  6,  7,  7,  7,  8,  9,  10, 9,  8,  7,
+                                     7,  6,  11, 12, 13, 11, 6,  7,  8,  9,  14, 10, 9,  8,  6,  11,
+                                     12, 13, 11, 6,  9,  14, 10, 9,  11, 12, 13, 11, 14, 10, 12};
+static const uint8_t LAST8_INC[63] = {0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 2,
+                                      2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 2, 3, 3, 3, 3, 3, 3, 3, 3, 4, 4,
+                                      4, 4, 4, 4, 4, 4, 5, 5, 5, 5, 6, 6, 6, 6, 7, 7, 7, 7, 8, 8, 8};
+static int cabac_block(cabac_t *c, int cat, int cbf_inc, const int16_t *sc, int n) {
+    int last = n - 1;
+    while (last >= 0 && sc[last] == 0) last--;
+    if (cat < 5) {
+        cabac_bin(c, CB_CBF[cat] + cbf_inc, last >= 0);
+        if (last < 0) return 0;
+    }
+    for (int i = 0; i < n - 1 && i <= last; i++) {
+        int si = cat == 5 ? SIG8_INC[i] : (cat == 3 ? imin(i, 2) : i);
+        int li = cat == 5 ? LAST8_INC[i] : (cat == 3 ? imin(i, 2) : i);
+        cabac_bin(c, CB_SIG[cat] + si, sc[i] != 0);
+        if (sc[i]) cabac_bin(c, CB_LAST[cat] + li, i == last);
+    }
+    int eq1 = 0, gt1 = 0, gmax = cat == 3 ? 3 : 4, base = CB_ABS[cat];
+    for (int i = last; i >= 0; i--) {
+        if (!sc[i]) continue;
+        int a = iabs(sc[i]) - 1;
+        cabac_bin(c, base + (gt1 ? 0 : imin(4, 1 + eq1)), a != 0);
+        if (a) {
+            int c1 = base + 5 + imin(gt1, gmax);
+            for (int k = 1; k < 14 && k < a; k++) cabac_bin(c, c1, 1);
+            if (a < 14)
+                cabac_bin(c, c1, 0);
+            else
+                cabac_eg(c, (uint32_t)(a - 14), 0);
+            gt1++;
+        } else {
+            eq1++;
+        }
+        cabac_bypass(c, sc[i] < 0);
+    }
+    return 1;
+}
+
+/* Neighbour state of the picture as coded so far, for the ctxIdxInc derivations of 9.3.3.1.1. */
+typedef struct {
+    int mb_w, mb_h;
+    uint8_t *skip, *intra, *i16, *cbpl, *cbpc, *cmode, *t8, *dc_cbf, *cdc_cbf[2];
+    uint8_t *cbf4;        /* luma 4x4 (8x8 blocks: all four entries) coded_block_flag, 4 mb_w x 4 mb_h */
+    uint8_t *ccbf[2];     /* chroma AC coded_block_flag, 2 mb_w x 2 mb_h per plane */
+    uint8_t *modes;       /* intra 4x4 modes (DC for other macroblocks) */
+    int16_t (*mvd)[2];    /* mvd of each 4x4 block (0 for skip and intra) */
+} cabac_nb_t;
+
+static void cabac_nb_init(cabac_nb_t *s, int mb_w, int mb_h) {
+    int n = mb_w * mb_h;
+    s->mb_w = mb_w;
+    s->mb_h = mb_h;
+    uint8_t **f[] = {&s->skip, &s->intra, &s->i16, &s->cbpl, &s->cbpc, &s->cmode, &s->t8, &s->dc_cbf, &s->cdc_cbf[0],
+                     &s->cdc_cbf[1]};
+    for (size_t i = 0; i < sizeof(f) / sizeof(f[0]); i++) *f[i] = (uint8_t *)calloc((size_t)n, 1);
+    s->cbf4 = (uint8_t *)calloc((size_t)n * 16, 1);
+    s->ccbf[0] = (uint8_t *)calloc((size_t)n * 4, 1);
+    s->ccbf[1] = (uint8_t *)calloc((size_t)n * 4, 1);
+    s->modes = (uint8_t *)malloc((size_t)n * 16);
+    memset(s->modes, 2, (size_t)n * 16);
+    s->mvd = calloc((size_t)n * 16, sizeof(*s->mvd));
+}
+static void cabac_nb_free(cabac_nb_t *s) {
+    uint8_t *f[] = {s->skip, s->intra, s->i16, s->cbpl, s->cbpc, s->cmode, s->t8, s->dc_cbf, s->cdc_cbf[0],
+                    s->cdc_cbf[1], s->cbf4, s->ccbf[0], s->ccbf[1], s->modes};
+    for (size_t i = 0; i < sizeof(f) / sizeof(f[0]); i++) free(f[i]);
+    free(s->mvd);
+}
+
+/* coded_block_flag ctxIdxInc of a luma 4x4 block at grid (gx, gy) (9.3.3.1.1.9 for
+ * ctxBlockCat 1 and 2): a neighbour block counts its flag when its macroblock is available,
+ * not skipped, and its 8x8 quadrant is coded (an 8x8-transform quadrant's flag is 1); a
+ * neighbour outside the picture counts 1 for an intra macroblock, 0 for an inter one. */
+static int cbf4_cond(const cabac_nb_t *s, int gx, int gy, int intra) {
+    if (gx < 0 || gy < 0) return intra;
+    int m = (gy >> 2) * s->mb_w + (gx >> 2);
+    if (s->skip[m]) return 0;
+    int q = ((gy & 2) ? 2 : 0) + ((gx & 2) ? 1 : 0);
+    if (!((s->cbpl[m] >> q) & 1)) return 0;
+    return s->cbf4[gy * s->mb_w * 4 + gx];
+}
+/* Chroma AC (ctxBlockCat 4): the neighbour's flag when it codes chroma AC. */
+static int ccbf_cond(const cabac_nb_t *s, int p, int cx, int cy, int intra) {
+    if (cx < 0 || cy < 0) return intra;
+    int m = (cy >> 1) * s->mb_w + (cx >> 1);
+    if (s->skip[m] || s->cbpc[m] != 2) return 0;
+    return s->ccbf[p][cy * s->mb_w * 2 + cx];
+}
+
+/* coded_block_pattern (9.3.2.6 / 9.3.3.1.1.4), transform_size_8x8_flag for inter macroblocks,
+ * mb_qp_delta and the residual blocks: the part of a macroblock after its prediction syntax. */
+static void cabac_cbp_residual(cabac_t *c, cabac_nb_t *s, int mb_x, int mb_y, int intra, int t8_flag_here,
+                               int cbpl, int cbpc, int use8, const int16_t *lv, int *prev_dqp) {
+    int mw = s->mb_w, m = mb_y * mw + mb_x;
+    for (int q = 0; q < 4; q++) {
+        int bx = q & 1, by = q >> 1, a, b;
+        if (bx) a = !((cbpl >> (q - 1)) & 1);
+        else a = mb_x > 0 && (s->skip[m - 1] || !((s->cbpl[m - 1] >> (by * 2 + 1)) & 1));
+        if (by) b = !((cbpl >> (q - 2)) & 1);
+        else b = mb_y > 0 && (s->skip[m - mw] || !((s->cbpl[m - mw] >> (2 + bx)) & 1));
+        cabac_bin(c, 73 + a + 2 * b, (cbpl >> q) & 1);
+    }
+    int la = mb_x > 0 && !s->skip[m - 1] ? s->cbpc[m - 1] : 0;
+    int ta = mb_y > 0 && !s->skip[m - mw] ? s->cbpc[m - mw] : 0;
+    cabac_bin(c, 77 + (la != 0) + 2 * (ta != 0), cbpc != 0);
+    if (cbpc) cabac_bin(c, 81 + (la == 2) + 2 * (ta == 2), cbpc == 2);
+    s->cbpl[m] = (uint8_t)cbpl;
+    s->cbpc[m] = (uint8_t)cbpc;
+    if (!cbpl && !cbpc) {
+        *prev_dqp = 0;
+        return;
+    }
+    if (t8_flag_here && cbpl) {
+        int inc = (mb_x > 0 && s->t8[m - 1]) + (mb_y > 0 && s->t8[m - mw]);
+        cabac_bin(c, 399 + inc, use8);
+    }
+    cabac_bin(c, 60 + (*prev_dqp != 0), 0); /* mb_qp_delta 0: fixed QP */
+    *prev_dqp = 0;
+    int gs = mw * 4;
+    if (use8) {
+        for (int q = 0; q < 4; q++) {
+            int gx = mb_x * 4 + (q & 1) * 2, gy = mb_y * 4 + (q >> 1) * 2, on = (cbpl >> q) & 1;
+            if (on) cabac_block(c, 5, 0, lv + ORA_LV_LUMA + q * 64, 64);
+            for (int dy = 0; dy < 2; dy++)
+                for (int dx = 0; dx < 2; dx++) s->cbf4[(gy + dy) * gs + gx + dx] = (uint8_t)on;
+        }
+    } else {
+        for (int b = 0; b < 16; b++) {
+            int bx, by;
+            blk_xy(b, &bx, &by);
+            int gx = mb_x * 4 + bx, gy = mb_y * 4 + by, f = 0;
+            if ((cbpl >> (b >> 2)) & 1) {
+                int inc = cbf4_cond(s, gx - 1, gy, intra) + 2 * cbf4_cond(s, gx, gy - 1, intra);
+                f = cabac_block(c, 2, inc, lv + ORA_LV_LUMA + b * 16, 16);
+            }
+            s->cbf4[gy * gs + gx] = (uint8_t)f;
+        }
+    }
+    if (cbpc >= 1)
+        for (int p = 0; p < 2; p++) {
+            int a = mb_x > 0 ? (!s->skip[m - 1] && s->cdc_cbf[p][m - 1]) : intra;
+            int b = mb_y > 0 ? (!s->skip[m - mw] && s->cdc_cbf[p][m - mw]) : intra;
+            s->cdc_cbf[p][m] = (uint8_t)cabac_block(c, 3, a + 2 * b, lv + ORA_LV_CHROMA_DC + p * 4, 4);
+        }
+    if (cbpc >= 2)
+        for (int p = 0; p < 2; p++)
+            for (int b = 0; b < 4; b++) {
+                int cx = mb_x * 2 + (b & 1), cy = mb_y * 2 + (b >> 1);
+                int inc = ccbf_cond(s, p, cx - 1, cy, intra) + 2 * ccbf_cond(s, p, cx, cy - 1, intra);
+                s->ccbf[p][cy * mw * 2 + cx] =
+                    (uint8_t)cabac_block(c, 4, inc, lv + ORA_LV_CHROMA_AC + (p * 4 + b) * 16 + 1, 15);
+            }
+}
+
+static void cabac_chroma_mode(cabac_t *c, const cabac_nb_t *s, int mb_x, int mb_y, int mode) {
+    int mw = s->mb_w, m = mb_y * mw + mb_x;
+    int a = mb_x > 0 && s->intra[m - 1] && s->cmode[m - 1] != 0;
+    int b = mb_y > 0 && s->intra[m - mw] && s->cmode[m - mw] != 0;
+    cabac_bin(c, 64 + a + b, mode != 0);
+    if (mode) {
+        cabac_bin(c, 67, mode != 1);
+        if (mode != 1) cabac_bin(c, 67, mode == 3);
+    }
+}
+
+/* The slice data of the picture just analysed: every macroblock's syntax, then end_of_slice_flag
+ * (terminate) after each (enc.rs:2382-2434). */
+static void cabac_slice(ora_enc *e, bits_t *w, int is_idr, int slice_qp) {
+    cabac_t c;
+    cabac_init(&c, w, slice_qp, is_idr ? 0 : 1);
+    cabac_nb_t s;
+    int mw = e->mb_w, mh = e->mb_h, gs = mw * 4, prev_dqp = 0;
+    cabac_nb_init(&s, mw, mh);
+    for (int mb_y = 0; mb_y < mh; mb_y++)
+        for (int mb_x = 0; mb_x < mw; mb_x++) {
+            int m = mb_y * mw + mb_x;
+            const ora_mb_decision *d = &e->dec[m];
+            const int16_t *lv = e->levels + (size_t)m * ORA_LEVELS_PER_MB;
+            int sa = mb_x > 0 && !s.skip[m - 1], sb = mb_y > 0 && !s.skip[m - mw];
+            if (!is_idr) cabac_bin(&c, 11 + sa + sb, d->mb_type == 0);
+            if (d->mb_type == 0) {
+                s.skip[m] = 1;
+                prev_dqp = 0;
+            } else if (d->mb_type == 1) { /* P_L0_16x16 (enc.rs:3196 code_inter) */
+                cabac_bin(&c, 14, 0);
+                cabac_bin(&c, 15, 0);
+                cabac_bin(&c, 16, 0);
+                int gx = mb_x * 4, gy = mb_y * 4, mvd[2] = {d->mv[0] - d->mvp[0], d->mv[1] - d->mvp[1]};
+                for (int k = 0; k < 2; k++) {
+                    int amvd = (gx > 0 ? iabs(s.mvd[gy * gs + gx - 1][k]) : 0) + (gy > 0 ? iabs(s.mvd[(gy - 1) * gs + gx][k]) : 0);
+                    cabac_mvd(&c, k, amvd, mvd[k]);
+                }
+                int use8 = d->transform_8x8 && d->cbp_luma;
+                s.t8[m] = (uint8_t)use8;
+                cabac_cbp_residual(&c, &s, mb_x, mb_y, 0, e->t8x8, d->cbp_luma, d->cbp_chroma, use8, lv, &prev_dqp);
+                for (int by = 0; by < 4; by++)
+                    for (int bx = 0; bx < 4; bx++) {
+                        s.mvd[(gy + by) * gs + gx + bx][0] = (int16_t)mvd[0];
+                        s.mvd[(gy + by) * gs + gx + bx][1] = (int16_t)mvd[1];
+                    }
+            } else if (d->mb_type == 2) { /* I_16x16 (enc.rs:3587 code_i16) */
+                int t = 1 + d->i16_mode + 4 * d->cbp_chroma + (d->cbp_luma ? 12 : 0);
+                if (is_idr) {
+                    cabac_intra_type(&c, 0, (mb_x > 0 && s.i16[m - 1]) + (mb_y > 0 && s.i16[m - mw]), t);
+                } else {
+                    cabac_bin(&c, 14, 1);
+                    cabac_intra_type(&c, 1, 0, t);
+                }
+                s.intra[m] = s.i16[m] = 1;
+                cabac_chroma_mode(&c, &s, mb_x, mb_y, d->chroma_mode);
+                s.cmode[m] = d->chroma_mode;
+                cabac_bin(&c, 60 + (prev_dqp != 0), 0); /* mb_qp_delta 0 */
+                prev_dqp = 0;
+                int a = mb_x > 0 ? s.dc_cbf[m - 1] : 1, b = mb_y > 0 ? s.dc_cbf[m - mw] : 1;
+                s.dc_cbf[m] = (uint8_t)cabac_block(&c, 0, a + 2 * b, lv + ORA_LV_LUMA_DC, 16);
+                s.cbpl[m] = d->cbp_luma;
+                s.cbpc[m] = d->cbp_chroma;
+                if (d->cbp_luma)
+                    for (int k = 0; k < 16; k++) {
+                        int bx, by;
+                        blk_xy(k, &bx, &by);
+                        int gx = mb_x * 4 + bx, gy = mb_y * 4 + by;
+                        int inc = cbf4_cond(&s, gx - 1, gy, 1) + 2 * cbf4_cond(&s, gx, gy - 1, 1);
+                        s.cbf4[gy * gs + gx] = (uint8_t)cabac_block(&c, 1, inc, lv + ORA_LV_LUMA + k * 16 + 1, 15);
+                    }
+                int cbpc = d->cbp_chroma;
+                if (cbpc >= 1)
+                    for (int p = 0; p < 2; p++) {
+                        int pa = mb_x > 0 ? (!s.skip[m - 1] && s.cdc_cbf[p][m - 1]) : 1;
+                        int pb = mb_y > 0 ? (!s.skip[m - mw] && s.cdc_cbf[p][m - mw]) : 1;
+                        s.cdc_cbf[p][m] = (uint8_t)cabac_block(&c, 3, pa + 2 * pb, lv + ORA_LV_CHROMA_DC + p * 4, 4);
+                    }
+                if (cbpc >= 2)
+                    for (int p = 0; p < 2; p++)
+                        for (int k = 0; k < 4; k++) {
+                            int cx = mb_x * 2 + (k & 1), cy = mb_y * 2 + (k >> 1);
+                            int inc = ccbf_cond(&s, p, cx - 1, cy, 1) + 2 * ccbf_cond(&s, p, cx, cy - 1, 1);
+                            s.ccbf[p][cy * mw * 2 + cx] =
+                                (uint8_t)cabac_block(&c, 4, inc, lv + ORA_LV_CHROMA_AC + (p * 4 + k) * 16 + 1, 15);
+                        }
+            } else { /* I_4x4 (enc.rs:3441 code_i4) */
+                if (is_idr) {
+                    cabac_intra_type(&c, 0, (mb_x > 0 && s.i16[m - 1]) + (mb_y > 0 && s.i16[m - mw]), 0);
+                } else {
+                    cabac_bin(&c, 14, 1);
+                    cabac_intra_type(&c, 1, 0, 0);
+                }
+                if (e->t8x8) cabac_bin(&c, 399 + (mb_x > 0 && s.t8[m - 1]) + (mb_y > 0 && s.t8[m - mw]), 0);
+                uint64_t packed;
+                memcpy(&packed, d->mv, sizeof(packed));
+                for (int k = 0; k < 16; k++) {
+                    int bx, by;
+                    blk_xy(k, &bx, &by);
+                    int gx = mb_x * 4 + bx, gy = mb_y * 4 + by;
+                    int mode = (int)((packed >> (4 * k)) & 15), pm = 2;
+                    if (gx > 0 && gy > 0) pm = imin(s.modes[gy * gs + gx - 1], s.modes[(gy - 1) * gs + gx]);
+                    cabac_bin(&c, 68, mode == pm);
+                    if (mode != pm) {
+                        int rem = mode < pm ? mode : mode - 1;
+                        for (int i = 0; i < 3; i++) cabac_bin(&c, 69, (rem >> i) & 1);
+                    }
+                    s.modes[gy * gs + gx] = (uint8_t)mode;
+                }
+                s.intra[m] = 1;
+                cabac_chroma_mode(&c, &s, mb_x, mb_y, d->chroma_mode);
+                s.cmode[m] = d->chroma_mode;
+                cabac_cbp_residual(&c, &s, mb_x, mb_y, 1, 0, d->cbp_luma, d->cbp_chroma, 0, lv, &prev_dqp);
+            }
+            cabac_terminate(&c, m + 1 == mw * mh);
+        }
+    cabac_nb_free(&s);
+}
+
 /* enc.rs:223 EncState::prepare */
 static void prepare(ora_enc *e, int slice_qp) {
     int n = e->mb_w * e->mb_h;
@@ -3416,6 +3831,36 @@ long ora_encode_frame(ora_enc *e, const uint8_t *y, const uint8_t *u, const uint
     if (wp) {
         if (!is_idr && e->skip_run > 0) put_ue(wp, (uint64_t)e->skip_run);
         bits_trailing(wp);
+    } else if (e->cabac) {
+        /* h264.rs:1071 SliceHeader::write with cabac_init_idc 0 for P slices, then
+         * cabac_alignment_one_bit up to the byte boundary and the arithmetic-coded slice data
+         * (enc.rs:2346-2440). */
+        wp = &w;
+        put_ue(wp, 0);
+        put_ue(wp, is_idr ? 7 : 5);
+        put_ue(wp, 0);
+        put32(wp, gop_idx & 15, 4);
+        if (is_idr) put_ue(wp, e->frame_count / (unsigned)e->keyint);
+        put32(wp, (gop_idx * 2) & 63, 6);
+        if (is_idr) {
+            put32(wp, 0, 1);
+            put32(wp, 0, 1);
+        } else {
+            put32(wp, 0, 1);
+            put32(wp, 0, 1);
+            put32(wp, 0, 1);
+            put_ue(wp, 0); /* cabac_init_idc */
+        }
+        put_se(wp, slice_qp - e->qp);
+        int dis = slice_qp == 0 ? 1 : 0;
+        put_ue(wp, (uint64_t)dis);
+        if (dis != 1) {
+            put_se(wp, 0);
+            put_se(wp, 0);
+        }
+        while (wp->n & 7) put32(wp, 1, 1);
+        cabac_slice(e, wp, is_idr, slice_qp);
+        while (wp->n & 7) put32(wp, 0, 1);
     }
     copy_plane(&e->cur.y, e->pre_y);
     copy_plane(&e->cur.u, e->pre_u);

@@ -1,13 +1,14 @@
 /*
  * CPU oracle: a plain-C restatement of ragdoll's Balanced H.264 encoder (IDR + P frames,
- * CAVLC, in-loop deblocking), written from the reference's source text.
+ * CAVLC and CABAC, in-loop deblocking), written from the reference's source text.
  *
  * TEST INFRASTRUCTURE ONLY. Nothing in the product (ragdoll_amd/, include/) links, imports or
  * calls this code; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
  * and only as the checker or the timed CPU baseline.
  *
  * Reference files restated (paths relative to the ragdoll repository root):
- *   crates/ragdoll-codec/src/h264/enc.rs      frame coder, P analysis, CAVLC writer, deblock
+ *   crates/ragdoll-codec/src/h264/enc.rs      frame coder, P analysis, CAVLC/CABAC syntax, deblock
+ *   crates/ragdoll-codec/src/cabac.rs         CABAC engine, context tables, binarizations
  *   crates/ragdoll-codec/src/h264.rs          tables, SPS/PPS/slice header, intra predictors,
  *                                             MVP (pred_motion_partition), H264Encoder
  *   crates/ragdoll-simd/src/h264_enc.rs       SAD/SSD/SATD, DCT/quant, hpel window, chroma MC
@@ -19,8 +20,9 @@
  * Pinning (see tests/): the Baseline path reproduces the reference's committed golden clips
  * qa/corpus/synth/clip_160x90_30fps.mp4 and clip_320x180_25fps.mkv byte for byte; kernels are
  * further checked against closed forms and brute force. The CABAC (Main/High) configuration
- * runs the same decisions with the 8x8-transform choice but writes no bitstream: its only pins
- * are self-consistency properties ("restatement-pinned").
+ * runs the same decisions with the 8x8-transform choice and writes CABAC slices; those are
+ * pinned by an independent parser of ITU-T H.264 9.3 (tests/cabac_decoder.py) that also reads
+ * libx264's CABAC slices to their exact ends.
  */
 #ifndef RAGDOLL_H264_ORACLE_H
 #define RAGDOLL_H264_ORACLE_H

@@ -11,8 +11,9 @@ import os
 import numpy as np
 
 from .abi import (DECISION_DTYPE, LEVELS_PER_MB, PROFILES, RghError, StreamAnalyzer, load_library,
-                  analyse_batch, analyse_frames, write_p_slice_cavlc, write_idr_slice_cavlc, write_headers)
+                  analyse_batch, analyse_frames, write_p_slice_cavlc, write_idr_slice_cavlc, write_p_slice_cabac,
+                  write_idr_slice_cabac, write_headers)
 
 __all__ = ["DECISION_DTYPE", "LEVELS_PER_MB", "PROFILES", "RghError", "StreamAnalyzer",
            "load_library", "analyse_batch", "analyse_frames", "write_p_slice_cavlc", "write_idr_slice_cavlc",
-           "write_headers"]
+           "write_p_slice_cabac", "write_idr_slice_cabac", "write_headers"]

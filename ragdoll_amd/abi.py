@@ -22,8 +22,9 @@ assert DECISION_DTYPE.itemsize == 32
 
 EXPORTS = ["rgh_create", "rgh_destroy", "rgh_set_reference", "rgh_analyse_p_frame", "rgh_analyse_idr",
            "rgh_analyse_batch", "rgh_analyse_frames", "rgh_analyse_frames_noalloc", "rgh_write_p_slice_cavlc",
-           "rgh_write_idr_slice_cavlc", "rgh_write_headers", "rgh_get_outputs", "rgh_get_mb_dims", "rgh_synchronize",
-           "rgh_get_stream", "rgh_enable_timing", "rgh_get_timing", "rgh_status_str", "rgh_last_error"]
+           "rgh_write_idr_slice_cavlc", "rgh_write_p_slice_cabac", "rgh_write_idr_slice_cabac", "rgh_write_headers",
+           "rgh_get_outputs", "rgh_get_mb_dims", "rgh_synchronize", "rgh_get_stream", "rgh_enable_timing",
+           "rgh_get_timing", "rgh_status_str", "rgh_last_error"]
 
 
 class RgConfig(ctypes.Structure):
@@ -100,6 +101,12 @@ def load_library(path=LIB_PATH):
                                           ctypes.POINTER(ctypes.c_int64)]
     L.rgh_write_idr_slice_cavlc.restype = I32
     L.rgh_write_idr_slice_cavlc.argtypes = [ctypes.POINTER(RgSliceConfig), I32, P, P, P, ctypes.c_int64,
+                                            ctypes.POINTER(ctypes.c_int64)]
+    L.rgh_write_p_slice_cabac.restype = I32
+    L.rgh_write_p_slice_cabac.argtypes = [ctypes.POINTER(RgSliceConfig), I32, P, P, P, ctypes.c_int64,
+                                          ctypes.POINTER(ctypes.c_int64)]
+    L.rgh_write_idr_slice_cabac.restype = I32
+    L.rgh_write_idr_slice_cabac.argtypes = [ctypes.POINTER(RgSliceConfig), I32, I32, P, P, P, ctypes.c_int64,
                                             ctypes.POINTER(ctypes.c_int64)]
     L.rgh_write_headers.restype = I32
     L.rgh_write_headers.argtypes = [ctypes.POINTER(RgStreamConfig), P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)]
@@ -309,6 +316,35 @@ def write_idr_slice_cavlc(decisions, levels, mb_w, mb_h, qp, idr_pic_id):
     n = ctypes.c_int64()
     _check(L.rgh_write_idr_slice_cavlc(ctypes.byref(cfg), idr_pic_id, _ptr(dec), _ptr(lv), _ptr(out), cap,
                                        ctypes.byref(n)), "rgh_write_idr_slice_cavlc")
+    return bytes(out[:n.value])
+
+
+def _slice_args(decisions, levels, mb_w, mb_h):
+    dec = np.ascontiguousarray(decisions, DECISION_DTYPE)
+    lv = np.ascontiguousarray(levels, np.int16)
+    if dec.shape != (mb_w * mb_h,) or lv.shape != (mb_w * mb_h, LEVELS_PER_MB):
+        raise ValueError("decisions/levels do not match the macroblock grid")
+    cap = mb_w * mb_h * 1024 + 64
+    return dec, lv, cap, np.zeros(cap, np.uint8), ctypes.c_int64()
+
+
+def write_p_slice_cabac(decisions, levels, mb_w, mb_h, qp, frame_num, poc_lsb, profile="high"):
+    """The Annex B NAL unit (bytes) of one analysed P picture, Main or High CABAC."""
+    L = load_library()
+    dec, lv, cap, out, n = _slice_args(decisions, levels, mb_w, mb_h)
+    cfg = RgSliceConfig(mb_w, mb_h, qp, frame_num, poc_lsb, SPEEDS["balanced"])
+    _check(L.rgh_write_p_slice_cabac(ctypes.byref(cfg), PROFILES[profile], _ptr(dec), _ptr(lv), _ptr(out), cap,
+                                     ctypes.byref(n)), "rgh_write_p_slice_cabac")
+    return bytes(out[:n.value])
+
+
+def write_idr_slice_cabac(decisions, levels, mb_w, mb_h, qp, idr_pic_id, profile="high"):
+    """The Annex B NAL unit (bytes) of one analysed IDR picture, Main or High CABAC."""
+    L = load_library()
+    dec, lv, cap, out, n = _slice_args(decisions, levels, mb_w, mb_h)
+    cfg = RgSliceConfig(mb_w, mb_h, qp, 0, 0, SPEEDS["balanced"])
+    _check(L.rgh_write_idr_slice_cabac(ctypes.byref(cfg), PROFILES[profile], idr_pic_id, _ptr(dec), _ptr(lv),
+                                       _ptr(out), cap, ctypes.byref(n)), "rgh_write_idr_slice_cabac")
     return bytes(out[:n.value])
 
 

@@ -18,7 +18,7 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(PKG, "libragdoll_hip.so")
 SOURCES = ["rgh_analyse.hip", "rgh_api.hip", "rgh_learned.hip"]
-HOST_SOURCES = ["rgh_cavlc.cpp"]
+HOST_SOURCES = ["rgh_cavlc.cpp", "rgh_cabac.cpp"]
 ARCH = os.environ.get("RGH_OFFLOAD_ARCH", "gfx950")
 # Device instructions no kernel may contain. v_ashr_pk_{u8,i8}_i32: the backend assumes they
 # zero bits 31:16 of their destination, which MI355X leaves unchanged. Packed FP32 arithmetic:

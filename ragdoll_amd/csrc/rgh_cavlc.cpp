@@ -14,48 +14,13 @@
 #include <vector>
 
 #include "../../include/ragdoll_hip.h"
+#include "rgh_bits.h"
 #include "rgh_cavlc_tables.h"
 
 namespace {
 
 using namespace rgh_cavlc;
-
-// MSB-first bit writer.
-class BitWriter {
-  public:
-    void put(uint32_t value, int len) {  // len <= 32
-        for (int i = len - 1; i >= 0; i--) bit((value >> i) & 1u);
-    }
-    void zeros(int n) {
-        for (int i = 0; i < n; i++) bit(0);
-    }
-    void ue(uint32_t v) {
-        const uint64_t x = (uint64_t)v + 1;
-        int len = 0;
-        while ((x >> len) > 1) len++;
-        zeros(len);
-        for (int i = len; i >= 0; i--) bit((uint32_t)(x >> i) & 1u);
-    }
-    void se(int v) { ue(v > 0 ? 2u * (uint32_t)v - 1u : 2u * (uint32_t)(-v)); }
-    void trailing() {  // rbsp_stop_one_bit and alignment
-        bit(1);
-        while (fill_ != 0) bit(0);
-    }
-    const std::vector<uint8_t> &bytes() const { return buf_; }
-
-  private:
-    void bit(uint32_t b) {
-        cur_ = (uint8_t)((cur_ << 1) | b);
-        if (++fill_ == 8) {
-            buf_.push_back(cur_);
-            cur_ = 0;
-            fill_ = 0;
-        }
-    }
-    std::vector<uint8_t> buf_;
-    uint8_t cur_ = 0;
-    int fill_ = 0;
-};
+using namespace rgh_bits;
 
 int absi(int v) { return v < 0 ? -v : v; }
 
@@ -192,27 +157,6 @@ uint32_t intra4_cbp_code(int cbp) {
     for (uint32_t i = 0; i < 48; i++)
         if (kGolombToIntra4x4Cbp[i] == cbp) return i;
     return 3;
-}
-
-// NAL unit (enc.rs:2643): start code, header byte, emulation prevention; appended to nal.
-void emit_nal(std::vector<uint8_t> &nal, int ref_idc, int type, const std::vector<uint8_t> &rbsp) {
-    nal.insert(nal.end(), {0, 0, 0, 1, (uint8_t)((ref_idc << 5) | type)});
-    int zeros = 0;
-    for (uint8_t b : rbsp) {
-        if (zeros == 2 && b <= 3) {
-            nal.push_back(3);
-            zeros = 0;
-        }
-        nal.push_back(b);
-        zeros = b == 0 ? zeros + 1 : 0;
-    }
-}
-
-rgh_status deliver(const std::vector<uint8_t> &nal, uint8_t *out, int64_t cap, int64_t *out_len) {
-    *out_len = (int64_t)nal.size();
-    if ((int64_t)nal.size() > cap) return RGH_ERR_INVALID_ARGUMENT;
-    memcpy(out, nal.data(), nal.size());
-    return RGH_OK;
 }
 
 // Luma 4x4 residual blocks of a coded 8x8 pattern (enc.rs:976 write_cavlc_residuals).

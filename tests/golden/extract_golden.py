@@ -13,7 +13,8 @@ parameters. The tests compare the oracle's output against these bytes.
 
 It also copies qa/corpus/synth/clip_64x36_30fps.y4m (manifest.toml:23-29; raw
 synth_video_frame(64, 36, 0..5), the reference's content generator) as `clip_64x36_30fps.y4m`,
-which pins the test-content generator byte for byte.
+which pins the test-content generator byte for byte, and two libx264 CABAC streams of the
+reference's decoder fixtures (crates/ragdoll-codec/tests/data/h264) for the CABAC parser.
 """
 import hashlib
 import json
@@ -190,6 +191,19 @@ def main():
     meta["clip_64x36_30fps"] = {"source": "qa/corpus/synth/clip_64x36_30fps.y4m",
                                 "source_sha256": hashlib.sha256(y4m).hexdigest(), "width": 64, "height": 36,
                                 "frames": 6}
+    # Two progressive CABAC streams written by libx264 that the reference's decoder tests hold
+    # (crates/ragdoll-codec/tests/h264_reference.rs:11-30, data in tests/data/h264): an
+    # independent encoder's I and P slices for the CABAC parser of the tests.
+    for fname, note in (("cabac_bpyramid_weighted.264",
+                         "Main, CABAC, 3 references, B pyramid, explicit weighted prediction, 128x96"),
+                        ("cabac_temporal_direct_8x8_cqm.264",
+                         "High, CABAC, 8x8 transform, default scaling matrices, temporal direct, 128x96")):
+        data = open(os.path.join(REF, "crates", "ragdoll-codec", "tests", "data", "h264", fname), "rb").read()
+        with open(os.path.join(HERE, fname), "wb") as f:
+            f.write(data)
+        meta[fname[:-4]] = {"source": "crates/ragdoll-codec/tests/data/h264/" + fname,
+                            "source_sha256": hashlib.sha256(data).hexdigest(), "encoder": "libx264",
+                            "note": note, "width": 128, "height": 96, "frames": 12}
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump(meta, f, indent=1)
 

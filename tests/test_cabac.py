@@ -1,0 +1,219 @@
+"""CABAC (Main / High) entropy coding, pinned by an independent parser (tests/cabac_decoder.py,
+written from ITU-T H.264 7.3 and 9.3).
+
+* The parser reads every I and P slice of two libx264 CABAC streams that the reference's
+  decoder tests hold (crates/ragdoll-codec/tests/data/h264) to their exact ends: all P
+  partition shapes, up to 4 reference pictures, the 8x8 transform in I and P slices. That pins
+  the arithmetic decoding engine, the context tables (m, n) of I slices and of
+  cabac_init_idc 0, the binarizations and the ctxIdxInc derivations against an encoder written
+  elsewhere; a wrong table entry makes the parse fail (shown below by planting one).
+* The oracle's Main and High streams (IDR and P pictures of moving content over a range of
+  QPs) parse to their ends with the standard's stop-bit position, and the parsed syntax equals
+  the oracle's decision records and levels.
+* The engine's tables satisfy the probability model of 9.3.1.1 / Table 9-44.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cabac_decoder as cd
+from oracle_lib import OracleEncoder
+from ragdoll_amd.synth import CONFIGS, SceneGenerator
+
+GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+X264 = ["cabac_temporal_direct_8x8_cqm.264", "cabac_bpyramid_weighted.264"]
+
+
+def x264_stream(name):
+    return open(GOLDEN + "/" + name, "rb").read()
+
+
+@pytest.mark.parametrize("name", X264)
+def test_parses_libx264_slices_to_their_ends(name):
+    pics = cd.parse_stream(x264_stream(name), strict_end=False)
+    assert len(pics) == 12
+    coded = [p for p in pics if not p.get("skipped")]
+    assert [p["type"] for p in coded] == [2, 0, 0, 0, 0]
+    kinds = set()
+    for p in coded:
+        assert len(p["mbs"]) == 48
+        kinds |= {m["type"] for m in p["mbs"]}
+        for m in p["mbs"]:
+            if "ref_idx" in m:
+                assert all(0 <= r < p["headers"][0]["num_ref_l0"] for r in m["ref_idx"])
+            for mode in m.get("intra_modes", []):
+                assert 0 <= mode <= 8
+    assert {"skip", "P16x16", "P16x8", "P8x16", "P8x8", "I_NxN", "I16"} <= kinds
+    if "8x8" in name:
+        assert any(m["t8"] for p in coded for m in p["mbs"] if m["type"] == "I_NxN")
+        assert any(m["t8"] for p in coded for m in p["mbs"] if m["type"].startswith("P"))
+    else:
+        assert max(p["headers"][0]["num_ref_l0"] for p in coded) == 4
+
+
+@pytest.mark.parametrize("table,ctx", [(0, 3), (0, 68), (0, 105), (0, 227), (1, 11), (1, 14), (1, 40), (1, 54),
+                                       (1, 73), (1, 399), (1, 402), (1, 426)])
+def test_a_wrong_context_initialisation_breaks_the_parse(table, ctx, monkeypatch):
+    # One (m, n) pair moved by a few steps of probability: the parse of the libx264 streams no
+    # longer ends where the slices end (or reads impossible syntax).
+    mn = [list(t) for t in cd.INIT_MN]
+    m, n = mn[table][ctx]
+    mn[table][ctx] = (m, n + 12 if n < 100 else n - 12)
+    monkeypatch.setattr(cd, "INIT_MN", mn)
+    with pytest.raises((cd.ParseError, IndexError, KeyError)):
+        for name in X264:
+            cd.parse_stream(x264_stream(name), strict_end=False)
+
+
+def test_a_wrong_8x8_significance_offset_breaks_the_parse(monkeypatch):
+    sig = list(cd.SIG8)
+    sig[5], sig[9] = sig[9], sig[5]
+    monkeypatch.setattr(cd, "SIG8", sig)
+    with pytest.raises((cd.ParseError, IndexError, KeyError)):
+        cd.parse_stream(x264_stream(X264[0]), strict_end=False)
+
+
+def test_range_table_follows_the_probability_model():
+    # 9.3.1.1: pLPS(s) = 0.5 a^s with a = (0.01875 / 0.5)^(1/63); rangeTabLPS[s][q] is pLPS(s)
+    # times the interval's representative range 288 + 64 q (Table 9-44), at most 128 for q = 0
+    # (an LPS range stays below half of the smallest range, 256). transIdxMPS is s + 1 up to 62;
+    # transIdxLPS is the state nearest to a pLPS + (1 - a).
+    a = (0.01875 / 0.5) ** (1 / 63)
+    for s in range(63):
+        p = 0.5 * a ** s
+        for q in range(4):
+            want = min(p * (288 + 64 * q), 128 if q == 0 else 1e9)
+            assert abs(cd.RANGE_LPS[s][q] - want) <= 1.5, (s, q, cd.RANGE_LPS[s][q], want)
+        if s > 0:
+            assert all(cd.RANGE_LPS[s][q] <= cd.RANGE_LPS[s - 1][q] for q in range(4))
+        target = a * p + (1 - a)
+        near = min(range(63), key=lambda k: abs(math.log(0.5 * a ** k) - math.log(target)))
+        assert abs(cd.NEXT_LPS[s] - near) <= 1, (s, cd.NEXT_LPS[s], near)
+    assert cd.RANGE_LPS[63] == [2, 2, 2, 2] and cd.NEXT_LPS[63] == 63  # the terminate state
+
+
+# ---------------------------------------------------------------------------------------------
+# The oracle's CABAC streams
+# ---------------------------------------------------------------------------------------------
+def level_record(rec):
+    """The parsed levels of one macroblock in the records' layout (408 int16)."""
+    lv = np.zeros(408, np.int16)
+    if rec["type"] in ("skip",):
+        return lv
+    L = rec.get("levels")
+    if not L:
+        return lv
+    if "luma8" in L:
+        for q, blk in enumerate(L["luma8"]):
+            lv[q * 64:(q + 1) * 64] = blk
+    if "luma4" in L:
+        for b, blk in enumerate(L["luma4"]):
+            if len(blk) == 15:
+                lv[b * 16 + 1:b * 16 + 16] = blk
+            else:
+                lv[b * 16:b * 16 + 16] = blk
+    if "i16dc" in L:
+        lv[256:272] = L["i16dc"]
+    for p in range(2):
+        lv[272 + 4 * p:276 + 4 * p] = L["cdc"][p]
+        for k in range(4):
+            o = 280 + (p * 4 + k) * 16
+            lv[o + 1:o + 16] = L["cac"][p][k]
+    return lv
+
+
+def check_picture(pic, dec, lev, label):
+    mbs = pic["mbs"]
+    assert len(mbs) == len(dec), label
+    for i, (m, d) in enumerate(zip(mbs, dec)):
+        where = "%s MB %d" % (label, i)
+        kind = {"skip": 0, "P16x16": 1, "I16": 2, "I_NxN": 3}[m["type"]]
+        assert kind == d["mb_type"], where
+        if kind == 0:
+            continue
+        assert m["dqp"] == 0, where
+        assert m["cbp_luma"] == d["cbp_luma"] and m["cbp_chroma"] == d["cbp_chroma"], where
+        assert m["t8"] == (1 if (d["transform_8x8"] and d["cbp_luma"]) else 0), where
+        if kind == 1:
+            assert m["mvd"] == [(int(d["mv"][0]) - int(d["mvp"][0]), int(d["mv"][1]) - int(d["mvp"][1]))], where
+            assert m["ref_idx"] == [0], where
+        else:
+            assert m["chroma_mode"] == d["chroma_mode"], where
+        if kind == 2:
+            assert m["i16_mode"] == d["i16_mode"], where
+        if kind == 3:
+            packed = int(np.frombuffer(d.tobytes()[8:16], np.uint64)[0])
+            assert m["intra_modes"] == [(packed >> (4 * b)) & 15 for b in range(16)], where
+        assert np.array_equal(level_record(m), lev[i]), where + ": levels differ"
+
+
+def scene(w, h, n, seed):
+    kw = dict(CONFIGS["cif"])
+    kw.pop("width"), kw.pop("height")
+    g = SceneGenerator(w, h, seed=seed, **kw)
+    return [g.next_frame() for _ in range(n)]
+
+
+@pytest.mark.parametrize("profile", ["main", "high"])
+@pytest.mark.parametrize("qp", [12, 24, 36, 48])
+def test_oracle_cabac_stream_parses_to_its_records(profile, qp):
+    w, h = 176, 144
+    frames = scene(w, h, 4, 700 + qp)
+    enc = OracleEncoder(w, h, qp, profile=profile, keyint=3)
+    stream = b""
+    records = []
+    for f in frames:
+        stream += enc.encode(*f)
+        records.append((enc.decisions(), enc.levels()))
+    pics = cd.parse_stream(stream, strict_end=True)
+    assert [p["type"] for p in pics] == [2, 0, 0, 2]
+    for k, (pic, (dec, lev)) in enumerate(zip(pics, records)):
+        check_picture(pic, dec, lev, "%s qp %d frame %d" % (profile, qp, k))
+    if profile == "high" and qp <= 24:
+        assert any(d["transform_8x8"] and d["cbp_luma"] for dec, _ in records[1:3] for d in dec)
+
+
+# ---------------------------------------------------------------------------------------------
+# The product's writer (rgh_write_p_slice_cabac / rgh_write_idr_slice_cabac) against the oracle
+# ---------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("profile", ["main", "high"])
+@pytest.mark.parametrize("qp", [10, 22, 34, 46])
+def test_writer_matches_the_oracle(profile, qp):
+    from ragdoll_amd import write_headers, write_idr_slice_cabac, write_p_slice_cabac
+    w, h, keyint = 208, 112, 4
+    frames = scene(w, h, 6, 800 + qp)
+    enc = OracleEncoder(w, h, qp, profile=profile, keyint=keyint)
+    for k, f in enumerate(frames):
+        want = enc.encode(*f)
+        dec, lev = enc.decisions(), enc.levels()
+        gop = k % keyint
+        if enc.last_was_idr:
+            got = write_headers(w, h, qp, profile=profile) + write_idr_slice_cabac(
+                dec, lev, enc.mb_w, enc.mb_h, qp, k // keyint, profile=profile)
+        else:
+            got = write_p_slice_cabac(dec, lev, enc.mb_w, enc.mb_h, qp, gop, (2 * gop) & 63, profile=profile)
+        assert got == want, "%s qp %d frame %d: %d vs %d bytes" % (profile, qp, k, len(got), len(want))
+
+
+def test_writer_refuses_what_it_cannot_write():
+    from ragdoll_amd import RghError, write_idr_slice_cabac, write_p_slice_cabac
+    from ragdoll_amd.abi import DECISION_DTYPE, LEVELS_PER_MB
+    dec = np.zeros(4, DECISION_DTYPE)
+    lev = np.zeros((4, LEVELS_PER_MB), np.int16)
+    dec["qp"] = 26
+    assert write_p_slice_cabac(dec, lev, 2, 2, 26, 1, 2)  # all P_Skip
+    with pytest.raises(RghError):
+        write_p_slice_cabac(dec, lev, 2, 2, 26, 1, 2, profile="baseline")
+    bad = dec.copy()
+    bad["mb_type"][1] = 1
+    bad["transform_8x8"][1] = 1
+    bad["cbp_luma"][1] = 1
+    with pytest.raises(RghError):  # the 8x8 transform needs High
+        write_p_slice_cabac(bad, lev, 2, 2, 26, 1, 2, profile="main")
+    bad = dec.copy()
+    bad["mb_type"][2] = 4  # I_PCM
+    with pytest.raises(RghError):
+        write_p_slice_cabac(bad, lev, 2, 2, 26, 1, 2)
+    with pytest.raises(RghError):  # an IDR picture holds intra records only
+        write_idr_slice_cabac(dec, lev, 2, 2, 26, 0)

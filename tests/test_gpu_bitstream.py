@@ -1,4 +1,4 @@
-"""End to end on the GPU: the reference's golden bitstreams from GPU decisions.
+"""End to end on the GPU: the reference's golden bitstreams from GPU decisions, and CABAC GOPs.
 
 Each GOP's IDR is coded by the oracle (the IDR is not on this path); every P picture is
 analysed on the GPU (single-frame and pipelined entry points) and entropy-coded by the
@@ -97,3 +97,35 @@ def test_golden_clip_from_gpu_pipelined_gops(name):
             a.close()
         for b in bufs:
             hip.free(b)
+
+
+# Main / High CABAC: a whole GOP analysed on the GPU (IDR, then P frames) and written by the
+# library's CABAC writer must equal the oracle's stream NAL unit for NAL unit (the oracle's
+# CABAC slices are pinned by the independent parser of tests/cabac_decoder.py, test_cabac.py).
+@pytest.mark.parametrize("cfg,profile,qp,me", [("cif", "high", 26, 16), ("cif", "main", 34, 16),
+                                                ("cif", "high", 18, 24), ("720p", "high", 26, 32)])
+def test_cabac_gop_from_gpu(cfg, profile, qp, me):
+    from ragdoll_amd import write_idr_slice_cabac, write_p_slice_cabac
+    from ragdoll_amd.synth import CONFIGS, SceneGenerator, config_seed
+    kw = dict(CONFIGS[cfg])
+    W, H = kw.pop("width"), kw.pop("height")
+    g = SceneGenerator(W, H, seed=config_seed({"cif": 0, "720p": 1}[cfg], 40 + qp), **kw)
+    frames = [g.next_frame() for _ in range(5)]
+    print("cabac %s %s qp %d me %d" % (cfg, profile, qp, me), flush=True)
+    ora = OracleEncoder(W, H, qp, profile=profile, keyint=1 << 20, me_range=me)
+    want = [split_annexb(ora.encode(*f))[-1] for f in frames]
+    a = StreamAnalyzer(W, H, qp, profile=profile, me_range=me)
+    try:
+        for k, f in enumerate(frames):
+            if k == 0:
+                dec, lv, _ = a.analyse_idr(*f, want_recon=False)
+                nal = write_idr_slice_cabac(dec, lv, a.mb_w, a.mb_h, qp, 0, profile=profile)
+            else:
+                dec, lv, _ = a.analyse(*f, want_recon=False)
+                nal = write_p_slice_cabac(dec, lv, a.mb_w, a.mb_h, qp, k & 15, (2 * k) & 63, profile=profile)
+            got = split_annexb(nal)
+            assert len(got) == 1
+            assert got[0] == want[k], "%s %s qp %d frame %d: slice differs (%d vs %d bytes)" % (
+                cfg, profile, qp, k, len(got[0]), len(want[k]))
+    finally:
+        a.close()
