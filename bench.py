@@ -229,7 +229,7 @@ def main():
     ap.add_argument("--intra-first", type=int, default=153)
     ap.add_argument("--proposal-groups", type=int, default=0,
                     help="workgroup cap of the FP8 inference launch beside the analysis (0: none)")
-    ap.add_argument("--bd-frames", type=int, default=30, help="frames per QP of the BD-rate sample (0: skip)")
+    ap.add_argument("--bd-frames", type=int, default=120, help="frames per QP of the BD-rate sample (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run device-to-device copies of MIB MiB (HBM counter calibration)")
@@ -265,26 +265,83 @@ def learned_config(args):
                 fallback_pct=args.fallback_pct, intra_first=args.intra_first)
 
 
+def learned_ceiling(gops, ans, srcs, decs, lvls, sched, args, total_mbs, hip, analyse_frames):
+    """The learned form's speed with perfect proposals, proposal cost excluded: every P
+    macroblock's proposal is the plain analysis's own result (its vector as all four cells, its
+    mode as a certain mode), and the search takes it with no diamond iteration and no fallback
+    (iters 0, fallback 100000%, skip probe / intra check only for macroblocks proposed as such).
+    That is the most the learned form could gain from any proposals on these analysis kernels."""
+    from ragdoll_amd import DECISION_DTYPE
+    from ragdoll_amd.learned import PROPOSAL_DTYPE
+    analyse_frames(ans, srcs, decs, lvls, lag=sched[0], groups=sched[1], first_idr=True)
+    bufs, P = [], []
+    for i, gp in enumerate(gops):
+        n = ans[i].mb_w * ans[i].mb_h
+        row = [None]
+        for k in range(1, len(gp["frames"])):
+            d = hip.download(decs[i][k], np.zeros(n, DECISION_DTYPE))
+            p = np.zeros(n, PROPOSAL_DTYPE)
+            cell = np.stack([np.round(d["mv"][:, 0] / 16.0), np.round(d["mv"][:, 1] / 16.0)], 1)
+            cell = np.clip(cell, -127, 127).astype(np.int8)
+            intra = (d["mb_type"] == 2) | (d["mb_type"] == 3)
+            cell[intra] = 0
+            p["cell"] = cell[:, None, :]
+            p["mode_prob"][:, 0] = np.where(d["mb_type"] == 0, 255, 0)
+            p["mode_prob"][:, 1] = np.where(d["mb_type"] == 1, 255, 0)
+            p["mode_prob"][:, 2] = np.where(intra, 255, 0)
+            buf = hip.upload(p.view(np.uint8))
+            bufs.append(buf)
+            row.append(buf)
+        P.append(row)
+    cfg = dict(iters=0, skip_gate=128, intra_gate=128, fallback_pct=100000, intra_first=128)
+
+    def step():
+        t = time.perf_counter()
+        analyse_frames(ans, srcs, decs, lvls, lag=sched[0], groups=sched[1], first_idr=True, proposals=P, learned=cfg)
+        return time.perf_counter() - t
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    el = sum(step() for _ in range(args.steps))
+    for b in bufs:
+        hip.free(b)
+    return {"value": round(total_mbs * args.steps / el, 1), "ms_per_step": round(el / args.steps * 1e3, 3),
+            "config": cfg, "note": "proposals = the plain analysis's own vectors and modes (not timed)"}
+
+
 def bd_rate_sample(args):
-    """BD-rate of the learned form against the plain one: the first --bd-frames frames of the
-    headline content coded at QP 22/27/32/37 by both (Constrained Baseline CAVLC bits, luma PSNR)."""
+    """BD-rate of the learned form against the plain one on the headline content: the first
+    --bd-frames frames (GOPs of 30, each an IDR and P frames) coded at QP 22/27/32/37 by both,
+    High profile, +-64 search, CABAC bits (rgh_write_*_slice_cabac) against mean luma PSNR,
+    piecewise cubic (pchip) interpolation of log rate in PSNR, with the least-squares cubic beside
+    it."""
     from ragdoll_amd.learned import Proposer, load_trained
     from ragdoll_amd.rd import bd_rate, code_gop
     frames = render_frames(120)[:args.bd_frames]
+    gops = [frames[k:k + GOP] for k in range(0, len(frames), GOP)]
     net = load_trained()
     proposer = Proposer(net, ((W + 15) // 16) * ((H + 15) // 16))
     t0 = time.time()
     curves = {"plain": [], "learned": []}
     for qp in (22, 27, 32, 37):
-        curves["plain"].append(code_gop(frames, W, H, qp))
-        curves["learned"].append(code_gop(frames, W, H, qp, proposer=proposer, learned=learned_config(args)))
+        for form in curves:
+            parts = [code_gop(g, W, H, qp, profile="high", me_range=ME_RANGE,
+                              proposer=proposer if form == "learned" else None,
+                              learned=learned_config(args) if form == "learned" else None) for g in gops]
+            curves[form].append({"bytes": sum(p["bytes"] for p in parts),
+                                 "psnr_y": float(np.mean([v for p in parts for v in p["frame_psnr_y"]])),
+                                 "plain_search": sum(p["plain_search"] for p in parts),
+                                 "p_macroblocks": sum(p["p_macroblocks"] for p in parts)})
+        print("bd-rate sample: QP %d done at %.1f s" % (qp, time.time() - t0), file=sys.stderr, flush=True)
     proposer.close()
     net.close()
     rate = {k: [c["bytes"] for c in v] for k, v in curves.items()}
     q = {k: [c["psnr_y"] for c in v] for k, v in curves.items()}
     out = {"bd_rate_pct": round(bd_rate(rate["plain"], q["plain"], rate["learned"], q["learned"]), 3),
-           "sample": "first %d frames of the 1080p content (1 IDR + %d P), QP 22/27/32/37, Constrained Baseline "
-                     "CAVLC bits vs luma PSNR" % (len(frames), len(frames) - 1),
+           "bd_rate_pct_cubic": round(bd_rate(rate["plain"], q["plain"], rate["learned"], q["learned"], "cubic"), 3),
+           "method": "piecewise cubic (pchip) interpolation of log rate in luma PSNR",
+           "sample": "first %d frames of the 1080p content (%d GOPs of an IDR and P frames), QP 22/27/32/37, "
+                     "High profile, +-64 search, CABAC bits vs mean luma PSNR" % (len(frames), len(gops)),
            "qps": [22, 27, 32, 37], "plain_bytes": rate["plain"], "learned_bytes": rate["learned"],
            "plain_psnr_y": [round(v, 4) for v in q["plain"]], "learned_psnr_y": [round(v, 4) for v in q["learned"]],
            "learned_plain_search": [c["plain_search"] for c in curves["learned"]],
@@ -568,6 +625,7 @@ def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value
         e, props = pipelined_step(b, props)
         el += e
         b ^= 1
+    ceiling = learned_ceiling(gops, ans, srcs, decs, lvls, sched, args, total_mbs, hip, analyse_frames)
     # FP8 inference alone (one launch over every P macroblock of the step, the whole device), for
     # its roofline.
     net.set_launch_limit(0)
@@ -592,7 +650,9 @@ def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value
         q.close()
     net.close()
     value = total_mbs * args.steps / el_serial
+    ceiling["vs_plain"] = round(ceiling["value"] / plain_value, 4)
     return {"value": round(value, 1), "unit": "macroblocks/s", "vs_plain": round(value / plain_value, 4),
+            "ceiling": ceiling,
             "ms_per_step": round(el_serial / args.steps * 1e3, 3),
             "pipelined_value": round(total_mbs * args.steps / el, 1),
             "pipelined_ms_per_step": round(el / args.steps * 1e3, 3),

@@ -152,15 +152,34 @@ def test_quantize_reference_reconstructs_the_weights():
         assert np.all(np.abs(back - w) <= np.abs(w) / 16 + np.abs(w).max(0) / 448 * 2.0 ** -9 + 1e-12)
 
 
-def test_bd_rate_of_a_scaled_curve_is_the_scale():
+@pytest.mark.parametrize("method", ["pchip", "cubic"])
+def test_bd_rate_of_a_scaled_curve_is_the_scale(method):
     from ragdoll_amd.rd import bd_rate, psnr
     rate = np.array([1000.0, 1800.0, 3500.0, 7000.0])
     q = np.array([30.0, 33.0, 36.5, 40.0])
-    assert abs(bd_rate(rate, q, rate, q)) < 1e-9
-    assert abs(bd_rate(rate, q, rate * 0.9, q) + 10.0) < 1e-6
-    assert abs(bd_rate(rate, q, rate * 1.01, q) - 1.0) < 1e-6
+    assert abs(bd_rate(rate, q, rate, q, method)) < 1e-9
+    assert abs(bd_rate(rate, q, rate * 0.9, q, method) + 10.0) < 1e-6
+    assert abs(bd_rate(rate, q, rate * 1.01, q, method) - 1.0) < 1e-6
     # A curve shifted up in PSNR reaches the same quality with fewer bits.
-    assert bd_rate(rate, q, rate, q + 0.5) < 0
+    assert bd_rate(rate, q, rate, q + 0.5, method) < 0
     a = np.full((4, 4), 10, np.uint8)
     assert psnr(a, a) == 100.0
     assert abs(psnr(a, a + 1) - 10 * np.log10(255.0 ** 2)) < 1e-9
+
+
+def test_pchip_integral_matches_scipy():
+    # The piecewise cubic BD-rate's interpolant and its integral against scipy's
+    # PchipInterpolator (the same Fritsch-Carlson construction) on random monotone and
+    # non-monotone data, over partial intervals.
+    from scipy.interpolate import PchipInterpolator
+    from ragdoll_amd.rd import pchip_integral
+    rng = np.random.default_rng(5)
+    for trial in range(200):
+        n = int(rng.integers(3, 8))
+        x = np.sort(rng.uniform(25, 45, n))
+        if np.any(np.diff(x) < 1e-3):
+            continue
+        y = np.cumsum(rng.uniform(-0.5 if trial % 3 == 0 else 0.05, 1.0, n))
+        lo, hi = np.sort(rng.uniform(x[0], x[-1], 2))
+        want = PchipInterpolator(x, y).integrate(lo, hi)
+        assert abs(pchip_integral(x, y, lo, hi) - want) < 1e-9 * max(1.0, abs(want)), trial
