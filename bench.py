@@ -214,9 +214,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--lag", type=int, default=0, help="frame lag in wavefront steps (0: autotune)")
     ap.add_argument("--groups", type=int, default=0, help="concurrent schedules (0: autotune)")
-    ap.add_argument("--workload", choices=["headline", "batched", "compact"], default="headline",
+    ap.add_argument("--workload", choices=["headline", "batched", "compact", "fast"], default="headline",
                     help="headline: the 1080p metric (default); batched: the 24-stream uneven-size set; "
-                         "compact: the headline streams with the Compact preset (Baseline)")
+                         "compact: the headline streams with the Compact preset (Baseline); "
+                         "fast: the headline streams with the Fast preset (High)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--end-to-end", type=int, default=1, help="also time the step with its host transfers (1: yes)")
     ap.add_argument("--compare-persistent", type=int, default=1,
@@ -250,6 +251,8 @@ def main():
         gops, label = batched_gops(), "batched"
     elif args.workload == "compact":
         gops, label = [dict(g, profile="baseline", speed="compact") for g in headline_gops()], "compact"
+    elif args.workload == "fast":
+        gops, label = [dict(g, speed="fast") for g in headline_gops()], "fast"
     else:
         gops, label = headline_gops(), "headline"
     out = run_workload(gops, args)
@@ -452,7 +455,8 @@ def run_workload(gops, args):
     traffic = measured_traffic() if headline else None
     metric = {"headline": "1080p ME macroblocks/sec, 1 GPU: plain vs learned-proposal at <=1% BD-rate",
               "batched": "batched 24-stream ME macroblocks/sec, 1 GPU, plain form",
-              "compact": "1080p ME macroblocks/sec, 1 GPU, Compact preset"}[args.workload]
+              "compact": "1080p ME macroblocks/sec, 1 GPU, Compact preset",
+              "fast": "1080p ME macroblocks/sec, 1 GPU, Fast preset"}[args.workload]
     workload = {
         "headline": "plain GPU form: 1920x1080 analysis, High decisions, QP 24/30/36, search range +-64 (64 "
                     "diamond iterations) with quarter-pel refinement, keyint 30, 12 GOP streams x (1 IDR + 29 P "
@@ -463,7 +467,10 @@ def run_workload(gops, args):
                    "frames) in one pipelined schedule; value counts P macroblocks",
         "compact": "Compact preset (hexagon search, range 16, 8x8 partitions, I_4x4 in P; Constrained Baseline "
                    "decisions): 1920x1080, QP 24/30/36, keyint 30, 12 GOP streams x (1 IDR + 29 P frames) per step in one "
-                   "pipelined schedule, bit-exact with the CPU oracle; value counts P macroblocks"}[args.workload]
+                   "pipelined schedule, bit-exact with the CPU oracle; value counts P macroblocks",
+        "fast": "Fast preset (diamond search +-64 with a half-pel step, Intra 16x16 only, no loop filter; High "
+                "decisions): 1920x1080, QP 24/30/36, keyint 30, 12 GOP streams x (1 IDR + 29 P frames) per step in "
+                "one pipelined schedule, bit-exact with the CPU oracle; value counts P macroblocks"}[args.workload]
     kernel = "rgh_k_compact_step" if args.workload == "compact" else "rgh_k_frame_step"
     out = {
         "metric": metric,

@@ -73,6 +73,12 @@ typedef struct rgh_config {
  * an IDR takes zero 16x16 search vectors of the previous picture as candidates, where the
  * reference, continuing across a mid-stream IDR, takes those of the picture two before it. */
 #define RGH_FLAG_COMPACT 2
+/* rgh_config.flags: the Fast preset (rgh_speed RGH_SPEED_FAST): half-pel motion toward the
+ * cheaper full-pel neighbour, the skip probe only next to a skipped macroblock, intra 16x16 only
+ * when its SAD comes within 1.75x of the inter SAD (and must win by 16 lambda), no plane modes,
+ * no intra 4x4, chroma from the luma mode in P pictures, and no loop filter (enc.rs:503, :2270,
+ * :3050-3152, :3322, :3605). Not combinable with RGH_FLAG_COMPACT or learned proposals. */
+#define RGH_FLAG_FAST 4
 
 /* An I420 picture: three planes with their strides; width/height are display sizes.
  * Chroma planes are (width+1)/2 x (height+1)/2. */
@@ -110,9 +116,11 @@ enum {
 /* Encoder presets (H264Speed, h264.rs). */
 typedef enum rgh_speed {
     RGH_SPEED_BALANCED = 0, /* diamond search of 16x16 vectors (enc.rs:1182-1339) */
-    RGH_SPEED_COMPACT = 1   /* hexagon search over half-sample planes, 8x8 partitions, I_4x4 in P,
+    RGH_SPEED_COMPACT = 1,  /* hexagon search over half-sample planes, 8x8 partitions, I_4x4 in P,
                                fine inter rounding, chroma QP offset 1 (enc/inter.rs, enc/compact.rs);
                                Constrained Baseline only (with CABAC the preset codes B pictures) */
+    RGH_SPEED_FAST = 2      /* Balanced records with the Fast decisions; slices without the loop
+                               filter (disable_deblocking_filter_idc 1, enc.rs:2334) */
 } rgh_speed;
 
 /* Fixed 32-byte decision record per macroblock, raster order. */
@@ -189,12 +197,14 @@ rgh_status rgh_analyse_idr(void *ctx, const rgh_i420 *src, rgh_mb_decision *deci
 
 /* Batched form: one P frame of each of n contexts, whose sources are already in device
  * memory, analysed by one set of kernel launches on one HIP stream (all contexts must have
- * been created on the same device). Results stay on the device; read them with
+ * been created on the same device and use one preset: Balanced, Compact or Fast,
+ * RGH_ERR_INVALID_ARGUMENT otherwise). Results stay on the device; read them with
  * rgh_get_outputs. No counterpart in the reference, whose GOP workers run one encoder per
  * thread (crates/ragdoll-pipeline/src/lib.rs:543-603); this is that wave of GOPs on one GPU. */
 rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src_device);
 
-/* Pipelined form: n_frames consecutive P frames of each of n contexts (same device), sources
+/* Pipelined form: n_frames consecutive P frames of each of n contexts (same device and one
+ * preset, as rgh_analyse_batch), sources
  * and outputs in device memory, analysed by one schedule of kernel launches. Frame k + 1 of a
  * stream starts `lag` wavefront steps after frame k (0 selects 20) and each of its macroblocks
  * proceeds as soon as the reference samples it can read are final, so up to ring - 1 frames
@@ -218,7 +228,7 @@ rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src_d
  * is homed on one XCD whose waves take its items (macroblock analyses, deblock/pad items) in
  * planned order and wait for the items they read; if an item cannot wait (it needs one planned
  * after it) or the launch gives up, the per-step schedule finishes the call. Results are
- * identical. It serves Balanced and IDR pictures: calls with Compact contexts, and
+ * identical. It serves Balanced, Fast and IDR pictures: calls with Compact contexts, and
  * rgh_analyse_frames_noalloc, use the per-step schedule. On MI355X it is slower than the per-step
  * schedule on the measured workloads (the waits and their cache invalidations cost more than the
  * launches they save), so it is not the default. */

@@ -839,6 +839,7 @@ struct ora_enc {
      * half-sample planes of the reference, the 16x16 search vectors of this and the previous
      * picture (mvr), the Compact motion vector costs. */
     int compact, cqo;
+    int fast; /* H264Speed::Fast (enc.rs:503): half-pel motion, intra 16x16 only, no loop filter */
     plane_t hh, hv, hc;
     int16_t (*mvr)[2], (*prev_mvr)[2];
     uint16_t *mv_costs_c;
@@ -912,7 +913,7 @@ ora_enc *ora_create(int width, int height, int fps_num, int fps_den, int qp, int
 /* H264Encoder::with_speed(Compact) (h264.rs:3066-3098): chroma QP offset 1; with CAVLC no B
  * pictures. Returns -1 for the CABAC profiles, whose Compact configuration codes B pictures. */
 int ora_set_compact(ora_enc *e, int on) {
-    if (on && e->cabac) return -1;
+    if (on && (e->cabac || e->fast)) return -1;
     e->compact = on != 0;
     e->cqo = on ? 1 : 0;
     if (on && !e->hh.buf) {
@@ -921,6 +922,16 @@ int ora_set_compact(ora_enc *e, int on) {
         plane_alloc(&e->hv, W, H, PAD_Y);
         plane_alloc(&e->hc, W, H, PAD_Y);
     }
+    return 0;
+}
+
+/* H264Encoder::with_speed(Fast) (h264.rs:3032-3065 with enc.rs:2270): the Fast gates of
+ * code_p_mb (enc.rs:3050-3152), half_pel_toward (enc.rs:1341), no intra 4x4 and no plane modes
+ * (enc.rs:3322, :3457), chroma from the luma mode in P (enc.rs:3605) and no loop filter
+ * (enc.rs:2334). Not combinable with Compact. */
+int ora_set_fast(ora_enc *e, int on) {
+    if (on && e->compact) return -1;
+    e->fast = on != 0;
     return 0;
 }
 
@@ -1619,8 +1630,9 @@ static int i16_dc(const edges_t *ed) { /* enc.rs:1624 */
     if (ed->has_left) return (int)((sl + 8) >> 4);
     return 128;
 }
-/* enc.rs:1637 analyse_i16_edges (plane mode always tried by Balanced) */
-static void analyse_i16(const edges_t *ed, const uint8_t *src, uint32_t *satd_out, int *mode_out, uint16_t *n_satd) {
+/* enc.rs:1637 analyse_i16_edges (the plane mode is tried unless `plane` is 0: Fast) */
+static void analyse_i16(const edges_t *ed, const uint8_t *src, uint32_t *satd_out, int *mode_out, uint16_t *n_satd,
+                        int plane) {
     static const uint8_t zero[16] = {0};
     int dc = i16_dc(ed);
     uint32_t vhd[3];
@@ -1635,7 +1647,7 @@ static void analyse_i16(const edges_t *ed, const uint8_t *src, uint32_t *satd_ou
         best = vhd[1];
         mode = 1;
     }
-    if (ed->has_top && ed->has_left) {
+    if (plane && ed->has_top && ed->has_left) {
         uint8_t pred[256];
         pred16(3, ed->top, ed->left, ed->tl, pred);
         uint32_t c = satd16(src, 16, pred, 16);
@@ -1648,15 +1660,16 @@ static void analyse_i16(const edges_t *ed, const uint8_t *src, uint32_t *satd_ou
     *satd_out = best;
     *mode_out = mode;
 }
-/* enc.rs:1523 analyse_chroma_intra (plane allowed) */
-static int analyse_chroma_intra(const ora_enc *e, int mb_x, int mb_y, const mbsrc_t *s, uint8_t *pu, uint8_t *pv) {
+/* enc.rs:1523 analyse_chroma_intra (the plane mode only when `plane`) */
+static int analyse_chroma_intra(const ora_enc *e, int mb_x, int mb_y, const mbsrc_t *s, uint8_t *pu, uint8_t *pv,
+                                int plane) {
     edges_t eu, ev;
     get_edges(&e->cur.u, mb_x, mb_y, 8, &eu);
     get_edges(&e->cur.v, mb_x, mb_y, 8, &ev);
     uint32_t best = UINT32_MAX;
     int bm = 0;
     for (int mode = 0; mode < 4; mode++) {
-        int ok = mode == 1 ? eu.has_left : mode == 2 ? eu.has_top : mode == 3 ? (eu.has_top && eu.has_left) : 1;
+        int ok = mode == 1 ? eu.has_left : mode == 2 ? eu.has_top : mode == 3 ? (plane && eu.has_top && eu.has_left) : 1;
         if (!ok) continue;
         uint8_t a[64], b[64];
         pred_chroma(mode, eu.has_top ? eu.top : NULL, eu.has_left ? eu.left : NULL, eu.tl, a);
@@ -1681,7 +1694,18 @@ static void code_i16(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const mb
     get_edges(&e->cur.y, mb_x, mb_y, 16, &ed);
     uint8_t pred[256], pu[64], pv[64];
     pred16(mode, ed.has_top ? ed.top : NULL, ed.has_left ? ed.left : NULL, ed.tl, pred);
-    int cmode = analyse_chroma_intra(e, mb_x, mb_y, s, pu, pv);
+    int cmode;
+    if (e->fast && !e->is_idr) { /* enc.rs:3605: chroma predicted in the direction of the luma mode */
+        static const int from_luma[4] = {2, 1, 0, 3};
+        cmode = from_luma[mode & 3];
+        edges_t eu, ev;
+        get_edges(&e->cur.u, mb_x, mb_y, 8, &eu);
+        get_edges(&e->cur.v, mb_x, mb_y, 8, &ev);
+        pred_chroma(cmode, eu.has_top ? eu.top : NULL, eu.has_left ? eu.left : NULL, eu.tl, pu);
+        pred_chroma(cmode, ev.has_top ? ev.top : NULL, ev.has_left ? ev.left : NULL, ev.tl, pv);
+    } else {
+        cmode = analyse_chroma_intra(e, mb_x, mb_y, s, pu, pv, !e->fast);
+    }
     e->mb_is_intra[idx] = 1;
     e->mb_is_i16[idx] = 1;
     set_intra_motion(e, mb_x, mb_y);
@@ -1957,7 +1981,7 @@ static void code_i4(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const mbs
     ora_mb_decision *d = &e->dec[idx];
     int16_t *lv = e->levels + (long)idx * ORA_LEVELS_PER_MB;
     uint8_t pu[64], pv[64];
-    int cmode = analyse_chroma_intra(e, mb_x, mb_y, s, pu, pv);
+    int cmode = analyse_chroma_intra(e, mb_x, mb_y, s, pu, pv, !e->fast);
     int qpc = chroma_qp(qp);
     chroma_levels_t c;
     quant_chroma(s, pu, pv, qpc, 1, &c);
@@ -2034,8 +2058,8 @@ static void code_intra(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const 
     get_edges(&e->cur.y, mb_x, mb_y, 16, &ed);
     uint32_t i16_satd;
     int i16_mode;
-    analyse_i16(&ed, s->y, &i16_satd, &i16_mode, NULL);
-    if (i16_satd > 40 * lambda) {
+    analyse_i16(&ed, s->y, &i16_satd, &i16_mode, NULL, !e->fast);
+    if (!e->fast && i16_satd > 40 * lambda) {
         i4_choice_t ch;
         if (analyse_i4(e, mb_x, mb_y, qp, s, i16_satd + lambda * 4, &ch)) {
             code_i4(e, w, mb_x, mb_y, qp, s, &ch);
@@ -2079,8 +2103,12 @@ static uint32_t ms_cost_at(const ms_t *m, int fx, int fy) { /* enc.rs:1176 */
     m->d->n_sad++;
     return sad_wh(m->src, 16, m->r->buf + off, m->r->stride, 16, 16) + ms_mv_cost(m, fx * 4, fy * 4);
 }
-/* enc.rs:1185 full_pel */
-static void ms_full_pel(const ms_t *m, const int16_t (*cands)[2], int nc, int iters, int best[2], uint32_t *best_cost) {
+/* enc.rs:1185 full_pel; nb (may be NULL): the costs of the last iteration's four points (up,
+ * left, right, down; UINT32_MAX outside the range or when no iteration ran), which the Fast
+ * preset's half-pel step reads. */
+static void ms_full_pel(const ms_t *m, const int16_t (*cands)[2], int nc, int iters, int best[2], uint32_t *best_cost,
+                        uint32_t nb[4]) {
+    if (nb) nb[0] = nb[1] = nb[2] = nb[3] = UINT32_MAX;
     int imn[2] = {(m->b.min[0] + 3) >> 2, (m->b.min[1] + 3) >> 2};
     int imx[2] = {m->b.max[0] >> 2, m->b.max[1] >> 2};
     best[0] = best[1] = 0;
@@ -2109,8 +2137,10 @@ static void ms_full_pel(const ms_t *m, const int16_t (*cands)[2], int nc, int it
         /* Both branches of the reference pick the first of the cheapest points below the
          * current best (sad_x4 with index tie-break, or sequential strict compares). */
         for (int k = 0; k < 4; k++) {
+            if (nb) nb[k] = UINT32_MAX;
             if (pts[k][0] < imn[0] || pts[k][0] > imx[0] || pts[k][1] < imn[1] || pts[k][1] > imx[1]) continue;
             uint32_t c = ms_cost_at(m, pts[k][0], pts[k][1]);
+            if (nb) nb[k] = c;
             if (c < *best_cost) {
                 *best_cost = c;
                 moved = k;
@@ -2199,6 +2229,65 @@ static void ms_sub_pel(const ms_t *m, hpel_win_t *w, int full[2], uint32_t full_
     mv_out[0] = (int16_t)best[0];
     mv_out[1] = (int16_t)best[1];
     *cost_out = best_cost;
+}
+
+/* enc.rs:1341 half_pel_toward (Fast): on each axis only the half sample toward the cheaper
+ * full-pel neighbour is a candidate, estimated by the SAD against the average of its two integer
+ * neighbours (enc.rs:1380 half_estimate; UINT32_MAX outside the bounds); the better estimate (the
+ * vertical one on a tie), if below the full-pel cost, is interpolated with the 6-tap filter
+ * (simd/h264_enc.rs:1345 hpel_16x16) and taken if its SAD cost is below the full-pel cost.
+ * Counts each estimate and the interpolated candidate as sub-pel evaluations, the
+ * interpolation as a half-pel window. */
+static void ms_half_pel_toward(const ms_t *m, const int full[2], uint32_t full_cost, const uint32_t nb[4],
+                               int16_t mv_out[2], uint32_t *cost_out, uint8_t *pred) {
+    const plane_t *r = m->r;
+    int cx = full[0] * 4, cy = full[1] * 4;
+    int up = nb[0] < nb[3], left = nb[1] < nb[2];
+    long off = pat(r, m->x0 + full[0], m->y0 + full[1]), st = r->stride;
+    int mvv[2] = {cx, cy + (up ? -2 : 2)}, mvh[2] = {cx + (left ? -2 : 2), cy};
+    uint32_t est[2];
+    for (int k = 0; k < 2; k++) {
+        const int *mv = k == 0 ? mvv : mvh;
+        est[k] = UINT32_MAX;
+        if (!mvb_contains(&m->b, mv[0], mv[1])) continue;
+        long o2 = k == 0 ? (up ? off - st : off + st) : (left ? off - 1 : off + 1);
+        uint32_t sad = 0;
+        for (int y = 0; y < 16; y++)
+            for (int x = 0; x < 16; x++) {
+                int a = r->buf[off + y * st + x], b = r->buf[o2 + y * st + x];
+                sad += (uint32_t)iabs((int)m->src[y * 16 + x] - ((a + b + 1) >> 1));
+            }
+        m->d->n_subpel++;
+        est[k] = sad + ms_mv_cost(m, mv[0], mv[1]);
+    }
+    int vertical = est[0] <= est[1];
+    const int *mv = vertical ? mvv : mvh;
+    int back = vertical ? up : left;
+    if ((vertical ? est[0] : est[1]) < full_cost) {
+        uint8_t block[256];
+        long base = off - (back ? (vertical ? st : 1) : 0);
+        for (int y = 0; y < 16; y++)
+            for (int x = 0; x < 16; x++) {
+                const uint8_t *q = r->buf + base + y * st + x;
+                long step = vertical ? st : 1;
+                int v = tap6(q[-2 * step], q[-step], q[0], q[step], q[2 * step], q[3 * step]);
+                block[y * 16 + x] = clip_u8((v + 16) >> 5);
+            }
+        m->d->n_subpel++;
+        m->d->n_hpel++;
+        uint32_t cost = sad_wh(m->src, 16, block, 16, 16, 16) + ms_mv_cost(m, mv[0], mv[1]);
+        if (cost < full_cost) {
+            memcpy(pred, block, 256);
+            mv_out[0] = (int16_t)mv[0];
+            mv_out[1] = (int16_t)mv[1];
+            *cost_out = cost;
+            return;
+        }
+    }
+    load_block(r, off, pred, 16);
+    mv_out[0] = (int16_t)cx;
+    mv_out[1] = (int16_t)cy;
+    *cost_out = full_cost;
 }
 
 /* enc.rs:2906 try_p_skip (probe enabled for Balanced) */
@@ -2330,7 +2419,11 @@ static void code_p_mb(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const m
     d->qp = (uint8_t)qp;
     uint8_t py[256], pu[64], pv[64];
     if (e->sb.qp != qp) skip_bounds_init(&e->sb, qp, qpc, e->compact);
-    if (try_p_skip(e, mb_x, mb_y, qp, qpc, s, pskip, mvb_contains(&b, pskip[0], pskip[1]), py, pu, pv)) {
+    /* Fast probes only next to a skipped macroblock (enc.rs:3050-3063). */
+    int probe = !e->fast || (mb_x > 0 && e->mb_is_skip[idx - 1]) ||
+                (mb_y > 0 && (e->mb_is_skip[idx - mb_w] || (mb_x > 0 && e->mb_is_skip[idx - mb_w - 1]) ||
+                              (mb_x + 1 < mb_w && e->mb_is_skip[idx - mb_w + 1])));
+    if (try_p_skip(e, mb_x, mb_y, qp, qpc, s, pskip, probe && mvb_contains(&b, pskip[0], pskip[1]), py, pu, pv)) {
         d->mb_type = 0;
         d->flags = 1;
         d->mv[0] = pskip[0];
@@ -2356,8 +2449,8 @@ static void code_p_mb(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const m
     ensure_mv_costs(e, qp);
     ms_t ms = {&e->refp.y, s->y, x0, y0, {mvp[0], mvp[1]}, lambda, e, b, d};
     int full[2];
-    uint32_t full_cost;
-    ms_full_pel(&ms, cands, n, e->me_iters, full, &full_cost);
+    uint32_t full_cost, nb[4];
+    ms_full_pel(&ms, cands, n, e->me_iters, full, &full_cost, nb);
     int16_t mv[2];
     uint32_t me_cost;
     if (full_cost - ms_mv_cost(&ms, full[0] * 4, full[1] * 4) <= e->sb.luma) {
@@ -2365,20 +2458,43 @@ static void code_p_mb(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const m
         mv[0] = (int16_t)(full[0] * 4);
         mv[1] = (int16_t)(full[1] * 4);
         me_cost = full_cost;
+    } else if (e->fast) {
+        ms_half_pel_toward(&ms, full, full_cost, nb, mv, &me_cost, py);
     } else {
         ms_sub_pel(&ms, &e->win, full, full_cost, 2, mv, &me_cost, py);
     }
     d->cost = me_cost;
     edges_t ed;
     get_edges(&e->cur.y, mb_x, mb_y, 16, &ed);
-    uint32_t inter_satd = satd16(s->y, 16, py, 16);
-    d->n_satd++;
+    /* Fast analyses intra only when the best of the V / H / DC SADs is within 1.75 times the inter
+     * SAD (enc.rs:3119-3140, simd/h264_enc.rs:612 intra_sad_x3_16x16). */
+    int try_intra = 1;
+    if (e->fast) {
+        uint32_t inter_sad = me_cost - ms_mv_cost(&ms, mv[0], mv[1]);
+        int dc = i16_dc(&ed);
+        uint32_t sv = 0, sh = 0, sd = 0;
+        for (int i = 0; i < 256; i++) {
+            int v = s->y[i];
+            sv += (uint32_t)iabs(v - (ed.has_top ? ed.top[i % 16] : 0));
+            sh += (uint32_t)iabs(v - (ed.has_left ? ed.left[i / 16] : 0));
+            sd += (uint32_t)iabs(v - dc);
+        }
+        uint32_t best = sd;
+        if (ed.has_top && sv < best) best = sv;
+        if (ed.has_left && sh < best) best = sh;
+        try_intra = 4 * (uint64_t)best < 7 * (uint64_t)inter_sad;
+    }
+    uint32_t inter_satd = 0;
+    if (try_intra) {
+        inter_satd = satd16(s->y, 16, py, 16);
+        d->n_satd++;
+    }
     uint32_t inter_cost = inter_satd + lambda * (se_bits(mv[0] - mvp[0]) + se_bits(mv[1] - mvp[1]) + 1);
     if (inter_satd > 24 * lambda) {
         uint32_t i16_satd;
         int i16_mode;
-        analyse_i16(&ed, s->y, &i16_satd, &i16_mode, &d->n_satd);
-        if (i16_satd + lambda * 8 < inter_cost) {
+        analyse_i16(&ed, s->y, &i16_satd, &i16_mode, &d->n_satd, !e->fast);
+        if (i16_satd + lambda * (e->fast ? 16 : 8) < inter_cost) {
             begin_coded_mb(e, w);
             code_i16(e, w, mb_x, mb_y, qp, s, i16_mode);
             d->mv[0] = d->mv[1] = 0;
@@ -3099,7 +3215,7 @@ static void code_p_mb_compact(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp,
     get_edges(&e->cur.y, mb_x, mb_y, 16, &ed);
     uint32_t i16_satd;
     int i16_mode;
-    analyse_i16(&ed, s->y, &i16_satd, &i16_mode, NULL);
+    analyse_i16(&ed, s->y, &i16_satd, &i16_mode, NULL, 1);
     uint32_t i16_cost = i16_satd + lambda * 4;
     static const int zq[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
     if (i16_cost < best_cost) {
@@ -3800,7 +3916,7 @@ long ora_encode_frame(ora_enc *e, const uint8_t *y, const uint8_t *u, const uint
             put32(wp, 0, 1);
         }
         put_se(wp, slice_qp - e->qp);
-        int dis = slice_qp == 0 ? 1 : 0;
+        int dis = slice_qp == 0 || e->fast ? 1 : 0; /* enc.rs:2334 */
         put_ue(wp, (uint64_t)dis);
         if (dis != 1) {
             put_se(wp, 0);
@@ -3852,7 +3968,7 @@ long ora_encode_frame(ora_enc *e, const uint8_t *y, const uint8_t *u, const uint
             put_ue(wp, 0); /* cabac_init_idc */
         }
         put_se(wp, slice_qp - e->qp);
-        int dis = slice_qp == 0 ? 1 : 0;
+        int dis = slice_qp == 0 || e->fast ? 1 : 0;
         put_ue(wp, (uint64_t)dis);
         if (dis != 1) {
             put_se(wp, 0);
@@ -3865,7 +3981,7 @@ long ora_encode_frame(ora_enc *e, const uint8_t *y, const uint8_t *u, const uint
     copy_plane(&e->cur.y, e->pre_y);
     copy_plane(&e->cur.u, e->pre_u);
     copy_plane(&e->cur.v, e->pre_v);
-    if (slice_qp != 0) {
+    if (slice_qp != 0 && !e->fast) {
         if (e->compact)
             deblock_parts(e);
         else

@@ -72,6 +72,10 @@ void ora_destroy(ora_enc *e);
 void ora_set_me_range(ora_enc *e, int iters);
 /* H264Encoder::with_speed(H264Speed::Compact) for Baseline (CAVLC) streams; -1 for CABAC. */
 int ora_set_compact(ora_enc *e, int on);
+/* The Fast preset (H264Speed::Fast): half-pel motion toward the cheaper neighbour, intra 16x16
+ * without the plane mode, the Fast skip-probe and intra gates, no loop filter. Returns -1 with
+ * Compact. */
+int ora_set_fast(ora_enc *e, int on);
 /* Encodes one I420 frame. Writes the Annex-B access unit (SPS+PPS before an IDR slice) to out
  * and returns its length; returns -1 on a too small buffer. For CABAC profiles no bitstream is
  * written (returns 0) but every decision, level and reconstruction is produced. */

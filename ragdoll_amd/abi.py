@@ -8,9 +8,10 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libragdoll_hip.so")
 LEVELS_PER_MB = 408
 PROFILES = {"baseline": 0, "main": 1, "high": 2}
-SPEEDS = {"balanced": 0, "compact": 1}
+SPEEDS = {"balanced": 0, "compact": 1, "fast": 2}
 FLAG_NO_UNDEBLOCKED_COPY = 1
 FLAG_COMPACT = 2
+FLAG_FAST = 4
 
 DECISION_DTYPE = np.dtype([
     ("mb_type", "u1"), ("i16_mode", "u1"), ("chroma_mode", "u1"), ("cbp_luma", "u1"),
@@ -150,7 +151,8 @@ class StreamAnalyzer:
     def __init__(self, width, height, qp, profile="baseline", me_range=0, qpel_iters=0, device=0,
                  keep_undeblocked=True, ring=0, speed="balanced"):
         self.L = load_library()
-        flags = (0 if keep_undeblocked else FLAG_NO_UNDEBLOCKED_COPY) | (FLAG_COMPACT if speed == "compact" else 0)
+        flags = ((0 if keep_undeblocked else FLAG_NO_UNDEBLOCKED_COPY) | (FLAG_COMPACT if speed == "compact" else 0) |
+                 (FLAG_FAST if speed == "fast" else 0))
         cfg = RgConfig(width, height, PROFILES[profile], qp, me_range, qpel_iters, device, flags, ring)
         h = ctypes.c_void_p()
         _check(self.L.rgh_create(ctypes.byref(cfg), ctypes.byref(h)), "rgh_create")
@@ -303,14 +305,14 @@ def write_p_slice_cavlc(decisions, levels, mb_w, mb_h, qp, frame_num, poc_lsb, s
     return bytes(out[:n.value])
 
 
-def write_idr_slice_cavlc(decisions, levels, mb_w, mb_h, qp, idr_pic_id):
+def write_idr_slice_cavlc(decisions, levels, mb_w, mb_h, qp, idr_pic_id, speed="balanced"):
     """The Annex B NAL unit (bytes) of one analysed IDR picture, Constrained Baseline CAVLC."""
     L = load_library()
     dec = np.ascontiguousarray(decisions, DECISION_DTYPE)
     lv = np.ascontiguousarray(levels, np.int16)
     if dec.shape != (mb_w * mb_h,) or lv.shape != (mb_w * mb_h, LEVELS_PER_MB):
         raise ValueError("decisions/levels do not match the macroblock grid")
-    cfg = RgSliceConfig(mb_w, mb_h, qp, 0, 0, 0)
+    cfg = RgSliceConfig(mb_w, mb_h, qp, 0, 0, SPEEDS[speed])
     cap = mb_w * mb_h * 1024 + 64
     out = np.zeros(cap, np.uint8)
     n = ctypes.c_int64()
@@ -328,21 +330,21 @@ def _slice_args(decisions, levels, mb_w, mb_h):
     return dec, lv, cap, np.zeros(cap, np.uint8), ctypes.c_int64()
 
 
-def write_p_slice_cabac(decisions, levels, mb_w, mb_h, qp, frame_num, poc_lsb, profile="high"):
+def write_p_slice_cabac(decisions, levels, mb_w, mb_h, qp, frame_num, poc_lsb, profile="high", speed="balanced"):
     """The Annex B NAL unit (bytes) of one analysed P picture, Main or High CABAC."""
     L = load_library()
     dec, lv, cap, out, n = _slice_args(decisions, levels, mb_w, mb_h)
-    cfg = RgSliceConfig(mb_w, mb_h, qp, frame_num, poc_lsb, SPEEDS["balanced"])
+    cfg = RgSliceConfig(mb_w, mb_h, qp, frame_num, poc_lsb, SPEEDS[speed])
     _check(L.rgh_write_p_slice_cabac(ctypes.byref(cfg), PROFILES[profile], _ptr(dec), _ptr(lv), _ptr(out), cap,
                                      ctypes.byref(n)), "rgh_write_p_slice_cabac")
     return bytes(out[:n.value])
 
 
-def write_idr_slice_cabac(decisions, levels, mb_w, mb_h, qp, idr_pic_id, profile="high"):
+def write_idr_slice_cabac(decisions, levels, mb_w, mb_h, qp, idr_pic_id, profile="high", speed="balanced"):
     """The Annex B NAL unit (bytes) of one analysed IDR picture, Main or High CABAC."""
     L = load_library()
     dec, lv, cap, out, n = _slice_args(decisions, levels, mb_w, mb_h)
-    cfg = RgSliceConfig(mb_w, mb_h, qp, 0, 0, SPEEDS["balanced"])
+    cfg = RgSliceConfig(mb_w, mb_h, qp, 0, 0, SPEEDS[speed])
     _check(L.rgh_write_idr_slice_cabac(ctypes.byref(cfg), PROFILES[profile], idr_pic_id, _ptr(dec), _ptr(lv),
                                        _ptr(out), cap, ctypes.byref(n)), "rgh_write_idr_slice_cabac")
     return bytes(out[:n.value])

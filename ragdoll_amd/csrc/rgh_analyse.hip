@@ -40,6 +40,7 @@ struct MbLds {
     alignas(16) uint8_t pred[256], predu[64], predv[64];
     alignas(16) uint8_t etop[16], eleft[16], ctop[2][8], cleft[2][8];
     alignas(16) int32_t etl, ctl[2];
+    alignas(16) uint32_t nb[4];  // Fast: costs of the last diamond iteration's four points
     union {
         struct {
             alignas(16) uint8_t sr[SR_H * SR_W];
@@ -828,9 +829,24 @@ struct I16Analysis {
     int mode, pa, pb, pc, dc;
 };
 
+// enc.rs:1626 i16_dc from the edge samples in S (zero when a neighbour is missing).
+__device__ __forceinline__ int i16_dc_of(const MbLds &S, int has_t, int has_l) {
+    const uint4 te = *(const uint4 *)S.etop, le = *(const uint4 *)S.eleft;
+    uint32_t st = __builtin_amdgcn_sad_u8(te.x, 0u, 0u), sl = __builtin_amdgcn_sad_u8(le.x, 0u, 0u);
+    st = __builtin_amdgcn_sad_u8(te.y, 0u, st);
+    st = __builtin_amdgcn_sad_u8(te.z, 0u, st);
+    st = __builtin_amdgcn_sad_u8(te.w, 0u, st);
+    sl = __builtin_amdgcn_sad_u8(le.y, 0u, sl);
+    sl = __builtin_amdgcn_sad_u8(le.z, 0u, sl);
+    sl = __builtin_amdgcn_sad_u8(le.w, 0u, sl);
+    return (has_t && has_l) ? (int)((st + sl + 16) >> 5) : (has_t ? (int)((st + 8) >> 4) : (has_l ? (int)((sl + 8) >> 4) : 128));
+}
+
 // V, H and DC SATDs from one source Hadamard, plane SATD by prediction; the intra edges and
 // the source are in S. One 4x4 block row per lane (block (lane >> 2) & 3, lane >> 4 in raster
 // order, row lane & 3): rows in the lane, columns across the quad.
+// PLANE false: no plane mode (the Fast preset, enc.rs:3322).
+template <bool PLANE = true>
 __device__ I16Analysis analyse_i16(MbLds &S, int mb_x, int mb_y, uint16_t &n_satd) {
     const int lane = threadIdx.x;
     const int has_t = mb_y > 0, has_l = mb_x > 0, tl = S.etl;
@@ -913,7 +929,7 @@ __device__ I16Analysis analyse_i16(MbLds &S, int mb_x, int mb_y, uint16_t &n_sat
     int im = 2;
     if (has_t && sv < ib) { ib = sv; im = 0; }
     if (has_l && sh < ib) { ib = sh; im = 1; }
-    if (has_t && has_l) {
+    if (PLANE && has_t && has_l) {
         const uint32_t ps = sums[3] >> 1;
         n_satd++;
         if (ps < ib) { ib = ps; im = 3; }
@@ -929,8 +945,10 @@ struct ChromaIntra {
 // enc.rs:1523 analyse_chroma_intra: the chroma mode by SAD + 8 for non-DC modes (predictors
 // h264.rs:1967 and enc.rs:1588) from the chroma edges in S; the lane's predicted U and V samples.
 // All four modes are predicted and their SADs reduced together; unavailable modes are computed
-// from whatever the edge arrays hold and then ignored.
-__device__ ChromaIntra chroma_intra(MbLds &S, int mb_x, int mb_y, uint32_t src_u, uint32_t src_v) {
+// from whatever the edge arrays hold and then ignored. PLANE false: no plane mode (Fast); FORCED:
+// the mode `forced` without analysis (Fast P pictures: the direction of the luma mode).
+template <bool PLANE = true, bool FORCED = false>
+__device__ ChromaIntra chroma_intra(MbLds &S, int mb_x, int mb_y, uint32_t src_u, uint32_t src_v, int forced = 0) {
     const int lane = threadIdx.x;
     const int has_t = mb_y > 0, has_l = mb_x > 0;
     const int cr = lane >> 3, cc = lane & 7;
@@ -987,7 +1005,8 @@ __device__ ChromaIntra chroma_intra(MbLds &S, int mb_x, int mb_y, uint32_t src_u
         cbest = cost[2] + 8u;
         cmode = 2;
     }
-    if (has_t && has_l && cost[3] + 8u < cbest) cmode = 3;
+    if (PLANE && has_t && has_l && cost[3] + 8u < cbest) cmode = 3;
+    if (FORCED) cmode = forced;
     const int pu = cmode == 0 ? smp[0][0] : (cmode == 1 ? smp[1][0] : (cmode == 2 ? smp[2][0] : smp[3][0]));
     const int pv = cmode == 0 ? smp[0][1] : (cmode == 1 ? smp[1][1] : (cmode == 2 ? smp[2][1] : smp[3][1]));
     return {cmode, (uint32_t)pu, (uint32_t)pv};
@@ -1059,8 +1078,9 @@ struct I16Code {
 // Codes the macroblock as I_16x16 in mode A.mode: the prediction (into S.pred, then the
 // reconstruction), chroma mode and prediction, luma AC / DC levels (S.coef, S.dcq), chroma
 // levels and reconstruction.
+template <bool CPLANE = true, bool CFORCED = false>
 __device__ I16Code code_i16(MbLds &S, int mb_x, int mb_y, int qp, int qpc, const I16Analysis &A, uint32_t src_u,
-                            uint32_t src_v) {
+                            uint32_t src_v, int cforced = 0) {
     const int lane = threadIdx.x, rr = lane >> 2, c0 = (lane & 3) * 4;
     const int has_t = mb_y > 0, has_l = mb_x > 0;
     const int im = A.mode, pa = A.pa, pb = A.pb, pcc = A.pc, dc = A.dc;
@@ -1082,7 +1102,7 @@ __device__ I16Code code_i16(MbLds &S, int mb_x, int mb_y, int qp, int qpc, const
         }
         *(uint32_t *)&S.pred[lane * 4] = w;
     }
-    const ChromaIntra ci = chroma_intra(S, mb_x, mb_y, src_u, src_v);
+    const ChromaIntra ci = chroma_intra<CPLANE, CFORCED>(S, mb_x, mb_y, src_u, src_v, cforced);
     cmode = ci.mode;
     pu = ci.pu;
     pv = ci.pv;
@@ -1264,7 +1284,7 @@ __device__ bool ref_box_wait(const RghJobD &J, int px0, int px1, int py0, int py
 // the neighbours, the co-located decision and the reference item of this macroblock were waited
 // for by the caller); otherwise per-step launches (reads issued before the readiness verdict are
 // discarded when the macroblock defers).
-template <bool LEARNED, bool PERSIST = false>
+template <bool LEARNED, bool PERSIST = false, bool FAST = false>
 __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, int mb_x, int mb_y, MbLds &S,
                                            const Gate &G) {
     const int lane = threadIdx.x;
@@ -1314,8 +1334,11 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
     const bool has_l = mb_x > 0, has_t = mb_y > 0, has_c = has_t && (mb_x + 1 < mb_w || has_l);
     const int ci = mb_x + 1 < mb_w ? idx - mb_w + 1 : idx - mb_w - 1;
     uint32_t dw0 = 0, dw2 = 0;
-    if (lane < 4) {
-        const int di = lane == 0 ? (has_l ? idx - 1 : idx) : lane == 1 ? (has_t ? idx - mb_w : idx) : (has_c ? ci : idx);
+    if (lane < (FAST ? 5 : 4)) {
+        // Lane 4: the top-left decision (the Fast preset's skip-probe gate).
+        const int di = lane == 0 ? (has_l ? idx - 1 : idx)
+                                 : lane == 1 ? (has_t ? idx - mb_w : idx)
+                                             : lane == 4 ? (has_l && has_t ? idx - mb_w - 1 : idx) : (has_c ? ci : idx);
         const RGH_G uint32_t *dp = (const RGH_G uint32_t *)(lane == 3 ? &J.prev[idx] : &J.dec[di]);
         dw0 = dp[0];
         dw2 = dp[2];
@@ -1342,10 +1365,16 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
         ew = *(const RGH_G uint32_t *)(ea - ((uintptr_t)ea & 3));
     }
     int lref = -2, tref = -2, dref = -2, ax = 0, ay = 0, bx_ = 0, by_ = 0, cxv = 0, cyv = 0;
+    int nskip = 1;  // Fast: a P_Skip among the left, top, top-left and top-right neighbours
     {
         const uint32_t l0 = __builtin_amdgcn_readlane(dw0, 0), l2 = __builtin_amdgcn_readlane(dw2, 0);
         const uint32_t t0 = __builtin_amdgcn_readlane(dw0, 1), t2 = __builtin_amdgcn_readlane(dw2, 1);
         const uint32_t c0w = __builtin_amdgcn_readlane(dw0, 2), c2 = __builtin_amdgcn_readlane(dw2, 2);
+        if constexpr (FAST) {
+            const uint32_t tl0 = __builtin_amdgcn_readlane(dw0, 4);
+            nskip = (int)(has_l && (l0 & 0xFF) == 0) | (int)(has_t && (t0 & 0xFF) == 0) |
+                    (int)(has_t && mb_x + 1 < mb_w && (c0w & 0xFF) == 0) | (int)(has_l && has_t && (tl0 & 0xFF) == 0);
+        }
         // Intra neighbours have no motion (an I_4x4 record's mv words hold its modes).
         if (has_l) {
             lref = (l0 & 0xFF) >= 2 ? -1 : 0;
@@ -1487,7 +1516,8 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
         for (int k = 0; k < NC; k++) rw[k] = live[k] ? ld4u(rb + (long)fys[k] * J.sy + fxs[k]) : 0;
     }
     const int p_skip = (int)(pw2 & 0xFF), p_intra = (int)((pw2 >> 16) & 0xFF);
-    const bool probe = in_bounds(pskx, psky) && (!LEARNED || p_skip >= J.learn_skip_gate);
+    // The Fast preset probes only next to a skipped macroblock (enc.rs:3050-3063).
+    const bool probe = in_bounds(pskx, psky) && (!LEARNED || p_skip >= J.learn_skip_gate) && (!FAST || nskip);
     const int int_y = ((pskx | psky) & 3) == 0;
     uint32_t predw = 0, pu = 0, pv = 0;
     ChromaTaps ct{};
@@ -1664,6 +1694,8 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
         const int dk = lane >> 4, drow = lane & 15;
         const int ddx = dk == 1 ? -1 : (dk == 2 ? 1 : 0), ddy = dk == 0 ? -1 : (dk == 3 ? 1 : 0);
         const uint4 srow = *(const uint4 *)&S.src[drow * 16];
+        // Fast: the four points' costs of the last iteration (enc.rs:1207 nb), in LDS.
+        if (FAST && lane < 4) S.nb[lane] = 0xFFFFFFFFu;
         for (int it = 0; it < iters; it++) {
             if (iabs(bestx - scx) >= 16 || iabs(besty - scy) >= 16) {
                 if (J.ref_fin) {
@@ -1703,13 +1735,19 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
             }
             // The first strictly cheapest in-range point: the least (cost, k) key (costs stay
             // below 2^30).
-            uint32_t key = 0xFFFFFFFFu;
+            uint32_t key = 0xFFFFFFFFu, cst[4];
             int nok = 0;
 #pragma unroll
             for (int k = 0; k < 4; k++) {
-                const uint32_t cst = sums[k] + mv_cost(px[k] * 4, py[k] * 4);
+                cst[k] = sums[k] + mv_cost(px[k] * 4, py[k] * 4);
                 nok += ok[k];
-                key = min(key, ok[k] ? (cst << 2) | (uint32_t)k : 0xFFFFFFFFu);
+                key = min(key, ok[k] ? (cst[k] << 2) | (uint32_t)k : 0xFFFFFFFFu);
+            }
+            if (FAST && lane < 4) {
+                const int k = lane;
+                S.nb[k] = (k == 0 ? ok[0] : k == 1 ? ok[1] : k == 2 ? ok[2] : ok[3])
+                              ? (k == 0 ? cst[0] : k == 1 ? cst[1] : k == 2 ? cst[2] : cst[3])
+                              : 0xFFFFFFFFu;
             }
             D.n_sad += nok;
             if ((key >> 2) >= best_cost) break;
@@ -1733,6 +1771,53 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
             predw = lds4u(&S.sr[(fby - roy + rr) * SR_W], fbx - rox + c0);
             mvx = bestx * 4;
             mvy = besty * 4;
+        } else if (FAST) {
+            // enc.rs:1341 half_pel_toward: on each axis the half sample toward the cheaper
+            // full-pel neighbour, estimated by the SAD against the average of its two integer
+            // neighbours (out of bounds: no estimate); the better estimate (vertical on a tie),
+            // if below the full-pel cost, is interpolated (the window's 6-tap planes) and taken if
+            // its cost is below the full-pel cost.
+            __syncthreads();
+            const uint32_t n0 = S.nb[0], n1 = S.nb[1], n2 = S.nb[2], n3 = S.nb[3];
+            const bool up = n0 < n3, left = n1 < n2;
+            const int cx = bestx * 4, cy = besty * 4;
+            const int vx = cx, vy = cy + (up ? -2 : 2), hx = cx + (left ? -2 : 2), hy = cy;
+            const int ry = fby - roy + rr, rx = fbx - rox + c0;
+            const uint32_t a = lds4u(&S.sr[ry * SR_W], rx);
+            const uint32_t bv = lds4u(&S.sr[(ry + (up ? -1 : 1)) * SR_W], rx);
+            const uint32_t bh = lds4u(&S.sr[ry * SR_W], rx + (left ? -1 : 1));
+            const uint32_t sw = *(const uint32_t *)&S.src[lane * 4];
+            const uint32_t in2[2] = {__builtin_amdgcn_sad_u8(sw, avg4(a, bv), 0u), __builtin_amdgcn_sad_u8(sw, avg4(a, bh), 0u)};
+            uint32_t e2[2];
+            wave_sums<2>(in2, e2);
+            const int okv = in_bounds(vx, vy), okh = in_bounds(hx, hy);
+            const uint32_t ev = okv ? e2[0] + mv_cost(vx, vy) : 0xFFFFFFFFu;
+            const uint32_t eh = okh ? e2[1] + mv_cost(hx, hy) : 0xFFFFFFFFu;
+            D.n_subpel += okv + okh;
+            const bool vert = ev <= eh;
+            const int hmx = vert ? vx : hx, hmy = vert ? vy : hy;
+            bool took = false;
+            if ((vert ? ev : eh) < me_cost) {
+                compute_window(S, fbx - 1 - rox, fby - 1 - roy);
+                D.n_hpel++;
+                D.n_subpel++;
+                const int qx[4] = {hmx, hmx, hmx, hmx}, qy[4] = {hmy, hmy, hmy, hmy};
+                uint32_t sads[4];
+                subpel_sad4(x0, y0, fbx, fby, rox, roy, qx, qy, S, sads);
+                const uint32_t c = sads[0] + mv_cost(hmx, hmy);
+                if (c < me_cost) {
+                    me_cost = c;
+                    predw = window_pred4(x0, y0, fbx, fby, rox, roy, hmx, hmy, S);
+                    mvx = hmx;
+                    mvy = hmy;
+                    took = true;
+                }
+            }
+            if (!took) {
+                predw = a;
+                mvx = cx;
+                mvy = cy;
+            }
         } else {
             const int cx = bestx * 4, cy = besty * 4;
             compute_window(S, fbx - 1 - rox, fby - 1 - roy);
@@ -1791,9 +1876,29 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
         *(uint32_t *)&S.pred[lane * 4] = predw;
         __syncthreads();
 
-        // Intra 16x16 when the inter prediction is poor (enc.rs:3119-3152).
-        const uint32_t inter_satd = satd16(S.src, S.pred);
-        D.n_satd++;
+        // Intra 16x16 when the inter prediction is poor (enc.rs:3119-3152). The Fast preset first
+        // needs the best of the V / H / DC SADs within 1.75 times the inter SAD
+        // (simd/h264_enc.rs:612 intra_sad_x3_16x16; missing edges read as zero).
+        bool try_intra = true;
+        if constexpr (FAST) {
+            const uint32_t inter_sad = me_cost - mv_cost(mvx, mvy);
+            const int dcv = i16_dc_of(S, mb_y > 0, mb_x > 0);
+            const uint32_t sw = *(const uint32_t *)&S.src[lane * 4];
+            const uint32_t tw = *(const uint32_t *)&S.etop[c0], lw = (uint32_t)S.eleft[rr] * 0x01010101u;
+            const uint32_t in2[2] = {__builtin_amdgcn_sad_u8(sw, tw, 0u) | (__builtin_amdgcn_sad_u8(sw, lw, 0u) << 16),
+                                     __builtin_amdgcn_sad_u8(sw, (uint32_t)dcv * 0x01010101u, 0u)};
+            uint32_t o2[2];
+            wave_sums<2>(in2, o2);
+            uint32_t best = o2[1];
+            if (mb_y > 0) best = min(best, o2[0] & 0xFFFFu);
+            if (mb_x > 0) best = min(best, o2[0] >> 16);
+            try_intra = 4u * best < 7u * inter_sad;
+        }
+        uint32_t inter_satd = 0;
+        if (try_intra) {
+            inter_satd = satd16(S.src, S.pred);
+            D.n_satd++;
+        }
         const uint32_t inter_cost = inter_satd + lambda * (se_bits(mvx - mvpx) + se_bits(mvy - mvpy) + 1);
         bool try_i16 = inter_satd > 24 * lambda;
         if constexpr (LEARNED) {
@@ -1803,13 +1908,15 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
             }
         }
         if (try_i16) {
-            const I16Analysis ia = analyse_i16(S, mb_x, mb_y, D.n_satd);
-            if (ia.cost + lambda * 8 < inter_cost) {
-                // enc.rs:3587 code_i16 in a P slice.
+            const I16Analysis ia = analyse_i16<!FAST>(S, mb_x, mb_y, D.n_satd);
+            // The Fast preset asks intra to win by twice the margin.
+            if (ia.cost + lambda * (FAST ? 16u : 8u) < inter_cost) {
+                // enc.rs:3587 code_i16 in a P slice (Fast: chroma in the direction of the luma mode).
                 mb_type = 2;
                 i16_mode = ia.mode;
                 mvx = mvy = 0;
-                const I16Code ic = code_i16(S, mb_x, mb_y, qp, qpc, ia, src_u, src_v);
+                const int from_luma = ia.mode == 0 ? 2 : (ia.mode == 1 ? 1 : (ia.mode == 2 ? 0 : 3));
+                const I16Code ic = code_i16<true, FAST>(S, mb_x, mb_y, qp, qpc, ia, src_u, src_v, from_luma);
                 cbpl = ic.cbpl;
                 cbpc = ic.cbpc;
                 cmode = ic.cmode;
@@ -2208,7 +2315,9 @@ __device__ bool analyse_i4(MbLds &S, int mb_x, int mb_y, int mb_w, int qp, uint3
     return true;
 }
 
-// One IDR macroblock: decision record, levels and reconstruction.
+// One IDR macroblock: decision record, levels and reconstruction. FAST: the Fast preset's IDR
+// analysis (no plane modes, no intra 4x4; enc.rs:3322-3330).
+template <bool FAST = false>
 __device__ __forceinline__ void analyse_intra_mb(const RghJobD &J, const RghJob *jp, int mb_x, int mb_y, MbLds &S) {
     const int lane = threadIdx.x, rr = lane >> 2, c0 = (lane & 3) * 4;
     const int mb_w = J.mb_w, idx = mb_y * mb_w + mb_x;
@@ -2299,12 +2408,12 @@ __device__ __forceinline__ void analyse_intra_mb(const RghJobD &J, const RghJob 
     const uint32_t tr_word = has_tr ? (uint32_t)__builtin_amdgcn_readlane((int)ew, 36) : 0u;
 
     uint16_t n_satd = 0;
-    const I16Analysis ia = analyse_i16(S, mb_x, mb_y, n_satd);
+    const I16Analysis ia = analyse_i16<!FAST>(S, mb_x, mb_y, n_satd);
     int mb_type = 2, cbpl = 0, cbpc = 0, cmode = 0;
     uint32_t pu = 0, pv = 0;
     uint64_t modes = 0;
     bool i4 = false;
-    if (ia.cost > 40 * lambda) {
+    if (!FAST && ia.cost > 40 * lambda) {
         // The canvas shares storage with the 16x16 analysis' plane prediction: filled now.
         __syncthreads();
         if (lane < 16)
@@ -2334,7 +2443,7 @@ __device__ __forceinline__ void analyse_intra_mb(const RghJobD &J, const RghJob 
         recon_chroma(S, cbpc, qpc);
     } else {
         cbpl = 0;
-        const I16Code ic = code_i16(S, mb_x, mb_y, qp, qpc, ia, src_u, src_v);
+        const I16Code ic = code_i16<!FAST>(S, mb_x, mb_y, qp, qpc, ia, src_u, src_v);
         cbpl = ic.cbpl;
         cbpc = ic.cbpc;
         cmode = ic.cmode;
@@ -2420,8 +2529,11 @@ struct StepLdsOf<2> {
 // budget; Balanced P pictures searched from learned proposals run rgh_k_learned_step. A launch
 // that holds both IDR and Balanced P pictures runs them in one kernel (rgh_k_mixed_step, or
 // rgh_k_mixed_learned_step with learned proposals), each block taking its picture's analysis,
-// so that neither waits for the other. MODE: 0 Balanced P, 1 IDR, 2 Compact P, 3 Balanced P with
-// learned proposals, 4 IDR and Balanced P, 5 IDR and Balanced P with learned proposals.
+// so that neither waits for the other. Fast-preset P pictures run their own pair (rgh_k_fast_step,
+// rgh_k_mixed_fast_step, and rgh_k_intra_fast_step for its IDR pictures): the preset's branches
+// stay out of the Balanced kernels' code. MODE: 0 Balanced P, 1 IDR, 2 Compact P, 3 Balanced P
+// with learned proposals, 4 IDR and Balanced P, 5 IDR and Balanced P with learned proposals,
+// 6 Fast P, 7 IDR and Fast P, 8 Fast IDR.
 template <int MODE>
 __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int j0, int launch, uint32_t seq) {
     const RghJobD J = device_view(jobs[j0 + (int)blockIdx.y]);
@@ -2445,17 +2557,17 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
         if (s == 0 && J.wait_analysed) ok = __atomic_load_n(J.wait_analysed, __ATOMIC_RELAXED) >= J.wait_total;
         if (ok) {
             const RghJob *jp = jobs + j0 + (int)blockIdx.y;
-            if constexpr (MODE == 1) {
-                analyse_intra_mb(J, jp, x, y, S.a);
-            } else if constexpr (MODE == 4 || MODE == 5) {
+            if constexpr (MODE == 1 || MODE == 8) {
+                analyse_intra_mb<MODE == 8>(J, jp, x, y, S.a);
+            } else if constexpr (MODE == 4 || MODE == 5 || MODE == 7) {
                 if (J.intra)
-                    analyse_intra_mb(J, jp, x, y, S.a);
+                    analyse_intra_mb<MODE == 7>(J, jp, x, y, S.a);
                 else
-                    ok = analyse_mb<MODE == 5>(J, jp, x, y, S.a, Gate{seq, nullptr, 0, -1, 0});
+                    ok = analyse_mb<MODE == 5, false, MODE == 7>(J, jp, x, y, S.a, Gate{seq, nullptr, 0, -1, 0});
             } else if constexpr (MODE == 2)
                 ok = analyse_compact_mb(J, x, y, S.a, seq);
             else
-                ok = analyse_mb<MODE == 3>(J, jp, x, y, S.a, Gate{seq, nullptr, 0, -1, 0});
+                ok = analyse_mb<MODE == 3, false, MODE == 6>(J, jp, x, y, S.a, Gate{seq, nullptr, 0, -1, 0});
         }
         if (lane == 0) {
             // The stamp and progress buffers, read again from the descriptor (as late_outputs).
@@ -2507,6 +2619,21 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_mixed_step(const RghJob *
 extern "C" __global__ __launch_bounds__(64) void rgh_k_mixed_learned_step(const RghJob *__restrict__ jobs, int j0,
                                                                            int launch, uint32_t seq) {
     frame_step<5>(jobs, j0, launch, seq);
+}
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_intra_fast_step(const RghJob *__restrict__ jobs, int j0,
+                                                                        int launch, uint32_t seq) {
+    frame_step<8>(jobs, j0, launch, seq);
+}
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_fast_step(const RghJob *__restrict__ jobs, int j0, int launch,
+                                                                  uint32_t seq) {
+    frame_step<6>(jobs, j0, launch, seq);
+}
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_mixed_fast_step(const RghJob *__restrict__ jobs, int j0,
+                                                                        int launch, uint32_t seq) {
+    frame_step<7>(jobs, j0, launch, seq);
 }
 
 extern "C" __global__ __launch_bounds__(64) void rgh_k_compact_step(const RghJob *__restrict__ jobs, int j0, int launch,
@@ -2644,9 +2771,9 @@ __device__ __forceinline__ void persist_item(const RghJob *__restrict__ jobs, co
         const uint64_t t_run0 = __builtin_amdgcn_s_memtime();
         bool ok = true;
         if (J.intra)
-            analyse_intra_mb(J, jp, x, y, S.a);
+            analyse_intra_mb<MODE == 2>(J, jp, x, y, S.a);
         else
-            ok = analyse_mb<MODE == 1, true>(J, jp, x, y, S.a, Gate{seq, P, T, J.ref_job, ref_base});
+            ok = analyse_mb<MODE == 1, true, MODE == 2>(J, jp, x, y, S.a, Gate{seq, P, T, J.ref_job, ref_base});
         if (!ok) {
             defer_item(P, j, T);
             return;
@@ -2728,7 +2855,8 @@ __device__ __forceinline__ void persist_item(const RghJob *__restrict__ jobs, co
     }
 }
 
-// MODE 0: Balanced P and IDR pictures; 1: Balanced P searched from learned proposals, and IDR.
+// MODE 0: Balanced P and IDR pictures; 1: Balanced P searched from learned proposals, and IDR;
+// 2: Fast P and IDR.
 // Each workgroup (one wave) runs one ticket of the queue of the XCD it runs on. The grid is dealt
 // round-robin over the n_xcc XCDs in workgroup order (from whichever XCD the dispatcher's turn
 // starts at), so each group of n_xcc consecutive workgroups puts one on every XCD and workgroup b
@@ -2792,4 +2920,9 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_persist(const RghJob *__r
 extern "C" __global__ __launch_bounds__(64) void rgh_k_persist_learned(const RghJob *__restrict__ jobs,
                                                                         const RghPersist *__restrict__ P, uint32_t seq) {
     persist<1>(jobs, P, seq);
+}
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_persist_fast(const RghJob *__restrict__ jobs,
+                                                                     const RghPersist *__restrict__ P, uint32_t seq) {
+    persist<2>(jobs, P, seq);
 }

@@ -22,6 +22,10 @@ extern "C" __global__ void rgh_k_mixed_step(const RghJob *jobs, int j0, int laun
 extern "C" __global__ void rgh_k_mixed_learned_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_persist(const RghJob *jobs, const RghPersist *P, uint32_t seq);
 extern "C" __global__ void rgh_k_persist_learned(const RghJob *jobs, const RghPersist *P, uint32_t seq);
+extern "C" __global__ void rgh_k_persist_fast(const RghJob *jobs, const RghPersist *P, uint32_t seq);
+extern "C" __global__ void rgh_k_intra_fast_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
+extern "C" __global__ void rgh_k_fast_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
+extern "C" __global__ void rgh_k_mixed_fast_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_xcc_probe(uint32_t *mask);
 extern "C" __global__ void rgh_k_recount(const RghJob *jobs);
 extern "C" __global__ void rgh_k_extend(const RghJob *jobs);
@@ -75,6 +79,7 @@ struct Ctx {
     hipStream_t stream = nullptr;
     int width = 0, height = 0, mb_w = 0, mb_h = 0, profile = 0, qp = 0, me_iters = 16, qpel_iters = 2;
     bool compact = false;  // RGH_FLAG_COMPACT
+    bool fast = false;     // RGH_FLAG_FAST
     int cqo = 0;           // chroma_qp_index_offset
     int sy = 0, sc = 0, rows_y = 0, rows_c = 0;
     std::vector<Picture> pic;  // ring of reconstructed pictures
@@ -204,8 +209,9 @@ RghJob make_job(Ctx *c, int k, const rgh_i420 &src, rgh_mb_decision *dec, const 
     j.sb_chroma_ssd = c->sb[3];
     j.sb_probe = c->sb[4];
     j.steps = frame_steps(c);
-    j.deblock = c->qp != 0;
+    j.deblock = c->qp != 0 && !c->fast;  // Fast slices disable the loop filter (enc.rs:2334)
     j.compact = c->compact ? 1 : 0;
+    j.fast = c->fast ? 1 : 0;
     j.cqo = c->cqo;
     j.base = base;
     j.serial = c->serial + 1 + (uint32_t)k;
@@ -243,7 +249,7 @@ struct Schedule {
     RghFrameCtl *ctl_dev = nullptr;
     RghFrameCtl *ctl_host = nullptr;
     int n = 0, n_frames = 0, lag = 1, nj = 0, n_intra = 0;
-    bool compact = false, learned = false;
+    bool compact = false, fast = false, learned = false;
     bool recount = false;  // recount analysed macroblocks from their stamps (after a persistent launch)
     std::vector<RghJob> jobs;
     std::vector<int> span, last;
@@ -256,8 +262,9 @@ struct Schedule {
         while (n_intra < nj && jobs[(size_t)n_intra].intra) n_intra++;
         // One P kernel per schedule: every job of it uses one preset.
         compact = nj > 0 && jobs[0].compact;
+        fast = nj > 0 && jobs[0].fast;
         for (int j = 0; j < nj; j++)
-            if ((jobs[(size_t)j].compact != 0) != compact) {
+            if ((jobs[(size_t)j].compact != 0) != compact || (jobs[(size_t)j].fast != 0) != fast) {
                 set_error("run_schedule", "contexts of one call must use one preset", __LINE__);
                 return RGH_ERR_INVALID_ARGUMENT;
             }
@@ -302,7 +309,7 @@ struct Schedule {
         const int split = std::min(std::max(n_intra, j_lo), j_hi);
         if (split > j_lo && j_hi > split && !compact) {
             // IDR and Balanced P jobs together: one kernel, so that neither waits for the other.
-            hipLaunchKernelGGL(learned ? rgh_k_mixed_learned_step : rgh_k_mixed_step,
+            hipLaunchKernelGGL(learned ? rgh_k_mixed_learned_step : (fast ? rgh_k_mixed_fast_step : rgh_k_mixed_step),
                                dim3((unsigned)gx, (unsigned)(j_hi - j_lo)), dim3(64), 0, owner->stream, jobs_dev,
                                j_lo, L, seq);
             for (int j = j_lo; j < j_hi; j++) last[(size_t)j] = L;
@@ -310,10 +317,11 @@ struct Schedule {
             return;
         }
         if (split > j_lo)
-            hipLaunchKernelGGL(rgh_k_intra_step, dim3((unsigned)gx, (unsigned)(split - j_lo)), dim3(64), 0, owner->stream,
-                               jobs_dev, j_lo, L, seq);
+            hipLaunchKernelGGL(fast ? rgh_k_intra_fast_step : rgh_k_intra_step, dim3((unsigned)gx, (unsigned)(split - j_lo)),
+                               dim3(64), 0, owner->stream, jobs_dev, j_lo, L, seq);
         if (j_hi > split)
-            hipLaunchKernelGGL(learned ? rgh_k_learned_step : (compact ? rgh_k_compact_step : rgh_k_frame_step),
+            hipLaunchKernelGGL(learned ? rgh_k_learned_step
+                                       : (compact ? rgh_k_compact_step : (fast ? rgh_k_fast_step : rgh_k_frame_step)),
                                dim3((unsigned)gx, (unsigned)(j_hi - split)),
                                dim3(64), 0, owner->stream, jobs_dev, split, L, seq);
         for (int j = j_lo; j < j_hi; j++) last[(size_t)j] = L;
@@ -640,7 +648,8 @@ rgh_status persistent_launch(Ctx *owner, const XccMap &xm, std::vector<RghJob> &
         return RGH_ERR_INVALID_ARGUMENT;
     }
     if (grid)
-        hipLaunchKernelGGL(learned ? rgh_k_persist_learned : rgh_k_persist, dim3((unsigned)grid), dim3(64), 0,
+        hipLaunchKernelGGL(learned ? rgh_k_persist_learned : (lj[0].fast ? rgh_k_persist_fast : rgh_k_persist),
+                           dim3((unsigned)grid), dim3(64), 0,
                            owner->stream, owner->jobs_dev, (const RghPersist *)owner->pbuf, seq);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(owner->phost, owner->pbuf + pl.deferred_off, rb, hipMemcpyDeviceToHost, owner->stream));
@@ -844,6 +853,8 @@ rgh_status rgh_create(const rgh_config *cfg, void **ctx_out) {
     if (cfg->ring < 0 || cfg->ring == 1 || cfg->ring > 64) return RGH_ERR_INVALID_ARGUMENT;
     // The Compact preset: with CABAC it codes B pictures (h264.rs:3075), not analysed here.
     if ((cfg->flags & RGH_FLAG_COMPACT) && cfg->profile != RGH_PROFILE_BASELINE) return RGH_ERR_UNSUPPORTED;
+    if ((cfg->flags & RGH_FLAG_COMPACT) && (cfg->flags & RGH_FLAG_FAST)) return RGH_ERR_INVALID_ARGUMENT;
+    if (cfg->flags & ~(RGH_FLAG_NO_UNDEBLOCKED_COPY | RGH_FLAG_COMPACT | RGH_FLAG_FAST)) return RGH_ERR_INVALID_ARGUMENT;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RGH_ERR_DEVICE;
     if (cfg->device < 0 || cfg->device >= ndev) return RGH_ERR_INVALID_ARGUMENT;
@@ -860,6 +871,7 @@ rgh_status rgh_create(const rgh_config *cfg, void **ctx_out) {
     c->qpel_iters = cfg->qpel_iters ? cfg->qpel_iters : 2;
     c->keep_pre = !(cfg->flags & RGH_FLAG_NO_UNDEBLOCKED_COPY);
     c->compact = (cfg->flags & RGH_FLAG_COMPACT) != 0;
+    c->fast = (cfg->flags & RGH_FLAG_FAST) != 0;
     c->cqo = c->compact ? 1 : 0;
     const int W = c->mb_w * 16, H = c->mb_h * 16;
     c->sy = (W + 2 * RGH_PAD_Y + 31) & ~31;
@@ -982,6 +994,16 @@ rgh_status rgh_set_reference(void *ctx, const rgh_i420 *recon) {
 }
 
 // One frame of each context, outputs into the contexts' own buffers.
+// The contexts of one call share a preset: each preset's P pictures run their own kernels.
+rgh_status check_one_preset(Ctx *const *cs, int n, const char *fn) {
+    for (int i = 1; i < n; i++)
+        if (cs[i]->compact != cs[0]->compact || cs[i]->fast != cs[0]->fast) {
+            set_error(fn, "contexts of one call must use one preset", __LINE__);
+            return RGH_ERR_INVALID_ARGUMENT;
+        }
+    return RGH_OK;
+}
+
 rgh_status analyse_single(Ctx *const *cs, int n, const rgh_i420 *src, bool intra = false) {
     Ctx *c0 = cs[0];
     HIP_TRY(hipSetDevice(c0->device));
@@ -1027,6 +1049,8 @@ rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src) 
         if (s != RGH_OK) return s;
         cs[(size_t)i] = c;
     }
+    rgh_status s = check_one_preset(cs.data(), n, "rgh_analyse_batch");
+    if (s != RGH_OK) return s;
     return analyse_single(cs.data(), n, src);
 }
 
@@ -1055,7 +1079,7 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
         }
         for (int i = 0; i < n; i++) {
             const Ctx *c = static_cast<Ctx *>(ctxs[i]);
-            if (c && c->compact) {
+            if (c && (c->compact || c->fast)) {
                 set_error("rgh_analyse_frames_learned", "learned proposals serve the Balanced preset", __LINE__);
                 return RGH_ERR_UNSUPPORTED;
             }
@@ -1081,6 +1105,10 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
             if (((uintptr_t)decisions[q] & 7) || ((uintptr_t)levels[q] & 7)) return RGH_ERR_INVALID_ARGUMENT;
         }
         cs[(size_t)i] = c;
+    }
+    {
+        rgh_status s = check_one_preset(cs.data(), n, "rgh_analyse_frames");
+        if (s != RGH_OK) return s;
     }
     if (noalloc && (int64_t)n * n_frames > RGH_NOALLOC_MAX_FRAMES) {
         set_error("rgh_analyse_frames_noalloc", "more frames than RGH_NOALLOC_MAX_FRAMES in one call", __LINE__);

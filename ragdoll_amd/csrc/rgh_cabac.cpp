@@ -362,15 +362,16 @@ rgh_status check_common(const rgh_slice_config *cfg, int32_t profile, const void
     if (cfg->qp < 1 || cfg->qp > 51) return RGH_ERR_INVALID_ARGUMENT;
     if (profile != RGH_PROFILE_MAIN && profile != RGH_PROFILE_HIGH) return RGH_ERR_INVALID_ARGUMENT;
     // Compact with CABAC codes B pictures (h264.rs:3075), not written here.
-    if (cfg->speed != RGH_SPEED_BALANCED) return RGH_ERR_UNSUPPORTED;
+    if (cfg->speed != RGH_SPEED_BALANCED && cfg->speed != RGH_SPEED_FAST) return RGH_ERR_UNSUPPORTED;
     return RGH_OK;
 }
 
-// Slice header tail from cabac_init_idc on, and the alignment before the slice data.
-void header_tail(BitWriter &w, bool p_slice, int qp) {
+// Slice header tail from cabac_init_idc on, and the alignment before the slice data. The Fast
+// preset's slices disable the loop filter (enc.rs:2334).
+void header_tail(BitWriter &w, bool p_slice, int qp, bool fast) {
     if (p_slice) w.ue(0);  // cabac_init_idc
     w.se(0);               // slice_qp_delta: the PPS carries the slice QP
-    const int disable = qp == 0 ? 1 : 0;
+    const int disable = qp == 0 || fast ? 1 : 0;
     w.ue((uint32_t)disable);
     if (disable != 1) {
         w.se(0);
@@ -410,7 +411,7 @@ extern "C" rgh_status rgh_write_p_slice_cabac(const rgh_slice_config *cfg, int32
     w.put((uint32_t)cfg->frame_num & 15u, 4);
     w.put((uint32_t)cfg->poc_lsb & 63u, 6);
     w.put(0, 3);
-    header_tail(w, true, cfg->qp);
+    header_tail(w, true, cfg->qp, cfg->speed == RGH_SPEED_FAST);
     std::vector<uint8_t> &rbsp = w.bytes();
     Arith a(rbsp, cfg->qp, 1);
     Neighbours nb(mw, mh);
@@ -489,7 +490,7 @@ extern "C" rgh_status rgh_write_idr_slice_cabac(const rgh_slice_config *cfg, int
     w.ue((uint32_t)idr_pic_id);
     w.put(0, 6);
     w.put(0, 2);
-    header_tail(w, false, cfg->qp);
+    header_tail(w, false, cfg->qp, cfg->speed == RGH_SPEED_FAST);
     std::vector<uint8_t> &rbsp = w.bytes();
     Arith a(rbsp, cfg->qp, 0);
     Neighbours nb(mw, mh);

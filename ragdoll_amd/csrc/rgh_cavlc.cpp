@@ -301,7 +301,7 @@ extern "C" rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const
     const int mb_w = cfg->mb_w, mb_h = cfg->mb_h;
     if (mb_w <= 0 || mb_h <= 0 || mb_w > 512 || mb_h > 512) return RGH_ERR_INVALID_DIMENSIONS;
     if (cfg->qp < 0 || cfg->qp > 51 || cfg->frame_num < 0 || cfg->poc_lsb < 0 ||
-        (cfg->speed != RGH_SPEED_BALANCED && cfg->speed != RGH_SPEED_COMPACT))
+        (cfg->speed != RGH_SPEED_BALANCED && cfg->speed != RGH_SPEED_COMPACT && cfg->speed != RGH_SPEED_FAST))
         return RGH_ERR_INVALID_ARGUMENT;
     const bool compact = cfg->speed == RGH_SPEED_COMPACT;
     BitWriter w;
@@ -319,7 +319,8 @@ extern "C" rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const
     w.put(0, 1);
     w.put(0, 1);
     w.se(0);
-    const int disable = cfg->qp == 0 ? 1 : 0;
+    // The Fast preset's slices disable the loop filter (enc.rs:2334).
+    const int disable = cfg->qp == 0 || cfg->speed == RGH_SPEED_FAST ? 1 : 0;
     w.ue((uint32_t)disable);
     if (disable != 1) {
         w.se(0);
@@ -409,7 +410,9 @@ extern "C" rgh_status rgh_write_idr_slice_cavlc(const rgh_slice_config *cfg, int
     if (!cfg || !decisions || !levels || !out || !out_len || cap < 0 || idr_pic_id < 0) return RGH_ERR_INVALID_ARGUMENT;
     const int mb_w = cfg->mb_w, mb_h = cfg->mb_h;
     if (mb_w <= 0 || mb_h <= 0 || mb_w > 512 || mb_h > 512) return RGH_ERR_INVALID_DIMENSIONS;
-    if (cfg->qp < 0 || cfg->qp > 51) return RGH_ERR_INVALID_ARGUMENT;
+    if (cfg->qp < 0 || cfg->qp > 51 ||
+        (cfg->speed != RGH_SPEED_BALANCED && cfg->speed != RGH_SPEED_COMPACT && cfg->speed != RGH_SPEED_FAST))
+        return RGH_ERR_INVALID_ARGUMENT;
     BitWriter w;
     // Slice header of an IDR picture: first_mb_in_slice, slice_type I (7), pps id, frame_num 0,
     // idr_pic_id, pic_order_cnt_lsb 0, no_output_of_prior_pics / long_term_reference flags,
@@ -423,7 +426,7 @@ extern "C" rgh_status rgh_write_idr_slice_cavlc(const rgh_slice_config *cfg, int
     w.put(0, 1);
     w.put(0, 1);
     w.se(0);
-    const int disable = cfg->qp == 0 ? 1 : 0;
+    const int disable = cfg->qp == 0 || cfg->speed == RGH_SPEED_FAST ? 1 : 0;
     w.ue((uint32_t)disable);
     if (disable != 1) {
         w.se(0);
@@ -464,7 +467,8 @@ extern "C" rgh_status rgh_write_headers(const rgh_stream_config *cfg, uint8_t *o
         cfg->height > 8192)
         return RGH_ERR_INVALID_DIMENSIONS;
     if (cfg->profile < RGH_PROFILE_BASELINE || cfg->profile > RGH_PROFILE_HIGH || cfg->qp < 0 || cfg->qp > 51 ||
-        cfg->fps_num <= 0 || cfg->fps_den <= 0 || (cfg->speed != RGH_SPEED_BALANCED && cfg->speed != RGH_SPEED_COMPACT))
+        cfg->fps_num <= 0 || cfg->fps_den <= 0 ||
+        (cfg->speed != RGH_SPEED_BALANCED && cfg->speed != RGH_SPEED_COMPACT && cfg->speed != RGH_SPEED_FAST))
         return RGH_ERR_INVALID_ARGUMENT;
     const int cqo = cfg->speed == RGH_SPEED_COMPACT ? 1 : 0;  // h264.rs:3087 sync_reorder
     const int mb_w = (cfg->width + 15) / 16, mb_h = (cfg->height + 15) / 16;

@@ -62,7 +62,7 @@ struct RghJob {
     // Persistent schedule: the jobs of this launch that hold the reference frame and the frame
     // gating the picture slot (-1: not in this launch, so complete before it started).
     int32_t ref_job, wait_job;
-    int32_t pad_;
+    int32_t fast;  // the Fast preset's decisions (RGH_FLAG_FAST)
 };
 
 // The device-side view of a job: the same fields with its buffers typed as global memory, so
@@ -92,6 +92,7 @@ struct RghJobD {
     const RGH_G int32_t *wait_analysed;
     int32_t wait_total;
     int32_t ref_job, wait_job;
+    int32_t fast;
 };
 
 template <typename T>
@@ -156,6 +157,7 @@ __device__ __forceinline__ RghJobD device_view(const RghJob &j) {
     d.wait_total = j.wait_total;
     d.ref_job = j.ref_job;
     d.wait_job = j.wait_job;
+    d.fast = j.fast;
     return d;
 }
 

@@ -276,8 +276,10 @@ def parse_slice_header(b, nal_type, ref_idc, spss, ppss):
     h["qp"] = pps["init_qp"] + b.se()
     if st in (3, 4):
         raise ParseError("SP/SI slices are not parsed")
+    h["disable_deblock"] = 0
     if pps["deblock_ctrl"]:
-        if b.ue() != 1:
+        h["disable_deblock"] = b.ue()
+        if h["disable_deblock"] != 1:
             b.se()
             b.se()
     return h

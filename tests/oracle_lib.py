@@ -36,6 +36,8 @@ def lib():
         L.ora_compact_pred.argtypes = [P, I, I, I, I, I, I, I, I, P]
         L.ora_set_compact.restype = I
         L.ora_set_compact.argtypes = [P, I]
+        L.ora_set_fast.restype = I
+        L.ora_set_fast.argtypes = [P, I]
         L.ora_encode_frame.restype = L64
         L.ora_encode_frame.argtypes = [P, P, P, P, I, I, I, P, L64]
         L.ora_headers.restype = L64
@@ -86,6 +88,8 @@ class OracleEncoder:
         self.L.ora_set_me_range(self.h, me_range)
         if speed == "compact" and self.L.ora_set_compact(self.h, 1) != 0:
             raise ValueError("the Compact preset codes B pictures with CABAC")
+        if speed == "fast" and self.L.ora_set_fast(self.h, 1) != 0:
+            raise ValueError("the Fast preset does not combine with Compact")
         self.width, self.height, self.qp = width, height, qp
         self.mb_w = self.L.ora_mb_w(self.h)
         self.mb_h = self.L.ora_mb_h(self.h)

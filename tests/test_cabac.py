@@ -196,6 +196,33 @@ def test_writer_matches_the_oracle(profile, qp):
         assert got == want, "%s qp %d frame %d: %d vs %d bytes" % (profile, qp, k, len(got), len(want))
 
 
+@pytest.mark.parametrize("profile", ["main", "high"])
+@pytest.mark.parametrize("qp", [6, 26, 40])
+def test_fast_writer_matches_the_oracle(profile, qp):
+    # Fast records (half-pel vectors, Intra 16x16 only) and slice headers with the loop filter off.
+    from ragdoll_amd import write_headers, write_idr_slice_cabac, write_p_slice_cabac
+    w, h, keyint = 176, 144, 3
+    frames = scene(w, h, 5, 900 + qp)
+    enc = OracleEncoder(w, h, qp, profile=profile, keyint=keyint, speed="fast")
+    stream = b""
+    for k, f in enumerate(frames):
+        want = enc.encode(*f)
+        stream += want
+        dec, lev = enc.decisions(), enc.levels()
+        assert not (dec["mb_type"] == 3).any()
+        gop = k % keyint
+        if enc.last_was_idr:
+            got = write_headers(w, h, qp, profile=profile, speed="fast") + write_idr_slice_cabac(
+                dec, lev, enc.mb_w, enc.mb_h, qp, k // keyint, profile=profile, speed="fast")
+        else:
+            got = write_p_slice_cabac(dec, lev, enc.mb_w, enc.mb_h, qp, gop, (2 * gop) & 63, profile=profile,
+                                      speed="fast")
+        assert got == want, "%s qp %d frame %d: %d vs %d bytes" % (profile, qp, k, len(got), len(want))
+    pics = cd.parse_stream(stream, strict_end=True)
+    assert len(pics) == len(frames)
+    assert all(p["headers"][0]["disable_deblock"] == 1 for p in pics)
+
+
 def test_writer_refuses_what_it_cannot_write():
     from ragdoll_amd import RghError, write_idr_slice_cabac, write_p_slice_cabac
     from ragdoll_amd.abi import DECISION_DTYPE, LEVELS_PER_MB

@@ -141,3 +141,28 @@ def test_writer_matches_oracle_compact_slices(qp):
         assert split_annexb(nal) == nals[-1:], "qp %d frame %d" % (qp, k)
         types |= set(np.unique(ora.decisions()["mb_type"]).tolist())
     assert 5 in types and 1 in types
+
+
+@pytest.mark.parametrize("qp", [0, 8, 20, 30, 42])
+def test_writer_matches_oracle_fast_slices(qp):
+    # The Fast preset (enc.rs:503): half-pel vectors, Intra 16x16 only, the loop filter off in
+    # every slice header (disable_deblocking_filter_idc 1, also at QP > 0).
+    kw = dict(CONFIGS["cif"])
+    w, h = kw.pop("width"), kw.pop("height")
+    g = SceneGenerator(w, h, seed=config_seed(4, qp), **kw)
+    ora = OracleEncoder(w, h, qp, keyint=4, speed="fast")
+    types = set()
+    for k in range(6):
+        nals = split_annexb(ora.encode(*g.next_frame()))
+        if ora.last_was_idr:
+            assert split_annexb(write_headers(w, h, qp, speed="fast")) == nals[:2]
+            got = write_idr_slice_cavlc(ora.decisions(), ora.levels(), ora.mb_w, ora.mb_h, qp, k // 4, speed="fast")
+            assert split_annexb(got) == nals[2:], "qp %d frame %d" % (qp, k)
+            continue
+        gop = k % 4
+        nal = write_p_slice_cavlc(ora.decisions(), ora.levels(), ora.mb_w, ora.mb_h, qp, gop, (2 * gop) & 63,
+                                  speed="fast")
+        assert split_annexb(nal) == nals[-1:], "qp %d frame %d" % (qp, k)
+        types |= set(np.unique(ora.decisions()["mb_type"]).tolist())
+    assert 3 not in types  # no Intra 4x4 in Fast
+    assert 1 in types

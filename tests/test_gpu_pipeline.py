@@ -24,8 +24,8 @@ def scene(name, n, seed_extra=0):
     return w, h, [g.next_frame() for _ in range(n)]
 
 
-def oracle_run(frames, w, h, qp, profile, me_range=16):
-    ora = OracleEncoder(w, h, qp, profile=profile, keyint=1 << 20, me_range=me_range)
+def oracle_run(frames, w, h, qp, profile, me_range=16, speed="balanced"):
+    ora = OracleEncoder(w, h, qp, profile=profile, keyint=1 << 20, me_range=me_range, speed=speed)
     out = []
     for i, f in enumerate(frames):
         ora.encode(*f)
@@ -37,7 +37,7 @@ def oracle_run(frames, w, h, qp, profile, me_range=16):
 
 
 def run_pipeline(streams, lag, ring, calls=1, groups=0, label="", persistent=False):
-    """streams: list of (frames, w, h, qp, profile[, me_range]). Runs frames 1.. of every stream
+    """streams: list of (frames, w, h, qp, profile[, me_range[, speed]]). Runs frames 1.. of every stream
     through rgh_analyse_frames (split into `calls` consecutive calls; the per-step launch schedule,
     or one persistent launch per call) and checks them against the oracle."""
     hip = Hip()
@@ -46,8 +46,10 @@ def run_pipeline(streams, lag, ring, calls=1, groups=0, label="", persistent=Fal
         for st in streams:
             frames, w, h, qp, profile = st[:5]
             me_range = st[5] if len(st) > 5 else 16
-            idr, want, final = oracle_run(frames, w, h, qp, profile, me_range)
-            a = StreamAnalyzer(w, h, qp, profile=profile, me_range=me_range, keep_undeblocked=False, ring=ring)
+            speed = st[6] if len(st) > 6 else "balanced"
+            idr, want, final = oracle_run(frames, w, h, qp, profile, me_range, speed)
+            a = StreamAnalyzer(w, h, qp, profile=profile, me_range=me_range, keep_undeblocked=False, ring=ring,
+                               speed=speed)
             a.set_reference(*idr)
             n = a.mb_w * a.mb_h
             srcs, decs, lvs = [], [], []
