@@ -15,7 +15,9 @@ the whole job, IDR pictures included.
 The schedule's frame lag (wavefront steps between consecutive frames of a stream) and its number
 of concurrent groups (schedules on separate HIP streams) are picked by a small autotuner during
 warm-up unless --lag / --groups are given. The persistent schedule (one launch per job) is timed
-beside it on the same job (--compare-persistent 0 skips it).
+beside it on the same job (--compare-persistent 0 skips it), and so is the per-step schedule with
+each group's planned launches captured as one HIP graph and replayed (--compare-graph 0 skips
+it; --graph 1 makes it the timed form).
 
 Prints one JSON line (see the keys below). --gpus above 1 is refused: this build measures one
 GPU.
@@ -222,6 +224,10 @@ def main():
     ap.add_argument("--end-to-end", type=int, default=1, help="also time the step with its host transfers (1: yes)")
     ap.add_argument("--compare-persistent", type=int, default=1,
                     help="also time the persistent schedule (one launch per job) on the same job (1: yes)")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="issue the per-step schedule's planned launches as one replayed HIP graph (1: yes)")
+    ap.add_argument("--compare-graph", type=int, default=1,
+                    help="also time the graph form of the schedule on the same job (1: yes)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-proposal form")
     ap.add_argument("--learned-iters", type=int, default=2)
     ap.add_argument("--skip-gate", type=int, default=26)
@@ -405,11 +411,17 @@ def run_workload(gops, args):
     idr_mbs = sum(a.mb_w * a.mb_h for a in ans)
     step_bytes = total_mbs * MB_BYTES + idr_mbs * IDR_MB_BYTES
 
-    def step(sched, persistent=False):
+    cpu = [0.0]  # host CPU time of the process over the steps since the last reset
+
+    def step(sched, persistent=False, graph=None):
         lag, groups = sched
+        graph = bool(args.graph) if graph is None else graph
+        c = time.process_time()
         t = time.perf_counter()
-        analyse_frames(ans, srcs, decs, lvls, lag=lag, groups=groups, first_idr=True, persistent=persistent)
+        analyse_frames(ans, srcs, decs, lvls, lag=lag, groups=groups, first_idr=True, persistent=persistent,
+                       graph=graph and not persistent)
         el = time.perf_counter() - t
+        cpu[0] += time.process_time() - c
         ms, _, launches = ans[0].timing()
         return el, ms, launches
 
@@ -436,15 +448,28 @@ def run_workload(gops, args):
         persist = {"value": round(total_mbs * args.steps / el_p, 1), "ms_per_step": round(el_p / args.steps * 1e3, 3),
                    "lag": plag, "launches_per_step": launches_p / args.steps,
                    "tuning_s": {"lag %d" % k: round(v, 4) for k, v in pt.items()}}
+    # The planned launches of each schedule as one captured HIP graph, replayed: the first call
+    # captures, the timed ones replay.
+    graphed = None
+    if args.compare_graph and not args.graph:
+        step(sched, graph=True)
+        cpu[0] = 0.0
+        el_g = 0.0
+        for _ in range(args.steps):
+            el_g += step(sched, graph=True)[0]
+        graphed = {"value": round(total_mbs * args.steps / el_g, 1), "ms_per_step": round(el_g / args.steps * 1e3, 3),
+                   "host_cpu_ms_per_step": round(cpu[0] / args.steps * 1e3, 3)}
     for _ in range(args.warmup):
         step(sched)
     el = dev_ms = 0.0
     launches = 0
+    cpu[0] = 0.0
     for _ in range(args.steps):
         e, ms, nl = step(sched)
         el += e
         dev_ms += ms
         launches += nl
+    host_cpu_ms = cpu[0] / args.steps * 1e3
 
     value = total_mbs * args.steps / el
     e2e = end_to_end(hip, gops, ans, srcs, decs, lvls, uploaded, sched, total_mbs, args) if args.end_to_end else None
@@ -489,10 +514,12 @@ def run_workload(gops, args):
                    "form": "plain", "gops": len(gops), "p_macroblocks_per_step": total_mbs,
                    "idr_macroblocks_per_step": idr_mbs,
                    "all_macroblocks_per_s": round((total_mbs + idr_mbs) * args.steps / el, 1),
-                   "schedule": "per-step launches", "lag": sched[0], "groups": sched[1],
+                   "schedule": "per-step launches" + (" (one captured HIP graph)" if args.graph else ""), "lag": sched[0], "groups": sched[1],
                    "launches_per_step": launches / args.steps,
                    "schedule_tuning_s": {"%d/%d" % k: round(v, 4) for k, v in tune.items()},
                    "persistent_schedule": persist,
+                   "graph_schedule": graphed,
+                   "host_cpu_ms_per_step": round(host_cpu_ms, 3),
                    "device_ms_per_step": round(dev_ms / args.steps, 3),
                    "end_to_end": e2e},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",

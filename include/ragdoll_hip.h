@@ -233,6 +233,14 @@ rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src_d
  * schedule on the measured workloads (the waits and their cache invalidations cost more than the
  * launches they save), so it is not the default. */
 #define RGH_FRAMES_PERSISTENT 2
+/* rgh_analyse_frames: issue each schedule's planned launches as one HIP graph on its stream,
+ * captured on the first call and replayed by later calls whose launches are the same (same
+ * contexts and groups, frame counts, sizes and lag; the job descriptors are uploaded before each
+ * replay). The launches that finish deferred frames after the planned ones are issued one by one
+ * as always. Results are identical. Not with rgh_analyse_frames_noalloc
+ * (RGH_ERR_INVALID_ARGUMENT): the graph is allocated. A schedule's graph is kept by its first
+ * context until that context is destroyed or captures another. */
+#define RGH_FRAMES_GRAPH 4
 rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src_device,
                               rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag,
                               int32_t groups, int32_t flags);

@@ -241,6 +241,7 @@ def analyse_batch(analyzers, device_frames):
 
 FRAMES_FIRST_IDR = 1
 FRAMES_PERSISTENT = 2
+FRAMES_GRAPH = 4
 
 
 class RgLearnedConfig(ctypes.Structure):
@@ -249,7 +250,7 @@ class RgLearnedConfig(ctypes.Structure):
 
 
 def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0, first_idr=False, proposals=None,
-                   learned=None, noalloc=False, persistent=False):
+                   learned=None, noalloc=False, persistent=False, graph=False):
     """n_frames consecutive P frames of every analyzer in one pipelined schedule.
 
     device_frames[i][k]: (y_ptr, u_ptr, v_ptr, stride_y, stride_u, stride_v) of frame k of
@@ -258,7 +259,9 @@ def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0,
     records; None for an IDR frame 0) the search runs from learned proposals
     (rgh_analyse_frames_learned) with learned = dict(iters, skip_gate, intra_gate, fallback_pct).
     noalloc: the plain form through rgh_analyse_frames_noalloc (no allocation by the library).
-    persistent: one persistent launch (RGH_FRAMES_PERSISTENT) instead of the per-step launches."""
+    persistent: one persistent launch (RGH_FRAMES_PERSISTENT) instead of the per-step launches.
+    graph: the planned per-step launches as one captured HIP graph (RGH_FRAMES_GRAPH), replayed by
+    later calls with the same launches."""
     L = load_library()
     n = len(analyzers)
     nf = len(device_frames[0])
@@ -273,7 +276,8 @@ def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0,
             srcs[i * nf + k] = RgI420(f[0], f[1], f[2], f[3], f[4], f[5], a.width, a.height)
             dp[i * nf + k] = decisions[i][k]
             lp[i * nf + k] = levels[i][k]
-    flags = (FRAMES_FIRST_IDR if first_idr else 0) | (FRAMES_PERSISTENT if persistent else 0)
+    flags = ((FRAMES_FIRST_IDR if first_idr else 0) | (FRAMES_PERSISTENT if persistent else 0) |
+             (FRAMES_GRAPH if graph else 0))
     if proposals is None:
         entry = "rgh_analyse_frames_noalloc" if noalloc else "rgh_analyse_frames"
         _check(getattr(L, entry)(arr, n, nf, srcs, dp, lp, lag, groups, flags), entry)

@@ -2511,8 +2511,9 @@ struct StepLdsOf<2> {
     using T = CompactStepLds;
 };
 
-// One launch of a frame schedule. Block row blockIdx.y is frame job j0 + blockIdx.y; the frame's
-// current wavefront step is s = launch - base - delay. Blocks [0, na) analyse the macroblocks of
+// One launch of a frame schedule, sequence number seq0 + rseq (seq0 from the job descriptors).
+// Block row blockIdx.y is frame job j0 + blockIdx.y; the frame's current wavefront step is
+// s = launch - base - delay. Blocks [0, na) analyse the macroblocks of
 // step s; blocks [na, na + nd) run the deblock/pad items (x, y), 0 <= x <= mb_w, 0 <= y <= mb_h,
 // of step s - 4 (item (x, y) deblocks macroblock (x, y) when it exists and pads macroblock
 // (x - 1, y - 1)). The lag of 4 steps lets every analysis that reads undeblocked samples of
@@ -2535,8 +2536,9 @@ struct StepLdsOf<2> {
 // with learned proposals, 4 IDR and Balanced P, 5 IDR and Balanced P with learned proposals,
 // 6 Fast P, 7 IDR and Fast P, 8 Fast IDR.
 template <int MODE>
-__device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int j0, int launch, uint32_t seq) {
+__device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int j0, int launch, uint32_t rseq) {
     const RghJobD J = device_view(jobs[j0 + (int)blockIdx.y]);
+    const uint32_t seq = J.seq0 + rseq;  // this launch's sequence number
     RGH_G RghFrameCtl *ctl = J.ctl;
     const int lane = threadIdx.x;
     const int delay = ctl->delay[launch & 1];
@@ -2601,7 +2603,7 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
     }
 }
 
-extern "C" __global__ __launch_bounds__(64) void rgh_k_frame_step(const RghJob *__restrict__ jobs, int j0, int launch,
+extern "C" __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void rgh_k_frame_step(const RghJob *__restrict__ jobs, int j0, int launch,
                                                                    uint32_t seq) {
     frame_step<0>(jobs, j0, launch, seq);
 }
@@ -2611,7 +2613,7 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_intra_step(const RghJob *
     frame_step<1>(jobs, j0, launch, seq);
 }
 
-extern "C" __global__ __launch_bounds__(64) void rgh_k_mixed_step(const RghJob *__restrict__ jobs, int j0, int launch,
+extern "C" __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void rgh_k_mixed_step(const RghJob *__restrict__ jobs, int j0, int launch,
                                                                    uint32_t seq) {
     frame_step<4>(jobs, j0, launch, seq);
 }

@@ -62,6 +62,20 @@ void zero_bounds(int qp, bool fine, uint32_t *sad, uint32_t *ssd, uint32_t *prob
     if (probe) *probe = 16 * (((1u << shift) - bias) / (uint32_t)kQuant4Mf[qp % 6][0] + 1);
 }
 
+// One launch of a per-step schedule: kernel, grid (gx x gy workgroups of one wave), the group
+// whose stream runs it, and its arguments (job descriptors, first job, launch number L, which is
+// also its sequence number relative to the call's seq0).
+using StepKernel = void (*)(const RghJob *, int, int, uint32_t);
+struct LaunchRec {
+    StepKernel fn;
+    int gx, gy, group;
+    RghJob *jobs;
+    int j0, L;
+    bool operator==(const LaunchRec &o) const {
+        return fn == o.fn && gx == o.gx && gy == o.gy && group == o.group && jobs == o.jobs && j0 == o.j0 && L == o.L;
+    }
+};
+
 struct Picture {
     uint8_t *base[3] = {nullptr, nullptr, nullptr};  // allocation
     uint8_t *org[3] = {nullptr, nullptr, nullptr};   // sample (0, 0)
@@ -108,6 +122,11 @@ struct Ctx {
     hipEvent_t go = nullptr;                // start of a multi-group call (other groups wait on it)
     bool timing = false, timed = false;
     int timed_launches = 0;
+    // The planned launches of the last RGH_FRAMES_GRAPH schedule this context owned, captured,
+    // and the launches and stream they were captured from (replayed while both are unchanged).
+    hipGraphExec_t graph = nullptr;
+    std::vector<LaunchRec> graph_sig;
+    hipStream_t graph_stream = nullptr;
 };
 
 // Numeric settings read from the environment (test hooks and debugging), dflt when unset.
@@ -164,6 +183,7 @@ void free_ctx(Ctx *c) {
     for (auto &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->go) (void)hipEventDestroy(c->go);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -250,12 +270,18 @@ struct Schedule {
     RghFrameCtl *ctl_host = nullptr;
     int n = 0, n_frames = 0, lag = 1, nj = 0, n_intra = 0;
     bool compact = false, fast = false, learned = false;
+    uint32_t seq0 = 0;  // sequence number of the call's launch 0
+    // While the planned launches are listed, launch() appends them here (group `group`) instead
+    // of issuing them.
+    std::vector<LaunchRec> *out = nullptr;
+    int group = 0;
     bool recount = false;  // recount analysed macroblocks from their stamps (after a persistent launch)
     std::vector<RghJob> jobs;
     std::vector<int> span, last;
     int gx = 1, max_span = 0, planned = 0, L = 0, extra_total = 0;
 
-    rgh_status begin() {
+    // Checks the jobs and plans the launches (span of each frame, planned launch count).
+    rgh_status plan() {
         nj = n * n_frames;
         // IDR jobs lead (frame-major order, IDR only as a stream's first frame).
         n_intra = 0;
@@ -290,6 +316,11 @@ struct Schedule {
             gx = std::max(gx, std::min(mb_h, (mb_w + 1) / 2) + std::min(mb_h + 1, (mb_w + 2) / 2));
         }
         planned = (n_frames - 1) * lag + max_span;
+        return RGH_OK;
+    }
+    // Uploads the job descriptors (with the call's seq0) and clears the progress records.
+    rgh_status upload() {
+        for (auto &j : jobs) j.seq0 = seq0;
         HIP_TRY(hipMemcpyAsync(jobs_dev, jobs.data(), sizeof(RghJob) * (size_t)nj, hipMemcpyHostToDevice,
                                owner->stream));
         // The progress records start at zero (a fill kernel, not a memset node, so that the
@@ -302,28 +333,32 @@ struct Schedule {
         HIP_TRY(hipGetLastError());
         return RGH_OK;
     }
+    // One kernel of launch L over jobs [j0, j0 + gy): listed while the planned launches are
+    // listed (relative sequence number L), otherwise issued with a fresh sequence number, above
+    // every number of the planned launches.
+    void emit(StepKernel fn, int j0, int gy) {
+        if (out) {
+            out->push_back(LaunchRec{fn, gx, gy, group, jobs_dev, j0, L});
+            return;
+        }
+        hipLaunchKernelGGL(fn, dim3((unsigned)gx, (unsigned)gy), dim3(64), 0, owner->stream, jobs_dev, j0, L,
+                           g_seq.fetch_add(1) - seq0);
+    }
     void launch(int j_lo, int j_hi) {
-        const uint32_t seq = g_seq.fetch_add(1);
         // IDR jobs (the leading n_intra) and P jobs of the launch, in one sequence number: a
         // stamp of one is not final for the other within the launch.
         const int split = std::min(std::max(n_intra, j_lo), j_hi);
         if (split > j_lo && j_hi > split && !compact) {
             // IDR and Balanced P jobs together: one kernel, so that neither waits for the other.
-            hipLaunchKernelGGL(learned ? rgh_k_mixed_learned_step : (fast ? rgh_k_mixed_fast_step : rgh_k_mixed_step),
-                               dim3((unsigned)gx, (unsigned)(j_hi - j_lo)), dim3(64), 0, owner->stream, jobs_dev,
-                               j_lo, L, seq);
-            for (int j = j_lo; j < j_hi; j++) last[(size_t)j] = L;
-            L++;
-            return;
+            emit(learned ? rgh_k_mixed_learned_step : (fast ? rgh_k_mixed_fast_step : rgh_k_mixed_step), j_lo,
+                 j_hi - j_lo);
+        } else {
+            if (split > j_lo) emit(fast ? rgh_k_intra_fast_step : rgh_k_intra_step, j_lo, split - j_lo);
+            if (j_hi > split)
+                emit(learned ? rgh_k_learned_step
+                             : (compact ? rgh_k_compact_step : (fast ? rgh_k_fast_step : rgh_k_frame_step)),
+                     split, j_hi - split);
         }
-        if (split > j_lo)
-            hipLaunchKernelGGL(fast ? rgh_k_intra_fast_step : rgh_k_intra_step, dim3((unsigned)gx, (unsigned)(split - j_lo)),
-                               dim3(64), 0, owner->stream, jobs_dev, j_lo, L, seq);
-        if (j_hi > split)
-            hipLaunchKernelGGL(learned ? rgh_k_learned_step
-                                       : (compact ? rgh_k_compact_step : (fast ? rgh_k_fast_step : rgh_k_frame_step)),
-                               dim3((unsigned)gx, (unsigned)(j_hi - split)),
-                               dim3(64), 0, owner->stream, jobs_dev, split, L, seq);
         for (int j = j_lo; j < j_hi; j++) last[(size_t)j] = L;
         L++;
     }
@@ -370,19 +405,88 @@ struct Schedule {
     }
 };
 
+// Captures the listed launches of one schedule (on its owner's stream) into an executable graph,
+// or returns the graph captured last time from the same launches on the same stream. Kernel
+// nodes only: the descriptors and progress records are uploaded and cleared outside the graph
+// before each replay. Each schedule has a graph of its own on its own stream, so that the
+// schedules' launches run concurrently as the streams run them.
+rgh_status schedule_graph(Schedule &sc, const std::vector<LaunchRec> &recs, hipGraphExec_t *exec) {
+    Ctx *c = sc.owner;
+    hipStream_t st = c->stream;
+    if (c->graph && c->graph_sig == recs && c->graph_stream == st) {
+        *exec = c->graph;
+        return RGH_OK;
+    }
+    if (c->graph) {
+        HIP_TRY(hipGraphExecDestroy(c->graph));
+        c->graph = nullptr;
+    }
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < recs.size() && e == hipSuccess; i++) {
+        const LaunchRec &r = recs[i];
+        hipLaunchKernelGGL(r.fn, dim3((unsigned)r.gx, (unsigned)r.gy), dim3(64), 0, st, r.jobs, r.j0, r.L,
+                           (uint32_t)r.L);
+        e = hipGetLastError();
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ee = hipStreamEndCapture(st, &graph);
+    if (e == hipSuccess) e = ee;
+    if (e == hipSuccess) e = hipGraphInstantiate(&c->graph, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        c->graph = nullptr;
+        set_error("graph capture", hipGetErrorString(e), __LINE__);
+        return RGH_ERR_DEVICE;
+    }
+    c->graph_sig = recs;
+    c->graph_stream = st;
+    *exec = c->graph;
+    return RGH_OK;
+}
+
 // Runs schedules concurrently (their launches interleaved across their streams) and waits for
-// all of them. Timing, when enabled on `timer`, spans the whole run.
-rgh_status run_schedules(std::vector<Schedule> &sch, Ctx *timer, bool record = true) {
+// all of them. The planned launches take sequence numbers seq0 + L from one range reserved for
+// the call; with `graph` they are issued as one captured graph (replayed while the launch list
+// is unchanged), otherwise one by one. Timing, when enabled on `timer`, spans the whole run.
+rgh_status run_schedules(std::vector<Schedule> &sch, Ctx *timer, bool record = true, bool graph = false) {
     if (record && timer->timing) HIP_TRY(hipEventRecord(timer->ev[0], timer->stream));
+    int most = 0;
     for (auto &s : sch) {
-        rgh_status st = s.begin();
+        rgh_status st = s.plan();
+        if (st != RGH_OK) return st;
+        most = std::max(most, s.planned);
+    }
+    const uint32_t seq0 = g_seq.fetch_add((uint32_t)most + 1);
+    for (auto &s : sch) {
+        s.seq0 = seq0;
+        rgh_status st = s.upload();
         if (st != RGH_OK) return st;
     }
-    int most = 0;
-    for (auto &s : sch) most = std::max(most, s.planned);
+    std::vector<LaunchRec> recs;
+    for (size_t g = 0; g < sch.size(); g++) {
+        sch[g].out = &recs;
+        sch[g].group = (int)g;
+    }
     for (int L = 0; L < most; L++)
         for (auto &s : sch)
             if (L < s.planned) s.planned_launch();
+    for (auto &s : sch) s.out = nullptr;
+    if (graph) {
+        for (size_t g = 0; g < sch.size(); g++) {
+            std::vector<LaunchRec> mine;
+            for (const LaunchRec &r : recs)
+                if (r.group == (int)g) mine.push_back(r);
+            hipGraphExec_t exec = nullptr;
+            rgh_status st = schedule_graph(sch[g], mine, &exec);
+            if (st != RGH_OK) return st;
+            HIP_TRY(hipGraphLaunch(exec, sch[g].owner->stream));
+        }
+    } else {
+        for (const LaunchRec &r : recs)
+            hipLaunchKernelGGL(r.fn, dim3((unsigned)r.gx, (unsigned)r.gy), dim3(64), 0,
+                               sch[(size_t)r.group].owner->stream, r.jobs, r.j0, r.L, (uint32_t)r.L);
+    }
     HIP_TRY(hipGetLastError());
     if (env_u32("RGH_TEST_FAIL_AFTER_LAUNCH", 0)) {
         // Test hook: a failure once launches have started (their work completes first).
@@ -1061,8 +1165,12 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
                                rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag, int32_t groups,
                                int32_t flags, bool noalloc = false) {
     if (!ctxs || n <= 0 || n_frames <= 0 || !src || !decisions || !levels || lag < 0 || groups < 0 ||
-        (flags & ~(RGH_FRAMES_FIRST_IDR | RGH_FRAMES_PERSISTENT)))
+        (flags & ~(RGH_FRAMES_FIRST_IDR | RGH_FRAMES_PERSISTENT | RGH_FRAMES_GRAPH)))
         return RGH_ERR_INVALID_ARGUMENT;
+    if (noalloc && (flags & RGH_FRAMES_GRAPH)) {
+        set_error("rgh_analyse_frames_noalloc", "RGH_FRAMES_GRAPH allocates the graph", __LINE__);
+        return RGH_ERR_INVALID_ARGUMENT;
+    }
     const bool first_idr = (flags & RGH_FRAMES_FIRST_IDR) != 0;
     rgh_learned_config lc{4, 0, 0, 105, 256};
     if (props) {
@@ -1193,7 +1301,7 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
             HIP_TRY(hipEventRecord(cs[0]->go, cs[0]->stream));
             for (int g = 1; g < G; g++) HIP_TRY(hipStreamWaitEvent(sch[(size_t)g].owner->stream, cs[0]->go, 0));
         }
-        s = run_schedules(sch, cs[0]);
+        s = run_schedules(sch, cs[0], true, (flags & RGH_FRAMES_GRAPH) != 0);
     }
     if (s != RGH_OK) {
         for (int i = 0; i < n; i++) poison(cs[(size_t)i], n_frames);

@@ -36,7 +36,7 @@ def oracle_run(frames, w, h, qp, profile, me_range=16, speed="balanced"):
     return idr, out, ora.recon(deblocked=True)
 
 
-def run_pipeline(streams, lag, ring, calls=1, groups=0, label="", persistent=False):
+def run_pipeline(streams, lag, ring, calls=1, groups=0, label="", persistent=False, graph=False):
     """streams: list of (frames, w, h, qp, profile[, me_range[, speed]]). Runs frames 1.. of every stream
     through rgh_analyse_frames (split into `calls` consecutive calls; the per-step launch schedule,
     or one persistent launch per call) and checks them against the oracle."""
@@ -69,7 +69,7 @@ def run_pipeline(streams, lag, ring, calls=1, groups=0, label="", persistent=Fal
         for c in range(calls):
             sl = slice(cuts[c], cuts[c + 1])
             analyse_frames(ans, [r[2][sl] for r in refs], [r[3][sl] for r in refs], [r[4][sl] for r in refs], lag=lag,
-                           groups=groups, persistent=persistent)
+                           groups=groups, persistent=persistent, graph=graph)
         for s, (a, (want, final, srcs, decs, lvs, n)) in enumerate(zip(ans, refs)):
             for k in range(nf):
                 dec = hip.download(decs[k], np.zeros(n, DECISION_DTYPE))
@@ -124,6 +124,50 @@ def test_batched_uneven_streams():
         g = SceneGenerator(w, h, seed=seed, **kw)
         streams.append(([g.next_frame() for _ in range(4)], w, h, qp, "high", me_range))
     run_pipeline(streams, lag=12, ring=4, groups=2, label="batched")
+
+
+# RGH_FRAMES_GRAPH: the planned launches captured once and replayed. Eight P frames over four
+# calls of two frames each: the first call captures, the next three replay the graph (between
+# them each call uploads its descriptors, finishes deferred frames with launches of its own and
+# synchronises). Lag 1 defers most macroblocks, so completion launches follow every replay; two
+# groups capture across two streams. Every frame is checked against the oracle.
+@pytest.mark.parametrize("lag,ring,groups", [(20, 8, 1), (1, 4, 1), (6, 6, 2)])
+def test_graph_replays(lag, ring, groups):
+    w, h, cif = scene("cif", 9, 31)
+    fx = [fixture_frame(160, 90, i) for i in range(9)]
+    print("graph lag %d ring %d groups %d" % (lag, ring, groups), flush=True)
+    run_pipeline([(cif, w, h, 26, "high"), (fx, 160, 90, 20, "baseline")], lag, ring, calls=4, groups=groups,
+                 graph=True, label="graph lag %d ring %d groups %d" % (lag, ring, groups))
+
+
+def test_graph_recaptures_when_the_launches_change():
+    # Calls of 3, 2 and 3 frames: each differs from the graph kept by the call before it.
+    w, h, cif = scene("cif", 9, 32)
+    print("graph recapture", flush=True)
+    run_pipeline([(cif, w, h, 30, "main")], 8, 6, calls=3, graph=True, label="graph recapture")
+
+
+def test_graph_is_refused_by_the_noalloc_entry_point():
+    from ragdoll_amd.abi import RghError
+    w, h, frames = scene("cif", 2, 33)
+    idr, _, _ = oracle_run(frames[:1], w, h, 28, "baseline")
+    hip = Hip()
+    a = StreamAnalyzer(w, h, 28, keep_undeblocked=False, ring=4)
+    bufs = []
+    try:
+        a.set_reference(*idr)
+        n = a.mb_w * a.mb_h
+        y, u, v = frames[1]
+        planes = [hip.upload(p) for p in (y, u, v)]
+        d, l = hip.alloc(n * 32), hip.alloc(n * LEVELS_PER_MB * 2)
+        bufs += planes + [d, l]
+        src = tuple(planes) + (y.shape[1], u.shape[1], v.shape[1])
+        with pytest.raises(RghError, match="RGH_FRAMES_GRAPH"):
+            analyse_frames([a], [[src]], [[d]], [[l]], noalloc=True, graph=True)
+    finally:
+        a.close()
+        for b in bufs:
+            hip.free(b)
 
 
 # The persistent schedule (RGH_FRAMES_PERSISTENT): one launch whose items wait for each other.
