@@ -840,6 +840,7 @@ struct ora_enc {
      * picture (mvr), the Compact motion vector costs. */
     int compact, cqo;
     int fast; /* H264Speed::Fast (enc.rs:503): half-pel motion, intra 16x16 only, no loop filter */
+    int parts; /* Balanced with P partitions 16x8 / 8x16 / 8x8 (ora_set_partitions) */
     plane_t hh, hv, hc;
     int16_t (*mvr)[2], (*prev_mvr)[2];
     uint16_t *mv_costs_c;
@@ -913,7 +914,7 @@ ora_enc *ora_create(int width, int height, int fps_num, int fps_den, int qp, int
 /* H264Encoder::with_speed(Compact) (h264.rs:3066-3098): chroma QP offset 1; with CAVLC no B
  * pictures. Returns -1 for the CABAC profiles, whose Compact configuration codes B pictures. */
 int ora_set_compact(ora_enc *e, int on) {
-    if (on && (e->cabac || e->fast)) return -1;
+    if (on && (e->cabac || e->fast || e->parts)) return -1;
     e->compact = on != 0;
     e->cqo = on ? 1 : 0;
     if (on && !e->hh.buf) {
@@ -930,8 +931,16 @@ int ora_set_compact(ora_enc *e, int on) {
  * (enc.rs:3322, :3457), chroma from the luma mode in P (enc.rs:3605) and no loop filter
  * (enc.rs:2334). Not combinable with Compact. */
 int ora_set_fast(ora_enc *e, int on) {
-    if (on && e->compact) return -1;
+    if (on && (e->compact || e->parts)) return -1;
     e->fast = on != 0;
+    return 0;
+}
+
+/* The partition extension of the Balanced preset (16x8, 8x16 and 8x8 P partitions; see
+ * parts_analyse). Not combinable with Compact or Fast. */
+int ora_set_partitions(ora_enc *e, int on) {
+    if (on && (e->compact || e->fast)) return -1;
+    e->parts = on != 0;
     return 0;
 }
 
@@ -2394,6 +2403,331 @@ static void code_inter(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const 
     store_mb(e, mb_x, mb_y, py, pu, pv);
 }
 
+/* ------------------------------------------------------------------------------------------ */
+/* P partitions 16x8 / 8x16 / 8x8 on top of the Balanced analysis (ora_set_partitions).         */
+/* The reference's Balanced preset codes P_L0_16x16 only (enc.rs:3008-3191); its Compact preset */
+/* adds 8x8 partitions (enc/inter.rs:651-652). This extension is defined here, by the oracle:   */
+/* when the 16x16 inter cost exceeds PART_GATE lambda, each shape is searched partition by      */
+/* partition in decoding order: the ITU-T H.264 8.4.1.3 predictor (directional for 16x8 and     */
+/* 8x16), candidates [predictor, 16x16 vector] rounded to full samples, a small diamond of at   */
+/* most PART_ITERS steps inside +-PART_RANGE samples of the 16x16 full-sample vector (and the    */
+/* macroblock's vector bounds), then the half-sample and quarter-sample diamonds of enc.rs:1283  */
+/* (SAD of the spec luma MC, +-3 quarter samples around the full-sample result). A shape costs   */
+/* the 16x16 SATD of its assembled prediction plus lambda x (its mvd bits and mb_type bits:      */
+/* ue(1) 16x8, ue(2) 8x16, ue(3) + 4 x ue(0) 8x8); the cheapest of 16x16, 16x8, 8x16, 8x8 wins   */
+/* (strict <, in that order). Records use the quadrant layout of set_compact_record.            */
+/* ------------------------------------------------------------------------------------------ */
+#define PART_GATE 40
+#define PART_RANGE 8
+#define PART_ITERS 8
+static void fill_part(ora_enc *e, int gx, int gy, int w4, int h4, int8_t r, int mx, int my);
+static void set_compact_record(ora_mb_decision *d, const int q[4][2], const int16_t mvr[2]);
+
+/* shape 0 16x16, 1 16x8, 2 8x16, 3 8x8: partition p's rectangle in samples within the macroblock */
+static void part_rect(int shape, int p, int *bx, int *by, int *w, int *h) {
+    *bx = shape == 2 ? 8 * p : (shape == 3 ? 8 * (p & 1) : 0);
+    *by = shape == 1 ? 8 * p : (shape == 3 ? 8 * (p >> 1) : 0);
+    *w = shape <= 1 ? 16 : 8;
+    *h = (shape == 0 || shape == 2) ? 16 : 8;
+}
+static const int PART_COUNT[4] = {1, 2, 2, 4};
+static const uint32_t PART_TYPE_BITS[4] = {1, 3, 3, 7};
+static const uint8_t PART_MB_TYPE[4] = {1, 6, 7, 5};
+
+/* ITU-T H.264 8.4.1.3 (with 6.4.11.7 for the neighbours) for reference 0: the partition whose
+ * top-left 4x4 block is (gx, gy) (picture 4x4 units), pw blocks wide, partition `part` of
+ * `shape`; `done` marks the 4x4 blocks (bit 4 y + x) of this macroblock already decided. */
+static void pred_mv_shape(const ora_enc *e, int gx, int gy, int pw, int shape, int part, uint32_t done, int16_t mvp[2]) {
+    int gs = e->mb_w * 4, gh = e->mb_h * 4, mx0 = gx & ~3, my0 = gy & ~3;
+    /* availability and contents of the 4x4 block at picture position (x, y) */
+    int av[3], ref[3];
+    int16_t mv[3][2];
+    const int pos[4][2] = {{gx - 1, gy}, {gx, gy - 1}, {gx + pw, gy - 1}, {gx - 1, gy - 1}};
+    int cav = 0, cref = -1;
+    int16_t cmv[2] = {0, 0};
+    for (int n = 0; n < 4; n++) {
+        int x = pos[n][0], y = pos[n][1], a;
+        if (x < 0 || y < 0 || x >= gs || y >= gh)
+            a = 0;
+        else if (y >= my0 && y < my0 + 4 && x >= mx0 && x < mx0 + 4)
+            a = (int)((done >> (4 * (y - my0) + (x - mx0))) & 1); /* this macroblock: decided yet? */
+        else if (y >= my0 && x >= mx0)
+            a = 0; /* right of this macroblock in its rows (or below it): not decoded */
+        else
+            a = 1; /* a macroblock row above, or the macroblock to the left */
+        int r = -1;
+        int16_t v[2] = {0, 0};
+        if (a) {
+            long i = (long)y * gs + x;
+            r = e->ref_grid[i];
+            if (r == 0) {
+                v[0] = e->mv_grid[i][0];
+                v[1] = e->mv_grid[i][1];
+            } else {
+                r = -1;
+            }
+        }
+        if (n < 2) {
+            av[n] = a;
+            ref[n] = r;
+            mv[n][0] = v[0];
+            mv[n][1] = v[1];
+        } else if (n == 2) {
+            cav = a;
+            cref = r;
+            cmv[0] = v[0];
+            cmv[1] = v[1];
+        } else if (!cav) { /* 8.4.1.3.2: C replaced by D */
+            cav = a;
+            cref = r;
+            cmv[0] = v[0];
+            cmv[1] = v[1];
+        }
+    }
+    av[2] = cav;
+    ref[2] = cref;
+    mv[2][0] = cmv[0];
+    mv[2][1] = cmv[1];
+    if (!av[1] && !av[2] && av[0]) { /* 8.4.1.3.1: B and C from A */
+        for (int n = 1; n < 3; n++) {
+            ref[n] = ref[0];
+            mv[n][0] = mv[0][0];
+            mv[n][1] = mv[0][1];
+        }
+    }
+    int dir = -1;
+    if (shape == 1) dir = part == 0 ? 1 : 0;
+    if (shape == 2) dir = part == 0 ? 0 : 2;
+    if (dir >= 0 && ref[dir] == 0) {
+        mvp[0] = mv[dir][0];
+        mvp[1] = mv[dir][1];
+        return;
+    }
+    int n0 = (ref[0] == 0) + (ref[1] == 0) + (ref[2] == 0);
+    if (n0 == 1) {
+        int k = ref[0] == 0 ? 0 : (ref[1] == 0 ? 1 : 2);
+        mvp[0] = mv[k][0];
+        mvp[1] = mv[k][1];
+        return;
+    }
+    mvp[0] = median3(mv[0][0], mv[1][0], mv[2][0]);
+    mvp[1] = median3(mv[0][1], mv[1][1], mv[2][1]);
+}
+
+/* One partition's search (see above). src: the macroblock's luma, stride 16; (bx, by, w, h): the
+ * partition; F: the 16x16 full-sample vector; pred: the macroblock's prediction (stride 16), whose
+ * partition area receives the result's prediction. Returns the SAD cost. */
+static uint32_t part_search(ora_enc *e, const uint8_t *src, int x0, int y0, const mvb_t *b, int bx, int by, int w,
+                            int h, const int16_t mvp[2], const int16_t mv16[2], const int F[2], uint32_t lambda,
+                            int16_t out[2], uint8_t *pred) {
+    const plane_t *r = &e->refp.y;
+    int lo[2], hi[2];
+    for (int k = 0; k < 2; k++) {
+        lo[k] = imax((b->min[k] + 3) >> 2, F[k] - PART_RANGE);
+        hi[k] = imin(b->max[k] >> 2, F[k] + PART_RANGE);
+    }
+#define PCOST(fx, fy)                                                                                                 \
+    (sad_wh(src + by * 16 + bx, 16, r->buf + pat(r, x0 + bx + (fx), y0 + by + (fy)), r->stride, w, h) +            \
+     mvd_cost(e, lambda, 4 * (fx)-mvp[0]) + mvd_cost(e, lambda, 4 * (fy)-mvp[1]))
+    int cand[2][2];
+    const int16_t *cs[2] = {mvp, mv16};
+    for (int k = 0; k < 2; k++) {
+        cand[k][0] = iclamp((cs[k][0] + 2) >> 2, lo[0], hi[0]);
+        cand[k][1] = iclamp((cs[k][1] + 2) >> 2, lo[1], hi[1]);
+    }
+    int best[2] = {cand[0][0], cand[0][1]};
+    uint32_t best_cost = PCOST(best[0], best[1]);
+    if (cand[1][0] != cand[0][0] || cand[1][1] != cand[0][1]) {
+        uint32_t c = PCOST(cand[1][0], cand[1][1]);
+        if (c < best_cost) {
+            best_cost = c;
+            best[0] = cand[1][0];
+            best[1] = cand[1][1];
+        }
+    }
+    for (int it = 0; it < PART_ITERS; it++) {
+        int pts[4][2] = {{best[0], best[1] - 1}, {best[0] - 1, best[1]}, {best[0] + 1, best[1]}, {best[0], best[1] + 1}};
+        int moved = -1;
+        for (int k = 0; k < 4; k++) {
+            if (pts[k][0] < lo[0] || pts[k][0] > hi[0] || pts[k][1] < lo[1] || pts[k][1] > hi[1]) continue;
+            uint32_t c = PCOST(pts[k][0], pts[k][1]);
+            if (c < best_cost) {
+                best_cost = c;
+                moved = k;
+            }
+        }
+        if (moved < 0) break;
+        best[0] = pts[moved][0];
+        best[1] = pts[moved][1];
+    }
+#undef PCOST
+    /* half-sample then quarter-sample diamonds around the full-sample result */
+    int cx = best[0] * 4, cy = best[1] * 4, bm[2] = {cx, cy};
+    uint8_t p[256];
+    for (int pass = 0; pass < 2; pass++) {
+        int step = pass == 0 ? 2 : 1, iters = pass == 0 ? 1 : 2;
+        for (int it = 0; it < iters; it++) {
+            int from[2] = {bm[0], bm[1]};
+            int ds[4][2] = {{0, -step}, {-step, 0}, {step, 0}, {0, step}};
+            for (int k = 0; k < 4; k++) {
+                int mx = from[0] + ds[k][0], my = from[1] + ds[k][1];
+                if (iabs(mx - cx) > 3 || iabs(my - cy) > 3 || !mvb_contains(b, mx, my)) continue;
+                luma_mc(p, 16, r->buf + pat(r, x0 + bx + (mx >> 2), y0 + by + (my >> 2)), r->stride, w, h, mx, my);
+                uint32_t c = sad_wh(src + by * 16 + bx, 16, p, 16, w, h) + mvd_cost(e, lambda, mx - mvp[0]) +
+                             mvd_cost(e, lambda, my - mvp[1]);
+                if (c < best_cost) {
+                    best_cost = c;
+                    bm[0] = mx;
+                    bm[1] = my;
+                }
+            }
+            if (bm[0] == from[0] && bm[1] == from[1]) break;
+        }
+    }
+    luma_mc(pred + by * 16 + bx, 16, r->buf + pat(r, x0 + bx + (bm[0] >> 2), y0 + by + (bm[1] >> 2)), r->stride, w, h,
+            bm[0], bm[1]);
+    out[0] = (int16_t)bm[0];
+    out[1] = (int16_t)bm[1];
+    return best_cost;
+}
+
+/* The 16x8, 8x16 and 8x8 shapes of one macroblock against the 16x16 result (cost16, its
+ * prediction in py). On return py holds the winner's prediction, q its quadrant vectors (raster),
+ * *cost its cost; returns the winning shape. The macroblock's motion grid holds the last shape
+ * searched (the caller codes the winner over it). */
+static int parts_analyse(ora_enc *e, int mb_x, int mb_y, const mbsrc_t *s, const mvb_t *b, uint32_t lambda,
+                         const int F[2], const int16_t mv16[2], uint8_t *py, int q[4][2], uint32_t *cost) {
+    int gx0 = mb_x * 4, gy0 = mb_y * 4, x0 = mb_x * 16, y0 = mb_y * 16, win = 0;
+    for (int k = 0; k < 4; k++) {
+        q[k][0] = mv16[0];
+        q[k][1] = mv16[1];
+    }
+    for (int shape = 1; shape < 4; shape++) {
+        uint8_t pred[256];
+        int qs[4][2];
+        uint32_t bits = 0, done = 0;
+        for (int p = 0; p < PART_COUNT[shape]; p++) {
+            int bx, by, w, h;
+            part_rect(shape, p, &bx, &by, &w, &h);
+            int gx = gx0 + bx / 4, gy = gy0 + by / 4;
+            int16_t mvp[2], mv[2];
+            pred_mv_shape(e, gx, gy, w / 4, shape, p, done, mvp);
+            part_search(e, s->y, x0, y0, b, bx, by, w, h, mvp, mv16, F, lambda, mv, pred);
+            fill_part(e, gx, gy, w / 4, h / 4, 0, mv[0], mv[1]);
+            for (int yy = by / 4; yy < (by + h) / 4; yy++)
+                for (int xx = bx / 4; xx < (bx + w) / 4; xx++) done |= 1u << (4 * yy + xx);
+            bits += se_bits(mv[0] - mvp[0]) + se_bits(mv[1] - mvp[1]);
+            for (int k = 0; k < 4; k++) {
+                int kx = (k & 1) * 8, ky = (k >> 1) * 8;
+                if (kx >= bx && kx < bx + w && ky >= by && ky < by + h) {
+                    qs[k][0] = mv[0];
+                    qs[k][1] = mv[1];
+                }
+            }
+        }
+        uint32_t c = satd16(s->y, 16, pred, 16) + lambda * (bits + PART_TYPE_BITS[shape]);
+        if (c < *cost) {
+            *cost = c;
+            win = shape;
+            memcpy(py, pred, 256);
+            memcpy(q, qs, sizeof(qs));
+        }
+    }
+    return win;
+}
+
+/* Chroma prediction of the four quadrants, each 4x4 block at its own vector (enc/inter.rs:229
+ * predict_inter's chroma; the 1/8-sample bilinear filter). */
+static void chroma_quads(const ora_enc *e, int mb_x, int mb_y, const int q[4][2], uint8_t *pu, uint8_t *pv) {
+    for (int k = 0; k < 4; k++) {
+        int bx = (k & 1) * 4, by = (k >> 1) * 4;
+        int cx = mb_x * 8 + bx, cy = mb_y * 8 + by, fx = q[k][0] & 7, fy = q[k][1] & 7;
+        int wa = (8 - fx) * (8 - fy), wb = fx * (8 - fy), wc = (8 - fx) * fy, wd = fx * fy;
+        for (int pl = 0; pl < 2; pl++) {
+            const plane_t *cp = pl ? &e->refp.v : &e->refp.u;
+            uint8_t *dst = (pl ? pv : pu) + by * 8 + bx;
+            long off = pat(cp, cx + (q[k][0] >> 3), cy + (q[k][1] >> 3));
+            for (int r = 0; r < 4; r++)
+                for (int c = 0; c < 4; c++) {
+                    const uint8_t *s0 = cp->buf + off + (long)r * cp->stride + c, *s1 = s0 + cp->stride;
+                    dst[r * 8 + c] = (uint8_t)((wa * s0[0] + wb * s0[1] + wc * s1[0] + wd * s1[1] + 32) >> 6);
+                }
+        }
+    }
+}
+
+/* code_inter for a macroblock of `shape` (CAVLC syntax inline; CABAC from the records later):
+ * mb_type ue(0..3), four sub_mb_type ue(0) for P_8x8, each partition's mvd against its 8.4.1.3
+ * predictor from the grid as it fills, then the residual exactly as code_inter. */
+static void code_inter_shape(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, int shape, const int q[4][2], int cbpl,
+                             int cbpc, int16_t q4[16][16], int16_t q8[4][64], int use8, const chroma_levels_t *c,
+                             uint8_t *py, uint8_t *pu, uint8_t *pv) {
+    int idx = mb_y * e->mb_w + mb_x, gx0 = mb_x * 4, gy0 = mb_y * 4;
+    int cbp = cbpl | (cbpc << 4);
+    int qp_delta = cbp ? qp - e->cur_qp : 0, eff_qp = iclamp(e->cur_qp + qp_delta, 0, 51);
+    int eff_qpc = chroma_qp(eff_qp);
+    if (use8 && cbpl) e->mb_8x8[idx] = 1;
+    if (w) {
+        put_ue(w, (uint64_t)shape);
+        if (shape == 3)
+            for (int k = 0; k < 4; k++) put_ue(w, 0);
+    }
+    uint32_t done = 0;
+    for (int p = 0; p < PART_COUNT[shape]; p++) {
+        int bx, by, pw, ph;
+        part_rect(shape, p, &bx, &by, &pw, &ph);
+        int k = (bx >> 3) + (by >> 3) * 2;
+        int16_t mvp[2];
+        pred_mv_shape(e, gx0 + bx / 4, gy0 + by / 4, pw / 4, shape, p, done, mvp);
+        if (w) {
+            put_se(w, q[k][0] - mvp[0]);
+            put_se(w, q[k][1] - mvp[1]);
+        }
+        fill_part(e, gx0 + bx / 4, gy0 + by / 4, pw / 4, ph / 4, 0, q[k][0], q[k][1]);
+        for (int yy = by / 4; yy < (by + ph) / 4; yy++)
+            for (int xx = bx / 4; xx < (bx + pw) / 4; xx++) done |= 1u << (4 * yy + xx);
+    }
+    if (e->cabac) {
+        cabac_luma_nnz(e, mb_x, mb_y, cbpl, cbpc, q4, q8, use8);
+    } else {
+        if (w) put_ue(w, inter_cbp_golomb(cbp));
+        if (cbp) {
+            if (w) {
+                if (e->t8x8) put32(w, 0, 1);
+                put_se(w, qp_delta);
+            }
+            write_cavlc_residuals(e, w, mb_x, mb_y, cbpl, cbpc, q4, c);
+        }
+    }
+    e->mb_cbp_luma[idx] = (uint8_t)cbpl;
+    e->mb_cbp_chroma[idx] = (uint8_t)cbpc;
+    e->cur_qp = eff_qp;
+    e->mb_qps[idx] = (uint8_t)eff_qp;
+    if (use8) {
+        for (int i8 = 0; i8 < 4; i8++)
+            if (cbpl & (1 << i8)) recon_8x8(py + (i8 >> 1) * 128 + (i8 & 1) * 8, q8[i8], eff_qp);
+    } else {
+        recon_luma_4x4(py, q4, cbpl, eff_qp);
+    }
+    recon_chroma(pu, pv, c, cbpc, eff_qpc);
+    store_mb(e, mb_x, mb_y, py, pu, pv);
+}
+
+/* The record of a P-picture macroblock with partitions on: words 2-5 the quadrant vectors from
+ * the motion grid (zero for intra), word 6 zero. */
+static void parts_record(ora_enc *e, int mb_x, int mb_y) {
+    ora_mb_decision *d = &e->dec[mb_y * e->mb_w + mb_x];
+    int gs = e->mb_w * 4, q[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+    int16_t zero[2] = {0, 0};
+    if (d->mb_type <= 1 || d->mb_type >= 5)
+        for (int k = 0; k < 4; k++) {
+            long g = (long)(mb_y * 4 + (k >> 1) * 2) * gs + mb_x * 4 + (k & 1) * 2;
+            q[k][0] = e->mv_grid[g][0];
+            q[k][1] = e->mv_grid[g][1];
+        }
+    set_compact_record(d, q, zero);
+}
+
 /* enc.rs:3011 code_p_mb */
 static void code_p_mb(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const mbsrc_t *s) {
     int mb_w = e->mb_w, gs = mb_w * 4, gx0 = mb_x * 4, gy0 = mb_y * 4, x0 = mb_x * 16, y0 = mb_y * 16;
@@ -2490,6 +2824,11 @@ static void code_p_mb(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const m
         d->n_satd++;
     }
     uint32_t inter_cost = inter_satd + lambda * (se_bits(mv[0] - mvp[0]) + se_bits(mv[1] - mvp[1]) + 1);
+    /* Partitions (ora_set_partitions): the 16x8, 8x16 and 8x8 shapes against the 16x16 cost. */
+    int shape = 0, q[4][2];
+    if (e->parts && inter_cost > PART_GATE * lambda) {
+        shape = parts_analyse(e, mb_x, mb_y, s, &b, lambda, full, mv, py, q, &inter_cost);
+    }
     if (inter_satd > 24 * lambda) {
         uint32_t i16_satd;
         int i16_mode;
@@ -2501,7 +2840,10 @@ static void code_p_mb(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const m
             return;
         }
     }
-    mc_chroma_mb(&e->refp, mb_x * 8, mb_y * 8, mv, pu, pv);
+    if (shape)
+        chroma_quads(e, mb_x, mb_y, q, pu, pv);
+    else
+        mc_chroma_mb(&e->refp, mb_x * 8, mb_y * 8, mv, pu, pv);
     int16_t q4[16][16], q8[4][64];
     int cbpl = quant_luma_4x4_inter(s->y, py, qp, q4, 0);
     int use8 = 0;
@@ -2522,15 +2864,24 @@ static void code_p_mb(ora_enc *e, bits_t *w, int mb_x, int mb_y, int qp, const m
     int cbpc = chroma_cbp(&c);
     d->mv[0] = mv[0];
     d->mv[1] = mv[1];
-    if (cbpl == 0 && cbpc == 0 && mv[0] == pskip[0] && mv[1] == pskip[1]) {
+    if (shape == 0 && cbpl == 0 && cbpc == 0 && mv[0] == pskip[0] && mv[1] == pskip[1]) {
         code_skip(e, mb_x, mb_y, mv);
         store_mb(e, mb_x, mb_y, py, pu, pv);
         d->mb_type = 0;
         return;
     }
     begin_coded_mb(e, w);
-    code_inter(e, w, mb_x, mb_y, qp, mv, mvp, cbpl, cbpc, q4, q8, use8, &c, py, pu, pv);
-    d->mb_type = 1;
+    if (e->parts) {
+        if (shape == 0)
+            for (int k = 0; k < 4; k++) {
+                q[k][0] = mv[0];
+                q[k][1] = mv[1];
+            }
+        code_inter_shape(e, w, mb_x, mb_y, qp, shape, q, cbpl, cbpc, q4, q8, use8, &c, py, pu, pv);
+    } else {
+        code_inter(e, w, mb_x, mb_y, qp, mv, mvp, cbpl, cbpc, q4, q8, use8, &c, py, pu, pv);
+    }
+    d->mb_type = PART_MB_TYPE[shape];
     d->cbp_luma = (uint8_t)cbpl;
     d->cbp_chroma = (uint8_t)cbpc;
     d->transform_8x8 = (uint8_t)(use8 && cbpl);
@@ -3779,7 +4130,7 @@ static void cabac_slice(ora_enc *e, bits_t *w, int is_idr, int slice_qp) {
             if (d->mb_type == 0) {
                 s.skip[m] = 1;
                 prev_dqp = 0;
-            } else if (d->mb_type == 1) { /* P_L0_16x16 (enc.rs:3196 code_inter) */
+            } else if (d->mb_type == 1 && !e->parts) { /* P_L0_16x16 (enc.rs:3196 code_inter) */
                 cabac_bin(&c, 14, 0);
                 cabac_bin(&c, 15, 0);
                 cabac_bin(&c, 16, 0);
@@ -3796,6 +4147,41 @@ static void cabac_slice(ora_enc *e, bits_t *w, int is_idr, int slice_qp) {
                         s.mvd[(gy + by) * gs + gx + bx][0] = (int16_t)mvd[0];
                         s.mvd[(gy + by) * gs + gx + bx][1] = (int16_t)mvd[1];
                     }
+            } else if (d->mb_type == 1 || d->mb_type >= 5) {
+                /* Partitions on: P_L0_16x16, P_L0_L0_16x8, P_L0_L0_8x16 or P_8x8 (four P_L0_8x8);
+                 * mb_type bins of Table 9-37 (ctxIdx 14-17), sub_mb_type "1" (ctxIdx 21), then
+                 * each partition's mvd against its 8.4.1.3 predictor over the final motion grid. */
+                int shape = d->mb_type == 1 ? 0 : (d->mb_type == 6 ? 1 : (d->mb_type == 7 ? 2 : 3));
+                cabac_bin(&c, 14, 0);
+                cabac_bin(&c, 15, shape == 1 || shape == 2);
+                if (shape == 1 || shape == 2)
+                    cabac_bin(&c, 17, shape == 1);
+                else
+                    cabac_bin(&c, 16, shape == 3);
+                if (shape == 3)
+                    for (int k = 0; k < 4; k++) cabac_bin(&c, 21, 1);
+                uint32_t done = 0;
+                for (int p = 0; p < PART_COUNT[shape]; p++) {
+                    int bx, by, pw, ph;
+                    part_rect(shape, p, &bx, &by, &pw, &ph);
+                    int gx = mb_x * 4 + bx / 4, gy = mb_y * 4 + by / 4;
+                    int16_t mvp[2];
+                    pred_mv_shape(e, gx, gy, pw / 4, shape, p, done, mvp);
+                    int mvd[2] = {e->mv_grid[gy * gs + gx][0] - mvp[0], e->mv_grid[gy * gs + gx][1] - mvp[1]};
+                    for (int k = 0; k < 2; k++) {
+                        int amvd = (gx > 0 ? iabs(s.mvd[gy * gs + gx - 1][k]) : 0) + (gy > 0 ? iabs(s.mvd[(gy - 1) * gs + gx][k]) : 0);
+                        cabac_mvd(&c, k, amvd, mvd[k]);
+                    }
+                    for (int yy = 0; yy < ph / 4; yy++)
+                        for (int xx = 0; xx < pw / 4; xx++) {
+                            s.mvd[(gy + yy) * gs + gx + xx][0] = (int16_t)mvd[0];
+                            s.mvd[(gy + yy) * gs + gx + xx][1] = (int16_t)mvd[1];
+                            done |= 1u << (4 * (by / 4 + yy) + bx / 4 + xx);
+                        }
+                }
+                int use8 = d->transform_8x8 && d->cbp_luma;
+                s.t8[m] = (uint8_t)use8;
+                cabac_cbp_residual(&c, &s, mb_x, mb_y, 0, e->t8x8, d->cbp_luma, d->cbp_chroma, use8, lv, &prev_dqp);
             } else if (d->mb_type == 2) { /* I_16x16 (enc.rs:3587 code_i16) */
                 int t = 1 + d->i16_mode + 4 * d->cbp_chroma + (d->cbp_luma ? 12 : 0);
                 if (is_idr) {
@@ -3936,8 +4322,10 @@ long ora_encode_frame(ora_enc *e, const uint8_t *y, const uint8_t *u, const uint
                 code_intra(e, wp, mb_x, mb_y, slice_qp, &s);
             else if (e->compact)
                 code_p_mb_compact(e, wp, mb_x, mb_y, slice_qp, &s);
-            else
+            else {
                 code_p_mb(e, wp, mb_x, mb_y, slice_qp, &s);
+                if (e->parts) parts_record(e, mb_x, mb_y);
+            }
             uint16_t nnz = 0;
             for (int by = 0; by < 4; by++)
                 for (int bx = 0; bx < 4; bx++)
@@ -3982,7 +4370,7 @@ long ora_encode_frame(ora_enc *e, const uint8_t *y, const uint8_t *u, const uint
     copy_plane(&e->cur.u, e->pre_u);
     copy_plane(&e->cur.v, e->pre_v);
     if (slice_qp != 0 && !e->fast) {
-        if (e->compact)
+        if (e->compact || e->parts)
             deblock_parts(e);
         else
             deblock(e);

@@ -36,7 +36,8 @@ extern "C" {
 
 /* Per-macroblock decision record, 32 bytes (layout shared with the product's rgh_mb_decision). */
 typedef struct ora_mb_decision {
-    uint8_t mb_type;      /* 0 P_Skip, 1 P_L0_16x16, 2 I_16x16, 3 I_4x4, 4 I_PCM, 5 P_8x8 */
+    uint8_t mb_type;      /* 0 P_Skip, 1 P_L0_16x16, 2 I_16x16, 3 I_4x4, 4 I_PCM, 5 P_8x8,
+                             6 P_L0_L0_16x8, 7 P_L0_L0_8x16 */
     uint8_t i16_mode;     /* I_16x16 luma mode (0 V, 1 H, 2 DC, 3 plane) */
     uint8_t chroma_mode;  /* intra chroma mode (0 DC, 1 H, 2 V, 3 plane) */
     uint8_t cbp_luma;
@@ -76,6 +77,12 @@ int ora_set_compact(ora_enc *e, int on);
  * without the plane mode, the Fast skip-probe and intra gates, no loop filter. Returns -1 with
  * Compact. */
 int ora_set_fast(ora_enc *e, int on);
+/* P partitions on top of the Balanced preset: when a macroblock's 16x16 inter cost exceeds 40
+ * lambda, 16x8, 8x16 and 8x8 partitions are searched around the 16x16 vector and the cheapest
+ * shape by SATD + lambda x bits is coded (an extension the reference does not define; see the
+ * oracle source). P-picture records then hold the four quadrant vectors in words 2-5 (the Compact
+ * layout; word 6 zero). Returns -1 with Compact or Fast. */
+int ora_set_partitions(ora_enc *e, int on);
 /* Encodes one I420 frame. Writes the Annex-B access unit (SPS+PPS before an IDR slice) to out
  * and returns its length; returns -1 on a too small buffer. For CABAC profiles no bitstream is
  * written (returns 0) but every decision, level and reconstruction is produced. */

@@ -920,3 +920,76 @@ def parse_stream(data, stop_after=None, strict_end=True):
             raise ParseError("a picture's slices cover %d of %d macroblocks"
                              % (len(pic["mbs"]), pic["pic"].mb_w * pic["pic"].mb_h))
     return pics
+
+
+def derive_motion(pic):
+    """Luma motion vectors of every 4x4 block of a parsed P picture that uses one reference
+    picture, from the decoded mvds (8.4.1): P_Skip by 8.4.1.1, every partition by 8.4.1.3 (the
+    median rule, the directional rules of 16x8 and 8x16 partitions, B and C taken from A when only
+    A is available), with the neighbouring partitions A, B, C (else D) of 6.4.11.7, a neighbour
+    being available when it lies in the picture and is already decoded. Returns a list over the 4x4
+    blocks in raster order: (mvx, mvy) in quarter samples, or None for intra blocks. Sub-macroblock
+    partitions smaller than 8x8 are not derived (NotImplementedError)."""
+    P = pic["pic"]
+    w4, h4 = 4 * P.mb_w, 4 * P.mb_h
+    mv = [None] * (w4 * h4)
+    done = [False] * (w4 * h4)
+
+    def nbr(x, y):  # (available, refIdx, mv)
+        if x < 0 or y < 0 or x >= w4 or y >= h4 or not done[y * w4 + x]:
+            return (False, -1, (0, 0))
+        v = mv[y * w4 + x]
+        return (True, -1, (0, 0)) if v is None else (True, 0, v)
+
+    def med(a, b, c):
+        return max(min(a, b), min(max(a, b), c))
+
+    def mvp(x, y, pw, shape, part):
+        A, B, C = nbr(x - 1, y), nbr(x, y - 1), nbr(x + pw, y - 1)
+        if not C[0]:
+            C = nbr(x - 1, y - 1)
+        if not B[0] and not C[0] and A[0]:
+            B = C = A
+        if shape == "P16x8" and part == 0 and B[1] == 0:
+            return B[2]
+        if shape == "P16x8" and part == 1 and A[1] == 0:
+            return A[2]
+        if shape == "P8x16" and part == 0 and A[1] == 0:
+            return A[2]
+        if shape == "P8x16" and part == 1 and C[1] == 0:
+            return C[2]
+        hits = [n for n in (A, B, C) if n[1] == 0]
+        if len(hits) == 1:
+            return hits[0][2]
+        return (med(A[2][0], B[2][0], C[2][0]), med(A[2][1], B[2][1], C[2][1]))
+
+    def fill(x, y, bw, bh, v):
+        for yy in range(y, y + bh):
+            for xx in range(x, x + bw):
+                mv[yy * w4 + xx] = v
+                done[yy * w4 + xx] = True
+
+    for rec in pic["mbs"]:
+        m = rec["addr"]
+        gx, gy = 4 * (m % P.mb_w), 4 * (m // P.mb_w)
+        t = rec["type"]
+        if rec["intra"]:
+            fill(gx, gy, 4, 4, None)
+        elif t == "skip":
+            A, B = nbr(gx - 1, gy), nbr(gx, gy - 1)
+            if not A[0] or not B[0] or (A[1] == 0 and A[2] == (0, 0)) or (B[1] == 0 and B[2] == (0, 0)):
+                v = (0, 0)
+            else:
+                v = mvp(gx, gy, 4, "P16x16", 0)
+            fill(gx, gy, 4, 4, v)
+        else:
+            if t == "P8x8":
+                if any(s != 0 for s in rec["sub"]):
+                    raise NotImplementedError("sub-macroblock partitions below 8x8")
+                parts = [(8 * (k & 1), 8 * (k >> 1), 8, 8) for k in range(4)]
+            else:
+                parts = P_PARTS[["P16x16", "P16x8", "P8x16"].index(t)]
+            for k, ((px, py, pw, ph), d) in enumerate(zip(parts, rec["mvd"])):
+                p = mvp(gx + px // 4, gy + py // 4, pw // 4, t, k)
+                fill(gx + px // 4, gy + py // 4, pw // 4, ph // 4, (p[0] + d[0], p[1] + d[1]))
+    return mv

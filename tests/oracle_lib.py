@@ -38,6 +38,8 @@ def lib():
         L.ora_set_compact.argtypes = [P, I]
         L.ora_set_fast.restype = I
         L.ora_set_fast.argtypes = [P, I]
+        L.ora_set_partitions.restype = I
+        L.ora_set_partitions.argtypes = [P, I]
         L.ora_encode_frame.restype = L64
         L.ora_encode_frame.argtypes = [P, P, P, P, I, I, I, P, L64]
         L.ora_headers.restype = L64
@@ -90,6 +92,8 @@ class OracleEncoder:
             raise ValueError("the Compact preset codes B pictures with CABAC")
         if speed == "fast" and self.L.ora_set_fast(self.h, 1) != 0:
             raise ValueError("the Fast preset does not combine with Compact")
+        if speed == "partitions" and self.L.ora_set_partitions(self.h, 1) != 0:
+            raise ValueError("partitions extend the Balanced preset only")
         self.width, self.height, self.qp = width, height, qp
         self.mb_w = self.L.ora_mb_w(self.h)
         self.mb_h = self.L.ora_mb_h(self.h)

@@ -175,6 +175,101 @@ def test_oracle_cabac_stream_parses_to_its_records(profile, qp):
 
 
 # ---------------------------------------------------------------------------------------------
+# The partition extension (16x8 / 8x16 / 8x8 P partitions; records in the quadrant layout)
+# ---------------------------------------------------------------------------------------------
+PART_KIND = {"skip": 0, "P16x16": 1, "I16": 2, "I_NxN": 3, "P8x8": 5, "P16x8": 6, "P8x16": 7}
+
+
+def quad_vectors(d):
+    """The four quadrant vectors (raster order) of a partition-mode record (words 2-5)."""
+    return np.frombuffer(d.tobytes()[8:24], np.int16).reshape(4, 2)
+
+
+def check_parts_picture(pic, dec, lev, label):
+    """Types, coded block patterns, transform flags and levels of every macroblock as parsed,
+    and every P-picture vector as the parser derives it (8.4.1) from the parsed mvds."""
+    mbs = pic["mbs"]
+    assert len(mbs) == len(dec), label
+    mvs = cd.derive_motion(pic) if pic["type"] == 0 else None
+    w4 = 4 * pic["pic"].mb_w
+    for i, (m, d) in enumerate(zip(mbs, dec)):
+        where = "%s MB %d" % (label, i)
+        kind = PART_KIND[m["type"]]
+        assert kind == d["mb_type"], where
+        if kind == 5:
+            assert m["sub"] == [0, 0, 0, 0], where
+        if mvs is not None and kind in (0, 1, 5, 6, 7):
+            gx, gy = 4 * (i % pic["pic"].mb_w), 4 * (i // pic["pic"].mb_w)
+            got = [mvs[(gy + 2 * (k >> 1)) * w4 + gx + 2 * (k & 1)] for k in range(4)]
+            assert [tuple(int(c) for c in v) for v in quad_vectors(d)] == got, where
+        if kind == 0:
+            continue
+        assert m["dqp"] == 0, where
+        assert m["cbp_luma"] == d["cbp_luma"] and m["cbp_chroma"] == d["cbp_chroma"], where
+        assert m["t8"] == (1 if (d["transform_8x8"] and d["cbp_luma"]) else 0), where
+        if kind in (2, 3):
+            assert m["chroma_mode"] == d["chroma_mode"], where
+        if kind == 2:
+            assert m["i16_mode"] == d["i16_mode"], where
+        assert np.array_equal(level_record(m), lev[i]), where + ": levels differ"
+
+
+def test_derive_motion_follows_the_standard_on_known_cases():
+    # A hand-built picture of 2x2 macroblocks: the directional rules and the P_Skip zero rule.
+    pic = {"type": 0, "pic": cd.Picture(2, 2), "mbs": [
+        {"addr": 0, "type": "P16x8", "intra": False, "mvd": [(8, 4), (-3, 2)]},
+        {"addr": 1, "type": "P8x16", "intra": False, "mvd": [(1, 1), (2, -2)]},
+        {"addr": 2, "type": "skip", "intra": False},
+        {"addr": 3, "type": "P8x8", "intra": False, "sub": [0, 0, 0, 0], "mvd": [(0, 0), (4, 0), (0, 4), (1, 1)]},
+    ]}
+    mv = cd.derive_motion(pic)
+    at = lambda x, y: mv[y * 8 + x]
+    # MB 0 upper half: no neighbours at all -> median of zeros.
+    assert at(0, 0) == (8, 4)
+    # MB 0 lower half: A unavailable, B = the upper half (C unavailable, D unavailable) -> only B
+    # matches -> B = (8, 4).
+    assert at(0, 2) == (5, 6)
+    # MB 1 left half (8x16 part 0): A = MB 0's upper right block (8, 4), directional -> A.
+    assert at(4, 0) == (9, 5)
+    # MB 1 right half (8x16 part 1): C = (16, -1) outside, D = (3, -1) outside (picture top);
+    # B outside; A = MB 1's left half (9, 5) -> B and C from A -> directional C = (9, 5).
+    assert at(6, 0) == (11, 3)
+    # MB 2 (skip): A unavailable (picture edge) -> zero vector.
+    assert at(0, 4) == (0, 0)
+    # MB 3 quadrant 0: A = MB 2 (0, 0), B = MB 1 lower-left (9, 5), C = MB 1 lower-right
+    # block 2 (11, 3)? C = (16 + 8, 15) -> x 6 in 4x4 units at row 3: MB 1's right half (11, 3)
+    # -> median((0,0), (9,5), (11,3)) = (9, 3).
+    assert at(4, 4) == (9, 3)
+    # quadrant 1: A = quadrant 0 (9, 3), B = MB 1 (11, 3), C outside the picture -> D = MB 1's
+    # block at x 5 = (9, 5) -> median = (9, 3) -> + (4, 0).
+    assert at(6, 4) == (13, 3)
+    # quadrant 2: A = MB 2 (0, 0), B = quadrant 0 (9, 3), C = quadrant 1 (13, 3) -> (9, 3).
+    assert at(4, 6) == (9, 7)
+    # quadrant 3: A = quadrant 2 (9, 7), B = quadrant 1 (13, 3), C unavailable -> D = quadrant 0
+    # (9, 3) -> median = (9, 3) -> + (1, 1).
+    assert at(6, 6) == (10, 4)
+
+
+@pytest.mark.parametrize("profile", ["main", "high"])
+@pytest.mark.parametrize("qp", [14, 26, 38])
+def test_oracle_partition_stream_parses_to_its_records(profile, qp):
+    w, h = 208, 144
+    frames = scene(w, h, 5, 1200 + qp)
+    enc = OracleEncoder(w, h, qp, profile=profile, keyint=4, speed="partitions", me_range=32)
+    stream = b""
+    records = []
+    for f in frames:
+        stream += enc.encode(*f)
+        records.append((enc.decisions(), enc.levels()))
+    pics = cd.parse_stream(stream, strict_end=True)
+    assert [p["type"] for p in pics] == [2, 0, 0, 0, 2]
+    for k, (pic, (dec, lev)) in enumerate(zip(pics, records)):
+        check_parts_picture(pic, dec, lev, "%s qp %d frame %d" % (profile, qp, k))
+    kinds = set(int(t) for dec, _ in records[1:4] for t in dec["mb_type"])
+    assert {5, 6, 7} <= kinds, kinds
+
+
+# ---------------------------------------------------------------------------------------------
 # The product's writer (rgh_write_p_slice_cabac / rgh_write_idr_slice_cabac) against the oracle
 # ---------------------------------------------------------------------------------------------
 @pytest.mark.parametrize("profile", ["main", "high"])
