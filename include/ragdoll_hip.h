@@ -79,6 +79,19 @@ typedef struct rgh_config {
  * no intra 4x4, chroma from the luma mode in P pictures, and no loop filter (enc.rs:503, :2270,
  * :3050-3152, :3322, :3605). Not combinable with RGH_FLAG_COMPACT or learned proposals. */
 #define RGH_FLAG_FAST 4
+/* rgh_config.flags: P partitions on top of the Balanced preset (rgh_speed RGH_SPEED_PARTITIONS),
+ * an extension the reference does not define (its Balanced preset codes 16x16 partitions only,
+ * enc.rs:3008-3191): when a P macroblock's 16x16 inter cost (SATD + lambda x bits) exceeds 40
+ * lambda, the 16x8, 8x16 and 8x8 shapes are searched partition by partition (the 8.4.1.3
+ * predictor, directional for 16x8 / 8x16; candidates [predictor, 16x16 vector]; a diamond of at
+ * most 8 steps within +-8 samples of the 16x16 full-sample vector; half- and quarter-sample
+ * diamonds) and the cheapest shape by the 16x16 SATD of its prediction + lambda x (mvd and
+ * mb_type bits) is coded. Every P-picture record then holds its four 8x8 quadrant vectors in
+ * words 2-5 (mv, mvp, cost, n_sad/n_subpel; zero for intra) and zero in word 6, and mb_type may be
+ * RGH_MB_P_8X8, RGH_MB_P_L0_L0_16X8 or RGH_MB_P_L0_L0_8X16. Any profile; not combinable with
+ * RGH_FLAG_COMPACT, RGH_FLAG_FAST or learned proposals; rgh_analyse_frames runs it on the
+ * per-step schedule (RGH_FRAMES_PERSISTENT is ignored for such contexts). */
+#define RGH_FLAG_PARTITIONS 8
 
 /* An I420 picture: three planes with their strides; width/height are display sizes.
  * Chroma planes are (width+1)/2 x (height+1)/2. */
@@ -110,7 +123,9 @@ enum {
     RGH_MB_I16X16 = 2,
     RGH_MB_I4X4 = 3,
     RGH_MB_I_PCM = 4,
-    RGH_MB_P_8X8 = 5 /* four P_L0_8x8 partitions (Compact preset) */
+    RGH_MB_P_8X8 = 5,        /* four P_L0_8x8 partitions (Compact preset, RGH_FLAG_PARTITIONS) */
+    RGH_MB_P_L0_L0_16X8 = 6, /* two 16x8 partitions (RGH_FLAG_PARTITIONS) */
+    RGH_MB_P_L0_L0_8X16 = 7  /* two 8x16 partitions (RGH_FLAG_PARTITIONS) */
 };
 
 /* Encoder presets (H264Speed, h264.rs). */
@@ -119,8 +134,10 @@ typedef enum rgh_speed {
     RGH_SPEED_COMPACT = 1,  /* hexagon search over half-sample planes, 8x8 partitions, I_4x4 in P,
                                fine inter rounding, chroma QP offset 1 (enc/inter.rs, enc/compact.rs);
                                Constrained Baseline only (with CABAC the preset codes B pictures) */
-    RGH_SPEED_FAST = 2      /* Balanced records with the Fast decisions; slices without the loop
+    RGH_SPEED_FAST = 2,     /* Balanced records with the Fast decisions; slices without the loop
                                filter (disable_deblocking_filter_idc 1, enc.rs:2334) */
+    RGH_SPEED_PARTITIONS = 3 /* Balanced with P partitions (RGH_FLAG_PARTITIONS): records in the
+                                quadrant layout */
 } rgh_speed;
 
 /* Fixed 32-byte decision record per macroblock, raster order. */

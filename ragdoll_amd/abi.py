@@ -8,10 +8,11 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libragdoll_hip.so")
 LEVELS_PER_MB = 408
 PROFILES = {"baseline": 0, "main": 1, "high": 2}
-SPEEDS = {"balanced": 0, "compact": 1, "fast": 2}
+SPEEDS = {"balanced": 0, "compact": 1, "fast": 2, "partitions": 3}
 FLAG_NO_UNDEBLOCKED_COPY = 1
 FLAG_COMPACT = 2
 FLAG_FAST = 4
+FLAG_PARTITIONS = 8
 
 DECISION_DTYPE = np.dtype([
     ("mb_type", "u1"), ("i16_mode", "u1"), ("chroma_mode", "u1"), ("cbp_luma", "u1"),
@@ -152,7 +153,7 @@ class StreamAnalyzer:
                  keep_undeblocked=True, ring=0, speed="balanced"):
         self.L = load_library()
         flags = ((0 if keep_undeblocked else FLAG_NO_UNDEBLOCKED_COPY) | (FLAG_COMPACT if speed == "compact" else 0) |
-                 (FLAG_FAST if speed == "fast" else 0))
+                 (FLAG_FAST if speed == "fast" else 0) | (FLAG_PARTITIONS if speed == "partitions" else 0))
         cfg = RgConfig(width, height, PROFILES[profile], qp, me_range, qpel_iters, device, flags, ring)
         h = ctypes.c_void_p()
         _check(self.L.rgh_create(ctypes.byref(cfg), ctypes.byref(h)), "rgh_create")

@@ -16,6 +16,7 @@
 #include "../../include/ragdoll_hip.h"
 #include "rgh_bits.h"
 #include "rgh_cabac_tables.h"
+#include "rgh_motion.h"
 
 namespace {
 
@@ -362,7 +363,8 @@ rgh_status check_common(const rgh_slice_config *cfg, int32_t profile, const void
     if (cfg->qp < 1 || cfg->qp > 51) return RGH_ERR_INVALID_ARGUMENT;
     if (profile != RGH_PROFILE_MAIN && profile != RGH_PROFILE_HIGH) return RGH_ERR_INVALID_ARGUMENT;
     // Compact with CABAC codes B pictures (h264.rs:3075), not written here.
-    if (cfg->speed != RGH_SPEED_BALANCED && cfg->speed != RGH_SPEED_FAST) return RGH_ERR_UNSUPPORTED;
+    if (cfg->speed != RGH_SPEED_BALANCED && cfg->speed != RGH_SPEED_FAST && cfg->speed != RGH_SPEED_PARTITIONS)
+        return RGH_ERR_UNSUPPORTED;
     return RGH_OK;
 }
 
@@ -389,17 +391,18 @@ extern "C" rgh_status rgh_write_p_slice_cabac(const rgh_slice_config *cfg, int32
     if (st != RGH_OK) return st;
     if (cfg->frame_num < 0 || cfg->poc_lsb < 0) return RGH_ERR_INVALID_ARGUMENT;
     const int mw = cfg->mb_w, mh = cfg->mb_h;
-    const bool high = profile == RGH_PROFILE_HIGH;
+    const bool high = profile == RGH_PROFILE_HIGH, parts = cfg->speed == RGH_SPEED_PARTITIONS;
     // Validate every record before writing: the writer has no way back.
     for (int i = 0; i < mw * mh; i++) {
         const rgh_mb_decision &d = decisions[i];
         if (d.mb_type == RGH_MB_P_SKIP) continue;
         if (d.qp != cfg->qp) return RGH_ERR_UNSUPPORTED;
-        if (d.mb_type != RGH_MB_P_L0_16X16 && d.mb_type != RGH_MB_I16X16) return RGH_ERR_UNSUPPORTED;
+        const bool inter = d.mb_type == RGH_MB_P_L0_16X16 || (parts && rgh_motion::shape_of(d.mb_type) > 0);
+        if (!inter && d.mb_type != RGH_MB_I16X16) return RGH_ERR_UNSUPPORTED;
         if (d.cbp_chroma > 2 || (d.cbp_luma & ~15) || d.chroma_mode > 3) return RGH_ERR_INVALID_ARGUMENT;
         if (d.mb_type == RGH_MB_I16X16 && (d.i16_mode > 3 || (d.cbp_luma != 0 && d.cbp_luma != 15)))
             return RGH_ERR_INVALID_ARGUMENT;
-        if (d.transform_8x8 && (!high || d.mb_type != RGH_MB_P_L0_16X16)) return RGH_ERR_INVALID_ARGUMENT;
+        if (d.transform_8x8 && (!high || !inter)) return RGH_ERR_INVALID_ARGUMENT;
     }
     BitWriter w;
     // first_mb_in_slice, slice_type P, pps id, frame_num (4 bits), pic_order_cnt_lsb (6 bits),
@@ -415,6 +418,7 @@ extern "C" rgh_status rgh_write_p_slice_cabac(const rgh_slice_config *cfg, int32
     std::vector<uint8_t> &rbsp = w.bytes();
     Arith a(rbsp, cfg->qp, 1);
     Neighbours nb(mw, mh);
+    rgh_motion::MotionGrid mg(parts ? 4 * mw : 0, parts ? 4 * mh : 0);
     int dqp_nz = 0;
     const size_t gs = (size_t)4 * mw;
     for (int mb_y = 0; mb_y < mh; mb_y++)
@@ -424,9 +428,52 @@ extern "C" rgh_status rgh_write_p_slice_cabac(const rgh_slice_config *cfg, int32
             const int16_t *lv = levels + (size_t)m * RGH_LEVELS_PER_MB;
             const int ca = mb_x > 0 && !nb.skip[(size_t)m - 1], cb = mb_y > 0 && !nb.skip[(size_t)(m - mw)];
             a.bin(11 + ca + cb, d.mb_type == RGH_MB_P_SKIP);  // mb_skip_flag
+            const int shape = rgh_motion::shape_of(d.mb_type);
+            int q[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
+            if (parts) {
+                // Records in the quadrant layout; partitions fill their blocks as they are coded.
+                if (d.mb_type != RGH_MB_I16X16) rgh_motion::quads_of(d, q);
+                if (shape <= 0) mg.fill(4 * mb_x, 4 * mb_y, 4, 4, d.mb_type == RGH_MB_I16X16 ? -1 : 0, q[0][0], q[0][1]);
+            }
             if (d.mb_type == RGH_MB_P_SKIP) {
                 nb.skip[(size_t)m] = 1;
                 dqp_nz = 0;
+            } else if (parts && shape >= 0) {
+                // mb_type (Table 9-37: P_L0_16x16 0 0 0, P_L0_L0_16x8 0 1 1, P_L0_L0_8x16 0 1 0,
+                // P_8x8 0 0 1; ctxIdx 14, 15, then 16 or 17 after a 1), sub_mb_type P_L0_8x8 "1"
+                // (ctxIdx 21) four times, then each partition's mvd against its 8.4.1.3 predictor.
+                a.bin(14, 0);
+                const bool two = shape == rgh_motion::S16x8 || shape == rgh_motion::S8x16;
+                a.bin(15, two);
+                if (two)
+                    a.bin(17, shape == rgh_motion::S16x8);
+                else
+                    a.bin(16, shape == rgh_motion::S8x8);
+                if (shape == rgh_motion::S8x8)
+                    for (int k = 0; k < 4; k++) a.bin(21, 1);
+                for (int p = 0; p < rgh_motion::part_count(shape); p++) {
+                    int bx, by, pw, ph;
+                    rgh_motion::part_rect(shape, p, bx, by, pw, ph);
+                    const int gx = 4 * mb_x + bx / 4, gy = 4 * mb_y + by / 4, k = (bx >> 3) + (by >> 3) * 2;
+                    int mvp[2];
+                    mg.pred(gx, gy, pw / 4, shape, p, mvp);
+                    const int mvd[2] = {q[k][0] - mvp[0], q[k][1] - mvp[1]};
+                    for (int c = 0; c < 2; c++) {
+                        int sum = 0;
+                        if (gx > 0) sum += absi(nb.mvd[((size_t)gy * gs + gx - 1) * 2 + c]);
+                        if (gy > 0) sum += absi(nb.mvd[((size_t)(gy - 1) * gs + gx) * 2 + c]);
+                        mvd_component(a, c, sum, mvd[c]);
+                    }
+                    for (int y = 0; y < ph / 4; y++)
+                        for (int x = 0; x < pw / 4; x++) {
+                            nb.mvd[((size_t)(gy + y) * gs + gx + x) * 2] = (int16_t)mvd[0];
+                            nb.mvd[((size_t)(gy + y) * gs + gx + x) * 2 + 1] = (int16_t)mvd[1];
+                        }
+                    if (shape != rgh_motion::S16x16) mg.fill(gx, gy, pw / 4, ph / 4, 0, q[k][0], q[k][1]);
+                }
+                const bool use8 = d.transform_8x8 && d.cbp_luma;
+                nb.t8[(size_t)m] = use8;
+                cbp_and_residual(a, nb, mb_x, mb_y, 0, high, d.cbp_luma, d.cbp_chroma, use8, lv, dqp_nz);
             } else if (d.mb_type == RGH_MB_P_L0_16X16) {
                 a.bin(14, 0);  // P_L0_16x16: 0 0 0
                 a.bin(15, 0);

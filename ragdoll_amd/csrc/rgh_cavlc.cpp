@@ -16,11 +16,13 @@
 #include "../../include/ragdoll_hip.h"
 #include "rgh_bits.h"
 #include "rgh_cavlc_tables.h"
+#include "rgh_motion.h"
 
 namespace {
 
 using namespace rgh_cavlc;
 using namespace rgh_bits;
+using namespace rgh_motion;
 
 int absi(int v) { return v < 0 ? -v : v; }
 
@@ -191,72 +193,6 @@ void write_i16_body(BitWriter &w, NnzGrid &lg, NnzGrid *cg, int mb_x, int mb_y, 
     write_chroma(w, cg, mb_x, mb_y, d.cbp_chroma, lv);
 }
 
-// Motion of the picture's 4x4 blocks as coded so far (reference index: -1 intra, 0 list 0), for
-// the motion vector predictors of Compact records: h264.rs:3577 pred_motion_partition /
-// enc/inter.rs:104 neighbours and :143 mvp_from, reference 0, directional rule 0 (16x16 and
-// 8x8 partitions).
-struct MotionGrid {
-    int gs, gh;
-    std::vector<int8_t> ref;
-    std::vector<int16_t> mv;  // x, y per block
-    MotionGrid(int w4, int h4) : gs(w4), gh(h4), ref((size_t)w4 * h4, -2), mv((size_t)w4 * h4 * 2, 0) {}
-    void fill(int gx, int gy, int w4, int h4, int r, int mx, int my) {
-        for (int y = gy; y < gy + h4; y++)
-            for (int x = gx; x < gx + w4; x++) {
-                const size_t i = (size_t)y * gs + x;
-                ref[i] = (int8_t)r;
-                mv[2 * i] = (int16_t)mx;
-                mv[2 * i + 1] = (int16_t)my;
-            }
-    }
-    static int median3(int a, int b, int c) {
-        const int mx = a > b ? a : b, mn = a < b ? a : b, t = mn > c ? mn : c;
-        return mx < t ? mx : t;
-    }
-    void pred(int gx, int gy, int w4, int out[2]) const {
-        int ra = -2, rb = -2, rc = -2, a[2] = {0, 0}, b[2] = {0, 0}, c[2] = {0, 0};
-        auto at = [&](int x, int y, int &r, int *v) {
-            const size_t i = (size_t)y * gs + x;
-            r = ref[i];
-            v[0] = mv[2 * i];
-            v[1] = mv[2 * i + 1];
-        };
-        if (gx > 0) at(gx - 1, gy, ra, a);
-        if (gy > 0) at(gx, gy - 1, rb, b);
-        const int bx = gx & 3, by = gy & 3, trx = gx + w4;
-        const bool tr = gy > 0 && trx < gs &&
-                        !((by == 1 && (bx + w4 == 2 || bx + w4 == 4)) || (by == 2 && bx + w4 == 4) ||
-                          (by == 3 && (bx + w4 == 2 || bx + w4 == 4)));
-        if (tr && ref[(size_t)(gy - 1) * gs + trx] != -2)
-            at(trx, gy - 1, rc, c);
-        else if (gy > 0 && gx > 0)
-            at(gx - 1, gy - 1, rc, c);
-        const int match = (ra == 0) + (rb == 0) + (rc == 0);
-        if (match == 1) {
-            const int *v = ra == 0 ? a : (rb == 0 ? b : c);
-            out[0] = v[0];
-            out[1] = v[1];
-        } else if (match == 0 && rb == -2 && rc == -2 && ra != -2) {
-            out[0] = a[0];
-            out[1] = a[1];
-        } else {
-            out[0] = median3(a[0], b[0], c[0]);
-            out[1] = median3(a[1], b[1], c[1]);
-        }
-    }
-};
-
-// The quadrant vectors of a Compact record (raster order).
-void compact_quads(const rgh_mb_decision &d, int q[4][2]) {
-    const uint8_t *p = (const uint8_t *)&d + 8;
-    for (int k = 0; k < 4; k++) {
-        int16_t v[2];
-        memcpy(v, p + 4 * k, 4);
-        q[k][0] = v[0];
-        q[k][1] = v[1];
-    }
-}
-
 // An I_4x4 macroblock (enc.rs:3441 code_i4) after its mb_type: the 16 prediction modes as
 // predicted-mode flags and 3-bit remainders (8.3.1.1: the smaller of the left and top modes, DC
 // at the picture's edges; `modes` holds the coded modes, DC elsewhere), the chroma mode, the
@@ -301,9 +237,10 @@ extern "C" rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const
     const int mb_w = cfg->mb_w, mb_h = cfg->mb_h;
     if (mb_w <= 0 || mb_h <= 0 || mb_w > 512 || mb_h > 512) return RGH_ERR_INVALID_DIMENSIONS;
     if (cfg->qp < 0 || cfg->qp > 51 || cfg->frame_num < 0 || cfg->poc_lsb < 0 ||
-        (cfg->speed != RGH_SPEED_BALANCED && cfg->speed != RGH_SPEED_COMPACT && cfg->speed != RGH_SPEED_FAST))
+        (cfg->speed < RGH_SPEED_BALANCED || cfg->speed > RGH_SPEED_PARTITIONS))
         return RGH_ERR_INVALID_ARGUMENT;
-    const bool compact = cfg->speed == RGH_SPEED_COMPACT;
+    const bool compact = cfg->speed == RGH_SPEED_COMPACT, parts = cfg->speed == RGH_SPEED_PARTITIONS;
+    const bool quad = compact || parts;  // records in the quadrant layout
     BitWriter w;
     // Slice header (h264.rs:1071 SliceHeader::write): first_mb_in_slice, slice_type P (5),
     // pic_parameter_set_id, frame_num (4 bits), pic_order_cnt_lsb (6 bits),
@@ -328,7 +265,7 @@ extern "C" rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const
     }
     NnzGrid lg(4 * mb_w, 4 * mb_h);
     NnzGrid cg[2] = {NnzGrid(2 * mb_w, 2 * mb_h), NnzGrid(2 * mb_w, 2 * mb_h)};
-    MotionGrid mg(compact ? 4 * mb_w : 0, compact ? 4 * mb_h : 0);
+    MotionGrid mg(quad ? 4 * mb_w : 0, quad ? 4 * mb_h : 0);
     std::vector<uint8_t> modes((size_t)16 * mb_w * mb_h, 2);
     uint32_t skip_run = 0;
     for (int mb_y = 0; mb_y < mb_h; mb_y++)
@@ -337,10 +274,11 @@ extern "C" rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const
             const rgh_mb_decision &d = decisions[idx];
             const int16_t *lv = levels + (size_t)idx * RGH_LEVELS_PER_MB;
             int q[4][2] = {{0, 0}, {0, 0}, {0, 0}, {0, 0}};
-            if (compact) {
+            const int shape = shape_of(d.mb_type);
+            if (quad) {
                 const bool intra = d.mb_type == RGH_MB_I16X16 || d.mb_type == RGH_MB_I4X4;
-                if (!intra) compact_quads(d, q);
-                if (d.mb_type != RGH_MB_P_8X8)  // the partitions fill their blocks as they are coded
+                if (!intra) quads_of(d, q);
+                if (shape <= S16x16)  // partitions fill their blocks as they are coded
                     mg.fill(4 * mb_x, 4 * mb_y, 4, 4, intra ? -1 : 0, q[0][0], q[0][1]);
             }
             if (d.mb_type == RGH_MB_P_SKIP) {
@@ -352,30 +290,28 @@ extern "C" rgh_status rgh_write_p_slice_cavlc(const rgh_slice_config *cfg, const
             if (cbpc > 2 || (cbpl & ~15)) return RGH_ERR_INVALID_ARGUMENT;
             w.ue(skip_run);
             skip_run = 0;
-            if (d.mb_type == RGH_MB_P_L0_16X16 || (compact && d.mb_type == RGH_MB_P_8X8)) {
+            if (d.mb_type == RGH_MB_P_L0_16X16 || (compact && d.mb_type == RGH_MB_P_8X8) || (parts && shape >= 0)) {
                 const int cbp = cbpl | (cbpc << 4);
-                if (!compact) {
+                if (!quad) {
                     w.ue(0);
                     w.se(d.mv[0] - d.mvp[0]);
                     w.se(d.mv[1] - d.mvp[1]);
-                } else if (d.mb_type == RGH_MB_P_L0_16X16) {
-                    int mvp[2];
-                    mg.pred(4 * mb_x, 4 * mb_y, 4, mvp);
-                    w.ue(0);
-                    w.se(q[0][0] - mvp[0]);
-                    w.se(q[0][1] - mvp[1]);
                 } else {
-                    // P_8x8 of four P_L0_8x8 (enc/inter.rs:331-358): each partition's predictor
-                    // sees the partitions coded before it.
-                    w.ue(3);
-                    for (int k = 0; k < 4; k++) w.ue(0);
-                    for (int k = 0; k < 4; k++) {
-                        const int gx = 4 * mb_x + (k & 1) * 2, gy = 4 * mb_y + (k >> 1) * 2;
+                    // mb_type (P_L0_16x16 0, P_L0_L0_16x8 1, P_L0_L0_8x16 2, P_8x8 3 with four
+                    // P_L0_8x8 sub-macroblocks: enc/inter.rs:331-358), then each partition's mvd
+                    // against its predictor, which sees the partitions coded before it.
+                    w.ue((uint32_t)shape);
+                    if (shape == S8x8)
+                        for (int k = 0; k < 4; k++) w.ue(0);
+                    for (int p = 0; p < part_count(shape); p++) {
+                        int bx, by, pw, ph;
+                        part_rect(shape, p, bx, by, pw, ph);
+                        const int gx = 4 * mb_x + bx / 4, gy = 4 * mb_y + by / 4, k = (bx >> 3) + (by >> 3) * 2;
                         int mvp[2];
-                        mg.pred(gx, gy, 2, mvp);
+                        mg.pred(gx, gy, pw / 4, shape, p, mvp);
                         w.se(q[k][0] - mvp[0]);
                         w.se(q[k][1] - mvp[1]);
-                        mg.fill(gx, gy, 2, 2, 0, q[k][0], q[k][1]);
+                        if (shape != S16x16) mg.fill(gx, gy, pw / 4, ph / 4, 0, q[k][0], q[k][1]);
                     }
                 }
                 w.ue(inter_cbp_code(cbp));
@@ -411,7 +347,7 @@ extern "C" rgh_status rgh_write_idr_slice_cavlc(const rgh_slice_config *cfg, int
     const int mb_w = cfg->mb_w, mb_h = cfg->mb_h;
     if (mb_w <= 0 || mb_h <= 0 || mb_w > 512 || mb_h > 512) return RGH_ERR_INVALID_DIMENSIONS;
     if (cfg->qp < 0 || cfg->qp > 51 ||
-        (cfg->speed != RGH_SPEED_BALANCED && cfg->speed != RGH_SPEED_COMPACT && cfg->speed != RGH_SPEED_FAST))
+        (cfg->speed < RGH_SPEED_BALANCED || cfg->speed > RGH_SPEED_PARTITIONS))
         return RGH_ERR_INVALID_ARGUMENT;
     BitWriter w;
     // Slice header of an IDR picture: first_mb_in_slice, slice_type I (7), pps id, frame_num 0,
@@ -468,7 +404,7 @@ extern "C" rgh_status rgh_write_headers(const rgh_stream_config *cfg, uint8_t *o
         return RGH_ERR_INVALID_DIMENSIONS;
     if (cfg->profile < RGH_PROFILE_BASELINE || cfg->profile > RGH_PROFILE_HIGH || cfg->qp < 0 || cfg->qp > 51 ||
         cfg->fps_num <= 0 || cfg->fps_den <= 0 ||
-        (cfg->speed != RGH_SPEED_BALANCED && cfg->speed != RGH_SPEED_COMPACT && cfg->speed != RGH_SPEED_FAST))
+        (cfg->speed < RGH_SPEED_BALANCED || cfg->speed > RGH_SPEED_PARTITIONS))
         return RGH_ERR_INVALID_ARGUMENT;
     const int cqo = cfg->speed == RGH_SPEED_COMPACT ? 1 : 0;  // h264.rs:3087 sync_reorder
     const int mb_w = (cfg->width + 15) / 16, mb_h = (cfg->height + 15) / 16;

@@ -318,6 +318,30 @@ def test_fast_writer_matches_the_oracle(profile, qp):
     assert all(p["headers"][0]["disable_deblock"] == 1 for p in pics)
 
 
+@pytest.mark.parametrize("profile", ["main", "high"])
+@pytest.mark.parametrize("qp", [12, 26, 40])
+def test_partition_writer_matches_the_oracle(profile, qp):
+    from ragdoll_amd import write_headers, write_idr_slice_cabac, write_p_slice_cabac
+    w, h, keyint = 208, 144, 4
+    frames = scene(w, h, 6, 1300 + qp)
+    enc = OracleEncoder(w, h, qp, profile=profile, keyint=keyint, speed="partitions", me_range=32)
+    kinds = set()
+    for k, f in enumerate(frames):
+        want = enc.encode(*f)
+        dec, lev = enc.decisions(), enc.levels()
+        gop = k % keyint
+        if enc.last_was_idr:
+            got = write_headers(w, h, qp, profile=profile, speed="partitions") + write_idr_slice_cabac(
+                dec, lev, enc.mb_w, enc.mb_h, qp, k // keyint, profile=profile, speed="partitions")
+        else:
+            got = write_p_slice_cabac(dec, lev, enc.mb_w, enc.mb_h, qp, gop, (2 * gop) & 63, profile=profile,
+                                      speed="partitions")
+            kinds |= set(int(t) for t in dec["mb_type"])
+        assert got == want, "%s qp %d frame %d: %d vs %d bytes" % (profile, qp, k, len(got), len(want))
+    if qp <= 26:
+        assert {5, 6, 7} <= kinds, kinds
+
+
 def test_writer_refuses_what_it_cannot_write():
     from ragdoll_amd import RghError, write_idr_slice_cabac, write_p_slice_cabac
     from ragdoll_amd.abi import DECISION_DTYPE, LEVELS_PER_MB

@@ -166,3 +166,30 @@ def test_writer_matches_oracle_fast_slices(qp):
         types |= set(np.unique(ora.decisions()["mb_type"]).tolist())
     assert 3 not in types  # no Intra 4x4 in Fast
     assert 1 in types
+
+
+@pytest.mark.parametrize("qp", [10, 24, 34, 46])
+def test_writer_matches_oracle_partition_slices(qp):
+    # The partition extension (16x8 / 8x16 / 8x8 P partitions on the Balanced search): mb_type
+    # ue(0..3), sub_mb_type ue(0) x 4, each partition's mvd against its directional / median
+    # predictor, from the oracle's records in the quadrant layout.
+    kw = dict(CONFIGS["cif"])
+    w, h = kw.pop("width"), kw.pop("height")
+    g = SceneGenerator(w, h, seed=config_seed(5, qp), **kw)
+    ora = OracleEncoder(w, h, qp, keyint=5, speed="partitions", me_range=32)
+    types = set()
+    for k in range(6):
+        nals = split_annexb(ora.encode(*g.next_frame()))
+        if ora.last_was_idr:
+            assert split_annexb(write_headers(w, h, qp, speed="partitions")) == nals[:2]
+            got = write_idr_slice_cavlc(ora.decisions(), ora.levels(), ora.mb_w, ora.mb_h, qp, k // 5,
+                                        speed="partitions")
+            assert split_annexb(got) == nals[2:], "qp %d frame %d" % (qp, k)
+            continue
+        gop = k % 5
+        nal = write_p_slice_cavlc(ora.decisions(), ora.levels(), ora.mb_w, ora.mb_h, qp, gop, (2 * gop) & 63,
+                                  speed="partitions")
+        assert split_annexb(nal) == nals[-1:], "qp %d frame %d" % (qp, k)
+        types |= set(np.unique(ora.decisions()["mb_type"]).tolist())
+    if qp <= 34:
+        assert {5, 6, 7} <= types, types
