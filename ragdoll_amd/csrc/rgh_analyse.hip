@@ -41,6 +41,7 @@ struct MbLds {
     alignas(16) uint8_t etop[16], eleft[16], ctop[2][8], cleft[2][8];
     alignas(16) int32_t etl, ctl[2];
     alignas(16) uint32_t nb[4];  // Fast: costs of the last diamond iteration's four points
+    alignas(16) uint8_t pred2[256];  // P partitions: the prediction of the shape being searched
     union {
         struct {
             alignas(16) uint8_t sr[SR_H * SR_W];
@@ -549,6 +550,71 @@ __device__ void subpel_sad4(int x0, int y0, int bx, int by, int ox, int oy, cons
         const uint32_t a = __builtin_amdgcn_alignbyte(da[c + 1], da[c], fa);
         const uint32_t b = __builtin_amdgcn_alignbyte(db[c + 1], db[c], fb);
         sad = __builtin_amdgcn_sad_u8(src[c], avg4(a, b), sad);
+    }
+    sad = row16_sum(sad);
+#pragma unroll
+    for (int q = 0; q < 4; q++) out[q] = (uint32_t)__builtin_amdgcn_readlane((int)sad, 16 * q);
+}
+
+// ---------------------------------------------------------------------------------------
+// P partitions (RGH_FLAG_PARTITIONS): costs of a partition of W x h samples at (px, py) in the
+// macroblock, in the 16-lane row layout of the diamond (lane 16 k + r: row r < h of point k).
+// ---------------------------------------------------------------------------------------
+// SADs of four full-sample vectors (fx[k], fy[k]) from the staged region whose origin is
+// (ox, oy); every point's rows lie inside it.
+template <int W>
+__device__ void part_sad4(int x0, int y0, int px, int py, int h, int ox, int oy, const int *fx, const int *fy,
+                          const MbLds &S, uint32_t *out) {
+    const int lane = threadIdx.x, k = lane >> 4, r = lane & 15;
+    const int qx = sel4(k, fx[0], fx[1], fx[2], fx[3]), qy = sel4(k, fy[0], fy[1], fy[2], fy[3]);
+    uint32_t sad = 0;
+    if (r < h) {
+        const int lx = x0 + px + qx - ox, ly = y0 + py + qy - oy + r;
+        const uint32_t *q = (const uint32_t *)&S.sr[ly * SR_W + (lx & ~3)];
+        const uint32_t f = (uint32_t)(lx & 3);
+        const uint32_t *sw = (const uint32_t *)&S.src[(py + r) * 16 + px];
+        uint32_t d[W / 4 + 1];
+#pragma unroll
+        for (int c = 0; c <= W / 4; c++) d[c] = q[c];
+#pragma unroll
+        for (int c = 0; c < W / 4; c++) sad = __builtin_amdgcn_sad_u8(sw[c], __builtin_amdgcn_alignbyte(d[c + 1], d[c], f), sad);
+    }
+    sad = row16_sum(sad);
+#pragma unroll
+    for (int q = 0; q < 4; q++) out[q] = (uint32_t)__builtin_amdgcn_readlane((int)sad, 16 * q);
+}
+
+// SADs of four quarter-sample vectors (mx[k], my[k]) of the partition through the half-sample
+// window anchored at full sample (ax, ay) (the partition's full-sample result in the picture),
+// each within +-3 quarter samples of it: window_pred4's plane pairs (kPhaseLo / kPhaseHi).
+template <int W>
+__device__ void part_subpel_sad4(int x0, int y0, int px, int py, int h, int ax, int ay, int ox, int oy, const int *mx,
+                                 const int *my, MbLds &S, uint32_t *out) {
+    const int lane = threadIdx.x, k = lane >> 4, r = lane & 15;
+    const int qx = sel4(k, mx[0], mx[1], mx[2], mx[3]), qy = sel4(k, my[0], my[1], my[2], my[3]);
+    uint32_t sad = 0;
+    if (r < h) {
+        const int ph = (qy & 3) * 4 + (qx & 3);
+        const uint32_t e = (uint32_t)(((ph < 8 ? kPhaseLo : kPhaseHi) >> (8 * (ph & 7))) & 0xFF);
+        int sa, sb;
+        const int oa = plane_offset(e & 3, x0 + px, y0 + py, ax, ay, ox, oy, qx, qy, (e >> 4) & 1, 0, sa);
+        const int ob = plane_offset((e >> 2) & 3, x0 + px, y0 + py, ax, ay, ox, oy, qx, qy, (e >> 5) & 1, (e >> 6) & 1, sb);
+        const uint8_t *base = (const uint8_t *)&S;
+        const uint32_t *qa = (const uint32_t *)(base + (oa & ~3) + r * sa), *qb = (const uint32_t *)(base + (ob & ~3) + r * sb);
+        const uint32_t fa = (uint32_t)(oa & 3), fb = (uint32_t)(ob & 3);
+        uint32_t da[W / 4 + 1], db[W / 4 + 1];
+#pragma unroll
+        for (int c = 0; c <= W / 4; c++) {
+            da[c] = qa[c];
+            db[c] = qb[c];
+        }
+        const uint32_t *sw = (const uint32_t *)&S.src[(py + r) * 16 + px];
+#pragma unroll
+        for (int c = 0; c < W / 4; c++) {
+            const uint32_t a = __builtin_amdgcn_alignbyte(da[c + 1], da[c], fa);
+            const uint32_t b = __builtin_amdgcn_alignbyte(db[c + 1], db[c], fb);
+            sad = __builtin_amdgcn_sad_u8(sw[c], avg4(a, b), sad);
+        }
     }
     sad = row16_sum(sad);
 #pragma unroll
@@ -1284,7 +1350,101 @@ __device__ bool ref_box_wait(const RghJobD &J, int px0, int px1, int py0, int py
 // the neighbours, the co-located decision and the reference item of this macroblock were waited
 // for by the caller); otherwise per-step launches (reads issued before the readiness verdict are
 // discarded when the macroblock defers).
-template <bool LEARNED, bool PERSIST = false, bool FAST = false>
+constexpr int PART_GATE = 40;   // partitions are searched above this 16x16 cost (lambda units)
+constexpr int PART_RANGE = 8;   // their full-sample vectors lie within +-8 of the 16x16 one
+constexpr int PART_ITERS = 8;   // diamond steps per partition
+
+// One partition's search (RGH_FLAG_PARTITIONS) of W x ph samples at (px, py) in the macroblock,
+// in the region staged at (pox, poy) around the 16x16 full-sample vector: candidates [predictor
+// (pmx, pmy), 16x16 vector (m16x, m16y)] rounded to full samples and clamped into
+// [lo0, hi0] x [lo1, hi1] (the second only when it differs), the first strictly cheapest of SAD +
+// lambda x mvd bits; at most PART_ITERS diamond steps inside the box (enc.rs:1207's rule: the
+// first of the cheapest points below the best moves); the half-sample diamond (one step) and the
+// quarter-sample diamond (qpel_iters steps) within +-3 quarter samples of the full-sample result
+// and inside the macroblock's vector bounds (enc.rs:1283's rule, the SAD through the window).
+// Writes the partition's prediction into S.pred2 and returns its vector (x | y << 16).
+template <int W>
+__device__ __forceinline__ uint32_t part_search(const RghJobD &J, int x0, int y0, int px, int py, int ph, int pox,
+                                                int poy, int lo0, int hi0, int lo1, int hi1, int pmx, int pmy,
+                                                int m16x, int m16y, uint32_t lambda, int bmin0, int bmax0, int bmin1,
+                                                int bmax1, MbLds &S) {
+    const int lane = threadIdx.x, rr = lane >> 2, c0 = (lane & 3) * 4;
+    auto mvc = [&](int qx, int qy) { return lambda * (se_bits(qx - pmx) + se_bits(qy - pmy)); };
+    const int c0x = iclamp((pmx + 2) >> 2, lo0, hi0), c0y = iclamp((pmy + 2) >> 2, lo1, hi1);
+    const int c1x = iclamp((m16x + 2) >> 2, lo0, hi0), c1y = iclamp((m16y + 2) >> 2, lo1, hi1);
+    int bx = c0x, by = c0y;
+    uint32_t best;
+    {
+        const int fx[4] = {c0x, c1x, c0x, c0x}, fy[4] = {c0y, c1y, c0y, c0y};
+        uint32_t sads[4];
+        part_sad4<W>(x0, y0, px, py, ph, pox, poy, fx, fy, S, sads);
+        best = sads[0] + mvc(4 * c0x, 4 * c0y);
+        if (c1x != c0x || c1y != c0y) {
+            const uint32_t c = sads[1] + mvc(4 * c1x, 4 * c1y);
+            if (c < best) {
+                best = c;
+                bx = c1x;
+                by = c1y;
+            }
+        }
+    }
+#pragma unroll 1
+    for (int it = 0; it < PART_ITERS; it++) {
+        const int fx[4] = {bx, bx - 1, bx + 1, bx}, fy[4] = {by - 1, by, by, by + 1};
+        uint32_t sads[4];
+        part_sad4<W>(x0, y0, px, py, ph, pox, poy, fx, fy, S, sads);
+        uint32_t key = 0xFFFFFFFFu;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int ok = (int)(fx[k] >= lo0) & (int)(fx[k] <= hi0) & (int)(fy[k] >= lo1) & (int)(fy[k] <= hi1);
+            key = min(key, ok ? ((sads[k] + mvc(4 * fx[k], 4 * fy[k])) << 2) | (uint32_t)k : 0xFFFFFFFFu);
+        }
+        if ((key >> 2) >= best) break;
+        best = key >> 2;
+        const int moved = (int)(key & 3u);
+        bx += moved == 1 ? -1 : (moved == 2 ? 1 : 0);
+        by += moved == 0 ? -1 : (moved == 3 ? 1 : 0);
+    }
+    // Sub-sample steps through the window anchored at the full-sample result.
+    const int ax = x0 + px + bx, ay = y0 + py + by;
+    compute_window(S, ax - 1 - pox, ay - 1 - poy);
+    const int cx = bx * 4, cy = by * 4;
+    int bmx = cx, bmy = cy;
+#pragma unroll 1
+    for (int pass = 0; pass < 2; pass++) {
+        const int step = pass == 0 ? 2 : 1, iters = pass == 0 ? 1 : J.qpel_iters;
+#pragma unroll 1
+        for (int it = 0; it < iters; it++) {
+            const int fx = bmx, fy = bmy;
+            const int dx[4] = {0, -step, step, 0}, dy[4] = {-step, 0, 0, step};
+            int mx[4], my[4], ok[4], qx[4], qy[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                mx[k] = fx + dx[k];
+                my[k] = fy + dy[k];
+                ok[k] = (int)(iabs(mx[k] - cx) <= 3) & (int)(iabs(my[k] - cy) <= 3) & (int)(mx[k] >= bmin0) &
+                        (int)(mx[k] <= bmax0) & (int)(my[k] >= bmin1) & (int)(my[k] <= bmax1);
+                qx[k] = ok[k] ? mx[k] : cx;
+                qy[k] = ok[k] ? my[k] : cy;
+            }
+            uint32_t sads[4];
+            part_subpel_sad4<W>(x0, y0, px, py, ph, ax, ay, pox, poy, qx, qy, S, sads);
+            uint32_t key = 0xFFFFFFFFu;
+#pragma unroll
+            for (int k = 0; k < 4; k++) key = min(key, ok[k] ? ((sads[k] + mvc(mx[k], my[k])) << 2) | (uint32_t)k : 0xFFFFFFFFu);
+            if ((key >> 2) >= best) break;
+            best = key >> 2;
+            const int kb = (int)(key & 3u);
+            bmx = fx + (kb == 1 ? -step : (kb == 2 ? step : 0));
+            bmy = fy + (kb == 0 ? -step : (kb == 3 ? step : 0));
+        }
+    }
+    if (rr >= py && rr < py + ph && c0 >= px && c0 < px + W)
+        *(uint32_t *)&S.pred2[rr * 16 + c0] = window_pred4(x0, y0, ax, ay, pox, poy, bmx, bmy, S);
+    return ((uint32_t)bmx & 0xFFFFu) | ((uint32_t)bmy << 16);
+}
+
+template <bool LEARNED, bool PERSIST = false, bool FAST = false, bool PARTS = false>
 __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, int mb_x, int mb_y, MbLds &S,
                                            const Gate &G) {
     const int lane = threadIdx.x;
@@ -1333,15 +1493,28 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
     // stand-in address and are discarded.
     const bool has_l = mb_x > 0, has_t = mb_y > 0, has_c = has_t && (mb_x + 1 < mb_w || has_l);
     const int ci = mb_x + 1 < mb_w ? idx - mb_w + 1 : idx - mb_w - 1;
-    uint32_t dw0 = 0, dw2 = 0;
-    if (lane < (FAST ? 5 : 4)) {
-        // Lane 4: the top-left decision (the Fast preset's skip-probe gate).
+    uint32_t dw0 = 0, dw2 = 0, dw5 = 0;
+    const bool has_tr = has_t && mb_x + 1 < mb_w;
+    if (lane < (FAST || PARTS ? 5 : 4)) {
+        // Lane 4: the top-left decision (the Fast preset's skip-probe gate; with partitions the
+        // top-left quadrant 3 vector, and lane 2 reads the top-right record only).
         const int di = lane == 0 ? (has_l ? idx - 1 : idx)
                                  : lane == 1 ? (has_t ? idx - mb_w : idx)
-                                             : lane == 4 ? (has_l && has_t ? idx - mb_w - 1 : idx) : (has_c ? ci : idx);
+                                             : lane == 4 ? (has_l && has_t ? idx - mb_w - 1 : idx)
+                                                         : (PARTS ? (has_tr ? idx - mb_w + 1 : idx) : (has_c ? ci : idx));
         const RGH_G uint32_t *dp = (const RGH_G uint32_t *)(lane == 3 ? &J.prev[idx] : &J.dec[di]);
         dw0 = dp[0];
-        dw2 = dp[2];
+        if constexpr (PARTS) {
+            // Records in the quadrant layout (words 2-5): the blocks next to this macroblock are
+            // the left one's quadrants 1 and 3, the top one's 2 and 3, the top-right one's 2, the
+            // top-left one's 3 and the co-located one's 0 (the reference's prev_mb_mv).
+            const uint64_t qa = *(const RGH_G uint64_t *)(dp + 2), qb = *(const RGH_G uint64_t *)(dp + 4);
+            const uint32_t q0 = (uint32_t)qa, q1 = (uint32_t)(qa >> 32), q2 = (uint32_t)qb, q3 = (uint32_t)(qb >> 32);
+            dw2 = lane == 0 ? q1 : (lane == 1 || lane == 2) ? q2 : lane == 3 ? q0 : q3;
+            dw5 = q3;
+        } else {
+            dw2 = dp[2];
+        }
     }
     uint32_t ew = 0;
     int eshift = 0;
@@ -1364,12 +1537,16 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
         eshift = 8 * (int)((uintptr_t)ea & 3);
         ew = *(const RGH_G uint32_t *)(ea - ((uintptr_t)ea & 3));
     }
+    // Intra records (I_16x16, I_4x4, I_PCM; with partitions 5-7 are inter).
+    auto intra_rec = [](uint32_t w0) { return PARTS ? (w0 & 0xFFu) - 2u <= 2u : (w0 & 0xFFu) >= 2u; };
     int lref = -2, tref = -2, dref = -2, ax = 0, ay = 0, bx_ = 0, by_ = 0, cxv = 0, cyv = 0;
     int nskip = 1;  // Fast: a P_Skip among the left, top, top-left and top-right neighbours
     {
         const uint32_t l0 = __builtin_amdgcn_readlane(dw0, 0), l2 = __builtin_amdgcn_readlane(dw2, 0);
         const uint32_t t0 = __builtin_amdgcn_readlane(dw0, 1), t2 = __builtin_amdgcn_readlane(dw2, 1);
-        const uint32_t c0w = __builtin_amdgcn_readlane(dw0, 2), c2 = __builtin_amdgcn_readlane(dw2, 2);
+        // With partitions lane 2 holds the top-right record only; C falls back to D (lane 4).
+        const int cl = PARTS && mb_x + 1 >= mb_w ? 4 : 2;
+        const uint32_t c0w = __builtin_amdgcn_readlane(dw0, cl), c2 = __builtin_amdgcn_readlane(dw2, cl);
         if constexpr (FAST) {
             const uint32_t tl0 = __builtin_amdgcn_readlane(dw0, 4);
             nskip = (int)(has_l && (l0 & 0xFF) == 0) | (int)(has_t && (t0 & 0xFF) == 0) |
@@ -1377,24 +1554,42 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
         }
         // Intra neighbours have no motion (an I_4x4 record's mv words hold its modes).
         if (has_l) {
-            lref = (l0 & 0xFF) >= 2 ? -1 : 0;
+            lref = intra_rec(l0) ? -1 : 0;
             ax = lref ? 0 : (int16_t)(l2 & 0xFFFF);
             ay = lref ? 0 : (int16_t)(l2 >> 16);
         }
         if (has_t) {
-            tref = (t0 & 0xFF) >= 2 ? -1 : 0;
+            tref = intra_rec(t0) ? -1 : 0;
             bx_ = tref ? 0 : (int16_t)(t2 & 0xFFFF);
             by_ = tref ? 0 : (int16_t)(t2 >> 16);
         }
         if (has_c) {
-            dref = (c0w & 0xFF) >= 2 ? -1 : 0;
+            dref = intra_rec(c0w) ? -1 : 0;
             cxv = dref ? 0 : (int16_t)(c2 & 0xFFFF);
             cyv = dref ? 0 : (int16_t)(c2 >> 16);
         }
     }
     const uint32_t col0 = __builtin_amdgcn_readlane(dw0, 3), col2 = __builtin_amdgcn_readlane(dw2, 3);
-    const bool col_intra = (col0 & 0xFF) >= 2;
+    const bool col_intra = intra_rec(col0);
     const int colx = col_intra ? 0 : (int16_t)(col2 & 0xFFFF), coly = col_intra ? 0 : (int16_t)(col2 >> 16);
+    // With partitions: the neighbours' quadrant vectors that partition predictors read (packed
+    // x | y << 16; zero for intra) and their reference indices (-2 outside the picture, -1 intra).
+    uint32_t nq_l1 = 0, nq_l3 = 0, nq_t2 = 0, nq_t3 = 0, nq_tr = 0, nq_tl = 0;
+    int nr_tr = -2, nr_tl = -2;
+    if constexpr (PARTS) {
+        nq_l1 = ((uint32_t)ax & 0xFFFFu) | ((uint32_t)ay << 16);
+        nq_t2 = ((uint32_t)bx_ & 0xFFFFu) | ((uint32_t)by_ << 16);
+        nq_l3 = lref == 0 ? __builtin_amdgcn_readlane(dw5, 0) : 0u;
+        nq_t3 = tref == 0 ? __builtin_amdgcn_readlane(dw5, 1) : 0u;
+        if (has_tr) {
+            nr_tr = intra_rec(__builtin_amdgcn_readlane(dw0, 2)) ? -1 : 0;
+            nq_tr = nr_tr == 0 ? __builtin_amdgcn_readlane(dw2, 2) : 0u;
+        }
+        if (has_l && has_t) {
+            nr_tl = intra_rec(__builtin_amdgcn_readlane(dw0, 4)) ? -1 : 0;
+            nq_tl = nr_tl == 0 ? __builtin_amdgcn_readlane(dw2, 4) : 0u;
+        }
+    }
     {
         const uint32_t eb = (ew >> eshift) & 0xFF;
         if (lane < 16)
@@ -1534,6 +1729,8 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
     D.qp = (uint8_t)qp;
     int mb_type = -1, mvx = 0, mvy = 0;
     int cbpl = 0, cbpc = 0, use8 = 0, i16_mode = 0, cmode = 0;
+    int shape = 0;                    // partitions: 0 16x16, 1 16x8, 2 8x16, 3 8x8
+    uint32_t qv[4] = {0, 0, 0, 0};    // ... and the chosen shape's quadrant vectors (x | y << 16)
 
     // ---- Skip probe (enc.rs:2906 try_p_skip; Balanced probes every macroblock). ----
     if (probe) {
@@ -1899,7 +2096,120 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
             inter_satd = satd16(S.src, S.pred);
             D.n_satd++;
         }
-        const uint32_t inter_cost = inter_satd + lambda * (se_bits(mvx - mvpx) + se_bits(mvy - mvpy) + 1);
+        uint32_t inter_cost = inter_satd + lambda * (se_bits(mvx - mvpx) + se_bits(mvy - mvpy) + 1);
+        if constexpr (PARTS) {
+            // The 16x8, 8x16 and 8x8 shapes when the 16x16 cost exceeds PART_GATE lambda: each
+            // partition searched in decoding order (part_search) around the 16x16 full-sample
+            // vector (bestx, besty), all inside one staged region; a shape costs the 16x16 SATD
+            // of its prediction + lambda x (mvd bits + mb_type bits: 3, 3, 3 + 4); the first
+            // strictly cheapest of 16x16, 16x8, 8x16, 8x8 wins.
+            const uint32_t v16 = ((uint32_t)mvx & 0xFFFFu) | ((uint32_t)mvy << 16);
+            qv[0] = qv[1] = qv[2] = qv[3] = v16;
+            if (inter_cost > (uint32_t)PART_GATE * lambda) {
+                // Region: columns [x0 + bestx - 14 (aligned down), +68), rows [y0 + besty - 11,
+                // +47): every partition vector within +-8 samples, its window and chroma taps.
+                const int pox = (x0 + bestx - PART_RANGE - 3) & ~3, poy = y0 + besty - PART_RANGE - 3;
+                if (J.ref_fin && __ballot(ref_box_pending(J, pox, pox + SR_COLS, poy, poy + 47, G.seq))) return false;
+                stage_region<47>(J, pox, poy, S);
+                const int lo0 = max(imn0, bestx - PART_RANGE), hi0 = min(imx0, bestx + PART_RANGE);
+                const int lo1 = max(imn1, besty - PART_RANGE), hi1 = min(imx1, besty + PART_RANGE);
+                // The 4x4 block (x, y) (4x4 units, relative to this macroblock) a predictor reads:
+                // a neighbour's quadrant vector, or a decided quadrant of this shape (cq, done).
+                auto nbk = [&](int x, int y, const uint32_t *cq, int done, int &r) -> uint32_t {
+                    if (y < 0) {
+                        if (x < 0) {
+                            r = nr_tl;
+                            return nq_tl;
+                        }
+                        if (x < 4) {
+                            r = tref;
+                            return x >= 2 ? nq_t3 : nq_t2;
+                        }
+                        r = nr_tr;
+                        return nq_tr;
+                    }
+                    if (x < 0) {
+                        r = lref;
+                        return y >= 2 ? nq_l3 : nq_l1;
+                    }
+                    if (x >= 4) {
+                        r = -2;
+                        return 0u;
+                    }
+                    const int k = (y >> 1) * 2 + (x >> 1);
+                    r = ((done >> k) & 1) ? 0 : -2;
+                    return r == 0 ? cq[k] : 0u;
+                };
+                auto vx = [](uint32_t v) { return (int)(int16_t)(v & 0xFFFFu); };
+                auto vy = [](uint32_t v) { return (int)(int16_t)(v >> 16); };
+                uint32_t best_cost = inter_cost;
+#pragma unroll 1
+                for (int sh = 1; sh < 4; sh++) {
+                    uint32_t cq[4] = {0, 0, 0, 0};
+                    int done = 0;
+                    uint32_t bits = sh == 3 ? 7u : 3u;
+                    const int np = sh == 3 ? 4 : 2;
+#pragma unroll 1
+                    for (int p = 0; p < np; p++) {
+                        const int px = sh == 2 ? 8 * p : (sh == 3 ? 8 * (p & 1) : 0);
+                        const int py = sh == 1 ? 8 * p : (sh == 3 ? 8 * (p >> 1) : 0);
+                        const int pw = sh == 1 ? 16 : 8, ph = sh == 2 ? 16 : 8;
+                        // 8.4.1.3: A, B, C (else D), B and C from A when only A is available, the
+                        // directional rules of 16x8 / 8x16, else the single match or the median.
+                        const int bx4 = px >> 2, by4 = py >> 2, pw4 = pw >> 2;
+                        int ra, rb, rc;
+                        uint32_t va = nbk(bx4 - 1, by4, cq, done, ra), vb = nbk(bx4, by4 - 1, cq, done, rb);
+                        uint32_t vc = nbk(bx4 + pw4, by4 - 1, cq, done, rc);
+                        if (rc == -2) vc = nbk(bx4 - 1, by4 - 1, cq, done, rc);
+                        if (rb == -2 && rc == -2 && ra != -2) {
+                            vb = vc = va;
+                            rb = rc = ra;
+                        }
+                        int dirk = -1;
+                        if (sh == 1) dirk = p == 0 ? 1 : 0;
+                        if (sh == 2) dirk = p == 0 ? 0 : 2;
+                        uint32_t pv;
+                        if (dirk >= 0 && (dirk == 0 ? ra : dirk == 1 ? rb : rc) == 0) {
+                            pv = dirk == 0 ? va : dirk == 1 ? vb : vc;
+                        } else {
+                            const int hits = (ra == 0) + (rb == 0) + (rc == 0);
+                            if (hits == 1) {
+                                pv = ra == 0 ? va : (rb == 0 ? vb : vc);
+                            } else {
+                                const int mx_ = max(min(vx(va), vx(vb)), min(max(vx(va), vx(vb)), vx(vc)));
+                                const int my_ = max(min(vy(va), vy(vb)), min(max(vy(va), vy(vb)), vy(vc)));
+                                pv = ((uint32_t)mx_ & 0xFFFFu) | ((uint32_t)my_ << 16);
+                            }
+                        }
+                        const int pmx = vx(pv), pmy = vy(pv);
+                        const uint32_t mv = pw == 16 ? part_search<16>(J, x0, y0, px, py, ph, pox, poy, lo0, hi0, lo1, hi1,
+                                                                        pmx, pmy, mvx, mvy, lambda, bmin0, bmax0, bmin1, bmax1, S)
+                                                     : part_search<8>(J, x0, y0, px, py, ph, pox, poy, lo0, hi0, lo1, hi1,
+                                                                       pmx, pmy, mvx, mvy, lambda, bmin0, bmax0, bmin1, bmax1, S);
+                        bits += se_bits(vx(mv) - pmx) + se_bits(vy(mv) - pmy);
+#pragma unroll
+                        for (int k = 0; k < 4; k++) {
+                            const int kx = (k & 1) * 8, ky = (k >> 1) * 8;
+                            if (kx >= px && kx < px + pw && ky >= py && ky < py + ph) {
+                                cq[k] = mv;
+                                done |= 1 << k;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                    const uint32_t c = satd16(S.src, S.pred2) + lambda * bits;
+                    if (c < best_cost) {
+                        best_cost = c;
+                        shape = sh;
+                        *(uint32_t *)&S.pred[lane * 4] = *(const uint32_t *)&S.pred2[lane * 4];
+#pragma unroll
+                        for (int k = 0; k < 4; k++) qv[k] = cq[k];
+                    }
+                    __syncthreads();
+                }
+                inter_cost = best_cost;
+            }
+        }
         bool try_i16 = inter_satd > 24 * lambda;
         if constexpr (LEARNED) {
             if (try_i16 && p_intra < J.learn_intra_gate) {
@@ -1927,7 +2237,16 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
 
         if (mb_type < 0) {
             // P_L0_16x16 residual (enc.rs:3154-3191), or P_Skip when it vanishes at the skip vector.
-            chroma_mc_combine(ct, mvx, mvy, &pu, &pv);
+            if (PARTS && shape) {
+                // Each chroma 4x4 block at its quadrant's vector (enc/inter.rs:229 predict_inter).
+                const int k = ((lane >> 5) << 1) | ((lane >> 2) & 1);
+                const uint32_t v = k == 0 ? qv[0] : k == 1 ? qv[1] : k == 2 ? qv[2] : qv[3];
+                const int lmx = (int)(int16_t)(v & 0xFFFFu), lmy = (int)(int16_t)(v >> 16);
+                const ChromaTaps t = chroma_taps(J, mb_x, mb_y, lmx, lmy);
+                chroma_mc_combine(t, lmx, lmy, &pu, &pv);
+            } else {
+                chroma_mc_combine(ct, mvx, mvy, &pu, &pv);
+            }
             S.predu[lane] = (uint8_t)pu;
             S.predv[lane] = (uint8_t)pv;
             // enc.rs:1810 quant_luma_4x4_inter: decimated 8x8 quadrants, then the whole MB. One
@@ -2054,10 +2373,10 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
                 }
             }
             cbpc = chroma_cbp(S);
-            if (cbpl == 0 && cbpc == 0 && mvx == pskx && mvy == psky) {
+            if (shape == 0 && cbpl == 0 && cbpc == 0 && mvx == pskx && mvy == psky) {
                 mb_type = 0;
             } else {
-                mb_type = 1;
+                mb_type = shape == 0 ? 1 : (shape == 1 ? 6 : (shape == 2 ? 7 : 5));
                 // Reconstruction (enc.rs:3282-3306).
                 if (use8) {
                     recon8(S, cbpl, qp);
@@ -2093,8 +2412,9 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
 
     RGH_G int16_t *lv = O.levels + (long)idx * RGH_LEVELS_PER_MB;
     uint32_t nnz = 0;
-    if (mb_type == 1 || mb_type == 2) {
-        if (mb_type == 1 && use8) {
+    const bool inter_coded = PARTS ? (mb_type == 1 || mb_type >= 5) : mb_type == 1;
+    if (inter_coded || mb_type == 2) {
+        if (inter_coded && use8) {
             for (int i = lane; i < 256; i += 64) lv[i] = S.q8[i >> 6][c_zigzag8[i & 63]];
             nnz = (cbpl & 1 ? 0x0033u : 0) | (cbpl & 2 ? 0x00CCu : 0) | (cbpl & 4 ? 0x3300u : 0) | (cbpl & 8 ? 0xCC00u : 0);
         } else {
@@ -2122,10 +2442,25 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
         D.chroma_mode = (uint8_t)(mb_type == 2 ? cmode : 0);
         D.cbp_luma = (uint8_t)cbpl;
         D.cbp_chroma = (uint8_t)cbpc;
-        D.transform_8x8 = (uint8_t)(mb_type == 1 && use8 && cbpl);
+        D.transform_8x8 = (uint8_t)(inter_coded && use8 && cbpl);
         D.mv[0] = (int16_t)mvx;
         D.mv[1] = (int16_t)mvy;
         D.nnz = (uint16_t)nnz;
+        if constexpr (PARTS) {
+            // The quadrant layout: words 2-5 the quadrant vectors (zero for intra), word 6 zero.
+            uint32_t w[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                w[k] = mb_type == 2 ? 0u
+                       : mb_type == 0 ? (((uint32_t)pskx & 0xFFFFu) | ((uint32_t)psky << 16))
+                                      : (shape ? qv[k] : (((uint32_t)mvx & 0xFFFFu) | ((uint32_t)mvy << 16)));
+            memcpy(D.mv, &w[0], 4);
+            memcpy(D.mvp, &w[1], 4);
+            memcpy(&D.cost, &w[2], 4);
+            D.n_sad = (uint16_t)(w[3] & 0xFFFFu);
+            D.n_subpel = (uint16_t)(w[3] >> 16);
+            D.n_satd = D.n_hpel = 0;
+        }
         store_dec(&O.dec[idx], D);
     }
     return true;
@@ -2534,7 +2869,8 @@ struct StepLdsOf<2> {
 // rgh_k_mixed_fast_step, and rgh_k_intra_fast_step for its IDR pictures): the preset's branches
 // stay out of the Balanced kernels' code. MODE: 0 Balanced P, 1 IDR, 2 Compact P, 3 Balanced P
 // with learned proposals, 4 IDR and Balanced P, 5 IDR and Balanced P with learned proposals,
-// 6 Fast P, 7 IDR and Fast P, 8 Fast IDR.
+// 6 Fast P, 7 IDR and Fast P, 8 Fast IDR, 9 Balanced P with partitions (rgh_k_parts_step), 10 IDR
+// and Balanced P with partitions (rgh_k_mixed_parts_step).
 template <int MODE>
 __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int j0, int launch, uint32_t rseq) {
     const RghJobD J = device_view(jobs[j0 + (int)blockIdx.y]);
@@ -2566,6 +2902,13 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
                     analyse_intra_mb<MODE == 7>(J, jp, x, y, S.a);
                 else
                     ok = analyse_mb<MODE == 5, false, MODE == 7>(J, jp, x, y, S.a, Gate{seq, nullptr, 0, -1, 0});
+            } else if constexpr (MODE == 9) {
+                ok = analyse_mb<false, false, false, true>(J, jp, x, y, S.a, Gate{seq, nullptr, 0, -1, 0});
+            } else if constexpr (MODE == 10) {
+                if (J.intra)
+                    analyse_intra_mb<false>(J, jp, x, y, S.a);
+                else
+                    ok = analyse_mb<false, false, false, true>(J, jp, x, y, S.a, Gate{seq, nullptr, 0, -1, 0});
             } else if constexpr (MODE == 2)
                 ok = analyse_compact_mb(J, x, y, S.a, seq);
             else
@@ -2641,6 +2984,16 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_mixed_fast_step(const Rgh
 extern "C" __global__ __launch_bounds__(64) void rgh_k_compact_step(const RghJob *__restrict__ jobs, int j0, int launch,
                                                                      uint32_t seq) {
     frame_step<2>(jobs, j0, launch, seq);
+}
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_parts_step(const RghJob *__restrict__ jobs, int j0, int launch,
+                                                                   uint32_t seq) {
+    frame_step<9>(jobs, j0, launch, seq);
+}
+
+extern "C" __global__ __launch_bounds__(64) void rgh_k_mixed_parts_step(const RghJob *__restrict__ jobs, int j0,
+                                                                         int launch, uint32_t seq) {
+    frame_step<10>(jobs, j0, launch, seq);
 }
 
 extern "C" __global__ __launch_bounds__(64) void rgh_k_learned_step(const RghJob *__restrict__ jobs, int j0, int launch,

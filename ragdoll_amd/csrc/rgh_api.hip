@@ -26,6 +26,8 @@ extern "C" __global__ void rgh_k_persist_fast(const RghJob *jobs, const RghPersi
 extern "C" __global__ void rgh_k_intra_fast_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_fast_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_mixed_fast_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
+extern "C" __global__ void rgh_k_parts_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
+extern "C" __global__ void rgh_k_mixed_parts_step(const RghJob *jobs, int j0, int launch, uint32_t seq);
 extern "C" __global__ void rgh_k_xcc_probe(uint32_t *mask);
 extern "C" __global__ void rgh_k_recount(const RghJob *jobs);
 extern "C" __global__ void rgh_k_extend(const RghJob *jobs);
@@ -94,6 +96,7 @@ struct Ctx {
     int width = 0, height = 0, mb_w = 0, mb_h = 0, profile = 0, qp = 0, me_iters = 16, qpel_iters = 2;
     bool compact = false;  // RGH_FLAG_COMPACT
     bool fast = false;     // RGH_FLAG_FAST
+    bool parts = false;    // RGH_FLAG_PARTITIONS
     int cqo = 0;           // chroma_qp_index_offset
     int sy = 0, sc = 0, rows_y = 0, rows_c = 0;
     std::vector<Picture> pic;  // ring of reconstructed pictures
@@ -232,6 +235,7 @@ RghJob make_job(Ctx *c, int k, const rgh_i420 &src, rgh_mb_decision *dec, const 
     j.deblock = c->qp != 0 && !c->fast;  // Fast slices disable the loop filter (enc.rs:2334)
     j.compact = c->compact ? 1 : 0;
     j.fast = c->fast ? 1 : 0;
+    j.parts = c->parts ? 1 : 0;
     j.cqo = c->cqo;
     j.base = base;
     j.serial = c->serial + 1 + (uint32_t)k;
@@ -269,7 +273,7 @@ struct Schedule {
     RghFrameCtl *ctl_dev = nullptr;
     RghFrameCtl *ctl_host = nullptr;
     int n = 0, n_frames = 0, lag = 1, nj = 0, n_intra = 0;
-    bool compact = false, fast = false, learned = false;
+    bool compact = false, fast = false, parts = false, learned = false;
     uint32_t seq0 = 0;  // sequence number of the call's launch 0
     // While the planned launches are listed, launch() appends them here (group `group`) instead
     // of issuing them.
@@ -289,8 +293,10 @@ struct Schedule {
         // One P kernel per schedule: every job of it uses one preset.
         compact = nj > 0 && jobs[0].compact;
         fast = nj > 0 && jobs[0].fast;
+        parts = nj > 0 && jobs[0].parts;
         for (int j = 0; j < nj; j++)
-            if ((jobs[(size_t)j].compact != 0) != compact || (jobs[(size_t)j].fast != 0) != fast) {
+            if ((jobs[(size_t)j].compact != 0) != compact || (jobs[(size_t)j].fast != 0) != fast ||
+                (jobs[(size_t)j].parts != 0) != parts) {
                 set_error("run_schedule", "contexts of one call must use one preset", __LINE__);
                 return RGH_ERR_INVALID_ARGUMENT;
             }
@@ -350,13 +356,15 @@ struct Schedule {
         const int split = std::min(std::max(n_intra, j_lo), j_hi);
         if (split > j_lo && j_hi > split && !compact) {
             // IDR and Balanced P jobs together: one kernel, so that neither waits for the other.
-            emit(learned ? rgh_k_mixed_learned_step : (fast ? rgh_k_mixed_fast_step : rgh_k_mixed_step), j_lo,
-                 j_hi - j_lo);
+            emit(learned ? rgh_k_mixed_learned_step
+                         : (fast ? rgh_k_mixed_fast_step : (parts ? rgh_k_mixed_parts_step : rgh_k_mixed_step)),
+                 j_lo, j_hi - j_lo);
         } else {
             if (split > j_lo) emit(fast ? rgh_k_intra_fast_step : rgh_k_intra_step, j_lo, split - j_lo);
             if (j_hi > split)
                 emit(learned ? rgh_k_learned_step
-                             : (compact ? rgh_k_compact_step : (fast ? rgh_k_fast_step : rgh_k_frame_step)),
+                             : (compact ? rgh_k_compact_step
+                                        : (fast ? rgh_k_fast_step : (parts ? rgh_k_parts_step : rgh_k_frame_step))),
                      split, j_hi - split);
         }
         for (int j = j_lo; j < j_hi; j++) last[(size_t)j] = L;
@@ -958,7 +966,10 @@ rgh_status rgh_create(const rgh_config *cfg, void **ctx_out) {
     // The Compact preset: with CABAC it codes B pictures (h264.rs:3075), not analysed here.
     if ((cfg->flags & RGH_FLAG_COMPACT) && cfg->profile != RGH_PROFILE_BASELINE) return RGH_ERR_UNSUPPORTED;
     if ((cfg->flags & RGH_FLAG_COMPACT) && (cfg->flags & RGH_FLAG_FAST)) return RGH_ERR_INVALID_ARGUMENT;
-    if (cfg->flags & ~(RGH_FLAG_NO_UNDEBLOCKED_COPY | RGH_FLAG_COMPACT | RGH_FLAG_FAST)) return RGH_ERR_INVALID_ARGUMENT;
+    if ((cfg->flags & RGH_FLAG_PARTITIONS) && (cfg->flags & (RGH_FLAG_COMPACT | RGH_FLAG_FAST)))
+        return RGH_ERR_INVALID_ARGUMENT;
+    if (cfg->flags & ~(RGH_FLAG_NO_UNDEBLOCKED_COPY | RGH_FLAG_COMPACT | RGH_FLAG_FAST | RGH_FLAG_PARTITIONS))
+        return RGH_ERR_INVALID_ARGUMENT;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return RGH_ERR_DEVICE;
     if (cfg->device < 0 || cfg->device >= ndev) return RGH_ERR_INVALID_ARGUMENT;
@@ -976,6 +987,7 @@ rgh_status rgh_create(const rgh_config *cfg, void **ctx_out) {
     c->keep_pre = !(cfg->flags & RGH_FLAG_NO_UNDEBLOCKED_COPY);
     c->compact = (cfg->flags & RGH_FLAG_COMPACT) != 0;
     c->fast = (cfg->flags & RGH_FLAG_FAST) != 0;
+    c->parts = (cfg->flags & RGH_FLAG_PARTITIONS) != 0;
     c->cqo = c->compact ? 1 : 0;
     const int W = c->mb_w * 16, H = c->mb_h * 16;
     c->sy = (W + 2 * RGH_PAD_Y + 31) & ~31;
@@ -1101,7 +1113,7 @@ rgh_status rgh_set_reference(void *ctx, const rgh_i420 *recon) {
 // The contexts of one call share a preset: each preset's P pictures run their own kernels.
 rgh_status check_one_preset(Ctx *const *cs, int n, const char *fn) {
     for (int i = 1; i < n; i++)
-        if (cs[i]->compact != cs[0]->compact || cs[i]->fast != cs[0]->fast) {
+        if (cs[i]->compact != cs[0]->compact || cs[i]->fast != cs[0]->fast || cs[i]->parts != cs[0]->parts) {
             set_error(fn, "contexts of one call must use one preset", __LINE__);
             return RGH_ERR_INVALID_ARGUMENT;
         }
@@ -1187,7 +1199,7 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
         }
         for (int i = 0; i < n; i++) {
             const Ctx *c = static_cast<Ctx *>(ctxs[i]);
-            if (c && (c->compact || c->fast)) {
+            if (c && (c->compact || c->fast || c->parts)) {
                 set_error("rgh_analyse_frames_learned", "learned proposals serve the Balanced preset", __LINE__);
                 return RGH_ERR_UNSUPPORTED;
             }
@@ -1225,8 +1237,8 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
     if (lag == 0) lag = kDefaultLag;
     // The persistent schedule (RGH_FRAMES_PERSISTENT) serves Balanced and IDR pictures; Compact
     // contexts and the no-allocation entry point use the per-step launch schedule.
-    bool compact_any = false;
-    for (int i = 0; i < n; i++) compact_any |= cs[(size_t)i]->compact;
+    bool compact_any = false;  // the Compact preset and partitions have no persistent kernel
+    for (int i = 0; i < n; i++) compact_any |= cs[(size_t)i]->compact || cs[(size_t)i]->parts;
     const bool persistent = !noalloc && !compact_any && (flags & RGH_FRAMES_PERSISTENT);
     const int G = persistent ? 1 : std::max(1, std::min(groups ? groups : 1, n));
     HIP_TRY(hipSetDevice(cs[0]->device));

@@ -113,7 +113,7 @@ __device__ __forceinline__ uint32_t nz_col(uint32_t nnz, int c) {
     return ((nnz >> c) & 1) | (((nnz >> (4 + c)) & 1) << 1) | (((nnz >> (8 + c)) & 1) << 2) | (((nnz >> (12 + c)) & 1) << 3);
 }
 
-// Vector of quadrant k (raster) of a Compact-preset record (words 2-5); zero for intra.
+// Vector of quadrant k (raster) of a record in the quadrant layout (words 2-5); zero for intra.
 __device__ __forceinline__ uint32_t quad_word(const rgh_mb_decision &d, int k) {
     const uint32_t w0 = (uint32_t)(uint16_t)d.mv[0] | ((uint32_t)(uint16_t)d.mv[1] << 16);
     const uint32_t w1 = (uint32_t)(uint16_t)d.mvp[0] | ((uint32_t)(uint16_t)d.mvp[1] << 16);
@@ -134,10 +134,10 @@ __device__ __forceinline__ void deblock_mb_inline(const RghJobD &J, int mb_x, in
     if (mb_x > 0) L = load_dec(&J.dec[m - 1]);
     if (mb_y > 0) T = load_dec(&J.dec[m - mb_w]);
     auto mveq = [](const rgh_mb_decision &a, const rgh_mb_decision &b) { return a.mv[0] == b.mv[0] && a.mv[1] == b.mv[1]; };
-    // Compact pictures: every 4x4 block carries its quadrant's vector (enc/inter.rs:781
-    // deblock_parts); the internal edges of an inter macroblock without coefficients and with
+    // Compact pictures and pictures with P partitions: every 4x4 block carries its quadrant's
+    // vector (enc/inter.rs:781 deblock_parts); the internal edges of an inter macroblock without coefficients and with
     // one vector throughout are not filtered.
-    const bool parts = J.compact != 0;
+    const bool parts = J.compact != 0 || J.parts != 0;
     const bool uniform = !is_intra(Q) && Q.nnz == 0 && quad_word(Q, 0) == quad_word(Q, 1) &&
                          quad_word(Q, 0) == quad_word(Q, 2) && quad_word(Q, 0) == quad_word(Q, 3);
     if (!parts && quiet(Q) && (mb_x == 0 || (quiet(L) && mveq(L, Q))) && (mb_y == 0 || (quiet(T) && mveq(T, Q)))) return;

@@ -63,6 +63,7 @@ struct RghJob {
     // gating the picture slot (-1: not in this launch, so complete before it started).
     int32_t ref_job, wait_job;
     int32_t fast;  // the Fast preset's decisions (RGH_FLAG_FAST)
+    int32_t parts; // P partitions on the Balanced search (RGH_FLAG_PARTITIONS)
     // Sequence number of the call's launch 0: a per-step launch's number is seq0 plus the
     // relative number it is given, so that a replayed launch sequence takes fresh numbers.
     uint32_t seq0;
@@ -96,6 +97,7 @@ struct RghJobD {
     int32_t wait_total;
     int32_t ref_job, wait_job;
     int32_t fast;
+    int32_t parts;
     uint32_t seq0;
 };
 
@@ -162,6 +164,7 @@ __device__ __forceinline__ RghJobD device_view(const RghJob &j) {
     d.ref_job = j.ref_job;
     d.wait_job = j.wait_job;
     d.fast = j.fast;
+    d.parts = j.parts;
     d.seq0 = j.seq0;
     return d;
 }

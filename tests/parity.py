@@ -48,7 +48,8 @@ def run_parity(frames, width, height, qp, profile="baseline", me_range=16, label
     from ragdoll_amd import StreamAnalyzer
     ora = OracleEncoder(width, height, qp, profile=profile, keyint=keyint, me_range=me_range, speed=speed)
     gpu = StreamAnalyzer(width, height, qp, profile=profile, me_range=me_range, speed=speed)
-    stats = {"frames": 0, "mbs": 0, "skip": 0, "inter": 0, "intra": 0, "idr": 0, "i4": 0, "p8x8": 0}
+    stats = {"frames": 0, "mbs": 0, "skip": 0, "inter": 0, "intra": 0, "idr": 0, "i4": 0, "p8x8": 0, "p16x8": 0,
+             "p8x16": 0}
     try:
         for i, (y, u, v) in enumerate(frames):
             ora.encode(y, u, v)
@@ -72,6 +73,8 @@ def run_parity(frames, width, height, qp, profile="baseline", me_range=16, label
             stats["intra"] += int((dec["mb_type"] == 2).sum())
             stats["i4"] += int((dec["mb_type"] == 3).sum())
             stats["p8x8"] += int((dec["mb_type"] == 5).sum())
+            stats["p16x8"] += int((dec["mb_type"] == 6).sum())
+            stats["p8x16"] += int((dec["mb_type"] == 7).sum())
     finally:
         gpu.close()
     return stats

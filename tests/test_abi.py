@@ -63,6 +63,16 @@ def test_compact_preset_is_baseline_only():
         assert L.rgh_create(ctypes.byref(cfg), ctypes.byref(h)) == 2
 
 
+def test_partitions_do_not_combine_with_other_presets():
+    # RGH_FLAG_PARTITIONS extends the Balanced search only: refused with Compact or Fast before
+    # any device use.
+    L = abi.load_library()
+    h = ctypes.c_void_p()
+    for extra in (abi.FLAG_COMPACT, abi.FLAG_FAST):
+        cfg = abi.RgConfig(64, 48, 0, 26, 0, 0, 0, abi.FLAG_PARTITIONS | extra, 0)
+        assert L.rgh_create(ctypes.byref(cfg), ctypes.byref(h)) == 1  # RGH_ERR_INVALID_ARGUMENT
+
+
 def test_writers_refuse_unknown_presets():
     import numpy as np
     L = abi.load_library()
