@@ -216,7 +216,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--lag", type=int, default=0, help="frame lag in wavefront steps (0: autotune)")
     ap.add_argument("--groups", type=int, default=0, help="concurrent schedules (0: autotune)")
-    ap.add_argument("--workload", choices=["headline", "batched", "compact", "fast"], default="headline",
+    ap.add_argument("--workload", choices=["headline", "batched", "compact", "fast", "partitions"], default="headline",
                     help="headline: the 1080p metric (default); batched: the 24-stream uneven-size set; "
                          "compact: the headline streams with the Compact preset (Baseline); "
                          "fast: the headline streams with the Fast preset (High)")
@@ -259,9 +259,21 @@ def main():
         gops, label = [dict(g, profile="baseline", speed="compact") for g in headline_gops()], "compact"
     elif args.workload == "fast":
         gops, label = [dict(g, speed="fast") for g in headline_gops()], "fast"
+    elif args.workload == "partitions":
+        gops, label = partition_gops(), "partitions"
+        args.compare_persistent = 0  # no persistent kernel for partitions: the per-step schedule runs them
     else:
         gops, label = headline_gops(), "headline"
     out = run_workload(gops, args)
+    if label == "partitions":
+        # The same job with 16x16 partitions only (the reference's Balanced decisions), and what the
+        # partitions gain in rate for the same quality.
+        args.end_to_end = args.compare_graph = 0
+        plain = run_workload([dict(g, speed="balanced") for g in gops], args)
+        out["config"]["without_partitions"] = {"value": plain["value"], "ms_per_step": plain["ms_per_step"],
+                                               "vs_partitions": round(plain["value"] / out["value"], 3)}
+        if args.bd_frames:
+            out["config"]["bd_rate"] = partition_bd_rate(gops[0]["frames"])
     if label == "headline" and not args.no_learned and args.bd_frames:
         out["learned"]["bd_rate"] = bd_rate_sample(args)
     if label == "headline" and not args.no_cpu_baseline:
@@ -365,6 +377,45 @@ def headline_gops():
     frames = render_frames(120)
     print("rendered 120 frames in %.1f s" % (time.time() - t0), file=sys.stderr, flush=True)
     return [dict(w=W, h=H, qp=qp, me=ME_RANGE, frames=frames[g * GOP:(g + 1) * GOP]) for qp in QPS for g in range(4)]
+
+
+def partition_gops():
+    """The 720p configuration with partitions: 1280x720, 60 frames of the 720p content (12 objects,
+    |v| <= 16 px/frame, a 0.75 px/frame pan) as two GOPs of 30, High, QP 26, +-32 search,
+    16x16/16x8/8x16/8x8 P partitions with the 8x8 transform."""
+    from ragdoll_amd.synth import CONFIGS, SceneGenerator, config_seed
+    kw = dict(CONFIGS["720p"])
+    w, h = kw.pop("width"), kw.pop("height")
+    t0 = time.time()
+    g = SceneGenerator(w, h, seed=config_seed(1), **kw)
+    frames = [g.next_frame() for _ in range(2 * GOP)]
+    print("rendered 60 frames of 720p in %.1f s" % (time.time() - t0), file=sys.stderr, flush=True)
+    return [dict(w=w, h=h, qp=26, me=32, speed="partitions", frames=frames[k * GOP:(k + 1) * GOP]) for k in range(2)]
+
+
+def partition_bd_rate(frames):
+    """BD-rate of the partition form against 16x16 partitions only on one GOP of the 720p content:
+    QP 22/26/30/34, High, +-32, CABAC bits of the library's writer against mean luma PSNR (pchip)."""
+    from ragdoll_amd.rd import bd_rate, code_gop
+    t0 = time.time()
+    qps = (22, 26, 30, 34)
+    cur = {"balanced": [], "partitions": []}
+    for qp in qps:
+        for form in cur:
+            cur[form].append(code_gop(frames, frames[0][0].shape[1], frames[0][0].shape[0], qp, profile="high",
+                                      me_range=32, speed=form))
+    rate = {k: [c["bytes"] for c in v] for k, v in cur.items()}
+    q = {k: [c["psnr_y"] for c in v] for k, v in cur.items()}
+    out = {"bd_rate_pct": round(bd_rate(rate["balanced"], q["balanced"], rate["partitions"], q["partitions"]), 3),
+           "method": "piecewise cubic (pchip) interpolation of log rate in luma PSNR",
+           "sample": "first GOP (an IDR and 29 P frames) of the 720p content, QP 22/26/30/34, High, +-32, CABAC "
+                     "bits vs mean luma PSNR; negative: partitions need fewer bits for the same quality",
+           "qps": list(qps), "bytes_16x16": rate["balanced"], "bytes_partitions": rate["partitions"],
+           "psnr_y_16x16": [round(v, 4) for v in q["balanced"]],
+           "psnr_y_partitions": [round(v, 4) for v in q["partitions"]],
+           "mb_types_partitions": cur["partitions"][1]["mb_types"], "seconds": round(time.time() - t0, 1)}
+    print("partition bd-rate: %s" % json.dumps(out), file=sys.stderr, flush=True)
+    return out
 
 
 def batched_gops():
@@ -481,7 +532,8 @@ def run_workload(gops, args):
     metric = {"headline": "1080p ME macroblocks/sec, 1 GPU: plain vs learned-proposal at <=1% BD-rate",
               "batched": "batched 24-stream ME macroblocks/sec, 1 GPU, plain form",
               "compact": "1080p ME macroblocks/sec, 1 GPU, Compact preset",
-              "fast": "1080p ME macroblocks/sec, 1 GPU, Fast preset"}[args.workload]
+              "fast": "1080p ME macroblocks/sec, 1 GPU, Fast preset",
+              "partitions": "720p ME macroblocks/sec, 1 GPU, High with 16x16/16x8/8x16/8x8 P partitions"}[args.workload]
     workload = {
         "headline": "plain GPU form: 1920x1080 analysis, High decisions, QP 24/30/36, search range +-64 (64 "
                     "diamond iterations) with quarter-pel refinement, keyint 30, 12 GOP streams x (1 IDR + 29 P "
@@ -495,8 +547,12 @@ def run_workload(gops, args):
                    "pipelined schedule, bit-exact with the CPU oracle; value counts P macroblocks",
         "fast": "Fast preset (diamond search +-64 with a half-pel step, Intra 16x16 only, no loop filter; High "
                 "decisions): 1920x1080, QP 24/30/36, keyint 30, 12 GOP streams x (1 IDR + 29 P frames) per step in "
-                "one pipelined schedule, bit-exact with the CPU oracle; value counts P macroblocks"}[args.workload]
-    kernel = "rgh_k_compact_step" if args.workload == "compact" else "rgh_k_frame_step"
+                "one pipelined schedule, bit-exact with the CPU oracle; value counts P macroblocks",
+        "partitions": "plain GPU form with P partitions 16x16/16x8/8x16/8x8 (the partition extension of the "
+                      "Balanced search) and the 8x8 transform: 1280x720, 60 frames as 2 GOP streams x (1 IDR + 29 P "
+                      "frames), High decisions, QP 26, search range +-32 (32 diamond iterations), one pipelined "
+                      "schedule per step, bit-exact with the CPU oracle; value counts P macroblocks"}[args.workload]
+    kernel = {"compact": "rgh_k_compact_step", "partitions": "rgh_k_parts_step"}.get(args.workload, "rgh_k_frame_step")
     out = {
         "metric": metric,
         "value": round(value, 1),
@@ -528,6 +584,8 @@ def run_workload(gops, args):
                      "traffic_per_macroblock": traffic["per_macroblock"] if traffic else None,
                      "traffic_ratio": traffic["ratio_to_algorithmic"] if traffic else None,
                      "kernel": kernel + (" (+ rgh_k_intra_step in launches with IDR frames)" if args.workload == "compact"
+                                         else " (rgh_k_mixed_parts_step in launches with IDR frames)"
+                                         if args.workload == "partitions"
                                          else " (rgh_k_mixed_step in launches with IDR frames)"),
                      "avg_launch_us": round(avg_launch_s * 1e6, 3),
                      "mbs_per_launch": round(mbs_per_launch, 2), "bytes_per_p_mb": MB_BYTES,

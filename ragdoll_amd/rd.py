@@ -93,7 +93,7 @@ def bd_rate(rate_a, psnr_a, rate_b, psnr_b, method="pchip"):
     return float((np.exp(mean_b - mean_a) - 1.0) * 100.0)
 
 
-def code_gop(frames, width, height, qp, proposer=None, learned=None, profile="baseline", me_range=0):
+def code_gop(frames, width, height, qp, proposer=None, learned=None, profile="baseline", me_range=0, speed="balanced"):
     """Codes one GOP (frame 0 IDR, the rest P) with the GPU analysis, one frame per call, and
     the library's slice writer: CAVLC for "baseline", CABAC for "main" / "high". With `proposer`
     (a learned.Proposer) every P frame is searched from learned proposals
@@ -105,10 +105,11 @@ def code_gop(frames, width, height, qp, proposer=None, learned=None, profile="ba
                       write_idr_slice_cavlc, write_p_slice_cabac, write_p_slice_cavlc)
     from .hip import Hip
     hip = Hip()
-    a = StreamAnalyzer(width, height, qp, profile=profile, me_range=me_range, keep_undeblocked=False, ring=2)
+    a = StreamAnalyzer(width, height, qp, profile=profile, me_range=me_range, keep_undeblocked=False, ring=2,
+                       speed=speed)
     n = a.mb_w * a.mb_h
     dec, lv = hip.alloc(n * 32), hip.alloc(n * LEVELS_PER_MB * 2)
-    total = len(write_headers(width, height, qp, profile=profile))
+    total = len(write_headers(width, height, qp, profile=profile, speed=speed))
     cabac = profile != "baseline"
     sizes, psnrs = [], []
     flags = np.zeros(16, np.int64)
@@ -130,11 +131,12 @@ def code_gop(frames, width, height, qp, proposer=None, learned=None, profile="ba
                 analyse_frames([a], src, [[dec]], [[lv]])
             d, l, rec = a.outputs(which_recon=1)
             if k == 0:
-                nal = (write_idr_slice_cabac(d, l, a.mb_w, a.mb_h, qp, 0, profile=profile) if cabac
-                       else write_idr_slice_cavlc(d, l, a.mb_w, a.mb_h, qp, 0))
+                nal = (write_idr_slice_cabac(d, l, a.mb_w, a.mb_h, qp, 0, profile=profile, speed=speed) if cabac
+                       else write_idr_slice_cavlc(d, l, a.mb_w, a.mb_h, qp, 0, speed=speed))
             else:
-                nal = (write_p_slice_cabac(d, l, a.mb_w, a.mb_h, qp, k & 15, (2 * k) & 63, profile=profile) if cabac
-                       else write_p_slice_cavlc(d, l, a.mb_w, a.mb_h, qp, k & 15, (2 * k) & 63))
+                nal = (write_p_slice_cabac(d, l, a.mb_w, a.mb_h, qp, k & 15, (2 * k) & 63, profile=profile,
+                                           speed=speed) if cabac
+                       else write_p_slice_cavlc(d, l, a.mb_w, a.mb_h, qp, k & 15, (2 * k) & 63, speed=speed))
                 types += np.bincount(d["mb_type"], minlength=8)[:8]
                 inter_cost += int(d["cost"][d["mb_type"] == 1].astype(np.int64).sum())
                 for b in range(4):
