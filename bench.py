@@ -661,8 +661,8 @@ def end_to_end(hip, gops, ans, srcs, decs, lvls, uploaded, sched, total_mbs, arg
 
 def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value, hip):
     """The learned-proposal form on the same streams and schedule. A step computes the proposals
-    of every P frame (features from the frame and its previous source frame, then FP8 inference
-    over all their macroblocks) and analyses every GOP from them (rgh_analyse_frames_learned):
+    of every P frame (features from the frame and its previous source frame and FP8 inference over
+    all their macroblocks, in one pass) and analyses every GOP from them (rgh_analyse_frames_learned):
     that serial step is the value. The pipelined form, where a step analyses from proposals
     computed beside the previous step's analysis on the network's own HIP stream (as a stream of
     GOP waves could run them), is timed too: on MI355X the concurrent kernels slow the
@@ -718,13 +718,15 @@ def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value
         el += e
         b ^= 1
     ceiling = learned_ceiling(gops, ans, srcs, decs, lvls, sched, args, total_mbs, hip, analyse_frames)
-    # FP8 inference alone (one launch over every P macroblock of the step, the whole device), for
-    # its roofline.
+    # The proposal pass alone (mean images, then features and the FP8 network over every P
+    # macroblock of the step in one k_propose launch on the whole device), for its roofline: timed
+    # per call around the synchronous pass, so the figure includes the mean images and the
+    # descriptor copies (the kernel trace has k_propose's own duration).
     net.set_launch_limit(0)
     t = time.perf_counter()
     reps = 3
     for _ in range(reps):
-        net.infer(proposers[0].feat, total_mbs, proposers[0].props)
+        proposers[0].run(pics)
     infer_s = (time.perf_counter() - t) / reps
     flops_mb = 2 * (192 * 128 + 128 * 128 + 128 * 176)
     flops = total_mbs * flops_mb
@@ -755,7 +757,8 @@ def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value
             "intra_gated_pct": round(100.0 * counts[3] / total_mbs, 3),
             "full_pel_only_pct": round(100.0 * counts[4] / total_mbs, 3),
             "mb_types": types.tolist(),
-            "mfma_roofline": {"bound": "mfma", "kernel": "k_infer (FP8 proposal network)",
+            "mfma_roofline": {"bound": "mfma", "kernel": "k_propose (features + FP8 proposal network, one pass; "
+                                                          "timed with k_means and the descriptor copies)",
                               "achieved": round(flops / infer_s / 1e12, 3), "peak": 5000.0, "unit": "TFLOP/s",
                               "frac": flops / infer_s / 1e12 / 5000.0, "flops_per_mb": flops_mb,
                               "launch_ms": round(infer_s * 1e3, 3)}}

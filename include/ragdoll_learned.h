@@ -125,6 +125,15 @@ rgh_status rgh_pnet_quantize_ranges(void *net, float h1_max, float h2_max);
 rgh_status rgh_pnet_infer(void *net, const uint16_t *features_device, int32_t n, rgh_proposal *proposals_device,
                           float *logits_device);
 
+/* Features and FP8 inference of n pictures in one pass (pics: host array; their `features`
+ * fields are not used): the proposals of picture i's macroblocks go to consecutive rows of
+ * proposals_device, picture after picture (ceil(width/16) * ceil(height/16) rows each), and
+ * logits_device [rows][176] fp32 may be NULL. Equals rgh_pnet_features_batch followed by
+ * rgh_pnet_infer over the same rows, bit for bit, without the feature rows in device memory.
+ * No counterpart in the reference. */
+rgh_status rgh_pnet_propose_batch(void *net, const rgh_pnet_picture *pics, int32_t n,
+                                  rgh_proposal *proposals_device, float *logits_device);
+
 /* The quantized network (host outputs, either may be NULL): w8 [RGH_PNET_W8_BYTES] bytes (E4M3
  * codes, layers in order, each [out][in]); scales [RGH_PNET_SCALES]: per-column dequantization
  * scales of each layer (176 slots each), then the three layers' input multipliers (an input x

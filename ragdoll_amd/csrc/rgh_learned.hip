@@ -529,6 +529,341 @@ __global__ __launch_bounds__(256) void k_infer(InferArgs g) {
 }
 
 // ---------------------------------------------------------------------------------------
+// Features and FP8 inference in one pass (rgh_pnet_propose_batch). A workgroup of four waves
+// takes 64 macroblocks of one picture at a time: each wave computes the features of 16 of them,
+// four lanes a macroblock, straight into the tile's E4M3 input rows (the values k_features
+// stores, bfloat16-rounded, then scaled and encoded as k_infer encodes them: a table of the 2048
+// possible cost-volume codes), then the three layers and the top-4 reduction run as in k_infer. The weights live in VGPRs (each wave holds the B fragments of its own output
+// columns: 64 registers), so LDS holds only the tile's activations (~60 KB with the logits
+// aliasing the input rows) and two workgroups share a CU, one computing features while the other
+// multiplies. The features never leave the chip.
+struct ProposePic {
+    const uint8_t *cur;           // luma samples (mean absolute deviation)
+    const uint8_t *mcur, *mprev;  // mean images of the picture and the previous picture
+    int cs, W, H, mw, qp;
+    int row0;                     // first output row of the picture
+};
+struct ProposeArgs {
+    const ProposePic *pics;
+    const int *tile0;  // tile0[i]: first tile of picture i; tile0[n]: the tile count
+    int n;
+    const uint8_t *w8;
+    const float *qs;
+    const float *bias;
+    rgh_proposal *out;
+    float *logits;
+};
+struct ProposeLds {
+    union {
+        struct {
+            uint8_t x[IT * XS];
+            uint8_t h1[IT * HS];
+        };
+        float z[IT * ZS];
+    };
+    union {
+        uint8_t h2[IT * HS];
+        struct {
+            float tv[IT][4][4];
+            int ti[IT][4][4];
+        };
+    };
+    float scale[3][OUT];
+    float bias[3][OUT];
+    float mul[3];
+    uint8_t code[2048];  // E4M3 input code of a cost-volume value (clamped SAD 0..2047)
+};
+static_assert(sizeof(ProposeLds) <= 64 * 1024, "two workgroups per CU");
+
+// fp8_layer with the B fragments in registers: bw[ks][j] holds the 8 weights (k 32 ks + 8 (lane /
+// 16) .. + 7) of column 16 (nb0 + j) + lane % 16.
+template <int K, int IS, int NB, int OS>
+__device__ __forceinline__ void fp8_layer_r(const uint8_t *in, const long (&bw)[K / 32][NB], int nb0, int nbn,
+                                            const float *scale, const float *bias, float mul_next, uint8_t *out8,
+                                            float *outf, int lane) {
+    const int kq = 8 * (lane >> 4), rl = lane & 15;
+    f32x4 acc[4][NB];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int j = 0; j < NB; j++) acc[r][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < K / 32; ks++) {
+        long a[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) a[r] = *(const long *)&in[(r * 16 + rl) * IS + 32 * ks + kq];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int j = 0; j < NB; j++)
+                acc[r][j] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(a[r], bw[ks][j], acc[r][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        if (j >= nbn) continue;
+        const int n = (nb0 + j) * 16 + rl;
+        const float sc = scale[n], bi = bias[n];
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int m = r * 16 + 4 * (lane >> 4) + q;
+                const float v = acc[r][j][q] * sc + bi;
+                if (out8)
+                    out8[m * OS + n] = (uint8_t)fp8_code(fminf(fmaxf(v, 0.0f) * mul_next, FP8_MAX));
+                else
+                    outf[m * ZS + n] = v;
+            }
+    }
+}
+
+// The B fragments of one layer's columns [16 nb0, 16 (nb0 + nbn)) from the quantized weights
+// (global, [out][K] rows).
+template <int K, int NB>
+__device__ __forceinline__ void load_frags(const uint8_t *w, int nb0, int nbn, long (&bw)[K / 32][NB], int lane) {
+    const int kq = 8 * (lane >> 4), rl = lane & 15;
+#pragma unroll
+    for (int ks = 0; ks < K / 32; ks++)
+#pragma unroll
+        for (int j = 0; j < NB; j++)
+            bw[ks][j] = j < nbn ? *(const long *)&w[((nb0 + j) * 16 + rl) * K + 32 * ks + kq] : 0L;
+}
+
+// E4M3 input code of one feature value: bfloat16-rounded (as k_features stores it), then scaled
+// and saturated as k_infer encodes its input.
+__device__ __forceinline__ uint8_t feature_code(float f, float mul0) {
+    const float a = fminf(fmaxf(bf2f(f2bf(f)) * mul0, -FP8_MAX), FP8_MAX);
+    return (uint8_t)(__builtin_amdgcn_cvt_pk_fp8_f32(a, 0.0f, 0, false) & 0xFF);
+}
+
+// The inputs of one tile's features (see k_propose): the picture's metadata and, for lane 4 m + q
+// of wave w (row 16 w + m), window rows dy0 .. dy0 + 6 of the previous picture's mean image
+// (dy0 = 0, 4, 7, 10 for q = 0..3), the macroblock's own 4x4 cells and its source rows 4 q ..
+// 4 q + 3 (loads only).
+struct TileIn {
+    uint32_t win[7][4], own[4], src[4][4];
+    int count, qp;
+    long orow0;  // output row of the tile's first macroblock
+};
+__device__ __forceinline__ void tile_inputs(const ProposeArgs &g, int t, int wave, int lane, TileIn &in) {
+    // The picture of tile t (tile0 ascending).
+    int lo = 0, hi = g.n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (g.tile0[mid] <= t)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const ProposePic P = g.pics[lo];
+    const int mb_w = (P.W + 15) >> 4, n_mbs = mb_w * ((P.H + 15) >> 4);
+    const int mb0 = (t - g.tile0[lo]) * IT, count = min(IT, n_mbs - mb0);
+    in.count = count;
+    in.qp = P.qp;
+    in.orow0 = (long)P.row0 + mb0;
+    const int m = lane >> 2, q = lane & 3, rr = 16 * wave + m;
+    const bool valid = rr < count;
+    const int mb = mb0 + rr, bx = valid ? mb % mb_w : 0, by = valid ? mb / mb_w : 0;
+    const int dy0 = q == 0 ? 0 : 1 + 3 * q;
+    const uint8_t *wb = P.mprev + (long)(4 * by + dy0) * P.mw + 4 * bx;
+#pragma unroll
+    for (int j = 0; j < 7; j++) {
+        const uint32_t *wr = (const uint32_t *)(wb + (long)min(j, 15 - dy0) * P.mw);
+#pragma unroll
+        for (int c = 0; c < 4; c++) in.win[j][c] = valid ? wr[c] : 0u;
+    }
+    const uint8_t *ob = P.mcur + (long)(4 * by + MPAD) * P.mw + 4 * bx + MPAD;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const uint8_t *o = ob + (long)r * P.mw;
+        in.own[r] = valid ? (uint32_t)o[0] | ((uint32_t)o[1] << 8) | ((uint32_t)o[2] << 16) | ((uint32_t)o[3] << 24) : 0u;
+    }
+    const int x0 = bx * 16, y0 = by * 16;
+    const bool fast = x0 + 15 < P.W && ((((uintptr_t)P.cur + x0) | (uintptr_t)P.cs) & 3) == 0;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const uint8_t *row = P.cur + (long)clampi(y0 + 4 * q + r, 0, P.H - 1) * P.cs;
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            uint32_t v = 0;
+            if (valid) {
+                if (fast) {
+                    v = *(const uint32_t *)(row + x0 + 4 * c);
+                } else {
+                    for (int i = 0; i < 4; i++) v |= (uint32_t)row[min(x0 + 4 * c + i, P.W - 1)] << (8 * i);
+                }
+            }
+            in.src[r][c] = v;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256, 2) void k_propose(ProposeArgs g) {
+    extern __shared__ __align__(16) uint8_t lds_raw[];
+    ProposeLds &S = *(ProposeLds *)lds_raw;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    long b1[F_IN / 32][2], b2[HID / 32][2], b3[HID / 32][3];
+    load_frags<F_IN, 2>(g.w8 + Q_W1, 2 * wave, 2, b1, lane);
+    load_frags<HID, 2>(g.w8 + Q_W2, 2 * wave, 2, b2, lane);
+    load_frags<HID, 3>(g.w8 + Q_W3, 3 * wave, wave == 3 ? 2 : 3, b3, lane);
+    for (int i = tid; i < 3 * OUT; i += 256) {
+        const int l = i / OUT, n = i % OUT;
+        S.scale[l][n] = g.qs[i];
+        const int boff = l == 0 ? OFF_B1 : (l == 1 ? OFF_B2 : OFF_B3);
+        S.bias[l][n] = n < (l == 2 ? OUT : HID) ? g.bias[boff + n] : 0.0f;
+    }
+    if (tid < 3) S.mul[tid] = g.qs[3 * OUT + tid];
+    __syncthreads();
+    const float mul0 = S.mul[0];
+    for (int v = tid; v < 2048; v += 256) S.code[v] = feature_code((float)v * (1.0f / 256.0f), mul0);
+    __syncthreads();
+    const int ntiles = g.tile0[g.n];
+    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        TileIn in;
+        tile_inputs(g, t, wave, lane, in);
+        // ---- Features of rows 16 wave .. 16 wave + 15, four lanes a macroblock: lane 4 m + q of
+        // wave w computes row 16 w + m's cost-volume entries of displacement rows dy in
+        // [dy0(q), dy0(q + 1)) (dy0 = 0, 4, 7, 10, 13) from the previous picture's window cells
+        // (4 bx - 6 + c, 4 by - 6 + r) = cells (4 bx + c, 4 by + r) of the offset mean image, held
+        // in registers (window rows dy0 .. dy0 + 6), the macroblock's own cells (4 bx + c, 4 by +
+        // r) of the current mean image, and a quarter of the mean absolute deviation (source rows
+        // 4 q .. 4 q + 3, reduced over the quad).
+        const int count = in.count;
+        const long orow0 = in.orow0;
+        const int m = lane >> 2, q = lane & 3, rr = 16 * wave + m;
+        const bool valid = rr < count;
+        const int dy0 = q == 0 ? 0 : 1 + 3 * q;
+        const uint32_t(&win)[7][4] = in.win;
+        const uint32_t(&own)[4] = in.own;
+        const uint32_t(&src)[4][4] = in.src;
+        uint8_t *xr = S.x + rr * XS;
+        // Mean and mean absolute deviation of the macroblock (as k_features: rounded / 256).
+        uint32_t sum = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) sum = __builtin_amdgcn_sad_u8(src[r][c], 0u, sum);
+        sum += __builtin_amdgcn_update_dpp(0, (int)sum, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+        sum += __builtin_amdgcn_update_dpp(0, (int)sum, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+        const uint32_t mean = (sum + 128) >> 8, mean4 = mean * 0x01010101u;
+        uint32_t dev = 0;
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+#pragma unroll
+            for (int c = 0; c < 4; c++) dev = __builtin_amdgcn_sad_u8(src[r][c], mean4, dev);
+        dev += __builtin_amdgcn_update_dpp(0, (int)dev, 0xB1, 0xF, 0xF, false);
+        dev += __builtin_amdgcn_update_dpp(0, (int)dev, 0x4E, 0xF, 0xF, false);
+        const uint32_t mad = (dev + 128) >> 8;
+        // Cost volume: entry (dy0 + t) * 13 + dx for t < 4 (lane q 0) or 3, dx 0..12.
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            if (t == 3 && q != 0) break;
+#pragma unroll
+            for (int dx = 0; dx < 13; dx++) {
+                const int d = dx >> 2, bsh = dx & 3;
+                uint32_t acc = 0;
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const uint32_t w = d + 1 < 4 ? __builtin_amdgcn_alignbyte(win[t + r][d + 1], win[t + r][d], (uint32_t)bsh)
+                                                 : (bsh ? __builtin_amdgcn_alignbyte(0u, win[t + r][d], (uint32_t)bsh)
+                                                        : win[t + r][d]);
+                    acc = __builtin_amdgcn_sad_u8(own[r], w, acc);
+                }
+                if (valid) xr[(dy0 + t) * 13 + dx] = S.code[min(acc, 2047u)];
+            }
+        }
+        if (valid) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint32_t oq = q == 0 ? own[0] : q == 1 ? own[1] : q == 2 ? own[2] : own[3];
+                xr[NCLS + 4 * q + i] = feature_code((float)((oq >> (8 * i)) & 0xFF) * (1.0f / 64.0f), mul0);
+            }
+            if (q == 0) {
+                xr[NCLS + 16] = feature_code((float)mad * (1.0f / 16.0f), mul0);
+                xr[NCLS + 17] = feature_code((float)in.qp * (1.0f / 32.0f), mul0);
+            } else if (q == 1) {
+#pragma unroll
+                for (int i = 18; i < F_IN - NCLS; i++) xr[NCLS + i] = 0;
+            }
+        } else {
+            // Rows past the picture: zero inputs, outputs discarded.
+#pragma unroll
+            for (int i = 0; i < 12; i++) *(uint32_t *)&xr[48 * q + 4 * i] = 0;
+        }
+        __syncthreads();
+        fp8_layer_r<F_IN, XS, 2, HS>(S.x, b1, 2 * wave, 2, S.scale[0], S.bias[0], S.mul[1], S.h1, nullptr, lane);
+        __syncthreads();
+        fp8_layer_r<HID, HS, 2, HS>(S.h1, b2, 2 * wave, 2, S.scale[1], S.bias[1], S.mul[2], S.h2, nullptr, lane);
+        __syncthreads();
+        fp8_layer_r<HID, HS, 3, HS>(S.h2, b3, 3 * wave, wave == 3 ? 2 : 3, S.scale[2], S.bias[2], 0.0f, nullptr, S.z,
+                                    lane);
+        __syncthreads();
+        if (g.logits)
+            for (int e = tid; e < count * OUT; e += 256) g.logits[(orow0 + e / OUT) * OUT + e % OUT] = S.z[(e / OUT) * ZS + e % OUT];
+        // The four largest displacement logits of each row (as k_infer).
+        {
+            const int rr = tid >> 2, q = tid & 3;
+            const float *zz = S.z + rr * ZS;
+            float v0 = -INFINITY, v1 = -INFINITY, v2 = -INFINITY, v3 = -INFINITY;
+            int i0 = -1, i1 = -1, i2 = -1, i3 = -1;
+            auto insert = [&](float v, int i) {
+                if (!(v > v3)) return;
+                v3 = v;
+                i3 = i;
+                if (v3 > v2) {
+                    v3 = v2; i3 = i2; v2 = v; i2 = i;
+                    if (v2 > v1) {
+                        v2 = v1; i2 = i1; v1 = v; i1 = i;
+                        if (v1 > v0) {
+                            v1 = v0; i1 = i0; v0 = v; i0 = i;
+                        }
+                    }
+                }
+            };
+            // A quarter of the row (43 logits, 40 in the last quarter), eight loads at a time.
+            const int lo4 = q * 43, hi4 = min(NCLS, lo4 + 43);
+#pragma unroll
+            for (int b0 = 0; b0 < 48; b0 += 8) {
+                float v[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) v[k] = lo4 + b0 + k < hi4 ? zz[lo4 + b0 + k] : -INFINITY;
+#pragma unroll
+                for (int k = 0; k < 8; k++)
+                    if (lo4 + b0 + k < hi4) insert(v[k], lo4 + b0 + k);
+            }
+            S.tv[rr][q][0] = v0; S.tv[rr][q][1] = v1; S.tv[rr][q][2] = v2; S.tv[rr][q][3] = v3;
+            S.ti[rr][q][0] = i0; S.ti[rr][q][1] = i1; S.ti[rr][q][2] = i2; S.ti[rr][q][3] = i3;
+            __syncthreads();
+            if (q == 0 && rr < count) {
+                v0 = v1 = v2 = v3 = -INFINITY;
+                i0 = i1 = i2 = i3 = -1;
+                for (int qq = 0; qq < 4; qq++)
+                    for (int k = 0; k < 4; k++)
+                        if (S.ti[rr][qq][k] >= 0) insert(S.tv[rr][qq][k], S.ti[rr][qq][k]);
+                const int c0 = i0 < 0 ? 84 : i0, c1 = i1 < 0 ? 84 : i1, c2 = i2 < 0 ? 84 : i2, c3 = i3 < 0 ? 84 : i3;
+                rgh_proposal o;
+                o.cell[0][0] = (int8_t)(c0 % 13 - 6);
+                o.cell[0][1] = (int8_t)(c0 / 13 - 6);
+                o.cell[1][0] = (int8_t)(c1 % 13 - 6);
+                o.cell[1][1] = (int8_t)(c1 / 13 - 6);
+                o.cell[2][0] = (int8_t)(c2 % 13 - 6);
+                o.cell[2][1] = (int8_t)(c2 / 13 - 6);
+                o.cell[3][0] = (int8_t)(c3 % 13 - 6);
+                o.cell[3][1] = (int8_t)(c3 / 13 - 6);
+                const float m0 = zz[MODE0], m1 = zz[MODE0 + 1], m2 = zz[MODE0 + 2], mx = fmaxf(m0, fmaxf(m1, m2));
+                const float e0 = expf(m0 - mx), e1 = expf(m1 - mx), e2 = expf(m2 - mx), inv = 255.0f / (e0 + e1 + e2);
+                o.mode_prob[0] = (uint8_t)(e0 * inv + 0.5f);
+                o.mode_prob[1] = (uint8_t)(e1 * inv + 0.5f);
+                o.mode_prob[2] = (uint8_t)(e2 * inv + 0.5f);
+                for (int k = 0; k < 5; k++) o.reserved[k] = 0;
+                g.out[orow0 + rr] = o;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------------------
 // Host side.
 rgh_status hip_status(hipError_t e) {
     if (e == hipSuccess) return RGH_OK;
@@ -663,6 +998,70 @@ hipError_t backward(Pnet *p, const uint16_t *X, const rgh_pnet_label *L, int B) 
     return colsum(p, p->dh1f, B, HID, p->grads + OFF_B1);
 }
 
+// Grows a device buffer to at least `bytes`.
+hipError_t grow(void **buf, size_t *cap, size_t bytes) {
+    if (*cap >= bytes) return hipSuccess;
+    if (*buf) {
+        hipError_t e = hipFree(*buf);
+        if (e != hipSuccess) return e;
+    }
+    *buf = nullptr;
+    *cap = 0;
+    hipError_t e = hipMalloc(buf, bytes);
+    if (e == hipSuccess) *cap = bytes;
+    return e;
+}
+
+// Checks a batch of pictures, then computes one 4x4-mean image per distinct (plane, stride, size)
+// into p->means (k_means, enqueued on p->stream); mean_of[i] receives the (cur, prev) mean
+// planes of picture i.
+rgh_status prepare_means(Pnet *p, const rgh_pnet_picture *pics, int32_t n, bool need_features,
+                         std::vector<std::pair<MeanPlane, MeanPlane>> &mean_of, int &max_mbs) {
+    max_mbs = 0;
+    for (int i = 0; i < n; i++) {
+        const rgh_pnet_picture &q = pics[i];
+        if (!q.cur_y || !q.prev_y || (need_features && !q.features)) FAIL(RGH_ERR_INVALID_ARGUMENT, "null pointer");
+        if (q.width < 1 || q.height < 1 || q.width > 16384 || q.height > 16384 || q.cur_stride < q.width ||
+            q.prev_stride < q.width)
+            FAIL(RGH_ERR_INVALID_DIMENSIONS, "bad picture size or stride");
+        if (q.qp < 0 || q.qp > 51) FAIL(RGH_ERR_INVALID_ARGUMENT, "qp outside 0..51");
+        max_mbs = std::max(max_mbs, ((q.width + 15) / 16) * ((q.height + 15) / 16));
+    }
+    if (n == 0) return RGH_OK;
+    TRY(hipSetDevice(p->device));
+    std::vector<MeanPlane> planes;
+    std::vector<size_t> offs;
+    size_t total = 0;
+    int max_cells = 0;
+    auto plane_of = [&](const uint8_t *y, int stride, int W, int H) {
+        for (size_t k = 0; k < planes.size(); k++)
+            if (planes[k].y == y && planes[k].stride == stride && planes[k].W == W && planes[k].H == H) return (int)k;
+        // Cells -6 .. 4 * mb + 6 in each direction: every macroblock's window and cells.
+        MeanPlane m{y, nullptr, stride, W, H, 4 * ((W + 15) / 16) + 2 * MPAD, 4 * ((H + 15) / 16) + 2 * MPAD};
+        planes.push_back(m);
+        offs.push_back(total);
+        total += (size_t)m.mw * m.mh;
+        max_cells = std::max(max_cells, m.mw * m.mh);
+        return (int)planes.size() - 1;
+    };
+    std::vector<std::pair<int, int>> pair_of(n);
+    for (int i = 0; i < n; i++)
+        pair_of[i] = {plane_of(pics[i].cur_y, pics[i].cur_stride, pics[i].width, pics[i].height),
+                      plane_of(pics[i].prev_y, pics[i].prev_stride, pics[i].width, pics[i].height)};
+    if (planes.size() > 65535) FAIL(RGH_ERR_INVALID_ARGUMENT, "too many distinct planes");
+    TRY(grow((void **)&p->means, &p->means_cap, total));
+    for (size_t k = 0; k < planes.size(); k++) planes[k].m = p->means + offs[k];
+    mean_of.resize((size_t)n);
+    for (int i = 0; i < n; i++) mean_of[(size_t)i] = {planes[(size_t)pair_of[i].first], planes[(size_t)pair_of[i].second]};
+    TRY(grow(&p->mplanes, &p->mplanes_cap, sizeof(MeanPlane) * planes.size()));
+    TRY(hipMemcpyAsync(p->mplanes, planes.data(), sizeof(MeanPlane) * planes.size(), hipMemcpyHostToDevice,
+                       p->stream));
+    hipLaunchKernelGGL(k_means, dim3((max_cells + 255) / 256, (unsigned)planes.size()), dim3(256), 0, p->stream,
+                       (const MeanPlane *)p->mplanes);
+    TRY(hipGetLastError());
+    return RGH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -744,76 +1143,66 @@ rgh_status rgh_pnet_get_params(void *net, float *params, int64_t n) {
     return RGH_OK;
 }
 
-// Grows a device buffer to at least `bytes`.
-static hipError_t grow(void **buf, size_t *cap, size_t bytes) {
-    if (*cap >= bytes) return hipSuccess;
-    if (*buf) {
-        hipError_t e = hipFree(*buf);
-        if (e != hipSuccess) return e;
-    }
-    *buf = nullptr;
-    *cap = 0;
-    hipError_t e = hipMalloc(buf, bytes);
-    if (e == hipSuccess) *cap = bytes;
-    return e;
-}
-
 rgh_status rgh_pnet_features_batch(void *net, const rgh_pnet_picture *pics, int32_t n) {
     Pnet *p = (Pnet *)net;
     if (!p || !pics || n < 0 || n > 65535) FAIL(RGH_ERR_INVALID_ARGUMENT, "null pointer or bad picture count");
+    std::vector<std::pair<MeanPlane, MeanPlane>> mean_of;
     int max_mbs = 0;
-    for (int i = 0; i < n; i++) {
-        const rgh_pnet_picture &q = pics[i];
-        if (!q.cur_y || !q.prev_y || !q.features) FAIL(RGH_ERR_INVALID_ARGUMENT, "null pointer");
-        if (q.width < 1 || q.height < 1 || q.width > 16384 || q.height > 16384 || q.cur_stride < q.width ||
-            q.prev_stride < q.width)
-            FAIL(RGH_ERR_INVALID_DIMENSIONS, "bad picture size or stride");
-        if (q.qp < 0 || q.qp > 51) FAIL(RGH_ERR_INVALID_ARGUMENT, "qp outside 0..51");
-        max_mbs = std::max(max_mbs, ((q.width + 15) / 16) * ((q.height + 15) / 16));
-    }
-    if (n == 0) return RGH_OK;
-    TRY(hipSetDevice(p->device));
-    // One mean image per distinct (plane, stride, size).
-    std::vector<MeanPlane> planes;
-    std::vector<size_t> offs;
-    size_t total = 0;
-    int max_cells = 0;
-    auto plane_of = [&](const uint8_t *y, int stride, int W, int H) {
-        for (size_t k = 0; k < planes.size(); k++)
-            if (planes[k].y == y && planes[k].stride == stride && planes[k].W == W && planes[k].H == H) return (int)k;
-        // Cells -6 .. 4 * mb + 6 in each direction: every macroblock's window and cells.
-        MeanPlane m{y, nullptr, stride, W, H, 4 * ((W + 15) / 16) + 2 * MPAD, 4 * ((H + 15) / 16) + 2 * MPAD};
-        planes.push_back(m);
-        offs.push_back(total);
-        total += (size_t)m.mw * m.mh;
-        max_cells = std::max(max_cells, m.mw * m.mh);
-        return (int)planes.size() - 1;
-    };
-    std::vector<std::pair<int, int>> pair_of(n);
-    for (int i = 0; i < n; i++)
-        pair_of[i] = {plane_of(pics[i].cur_y, pics[i].cur_stride, pics[i].width, pics[i].height),
-                      plane_of(pics[i].prev_y, pics[i].prev_stride, pics[i].width, pics[i].height)};
-    if (planes.size() > 65535) FAIL(RGH_ERR_INVALID_ARGUMENT, "too many distinct planes");
-    TRY(grow((void **)&p->means, &p->means_cap, total));
-    for (size_t k = 0; k < planes.size(); k++) planes[k].m = p->means + offs[k];
+    rgh_status st = prepare_means(p, pics, n, true, mean_of, max_mbs);
+    if (st != RGH_OK || n == 0) return st;
     std::vector<FeatPic> fp(n);
     for (int i = 0; i < n; i++) {
-        const MeanPlane &c = planes[(size_t)pair_of[i].first], &q = planes[(size_t)pair_of[i].second];
+        const MeanPlane &c = mean_of[(size_t)i].first, &q = mean_of[(size_t)i].second;
         fp[i] = FeatPic{pics[i].cur_y, c.m, q.m, pics[i].features, pics[i].cur_stride, pics[i].width, pics[i].height,
                         c.mw, pics[i].qp};
     }
-    TRY(grow(&p->mplanes, &p->mplanes_cap, sizeof(MeanPlane) * planes.size()));
     TRY(grow(&p->fpics, &p->fpics_cap, sizeof(FeatPic) * (size_t)n));
-    TRY(hipMemcpyAsync(p->mplanes, planes.data(), sizeof(MeanPlane) * planes.size(), hipMemcpyHostToDevice,
-                       p->stream));
     TRY(hipMemcpyAsync(p->fpics, fp.data(), sizeof(FeatPic) * (size_t)n, hipMemcpyHostToDevice, p->stream));
-    hipLaunchKernelGGL(k_means, dim3((max_cells + 255) / 256, (unsigned)planes.size()), dim3(256), 0, p->stream,
-                       (const MeanPlane *)p->mplanes);
-    TRY(hipGetLastError());
     const long items = (long)n * max_mbs;
     const long groups = (items + 3) / 4;
     hipLaunchKernelGGL(k_features, dim3((unsigned)std::min<long>(groups, 1L << 30)), dim3(256), 0, p->stream,
                        (const FeatPic *)p->fpics, max_mbs, items);
+    TRY(hipGetLastError());
+    TRY(hipStreamSynchronize(p->stream));
+    return RGH_OK;
+}
+
+rgh_status rgh_pnet_propose_batch(void *net, const rgh_pnet_picture *pics, int32_t n, rgh_proposal *proposals,
+                                  float *logits) {
+    Pnet *p = (Pnet *)net;
+    if (!p || !pics || !proposals || n < 0 || n > 65535) FAIL(RGH_ERR_INVALID_ARGUMENT, "null pointer or bad picture count");
+    if (!p->quantized) FAIL(RGH_ERR_INVALID_ARGUMENT, "rgh_pnet_quantize has not run since the parameters changed");
+    std::vector<std::pair<MeanPlane, MeanPlane>> mean_of;
+    int max_mbs = 0;
+    rgh_status st = prepare_means(p, pics, n, false, mean_of, max_mbs);
+    if (st != RGH_OK || n == 0) return st;
+    // Descriptors, then the tile table (first tile of each picture, and the count) behind them.
+    std::vector<ProposePic> pp((size_t)n);
+    std::vector<int> tile0((size_t)n + 1);
+    int rows = 0, tiles = 0;
+    for (int i = 0; i < n; i++) {
+        const MeanPlane &c = mean_of[(size_t)i].first, &q = mean_of[(size_t)i].second;
+        const int mbs = ((pics[i].width + 15) / 16) * ((pics[i].height + 15) / 16);
+        pp[(size_t)i] = ProposePic{pics[i].cur_y, c.m, q.m, pics[i].cur_stride, pics[i].width, pics[i].height, c.mw,
+                                   pics[i].qp, rows};
+        tile0[(size_t)i] = tiles;
+        rows += mbs;
+        tiles += (mbs + IT - 1) / IT;
+    }
+    tile0[(size_t)n] = tiles;
+    const size_t dbytes = sizeof(ProposePic) * (size_t)n, tbytes = sizeof(int) * ((size_t)n + 1);
+    TRY(grow(&p->fpics, &p->fpics_cap, dbytes + tbytes));
+    TRY(hipMemcpyAsync(p->fpics, pp.data(), dbytes, hipMemcpyHostToDevice, p->stream));
+    TRY(hipMemcpyAsync((uint8_t *)p->fpics + dbytes, tile0.data(), tbytes, hipMemcpyHostToDevice, p->stream));
+    int cus = 0;
+    TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device));
+    int grid = std::max(1, std::min(tiles, 2 * cus));
+    if (p->launch_limit > 0) grid = std::min(grid, p->launch_limit);
+    ProposeArgs g{(const ProposePic *)p->fpics, (const int *)((uint8_t *)p->fpics + dbytes), n, p->w8, p->qs,
+                  p->params, proposals, logits};
+    TRY(hipFuncSetAttribute((const void *)k_propose, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)sizeof(ProposeLds)));
+    hipLaunchKernelGGL(k_propose, dim3(grid), dim3(256), sizeof(ProposeLds), p->stream, g);
     TRY(hipGetLastError());
     TRY(hipStreamSynchronize(p->stream));
     return RGH_OK;

@@ -34,7 +34,8 @@ assert PROPOSAL_DTYPE.itemsize == 16
 EXPORTS = ["rgh_pnet_create", "rgh_pnet_destroy", "rgh_pnet_set_launch_limit", "rgh_pnet_set_params", "rgh_pnet_get_params",
            "rgh_pnet_features", "rgh_pnet_features_batch", "rgh_pnet_labels", "rgh_pnet_gather", "rgh_pnet_train_step",
            "rgh_pnet_gradients", "rgh_pnet_forward", "rgh_pnet_quantize", "rgh_pnet_quantize_ranges", "rgh_pnet_infer",
-           "rgh_pnet_get_quantized", "rgh_pnet_fp8_encode", "rgh_pnet_gemm_bf16", "rgh_analyse_frames_learned"]
+           "rgh_pnet_get_quantized", "rgh_pnet_fp8_encode", "rgh_pnet_gemm_bf16", "rgh_pnet_propose_batch",
+           "rgh_analyse_frames_learned"]
 
 _bound = False
 
@@ -67,6 +68,7 @@ def _bind():
         "rgh_pnet_quantize": [P, P, I32],
         "rgh_pnet_quantize_ranges": [P, F32, F32],
         "rgh_pnet_infer": [P, P, I32, P, P],
+        "rgh_pnet_propose_batch": [P, P, I32, P, P],
         "rgh_pnet_get_quantized": [P, P, P],
         "rgh_pnet_fp8_encode": [P, P, I32, P],
         "rgh_pnet_gemm_bf16": [P, P, I64, I64, P, I64, I64, P, P, I64, P, P, I32, I32, I32, I32, I32],
@@ -136,6 +138,16 @@ class ProposalNet:
         for i, (c, cs, p, ps, w, h, qp, out) in enumerate(pictures):
             arr[i] = RgPnetPicture(c, p, out, cs, ps, w, h, qp, 0)
         _check(self.L.rgh_pnet_features_batch(self.h, arr, len(pictures)), "rgh_pnet_features_batch")
+
+    def propose_batch(self, pictures, proposals_dev, logits_dev=None):
+        """Features and FP8 inference in one pass. pictures: (cur_y_dev, cur_stride, prev_y_dev,
+        prev_stride, width, height, qp) tuples; their proposals fill consecutive rows of
+        proposals_dev, picture after picture."""
+        arr = (RgPnetPicture * max(1, len(pictures)))()
+        for i, (c, cs, p, ps, w, h, qp) in enumerate(pictures):
+            arr[i] = RgPnetPicture(c, p, None, cs, ps, w, h, qp, 0)
+        _check(self.L.rgh_pnet_propose_batch(self.h, arr, len(pictures), proposals_dev, logits_dev),
+               "rgh_pnet_propose_batch")
 
     def labels(self, decisions_dev, n, out_dev):
         _check(self.L.rgh_pnet_labels(self.h, decisions_dev, n, out_dev), "rgh_pnet_labels")
@@ -263,32 +275,27 @@ def load_trained(device=0, max_batch=4096, path=WEIGHTS):
 
 
 class Proposer:
-    """Learned proposals for batches of device pictures: their features in one launch, then FP8
-    inference over all their macroblocks in one launch, into buffers kept for reuse."""
+    """Learned proposals for batches of device pictures: their features and FP8 inference over all
+    their macroblocks in one pass (rgh_pnet_propose_batch), into a buffer kept for reuse."""
 
     def __init__(self, net, max_rows):
         self.net = net
         self.max_rows = max_rows
-        self.feat = net.hip.alloc(max_rows * FEATURES * 2)
         self.props = net.hip.alloc(max_rows * 16)
 
     def close(self):
-        if self.feat:
-            self.net.hip.free(self.feat)
+        if self.props:
             self.net.hip.free(self.props)
-            self.feat = self.props = None
+            self.props = None
 
     def run(self, pictures):
         """pictures: (cur_y_dev, cur_stride, prev_y_dev, prev_stride, width, height, qp) tuples.
         Returns the device address of each picture's proposal records (valid until the next run)."""
-        rows, jobs, out = 0, [], []
+        rows, out = 0, []
         for (c, cs, p, ps, w, h, qp) in pictures:
-            n = ((w + 15) // 16) * ((h + 15) // 16)
-            jobs.append((c, cs, p, ps, w, h, qp, self.feat + rows * FEATURES * 2))
             out.append(self.props + rows * 16)
-            rows += n
+            rows += ((w + 15) // 16) * ((h + 15) // 16)
         if rows > self.max_rows:
             raise ValueError("%d macroblocks exceed the proposer's %d rows" % (rows, self.max_rows))
-        self.net.features_batch(jobs)
-        self.net.infer(self.feat, rows, self.props)
+        self.net.propose_batch(pictures, self.props)
         return out

@@ -203,6 +203,46 @@ def test_quantize_and_fp8_inference_match_numpy(net):
     assert np.abs(z - zb).max() <= 0.1 * np.abs(zb).max()
 
 
+def test_fused_proposals_equal_features_then_inference():
+    # rgh_pnet_propose_batch (features computed into the inference tiles) against
+    # rgh_pnet_features_batch + rgh_pnet_infer: the same proposals and logits, bit for bit, for a
+    # batch of pictures whose macroblock counts leave partial tiles (and one picture of one MB).
+    from ragdoll_amd.learned import PROPOSAL_DTYPE, load_trained
+    net = load_trained(max_batch=1024)
+    hip = net.hip
+    rng = np.random.default_rng(77)
+    sizes = [(144, 176), (90, 160), (16, 16), (200, 120), (48, 64)]
+    pics, bufs = [], []
+    try:
+        for k, (H, W) in enumerate(sizes):
+            cur, prev = synth_pair(rng, H, W, (k - 2, 3 - k))
+            dc, dp = hip.upload(cur), hip.upload(prev)
+            bufs += [dc, dp]
+            pics.append((dc, W, dp, W, W, H, 20 + 3 * k))
+        rows = sum(((W + 15) // 16) * ((H + 15) // 16) for H, W in sizes)
+        feat = hip.alloc(rows * 192 * 2)
+        p1, p2 = hip.alloc(rows * 16), hip.alloc(rows * 16)
+        z1, z2 = hip.alloc(rows * 176 * 4), hip.alloc(rows * 176 * 4)
+        bufs += [feat, p1, p2, z1, z2]
+        jobs, r = [], 0
+        for (c, cs, p, ps, w, h, qp) in pics:
+            jobs.append((c, cs, p, ps, w, h, qp, feat + r * 192 * 2))
+            r += ((w + 15) // 16) * ((h + 15) // 16)
+        net.features_batch(jobs)
+        net.infer(feat, rows, p1, z1)
+        net.propose_batch(pics, p2, z2)
+        a = hip.download(p1, np.zeros(rows, PROPOSAL_DTYPE))
+        b = hip.download(p2, np.zeros(rows, PROPOSAL_DTYPE))
+        za = hip.download(z1, np.zeros((rows, 176), np.float32))
+        zb = hip.download(z2, np.zeros((rows, 176), np.float32))
+        assert np.array_equal(za.view(np.uint32), zb.view(np.uint32))
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8))
+    finally:
+        for q in bufs:
+            hip.free(q)
+        net.close()
+
+
 def test_training_learns_a_known_displacement(net):
     """Pictures displaced by a whole number of cells: the network must learn to point there."""
     from ragdoll_amd.learned import LABEL_DTYPE, init_params
