@@ -258,6 +258,13 @@ rgh_status rgh_analyse_batch(void *const *ctxs, int32_t n, const rgh_i420 *src_d
  * (RGH_ERR_INVALID_ARGUMENT): the graph is allocated. A schedule's graph is kept by its first
  * context until that context is destroyed or captures another. */
 #define RGH_FRAMES_GRAPH 4
+/* rgh_analyse_frames: RGH_FRAMES_KEYINT(k) with RGH_FRAMES_FIRST_IDR, 1 <= k <= 32767: frames 0, k,
+ * 2k, ... of every context are IDR pictures (H264Encoder's keyint, h264.rs:3032-3065: the frame
+ * loop of encode_frame_body, enc.rs:2234, codes frame_count % keyint == 0 as an IDR), so a context
+ * runs several GOPs back to back in one call, each GOP's frames following the last ones of the
+ * GOP before at the call's lag. Every GOP's results equal those of the GOP analysed alone from its
+ * IDR. Such calls run on the per-step schedule (RGH_FRAMES_PERSISTENT is ignored). */
+#define RGH_FRAMES_KEYINT(k) ((int32_t)(k) << 16)
 rgh_status rgh_analyse_frames(void *const *ctxs, int32_t n, int32_t n_frames, const rgh_i420 *src_device,
                               rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag,
                               int32_t groups, int32_t flags);

@@ -251,7 +251,7 @@ class RgLearnedConfig(ctypes.Structure):
 
 
 def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0, first_idr=False, proposals=None,
-                   learned=None, noalloc=False, persistent=False, graph=False):
+                   learned=None, noalloc=False, persistent=False, graph=False, keyint=0):
     """n_frames consecutive P frames of every analyzer in one pipelined schedule.
 
     device_frames[i][k]: (y_ptr, u_ptr, v_ptr, stride_y, stride_u, stride_v) of frame k of
@@ -262,7 +262,8 @@ def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0,
     noalloc: the plain form through rgh_analyse_frames_noalloc (no allocation by the library).
     persistent: one persistent launch (RGH_FRAMES_PERSISTENT) instead of the per-step launches.
     graph: the planned per-step launches as one captured HIP graph (RGH_FRAMES_GRAPH), replayed by
-    later calls with the same launches."""
+    later calls with the same launches. keyint (with first_idr): frames 0, keyint, 2 keyint, ... are
+    IDR pictures (RGH_FRAMES_KEYINT), several GOPs of each analyzer back to back in one call."""
     L = load_library()
     n = len(analyzers)
     nf = len(device_frames[0])
@@ -278,7 +279,7 @@ def analyse_frames(analyzers, device_frames, decisions, levels, lag=0, groups=0,
             dp[i * nf + k] = decisions[i][k]
             lp[i * nf + k] = levels[i][k]
     flags = ((FRAMES_FIRST_IDR if first_idr else 0) | (FRAMES_PERSISTENT if persistent else 0) |
-             (FRAMES_GRAPH if graph else 0))
+             (FRAMES_GRAPH if graph else 0) | (int(keyint) << 16))
     if proposals is None:
         entry = "rgh_analyse_frames_noalloc" if noalloc else "rgh_analyse_frames"
         _check(getattr(L, entry)(arr, n, nf, srcs, dp, lp, lag, groups, flags), entry)

@@ -287,9 +287,14 @@ struct Schedule {
     // Checks the jobs and plans the launches (span of each frame, planned launch count).
     rgh_status plan() {
         nj = n * n_frames;
-        // IDR jobs lead (frame-major order, IDR only as a stream's first frame).
+        // IDR frames: frame-major jobs, every context's frame k of one kind (IDR or P).
         n_intra = 0;
-        while (n_intra < nj && jobs[(size_t)n_intra].intra) n_intra++;
+        for (int j = 0; j < nj; j++) n_intra += jobs[(size_t)j].intra != 0;
+        for (int j = 0; j < nj; j++)
+            if ((jobs[(size_t)j].intra != 0) != (jobs[(size_t)(j - j % n)].intra != 0)) {
+                set_error("run_schedule", "a frame index must be IDR in every context or in none", __LINE__);
+                return RGH_ERR_INVALID_ARGUMENT;
+            }
         // One P kernel per schedule: every job of it uses one preset.
         compact = nj > 0 && jobs[0].compact;
         fast = nj > 0 && jobs[0].fast;
@@ -300,15 +305,15 @@ struct Schedule {
                 set_error("run_schedule", "contexts of one call must use one preset", __LINE__);
                 return RGH_ERR_INVALID_ARGUMENT;
             }
-        learned = n_intra < nj && jobs[(size_t)n_intra].prop != nullptr;
-        for (int j = n_intra; j < nj; j++)
-            if ((jobs[(size_t)j].prop != nullptr) != learned) {
-                set_error("run_schedule", "every P frame of a learned call needs proposals", __LINE__);
-                return RGH_ERR_INVALID_ARGUMENT;
+        learned = false;
+        for (int j = 0; j < nj; j++)
+            if (!jobs[(size_t)j].intra) {
+                learned = jobs[(size_t)j].prop != nullptr;
+                break;
             }
-        for (int j = n_intra; j < nj; j++)
-            if (jobs[(size_t)j].intra) {
-                set_error("run_schedule", "IDR jobs must lead the schedule", __LINE__);
+        for (int j = 0; j < nj; j++)
+            if (!jobs[(size_t)j].intra && (jobs[(size_t)j].prop != nullptr) != learned) {
+                set_error("run_schedule", "every P frame of a learned call needs proposals", __LINE__);
                 return RGH_ERR_INVALID_ARGUMENT;
             }
         span.assign((size_t)nj, 0);
@@ -351,21 +356,29 @@ struct Schedule {
                            g_seq.fetch_add(1) - seq0);
     }
     void launch(int j_lo, int j_hi) {
-        // IDR jobs (the leading n_intra) and P jobs of the launch, in one sequence number: a
+        // IDR jobs and P jobs of the launch (whole frames, frame-major), in one sequence number: a
         // stamp of one is not final for the other within the launch.
-        const int split = std::min(std::max(n_intra, j_lo), j_hi);
-        if (split > j_lo && j_hi > split && !compact) {
+        bool any_i = false, any_p = false;
+        for (int f = j_lo / n; f <= (j_hi - 1) / n; f++) (jobs[(size_t)f * n].intra ? any_i : any_p) = true;
+        StepKernel pk = learned ? rgh_k_learned_step
+                                : (compact ? rgh_k_compact_step
+                                           : (fast ? rgh_k_fast_step : (parts ? rgh_k_parts_step : rgh_k_frame_step)));
+        StepKernel ik = fast ? rgh_k_intra_fast_step : rgh_k_intra_step;
+        if (any_i && any_p && !compact) {
             // IDR and Balanced P jobs together: one kernel, so that neither waits for the other.
             emit(learned ? rgh_k_mixed_learned_step
                          : (fast ? rgh_k_mixed_fast_step : (parts ? rgh_k_mixed_parts_step : rgh_k_mixed_step)),
                  j_lo, j_hi - j_lo);
         } else {
-            if (split > j_lo) emit(fast ? rgh_k_intra_fast_step : rgh_k_intra_step, j_lo, split - j_lo);
-            if (j_hi > split)
-                emit(learned ? rgh_k_learned_step
-                             : (compact ? rgh_k_compact_step
-                                        : (fast ? rgh_k_fast_step : (parts ? rgh_k_parts_step : rgh_k_frame_step))),
-                     split, j_hi - split);
+            // One kernel per run of frames of one kind (Compact: its IDR and P kernels).
+            for (int a = j_lo; a < j_hi;) {
+                const bool intra = jobs[(size_t)a].intra != 0;
+                int b = (a / n + 1) * n;  // the end of a's frame, then of the following frames of its kind
+                while (b < j_hi && (jobs[(size_t)b].intra != 0) == intra) b += n;
+                b = std::min(b, j_hi);
+                emit(intra ? ik : pk, a, b - a);
+                a = b;
+            }
         }
         for (int j = j_lo; j < j_hi; j++) last[(size_t)j] = L;
         L++;
@@ -1176,14 +1189,19 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
                                const rgh_proposal *const *props, const rgh_learned_config *lcfg,
                                rgh_mb_decision *const *decisions, int16_t *const *levels, int32_t lag, int32_t groups,
                                int32_t flags, bool noalloc = false) {
+    const int keyint = (flags >> 16) & 0x7FFF;
     if (!ctxs || n <= 0 || n_frames <= 0 || !src || !decisions || !levels || lag < 0 || groups < 0 ||
-        (flags & ~(RGH_FRAMES_FIRST_IDR | RGH_FRAMES_PERSISTENT | RGH_FRAMES_GRAPH)))
+        (flags & ~(RGH_FRAMES_FIRST_IDR | RGH_FRAMES_PERSISTENT | RGH_FRAMES_GRAPH | RGH_FRAMES_KEYINT(0x7FFF))) ||
+        (keyint && !(flags & RGH_FRAMES_FIRST_IDR)))
         return RGH_ERR_INVALID_ARGUMENT;
     if (noalloc && (flags & RGH_FRAMES_GRAPH)) {
         set_error("rgh_analyse_frames_noalloc", "RGH_FRAMES_GRAPH allocates the graph", __LINE__);
         return RGH_ERR_INVALID_ARGUMENT;
     }
     const bool first_idr = (flags & RGH_FRAMES_FIRST_IDR) != 0;
+    // Frame k of every context is an IDR picture: the first (RGH_FRAMES_FIRST_IDR), and every
+    // keyint-th with RGH_FRAMES_KEYINT.
+    auto is_idr = [&](int k) { return first_idr && (k == 0 || (keyint && k % keyint == 0)); };
     rgh_learned_config lc{4, 0, 0, 105, 256};
     if (props) {
         if (lcfg) {
@@ -1203,8 +1221,8 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
                 set_error("rgh_analyse_frames_learned", "learned proposals serve the Balanced preset", __LINE__);
                 return RGH_ERR_UNSUPPORTED;
             }
-            for (int k = first_idr ? 1 : 0; k < n_frames; k++)
-                if (!props[(size_t)i * n_frames + k]) return RGH_ERR_INVALID_ARGUMENT;
+            for (int k = 0; k < n_frames; k++)
+                if (!is_idr(k) && !props[(size_t)i * n_frames + k]) return RGH_ERR_INVALID_ARGUMENT;
         }
     }
     if ((int64_t)n * n_frames > (1 << 20)) return RGH_ERR_INVALID_ARGUMENT;
@@ -1239,7 +1257,7 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
     // contexts and the no-allocation entry point use the per-step launch schedule.
     bool compact_any = false;  // the Compact preset and partitions have no persistent kernel
     for (int i = 0; i < n; i++) compact_any |= cs[(size_t)i]->compact || cs[(size_t)i]->parts;
-    const bool persistent = !noalloc && !compact_any && (flags & RGH_FRAMES_PERSISTENT);
+    const bool persistent = !noalloc && !compact_any && !keyint && (flags & RGH_FRAMES_PERSISTENT);
     const int G = persistent ? 1 : std::max(1, std::min(groups ? groups : 1, n));
     HIP_TRY(hipSetDevice(cs[0]->device));
     uint8_t *scr = nullptr;
@@ -1285,7 +1303,7 @@ static rgh_status analyse_frames_impl(void *const *ctxs, int32_t n, int32_t n_fr
                 const rgh_mb_decision *prev = k ? decisions[q - 1] : c->prev;
                 RghJob &j = S.jobs[(size_t)k * m + t];
                 j = make_job(c, k, src[q], decisions[q], prev, levels[q], k * lag, ctl + (size_t)k * m + t, wait,
-                             first_idr && k == 0);
+                             is_idr(k));
                 if (props && !j.intra) {
                     j.prop = props[q];
                     j.learn_iters = std::min(lc.iters, c->me_iters);

@@ -239,3 +239,76 @@ def test_failed_call_leaves_context_without_reference(monkeypatch):
         a.close()
         for b in bufs:
             hip.free(b)
+
+
+def run_gops(streams, keyint, lag, ring, groups=0, label=""):
+    """streams: (frames, w, h, qp, profile, me_range, speed). Every stream's frames run in one
+    rgh_analyse_frames call with an IDR every `keyint` frames (RGH_FRAMES_KEYINT); each GOP must
+    equal the oracle coding that GOP alone from its IDR (a fresh encoder per GOP), and the last
+    deblocked picture the oracle's last."""
+    hip = Hip()
+    ans, refs, bufs = [], [], []
+    try:
+        for frames, w, h, qp, profile, me_range, speed in streams:
+            want, final = [], None
+            for g0 in range(0, len(frames), keyint):
+                ora = OracleEncoder(w, h, qp, profile=profile, keyint=1 << 20, me_range=me_range, speed=speed)
+                for f in frames[g0:g0 + keyint]:
+                    ora.encode(*f)
+                    want.append((ora.decisions(), ora.levels()))
+                final = ora.recon(deblocked=True)
+            a = StreamAnalyzer(w, h, qp, profile=profile, me_range=me_range, keep_undeblocked=False, ring=ring,
+                               speed=speed)
+            n = a.mb_w * a.mb_h
+            srcs, decs, lvs = [], [], []
+            for y, u, v in frames:
+                dy, du, dv = hip.upload(y), hip.upload(u), hip.upload(v)
+                bufs += [dy, du, dv]
+                srcs.append((dy, du, dv, y.shape[1], u.shape[1], v.shape[1]))
+                d, l = hip.alloc(n * 32), hip.alloc(n * LEVELS_PER_MB * 2)
+                hip.memset(d, 0xEE, n * 32)
+                bufs += [d, l]
+                decs.append(d)
+                lvs.append(l)
+            ans.append(a)
+            refs.append((want, final, srcs, decs, lvs, n))
+        print("keyint %d %s" % (keyint, label), flush=True)
+        analyse_frames(ans, [r[2] for r in refs], [r[3] for r in refs], [r[4] for r in refs], lag=lag, groups=groups,
+                       first_idr=True, keyint=keyint)
+        for s, (a, (want, final, srcs, decs, lvs, n)) in enumerate(zip(ans, refs)):
+            for k in range(len(want)):
+                dec = hip.download(decs[k], np.zeros(n, DECISION_DTYPE))
+                lv = hip.download(lvs[k], np.zeros((n, LEVELS_PER_MB), np.int16))
+                odec, olv = want[k]
+                bad = np.nonzero(dec.view(np.uint8).reshape(-1, 32) != odec.view(np.uint8).reshape(-1, 32))[0]
+                assert len(bad) == 0, "%s stream %d frame %d: decision of MB %d differs\n gpu %s\n ora %s" % (
+                    label, s, k, bad[0], dec[bad[0]], odec[bad[0]])
+                badl = np.nonzero((lv != olv).any(axis=1))[0]
+                assert len(badl) == 0, "%s stream %d frame %d: levels of MB %d differ" % (label, s, k, badl[0])
+            _, _, rec = a.outputs(which_recon=1, want_levels=False)
+            for p, (g, o) in enumerate(zip(rec, final)):
+                assert np.array_equal(g, o), "%s stream %d: final deblocked plane %d differs" % (label, s, p)
+    finally:
+        for a in ans:
+            a.close()
+        for b in bufs:
+            hip.free(b)
+
+
+@pytest.mark.parametrize("lag,groups", [(20, 1), (3, 2)])
+def test_gops_back_to_back(lag, groups):
+    # Three GOPs of 4 frames per stream in one call (IDRs at frames 0, 4, 8), two streams of
+    # different profiles; a small lag makes frames of consecutive GOPs share launches with
+    # deferrals, so the mixed IDR + P kernel runs IDR frames in the middle of the schedule.
+    w, h, cif = scene("cif", 12, 21)
+    fx = [fixture_frame(160, 90, i) for i in range(12)]
+    run_gops([(cif, w, h, 26, "high", 16, "balanced"), (fx, 160, 90, 20, "baseline", 16, "balanced")], keyint=4,
+             lag=lag, ring=6, groups=groups, label="lag %d groups %d" % (lag, groups))
+
+
+@pytest.mark.parametrize("speed,profile", [("fast", "high"), ("partitions", "main"), ("compact", "baseline")])
+def test_gops_back_to_back_presets(speed, profile):
+    # The other presets' kernels across mid-call IDRs (Compact runs its IDR and P frames as separate
+    # kernels within a launch).
+    w, h, cif = scene("cif", 9, 22)
+    run_gops([(cif, w, h, 24, profile, 32, speed)], keyint=3, lag=6, ring=5, label=speed)
