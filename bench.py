@@ -60,16 +60,17 @@ def profile_record(kernel):
         return None
 
 
-def measured_traffic():
+def measured_traffic(mbs_per_round):
     """HBM-side bytes of the analysis kernels from the committed counter profile of this workload
-    (raw L2-to-fabric read and write requests by size): per round of launches (one launch per
-    group, the unit of avg_launch_us), per macroblock, and their ratio to the algorithmic bytes;
-    None without the profile."""
+    (raw L2-to-fabric read and write requests by size, per macroblock of the profiled job): per
+    round of launches (one launch per group, the unit of avg_launch_us) as the profiled bytes per
+    macroblock times this run's macroblocks per round, per macroblock, and their ratio to the
+    algorithmic bytes; None without the profile."""
     try:
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
             t = json.load(f)
-        return {"per_round": t["bytes_per_round"], "per_macroblock": t["bytes_per_macroblock"],
-                "ratio_to_algorithmic": t["traffic_ratio"]}
+        return {"per_round": round(t["bytes_per_macroblock"] * mbs_per_round),
+                "per_macroblock": t["bytes_per_macroblock"], "ratio_to_algorithmic": t["traffic_ratio"]}
     except (OSError, KeyError, ValueError):
         return None
 
@@ -224,6 +225,11 @@ def main():
     ap.add_argument("--end-to-end", type=int, default=1, help="also time the step with its host transfers (1: yes)")
     ap.add_argument("--compare-persistent", type=int, default=1,
                     help="also time the persistent schedule (one launch per job) on the same job (1: yes)")
+    ap.add_argument("--continuous", type=int, default=1,
+                    help="run the timed steps back to back in one call, every GOP stream starting its next GOP "
+                         "behind its last frames (1), or one call per step (0)")
+    ap.add_argument("--isolated", type=int, default=1,
+                    help="also time the steps one call each, beside the back-to-back value (1: yes)")
     ap.add_argument("--graph", type=int, default=0,
                     help="issue the per-step schedule's planned launches as one replayed HIP graph (1: yes)")
     ap.add_argument("--compare-graph", type=int, default=1,
@@ -464,13 +470,24 @@ def run_workload(gops, args):
 
     cpu = [0.0]  # host CPU time of the process over the steps since the last reset
 
-    def step(sched, persistent=False, graph=None):
+    gop_len = len(gops[0]["frames"])
+    assert all(len(gp["frames"]) == gop_len for gp in gops)
+
+    def step(sched, persistent=False, graph=None, k=1):
+        """k steps: one call per step (k = 1), or k steps in one call, every stream running its GOP
+        k times back to back (an IDR every gop_len frames, RGH_FRAMES_KEYINT): a stream's next GOP
+        starts at the call's lag behind the last frame of the GOP before, as a stream of GOPs runs.
+        The output buffers of frame f serve frame f + gop_len too (no two are in flight at once)."""
         lag, groups = sched
         graph = bool(args.graph) if graph is None else graph
         c = time.process_time()
         t = time.perf_counter()
-        analyse_frames(ans, srcs, decs, lvls, lag=lag, groups=groups, first_idr=True, persistent=persistent,
-                       graph=graph and not persistent)
+        if k == 1:
+            analyse_frames(ans, srcs, decs, lvls, lag=lag, groups=groups, first_idr=True, persistent=persistent,
+                           graph=graph and not persistent)
+        else:
+            analyse_frames(ans, [x * k for x in srcs], [x * k for x in decs], [x * k for x in lvls], lag=lag,
+                           groups=groups, first_idr=True, keyint=gop_len, graph=graph)
         el = time.perf_counter() - t
         cpu[0] += time.process_time() - c
         ms, _, launches = ans[0].timing()
@@ -480,9 +497,10 @@ def run_workload(gops, args):
     lags = [args.lag] if args.lag else [12, 16, 20, 24]
     groups = [args.groups] if args.groups else [1, 2, 3, 4]
     tune = {}
+    tk = 2 if args.continuous else 1
     for lag in lags:
         for g in groups:
-            tune[(lag, g)] = step((lag, g))[0]
+            tune[(lag, g)] = step((lag, g), k=tk)[0] / tk
     sched = min(tune, key=tune.get)
     # The persistent schedule (one launch per job) on the same job, for comparison: its lag tuned,
     # then timed over the same number of steps.
@@ -510,16 +528,34 @@ def run_workload(gops, args):
             el_g += step(sched, graph=True)[0]
         graphed = {"value": round(total_mbs * args.steps / el_g, 1), "ms_per_step": round(el_g / args.steps * 1e3, 3),
                    "host_cpu_ms_per_step": round(cpu[0] / args.steps * 1e3, 3)}
-    for _ in range(args.warmup):
-        step(sched)
+    # The steps one call each (every job ramps up from its IDR frames and down to its last frames).
+    isolated = None
+    if args.isolated and args.continuous:
+        for _ in range(args.warmup):
+            step(sched)
+        el_i = 0.0
+        for _ in range(args.steps):
+            el_i += step(sched)[0]
+        isolated = {"value": round(total_mbs * args.steps / el_i, 1),
+                    "ms_per_step": round(el_i / args.steps * 1e3, 3), "note": "each step its own rgh_analyse_frames call"}
+    # The timed steps: back to back in one call (continuous), or one call each.
+    if args.warmup:
+        if args.continuous:
+            step(sched, k=max(2, args.warmup))
+        else:
+            for _ in range(args.warmup):
+                step(sched)
     el = dev_ms = 0.0
     launches = 0
     cpu[0] = 0.0
-    for _ in range(args.steps):
-        e, ms, nl = step(sched)
-        el += e
-        dev_ms += ms
-        launches += nl
+    if args.continuous:
+        el, dev_ms, launches = step(sched, k=args.steps)
+    else:
+        for _ in range(args.steps):
+            e, ms, nl = step(sched)
+            el += e
+            dev_ms += ms
+            launches += nl
     host_cpu_ms = cpu[0] / args.steps * 1e3
 
     value = total_mbs * args.steps / el
@@ -528,7 +564,7 @@ def run_workload(gops, args):
     mbs_per_launch = (total_mbs + idr_mbs) * args.steps / launches
     achieved = step_bytes * args.steps / launches / avg_launch_s / 1e9
     headline = args.workload == "headline"
-    traffic = measured_traffic() if headline else None
+    traffic = measured_traffic(mbs_per_launch) if headline else None
     metric = {"headline": "1080p ME macroblocks/sec, 1 GPU: plain vs learned-proposal at <=1% BD-rate",
               "batched": "batched 24-stream ME macroblocks/sec, 1 GPU, plain form",
               "compact": "1080p ME macroblocks/sec, 1 GPU, Compact preset",
@@ -538,7 +574,9 @@ def run_workload(gops, args):
         "headline": "plain GPU form: 1920x1080 analysis, High decisions, QP 24/30/36, search range +-64 (64 "
                     "diamond iterations) with quarter-pel refinement, keyint 30, 12 GOP streams x (1 IDR + 29 P "
                     "frames) per step in one pipelined schedule, bit-exact with the CPU oracle; value counts P "
-                    "macroblocks",
+                    "macroblocks" + ("; the timed steps run back to back in one schedule, each stream starting "
+                                     "its next GOP behind its last frames (IDR every 30 frames)"
+                                     if args.continuous else ""),
         "batched": "plain GPU form: 24 streams (8 x 640x360, 8 x 1280x720, 8 x 1920x1080), 60 frames each (2 GOPs, "
                    "keyint 30), seeded QP 22-38, search range scaled by width, High decisions, all 48 GOPs (IDR + P "
                    "frames) in one pipelined schedule; value counts P macroblocks",
@@ -571,6 +609,8 @@ def run_workload(gops, args):
                    "idr_macroblocks_per_step": idr_mbs,
                    "all_macroblocks_per_s": round((total_mbs + idr_mbs) * args.steps / el, 1),
                    "schedule": "per-step launches" + (" (one captured HIP graph)" if args.graph else ""), "lag": sched[0], "groups": sched[1],
+                   "steps_back_to_back": bool(args.continuous),
+                   "isolated_steps": isolated,
                    "launches_per_step": launches / args.steps,
                    "schedule_tuning_s": {"%d/%d" % k: round(v, 4) for k, v in tune.items()},
                    "persistent_schedule": persist,
@@ -690,6 +730,20 @@ def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value
         analyse(props)
         return time.perf_counter() - t, t1 - t
 
+    gop_len = len(gops[0]["frames"])
+    per_gop = gop_len - 1
+
+    def continuous_step(provs, k):
+        """k steps back to back as the plain form runs them: every step's proposals (one pass
+        each, into its own buffer), then one call in which each stream runs its GOP k times."""
+        t = time.perf_counter()
+        props = [provs[s].run(pics) for s in range(k)]
+        t1 = time.perf_counter()
+        P = [sum(([None] + props[s][i * per_gop:(i + 1) * per_gop] for s in range(k)), []) for i in range(len(gops))]
+        analyse_frames(ans, [x * k for x in srcs], [x * k for x in decs], [x * k for x in lvls], lag=sched[0],
+                       groups=sched[1], first_idr=True, proposals=P, learned=cfg, keyint=gop_len)
+        return time.perf_counter() - t, t1 - t
+
     def pipelined_step(b, props):
         box = {}
         th = threading.Thread(target=lambda: box.setdefault("p", proposers[b].run(pics)))
@@ -706,6 +760,14 @@ def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value
         e, p = serial_step()
         el_serial += e
         prop += p
+    isolated = {"value": round(total_mbs * args.steps / el_serial, 1),
+                "ms_per_step": round(el_serial / args.steps * 1e3, 3), "note": "each step its own call"}
+    if args.continuous:
+        provs = [Proposer(net, total_mbs) for _ in range(max(args.steps, 2))]
+        continuous_step(provs, 2)
+        el_serial, prop = continuous_step(provs, args.steps)
+        for q in provs:
+            q.close()
     net.set_launch_limit(args.proposal_groups)
     props = proposers[0].run(pics)
     b = 1
@@ -746,6 +808,7 @@ def run_learned(gops, ans, srcs, decs, lvls, sched, args, total_mbs, plain_value
     value = total_mbs * args.steps / el_serial
     ceiling["vs_plain"] = round(ceiling["value"] / plain_value, 4)
     return {"value": round(value, 1), "unit": "macroblocks/s", "vs_plain": round(value / plain_value, 4),
+            "steps_back_to_back": bool(args.continuous), "isolated_steps": isolated,
             "ceiling": ceiling,
             "ms_per_step": round(el_serial / args.steps * 1e3, 3),
             "pipelined_value": round(total_mbs * args.steps / el, 1),
