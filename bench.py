@@ -322,18 +322,29 @@ def learned_ceiling(gops, ans, srcs, decs, lvls, sched, args, total_mbs, hip, an
         P.append(row)
     cfg = dict(iters=0, skip_gate=128, intra_gate=128, fallback_pct=100000, intra_first=128)
 
-    def step():
+    def step(k=1):
         t = time.perf_counter()
-        analyse_frames(ans, srcs, decs, lvls, lag=sched[0], groups=sched[1], first_idr=True, proposals=P, learned=cfg)
+        if k == 1:
+            analyse_frames(ans, srcs, decs, lvls, lag=sched[0], groups=sched[1], first_idr=True, proposals=P,
+                           learned=cfg)
+        else:  # k steps back to back, as the plain form's timed steps run
+            analyse_frames(ans, [x * k for x in srcs], [x * k for x in decs], [x * k for x in lvls], lag=sched[0],
+                           groups=sched[1], first_idr=True, proposals=[r * k for r in P], learned=cfg,
+                           keyint=len(gops[0]["frames"]))
         return time.perf_counter() - t
 
-    for _ in range(max(1, args.warmup)):
-        step()
-    el = sum(step() for _ in range(args.steps))
+    if args.continuous:
+        step(2)
+        el = step(args.steps)
+    else:
+        for _ in range(max(1, args.warmup)):
+            step()
+        el = sum(step() for _ in range(args.steps))
     for b in bufs:
         hip.free(b)
     return {"value": round(total_mbs * args.steps / el, 1), "ms_per_step": round(el / args.steps * 1e3, 3),
-            "config": cfg, "note": "proposals = the plain analysis's own vectors and modes (not timed)"}
+            "config": cfg, "steps_back_to_back": bool(args.continuous),
+            "note": "proposals = the plain analysis's own vectors and modes (not timed)"}
 
 
 def bd_rate_sample(args):
