@@ -505,8 +505,8 @@ def run_workload(gops, args):
         return el, ms, launches
 
     # Warm-up, with the per-step schedule's frame lag and concurrent groups autotuned unless given.
-    lags = [args.lag] if args.lag else [12, 16, 20, 24]
-    groups = [args.groups] if args.groups else [1, 2, 3, 4]
+    lags = [args.lag] if args.lag else [16, 18, 20, 22, 24]
+    groups = [args.groups] if args.groups else [2, 3]
     tune = {}
     tk = 2 if args.continuous else 1
     for lag in lags:
