@@ -2846,14 +2846,20 @@ struct StepLdsOf<2> {
     using T = CompactStepLds;
 };
 
+// Waves per SIMD the Balanced P kernels are register-bounded for (4: at most 128 VGPRs).
+#ifndef RGH_P_WAVES
+#define RGH_P_WAVES 4
+#endif
+
 // One launch of a frame schedule, sequence number seq0 + rseq (seq0 from the job descriptors).
-// Block row blockIdx.y is frame job j0 + blockIdx.y; the frame's current wavefront step is
-// s = launch - base - delay. Blocks [0, na) analyse the macroblocks of
-// step s; blocks [na, na + nd) run the deblock/pad items (x, y), 0 <= x <= mb_w, 0 <= y <= mb_h,
-// of step s - 4 (item (x, y) deblocks macroblock (x, y) when it exists and pads macroblock
-// (x - 1, y - 1)). The lag of 4 steps lets every analysis that reads undeblocked samples of
-// (x, y) or of the samples its edges modify ((x + 1, y), (x - 1, y + 1), (x, y + 1),
-// (x + 1, y + 1)) finish first.
+// The grid has 2 F rows of workgroups for the launch's F frame jobs j0 .. j0 + F - 1: row r < F
+// analyses the macroblocks of frame job j0 + r's current wavefront step s = launch - base -
+// delay, and row F + r runs that frame's deblock/pad items (x, y), 0 <= x <= mb_w,
+// 0 <= y <= mb_h, of step s - 4 (item (x, y) deblocks macroblock (x, y) when it exists and pads
+// macroblock (x - 1, y - 1)); the analyses, the longer items, are dispatched first. Column bx
+// takes the item on row band_item_row(bx, first row of the step, band phase). The lag of 4 steps
+// lets every analysis that reads undeblocked samples of (x, y) or of the samples its edges modify
+// ((x + 1, y), (x - 1, y + 1), (x, y + 1), (x + 1, y + 1)) finish first.
 //
 // A macroblock whose reference samples are not final, or the first step of a frame whose picture
 // slot is still read by a later frame, defers: it raises the frame's delay for the next launch
@@ -2873,28 +2879,30 @@ struct StepLdsOf<2> {
 // and Balanced P with partitions (rgh_k_mixed_parts_step).
 template <int MODE>
 __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int j0, int launch, uint32_t rseq) {
-    const RghJobD J = device_view(jobs[j0 + (int)blockIdx.y]);
+    const int nf = (int)(gridDim.y >> 1);
+    const bool deb = (int)blockIdx.y >= nf;  // a deblock/pad row
+    const int fr = deb ? (int)blockIdx.y - nf : (int)blockIdx.y;
+    const RghJobD J = device_view(jobs[j0 + fr]);
     const uint32_t seq = J.seq0 + rseq;  // this launch's sequence number
     RGH_G RghFrameCtl *ctl = J.ctl;
     const int lane = threadIdx.x;
     const int delay = ctl->delay[launch & 1];
     const int s = launch - J.base - delay;
     int *next_delay = (int *)&ctl->delay[(launch + 1) & 1];
-    if (blockIdx.x == 0 && lane == 0 && delay > 0) atomicMax(next_delay, delay);
-    const int na = min(J.mb_h, (J.mb_w + 1) >> 1);
-    const int b = (int)blockIdx.x;
+    if (!deb && blockIdx.x == 0 && lane == 0 && delay > 0) atomicMax(next_delay, delay);
+    const int bx = (int)blockIdx.x, h = band_phase((uint64_t)J.src_y);
     __shared__ typename StepLdsOf<MODE>::T S;
-    if (b < na) {
+    if (!deb) {
         if (s < 0 || s >= J.steps) return;
         const int a = s - J.mb_w + 1;
-        const int y = (a > 0 ? (a + 1) >> 1 : 0) + b;
+        const int y = band_item_row(bx, a > 0 ? (a + 1) >> 1 : 0, h);
         if (y > min(J.mb_h - 1, s >> 1)) return;
         const int x = s - 2 * y, m = y * J.mb_w + x;
         if ((uint32_t)(J.ana[m] >> 32) == J.serial) return;  // analysed in an earlier launch
         bool ok = true;
         if (s == 0 && J.wait_analysed) ok = __atomic_load_n(J.wait_analysed, __ATOMIC_RELAXED) >= J.wait_total;
         if (ok) {
-            const RghJob *jp = jobs + j0 + (int)blockIdx.y;
+            const RghJob *jp = jobs + j0 + fr;
             if constexpr (MODE == 1 || MODE == 8) {
                 analyse_intra_mb<MODE == 8>(J, jp, x, y, S.a);
             } else if constexpr (MODE == 4 || MODE == 5 || MODE == 7) {
@@ -2916,7 +2924,7 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
         }
         if (lane == 0) {
             // The stamp and progress buffers, read again from the descriptor (as late_outputs).
-            const RGH_G RghJob *g = (const RGH_G RghJob *)(jobs + j0 + (int)blockIdx.y);
+            const RGH_G RghJob *g = (const RGH_G RghJob *)(jobs + j0 + fr);
             __asm__ volatile("" : "+s"(g));
             RGH_G RghFrameCtl *lctl = gp(g->ctl);
             if (ok) {
@@ -2930,7 +2938,7 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
         const int sd = s - 4;
         if (sd < 0 || sd > J.steps + 2) return;
         const int a = sd - J.mb_w;
-        const int y = (a > 0 ? (a + 1) >> 1 : 0) + (b - na);
+        const int y = band_item_row(bx, a > 0 ? (a + 1) >> 1 : 0, h);
         if (y > min(J.mb_h, sd >> 1)) return;
         const int x = sd - 2 * y, m = y * (J.mb_w + 1) + x;
         if ((uint32_t)(J.fin[m] >> 32) == J.serial) return;  // done in an earlier launch
@@ -2946,7 +2954,7 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
     }
 }
 
-extern "C" __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void rgh_k_frame_step(const RghJob *__restrict__ jobs, int j0, int launch,
+extern "C" __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RGH_P_WAVES))) void rgh_k_frame_step(const RghJob *__restrict__ jobs, int j0, int launch,
                                                                    uint32_t seq) {
     frame_step<0>(jobs, j0, launch, seq);
 }
@@ -2956,7 +2964,7 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_intra_step(const RghJob *
     frame_step<1>(jobs, j0, launch, seq);
 }
 
-extern "C" __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void rgh_k_mixed_step(const RghJob *__restrict__ jobs, int j0, int launch,
+extern "C" __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RGH_P_WAVES))) void rgh_k_mixed_step(const RghJob *__restrict__ jobs, int j0, int launch,
                                                                    uint32_t seq) {
     frame_step<4>(jobs, j0, launch, seq);
 }

@@ -324,7 +324,10 @@ struct Schedule {
             // further) kDeblockLag behind.
             span[(size_t)j] = jobs[(size_t)j].steps + 3 + kDeblockLag;
             max_span = std::max(max_span, span[(size_t)j]);
-            gx = std::max(gx, std::min(mb_h, (mb_w + 1) / 2) + std::min(mb_h + 1, (mb_w + 2) / 2));
+            // Items per step: analyses min(mb_h, ceil(mb_w / 2)), deblock/pad items one more
+            // at most; a grid row is whole groups of the XCD band mapping.
+            const int items = std::max(std::min(mb_h, (mb_w + 1) / 2), std::min(mb_h + 1, (mb_w + 2) / 2));
+            gx = std::max(gx, (items + RGH_ITEM_GROUP - 1) / RGH_ITEM_GROUP * RGH_ITEM_GROUP);
         }
         planned = (n_frames - 1) * lag + max_span;
         return RGH_OK;
@@ -344,10 +347,12 @@ struct Schedule {
         HIP_TRY(hipGetLastError());
         return RGH_OK;
     }
-    // One kernel of launch L over jobs [j0, j0 + gy): listed while the planned launches are
-    // listed (relative sequence number L), otherwise issued with a fresh sequence number, above
-    // every number of the planned launches.
-    void emit(StepKernel fn, int j0, int gy) {
+    // One kernel of launch L over jobs [j0, j0 + nf) (grid gx x 2 nf: the frames' analysis rows,
+    // then their deblock/pad rows): listed while the planned launches are listed (relative
+    // sequence number L), otherwise issued with a fresh sequence number, above every number of
+    // the planned launches.
+    void emit(StepKernel fn, int j0, int nf) {
+        const int gy = 2 * nf;
         if (out) {
             out->push_back(LaunchRec{fn, gx, gy, group, jobs_dev, j0, L});
             return;

@@ -34,16 +34,32 @@ struct Edge {
     int tc0[4];   // per 4-line segment; -1 = segment not filtered
 };
 
+// The deblocking tables indexed by QP, entry q in lane q (lanes 0-51): alpha, beta, tc0 for bS
+// 1-3 and the chroma QP of luma QP q. A macroblock loads them with its records, in one round
+// trip, so that no edge waits for a table read.
+struct DbTables {
+    uint32_t alpha, beta, tc1, tc2, tc3, cqp;
+};
+__device__ __forceinline__ DbTables load_db_tables(int cqo) {
+    const int q = min((int)threadIdx.x, 51);
+    return DbTables{c_alpha[q], c_beta[q], c_tc0[q][0], c_tc0[q][1], c_tc0[q][2],
+                    c_chroma_qp[iclamp(q + cqo, 0, 51)]};
+}
+// Entry q (uniform, 0..51) of a table register.
+__device__ __forceinline__ int db_at(uint32_t reg, int q) { return (int)__builtin_amdgcn_readlane(reg, q); }
+
 // enc.rs:3911 EdgeFilter::new
-__device__ Edge make_edge(int qp, const int bs[4]) {
+__device__ __forceinline__ Edge make_edge(int qp, const int bs[4], const DbTables &T) {
     Edge E;
-    int ia = iclamp(qp, 0, 51);
-    E.alpha = c_alpha[ia];
-    E.beta = c_beta[ia];
+    const int ia = iclamp(qp, 0, 51);
+    E.alpha = db_at(T.alpha, ia);
+    E.beta = db_at(T.beta, ia);
     E.on = E.alpha != 0 && E.beta != 0;
     E.strong = bs[0] == 4;
+    const int t1 = db_at(T.tc1, ia), t2 = db_at(T.tc2, ia), t3 = db_at(T.tc3, ia);
 #pragma unroll
-    for (int k = 0; k < 4; k++) E.tc0[k] = bs[k] == 0 ? -1 : (bs[k] >= 4 ? 0 : (int)c_tc0[ia][bs[k] - 1]);
+    for (int k = 0; k < 4; k++)
+        E.tc0[k] = bs[k] == 0 ? -1 : (bs[k] >= 4 ? 0 : (bs[k] == 1 ? t1 : (bs[k] == 2 ? t2 : t3)));
     return E;
 }
 
@@ -53,55 +69,49 @@ __device__ __forceinline__ int seg_tc0(const Edge &E, int g) {
     return g == 0 ? E.tc0[0] : (g == 1 ? E.tc0[1] : (g == 2 ? E.tc0[2] : E.tc0[3]));
 }
 
-// simd/h264_deblock.rs:69 h264_luma_deblock_scalar, one line (p3..q3 at q0 - 4s .. q0 + 3s).
-__device__ void luma_line(uint8_t *b, int q0, int s, const Edge &E, int k) {
-    int tc0 = seg_tc0(E, k >> 2);
-    if (!E.strong && tc0 < 0) return;
-    int p0 = b[q0 - s], q0v = b[q0], p1 = b[q0 - 2 * s], q1 = b[q0 + s];
-    if (iabs(p0 - q0v) >= E.alpha || iabs(p1 - p0) >= E.beta || iabs(q1 - q0v) >= E.beta) return;
-    int p2 = b[q0 - 3 * s], q2 = b[q0 + 2 * s];
-    int ap = iabs(p2 - p0) < E.beta, aq = iabs(q2 - q0v) < E.beta;
+// simd/h264_deblock.rs:69 h264_luma_deblock_scalar on one line of samples in registers: v[0..7]
+// hold p3 p2 p1 p0 q0 q1 q2 q3; tc0 is the line's segment's (-1: not filtered unless bS 4).
+// Every output is selected, so lanes whose line is not filtered keep their samples; a sample is
+// stored as its low byte (the reference's u8 cast).
+__device__ __forceinline__ void luma_px(int *v, const Edge &E, int tc0) {
+    const int p3 = v[0], p2 = v[1], p1 = v[2], p0 = v[3], q0 = v[4], q1 = v[5], q2 = v[6], q3 = v[7];
+    const bool f = (E.strong || tc0 >= 0) && iabs(p0 - q0) < E.alpha && iabs(p1 - p0) < E.beta &&
+                   iabs(q1 - q0) < E.beta;
+    const bool ap = iabs(p2 - p0) < E.beta, aq = iabs(q2 - q0) < E.beta;
     if (!E.strong) {
-        int tc = tc0 + ap + aq;
-        int d = iclamp((((q0v - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
-        b[q0 - s] = (uint8_t)clip255(p0 + d);
-        b[q0] = (uint8_t)clip255(q0v - d);
-        if (ap) b[q0 - 2 * s] = (uint8_t)(p1 + iclamp((p2 + ((p0 + q0v + 1) >> 1) - (p1 << 1)) >> 1, -tc0, tc0));
-        if (aq) b[q0 + s] = (uint8_t)(q1 + iclamp((q2 + ((p0 + q0v + 1) >> 1) - (q1 << 1)) >> 1, -tc0, tc0));
+        const int tc = tc0 + (int)ap + (int)aq;
+        const int d = iclamp((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
+        const int avg = (p0 + q0 + 1) >> 1;
+        v[3] = f ? clip255(p0 + d) : p0;
+        v[4] = f ? clip255(q0 - d) : q0;
+        v[2] = f && ap ? p1 + iclamp((p2 + avg - (p1 << 1)) >> 1, -tc0, tc0) : p1;
+        v[5] = f && aq ? q1 + iclamp((q2 + avg - (q1 << 1)) >> 1, -tc0, tc0) : q1;
     } else {
-        int p3 = b[q0 - 4 * s], q3 = b[q0 + 3 * s];
-        int small = iabs(p0 - q0v) < ((E.alpha >> 2) + 2);
-        if (ap && small) {
-            b[q0 - s] = (uint8_t)((p2 + 2 * p1 + 2 * p0 + 2 * q0v + q1 + 4) >> 3);
-            b[q0 - 2 * s] = (uint8_t)((p2 + p1 + p0 + q0v + 2) >> 2);
-            b[q0 - 3 * s] = (uint8_t)((2 * p3 + 3 * p2 + p1 + p0 + q0v + 4) >> 3);
-        } else {
-            b[q0 - s] = (uint8_t)((2 * p1 + p0 + q1 + 2) >> 2);
-        }
-        if (aq && small) {
-            b[q0] = (uint8_t)((p1 + 2 * p0 + 2 * q0v + 2 * q1 + q2 + 4) >> 3);
-            b[q0 + s] = (uint8_t)((p0 + q0v + q1 + q2 + 2) >> 2);
-            b[q0 + 2 * s] = (uint8_t)((2 * q3 + 3 * q2 + q1 + q0v + p0 + 4) >> 3);
-        } else {
-            b[q0] = (uint8_t)((2 * q1 + q0v + p1 + 2) >> 2);
-        }
+        const bool small = iabs(p0 - q0) < ((E.alpha >> 2) + 2);
+        const bool sp = f && ap && small, sq = f && aq && small;
+        v[3] = sp ? (p2 + 2 * p1 + 2 * p0 + 2 * q0 + q1 + 4) >> 3 : (f ? (2 * p1 + p0 + q1 + 2) >> 2 : p0);
+        v[2] = sp ? (p2 + p1 + p0 + q0 + 2) >> 2 : p1;
+        v[1] = sp ? (2 * p3 + 3 * p2 + p1 + p0 + q0 + 4) >> 3 : p2;
+        v[4] = sq ? (p1 + 2 * p0 + 2 * q0 + 2 * q1 + q2 + 4) >> 3 : (f ? (2 * q1 + q0 + p1 + 2) >> 2 : q0);
+        v[5] = sq ? (p0 + q0 + q1 + q2 + 2) >> 2 : q1;
+        v[6] = sq ? (2 * q3 + 3 * q2 + q1 + q0 + p0 + 4) >> 3 : q2;
     }
 }
 
-// simd/h264_deblock.rs:206 h264_chroma_deblock_scalar, one line (p1..q1).
-__device__ void chroma_line(uint8_t *b, int q0, int s, const Edge &E, int k) {
-    int tc0 = seg_tc0(E, k >> 1);
-    if (!E.strong && tc0 < 0) return;
-    int p1 = b[q0 - 2 * s], p0 = b[q0 - s], q0v = b[q0], q1 = b[q0 + s];
-    if (iabs(p0 - q0v) >= E.alpha || iabs(p1 - p0) >= E.beta || iabs(q1 - q0v) >= E.beta) return;
+// simd/h264_deblock.rs:206 h264_chroma_deblock_scalar on one line in registers: v[0..3] hold
+// p1 p0 q0 q1.
+__device__ __forceinline__ void chroma_px(int *v, const Edge &E, int tc0) {
+    const int p1 = v[0], p0 = v[1], q0 = v[2], q1 = v[3];
+    const bool f = (E.strong || tc0 >= 0) && iabs(p0 - q0) < E.alpha && iabs(p1 - p0) < E.beta &&
+                   iabs(q1 - q0) < E.beta;
     if (E.strong) {
-        b[q0 - s] = (uint8_t)((2 * p1 + p0 + q1 + 2) >> 2);
-        b[q0] = (uint8_t)((2 * q1 + q0v + p1 + 2) >> 2);
+        v[1] = f ? (2 * p1 + p0 + q1 + 2) >> 2 : p0;
+        v[2] = f ? (2 * q1 + q0 + p1 + 2) >> 2 : q0;
     } else {
-        int tc = tc0 + 1;
-        int d = iclamp((((q0v - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
-        b[q0 - s] = (uint8_t)clip255(p0 + d);
-        b[q0] = (uint8_t)clip255(q0v - d);
+        const int tc = tc0 + 1;
+        const int d = iclamp((((q0 - p0) << 2) + (p1 - q1) + 4) >> 3, -tc, tc);
+        v[1] = f ? clip255(p0 + d) : p0;
+        v[2] = f ? clip255(q0 - d) : q0;
     }
 }
 
@@ -129,23 +139,16 @@ __device__ __forceinline__ int motion_bs(uint32_t a, uint32_t b) {
 
 __device__ __forceinline__ void deblock_mb_inline(const RghJobD &J, int mb_x, int mb_y, DbLds &S) {
     const int lane = threadIdx.x, mb_w = J.mb_w, m = mb_y * mb_w + mb_x;
+    const int x0 = mb_x * 16, y0 = mb_y * 16, cx0 = mb_x * 8, cy0 = mb_y * 8;
+    // One round trip: the three records, the tables, and the samples (luma rows y0-4 .. y0+15,
+    // columns x0-4 .. x0+15, 5 aligned dwords a row; chroma rows cy0-2 .. cy0+7, columns
+    // cx0-4 .. cx0+7, 3 aligned dwords a row, of both planes), the samples whether or not the
+    // records leave any edge to filter.
     const rgh_mb_decision Q = load_dec(&J.dec[m]);
     rgh_mb_decision L{}, T{};
     if (mb_x > 0) L = load_dec(&J.dec[m - 1]);
     if (mb_y > 0) T = load_dec(&J.dec[m - mb_w]);
-    auto mveq = [](const rgh_mb_decision &a, const rgh_mb_decision &b) { return a.mv[0] == b.mv[0] && a.mv[1] == b.mv[1]; };
-    // Compact pictures and pictures with P partitions: every 4x4 block carries its quadrant's
-    // vector (enc/inter.rs:781 deblock_parts); the internal edges of an inter macroblock without coefficients and with
-    // one vector throughout are not filtered.
-    const bool parts = J.compact != 0 || J.parts != 0;
-    const bool uniform = !is_intra(Q) && Q.nnz == 0 && quad_word(Q, 0) == quad_word(Q, 1) &&
-                         quad_word(Q, 0) == quad_word(Q, 2) && quad_word(Q, 0) == quad_word(Q, 3);
-    if (!parts && quiet(Q) && (mb_x == 0 || (quiet(L) && mveq(L, Q))) && (mb_y == 0 || (quiet(T) && mveq(T, Q)))) return;
-
-    const int x0 = mb_x * 16, y0 = mb_y * 16, cx0 = mb_x * 8, cy0 = mb_y * 8;
-    // Stage luma rows y0-4 .. y0+15, columns x0-4 .. x0+15 (5 aligned dwords a row) and chroma
-    // rows cy0-2 .. cy0+7, columns cx0-4 .. cx0+7 (3 aligned dwords a row) of both planes; all
-    // loads are issued before the LDS stores.
+    const DbTables tab = load_db_tables(J.cqo);
     uint32_t ly[2], lc;
 #pragma unroll
     for (int k = 0; k < 2; k++) {
@@ -157,6 +160,15 @@ __device__ __forceinline__ void deblock_mb_inline(const RghJobD &J, int mb_x, in
         const RGH_G uint8_t *pl = p ? J.cur_v : J.cur_u;
         lc = lane < 60 ? *(const RGH_G uint32_t *)&pl[(long)(cy0 - 2 + i / 3) * J.sc + cx0 - 4 + (i % 3) * 4] : 0u;
     }
+    auto mveq = [](const rgh_mb_decision &a, const rgh_mb_decision &b) { return a.mv[0] == b.mv[0] && a.mv[1] == b.mv[1]; };
+    // Compact pictures and pictures with P partitions: every 4x4 block carries its quadrant's
+    // vector (enc/inter.rs:781 deblock_parts); the internal edges of an inter macroblock without coefficients and with
+    // one vector throughout are not filtered.
+    const bool parts = J.compact != 0 || J.parts != 0;
+    const bool uniform = !is_intra(Q) && Q.nnz == 0 && quad_word(Q, 0) == quad_word(Q, 1) &&
+                         quad_word(Q, 0) == quad_word(Q, 2) && quad_word(Q, 0) == quad_word(Q, 3);
+    if (!parts && quiet(Q) && (mb_x == 0 || (quiet(L) && mveq(L, Q))) && (mb_y == 0 || (quiet(T) && mveq(T, Q)))) return;
+
 #pragma unroll
     for (int k = 0; k < 2; k++) {
         const int i = lane + 64 * k;
@@ -169,84 +181,135 @@ __device__ __forceinline__ void deblock_mb_inline(const RghJobD &J, int mb_x, in
     __syncthreads();
 
     const int intra = is_intra(Q), internal = parts ? !uniform : (intra || Q.cbp_luma != 0), skip_odd = Q.transform_8x8;
-    const int qp_q = Q.qp, qpc_q = chroma_qp_of(qp_q, J.cqo);
-    for (int dir = 0; dir < 2; dir++) {
+    const int qp_q = Q.qp, qpc_q = db_at(tab.cqp, qp_q);
+    // The filter parameters of edge e (0: the macroblock's left or top edge) in direction dir
+    // (0 vertical edges, 1 horizontal edges), luma and chroma; false when the edge is not
+    // filtered. The macroblock on the p side of the edge is the left or top neighbour for edge 0,
+    // else this one (selected field by field: a pointer to a local record would live in scratch
+    // memory).
+    auto edge_params = [&](int dir, int e, Edge &EL, Edge &EC) -> bool {
+        const bool ext = e == 0;
+        if (ext) {
+            if (!(dir == 0 ? mb_x > 0 : mb_y > 0)) return false;
+        } else if (!internal || ((e & 1) && skip_odd)) {
+            return false;
+        }
+        const rgh_mb_decision &N = dir == 0 ? L : T;
+        const int p_intra = ext ? is_intra(N) : intra;
+        const uint32_t p_nnz = ext ? N.nnz : Q.nnz;
+        const int p_mvx = ext ? N.mv[0] : Q.mv[0], p_mvy = ext ? N.mv[1] : Q.mv[1];
+        const int qp_p = ext ? N.qp : Q.qp;
+        int bs[4];
+        if (intra || p_intra) {
+#pragma unroll
+            for (int k = 0; k < 4; k++) bs[k] = ext ? 4 : 3;
+        } else if (parts) {
+            // Segment k: q block (e, k) (vertical edges) or (k, e) of this macroblock, p
+            // block one to the left / above, in the neighbour for edge 0.
+            int any = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int qx = dir == 0 ? e : k, qy = dir == 0 ? k : e;
+                const int px = dir == 0 ? (e ? e - 1 : 3) : k, py = dir == 0 ? k : (e ? e - 1 : 3);
+                const int qn = (Q.nnz >> (4 * qy + qx)) & 1, pn = (p_nnz >> (4 * py + px)) & 1;
+                const uint32_t qv = quad_word(Q, (qx >> 1) + (qy & 2));
+                const int pk = (px >> 1) + (py & 2);
+                const uint32_t pv = ext ? (dir == 0 ? quad_word(L, pk) : quad_word(T, pk)) : quad_word(Q, pk);
+                bs[k] = (qn | pn) ? 2 : motion_bs(qv, pv);
+                any |= bs[k];
+            }
+            if (!any) return false;
+        } else {
+            uint32_t coded;
+            if (dir == 0 && ext)
+                coded = nz_col(Q.nnz, 0) | nz_col(p_nnz, 3);
+            else if (dir == 0)
+                coded = nz_col(Q.nnz, e) | nz_col(Q.nnz, e - 1);
+            else if (ext)
+                coded = (Q.nnz & 0xF) | (p_nnz >> 12);
+            else
+                coded = ((Q.nnz >> (4 * e)) & 0xF) | ((Q.nnz >> (4 * (e - 1))) & 0xF);
+            int motion = ext && (iabs(Q.mv[0] - p_mvx) >= 4 || iabs(Q.mv[1] - p_mvy) >= 4);
+            if (!coded && !motion) return false;
+#pragma unroll
+            for (int k = 0; k < 4; k++) bs[k] = ((coded >> k) & 1) ? 2 : motion;
+        }
+        int qp = qp_q, qpc = qpc_q;
+        if (ext) {
+            qp = (qp_q + qp_p + 1) >> 1;
+            qpc = (qpc_q + db_at(tab.cqp, qp_p) + 1) >> 1;
+        }
+        EL = make_edge(qp, bs, tab);
+        EC = make_edge(qpc, bs, tab);
+        return true;
+    };
+    // Each line crosses the four edges of a direction in one lane, in edge order, with its
+    // samples in registers: lanes 0-15 a luma line (row or column k = lane of the macroblock,
+    // the 4 samples before it included), lanes 16-23 a U line and 24-31 a V line (k = lane & 7).
+    // Lines of one direction are independent, so the only barrier is between the directions.
+    const bool lum = lane < 16, chr = lane >= 16 && lane < 32;
+    const int k = lum ? lane : (lane & 7);
+    uint8_t *cpl = lane < 24 ? S.u : S.v;
+    // Vertical edges: luma row 4 + k (columns 0-19 of the copy), chroma row 2 + k (columns 0-11).
+    {
+        int b[20];
+        uint32_t w[5] = {0, 0, 0, 0, 0};
+        if (lum) {
+#pragma unroll
+            for (int i = 0; i < 5; i++) w[i] = *(const uint32_t *)&S.y[(4 + k) * 20 + 4 * i];
+        } else if (chr) {
+#pragma unroll
+            for (int i = 0; i < 3; i++) w[i] = *(const uint32_t *)&cpl[(2 + k) * 12 + 4 * i];
+        }
+#pragma unroll
+        for (int i = 0; i < 20; i++) b[i] = (int)((w[i >> 2] >> (8 * (i & 3))) & 0xFFu);
+#pragma unroll
         for (int e = 0; e < 4; e++) {
-            // The macroblock on the p side of the edge: the left or top neighbour for edge 0, else
-            // this one (selected field by field: a pointer to a local record would live in
-            // scratch memory).
-            const bool ext = e == 0;
-            if (ext) {
-                if (!(dir == 0 ? mb_x > 0 : mb_y > 0)) continue;
-            } else if (!internal || ((e & 1) && skip_odd)) {
-                continue;
-            }
-            const rgh_mb_decision &N = dir == 0 ? L : T;
-            const int p_intra = ext ? is_intra(N) : intra;
-            const uint32_t p_nnz = ext ? N.nnz : Q.nnz;
-            const int p_mvx = ext ? N.mv[0] : Q.mv[0], p_mvy = ext ? N.mv[1] : Q.mv[1];
-            const int qp_p = ext ? N.qp : Q.qp;
-            int bs[4];
-            if (intra || p_intra) {
+            Edge EL, EC;
+            if (!edge_params(0, e, EL, EC)) continue;
+            if (lum && EL.on) luma_px(b + 4 * e, EL, seg_tc0(EL, k >> 2));
+            if (!(e & 1) && chr && EC.on) chroma_px(b + 2 + 2 * e, EC, seg_tc0(EC, k >> 1));
+        }
 #pragma unroll
-                for (int k = 0; k < 4; k++) bs[k] = ext ? 4 : 3;
-            } else if (parts) {
-                // Segment k: q block (e, k) (vertical edges) or (k, e) of this macroblock, p
-                // block one to the left / above, in the neighbour for edge 0.
-                int any = 0;
+        for (int i = 0; i < 5; i++)
+            w[i] = (uint32_t)(b[4 * i] & 0xFF) | (uint32_t)(b[4 * i + 1] & 0xFF) << 8 |
+                   (uint32_t)(b[4 * i + 2] & 0xFF) << 16 | (uint32_t)(b[4 * i + 3] & 0xFF) << 24;
+        if (lum) {
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int qx = dir == 0 ? e : k, qy = dir == 0 ? k : e;
-                    const int px = dir == 0 ? (e ? e - 1 : 3) : k, py = dir == 0 ? k : (e ? e - 1 : 3);
-                    const int qn = (Q.nnz >> (4 * qy + qx)) & 1, pn = (p_nnz >> (4 * py + px)) & 1;
-                    const uint32_t qv = quad_word(Q, (qx >> 1) + (qy & 2));
-                    const int pk = (px >> 1) + (py & 2);
-                    const uint32_t pv = ext ? (dir == 0 ? quad_word(L, pk) : quad_word(T, pk)) : quad_word(Q, pk);
-                    bs[k] = (qn | pn) ? 2 : motion_bs(qv, pv);
-                    any |= bs[k];
-                }
-                if (!any) continue;
-            } else {
-                uint32_t coded;
-                if (dir == 0 && ext)
-                    coded = nz_col(Q.nnz, 0) | nz_col(p_nnz, 3);
-                else if (dir == 0)
-                    coded = nz_col(Q.nnz, e) | nz_col(Q.nnz, e - 1);
-                else if (ext)
-                    coded = (Q.nnz & 0xF) | (p_nnz >> 12);
-                else
-                    coded = ((Q.nnz >> (4 * e)) & 0xF) | ((Q.nnz >> (4 * (e - 1))) & 0xF);
-                int motion = ext && (iabs(Q.mv[0] - p_mvx) >= 4 || iabs(Q.mv[1] - p_mvy) >= 4);
-                if (!coded && !motion) continue;
+            for (int i = 0; i < 5; i++) *(uint32_t *)&S.y[(4 + k) * 20 + 4 * i] = w[i];
+        } else if (chr) {
 #pragma unroll
-                for (int k = 0; k < 4; k++) bs[k] = ((coded >> k) & 1) ? 2 : motion;
-            }
-            int qp = qp_q, qpc = qpc_q;
-            if (ext) {
-                qp = (qp_q + qp_p + 1) >> 1;
-                qpc = (qpc_q + chroma_qp_of(qp_p, J.cqo) + 1) >> 1;
-            }
-            Edge EL = make_edge(qp, bs);
-            if (EL.on && lane < 16) {
-                if (dir == 0)
-                    luma_line(S.y, (4 + lane) * 20 + 4 + e * 4, 1, EL, lane);
-                else
-                    luma_line(S.y, (4 + e * 4) * 20 + 4 + lane, 20, EL, lane);
-            }
-            if (!(e & 1)) {
-                Edge EC = make_edge(qpc, bs);
-                if (EC.on && lane >= 16 && lane < 32) {
-                    int k = lane & 7;
-                    uint8_t *pl = lane < 24 ? S.u : S.v;
-                    if (dir == 0)
-                        chroma_line(pl, (2 + k) * 12 + 4 + e * 2, 1, EC, k);
-                    else
-                        chroma_line(pl, (2 + e * 2) * 12 + 4 + k, 12, EC, k);
-                }
-            }
-            __syncthreads();
+            for (int i = 0; i < 3; i++) *(uint32_t *)&cpl[(2 + k) * 12 + 4 * i] = w[i];
         }
     }
+    __syncthreads();
+    // Horizontal edges: luma column 4 + k (rows 0-19), chroma column 4 + k (rows 0-9).
+    {
+        int b[20];
+        if (lum) {
+#pragma unroll
+            for (int i = 0; i < 20; i++) b[i] = S.y[i * 20 + 4 + k];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 20; i++) b[i] = chr && i < 10 ? cpl[i * 12 + 4 + k] : 0;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            Edge EL, EC;
+            if (!edge_params(1, e, EL, EC)) continue;
+            if (lum && EL.on) luma_px(b + 4 * e, EL, seg_tc0(EL, k >> 2));
+            if (!(e & 1) && chr && EC.on) chroma_px(b + 2 * e, EC, seg_tc0(EC, k >> 1));
+        }
+        // Rows 1-18 (luma) and 1-8 (chroma) are the ones an edge can modify.
+        if (lum) {
+#pragma unroll
+            for (int i = 1; i < 19; i++) S.y[i * 20 + 4 + k] = (uint8_t)b[i];
+        } else if (chr) {
+#pragma unroll
+            for (int i = 1; i < 9; i++) cpl[i * 12 + 4 + k] = (uint8_t)b[i];
+        }
+    }
+    __syncthreads();
     // Write back all but the untouched top-left corner, as aligned dwords.
 #pragma unroll
     for (int k = 0; k < 2; k++) {
