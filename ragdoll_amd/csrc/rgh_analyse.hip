@@ -1279,6 +1279,14 @@ struct Gate {
     int T;        // this item's time
     int ref_job;  // the reference frame's job in this launch (-1: complete before the launch)
     int ref_base; // its base time
+    // Per-step launches: the item was taken on the guess that its frame has not fallen behind
+    // (spec). The analysis confirms the guess once its first round trip of loads is out and
+    // before it uses any of them: the frame's delay (spec_delay) is 0 and the item's stamp
+    // (spec_stamp) is not of this frame yet. Otherwise it sets *retry and returns false without
+    // a side effect.
+    int spec = 0, spec_delay = 0;
+    uint64_t spec_stamp = 0;
+    int *retry = nullptr;
 };
 
 // Waits until every lane's pending stamp *p carries `serial`. T_dep: the time of the item the lane
@@ -1536,6 +1544,10 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
         }
         eshift = 8 * (int)((uintptr_t)ea & 3);
         ew = *(const RGH_G uint32_t *)(ea - ((uintptr_t)ea & 3));
+    }
+    if (G.spec && (G.spec_delay != 0 || (uint32_t)(G.spec_stamp >> 32) == J.serial)) {
+        *G.retry = 1;
+        return false;
     }
     // Intra records (I_16x16, I_4x4, I_PCM; with partitions 5-7 are inter).
     auto intra_rec = [](uint32_t w0) { return PARTS ? (w0 & 0xFFu) - 2u <= 2u : (w0 & 0xFFu) >= 2u; };
@@ -2887,55 +2899,84 @@ __device__ __forceinline__ void frame_step(const RghJob *__restrict__ jobs, int 
     RGH_G RghFrameCtl *ctl = J.ctl;
     const int lane = threadIdx.x;
     const int delay = ctl->delay[launch & 1];
-    const int s = launch - J.base - delay;
-    int *next_delay = (int *)&ctl->delay[(launch + 1) & 1];
-    if (!deb && blockIdx.x == 0 && lane == 0 && delay > 0) atomicMax(next_delay, delay);
+    const int s0 = launch - J.base;  // the frame's step unless it has fallen behind
     const int bx = (int)blockIdx.x, h = band_phase((uint64_t)J.src_y);
     __shared__ typename StepLdsOf<MODE>::T S;
     if (!deb) {
-        if (s < 0 || s >= J.steps) return;
-        const int a = s - J.mb_w + 1;
-        const int y = band_item_row(bx, a > 0 ? (a + 1) >> 1 : 0, h);
-        if (y > min(J.mb_h - 1, s >> 1)) return;
-        const int x = s - 2 * y, m = y * J.mb_w + x;
-        if ((uint32_t)(J.ana[m] >> 32) == J.serial) return;  // analysed in an earlier launch
-        bool ok = true;
-        if (s == 0 && J.wait_analysed) ok = __atomic_load_n(J.wait_analysed, __ATOMIC_RELAXED) >= J.wait_total;
-        if (ok) {
-            const RghJob *jp = jobs + j0 + fr;
-            if constexpr (MODE == 1 || MODE == 8) {
-                analyse_intra_mb<MODE == 8>(J, jp, x, y, S.a);
-            } else if constexpr (MODE == 4 || MODE == 5 || MODE == 7) {
-                if (J.intra)
-                    analyse_intra_mb<MODE == 7>(J, jp, x, y, S.a);
-                else
-                    ok = analyse_mb<MODE == 5, false, MODE == 7>(J, jp, x, y, S.a, Gate{seq, nullptr, 0, -1, 0});
-            } else if constexpr (MODE == 9) {
-                ok = analyse_mb<false, false, false, true>(J, jp, x, y, S.a, Gate{seq, nullptr, 0, -1, 0});
-            } else if constexpr (MODE == 10) {
-                if (J.intra)
-                    analyse_intra_mb<false>(J, jp, x, y, S.a);
-                else
-                    ok = analyse_mb<false, false, false, true>(J, jp, x, y, S.a, Gate{seq, nullptr, 0, -1, 0});
-            } else if constexpr (MODE == 2)
-                ok = analyse_compact_mb(J, x, y, S.a, seq);
-            else
-                ok = analyse_mb<MODE == 3, false, MODE == 6>(J, jp, x, y, S.a, Gate{seq, nullptr, 0, -1, 0});
-        }
-        if (lane == 0) {
-            // The stamp and progress buffers, read again from the descriptor (as late_outputs).
-            const RGH_G RghJob *g = (const RGH_G RghJob *)(jobs + j0 + fr);
-            __asm__ volatile("" : "+s"(g));
-            RGH_G RghFrameCtl *lctl = gp(g->ctl);
-            if (ok) {
-                gp(g->ana)[m] = make_stamp(g->serial, seq);
-                atomicAdd((int *)&lctl->analysed, 1);
-            } else {
-                atomicMax((int *)&lctl->delay[(launch + 1) & 1], delay + 1);
+        // The item is first taken on the guess that the frame has not fallen behind (delay 0, the
+        // common case): its stamp and the analysis's first round trip of loads go out together
+        // with the delay, and the analysis confirms the guess before it uses any of them
+        // (Gate::spec). Otherwise, and for the analyses without that check (IDR pictures, the
+        // Compact preset), the item is taken from the delay read.
+        constexpr bool GATED = MODE == 0 || MODE == 3 || MODE == 4 || MODE == 5 || MODE == 6 || MODE == 7 ||
+                               MODE == 9 || MODE == 10;
+        bool spec = GATED && !J.intra;
+        int retry = 0;
+#pragma unroll 1
+        for (int pass = 0; pass < 2; pass++) {
+            const int s = spec ? s0 : s0 - delay;
+            bool valid = s >= 0 && s < J.steps;
+            int y = 0;
+            if (valid) {
+                const int a = s - J.mb_w + 1;
+                y = band_item_row(bx, a > 0 ? (a + 1) >> 1 : 0, h);
+                valid = y <= min(J.mb_h - 1, s >> 1);
             }
+            if (spec && (!valid || (s == 0 && J.wait_analysed))) {
+                spec = false;
+                continue;
+            }
+            if (!valid) break;
+            const int x = s - 2 * y, m = y * J.mb_w + x;
+            const uint64_t st = J.ana[m];
+            if (!spec && (uint32_t)(st >> 32) == J.serial) break;  // analysed in an earlier launch
+            bool ok = true;
+            if (s == 0 && J.wait_analysed) ok = __atomic_load_n(J.wait_analysed, __ATOMIC_RELAXED) >= J.wait_total;
+            if (ok) {
+                const RghJob *jp = jobs + j0 + fr;
+                const Gate G{seq, nullptr, 0, -1, 0, spec ? 1 : 0, delay, st, &retry};
+                if constexpr (MODE == 1 || MODE == 8) {
+                    analyse_intra_mb<MODE == 8>(J, jp, x, y, S.a);
+                } else if constexpr (MODE == 4 || MODE == 5 || MODE == 7) {
+                    if (J.intra)
+                        analyse_intra_mb<MODE == 7>(J, jp, x, y, S.a);
+                    else
+                        ok = analyse_mb<MODE == 5, false, MODE == 7>(J, jp, x, y, S.a, G);
+                } else if constexpr (MODE == 9) {
+                    ok = analyse_mb<false, false, false, true>(J, jp, x, y, S.a, G);
+                } else if constexpr (MODE == 10) {
+                    if (J.intra)
+                        analyse_intra_mb<false>(J, jp, x, y, S.a);
+                    else
+                        ok = analyse_mb<false, false, false, true>(J, jp, x, y, S.a, G);
+                } else if constexpr (MODE == 2)
+                    ok = analyse_compact_mb(J, x, y, S.a, seq);
+                else
+                    ok = analyse_mb<MODE == 3, false, MODE == 6>(J, jp, x, y, S.a, G);
+            }
+            if (retry) {  // the frame has fallen behind, or the item was done in an earlier launch
+                spec = false;
+                retry = 0;
+                continue;
+            }
+            if (lane == 0) {
+                // The stamp and progress buffers, read again from the descriptor (as late_outputs).
+                const RGH_G RghJob *g = (const RGH_G RghJob *)(jobs + j0 + fr);
+                __asm__ volatile("" : "+s"(g));
+                RGH_G RghFrameCtl *lctl = gp(g->ctl);
+                if (ok) {
+                    gp(g->ana)[m] = make_stamp(g->serial, seq);
+                    atomicAdd((int *)&lctl->analysed, 1);
+                } else {
+                    atomicMax((int *)&lctl->delay[(launch + 1) & 1], delay + 1);
+                }
+            }
+            break;
         }
+        // The delay carries over to the next launch (raised there by deferrals of this one).
+        if (bx == 0 && lane == 0 && delay > 0) atomicMax((int *)&ctl->delay[(launch + 1) & 1], delay);
     } else {
-        const int sd = s - 4;
+        const int sd = s0 - delay - 4;
         if (sd < 0 || sd > J.steps + 2) return;
         const int a = sd - J.mb_w;
         const int y = band_item_row(bx, a > 0 ? (a + 1) >> 1 : 0, h);
