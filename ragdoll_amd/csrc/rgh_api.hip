@@ -88,7 +88,10 @@ struct Picture {
 
 constexpr int kDefaultLag = 20;  // wavefront steps between consecutive frames of a stream
 constexpr int kDeblockLag = 4;   // deblock items run this many steps behind the analysis
-constexpr int kMaxDelay = 64;    // launches a frame stays in the planned range past its end
+#ifndef RGH_MAX_DELAY
+#define RGH_MAX_DELAY 4
+#endif
+constexpr int kMaxDelay = RGH_MAX_DELAY;  // launches a frame stays in the planned range past its end
 
 struct Ctx {
     int device = 0;
