@@ -3010,7 +3010,7 @@ extern "C" __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(R
     frame_step<4>(jobs, j0, launch, seq);
 }
 
-extern "C" __global__ __launch_bounds__(64) void rgh_k_mixed_learned_step(const RghJob *__restrict__ jobs, int j0,
+extern "C" __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RGH_P_WAVES))) void rgh_k_mixed_learned_step(const RghJob *__restrict__ jobs, int j0,
                                                                            int launch, uint32_t seq) {
     frame_step<5>(jobs, j0, launch, seq);
 }
@@ -3020,12 +3020,12 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_intra_fast_step(const Rgh
     frame_step<8>(jobs, j0, launch, seq);
 }
 
-extern "C" __global__ __launch_bounds__(64) void rgh_k_fast_step(const RghJob *__restrict__ jobs, int j0, int launch,
+extern "C" __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RGH_P_WAVES))) void rgh_k_fast_step(const RghJob *__restrict__ jobs, int j0, int launch,
                                                                   uint32_t seq) {
     frame_step<6>(jobs, j0, launch, seq);
 }
 
-extern "C" __global__ __launch_bounds__(64) void rgh_k_mixed_fast_step(const RghJob *__restrict__ jobs, int j0,
+extern "C" __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RGH_P_WAVES))) void rgh_k_mixed_fast_step(const RghJob *__restrict__ jobs, int j0,
                                                                         int launch, uint32_t seq) {
     frame_step<7>(jobs, j0, launch, seq);
 }
@@ -3035,17 +3035,17 @@ extern "C" __global__ __launch_bounds__(64) void rgh_k_compact_step(const RghJob
     frame_step<2>(jobs, j0, launch, seq);
 }
 
-extern "C" __global__ __launch_bounds__(64) void rgh_k_parts_step(const RghJob *__restrict__ jobs, int j0, int launch,
+extern "C" __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RGH_P_WAVES))) void rgh_k_parts_step(const RghJob *__restrict__ jobs, int j0, int launch,
                                                                    uint32_t seq) {
     frame_step<9>(jobs, j0, launch, seq);
 }
 
-extern "C" __global__ __launch_bounds__(64) void rgh_k_mixed_parts_step(const RghJob *__restrict__ jobs, int j0,
+extern "C" __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RGH_P_WAVES))) void rgh_k_mixed_parts_step(const RghJob *__restrict__ jobs, int j0,
                                                                          int launch, uint32_t seq) {
     frame_step<10>(jobs, j0, launch, seq);
 }
 
-extern "C" __global__ __launch_bounds__(64) void rgh_k_learned_step(const RghJob *__restrict__ jobs, int j0, int launch,
+extern "C" __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(RGH_P_WAVES))) void rgh_k_learned_step(const RghJob *__restrict__ jobs, int j0, int launch,
                                                                      uint32_t seq) {
     frame_step<3>(jobs, j0, launch, seq);
 }
