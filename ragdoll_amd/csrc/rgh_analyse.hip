@@ -1922,12 +1922,10 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
                 oy = y0 + scy - 19;
                 stage_region<SR_H>(J, ox, oy, S);
             }
-            const int px[4] = {bestx, bestx - 1, bestx + 1, bestx};
-            const int py[4] = {besty - 1, besty, besty, besty + 1};
-            int ok[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                ok[k] = (int)(px[k] >= imn0) & (int)(px[k] <= imx0) & (int)(py[k] >= imn1) & (int)(py[k] <= imx1);
+            // The centre is in range (the walk starts at a clamped candidate and moves only to
+            // points in range), so each point has one bound to check.
+            const int ok0 = (int)(besty > imn1), ok1 = (int)(bestx > imn0), ok2 = (int)(bestx < imx0),
+                      ok3 = (int)(besty < imx1);
             uint32_t sums[4];
             {
                 const int lx = x0 + bestx + ddx - ox, ly = y0 + besty + ddy - oy + drow;
@@ -1944,26 +1942,32 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
             }
             // The first strictly cheapest in-range point: the least (cost, k) key (costs stay
             // below 2^30).
-            uint32_t key = 0xFFFFFFFFu, cst[4];
-            int nok = 0;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                cst[k] = sums[k] + mv_cost(px[k] * 4, py[k] * 4);
-                nok += ok[k];
-                key = min(key, ok[k] ? (cst[k] << 2) | (uint32_t)k : 0xFFFFFFFFu);
-            }
+            // Vector costs from the three distinct lengths per component.
+            const uint32_t ex0 = se_bits(4 * bestx - mvpx), exm = se_bits(4 * bestx - 4 - mvpx),
+                           exp_ = se_bits(4 * bestx + 4 - mvpx);
+            const uint32_t ey0 = se_bits(4 * besty - mvpy), eym = se_bits(4 * besty - 4 - mvpy),
+                           eyp = se_bits(4 * besty + 4 - mvpy);
+            const uint32_t cst0 = sums[0] + lambda * (ex0 + eym), cst1 = sums[1] + lambda * (exm + ey0),
+                           cst2 = sums[2] + lambda * (exp_ + ey0), cst3 = sums[3] + lambda * (ex0 + eyp);
+            uint32_t key = ok0 ? cst0 << 2 : 0xFFFFFFFFu;
+            key = min(key, ok1 ? (cst1 << 2) | 1u : 0xFFFFFFFFu);
+            key = min(key, ok2 ? (cst2 << 2) | 2u : 0xFFFFFFFFu);
+            key = min(key, ok3 ? (cst3 << 2) | 3u : 0xFFFFFFFFu);
+            const int nok = ok0 + ok1 + ok2 + ok3;
             if (FAST && lane < 4) {
                 const int k = lane;
-                S.nb[k] = (k == 0 ? ok[0] : k == 1 ? ok[1] : k == 2 ? ok[2] : ok[3])
-                              ? (k == 0 ? cst[0] : k == 1 ? cst[1] : k == 2 ? cst[2] : cst[3])
+                S.nb[k] = (k == 0 ? ok0 : k == 1 ? ok1 : k == 2 ? ok2 : ok3)
+                              ? (k == 0 ? cst0 : k == 1 ? cst1 : k == 2 ? cst2 : cst3)
                               : 0xFFFFFFFFu;
             }
             D.n_sad += nok;
             if ((key >> 2) >= best_cost) break;
             best_cost = key >> 2;
-            const int moved = (int)(key & 3u);
-            bestx += moved == 1 ? -1 : (moved == 2 ? 1 : 0);
-            besty += moved == 0 ? -1 : (moved == 3 ? 1 : 0);
+            // The move (up, left, right, down) as nibble tables of offset + 1: scalar arithmetic
+            // (a comparison turned into an integer costs a VALU round trip).
+            const uint32_t sh = 4u * (key & 3u);
+            bestx += (int)((0x1201u >> sh) & 15u) - 1;
+            besty += (int)((0x2110u >> sh) & 15u) - 1;
         }
 
         // Sub-pel refinement unless the full-pel residual already quantizes to nothing
