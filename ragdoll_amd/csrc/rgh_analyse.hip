@@ -2040,42 +2040,40 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
             // within +-3 quarter samples of the centre (cx, cy), inside the window; points out of
             // range are computed at the centre and ignored.
             int bmx = cx, bmy = cy;
+            // The centre of each step is in range (the walk starts at the full-sample result and
+            // moves only to points in range), so each point has one bound to check: the nearer of
+            // the window's and the vector bounds' on its side.
+            const int lo_x = max(cx - 3, bmin0), hi_x = min(cx + 3, bmax0);
+            const int lo_y = max(cy - 3, bmin1), hi_y = min(cy + 3, bmax1);
             for (int pass = 0; pass < 2; pass++) {
                 const int step = pass == 0 ? 2 : 1, iters = pass == 0 ? 1 : J.qpel_iters;
                 for (int it = 0; it < iters; it++) {
                     const int fx = bmx, fy = bmy;
-                    const int dx[4] = {0, -step, step, 0}, dy[4] = {-step, 0, 0, step};
-                    int mx[4], my[4], ok[4];
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        mx[k] = fx + dx[k];
-                        my[k] = fy + dy[k];
-                        ok[k] = (int)(iabs(mx[k] - cx) <= 3) & (int)(iabs(my[k] - cy) <= 3) & (int)in_bounds(mx[k], my[k]);
-                    }
-                    int qx[4], qy[4];
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        qx[k] = ok[k] ? mx[k] : cx;
-                        qy[k] = ok[k] ? my[k] : cy;
-                    }
+                    const int ok0 = (int)(fy - step >= lo_y), ok1 = (int)(fx - step >= lo_x),
+                              ok2 = (int)(fx + step <= hi_x), ok3 = (int)(fy + step <= hi_y);
+                    // Points out of range are computed at the centre and ignored.
+                    const int qx[4] = {ok0 ? fx : cx, ok1 ? fx - step : cx, ok2 ? fx + step : cx, ok3 ? fx : cx};
+                    const int qy[4] = {ok0 ? fy - step : cy, ok1 ? fy : cy, ok2 ? fy : cy, ok3 ? fy + step : cy};
                     uint32_t sads[4];
                     subpel_sad4(x0, y0, fbx, fby, rox, roy, qx, qy, S, sads);
                     // The first strictly cheapest point below me_cost: the least (cost, k) key
-                    // over the in-range points (costs stay below 2^30), without branches.
-                    uint32_t key = 0xFFFFFFFFu;
-                    int nok = 0;
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        const uint32_t c = sads[k] + mv_cost(mx[k], my[k]);
-                        nok += ok[k];
-                        key = min(key, ok[k] ? (c << 2) | (uint32_t)k : 0xFFFFFFFFu);
-                    }
-                    D.n_subpel += nok;
+                    // over the in-range points (costs stay below 2^30), without branches; vector
+                    // costs from the three distinct lengths per component.
+                    const uint32_t ex0 = se_bits(fx - mvpx), exm = se_bits(fx - step - mvpx), exp_ = se_bits(fx + step - mvpx);
+                    const uint32_t ey0 = se_bits(fy - mvpy), eym = se_bits(fy - step - mvpy), eyp = se_bits(fy + step - mvpy);
+                    const uint32_t c0 = sads[0] + lambda * (ex0 + eym), c1 = sads[1] + lambda * (exm + ey0),
+                                   c2 = sads[2] + lambda * (exp_ + ey0), c3 = sads[3] + lambda * (ex0 + eyp);
+                    uint32_t key = ok0 ? c0 << 2 : 0xFFFFFFFFu;
+                    key = min(key, ok1 ? (c1 << 2) | 1u : 0xFFFFFFFFu);
+                    key = min(key, ok2 ? (c2 << 2) | 2u : 0xFFFFFFFFu);
+                    key = min(key, ok3 ? (c3 << 2) | 3u : 0xFFFFFFFFu);
+                    D.n_subpel += ok0 + ok1 + ok2 + ok3;
                     if ((key >> 2) >= me_cost) break;
                     me_cost = key >> 2;
-                    const int kb = (int)(key & 3u);
-                    bmx = fx + (kb == 1 ? -step : (kb == 2 ? step : 0));
-                    bmy = fy + (kb == 0 ? -step : (kb == 3 ? step : 0));
+                    // The move (up, left, right, down) from nibble tables of offset + 1.
+                    const uint32_t sh = 4u * (key & 3u);
+                    bmx = fx + ((int)((0x1201u >> sh) & 15u) - 1) * step;
+                    bmy = fy + ((int)((0x2110u >> sh) & 15u) - 1) * step;
                 }
             }
             predw = window_pred4(x0, y0, fbx, fby, rox, roy, bmx, bmy, S);
