@@ -1926,33 +1926,48 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
             // points in range), so each point has one bound to check.
             const int ok0 = (int)(besty > imn1), ok1 = (int)(bestx > imn0), ok2 = (int)(bestx < imx0),
                       ok3 = (int)(besty < imx1);
-            uint32_t sums[4];
+            uint32_t sad;
             {
                 const int lx = x0 + bestx + ddx - ox, ly = y0 + besty + ddy - oy + drow;
                 const uint32_t *q = (const uint32_t *)&S.sr[ly * SR_W + (lx & ~3)];
                 const uint32_t f = (uint32_t)(lx & 3);
                 const uint32_t w0 = q[0], w1 = q[1], w2 = q[2], w3 = q[3], w4 = q[4];
-                uint32_t sad = __builtin_amdgcn_sad_u8(srow.x, __builtin_amdgcn_alignbyte(w1, w0, f), 0u);
+                sad = __builtin_amdgcn_sad_u8(srow.x, __builtin_amdgcn_alignbyte(w1, w0, f), 0u);
                 sad = __builtin_amdgcn_sad_u8(srow.y, __builtin_amdgcn_alignbyte(w2, w1, f), sad);
                 sad = __builtin_amdgcn_sad_u8(srow.z, __builtin_amdgcn_alignbyte(w3, w2, f), sad);
                 sad = __builtin_amdgcn_sad_u8(srow.w, __builtin_amdgcn_alignbyte(w4, w3, f), sad);
                 sad = row16_sum(sad);
-#pragma unroll
-                for (int k = 0; k < 4; k++) sums[k] = (uint32_t)__builtin_amdgcn_readlane((int)sad, 16 * k);
             }
             // The first strictly cheapest in-range point: the least (cost, k) key (costs stay
-            // below 2^30).
-            // Vector costs from the three distinct lengths per component.
-            const uint32_t ex0 = se_bits(4 * bestx - mvpx), exm = se_bits(4 * bestx - 4 - mvpx),
-                           exp_ = se_bits(4 * bestx + 4 - mvpx);
-            const uint32_t ey0 = se_bits(4 * besty - mvpy), eym = se_bits(4 * besty - 4 - mvpy),
-                           eyp = se_bits(4 * besty + 4 - mvpy);
-            const uint32_t cst0 = sums[0] + lambda * (ex0 + eym), cst1 = sums[1] + lambda * (exm + ey0),
-                           cst2 = sums[2] + lambda * (exp_ + ey0), cst3 = sums[3] + lambda * (ex0 + eyp);
-            uint32_t key = ok0 ? cst0 << 2 : 0xFFFFFFFFu;
-            key = min(key, ok1 ? (cst1 << 2) | 1u : 0xFFFFFFFFu);
-            key = min(key, ok2 ? (cst2 << 2) | 2u : 0xFFFFFFFFu);
-            key = min(key, ok3 ? (cst3 << 2) | 3u : 0xFFFFFFFFu);
+            // below 2^30). Each 16-lane row forms its point's key (the vector cost needs only the
+            // centre, so it is ready before the SAD), then the rows' keys meet by row broadcasts
+            // in lane 63.
+            uint32_t key, cst0 = 0, cst1 = 0, cst2 = 0, cst3 = 0;
+            if constexpr (!FAST && !LEARNED) {  // (the learned kernels keep the scalar form: fewer spills there)
+                const int okk = dk == 0 ? ok0 : (dk == 1 ? ok1 : (dk == 2 ? ok2 : ok3));
+                const uint32_t mvk = lambda * (se_bits(4 * (bestx + ddx) - mvpx) + se_bits(4 * (besty + ddy) - mvpy));
+                uint32_t kk = okk ? ((sad + mvk) << 2) | (uint32_t)dk : 0xFFFFFFFFu;
+                kk = min(kk, (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)kk, 0x142, 0xA, 0xF, false));  // row_bcast:15
+                kk = min(kk, (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)kk, 0x143, 0xC, 0xF, false));  // row_bcast:31
+                key = (uint32_t)__builtin_amdgcn_readlane((int)kk, 63);
+            } else {
+                uint32_t sums[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) sums[k] = (uint32_t)__builtin_amdgcn_readlane((int)sad, 16 * k);
+                // Vector costs from the three distinct lengths per component.
+                const uint32_t ex0 = se_bits(4 * bestx - mvpx), exm = se_bits(4 * bestx - 4 - mvpx),
+                               exp_ = se_bits(4 * bestx + 4 - mvpx);
+                const uint32_t ey0 = se_bits(4 * besty - mvpy), eym = se_bits(4 * besty - 4 - mvpy),
+                               eyp = se_bits(4 * besty + 4 - mvpy);
+                cst0 = sums[0] + lambda * (ex0 + eym);
+                cst1 = sums[1] + lambda * (exm + ey0);
+                cst2 = sums[2] + lambda * (exp_ + ey0);
+                cst3 = sums[3] + lambda * (ex0 + eyp);
+                key = ok0 ? cst0 << 2 : 0xFFFFFFFFu;
+                key = min(key, ok1 ? (cst1 << 2) | 1u : 0xFFFFFFFFu);
+                key = min(key, ok2 ? (cst2 << 2) | 2u : 0xFFFFFFFFu);
+                key = min(key, ok3 ? (cst3 << 2) | 3u : 0xFFFFFFFFu);
+            }
             const int nok = ok0 + ok1 + ok2 + ok3;
             if (FAST && lane < 4) {
                 const int k = lane;
