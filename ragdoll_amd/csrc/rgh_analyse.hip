@@ -524,8 +524,9 @@ __device__ __forceinline__ int plane_offset(int k, int x0, int y0, int bx, int b
 // within +-3 quarter samples of the window anchor (bx, by), through the window: lane 16 k + r
 // sums row r of vector k from its plane pair (window_pred4's table; five aligned dwords a plane),
 // and each row of 16 lanes adds its vector's rows. Returns the four sums in out.
-__device__ void subpel_sad4(int x0, int y0, int bx, int by, int ox, int oy, const int *mx, const int *my, MbLds &S,
-                            uint32_t *out) {
+// The SAD of the lane's row's point (lane 16 k + r: row r of point k), summed over the 16-lane row.
+__device__ __forceinline__ uint32_t subpel_sad_rows(int x0, int y0, int bx, int by, int ox, int oy, const int *mx,
+                                                    const int *my, MbLds &S) {
     const int lane = threadIdx.x, k = lane >> 4, r = lane & 15;
     const int qx = sel4(k, mx[0], mx[1], mx[2], mx[3]), qy = sel4(k, my[0], my[1], my[2], my[3]);
     const int ph = (qy & 3) * 4 + (qx & 3);
@@ -551,7 +552,11 @@ __device__ void subpel_sad4(int x0, int y0, int bx, int by, int ox, int oy, cons
         const uint32_t b = __builtin_amdgcn_alignbyte(db[c + 1], db[c], fb);
         sad = __builtin_amdgcn_sad_u8(src[c], avg4(a, b), sad);
     }
-    sad = row16_sum(sad);
+    return row16_sum(sad);
+}
+__device__ void subpel_sad4(int x0, int y0, int bx, int by, int ox, int oy, const int *mx, const int *my, MbLds &S,
+                            uint32_t *out) {
+    const uint32_t sad = subpel_sad_rows(x0, y0, bx, by, ox, oy, mx, my, S);
 #pragma unroll
     for (int q = 0; q < 4; q++) out[q] = (uint32_t)__builtin_amdgcn_readlane((int)sad, 16 * q);
 }
@@ -2069,19 +2074,31 @@ __device__ __forceinline__ bool analyse_mb(const RghJobD &J, const RghJob *jp, i
                     // Points out of range are computed at the centre and ignored.
                     const int qx[4] = {ok0 ? fx : cx, ok1 ? fx - step : cx, ok2 ? fx + step : cx, ok3 ? fx : cx};
                     const int qy[4] = {ok0 ? fy - step : cy, ok1 ? fy : cy, ok2 ? fy : cy, ok3 ? fy + step : cy};
-                    uint32_t sads[4];
-                    subpel_sad4(x0, y0, fbx, fby, rox, roy, qx, qy, S, sads);
                     // The first strictly cheapest point below me_cost: the least (cost, k) key
-                    // over the in-range points (costs stay below 2^30), without branches; vector
-                    // costs from the three distinct lengths per component.
-                    const uint32_t ex0 = se_bits(fx - mvpx), exm = se_bits(fx - step - mvpx), exp_ = se_bits(fx + step - mvpx);
-                    const uint32_t ey0 = se_bits(fy - mvpy), eym = se_bits(fy - step - mvpy), eyp = se_bits(fy + step - mvpy);
-                    const uint32_t c0 = sads[0] + lambda * (ex0 + eym), c1 = sads[1] + lambda * (exm + ey0),
-                                   c2 = sads[2] + lambda * (exp_ + ey0), c3 = sads[3] + lambda * (ex0 + eyp);
-                    uint32_t key = ok0 ? c0 << 2 : 0xFFFFFFFFu;
-                    key = min(key, ok1 ? (c1 << 2) | 1u : 0xFFFFFFFFu);
-                    key = min(key, ok2 ? (c2 << 2) | 2u : 0xFFFFFFFFu);
-                    key = min(key, ok3 ? (c3 << 2) | 3u : 0xFFFFFFFFu);
+                    // over the in-range points (costs stay below 2^30), without branches.
+                    uint32_t key;
+                    if constexpr (!LEARNED) {
+                        // Each 16-lane row forms its point's key (as the diamond does).
+                        const uint32_t sadr = subpel_sad_rows(x0, y0, fbx, fby, rox, roy, qx, qy, S);
+                        const int okk = dk == 0 ? ok0 : (dk == 1 ? ok1 : (dk == 2 ? ok2 : ok3));
+                        const uint32_t mvk = lambda * (se_bits(fx + ddx * step - mvpx) + se_bits(fy + ddy * step - mvpy));
+                        uint32_t kk = okk ? ((sadr + mvk) << 2) | (uint32_t)dk : 0xFFFFFFFFu;
+                        kk = min(kk, (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)kk, 0x142, 0xA, 0xF, false));
+                        kk = min(kk, (uint32_t)__builtin_amdgcn_update_dpp((int)0xFFFFFFFF, (int)kk, 0x143, 0xC, 0xF, false));
+                        key = (uint32_t)__builtin_amdgcn_readlane((int)kk, 63);
+                    } else {
+                        uint32_t sads[4];
+                        subpel_sad4(x0, y0, fbx, fby, rox, roy, qx, qy, S, sads);
+                        // Vector costs from the three distinct lengths per component.
+                        const uint32_t ex0 = se_bits(fx - mvpx), exm = se_bits(fx - step - mvpx), exp_ = se_bits(fx + step - mvpx);
+                        const uint32_t ey0 = se_bits(fy - mvpy), eym = se_bits(fy - step - mvpy), eyp = se_bits(fy + step - mvpy);
+                        const uint32_t c0 = sads[0] + lambda * (ex0 + eym), c1 = sads[1] + lambda * (exm + ey0),
+                                       c2 = sads[2] + lambda * (exp_ + ey0), c3 = sads[3] + lambda * (ex0 + eyp);
+                        key = ok0 ? c0 << 2 : 0xFFFFFFFFu;
+                        key = min(key, ok1 ? (c1 << 2) | 1u : 0xFFFFFFFFu);
+                        key = min(key, ok2 ? (c2 << 2) | 2u : 0xFFFFFFFFu);
+                        key = min(key, ok3 ? (c3 << 2) | 3u : 0xFFFFFFFFu);
+                    }
                     D.n_subpel += ok0 + ok1 + ok2 + ok3;
                     if ((key >> 2) >= me_cost) break;
                     me_cost = key >> 2;
