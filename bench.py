@@ -213,7 +213,7 @@ def cpu_baseline(frames, seconds=12.0, single_seconds=5.0, max_threads=16):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--lag", type=int, default=0, help="frame lag in wavefront steps (0: autotune)")
     ap.add_argument("--groups", type=int, default=0, help="concurrent schedules (0: autotune)")
